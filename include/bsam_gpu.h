@@ -1,0 +1,215 @@
+/*
+ * bsam_gpu.h -- C ABI of the MI355X FAS multigrid V-cycle library.
+ *
+ * The library replaces the per-level work of BSAM 1.2's Fortran V-cycle
+ * (LIBRARY/afasroutines.f90, LIBRARY/boundary.f90, LIBRARY/gridutilities.f90
+ * and the PROBLEM-module hooks) with HIP kernels for gfx950.  The host keeps
+ * its own patch tree; it describes the mesh hierarchy with a flat table of
+ * bsam_patch_desc records (one per patch, including the below-root
+ * multigrid levels), moves field arrays in and out with bsam_upload /
+ * bsam_download in the reference's own array extents, and calls the
+ * per-level entry points below, each of which replaces one
+ * `CALL ApplyOnLevel(level, <callback>, dummy)` of the reference, or the
+ * coarse-grained bsam_vcycle / bsam_multigrid_iterations.
+ *
+ * Conventions (as in the reference):
+ *   - indices are 1-based with a ghost layer of width mbc=1 (lower bound 0);
+ *   - arrays are column-major, i fastest, variable index slowest;
+ *   - field arrays are FP64 on the host side of the ABI;
+ *   - every call returns int32 status: 0 = ok, <0 = error (see BSAM_E*),
+ *     >0 = informational (BSAM_NOT_CONVERGED).  No call ever exits the
+ *     process; bsam_last_error() returns the message of the last failure
+ *     on the calling thread.
+ *   - one host thread drives one context; calls are stream-ordered on the
+ *     context's HIP stream.  Only bsam_error_afas, bsam_multigrid_iterations,
+ *     bsam_download and bsam_synchronize wait for the device.
+ *   - the library keeps no host pointer past the call that received it.
+ */
+#ifndef BSAM_GPU_H
+#define BSAM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define BSAM_OK              0
+#define BSAM_NOT_CONVERGED   1   /* maxvcycles exhausted (the reference exits its loop silently, afasroutines.f90:79-108) */
+#define BSAM_EINVAL         -1   /* invalid argument or unsupported configuration */
+#define BSAM_EHIP           -2   /* HIP runtime error */
+#define BSAM_ENOMEM         -3   /* device allocation failed */
+#define BSAM_ESTATE         -4   /* call out of order (e.g. no hierarchy) */
+#define BSAM_EDIVERGED      -5   /* residual became non-finite */
+#define BSAM_ENODEV         -6   /* no usable GPU */
+
+/* ---- models (the PROBLEM-module hook sets the library implements) ---- */
+#define BSAM_MODEL_CH        1   /* PROBLEM/CH/problem.f90 hooks */
+#define BSAM_MODEL_BG        2   /* PROBLEM/BioGrowth/problem.f90 2D hooks */
+
+/* ---- field identifiers for upload/download (nodeinfodef.f90:196-214) - */
+#define BSAM_FIELD_Q         0   /* q   (1-mbc:mx+mbc)^d x nrvars */
+#define BSAM_FIELD_QOLD      1   /* qold, same extents as q */
+#define BSAM_FIELD_QC        2   /* qc  (1-mbc:mx/2+mbc)^d x nrvars */
+#define BSAM_FIELD_F         3   /* f   (1:mx)^d x nrvars */
+#define BSAM_FIELD_RF        4   /* rf  (1:mx)^d x nrvars */
+#define BSAM_FIELD_FTMP      5   /* ftmp (1:mx)^d x nrvars */
+
+/* ---- precision modes -------------------------------------------------- */
+#define BSAM_PREC_FP64       0   /* FP64 everywhere (the reference's r8) */
+#define BSAM_PREC_MIXED      1   /* FP64 outer defect correction, FP32 V-cycles */
+
+/* ---- per-field selector for bsam_fill_down (nodeinfodef.f90:69-70) --- */
+#define BSAM_SOURCEFIELD     1
+#define BSAM_SOLUTIONFIELD   2
+
+/*
+ * One patch of the hierarchy.  Mirrors the mesh part of TYPE(nodeinfo)
+ * (nodeinfodef.f90:118-220): level, mx, mbounds, mglobal, mthbc, dx,
+ * xlower.  `parent` replaces the tree's Parent pointer (treeops.f90:86-101,
+ * GetParentInfo :889-898) with an index into the same descriptor table;
+ * it is -1 for the single patch on the coarsest level (minlevel).  Below
+ * the root the reference keeps one patch per level (bsamroutines.f90:
+ * 443-510); above it any number of patches per level.  In 2D the third
+ * entries are ignored (mx[2]=1).  The order of the descriptors within a
+ * level is the order the library visits patches (the reference visits
+ * youngest to eldest, treeops.f90:539-550); only the floating-point order
+ * of the residual-norm sum depends on it.
+ */
+typedef struct bsam_patch_desc {
+  int32_t level;
+  int32_t parent;
+  int32_t mx[3];
+  int32_t mglobal[3][2];
+  int32_t mbounds[3][2];
+  int32_t mthbc[6];
+  double  dx;
+  double  xlower[3];
+} bsam_patch_desc;
+
+/*
+ * Run parameters.  The V-cycle knobs are the reference's rundata NAMELIST
+ * (bsamroutines.f90:77-96); the model constants replace the PROBLEM module
+ * globals (CH/problemdef.f90:5, set in CH/problem.f90:32) and `dt`
+ * (nodeinfodef.f90:81).
+ */
+typedef struct bsam_params {
+  int32_t ndims;             /* 2 or 3 */
+  int32_t nrvars;            /* 2 for CH and BG */
+  int32_t mbc;               /* must be 1 */
+  int32_t maux;              /* must be 0 (CH and BG use no aux fields) */
+  int32_t model;             /* BSAM_MODEL_* */
+  int32_t nsmoothingpasses;  /* V(nu,nu); reference default 2 */
+  int32_t maxvcycles;        /* reference default 20 */
+  int32_t errortype;         /* 1 (RMS) or 2 (max component RMS) */
+  int32_t updateauxfreq;     /* reference default 1 */
+  int32_t precision;         /* BSAM_PREC_* */
+  int32_t batching;          /* 1: one launch per level; 0: one launch per patch */
+  int32_t use_graph;         /* 1: capture the V-cycle loop body as a hipGraph */
+  double  dt;
+  double  qerrortol;         /* reference default 1e-6 */
+  double  model_params[8];   /* CH: [0]=eps^2.  BG: [0]=eps^2,[1]=gamma,[2]=lambda,[3]=D */
+} bsam_params;
+
+typedef struct bsam_ctx bsam_ctx;
+
+/* Create a context on the current HIP device.  Fails with BSAM_ENODEV when
+ * no GPU is visible.  Replaces the NAMELIST read (bsamroutines.f90:99-105)
+ * and SetProb (CH/problem.f90:12-34) as the source of parameters. */
+int32_t bsam_create(const bsam_params* params, bsam_ctx** out);
+int32_t bsam_destroy(bsam_ctx* ctx);
+/* Change dt between time steps (nodeinfodef.f90:81 is a global the host may
+ * change, e.g. for syncelliptic, bsamroutines.f90:641-643). */
+int32_t bsam_set_dt(bsam_ctx* ctx, double dt);
+
+/* Install a mesh hierarchy (after every regrid, bsamroutines.f90:682-689).
+ * Allocates the device pools in the reference's extents (bsamstorage.f90:
+ * 54-144), zero-initialised as AllocFields does, and builds the per-level
+ * work lists and ghost-fill gather maps.  `mxmax_root` is the root grid's
+ * mx, from which the periodic offsets per level follow (bsamroutines.f90:
+ * 387-390, boundary.f90:875-954).  Periodicity is taken from the root's
+ * mthbc (boundary.f90:805-872). */
+int32_t bsam_set_hierarchy(bsam_ctx* ctx, int32_t npatch,
+                           const bsam_patch_desc* desc,
+                           int32_t minlevel, int32_t finestlevel);
+
+/* Flat copies of one field of one patch, in the reference's extents and
+ * column-major order (see BSAM_FIELD_*).  `count` is the number of doubles
+ * the host buffer holds; it must equal the field's size. */
+int32_t bsam_upload(bsam_ctx* ctx, int32_t patch, int32_t field,
+                    const double* host, int64_t count);
+int32_t bsam_download(bsam_ctx* ctx, int32_t patch, int32_t field,
+                      double* host, int64_t count);
+/* Number of doubles of one field of one patch. */
+int64_t bsam_field_size(const bsam_ctx* ctx, int32_t patch, int32_t field);
+
+/* ---- per-level entry points: each replaces one ApplyOnLevel sweep ----- */
+/* RelaxPatch{iswitch=ipass} over the level (afasroutines.f90:560-607). */
+int32_t bsam_relax_level(bsam_ctx* ctx, int32_t level, int32_t ipass);
+/* SetGhost(level, ipass) (boundary.f90:67-97). */
+int32_t bsam_set_ghost(bsam_ctx* ctx, int32_t level, int32_t ipass);
+/* LevelRelax(level) (afasroutines.f90:532-557). */
+int32_t bsam_level_relax(bsam_ctx* ctx, int32_t level);
+/* RestrictSolution over the level (afasroutines.f90:610-652). */
+int32_t bsam_restrict_solution(bsam_ctx* ctx, int32_t level);
+/* GetCoarseGhostPoints over the level (boundary.f90:957-1015). */
+int32_t bsam_coarse_ghost_points(bsam_ctx* ctx, int32_t level);
+/* GetCoarseLevelLoading(level) (afasroutines.f90:655-736). */
+int32_t bsam_coarse_loading(bsam_ctx* ctx, int32_t level);
+/* CorrectFine over the level (afasroutines.f90:860-980). */
+int32_t bsam_correct_fine(bsam_ctx* ctx, int32_t level);
+/* FillDown(field) (afasroutines.f90:236-327). */
+int32_t bsam_fill_down(bsam_ctx* ctx, int32_t field);
+/* GetSourceFunction / UpdateSourceFunction / UpdateAuxInVcycle over the
+ * level (afasroutines.f90:330-492). */
+int32_t bsam_get_source(bsam_ctx* ctx, int32_t level);
+int32_t bsam_update_source(bsam_ctx* ctx, int32_t level);
+int32_t bsam_update_aux(bsam_ctx* ctx, int32_t level);
+/* Residual(info) over the level: rf = f - L(q) (afasroutines.f90:817-857). */
+int32_t bsam_residual(bsam_ctx* ctx, int32_t level);
+/* ErrorAFAS(level) (afasroutines.f90:113-233); waits for the device. */
+int32_t bsam_error_afas(bsam_ctx* ctx, int32_t level, double* norm);
+/* CopyQToQold over the level (bsamroutines.f90:592-605). */
+int32_t bsam_copy_q_to_qold(bsam_ctx* ctx, int32_t level);
+
+/* ---- coarse-grained entry points --------------------------------------- */
+/* AFASVcycle(level) (afasroutines.f90:495-529). */
+int32_t bsam_vcycle(bsam_ctx* ctx, int32_t level);
+/* MultigridIterations (afasroutines.f90:48-110).  On return *ncycles holds
+ * the number of V-cycles done and *residual the last ErrorAFAS value.
+ * Returns BSAM_NOT_CONVERGED when maxvcycles were used up, BSAM_EDIVERGED
+ * when the residual is not finite.  `history` (may be NULL) receives up to
+ * `history_len` residuals, one per V-cycle. */
+int32_t bsam_multigrid_iterations(bsam_ctx* ctx, int32_t* ncycles,
+                                  double* residual, double* history,
+                                  int32_t history_len);
+/* One time step of the uniform-mesh time loop without regrid
+ * (bsamroutines.f90:673-758 with maxlevel=0): SetGhost(root,0), FillDown on
+ * the first step, CopyQToQold on every level, MultigridIterations. */
+int32_t bsam_time_step(bsam_ctx* ctx, int32_t first_step, int32_t* ncycles,
+                       double* residual);
+
+/* Wait for all work queued on the context's stream. */
+int32_t bsam_synchronize(bsam_ctx* ctx);
+/* Message of the last failed call on this thread ("" if none). */
+const char* bsam_last_error(void);
+
+/* ---- instrumentation (no counterpart in the reference, which only times
+ * the whole run with CPU_TIME, bsamdriver.f90:56-60) -------------------- */
+#define BSAM_KSTAT_RELAX_FINEST   0   /* relax sweeps on the finest level */
+#define BSAM_KSTAT_RESID_FINEST   1   /* fused residual-norm on the finest level */
+#define BSAM_KSTAT_FASRHS_FINEST  2   /* fused FAS right-hand side, finest -> next */
+#define BSAM_KSTAT_COUNT          3
+/* Enable (1) or disable (0) HIP-event timing of selected kernels; enabling
+ * resets the counters. */
+int32_t bsam_kstat_enable(bsam_ctx* ctx, int32_t on);
+/* Launch count, summed kernel milliseconds and algorithmic bytes per launch
+ * of one instrumented kernel class. */
+int32_t bsam_kstat_read(bsam_ctx* ctx, int32_t which, int64_t* launches,
+                        double* total_ms, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSAM_GPU_H */

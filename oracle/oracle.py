@@ -1,0 +1,126 @@
+"""ctypes binding of liboracle.so, the CPU restatement of BSAM 1.2's V-cycle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker.  The product never uses
+it.  The method names mirror the GPU context (bsam_amd.gpu.Solver) so a test
+can drive both through the same sequence of per-level calls.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "bsam1.2_amd"))
+from bsam_amd import abi, hierarchy  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        lib = C.CDLL(LIB_PATH)
+        _P, _I, _D = C.c_void_p, C.c_int32, C.c_double
+        sig = {
+            "orc_create": ([C.POINTER(abi.Params)], _P),
+            "orc_destroy": ([_P], None),
+            "orc_set_dt": ([_P, _D], None),
+            "orc_set_hierarchy": ([_P, _I, C.POINTER(abi.PatchDesc), _I, _I], _I),
+            "orc_field_size": ([_P, _I, _I], C.c_int64),
+            "orc_field_ptr": ([_P, _I, _I], C.POINTER(C.c_double)),
+            "orc_relax_level": ([_P, _I, _I], _I),
+            "orc_set_ghost": ([_P, _I, _I], _I),
+            "orc_level_relax": ([_P, _I], _I),
+            "orc_restrict_solution": ([_P, _I], _I),
+            "orc_coarse_ghost_points": ([_P, _I], _I),
+            "orc_coarse_loading": ([_P, _I], _I),
+            "orc_correct_fine": ([_P, _I], _I),
+            "orc_fill_down": ([_P, _I], _I),
+            "orc_get_source": ([_P, _I], _I),
+            "orc_update_source": ([_P, _I], _I),
+            "orc_residual": ([_P, _I], _I),
+            "orc_error_afas": ([_P, _I, C.POINTER(_D)], _I),
+            "orc_copy_q_to_qold": ([_P, _I], _I),
+            "orc_vcycle": ([_P, _I], _I),
+            "orc_multigrid_iterations": ([_P, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D), _I], _I),
+            "orc_time_step": ([_P, _I, C.POINTER(_I), C.POINTER(_D)], _I),
+            "orc_last_error": ([], C.c_char_p),
+        }
+        for n, (a, r) in sig.items():
+            f = getattr(lib, n)
+            f.argtypes = a
+            f.restype = r
+        _lib = lib
+    return _lib
+
+
+class Oracle:
+    def __init__(self, params, hier):
+        self.lib = load()
+        self.params = params
+        self.ndims = params.ndims
+        self.nv = params.nrvars
+        self.hier = hier
+        self.ctx = self.lib.orc_create(C.byref(params))
+        self.minlevel, self.finestlevel = hierarchy.levels(hier)
+        self._descs = hierarchy.to_ctypes(hier)
+        self._chk(self.lib.orc_set_hierarchy(self.ctx, len(hier), self._descs, self.minlevel, self.finestlevel))
+
+    def __del__(self):
+        if getattr(self, "ctx", None):
+            self.lib.orc_destroy(self.ctx)
+            self.ctx = None
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise RuntimeError(f"oracle error {rc}: {self.lib.orc_last_error().decode()}")
+        return rc
+
+    def field(self, patch, name):
+        """Writable numpy view (var, [k,] j, i) of one field of one patch."""
+        fid = abi.FIELD_NAMES[name]
+        n = self.lib.orc_field_size(self.ctx, patch, fid)
+        ptr = self.lib.orc_field_ptr(self.ctx, patch, fid)
+        a = np.ctypeslib.as_array(ptr, shape=(n,))
+        return a.reshape(hierarchy.field_shape(self.hier[patch], self.ndims, self.nv, name))
+
+    def get(self, patch, name):
+        return self.field(patch, name).copy()
+
+    def put(self, patch, name, arr):
+        self.field(patch, name)[...] = arr
+
+    def set_dt(self, dt):
+        self.lib.orc_set_dt(self.ctx, dt)
+
+    def __getattr__(self, name):
+        fn = getattr(load(), "orc_" + name, None)
+        if fn is None:
+            raise AttributeError(name)
+
+        def call(*args):
+            return self._chk(fn(self.ctx, *args))
+        return call
+
+    def error_afas(self, level):
+        r = C.c_double()
+        self._chk(self.lib.orc_error_afas(self.ctx, level, C.byref(r)))
+        return r.value
+
+    def multigrid_iterations(self, history_len=64):
+        n = C.c_int32()
+        r = C.c_double()
+        h = (C.c_double * history_len)()
+        rc = self._chk(self.lib.orc_multigrid_iterations(self.ctx, C.byref(n), C.byref(r), h, history_len))
+        return rc, n.value, r.value, [h[i] for i in range(min(n.value, history_len))]
+
+    def time_step(self, first):
+        n = C.c_int32()
+        r = C.c_double()
+        rc = self._chk(self.lib.orc_time_step(self.ctx, 1 if first else 0, C.byref(n), C.byref(r)))
+        return rc, n.value, r.value

@@ -1,0 +1,19 @@
+! Model constants and harness controls of the probe PROBLEM module used to
+! run BSAM 1.2's own LIBRARY as the parity oracle (test infrastructure only).
+MODULE ProblemDef
+USE NodeInfoDef, ONLY: r8
+IMPLICIT NONE
+!
+REAL(KIND=r8), PUBLIC:: alpha, eps, eps2
+! Seeded initial data:
+!   initmode 1: phi = base + amp*U(-1,1)
+!   initmode 2: phi = tanh((radius-|x-xc|)/(sqrt(2)*eps)) + amp*U(-1,1)
+REAL(KIND=r8), PUBLIC:: base, amp, radius, hroot, tagtol
+REAL(KIND=r8), DIMENSION(1:3), PUBLIC:: xc
+INTEGER, PUBLIC:: initmode, seed
+! Dump controls: sweeps after relaxation (tag 2), state at source build
+! (tag 1), state after each time step (tag 3).
+INTEGER, PUBLIC:: dumpsweeps, ndumpedsweeps
+LOGICAL, PUBLIC:: dumpsrc, dumpend, dumpopen
+!
+END MODULE ProblemDef
