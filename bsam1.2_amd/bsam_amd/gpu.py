@@ -1,0 +1,163 @@
+"""Host-side mirror of the reference's V-cycle driver over the HIP library.
+
+`Solver` owns one bsam_ctx of libbsam_gpu.so (built from csrc/ by
+bsam_amd.build) and exposes the same per-level operations the reference
+applies with ApplyOnLevel (LIBRARY/afasroutines.f90, LIBRARY/boundary.f90),
+with the reference's names in snake case, plus field access in the
+reference's array extents.  There is no CPU fallback: constructing a Solver
+without the built library or without a visible GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi, hierarchy
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(PKG), "libbsam_gpu.so")
+_lib = None
+
+
+class BsamError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bsam error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libbsam_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        _lib = abi.bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+class Solver:
+    """One hierarchy on the GPU.  `params` is an abi.Params, `hier` a list of
+    descriptor dicts (bsam_amd.hierarchy)."""
+
+    def __init__(self, params, hier):
+        self.lib = load()
+        self.params = params
+        self.ndims = params.ndims
+        self.nv = params.nrvars
+        self.ctx = C.c_void_p()
+        self._chk(self.lib.bsam_create(C.byref(params), C.byref(self.ctx)))
+        self.set_hierarchy(hier)
+
+    def set_hierarchy(self, hier):
+        self.hier = hier
+        self.minlevel, self.finestlevel = hierarchy.levels(hier)
+        self._descs = hierarchy.to_ctypes(hier)
+        self._chk(self.lib.bsam_set_hierarchy(self.ctx, len(hier), self._descs, self.minlevel,
+                                              self.finestlevel))
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.lib.bsam_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise BsamError(rc, self.lib.bsam_last_error().decode())
+        return rc
+
+    # ---- fields -----------------------------------------------------------
+    def get(self, patch, name):
+        fid = abi.FIELD_NAMES[name]
+        n = self.lib.bsam_field_size(self.ctx, patch, fid)
+        out = np.empty(n, dtype=np.float64)
+        self._chk(self.lib.bsam_download(self.ctx, patch, fid, out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return out.reshape(hierarchy.field_shape(self.hier[patch], self.ndims, self.nv, name))
+
+    def put(self, patch, name, arr):
+        fid = abi.FIELD_NAMES[name]
+        n = self.lib.bsam_field_size(self.ctx, patch, fid)
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if a.size != n:
+            raise ValueError(f"field {name} of patch {patch} holds {n} values, got {a.size}")
+        self._chk(self.lib.bsam_upload(self.ctx, patch, fid, a.ctypes.data_as(C.POINTER(C.c_double)), n))
+
+    # ---- per-level operations (one ApplyOnLevel each) -----------------------
+    def relax_level(self, level, ipass):
+        self._chk(self.lib.bsam_relax_level(self.ctx, level, ipass))
+
+    def set_ghost(self, level, ipass):
+        self._chk(self.lib.bsam_set_ghost(self.ctx, level, ipass))
+
+    def level_relax(self, level):
+        self._chk(self.lib.bsam_level_relax(self.ctx, level))
+
+    def restrict_solution(self, level):
+        self._chk(self.lib.bsam_restrict_solution(self.ctx, level))
+
+    def coarse_ghost_points(self, level):
+        self._chk(self.lib.bsam_coarse_ghost_points(self.ctx, level))
+
+    def coarse_loading(self, level):
+        self._chk(self.lib.bsam_coarse_loading(self.ctx, level))
+
+    def correct_fine(self, level):
+        self._chk(self.lib.bsam_correct_fine(self.ctx, level))
+
+    def fill_down(self, field):
+        self._chk(self.lib.bsam_fill_down(self.ctx, field))
+
+    def get_source(self, level):
+        self._chk(self.lib.bsam_get_source(self.ctx, level))
+
+    def update_source(self, level):
+        self._chk(self.lib.bsam_update_source(self.ctx, level))
+
+    def residual(self, level):
+        self._chk(self.lib.bsam_residual(self.ctx, level))
+
+    def copy_q_to_qold(self, level):
+        self._chk(self.lib.bsam_copy_q_to_qold(self.ctx, level))
+
+    def vcycle(self, level):
+        self._chk(self.lib.bsam_vcycle(self.ctx, level))
+
+    def error_afas(self, level):
+        r = C.c_double()
+        self._chk(self.lib.bsam_error_afas(self.ctx, level, C.byref(r)))
+        return r.value
+
+    def multigrid_iterations(self, history_len=128):
+        n = C.c_int32()
+        r = C.c_double()
+        h = (C.c_double * history_len)()
+        rc = self._chk(self.lib.bsam_multigrid_iterations(self.ctx, C.byref(n), C.byref(r), h, history_len))
+        return rc, n.value, r.value, [h[i] for i in range(min(n.value, history_len))]
+
+    def time_step(self, first):
+        n = C.c_int32()
+        r = C.c_double()
+        rc = self._chk(self.lib.bsam_time_step(self.ctx, 1 if first else 0, C.byref(n), C.byref(r)))
+        return rc, n.value, r.value
+
+    def set_dt(self, dt):
+        self._chk(self.lib.bsam_set_dt(self.ctx, dt))
+
+    def synchronize(self):
+        self._chk(self.lib.bsam_synchronize(self.ctx))
+
+    # ---- instrumentation ----------------------------------------------------
+    def kstat_enable(self, on=True):
+        self._chk(self.lib.bsam_kstat_enable(self.ctx, 1 if on else 0))
+
+    def kstat_read(self, which):
+        n = C.c_int64()
+        ms = C.c_double()
+        b = C.c_double()
+        self._chk(self.lib.bsam_kstat_read(self.ctx, which, C.byref(n), C.byref(ms), C.byref(b)))
+        return n.value, ms.value, b.value

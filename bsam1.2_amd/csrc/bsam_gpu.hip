@@ -1,0 +1,1915 @@
+// bsam_gpu.hip -- MI355X (gfx950) implementation of BSAM 1.2's FAS V-cycle
+// behind the C ABI of include/bsam_gpu.h.
+//
+// One launch covers all patches of a level: threads are numbered over the
+// level's cells (flat, patch after patch, prefix sums per level) and find
+// their patch by a binary search of the prefix array, so 8^3 and 256^3
+// patches share the same kernels.  Block indices are remapped so that each
+// of the 8 XCDs works on a contiguous range of cells (its own L2 then holds
+// the neighbouring rows and planes of the stencil).  Ghost filling is a
+// per-patch interpolation kernel plus a list of box copies per stage, in
+// the reference's order (boundary.f90:67-97).
+//
+// Floating point: the file is compiled with -ffp-contract=off and every
+// expression keeps the reference's evaluation order, so results are the
+// reference's to the last bit except for the residual-norm sum.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bsam_gpu.h"
+#include "bsam_kernels.h"
+#include "bsam_mixed.h"
+
+using namespace bsam;
+
+// ============================================================================
+// error handling
+// ============================================================================
+static thread_local std::string g_err;
+
+static int32_t set_err(int32_t code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_err(BSAM_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));           \
+  } while (0)
+
+#define RETCHK(expr)              \
+  do {                            \
+    int32_t r_ = (expr);          \
+    if (r_ < 0) return r_;        \
+  } while (0)
+
+extern "C" const char* bsam_last_error(void) { return g_err.c_str(); }
+
+// ============================================================================
+// device helpers
+// ============================================================================
+namespace {
+
+constexpr int kBlock = 256;
+
+// Interpolation constants, boundary.f90:54-63.
+__constant__ double cA[3] = {-1.0 / 5.0, 2.0 / 3.0, 8.0 / 15.0};
+__constant__ double cB[3] = {5.0 / 32.0, 15.0 / 16.0, -3.0 / 32.0};
+__constant__ double cC[4] = {9.0 / 16.0, 3.0 / 16.0, 3.0 / 16.0, 1.0 / 16.0};
+
+// Logical block index with XCD-contiguous ranges: hardware places block b on
+// XCD b % 8; blocks of one XCD get consecutive logical indices.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7, s = b >> 3;
+  return (x < r) ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+
+__device__ __forceinline__ int find_patch(const int* __restrict__ prefix, int n, int g) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+struct LevelArgs {
+  const DPatch* __restrict__ pt;
+  const int* __restrict__ patches;  // descriptor indices of the level
+  const int* __restrict__ prefix;   // npatch+1 prefix sums of the kernel's domain
+  int npatch;
+  int total;
+};
+
+// ---------------------------------------------------------------------------
+// CH relaxation: RelaxGrid2D/3D, CH/problem.f90:318-369, 554-614.
+// Domain: cells of colour c, (mx+1)/2 per row.  Colour c relaxes cells with
+// (i+j+k) mod 2 == c; the reference's ipass = c+1 (problem.f90:350, 591).
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double dt, double eps2,
+                                                     double h, double* __restrict__ q,
+                                                     const double* __restrict__ f) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const int local = g - L.prefix[pi];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int t = local % cnt;
+  const int row = local / cnt;
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int ib = (c + j + k) & 1;
+  const int i = 2 * t + 2 - ib;
+  if (i > P.mx[0]) return;
+  const double h2 = h * h;
+  const double tmp1 = dt / h2, tmp2 = eps2 / h2;
+  const double tmp3 = (DIM == 3 ? 6.0 : 4.0) * eps2 / h2;
+  const double a12 = (DIM == 3 ? 6.0 : 4.0) * tmp1;
+  const int vs = P.qvs;
+  const int o = qidx(P, i, j, k);
+  const double phi = q[o];
+  const double f0 = f[o], f1 = f[o + vs];
+  const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
+  const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
+  double smu, sphi;
+  if (DIM == 3) {
+    const int zm = qidx(P, i, j, k - 1), zp = qidx(P, i, j, k + 1);
+    smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs] + q[zm + vs] + q[zp + vs];
+    sphi = q[xp] + q[xm] + q[yp] + q[ym] + q[zp] + q[zm];
+  } else {
+    smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs];
+    sphi = q[xp] + q[xm] + q[yp] + q[ym];
+  }
+  const double a21 = -3.0 * (phi * phi) - tmp3;
+  const double b1 = f0 + tmp1 * smu;
+  const double b2 = f1 - tmp2 * sphi - 2.0 * (phi * phi * phi);
+  const double det = 1.0 - a12 * a21;
+  q[o] = (b1 - a12 * b2) / det;
+  q[o + vs] = (b2 - a21 * b1) / det;
+}
+
+// BioGrowth RelaxGrid2D, BioGrowth/problem.f90:255-296 (decoupled GS).
+__global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, ModelK m, double h,
+                                                       double* __restrict__ q,
+                                                       const double* __restrict__ qold,
+                                                       const double* __restrict__ f) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const int local = g - L.prefix[pi];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int t = local % cnt;
+  const int j = local / cnt + 1;
+  const int ib = (c + j) & 1;
+  const int i = 2 * t + 2 - ib;
+  if (i > P.mx[0]) return;
+  const double h2 = h * h;
+  const double tmp1 = m.dt * m.eps2 / h2, tmp3 = 1.0 / h2;
+  const int vs = P.qvs;
+  const int o = qidx(P, i, j, 0);
+  const int xm = qidx(P, i - 1, j, 0), xp = qidx(P, i + 1, j, 0);
+  const int ym = qidx(P, i, j - 1, 0), yp = qidx(P, i, j + 1, 0);
+  const double p1 = q[o];
+  const double tmp2 = 4.0 * tmp1 + 1.0 + m.dt * (p1 * p1);
+  const double po = qold[o];
+  const double grow = m.gam * 0.5 * (po + 1.0);
+  const double upt = m.lam * 0.5 * (po + 1.0);
+  q[o] = (f[o] + m.dt * grow * (q[o + vs] - m.cz) + tmp1 * (q[xm] + q[xp] + q[ym] + q[yp])) / tmp2;
+  const double tmp4 = 4.0 * tmp3 + upt;
+  q[o + vs] = (f[o + vs] + tmp3 * (q[xp + vs] + q[xm + vs] + q[yp + vs] + q[ym + vs])) / tmp4;
+}
+
+// ---------------------------------------------------------------------------
+// Operators (Operator2D/3D with ULap2D/3D): CH/problem.f90:393-422, 638-669,
+// gridutilities.f90:50-72, 372-399; BioGrowth/problem.f90:320-353.  `po` is
+// the qold value of the cell (BG only).
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ double ulap(const double* __restrict__ a, const DPatch& P, int i, int j,
+                                       int k, int off) {
+  double s = a[qidx(P, i + 1, j, k) + off] + a[qidx(P, i - 1, j, k) + off] +
+             a[qidx(P, i, j + 1, k) + off] + a[qidx(P, i, j - 1, k) + off];
+  if (DIM == 3) {
+    s = s + a[qidx(P, i, j, k + 1) + off];
+    s = s + a[qidx(P, i, j, k - 1) + off];
+    return s - 6.0 * a[qidx(P, i, j, k) + off];
+  }
+  return s - 4.0 * a[qidx(P, i, j, k) + off];
+}
+// Same on the qc geometry.
+template <int DIM>
+__device__ __forceinline__ double ulap_c(const double* __restrict__ a, const DPatch& P, int i, int j,
+                                         int k, int off) {
+  double s = a[cidx(P, i + 1, j, k) + off] + a[cidx(P, i - 1, j, k) + off] +
+             a[cidx(P, i, j + 1, k) + off] + a[cidx(P, i, j - 1, k) + off];
+  if (DIM == 3) {
+    s = s + a[cidx(P, i, j, k + 1) + off];
+    s = s + a[cidx(P, i, j, k - 1) + off];
+    return s - 6.0 * a[cidx(P, i, j, k) + off];
+  }
+  return s - 4.0 * a[cidx(P, i, j, k) + off];
+}
+
+template <int DIM, int MODEL>
+__device__ __forceinline__ void op_from(double p, double mu, double lphi, double lmu, double po,
+                                        const ModelK& m, double h, double& o1, double& o2) {
+  const double h2 = h * h;
+  if (MODEL == BSAM_MODEL_CH) {
+    const double tmp1 = m.dt / h2, tmp2 = m.eps2 / h2;
+    o1 = p - tmp1 * lmu;
+    o2 = mu - (p * p * p) + tmp2 * lphi;
+  } else {
+    const double tmp1 = m.dt * m.eps2 / h2, tmp3 = 1.0 / h2;
+    const double grow = m.gam * 0.5 * (po + 1.0), upt = m.lam * 0.5 * (po + 1.0);
+    o1 = p - tmp1 * lphi + m.dt * (p * p * p) - m.dt * grow * (mu - m.cz);
+    o2 = -tmp3 * lmu + upt * mu;
+  }
+}
+
+template <int DIM, int MODEL>
+__device__ __forceinline__ void op_q(const double* __restrict__ q, const double* __restrict__ qold,
+                                     const DPatch& P, int i, int j, int k, const ModelK& m, double h,
+                                     double& o1, double& o2) {
+  const int o = qidx(P, i, j, k);
+  const double lmu = ulap<DIM>(q, P, i, j, k, P.qvs);
+  const double lphi = ulap<DIM>(q, P, i, j, k, 0);
+  const double po = (MODEL == BSAM_MODEL_BG) ? qold[o] : 0.0;
+  op_from<DIM, MODEL>(q[o], q[o + P.qvs], lphi, lmu, po, m, h, o1, o2);
+}
+
+// Residual (afasroutines.f90:817-857): rf = f - L(q) over the interior.
+// Also accumulates sum(rf^2) per variable into per-block partials when
+// `partial` is non-null (ErrorAFAS / L2Error / SquareSum, :113-233), and
+// writes rf when `rf` is non-null.
+template <int DIM, int MODEL>
+__global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
+                                                     const double* __restrict__ q,
+                                                     const double* __restrict__ qold,
+                                                     const double* __restrict__ f,
+                                                     double* __restrict__ rf,
+                                                     double* __restrict__ partial) {
+  const int lb = xcd_block();
+  const int g = lb * kBlock + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  if (g < L.total) {
+    const int pi = find_patch(L.prefix, L.npatch, g);
+    const DPatch& P = L.pt[L.patches[pi]];
+    const int local = g - L.prefix[pi];
+    const int i = local % P.mx[0] + 1;
+    const int row = local / P.mx[0];
+    const int j = row % P.mx[1] + 1;
+    const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+    double o1, o2;
+    op_q<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o1, o2);
+    const int o = qidx(P, i, j, k);
+    const double r0 = f[o] - o1, r1 = f[o + P.qvs] - o2;
+    if (rf) {
+      rf[o] = r0;
+      rf[o + P.qvs] = r1;
+    }
+    s0 = r0 * r0;
+    s1 = r1 * r1;
+  }
+  if (!partial) return;
+  // wavefront then block reduction in a fixed order (deterministic)
+  for (int d = 32; d > 0; d >>= 1) {
+    s0 += __shfl_down(s0, d, 64);
+    s1 += __shfl_down(s1, d, 64);
+  }
+  __shared__ double red[2][kBlock / 64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    red[0][w] = s0;
+    red[1][w] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int x = 0; x < kBlock / 64; x++) {
+      a += red[0][x];
+      b += red[1][x];
+    }
+    partial[2 * lb] = a;
+    partial[2 * lb + 1] = b;
+  }
+}
+
+// Sum the per-block partials in a fixed order into out[0..1].
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ partial,
+                                                            int nb, double* __restrict__ out) {
+  __shared__ double s[2][kBlock];
+  double a = 0.0, b = 0.0;
+  for (int x = threadIdx.x; x < nb; x += kBlock) {
+    a += partial[2 * x];
+    b += partial[2 * x + 1];
+  }
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + d];
+      s[1][threadIdx.x] += s[1][threadIdx.x + d];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s[0][0];
+    out[1] = s[1][0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Restriction (Restriction2D/3D, gridutilities.f90:103-126, 437-466) in the
+// reference's summation order: (e,e,e),(o,e,e),(e,o,e),(o,o,e),(e,e,o),...
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ double restrict_cell(const double* __restrict__ a, const DPatch& P, int ic,
+                                                int jc, int kc, int off) {
+  const int i2 = 2 * ic, i1 = i2 - 1, j2 = 2 * jc, j1 = j2 - 1;
+  if (DIM == 2) {
+    return 0.25 * (a[qidx(P, i2, j2, 0) + off] + a[qidx(P, i1, j2, 0) + off] +
+                   a[qidx(P, i2, j1, 0) + off] + a[qidx(P, i1, j1, 0) + off]);
+  }
+  const int k2 = 2 * kc, k1 = k2 - 1;
+  return 0.125 * (a[qidx(P, i2, j2, k2) + off] + a[qidx(P, i1, j2, k2) + off] +
+                  a[qidx(P, i2, j1, k2) + off] + a[qidx(P, i1, j1, k2) + off] +
+                  a[qidx(P, i2, j2, k1) + off] + a[qidx(P, i1, j2, k1) + off] +
+                  a[qidx(P, i2, j1, k1) + off] + a[qidx(P, i1, j1, k1) + off]);
+}
+
+// Decode a flat coarse-cell index of the level (domain: cmx^d per patch).
+struct CoarseCell {
+  int pid, ic, jc, kc;
+};
+template <int DIM>
+__device__ __forceinline__ bool decode_coarse(const LevelArgs& L, int g, CoarseCell& cc) {
+  if (g >= L.total) return false;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  cc.pid = L.patches[pi];
+  const DPatch& P = L.pt[cc.pid];
+  const int local = g - L.prefix[pi];
+  cc.ic = local % P.cmx[0] + 1;
+  const int row = local / P.cmx[0];
+  cc.jc = row % P.cmx[1] + 1;
+  cc.kc = (DIM == 3) ? row / P.cmx[1] + 1 : 0;
+  return true;
+}
+
+// RestrictSolution (afasroutines.f90:610-652) when `qc` is non-null: qc =
+// R q and parent%q(mb) = qc.  FillDownLevel (afasroutines.f90:257-327) when
+// `qc` is null: parent field(mb) = R field (field = q or f).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const double* __restrict__ src,
+                                                     double* __restrict__ dst_parent,
+                                                     double* __restrict__ qc) {
+  CoarseCell c;
+  if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
+  const DPatch& P = L.pt[c.pid];
+  const DPatch& Q = L.pt[P.parent];
+  const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
+  const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
+  for (int v = 0; v < nv; v++) {
+    const double r = restrict_cell<DIM>(src, P, c.ic, c.jc, c.kc, v * P.qvs);
+    if (qc) qc[cidx(P, c.ic, c.jc, c.kc) + v * P.cvs] = r;
+    dst_parent[qidx(Q, ip, jp, kp) + v * Q.qvs] = r;
+  }
+}
+
+// GetCoarseGhostPoints (boundary.f90:957-1015): the ring of qc <- parent q.
+// Domain: the (cmx+2)^d qc box; interior cells are skipped.  The reference's
+// face loops cover every ring cell exactly once with the value
+// parent%q(mb-1+i, mb-1+j, mb-1+k).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, const double* __restrict__ q,
+                                                        double* __restrict__ qc) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch& Q = L.pt[P.parent];
+  const int local = g - L.prefix[pi];
+  const int ic = local % P.nc[0];
+  const int row = local / P.nc[0];
+  const int jc = row % P.nc[1];
+  const int kc = (DIM == 3) ? row / P.nc[1] : 0;
+  const bool interior = ic >= 1 && ic <= P.cmx[0] && jc >= 1 && jc <= P.cmx[1] &&
+                        (DIM == 2 || (kc >= 1 && kc <= P.cmx[2]));
+  if (interior) return;
+  const int ip = P.mb[0][0] - 1 + ic, jp = P.mb[1][0] - 1 + jc;
+  const int kp = (DIM == 3) ? P.mb[2][0] - 1 + kc : 0;
+  for (int v = 0; v < nv; v++) qc[cidx(P, ic, jc, kc) + v * P.cvs] = q[qidx(Q, ip, jp, kp) + v * Q.qvs];
+}
+
+// CoarseLoadingFunction (afasroutines.f90:673-736), fused with Residual:
+// parent%f(mb) = R(f - L_h q) + L_2h(qc).  The fine residual is formed on
+// the fly for the 2^d children of each coarse cell, in the order the
+// restriction sums them.
+template <int DIM, int MODEL>
+__global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const double* __restrict__ q,
+                                                    const double* __restrict__ qold,
+                                                    const double* __restrict__ f,
+                                                    const double* __restrict__ qc,
+                                                    double* __restrict__ fpar) {
+  CoarseCell c;
+  if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
+  const DPatch& P = L.pt[c.pid];
+  const DPatch& Q = L.pt[P.parent];
+  const double h = P.dx;
+  double r0 = 0.0, r1 = 0.0;
+  const int i2 = 2 * c.ic, j2 = 2 * c.jc, k2 = 2 * c.kc;
+  const int nch = (DIM == 3) ? 8 : 4;
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int n = 0; n < nch; n++) {
+    // order: (e,e,e),(o,e,e),(e,o,e),(o,o,e),(e,e,o),(o,e,o),(e,o,o),(o,o,o)
+    const int i = i2 - (n & 1), j = j2 - ((n >> 1) & 1);
+    const int k = (DIM == 3) ? k2 - ((n >> 2) & 1) : 0;
+    double o1, o2;
+    op_q<DIM, MODEL>(q, qold, P, i, j, k, m, h, o1, o2);
+    const int o = qidx(P, i, j, k);
+    const double a0 = f[o] - o1, a1 = f[o + P.qvs] - o2;
+    if (n == 0) {
+      s0 = a0;
+      s1 = a1;
+    } else {
+      s0 = s0 + a0;
+      s1 = s1 + a1;
+    }
+  }
+  r0 = ((DIM == 3) ? 0.125 : 0.25) * s0;
+  r1 = ((DIM == 3) ? 0.125 : 0.25) * s1;
+  // coarse operator on qc with ch = parent%dx(1); BG reads parent%qold.
+  const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
+  const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
+  const int oc = cidx(P, c.ic, c.jc, c.kc);
+  const double lmu = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, P.cvs);
+  const double lphi = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, 0);
+  const int opar = qidx(Q, ip, jp, kp);
+  const double po = (MODEL == BSAM_MODEL_BG) ? qold[opar] : 0.0;
+  double o1, o2;
+  op_from<DIM, MODEL>(qc[oc], qc[oc + P.cvs], lphi, lmu, po, m, Q.dx, o1, o2);
+  fpar[opar] = r0 + o1;
+  fpar[opar + Q.qvs] = r1 + o2;
+}
+
+// CorrectFine, first half (afasroutines.f90:892-895, 934-938):
+// qc(0:cmx+1) = parent%q(mb-1:mb+1) - qc.  Domain: the qc box.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, const double* __restrict__ q,
+                                                       double* __restrict__ qc) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch& Q = L.pt[P.parent];
+  const int local = g - L.prefix[pi];
+  const int ic = local % P.nc[0];
+  const int row = local / P.nc[0];
+  const int jc = row % P.nc[1];
+  const int kc = (DIM == 3) ? row / P.nc[1] : 0;
+  const int ip = P.mb[0][0] - 1 + ic, jp = P.mb[1][0] - 1 + jc;
+  const int kp = (DIM == 3) ? P.mb[2][0] - 1 + kc : 0;
+  for (int v = 0; v < nv; v++) {
+    const int o = cidx(P, ic, jc, kc) + v * P.cvs;
+    qc[o] = q[qidx(Q, ip, jp, kp) + v * Q.qvs] - qc[o];
+  }
+}
+
+// Linear weights of Bi/TriLinProlongationP1 (gridutilities.f90:156-195,
+// 510-566): even fine index 3*a(lo)+a(lo+1), odd a(lo)+3*a(lo+1).
+__device__ __forceinline__ double linw(double lo, double hi, int fine) {
+  return (fine & 1) ? lo + 3.0 * hi : 3.0 * lo + hi;
+}
+
+// CorrectFine, second half: q(0:mx+1) += P1(qc).  Domain: (mx+2)^d.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, double* __restrict__ q,
+                                                        const double* __restrict__ qc) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const int local = g - L.prefix[pi];
+  const int i = local % P.nq[0];
+  const int row = local / P.nq[0];
+  const int j = row % P.nq[1];
+  const int k = (DIM == 3) ? row / P.nq[1] : 0;
+  const int il = i >> 1, jl = j >> 1, kl = k >> 1;
+  for (int v = 0; v < nv; v++) {
+    const int cv = v * P.cvs;
+    double pr;
+    if (DIM == 2) {
+      const double b0 = linw(qc[cidx(P, il, jl, 0) + cv], qc[cidx(P, il + 1, jl, 0) + cv], i);
+      const double b1 = linw(qc[cidx(P, il, jl + 1, 0) + cv], qc[cidx(P, il + 1, jl + 1, 0) + cv], i);
+      pr = linw(b0, b1, j) / 16.0;
+    } else {
+      double cc[2];
+#pragma unroll
+      for (int dk = 0; dk < 2; dk++) {
+        const double b0 = linw(qc[cidx(P, il, jl, kl + dk) + cv], qc[cidx(P, il + 1, jl, kl + dk) + cv], i);
+        const double b1 =
+            linw(qc[cidx(P, il, jl + 1, kl + dk) + cv], qc[cidx(P, il + 1, jl + 1, kl + dk) + cv], i);
+        cc[dk] = linw(b0, b1, j);
+      }
+      pr = linw(cc[0], cc[1], k) / 64.0;
+    }
+    const int o = qidx(P, i, j, k) + v * P.qvs;
+    q[o] = q[o] + pr;
+  }
+}
+
+// Source#D (CH/problem.f90:425-450, 672-695; BioGrowth/problem.f90:356-379)
+// into ftmp, then f = ftmp + SourceUpdate#D (= 0 for both models)
+// (afasroutines.f90:330-381, 438-492).  mode 0: source only, 1: update only.
+template <int DIM, int MODEL>
+__global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double dt,
+                                                   const double* __restrict__ qold,
+                                                   double* __restrict__ ftmp, double* __restrict__ f) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const int local = g - L.prefix[pi];
+  const int i = local % P.mx[0] + 1;
+  const int row = local / P.mx[0];
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int o = qidx(P, i, j, k);
+  if (mode == 0) {
+    const double po = qold[o];
+    if (MODEL == BSAM_MODEL_CH) {
+      ftmp[o] = po;
+      ftmp[o + P.qvs] = -po;
+    } else {
+      ftmp[o] = (1.0 + dt) * po;
+      ftmp[o + P.qvs] = 0.0;
+    }
+  } else {
+    f[o] = ftmp[o] + 0.0;
+    f[o + P.qvs] = ftmp[o + P.qvs] + 0.0;
+  }
+}
+
+// CopyQToQold (bsamroutines.f90:592-605): whole arrays including ghosts.
+__global__ __launch_bounds__(kBlock) void k_copy_span(const double* __restrict__ src,
+                                                      double* __restrict__ dst, long n) {
+  long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g < n) dst[g] = src[g];
+}
+
+// ---------------------------------------------------------------------------
+// Ghost filling.
+// ---------------------------------------------------------------------------
+// GetFaceIndex (boundary.f90:1018-1045): 0 ghost, 1 second interior,
+// 2 first interior, 3 coarse outside, 4 coarse inside.
+__device__ __forceinline__ void face_index(const DPatch& P, int dim, int parity, int fi[5]) {
+  if (parity == 1) {
+    fi[0] = 0; fi[1] = 2; fi[2] = 1; fi[3] = P.mb[dim][0] - 1; fi[4] = P.mb[dim][0];
+  } else {
+    const int mm = P.mx[dim];
+    fi[0] = mm + 1; fi[1] = mm - 1; fi[2] = mm; fi[3] = P.mb[dim][1] + 1; fi[4] = P.mb[dim][1];
+  }
+}
+
+// Transverse position kind of Interpolate2D/3D's statements: 1 -> L,
+// mx -> H, even -> E (sources at +2/+1), odd -> O (sources at -2/-1).
+enum { KE = 0, KO = 1, KL = 2, KH = 3 };
+__device__ __forceinline__ int pos_kind(int pos, int m) {
+  return pos == 1 ? KL : (pos == m ? KH : ((pos & 1) ? KO : KE));
+}
+__device__ __forceinline__ int kind_s1(int k) { return k == KE ? 2 : (k == KO ? -2 : 0); }
+__device__ __forceinline__ int kind_s2(int k) { return k == KE ? 1 : (k == KO ? -1 : 0); }
+// coarse index of an E/O position, or the three quadratic nodes of L/H
+__device__ __forceinline__ int kind_c(const DPatch& P, int dim, int k, int pos, int r) {
+  const int mb1 = P.mb[dim][0], mb2 = P.mb[dim][1];
+  if (k == KE) return mb1 + pos / 2 - 1;
+  if (k == KO) return mb1 + (pos - 1) / 2;
+  if (k == KL) return mb1 - 1 + r;   // mb1-1, mb1, mb1+1
+  return mb2 + 1 - r;                // mb2+1, mb2, mb2-1
+}
+__device__ __forceinline__ bool bc_interp(int code) { return code == 2 || code == 999; }
+
+template <int DIM>
+__device__ __forceinline__ int st_k(int k) { return DIM == 3 ? k : 0; }
+
+// One interpolated ghost value from fine sources s1, s2 (patch P) and the
+// coarse term (parent Q), in the reference's order.
+template <int DIM>
+__device__ __forceinline__ void interp_store(double* __restrict__ q, const DPatch& P, const DPatch& Q,
+                                             int nv, const int g[3], const int s1[3], const int s2[3],
+                                             int ncoarse, const int (*cq)[3], const double* w) {
+  for (int v = 0; v < nv; v++) {
+    const int fv = v * P.qvs, pv = v * Q.qvs;
+    double coarse;
+    if (ncoarse == 1) {
+      coarse = q[qidx(Q, cq[0][0], cq[0][1], st_k<DIM>(cq[0][2])) + pv];
+    } else if (ncoarse == 3) {
+      coarse = w[0] * q[qidx(Q, cq[0][0], cq[0][1], st_k<DIM>(cq[0][2])) + pv] +
+               w[1] * q[qidx(Q, cq[1][0], cq[1][1], st_k<DIM>(cq[1][2])) + pv] +
+               w[2] * q[qidx(Q, cq[2][0], cq[2][1], st_k<DIM>(cq[2][2])) + pv];
+    } else {
+      coarse = w[0] * q[qidx(Q, cq[0][0], cq[0][1], st_k<DIM>(cq[0][2])) + pv] +
+               w[1] * q[qidx(Q, cq[1][0], cq[1][1], st_k<DIM>(cq[1][2])) + pv] +
+               w[2] * q[qidx(Q, cq[2][0], cq[2][1], st_k<DIM>(cq[2][2])) + pv] +
+               w[3] * q[qidx(Q, cq[3][0], cq[3][1], st_k<DIM>(cq[3][2])) + pv];
+    }
+    q[qidx(P, g[0], g[1], st_k<DIM>(g[2])) + fv] =
+        cA[0] * q[qidx(P, s1[0], s1[1], st_k<DIM>(s1[2])) + fv] +
+        cA[1] * q[qidx(P, s2[0], s2[1], st_k<DIM>(s2[2])) + fv] + cA[2] * coarse;
+  }
+}
+
+// InterpolateCoarseFine (boundary.f90:100-140) with Interpolate2D (:143-333)
+// and Interpolate3D (:1048-1550).  One block per patch of the level; the
+// threads enumerate the patch's ghost cells (faces, then edges, then
+// corners) and apply the rule the reference applies to that cell, including
+// the colour filter, the physical-face skip for ipass=1 in 3D, and the
+// comparison with 99 instead of 999 at boundary.f90:1104.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt,
+                                                   const int* __restrict__ patches, int nv, int ipass,
+                                                   double* __restrict__ q) {
+  const DPatch& P = pt[patches[blockIdx.x]];
+  const DPatch& Q = pt[P.parent];
+  const int* mth = P.mthbc;
+  int fi[3][5];
+  if (DIM == 2) {
+    // faces: 4 faces x m_t cells; corners: 4
+    const int nf0 = P.mx[1], nf1 = P.mx[0];
+    const int total = 2 * nf0 + 2 * nf1 + 4;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+      int g[3], s1[3], s2[3], cq[4][3];
+      double w[4];
+      int nco = 1;
+      g[2] = s1[2] = s2[2] = 1;
+      cq[0][2] = cq[1][2] = cq[2][2] = cq[3][2] = 1;
+      if (e < 2 * nf0 + 2 * nf1) {
+        int d, p, pos;
+        if (e < 2 * nf0) { d = 0; p = 1 + e / nf0; pos = e % nf0 + 1; }
+        else { int e2 = e - 2 * nf0; d = 1; p = 1 + e2 / nf1; pos = e2 % nf1 + 1; }
+        const int face = 2 * d + p - 1;
+        if (!bc_interp(mth[face])) continue;
+        const int t = 1 - d;
+        face_index(P, d, p, fi[d]);
+        const int parity = (fi[d][0] + pos) & 1;
+        if (!(ipass == 0 || parity == ipass - 1)) continue;
+        const int k = pos_kind(pos, P.mx[t]);
+        g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
+        g[t] = pos; s1[t] = pos + kind_s1(k); s2[t] = pos + kind_s2(k);
+        if (k == KE || k == KO) {
+          cq[0][d] = fi[d][3]; cq[0][t] = kind_c(P, t, k, pos, 0);
+        } else {
+          nco = 3;
+          for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); w[r] = cB[r]; }
+        }
+      } else {
+        // corners (boundary.f90:155-181): applied whatever mthbc is
+        const int cn = e - (2 * nf0 + 2 * nf1);
+        const int p0 = 1 + (cn & 1), p1 = 1 + (cn >> 1);
+        face_index(P, 0, p0, fi[0]);
+        face_index(P, 1, p1, fi[1]);
+        const int parity = (fi[0][0] + fi[1][0]) & 1;
+        if (!(ipass == 0 || parity == ipass - 1)) continue;
+        for (int a = 0; a < 2; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
+      }
+      interp_store<2>(q, P, Q, nv, g, s1, s2, nco, cq, w);
+    }
+    return;
+  }
+  // 3D: faces (6), edges (12), corners (8)
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  const int nface[3] = {my * mz, mx * mz, mx * my};
+  const int nfaces = 2 * (nface[0] + nface[1] + nface[2]);
+  const int nedge[3] = {my, mz, mx};  // edges along dim 1 (e=0), dim 2 (e=1), dim 0 (e=2)
+  const int nedges = 4 * (nedge[0] + nedge[1] + nedge[2]);
+  const int total = nfaces + nedges + 8;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    int g[3], s1[3], s2[3], cq[4][3];
+    double w[4];
+    int nco = 1;
+    if (e < nfaces) {
+      int d = 0, rem = e;
+      while (rem >= 2 * nface[d]) { rem -= 2 * nface[d]; d++; }
+      const int p = 1 + rem / nface[d];
+      const int local = rem % nface[d];
+      const int face = 2 * d + p - 1;
+      if (!bc_interp(mth[face])) continue;
+      const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+      const int a = local % P.mx[t1] + 1, b = local / P.mx[t1] + 1;
+      const int k1 = pos_kind(a, P.mx[t1]), k2 = pos_kind(b, P.mx[t2]);
+      const bool q1 = k1 >= KL, q2 = k2 >= KL;
+      face_index(P, d, p, fi[d]);
+      g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
+      if (q1 && q2) {
+        // corner grey cells (boundary.f90:1320-1333 and the like): always
+        int ft1[5], ft2[5];
+        face_index(P, t1, a == 1 ? 1 : 2, ft1);
+        face_index(P, t2, b == 1 ? 1 : 2, ft2);
+        g[t1] = s1[t1] = s2[t1] = ft1[2];
+        g[t2] = s1[t2] = s2[t2] = ft2[2];
+        nco = 4;
+        for (int r = 0; r < 4; r++) cq[r][d] = fi[d][3];
+        cq[0][t1] = ft1[4]; cq[0][t2] = ft2[4];
+        cq[1][t1] = ft1[3]; cq[1][t2] = ft2[4];
+        cq[2][t1] = ft1[4]; cq[2][t2] = ft2[3];
+        cq[3][t1] = ft1[3]; cq[3][t2] = ft2[3];
+        for (int r = 0; r < 4; r++) w[r] = cC[r];
+      } else {
+        const int parity = (fi[d][0] + a + b) & 1;
+        if (!(ipass == 0 || parity == ipass - 1)) continue;
+        g[t1] = a; s1[t1] = a + kind_s1(k1); s2[t1] = a + kind_s2(k1);
+        g[t2] = b; s1[t2] = b + kind_s1(k2); s2[t2] = b + kind_s2(k2);
+        if (!q1 && !q2) {
+          cq[0][d] = fi[d][3]; cq[0][t1] = kind_c(P, t1, k1, a, 0); cq[0][t2] = kind_c(P, t2, k2, b, 0);
+        } else {
+          nco = 3;
+          for (int r = 0; r < 3; r++) {
+            cq[r][d] = fi[d][3];
+            cq[r][t1] = kind_c(P, t1, k1, a, q1 ? r : 0);
+            cq[r][t2] = kind_c(P, t2, k2, b, q2 ? r : 0);
+            w[r] = cB[r];
+          }
+        }
+      }
+    } else if (e < nfaces + nedges) {
+      int rem = e - nfaces, ee = 0;
+      while (rem >= 4 * nedge[ee]) { rem -= 4 * nedge[ee]; ee++; }
+      const int ed = (ee == 0) ? 1 : (ee == 1 ? 2 : 0);
+      const int fa = (ed == 0) ? 1 : 0, fb = (ed == 2) ? 1 : 2;
+      const int combo = rem / nedge[ee], pos = rem % nedge[ee] + 1;
+      const int pa = 1 + (combo >> 1), pb = 1 + (combo & 1);
+      if (!bc_interp(mth[2 * fa + pa - 1]) && ipass == 1) continue;
+      const int cb = mth[2 * fb + pb - 1];
+      if (ee == 0) {
+        if (cb != 2 && cb != 99 && ipass == 1) continue;  // boundary.f90:1104
+      } else if (!bc_interp(cb) && ipass == 1) {
+        continue;
+      }
+      face_index(P, fa, pa, fi[fa]);
+      face_index(P, fb, pb, fi[fb]);
+      const int k = pos_kind(pos, P.mx[ed]);
+      g[fa] = fi[fa][0]; s1[fa] = fi[fa][1]; s2[fa] = fi[fa][2];
+      g[fb] = fi[fb][0]; s1[fb] = fi[fb][1]; s2[fb] = fi[fb][2];
+      g[ed] = pos; s1[ed] = pos + kind_s1(k); s2[ed] = pos + kind_s2(k);
+      if (k == KE || k == KO) {
+        cq[0][fa] = fi[fa][3]; cq[0][fb] = fi[fb][3]; cq[0][ed] = kind_c(P, ed, k, pos, 0);
+      } else {
+        nco = 3;
+        for (int r = 0; r < 3; r++) {
+          cq[r][fa] = fi[fa][3]; cq[r][fb] = fi[fb][3]; cq[r][ed] = kind_c(P, ed, k, pos, r);
+          w[r] = cB[r];
+        }
+      }
+    } else {
+      const int cn = e - nfaces - nedges;
+      const int p1 = 1 + (cn & 1), p2 = 1 + ((cn >> 1) & 1), p3 = 1 + (cn >> 2);
+      if (!bc_interp(mth[p1 - 1]) && ipass == 1) continue;
+      if (!bc_interp(mth[p2 + 1]) && ipass == 1) continue;
+      if (!bc_interp(mth[p3 + 3]) && ipass == 1) continue;
+      face_index(P, 0, p1, fi[0]);
+      face_index(P, 1, p2, fi[1]);
+      face_index(P, 2, p3, fi[2]);
+      for (int a = 0; a < 3; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
+    }
+    interp_store<3>(q, P, Q, nv, g, s1, s2, nco, cq, w);
+  }
+}
+
+// Box copies of one ghost-fill stage: sibling transfers (GhostOverlap,
+// boundary.f90:359-431), self-periodic wraps (SelfPeriodicBC :480-577),
+// periodic transfers (TransferPeriodicBC :580-611) and physical boundary
+// conditions (SetPhysicalBC + UserBC).  One block per box.
+__global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt,
+                                                   const BoxDesc* __restrict__ boxes, int nv,
+                                                   double* __restrict__ q) {
+  const BoxDesc B = boxes[blockIdx.x];
+  const DPatch& D = pt[B.dp];
+  const DPatch& S = pt[B.sp];
+  const int n = B.ext[0] * B.ext[1] * B.ext[2];
+  for (int e = threadIdx.x; e < n * nv; e += blockDim.x) {
+    const int v = e / n, c = e % n;
+    const int a = c % B.ext[0], r = c / B.ext[0];
+    const int b = r % B.ext[1], z = r / B.ext[1];
+    const double x = q[qidx(S, B.slo[0] + a, B.slo[1] + b, B.slo[2] + z) + v * S.qvs];
+    double y;
+    switch (B.op) {
+      case kCopyPlus0: y = x + 0.0; break;
+      case kDirichlet2: y = 2.0 - x; break;
+      case kBgMixed: y = (v == 0) ? x : 2.0 - x; break;
+      default: y = x;
+    }
+    q[qidx(D, B.dlo[0] + a, B.dlo[1] + b, B.dlo[2] + z) + v * D.qvs] = y;
+  }
+}
+
+// Self-periodic wrap of a patch periodic in every dimension (SelfPeriodicBC
+// applied x, then y, then z): the net effect is that every ghost cell takes
+// the value of its periodic image, plus 0.0 (the qpo offset of the x wrap)
+// when the cell lies on an x ghost plane.  Domain: the ghost shell.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_wrap_full(const DPatch* __restrict__ pt, int pid, int nv,
+                                                      double* __restrict__ q) {
+  const DPatch& P = pt[pid];
+  const int nx = P.nq[0], ny = P.nq[1], nz = P.nq[2];
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  // shell enumeration: x planes (2*ny*nz), then y planes (2*(nx-2)*nz), then
+  // z planes (2*(nx-2)*(ny-2)) in 3D
+  const int nxp = 2 * ny * nz, nyp = 2 * (nx - 2) * nz, nzp = (DIM == 3) ? 2 * (nx - 2) * (ny - 2) : 0;
+  if (g >= nxp + nyp + nzp) return;
+  int i, j, k;
+  if (g < nxp) {
+    i = (g & 1) ? nx - 1 : 0;
+    const int r = g >> 1;
+    j = r % ny;
+    k = r / ny;
+  } else if (g < nxp + nyp) {
+    const int r0 = g - nxp;
+    j = (r0 & 1) ? ny - 1 : 0;
+    const int r = r0 >> 1;
+    i = r % (nx - 2) + 1;
+    k = r / (nx - 2);
+  } else {
+    const int r0 = g - nxp - nyp;
+    k = (r0 & 1) ? nz - 1 : 0;
+    const int r = r0 >> 1;
+    i = r % (nx - 2) + 1;
+    j = r / (nx - 2) + 1;
+  }
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  const int si = (i == 0) ? mx : (i == mx + 1 ? 1 : i);
+  const int sj = (j == 0) ? my : (j == my + 1 ? 1 : j);
+  const int sk = (DIM == 3) ? ((k == 0) ? mz : (k == mz + 1 ? 1 : k)) : 0;
+  const bool xg = (i == 0 || i == mx + 1);
+  for (int v = 0; v < nv; v++) {
+    const double x = q[qidx(P, si, sj, sk) + v * P.qvs];
+    q[qidx(P, i, j, k) + v * P.qvs] = xg ? x + 0.0 : x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion between the reference's column-major arrays (a staging
+// buffer) and the colour-split pools.  kind 0: q-shaped (ghosts included),
+// 1: interior-only (f, rf, ftmp), 2: qc-shaped.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_layout(const DPatch* __restrict__ pt, int pid, int kind,
+                                                   int nv, int to_pool, double* __restrict__ stage,
+                                                   double* __restrict__ pool) {
+  const DPatch& P = pt[pid];
+  int e0, e1, e2;
+  if (kind == 0) { e0 = P.nq[0]; e1 = P.nq[1]; e2 = P.nq[2]; }
+  else if (kind == 1) { e0 = P.mx[0]; e1 = P.mx[1]; e2 = P.mx[2]; }
+  else { e0 = P.nc[0]; e1 = P.nc[1]; e2 = P.nc[2]; }
+  const long n = (long)e0 * e1 * e2;
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= n * nv) return;
+  const int v = (int)(g / n);
+  const long c = g % n;
+  int i = (int)(c % e0);
+  const long r = c / e0;
+  int j = (int)(r % e1);
+  int k = (int)(r / e1);
+  int o;
+  if (kind == 1) {
+    i += 1; j += 1;
+    if (P.nq[2] > 1) k += 1;
+    o = qidx(P, i, j, k) + v * P.qvs;
+  } else if (kind == 0) {
+    o = qidx(P, i, j, k) + v * P.qvs;
+  } else {
+    o = cidx(P, i, j, k) + v * P.cvs;
+  }
+  if (to_pool) pool[o] = stage[g];
+  else stage[g] = pool[o];
+}
+
+}  // namespace
+
+// ============================================================================
+// host side
+// ============================================================================
+namespace {
+
+enum Dom { kHalf = 0, kCell, kCoarse, kFineG, kQcBox, kNumDom };
+
+struct Stage {
+  BoxDesc* d = nullptr;
+  int n = 0;
+};
+
+struct Level {
+  int level = 0;
+  std::vector<int> patches;
+  int* d_patches = nullptr;
+  int* d_prefix[kNumDom] = {};
+  int total[kNumDom] = {};
+  std::vector<Stage> stages;      // box stages after interpolation
+  int wrap_pid = -1;              // fully periodic single patch: fused wrap
+  double h = 0.0;
+  long qspan_off = 0, qspan_len = 0;  // contiguous q-pool span of the level
+};
+
+struct KStat {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // pairs
+  int64_t launches = 0;
+  double ms = 0.0;
+  double bytes = 0.0;
+};
+
+}  // namespace
+
+struct bsam_ctx {
+  bsam_params p{};
+  hipStream_t s = nullptr;
+  int dev = 0;
+  // hierarchy
+  int npatch = 0, minlevel = 0, finest = 0;
+  bool have_hier = false;
+  std::vector<bsam_patch_desc> desc;
+  std::vector<DPatch> hp;
+  DPatch* d_pt = nullptr;
+  double *q = nullptr, *qold = nullptr, *f = nullptr, *rf = nullptr, *ftmp = nullptr, *qc = nullptr;
+  long nq = 0, nc = 0;
+  std::vector<Level> lv;  // index level - minlevel
+  std::vector<BoxDesc*> box_allocs;
+  // reductions
+  double* d_partial = nullptr;
+  int partial_cap = 0;
+  double* d_norm = nullptr;
+  double* h_norm = nullptr;
+  // staging
+  double* d_stage = nullptr;
+  long stage_cap = 0;
+  // graph of the V-cycle loop body
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool capturing = false;
+  // instrumentation
+  KStat ks[BSAM_KSTAT_COUNT];
+  // mixed precision state
+  bsam::MixedState* mixed = nullptr;
+
+  Level& L(int level) { return lv[level - minlevel]; }
+  int nv() const { return p.nrvars; }
+  ModelK mk() const {
+    ModelK m;
+    m.dt = p.dt;
+    m.eps2 = p.model_params[0];
+    m.gam = p.model_params[1];
+    m.lam = p.model_params[2];
+    m.cz = p.model_params[3];
+    return m;
+  }
+};
+
+namespace {
+
+inline int nblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+LevelArgs largs(bsam_ctx* c, Level& L, Dom d) {
+  LevelArgs a;
+  a.pt = c->d_pt;
+  a.patches = L.d_patches;
+  a.prefix = L.d_prefix[d];
+  a.npatch = (int)L.patches.size();
+  a.total = L.total[d];
+  return a;
+}
+
+int32_t check_level(bsam_ctx* c, int level) {
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
+  if (level < c->minlevel || level > c->finest)
+    return set_err(BSAM_EINVAL, "level %d outside [%d, %d]", level, c->minlevel, c->finest);
+  return BSAM_OK;
+}
+
+#define LAUNCH_CHECK()                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess) return set_err(BSAM_EHIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+void kstat_begin(bsam_ctx* c, int which) {
+  KStat& k = c->ks[which];
+  if (!k.on || c->capturing) return;
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  hipEventRecord(a, c->s);
+  k.ev.push_back(a);
+  k.ev.push_back(b);
+}
+void kstat_end(bsam_ctx* c, int which, double bytes) {
+  KStat& k = c->ks[which];
+  if (!k.on || c->capturing) return;
+  hipEventRecord(k.ev.back(), c->s);
+  k.launches++;
+  k.bytes = bytes;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int32_t bsam_create(const bsam_params* params, bsam_ctx** out) {
+  if (!params || !out) return set_err(BSAM_EINVAL, "null argument");
+  *out = nullptr;
+  const bsam_params& p = *params;
+  if (p.ndims != 2 && p.ndims != 3) return set_err(BSAM_EINVAL, "ndims must be 2 or 3");
+  if (p.nrvars != 2) return set_err(BSAM_EINVAL, "nrvars must be 2 (CH and BG are 2-field models)");
+  if (p.mbc != 1) return set_err(BSAM_EINVAL, "only mbc=1 is supported");
+  if (p.maux != 0) return set_err(BSAM_EINVAL, "maux must be 0");
+  if (p.model != BSAM_MODEL_CH && p.model != BSAM_MODEL_BG)
+    return set_err(BSAM_EINVAL, "unknown model %d", p.model);
+  if (p.model == BSAM_MODEL_BG && p.ndims != 2)
+    return set_err(BSAM_EINVAL, "the BioGrowth hooks exist in 2D only (their 3D forms are stubs)");
+  if (p.errortype != 1 && p.errortype != 2)
+    return set_err(BSAM_EINVAL, "ErrorAFAS: only errortype = 1,2 are supported");
+  if (p.nsmoothingpasses < 1 || p.maxvcycles < 1) return set_err(BSAM_EINVAL, "bad V-cycle knobs");
+  if (p.precision != BSAM_PREC_FP64 && p.precision != BSAM_PREC_MIXED)
+    return set_err(BSAM_EINVAL, "unknown precision mode %d", p.precision);
+  if (p.precision == BSAM_PREC_MIXED && p.model != BSAM_MODEL_CH)
+    return set_err(BSAM_EINVAL, "mixed precision is implemented for the CH model only");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(BSAM_ENODEV, "no HIP device visible");
+  bsam_ctx* c = new bsam_ctx();
+  c->p = p;
+  hipGetDevice(&c->dev);
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_err(BSAM_EHIP, "hipStreamCreate failed");
+  }
+  if (hipMalloc(&c->d_norm, 4 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&c->h_norm, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    bsam_destroy(c);
+    return set_err(BSAM_ENOMEM, "allocation of reduction buffers failed");
+  }
+  *out = c;
+  return BSAM_OK;
+}
+
+static void free_hierarchy(bsam_ctx* c) {
+  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  for (Level& L : c->lv) {
+    hipFree(L.d_patches);
+    for (int d = 0; d < kNumDom; d++) hipFree(L.d_prefix[d]);
+  }
+  c->lv.clear();
+  for (BoxDesc* b : c->box_allocs) hipFree(b);
+  c->box_allocs.clear();
+  hipFree(c->d_pt); hipFree(c->q); hipFree(c->qold); hipFree(c->f); hipFree(c->rf); hipFree(c->ftmp);
+  hipFree(c->qc);
+  c->d_pt = nullptr; c->q = c->qold = c->f = c->rf = c->ftmp = c->qc = nullptr;
+  if (c->mixed) { bsam::mixed_free(c->mixed); c->mixed = nullptr; }
+  c->have_hier = false;
+}
+
+extern "C" int32_t bsam_destroy(bsam_ctx* c) {
+  if (!c) return BSAM_OK;
+  if (c->s) hipStreamSynchronize(c->s);
+  free_hierarchy(c);
+  hipFree(c->d_partial);
+  hipFree(c->d_norm);
+  hipFree(c->d_stage);
+  if (c->h_norm) hipHostFree(c->h_norm);
+  for (auto& k : c->ks)
+    for (hipEvent_t e : k.ev) hipEventDestroy(e);
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_set_dt(bsam_ctx* c, double dt) {
+  if (!c) return set_err(BSAM_EINVAL, "null context");
+  c->p.dt = dt;
+  // the graph bakes dt into its kernel arguments
+  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hierarchy installation: pools, level tables, ghost-fill stages
+// ---------------------------------------------------------------------------
+namespace {
+
+// GhostOverlap (boundary.f90:359-431) as two box copies; `off` shifts grid 2.
+void ghost_overlap_boxes(const bsam_ctx* c, int g1, int g2, const int off[3], int op,
+                         std::vector<BoxDesc>& out) {
+  const int nd = c->p.ndims;
+  const bsam_patch_desc& A = c->desc[g1];
+  const bsam_patch_desc& B = c->desc[g2];
+  int mg1[3][2], mg2[3][2], ovg[3][2];
+  for (int n = 0; n < nd; n++)
+    for (int s = 0; s < 2; s++) {
+      mg1[n][s] = A.mglobal[n][s];
+      mg2[n][s] = B.mglobal[n][s] + (off ? off[n] : 0);
+    }
+  for (int n = 0; n < nd; n++) {
+    ovg[n][0] = std::max(mg1[n][0] - 1, mg2[n][0] - 1);
+    ovg[n][1] = std::min(mg1[n][1] + 1, mg2[n][1] + 1);
+    if (ovg[n][1] - ovg[n][0] < 0) return;
+  }
+  for (int dir = 0; dir < 2; dir++) {
+    // dir 0: grid1 <- grid2 over ovg & mg2; dir 1: grid2 <- grid1 over ovg & mg1
+    const int (*ms)[2] = dir == 0 ? mg2 : mg1;  // source interior box
+    const int (*md)[2] = dir == 0 ? mg1 : mg2;
+    BoxDesc b{};
+    b.dp = dir == 0 ? g1 : g2;
+    b.sp = dir == 0 ? g2 : g1;
+    b.op = op;
+    bool empty = false;
+    for (int n = 0; n < 3; n++) {
+      if (n < nd) {
+        const int lo = std::max(ovg[n][0], ms[n][0]), hi = std::min(ovg[n][1], ms[n][1]);
+        if (hi < lo) empty = true;
+        b.dlo[n] = lo - md[n][0] + 1;
+        b.slo[n] = lo - ms[n][0] + 1;
+        b.ext[n] = hi - lo + 1;
+      } else {
+        b.dlo[n] = b.slo[n] = 0;
+        b.ext[n] = 1;
+      }
+    }
+    if (!empty) out.push_back(b);
+  }
+}
+
+// PeriodicSetup (boundary.f90:805-872) + GetPeriodicOffsets (:875-954).
+void periodic_offsets(const bsam_ctx* c, int level, std::vector<std::array<int, 3>>& offs, bool& periodic) {
+  static const int tab[13][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {-1, 1, 0},
+                                 {0, 1, 1}, {0, 1, -1}, {1, 0, 1}, {-1, 0, 1}, {1, 1, 1},
+                                 {1, 1, -1}, {-1, 1, 1}, {1, -1, 1}};
+  int m[6];
+  for (int i = 0; i < 6; i++) m[i] = 10;
+  const bsam_patch_desc* root = nullptr;
+  for (auto& d : c->desc)
+    if (d.level == 0) { root = &d; break; }
+  for (int i = 0; i < 2 * c->p.ndims; i++) m[i] = root->mthbc[i];
+  std::vector<int> idx;
+  periodic = true;
+  if (m[0] == 2) {
+    if (m[2] == 2) idx = (m[4] == 2) ? std::vector<int>{1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13}
+                                     : std::vector<int>{1, 2, 4, 5};
+    else idx = (m[4] == 2) ? std::vector<int>{1, 3, 8, 9} : std::vector<int>{1};
+  } else {
+    if (m[2] == 2) idx = (m[4] == 2) ? std::vector<int>{2, 3, 6, 7} : std::vector<int>{2};
+    else if (m[4] == 2) idx = {3};
+    else periodic = false;
+  }
+  offs.clear();
+  if (level < 0) return;
+  const int s = 1 << level;
+  for (int x : idx) {
+    std::array<int, 3> o;
+    for (int a = 0; a < 3; a++) o[a] = tab[x - 1][a] * root->mx[a] * s;
+    offs.push_back(o);
+  }
+}
+
+int32_t upload_stage(bsam_ctx* c, Level& L, const std::vector<BoxDesc>& boxes) {
+  if (boxes.empty()) return BSAM_OK;
+  BoxDesc* d = nullptr;
+  HIPCHK(hipMalloc(&d, boxes.size() * sizeof(BoxDesc)));
+  c->box_allocs.push_back(d);
+  HIPCHK(hipMemcpy(d, boxes.data(), boxes.size() * sizeof(BoxDesc), hipMemcpyHostToDevice));
+  Stage s;
+  s.d = d;
+  s.n = (int)boxes.size();
+  L.stages.push_back(s);
+  return BSAM_OK;
+}
+
+int32_t build_ghost_stages(bsam_ctx* c, Level& L) {
+  const int nd = c->p.ndims;
+  std::vector<std::array<int, 3>> offs;
+  bool periodic = false;
+  periodic_offsets(c, L.level, offs, periodic);
+  std::vector<BoxDesc> sib, px, py, pz, pp, phys[3];
+  const auto& ps = L.patches;
+  if (L.level > 0)
+    for (size_t a = 0; a < ps.size(); a++)
+      for (size_t b = a + 1; b < ps.size(); b++) ghost_overlap_boxes(c, ps[a], ps[b], nullptr, kCopy, sib);
+  bool all_wrap = periodic && ps.size() == 1;
+  if (periodic) {
+    for (int pid : ps) {
+      const bsam_patch_desc& d = c->desc[pid];
+      const DPatch& P = c->hp[pid];
+      for (int dim = 0; dim < nd; dim++) {
+        if (!(d.mthbc[2 * dim] == 2 && d.mthbc[2 * dim + 1] == 2)) {
+          all_wrap = false;
+          continue;
+        }
+        std::vector<BoxDesc>& dst = dim == 0 ? px : (dim == 1 ? py : pz);
+        for (int side = 0; side < 2; side++) {
+          BoxDesc b{};
+          b.dp = b.sp = pid;
+          b.op = (dim == 0 && side == 0) ? kCopyPlus0 : kCopy;
+          for (int n = 0; n < 3; n++) {
+            b.dlo[n] = b.slo[n] = 0;
+            b.ext[n] = P.nq[n];
+          }
+          b.ext[dim] = 1;
+          b.dlo[dim] = side == 0 ? P.mx[dim] + 1 : 0;
+          b.slo[dim] = side == 0 ? 1 : P.mx[dim];
+          dst.push_back(b);
+        }
+      }
+    }
+    if (L.level > 0)
+      for (size_t a = 0; a < ps.size(); a++)
+        for (size_t b = a + 1; b < ps.size(); b++)
+          for (int pol = -1; pol <= 1; pol += 2)
+            for (auto& o : offs) {
+              int off[3] = {pol * o[0], pol * o[1], pol * o[2]};
+              ghost_overlap_boxes(c, ps[a], ps[b], off, kCopy, pp);
+            }
+  }
+  // physical faces (SetPhysicalBC2D/3D, boundary.f90:642-802)
+  bool has_phys = false;
+  for (int pid : ps) {
+    const bsam_patch_desc& d = c->desc[pid];
+    const DPatch& P = c->hp[pid];
+    for (int face = 0; face < 2 * nd; face++) {
+      const int code = d.mthbc[face];
+      if (code == 2 || code == 999) continue;
+      if (code != 10)
+        return set_err(BSAM_EINVAL, "SetPhysicalBC: user boundary condition %d on boundary %d not coded", code,
+                       face + 1);
+      has_phys = true;
+      const int dim = face / 2, hi = face % 2;
+      BoxDesc b{};
+      b.dp = b.sp = pid;
+      if (c->p.model == BSAM_MODEL_CH) b.op = (nd == 3) ? kDirichlet2 : kCopy;
+      else b.op = kBgMixed;
+      for (int n = 0; n < 3; n++) {
+        b.dlo[n] = b.slo[n] = 0;
+        b.ext[n] = P.nq[n];
+      }
+      b.ext[dim] = 1;
+      b.dlo[dim] = hi ? P.mx[dim] + 1 : 0;
+      b.slo[dim] = hi ? P.mx[dim] : 1;
+      phys[dim].push_back(b);
+    }
+  }
+  if (has_phys) all_wrap = false;
+  L.stages.clear();
+  L.wrap_pid = -1;
+  if (all_wrap && L.level <= 0) {
+    L.wrap_pid = ps[0];
+    return BSAM_OK;
+  }
+  const bool no_self = px.empty() && py.empty() && pz.empty();
+  if (no_self) {
+    std::vector<BoxDesc> m = sib;
+    m.insert(m.end(), pp.begin(), pp.end());
+    RETCHK(upload_stage(c, L, m));
+  } else {
+    RETCHK(upload_stage(c, L, sib));
+    RETCHK(upload_stage(c, L, px));
+    RETCHK(upload_stage(c, L, py));
+    RETCHK(upload_stage(c, L, pz));
+    RETCHK(upload_stage(c, L, pp));
+  }
+  for (int dim = 0; dim < 3; dim++) RETCHK(upload_stage(c, L, phys[dim]));
+  return BSAM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc,
+                                      int32_t minlevel, int32_t finestlevel) {
+  if (!c || !desc || npatch < 1) return set_err(BSAM_EINVAL, "bad arguments");
+  if (minlevel > 0 || finestlevel < 0 || minlevel < -30 || finestlevel > 30)
+    return set_err(BSAM_EINVAL, "level range [%d, %d] invalid", minlevel, finestlevel);
+  HIPCHK(hipStreamSynchronize(c->s));
+  free_hierarchy(c);
+  const int nd = c->p.ndims, nv = c->p.nrvars;
+  c->desc.assign(desc, desc + npatch);
+  c->npatch = npatch;
+  c->minlevel = minlevel;
+  c->finest = finestlevel;
+  c->hp.resize(npatch);
+  long qo = 0, co = 0;
+  int nroot = 0;
+  for (int n = 0; n < npatch; n++) {
+    const bsam_patch_desc& d = desc[n];
+    DPatch& P = c->hp[n];
+    memset(&P, 0, sizeof P);
+    if (d.level < minlevel || d.level > finestlevel)
+      return set_err(BSAM_EINVAL, "patch %d: level %d outside [%d, %d]", n, d.level, minlevel, finestlevel);
+    if (d.level == 0) nroot++;
+    if (d.level > minlevel) {
+      if (d.parent < 0 || d.parent >= npatch || desc[d.parent].level != d.level - 1)
+        return set_err(BSAM_EINVAL, "patch %d: parent %d is not on level %d", n, d.parent, d.level - 1);
+    }
+    for (int a = 0; a < 3; a++) {
+      const bool used = a < nd;
+      P.mx[a] = used ? d.mx[a] : 1;
+      if (used && P.mx[a] < 2) return set_err(BSAM_EINVAL, "patch %d: mx[%d]=%d < 2", n, a, P.mx[a]);
+      P.cmx[a] = used ? ((P.mx[a] % 2) ? 1 : P.mx[a] / 2) : 1;
+      P.nq[a] = used ? P.mx[a] + 2 : 1;
+      P.nc[a] = used ? P.cmx[a] + 2 : 1;
+      P.mb[a][0] = used ? d.mbounds[a][0] : 0;
+      P.mb[a][1] = used ? d.mbounds[a][1] : 0;
+    }
+    if (d.level > minlevel) {
+      for (int a = 0; a < nd; a++) {
+        if (P.mx[a] % 2) return set_err(BSAM_EINVAL, "patch %d: odd mx on a refined level", n);
+        if (d.mbounds[a][1] - d.mbounds[a][0] + 1 != P.mx[a] / 2)
+          return set_err(BSAM_EINVAL, "patch %d: mbounds do not span mx/2 in dim %d", n, a);
+        const int pmx = desc[d.parent].mx[a];
+        if (d.mbounds[a][0] < 1 || d.mbounds[a][1] > pmx)
+          return set_err(BSAM_EINVAL, "patch %d: mbounds outside the parent in dim %d", n, a);
+      }
+    }
+    for (int f = 0; f < 6; f++) P.mthbc[f] = d.mthbc[f];
+    P.hq = (P.nq[0] + 1) / 2;
+    P.hc = (P.nc[0] + 1) / 2;
+    P.parent = d.level > minlevel ? d.parent : -1;
+    P.level = d.level;
+    P.dx = d.dx;
+    P.qvs = 2 * P.nq[2] * P.nq[1] * P.hq;
+    P.cvs = 2 * P.nc[2] * P.nc[1] * P.hc;
+    P.qoff = (int)qo;
+    P.coff = (int)co;
+    qo += (long)P.qvs * nv;
+    co += (long)P.cvs * nv;
+    if (qo > 0x7fffffffL || co > 0x7fffffffL)
+      return set_err(BSAM_EINVAL, "hierarchy too large for 32-bit offsets (%ld elements)", qo);
+  }
+  if (nroot != 1) return set_err(BSAM_EINVAL, "exactly one root-level patch required (found %d)", nroot);
+  c->nq = qo;
+  c->nc = co;
+  HIPCHK(hipMalloc(&c->d_pt, npatch * sizeof(DPatch)));
+  HIPCHK(hipMemcpy(c->d_pt, c->hp.data(), npatch * sizeof(DPatch), hipMemcpyHostToDevice));
+  double** pools[5] = {&c->q, &c->qold, &c->f, &c->rf, &c->ftmp};
+  for (double** pp : pools) {
+    if (hipMalloc(pp, qo * sizeof(double)) != hipSuccess)
+      return set_err(BSAM_ENOMEM, "device pool of %ld doubles", qo);
+    HIPCHK(hipMemset(*pp, 0, qo * sizeof(double)));
+  }
+  if (hipMalloc(&c->qc, co * sizeof(double)) != hipSuccess) return set_err(BSAM_ENOMEM, "qc pool");
+  HIPCHK(hipMemset(c->qc, 0, co * sizeof(double)));
+  // levels
+  c->lv.assign(finestlevel - minlevel + 1, Level());
+  for (int l = minlevel; l <= finestlevel; l++) c->L(l).level = l;
+  for (int n = 0; n < npatch; n++) c->L(desc[n].level).patches.push_back(n);
+  int maxblocks = 1;
+  for (Level& L : c->lv) {
+    if (L.patches.empty()) return set_err(BSAM_EINVAL, "level %d has no patches", L.level);
+    L.h = c->hp[L.patches[0]].dx;
+    HIPCHK(hipMalloc(&L.d_patches, L.patches.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), L.patches.size() * sizeof(int), hipMemcpyHostToDevice));
+    for (int d = 0; d < kNumDom; d++) {
+      std::vector<int> pre(L.patches.size() + 1, 0);
+      long acc = 0;
+      for (size_t a = 0; a < L.patches.size(); a++) {
+        const DPatch& P = c->hp[L.patches[a]];
+        long cnt;
+        switch (d) {
+          case kHalf: cnt = (long)((P.mx[0] + 1) / 2) * P.mx[1] * P.mx[2]; break;
+          case kCell: cnt = (long)P.mx[0] * P.mx[1] * P.mx[2]; break;
+          case kCoarse: cnt = (long)P.cmx[0] * P.cmx[1] * P.cmx[2]; break;
+          case kFineG: cnt = (long)P.nq[0] * P.nq[1] * P.nq[2]; break;
+          default: cnt = (long)P.nc[0] * P.nc[1] * P.nc[2]; break;
+        }
+        pre[a] = (int)acc;
+        acc += cnt;
+      }
+      pre[L.patches.size()] = (int)acc;
+      if (acc > 0x7fffffffL) return set_err(BSAM_EINVAL, "level %d too large", L.level);
+      L.total[d] = (int)acc;
+      HIPCHK(hipMalloc(&L.d_prefix[d], pre.size() * sizeof(int)));
+      HIPCHK(hipMemcpy(L.d_prefix[d], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    maxblocks = std::max(maxblocks, nblocks(L.total[kCell]));
+    // contiguous span of the level in the q pools (patches are laid out in
+    // descriptor order; levels are spans only if their patches are
+    // consecutive -- otherwise copies go patch by patch)
+    L.qspan_off = c->hp[L.patches[0]].qoff;
+    long end = L.qspan_off;
+    bool contiguous = true;
+    for (int pid : L.patches) {
+      if (c->hp[pid].qoff != end) contiguous = false;
+      end = c->hp[pid].qoff + (long)c->hp[pid].qvs * nv;
+    }
+    L.qspan_len = contiguous ? end - L.qspan_off : -1;
+    RETCHK(build_ghost_stages(c, L));
+  }
+  if (maxblocks > c->partial_cap) {
+    hipFree(c->d_partial);
+    HIPCHK(hipMalloc(&c->d_partial, 2 * (long)maxblocks * sizeof(double)));
+    c->partial_cap = maxblocks;
+  }
+  c->have_hier = true;
+  if (c->p.precision == BSAM_PREC_MIXED) {
+    RETCHK(bsam::mixed_init(c));
+  }
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// upload / download
+// ---------------------------------------------------------------------------
+namespace {
+int field_kind(int field) {
+  switch (field) {
+    case BSAM_FIELD_Q: case BSAM_FIELD_QOLD: return 0;
+    case BSAM_FIELD_F: case BSAM_FIELD_RF: case BSAM_FIELD_FTMP: return 1;
+    case BSAM_FIELD_QC: return 2;
+  }
+  return -1;
+}
+double* field_pool(bsam_ctx* c, int field) {
+  switch (field) {
+    case BSAM_FIELD_Q: return c->q;
+    case BSAM_FIELD_QOLD: return c->qold;
+    case BSAM_FIELD_F: return c->f;
+    case BSAM_FIELD_RF: return c->rf;
+    case BSAM_FIELD_FTMP: return c->ftmp;
+    case BSAM_FIELD_QC: return c->qc;
+  }
+  return nullptr;
+}
+}  // namespace
+
+extern "C" int64_t bsam_field_size(const bsam_ctx* c, int32_t patch, int32_t field) {
+  if (!c || !c->have_hier || patch < 0 || patch >= c->npatch) return -1;
+  const DPatch& P = c->hp[patch];
+  const int kind = field_kind(field);
+  if (kind == 0) return (int64_t)P.nq[0] * P.nq[1] * P.nq[2] * c->p.nrvars;
+  if (kind == 1) return (int64_t)P.mx[0] * P.mx[1] * P.mx[2] * c->p.nrvars;
+  if (kind == 2) return (int64_t)P.nc[0] * P.nc[1] * P.nc[2] * c->p.nrvars;
+  return -1;
+}
+
+static int32_t xfer(bsam_ctx* c, int32_t patch, int32_t field, double* host, int64_t count, int to_dev) {
+  if (!c || !host) return set_err(BSAM_EINVAL, "null argument");
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
+  if (patch < 0 || patch >= c->npatch) return set_err(BSAM_EINVAL, "patch %d out of range", patch);
+  const int kind = field_kind(field);
+  if (kind < 0) return set_err(BSAM_EINVAL, "unknown field %d", field);
+  const int64_t n = bsam_field_size(c, patch, field);
+  if (count != n) return set_err(BSAM_EINVAL, "field holds %lld doubles, buffer %lld", (long long)n, (long long)count);
+  if (n > c->stage_cap) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    hipFree(c->d_stage);
+    c->d_stage = nullptr;
+    if (hipMalloc(&c->d_stage, n * sizeof(double)) != hipSuccess) return set_err(BSAM_ENOMEM, "staging buffer");
+    c->stage_cap = n;
+  }
+  double* pool = field_pool(c, field);
+  if (to_dev) {
+    HIPCHK(hipMemcpyAsync(c->d_stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->s));
+    k_layout<<<nblocks(n), kBlock, 0, c->s>>>(c->d_pt, patch, kind, c->p.nrvars, 1, c->d_stage, pool);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->s));
+  } else {
+    k_layout<<<nblocks(n), kBlock, 0, c->s>>>(c->d_pt, patch, kind, c->p.nrvars, 0, c->d_stage, pool);
+    LAUNCH_CHECK();
+    HIPCHK(hipMemcpyAsync(host, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_upload(bsam_ctx* c, int32_t patch, int32_t field, const double* host, int64_t count) {
+  return xfer(c, patch, field, const_cast<double*>(host), count, 1);
+}
+extern "C" int32_t bsam_download(bsam_ctx* c, int32_t patch, int32_t field, double* host, int64_t count) {
+  return xfer(c, patch, field, host, count, 0);
+}
+
+// ---------------------------------------------------------------------------
+// per-level operations
+// ---------------------------------------------------------------------------
+#define DISPATCH_DIM(nd, KCALL2, KCALL3) \
+  do {                                   \
+    if ((nd) == 2) { KCALL2; } else { KCALL3; } \
+  } while (0)
+
+extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
+  RETCHK(check_level(c, level));
+  if (ipass != 1 && ipass != 2) return set_err(BSAM_EINVAL, "ipass must be 1 or 2");
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kHalf);
+  const int nb = nblocks(a.total);
+  const int col = ipass - 1;
+  const bool fin = level == c->finest;
+  if (fin) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
+  if (c->p.model == BSAM_MODEL_CH) {
+    const double eps2 = c->p.model_params[0];
+    if (c->p.ndims == 2)
+      k_relax_ch<2><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
+    else
+      k_relax_ch<3><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
+  } else {
+    k_relax_bg2d<<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), L.h, c->q, c->qold, c->f);
+  }
+  LAUNCH_CHECK();
+  if (fin) {
+    // algorithmic bytes of one colour sweep (CH): phi over N+G_f, mu of the
+    // other colour over (N+G_f)/2, f of the colour (2 fields) N/2, and the
+    // colour's phi and mu written, N/2 each.
+    double N = 0, G = 0;
+    for (int pid : L.patches) {
+      const DPatch& P = c->hp[pid];
+      const double n = (double)P.mx[0] * P.mx[1] * P.mx[2];
+      N += n;
+      G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
+                             : 2.0 * ((double)P.mx[0] + P.mx[1]);
+    }
+    const double bytes = (c->p.model == BSAM_MODEL_CH)
+                             ? 8.0 * ((N + G) + 0.5 * (N + G) + N + N)
+                             : 8.0 * (2.0 * (N + G) + 0.5 * N + N + N);
+    kstat_end(c, BSAM_KSTAT_RELAX_FINEST, bytes);
+  }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_set_ghost(bsam_ctx* c, int32_t level, int32_t ipass) {
+  RETCHK(check_level(c, level));
+  if (ipass < 0 || ipass > 2) return set_err(BSAM_EINVAL, "ipass must be 0, 1 or 2");
+  Level& L = c->L(level);
+  const int nv = c->p.nrvars;
+  if (level > 0) {
+    const int np = (int)L.patches.size();
+    if (c->p.ndims == 2) k_interp<2><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, c->q);
+    else k_interp<3><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, c->q);
+    LAUNCH_CHECK();
+  }
+  if (L.wrap_pid >= 0) {
+    const DPatch& P = c->hp[L.wrap_pid];
+    const long shell = (long)P.nq[0] * P.nq[1] * P.nq[2] - (long)P.mx[0] * P.mx[1] * P.mx[2];
+    if (c->p.ndims == 2) k_wrap_full<2><<<nblocks(shell), kBlock, 0, c->s>>>(c->d_pt, L.wrap_pid, nv, c->q);
+    else k_wrap_full<3><<<nblocks(shell), kBlock, 0, c->s>>>(c->d_pt, L.wrap_pid, nv, c->q);
+    LAUNCH_CHECK();
+  }
+  for (const Stage& s : L.stages) {
+    k_boxops<<<s.n, kBlock, 0, c->s>>>(c->d_pt, s.d, nv, c->q);
+    LAUNCH_CHECK();
+  }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_level_relax(bsam_ctx* c, int32_t level) {
+  RETCHK(check_level(c, level));
+  for (int s = 0; s < c->p.nsmoothingpasses; s++)
+    for (int rb = 1; rb <= 2; rb++) {
+      RETCHK(bsam_relax_level(c, level, rb));
+      RETCHK(bsam_set_ghost(c, level, rb));
+    }
+  return BSAM_OK;
+}
+
+static int32_t need_parent(bsam_ctx* c, int level) {
+  RETCHK(check_level(c, level));
+  if (level <= c->minlevel) return set_err(BSAM_EINVAL, "level %d has no parent level", level);
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_restrict_solution(bsam_ctx* c, int32_t level) {
+  RETCHK(need_parent(c, level));
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kCoarse);
+  DISPATCH_DIM(c->p.ndims, (k_restrict<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->q, c->qc)),
+               (k_restrict<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->q, c->qc)));
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_coarse_ghost_points(bsam_ctx* c, int32_t level) {
+  RETCHK(need_parent(c, level));
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kQcBox);
+  DISPATCH_DIM(c->p.ndims, (k_coarse_ring<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)),
+               (k_coarse_ring<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)));
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_coarse_loading(bsam_ctx* c, int32_t level) {
+  RETCHK(need_parent(c, level));
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kCoarse);
+  const int nb = nblocks(a.total);
+  const ModelK m = c->mk();
+  const bool fin = level == c->finest;
+  if (fin) kstat_begin(c, BSAM_KSTAT_FASRHS_FINEST);
+  if (c->p.model == BSAM_MODEL_CH) {
+    DISPATCH_DIM(c->p.ndims,
+                 (k_fas_rhs<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)),
+                 (k_fas_rhs<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)));
+  } else {
+    k_fas_rhs<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f);
+  }
+  LAUNCH_CHECK();
+  if (fin) {
+    double N = 0, G = 0;
+    for (int pid : L.patches) {
+      const DPatch& P = c->hp[pid];
+      N += (double)P.mx[0] * P.mx[1] * P.mx[2];
+      G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
+                             : 2.0 * ((double)P.mx[0] + P.mx[1]);
+    }
+    const double r = (c->p.ndims == 3) ? 8.0 : 4.0;
+    // q over N+G_f, f over N, qc over N/2^d plus its ring (~G_f/2^(d-1)), parent f write N/2^d
+    const double bytes = 8.0 * 2.0 * ((N + G) + N + (N / r + G / (r / 2.0)) + N / r);
+    kstat_end(c, BSAM_KSTAT_FASRHS_FINEST, bytes);
+  }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_correct_fine(bsam_ctx* c, int32_t level) {
+  RETCHK(need_parent(c, level));
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kQcBox);
+  LevelArgs b = largs(c, L, kFineG);
+  DISPATCH_DIM(c->p.ndims, (k_correct_qc<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)),
+               (k_correct_qc<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)));
+  LAUNCH_CHECK();
+  DISPATCH_DIM(c->p.ndims, (k_prolong_add<2><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), c->q, c->qc)),
+               (k_prolong_add<3><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), c->q, c->qc)));
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_fill_down(bsam_ctx* c, int32_t field) {
+  if (!c || !c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
+  if (field != BSAM_SOURCEFIELD && field != BSAM_SOLUTIONFIELD) return set_err(BSAM_EINVAL, "bad field %d", field);
+  for (int level = c->finest; level >= c->minlevel + 1; level--) {
+    Level& L = c->L(level);
+    LevelArgs a = largs(c, L, kCoarse);
+    double* fld = (field == BSAM_SOURCEFIELD) ? c->f : c->q;
+    DISPATCH_DIM(c->p.ndims, (k_restrict<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), fld, fld, nullptr)),
+                 (k_restrict<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), fld, fld, nullptr)));
+    LAUNCH_CHECK();
+    if (field == BSAM_SOLUTIONFIELD) RETCHK(bsam_set_ghost(c, level - 1, 0));
+  }
+  return BSAM_OK;
+}
+
+static int32_t source_launch(bsam_ctx* c, int level, int mode) {
+  RETCHK(check_level(c, level));
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kCell);
+  const int nb = nblocks(a.total);
+  if (c->p.model == BSAM_MODEL_CH) {
+    DISPATCH_DIM(c->p.ndims,
+                 (k_source<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)),
+                 (k_source<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)));
+  } else {
+    k_source<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f);
+  }
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_get_source(bsam_ctx* c, int32_t level) { return source_launch(c, level, 0); }
+extern "C" int32_t bsam_update_source(bsam_ctx* c, int32_t level) { return source_launch(c, level, 1); }
+extern "C" int32_t bsam_update_aux(bsam_ctx* c, int32_t level) {
+  // maux = 0: UpdateAuxInVcycle returns before calling the hook
+  // (afasroutines.f90:417).
+  return check_level(c, level);
+}
+
+static int32_t residual_launch(bsam_ctx* c, int level, double* rf, double* partial) {
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kCell);
+  const int nb = nblocks(a.total);
+  const ModelK m = c->mk();
+  if (c->p.model == BSAM_MODEL_CH) {
+    DISPATCH_DIM(c->p.ndims,
+                 (k_residual<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)),
+                 (k_residual<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)));
+  } else {
+    k_residual<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial);
+  }
+  LAUNCH_CHECK();
+  return nb;
+}
+
+extern "C" int32_t bsam_residual(bsam_ctx* c, int32_t level) {
+  RETCHK(check_level(c, level));
+  RETCHK(residual_launch(c, level, c->rf, nullptr));
+  return BSAM_OK;
+}
+
+// Enqueue the residual norm of `level` into d_norm[0..1] (no wait).
+static int32_t enqueue_norm(bsam_ctx* c, int level) {
+  const bool fin = level == c->finest;
+  if (fin) kstat_begin(c, BSAM_KSTAT_RESID_FINEST);
+  int nb = residual_launch(c, level, c->rf, c->d_partial);
+  if (nb < 0) return nb;
+  if (fin) {
+    double N = 0, G = 0;
+    for (int pid : c->L(level).patches) {
+      const DPatch& P = c->hp[pid];
+      N += (double)P.mx[0] * P.mx[1] * P.mx[2];
+      G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
+                             : 2.0 * ((double)P.mx[0] + P.mx[1]);
+    }
+    kstat_end(c, BSAM_KSTAT_RESID_FINEST, 8.0 * 2.0 * ((N + G) + N + N));
+  }
+  k_reduce_partials<<<1, kBlock, 0, c->s>>>(c->d_partial, nb, c->d_norm);
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
+static double norm_from(bsam_ctx* c, const double* s, int level) {
+  double cells = 0;
+  for (int pid : c->L(level).patches) {
+    const DPatch& P = c->hp[pid];
+    cells += (double)P.mx[0] * P.mx[1] * P.mx[2];
+  }
+  if (c->p.errortype == 1) return std::sqrt((s[0] + s[1]) / (c->p.nrvars * cells));
+  return std::max(std::sqrt(s[0] / cells), std::sqrt(s[1] / cells));
+}
+
+extern "C" int32_t bsam_error_afas(bsam_ctx* c, int32_t level, double* norm) {
+  RETCHK(check_level(c, level));
+  if (!norm) return set_err(BSAM_EINVAL, "null norm");
+  RETCHK(enqueue_norm(c, level));
+  HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  *norm = norm_from(c, c->h_norm, level);
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_copy_q_to_qold(bsam_ctx* c, int32_t level) {
+  RETCHK(check_level(c, level));
+  Level& L = c->L(level);
+  if (L.qspan_len > 0) {
+    // one copy of the level's span covers every variable of every patch
+    k_copy_span<<<nblocks(L.qspan_len), kBlock, 0, c->s>>>(c->q + L.qspan_off, c->qold + L.qspan_off, L.qspan_len);
+    LAUNCH_CHECK();
+  } else {
+    for (int pid : L.patches) {
+      const DPatch& P = c->hp[pid];
+      const long n = (long)P.qvs * c->p.nrvars;
+      k_copy_span<<<nblocks(n), kBlock, 0, c->s>>>(c->q + P.qoff, c->qold + P.qoff, n);
+      LAUNCH_CHECK();
+    }
+  }
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle and MultigridIterations
+// ---------------------------------------------------------------------------
+extern "C" int32_t bsam_vcycle(bsam_ctx* c, int32_t level) {
+  RETCHK(check_level(c, level));
+  RETCHK(bsam_level_relax(c, level));
+  if (level > c->minlevel) {
+    RETCHK(bsam_restrict_solution(c, level));
+    RETCHK(bsam_set_ghost(c, level - 1, 0));
+    RETCHK(bsam_coarse_ghost_points(c, level));
+    RETCHK(bsam_coarse_loading(c, level));
+    RETCHK(bsam_vcycle(c, level - 1));
+    RETCHK(bsam_correct_fine(c, level));
+    RETCHK(bsam_set_ghost(c, level, 0));
+    RETCHK(bsam_level_relax(c, level));
+  }
+  return BSAM_OK;
+}
+
+// Loop body of MultigridIterations after the V-cycle (afasroutines.f90:
+// 83-104), ending with the residual norm enqueued in d_norm.
+static int32_t enqueue_iteration(bsam_ctx* c) {
+  RETCHK(bsam_vcycle(c, c->finest));
+  RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_update_source(c, level));
+  RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
+  RETCHK(enqueue_norm(c, c->finest));
+  return BSAM_OK;
+}
+
+static bool kstat_any(bsam_ctx* c) {
+  for (auto& k : c->ks)
+    if (k.on) return true;
+  return false;
+}
+
+static int32_t run_iteration(bsam_ctx* c) {
+  if (c->p.use_graph && !kstat_any(c)) {
+    if (!c->gexec) {
+      HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+      c->capturing = true;
+      int32_t r = enqueue_iteration(c);
+      c->capturing = false;
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->s, &g);
+      if (r < 0) {
+        if (g) hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return set_err(BSAM_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      c->graph = g;
+      HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(c->gexec, c->s));
+  } else {
+    RETCHK(enqueue_iteration(c));
+  }
+  HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residual, double* history,
+                                             int32_t history_len) {
+  if (!c || !ncycles || !residual) return set_err(BSAM_EINVAL, "null argument");
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
+  if (c->p.precision == BSAM_PREC_MIXED) return bsam::mixed_multigrid_iterations(c, ncycles, residual, history, history_len);
+  RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+  for (int level = c->finest; level >= c->minlevel; level--) RETCHK(bsam_update_aux(c, level));
+  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_get_source(c, level));
+  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_update_source(c, level));
+  RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
+  double res = 0.0;
+  int it;
+  for (it = 1; it <= c->p.maxvcycles; it++) {
+    RETCHK(run_iteration(c));
+    HIPCHK(hipStreamSynchronize(c->s));
+    res = norm_from(c, c->h_norm, c->finest);
+    if (history && it - 1 < history_len) history[it - 1] = res;
+    if (!std::isfinite(res)) {
+      *ncycles = it;
+      *residual = res;
+      return set_err(BSAM_EDIVERGED, "residual not finite after V-cycle %d", it);
+    }
+    if (res < c->p.qerrortol) break;
+  }
+  *ncycles = it > c->p.maxvcycles ? c->p.maxvcycles : it;
+  *residual = res;
+  return it > c->p.maxvcycles ? BSAM_NOT_CONVERGED : BSAM_OK;
+}
+
+extern "C" int32_t bsam_time_step(bsam_ctx* c, int32_t first_step, int32_t* ncycles, double* residual) {
+  if (!c || !c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
+  RETCHK(bsam_set_ghost(c, 0, 0));
+  if (first_step) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+  for (int level = c->minlevel; level <= c->finest; level++) RETCHK(bsam_copy_q_to_qold(c, level));
+  return bsam_multigrid_iterations(c, ncycles, residual, nullptr, 0);
+}
+
+extern "C" int32_t bsam_synchronize(bsam_ctx* c) {
+  if (!c) return set_err(BSAM_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->s));
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// instrumentation
+// ---------------------------------------------------------------------------
+extern "C" int32_t bsam_kstat_enable(bsam_ctx* c, int32_t on) {
+  if (!c) return set_err(BSAM_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (auto& k : c->ks) {
+    for (hipEvent_t e : k.ev) hipEventDestroy(e);
+    k.ev.clear();
+    k.launches = 0;
+    k.ms = 0.0;
+    k.on = on != 0;
+  }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_kstat_read(bsam_ctx* c, int32_t which, int64_t* launches, double* total_ms,
+                                   double* bytes_per_launch) {
+  if (!c || which < 0 || which >= BSAM_KSTAT_COUNT) return set_err(BSAM_EINVAL, "bad kstat index");
+  HIPCHK(hipStreamSynchronize(c->s));
+  KStat& k = c->ks[which];
+  double ms = 0.0;
+  for (size_t i = 0; i + 1 < k.ev.size(); i += 2) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, k.ev[i], k.ev[i + 1]));
+    ms += t;
+  }
+  if (launches) *launches = k.launches;
+  if (total_ms) *total_ms = ms;
+  if (bytes_per_launch) *bytes_per_launch = k.bytes;
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision (filled in below the FP64 path)
+// ---------------------------------------------------------------------------
+namespace bsam {
+struct MixedState {
+  int dummy;
+};
+int32_t mixed_init(bsam_ctx* c) {
+  (void)c;
+  return set_err(BSAM_EINVAL, "mixed precision not available in this build");
+}
+void mixed_free(MixedState* m) { delete m; }
+int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residual, double* history,
+                                   int32_t history_len) {
+  (void)c; (void)ncycles; (void)residual; (void)history; (void)history_len;
+  return set_err(BSAM_EINVAL, "mixed precision not available in this build");
+}
+}  // namespace bsam

@@ -1,0 +1,14 @@
+// Mixed-precision (FP32 V-cycles inside an FP64 defect-correction loop)
+// entry points, implemented in bsam_gpu.hip next to the FP64 path.
+#pragma once
+#include <stdint.h>
+
+struct bsam_ctx;
+
+namespace bsam {
+struct MixedState;
+int32_t mixed_init(bsam_ctx* c);
+void mixed_free(MixedState* m);
+int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residual, double* history,
+                                   int32_t history_len);
+}  // namespace bsam

@@ -1,0 +1,104 @@
+"""HIP path against the reference's golden vectors and the CPU oracle.
+
+Every kernel except the residual-norm reduction keeps the reference's
+floating-point operation order (compiled with -ffp-contract=off), so states
+must agree bit for bit; residual norms are sums in a different order and
+must agree to 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import fixture_util as fu
+import oracle
+from bsam_amd import abi, configs, hierarchy
+
+pytestmark = pytest.mark.gpu
+RTOL_NORM = 1e-12
+
+
+def _hist_close(a, b):
+    assert len(a) == len(b), (a, b)
+    for x, y in zip(a, b):
+        assert abs(x - y) <= RTOL_NORM * abs(y), (x, y)
+
+
+@pytest.mark.parametrize("name", fu.NAMES)
+def test_golden_multigrid_iterations(name, gpu_solver_cls):
+    meta, z = fu.load(name)
+    for s, st in enumerate(meta["steps"]):
+        print(f"[gpu] {name} step {s}", flush=True)
+        S, hist = fu.run_step(gpu_solver_cls, meta, z, s)
+        _hist_close(hist, st["residuals"])
+        for i in st["end"]:
+            got = S.get(i, "q").ravel()
+            ref = fu.end_state(z, s, i)
+            assert np.array_equal(got, ref), (name, s, i, float(np.max(np.abs(got - ref))))
+        S.close()
+
+
+@pytest.mark.parametrize("name", fu.NAMES)
+def test_golden_sweeps(name, gpu_solver_cls):
+    meta, z = fu.load(name)
+    print(f"[gpu] sweeps {name}", flush=True)
+
+    def cmp(S, pid, golden, rec):
+        got = S.get(pid, "q").ravel()
+        assert np.array_equal(got, golden), (name, rec, float(np.max(np.abs(got - golden))))
+
+    n = fu.replay_sweeps(gpu_solver_cls, meta, z, 0, cmp)
+    assert n == len(meta["steps"][0]["sweeps"]) > 0
+
+
+def _uniform_pair(nd, n, eps, dt, seed, base, gpu_cls, **kw):
+    minlevel = -(int(np.log2(n)) - 1)
+    hier = hierarchy.uniform(nd, (n,) * nd, minlevel, [2] * (2 * nd))
+    p = configs.params(nd, dt, eps, 1e-9, **kw)
+    root = len(hier) - 1
+    phi = configs.init_phi(hier[root], nd, 1, seed, base, 0.05)
+    q0 = np.zeros((2,) + (n + 2,) * nd)
+    q0[(0,) + (slice(1, -1),) * nd] = phi
+    out = []
+    for S in (gpu_cls(p, hier), oracle.Oracle(configs.params(nd, dt, eps, 1e-9), hier)):
+        S.put(root, "q", q0)
+        S.set_ghost(0, 0)
+        S.copy_q_to_qold(0)
+        hists = []
+        for step in range(2):
+            S.set_ghost(0, 0)
+            if step == 0:
+                S.fill_down(abi.SOLUTIONFIELD)
+            for lv in range(minlevel, 1):
+                S.copy_q_to_qold(lv)
+            rc, ncyc, r, h = S.multigrid_iterations()
+            hists.append(h)
+        out.append((hists, S.get(root, "q")))
+    return out
+
+
+@pytest.mark.parametrize("nd,n", [(2, 128), (3, 32)])
+def test_uniform_two_steps_vs_oracle(nd, n, gpu_solver_cls):
+    print(f"[gpu] uniform {nd}D {n}", flush=True)
+    (hg, qg), (ho, qo) = _uniform_pair(nd, n, 0.03 if nd == 2 else 0.05, 1e-4, 1001, 0.0, gpu_solver_cls)
+    for a, b in zip(hg, ho):
+        _hist_close(a, b)
+    assert np.array_equal(qg, qo), float(np.max(np.abs(qg - qo)))
+
+
+def test_graph_replay_matches_eager(gpu_solver_cls):
+    """The hipGraph of the V-cycle loop body, replayed once per V-cycle with
+    host synchronisation in between, gives the eager result bit for bit."""
+    print("[gpu] graph vs eager", flush=True)
+    (hg, qg), _ = _uniform_pair(3, 32, 0.05, 1e-4, 1003, 0.1, gpu_solver_cls, use_graph=1)
+    (he, qe), _ = _uniform_pair(3, 32, 0.05, 1e-4, 1003, 0.1, gpu_solver_cls, use_graph=0)
+    assert hg == he
+    assert np.array_equal(qg, qe)
+
+
+def test_graph_replay_amr(gpu_solver_cls):
+    meta, z = fu.load("ch2d_amr")
+    st = meta["steps"][0]
+    print("[gpu] graph amr", flush=True)
+    S, hist = fu.run_step(gpu_solver_cls, meta, z, 0, use_graph=1)
+    _hist_close(hist, st["residuals"])
+    for i in st["end"]:
+        assert np.array_equal(S.get(i, "q").ravel(), fu.end_state(z, 0, i))
