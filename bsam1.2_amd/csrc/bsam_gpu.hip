@@ -240,12 +240,12 @@ __device__ __forceinline__ void op_q(const double* __restrict__ q, const double*
 // Also accumulates sum(rf^2) per variable into per-block partials when
 // `partial` is non-null (ErrorAFAS / L2Error / SquareSum, :113-233), and
 // writes rf when `rf` is non-null.
-template <int DIM, int MODEL>
+template <int DIM, int MODEL, typename TO>
 __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      const double* __restrict__ q,
                                                      const double* __restrict__ qold,
                                                      const double* __restrict__ f,
-                                                     double* __restrict__ rf,
+                                                     TO* __restrict__ rf,
                                                      double* __restrict__ partial) {
   const int lb = xcd_block();
   const int g = lb * kBlock + threadIdx.x;
@@ -263,8 +263,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
     const int o = qidx(P, i, j, k);
     const double r0 = f[o] - o1, r1 = f[o + P.qvs] - o2;
     if (rf) {
-      rf[o] = r0;
-      rf[o + P.qvs] = r1;
+      rf[o] = (TO)r0;
+      rf[o + P.qvs] = (TO)r1;
     }
     s0 = r0 * r0;
     s1 = r1 * r1;
@@ -322,16 +322,16 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __rest
 // Restriction (Restriction2D/3D, gridutilities.f90:103-126, 437-466) in the
 // reference's summation order: (e,e,e),(o,e,e),(e,o,e),(o,o,e),(e,e,o),...
 // ---------------------------------------------------------------------------
-template <int DIM>
-__device__ __forceinline__ double restrict_cell(const double* __restrict__ a, const DPatch& P, int ic,
-                                                int jc, int kc, int off) {
+template <int DIM, typename T>
+__device__ __forceinline__ T restrict_cell(const T* __restrict__ a, const DPatch& P, int ic, int jc, int kc,
+                                           int off) {
   const int i2 = 2 * ic, i1 = i2 - 1, j2 = 2 * jc, j1 = j2 - 1;
   if (DIM == 2) {
-    return 0.25 * (a[qidx(P, i2, j2, 0) + off] + a[qidx(P, i1, j2, 0) + off] +
+    return (T)0.25 * (a[qidx(P, i2, j2, 0) + off] + a[qidx(P, i1, j2, 0) + off] +
                    a[qidx(P, i2, j1, 0) + off] + a[qidx(P, i1, j1, 0) + off]);
   }
   const int k2 = 2 * kc, k1 = k2 - 1;
-  return 0.125 * (a[qidx(P, i2, j2, k2) + off] + a[qidx(P, i1, j2, k2) + off] +
+  return (T)0.125 * (a[qidx(P, i2, j2, k2) + off] + a[qidx(P, i1, j2, k2) + off] +
                   a[qidx(P, i2, j1, k2) + off] + a[qidx(P, i1, j1, k2) + off] +
                   a[qidx(P, i2, j2, k1) + off] + a[qidx(P, i1, j2, k1) + off] +
                   a[qidx(P, i2, j1, k1) + off] + a[qidx(P, i1, j1, k1) + off]);
@@ -358,10 +358,9 @@ __device__ __forceinline__ bool decode_coarse(const LevelArgs& L, int g, CoarseC
 // RestrictSolution (afasroutines.f90:610-652) when `qc` is non-null: qc =
 // R q and parent%q(mb) = qc.  FillDownLevel (afasroutines.f90:257-327) when
 // `qc` is null: parent field(mb) = R field (field = q or f).
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const double* __restrict__ src,
-                                                     double* __restrict__ dst_parent,
-                                                     double* __restrict__ qc) {
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const T* __restrict__ src,
+                                                     T* __restrict__ dst_parent, T* __restrict__ qc) {
   CoarseCell c;
   if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
   const DPatch& P = L.pt[c.pid];
@@ -369,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const 
   const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
   const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
   for (int v = 0; v < nv; v++) {
-    const double r = restrict_cell<DIM>(src, P, c.ic, c.jc, c.kc, v * P.qvs);
+    const T r = restrict_cell<DIM, T>(src, P, c.ic, c.jc, c.kc, v * P.qvs);
     if (qc) qc[cidx(P, c.ic, c.jc, c.kc) + v * P.cvs] = r;
     dst_parent[qidx(Q, ip, jp, kp) + v * Q.qvs] = r;
   }
@@ -379,9 +378,9 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const 
 // Domain: the (cmx+2)^d qc box; interior cells are skipped.  The reference's
 // face loops cover every ring cell exactly once with the value
 // parent%q(mb-1+i, mb-1+j, mb-1+k).
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, const double* __restrict__ q,
-                                                        double* __restrict__ qc) {
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, const T* __restrict__ q,
+                                                        T* __restrict__ qc) {
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
@@ -454,9 +453,9 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
 
 // CorrectFine, first half (afasroutines.f90:892-895, 934-938):
 // qc(0:cmx+1) = parent%q(mb-1:mb+1) - qc.  Domain: the qc box.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, const double* __restrict__ q,
-                                                       double* __restrict__ qc) {
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, const T* __restrict__ q,
+                                                       T* __restrict__ qc) {
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
@@ -477,14 +476,15 @@ __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, cons
 
 // Linear weights of Bi/TriLinProlongationP1 (gridutilities.f90:156-195,
 // 510-566): even fine index 3*a(lo)+a(lo+1), odd a(lo)+3*a(lo+1).
-__device__ __forceinline__ double linw(double lo, double hi, int fine) {
-  return (fine & 1) ? lo + 3.0 * hi : 3.0 * lo + hi;
+template <typename T>
+__device__ __forceinline__ T linw(T lo, T hi, int fine) {
+  return (fine & 1) ? lo + (T)3 * hi : (T)3 * lo + hi;
 }
 
 // CorrectFine, second half: q(0:mx+1) += P1(qc).  Domain: (mx+2)^d.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, double* __restrict__ q,
-                                                        const double* __restrict__ qc) {
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* __restrict__ q,
+                                                        const T* __restrict__ qc) {
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
@@ -497,21 +497,21 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, dou
   const int il = i >> 1, jl = j >> 1, kl = k >> 1;
   for (int v = 0; v < nv; v++) {
     const int cv = v * P.cvs;
-    double pr;
+    T pr;
     if (DIM == 2) {
-      const double b0 = linw(qc[cidx(P, il, jl, 0) + cv], qc[cidx(P, il + 1, jl, 0) + cv], i);
-      const double b1 = linw(qc[cidx(P, il, jl + 1, 0) + cv], qc[cidx(P, il + 1, jl + 1, 0) + cv], i);
-      pr = linw(b0, b1, j) / 16.0;
+      const T b0 = linw<T>(qc[cidx(P, il, jl, 0) + cv], qc[cidx(P, il + 1, jl, 0) + cv], i);
+      const T b1 = linw<T>(qc[cidx(P, il, jl + 1, 0) + cv], qc[cidx(P, il + 1, jl + 1, 0) + cv], i);
+      pr = linw<T>(b0, b1, j) / (T)16;
     } else {
-      double cc[2];
+      T cc[2];
 #pragma unroll
       for (int dk = 0; dk < 2; dk++) {
-        const double b0 = linw(qc[cidx(P, il, jl, kl + dk) + cv], qc[cidx(P, il + 1, jl, kl + dk) + cv], i);
-        const double b1 =
-            linw(qc[cidx(P, il, jl + 1, kl + dk) + cv], qc[cidx(P, il + 1, jl + 1, kl + dk) + cv], i);
-        cc[dk] = linw(b0, b1, j);
+        const T b0 = linw<T>(qc[cidx(P, il, jl, kl + dk) + cv], qc[cidx(P, il + 1, jl, kl + dk) + cv], i);
+        const T b1 =
+            linw<T>(qc[cidx(P, il, jl + 1, kl + dk) + cv], qc[cidx(P, il + 1, jl + 1, kl + dk) + cv], i);
+        cc[dk] = linw<T>(b0, b1, j);
       }
-      pr = linw(cc[0], cc[1], k) / 64.0;
+      pr = linw<T>(cc[0], cc[1], k) / (T)64;
     }
     const int o = qidx(P, i, j, k) + v * P.qvs;
     q[o] = q[o] + pr;
@@ -594,13 +594,13 @@ __device__ __forceinline__ int st_k(int k) { return DIM == 3 ? k : 0; }
 
 // One interpolated ghost value from fine sources s1, s2 (patch P) and the
 // coarse term (parent Q), in the reference's order.
-template <int DIM>
-__device__ __forceinline__ void interp_store(double* __restrict__ q, const DPatch& P, const DPatch& Q,
-                                             int nv, const int g[3], const int s1[3], const int s2[3],
-                                             int ncoarse, const int (*cq)[3], const double* w) {
+template <int DIM, typename T>
+__device__ __forceinline__ void interp_store(T* __restrict__ q, const DPatch& P, const DPatch& Q, int nv,
+                                             const int g[3], const int s1[3], const int s2[3], int ncoarse,
+                                             const int (*cq)[3], const T* w) {
   for (int v = 0; v < nv; v++) {
     const int fv = v * P.qvs, pv = v * Q.qvs;
-    double coarse;
+    T coarse;
     if (ncoarse == 1) {
       coarse = q[qidx(Q, cq[0][0], cq[0][1], st_k<DIM>(cq[0][2])) + pv];
     } else if (ncoarse == 3) {
@@ -614,8 +614,8 @@ __device__ __forceinline__ void interp_store(double* __restrict__ q, const DPatc
                w[3] * q[qidx(Q, cq[3][0], cq[3][1], st_k<DIM>(cq[3][2])) + pv];
     }
     q[qidx(P, g[0], g[1], st_k<DIM>(g[2])) + fv] =
-        cA[0] * q[qidx(P, s1[0], s1[1], st_k<DIM>(s1[2])) + fv] +
-        cA[1] * q[qidx(P, s2[0], s2[1], st_k<DIM>(s2[2])) + fv] + cA[2] * coarse;
+        (T)cA[0] * q[qidx(P, s1[0], s1[1], st_k<DIM>(s1[2])) + fv] +
+        (T)cA[1] * q[qidx(P, s2[0], s2[1], st_k<DIM>(s2[2])) + fv] + (T)cA[2] * coarse;
   }
 }
 
@@ -625,10 +625,10 @@ __device__ __forceinline__ void interp_store(double* __restrict__ q, const DPatc
 // corners) and apply the rule the reference applies to that cell, including
 // the colour filter, the physical-face skip for ipass=1 in 3D, and the
 // comparison with 99 instead of 999 at boundary.f90:1104.
-template <int DIM>
+template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt,
                                                    const int* __restrict__ patches, int nv, int ipass,
-                                                   double* __restrict__ q) {
+                                                   T* __restrict__ q) {
   const DPatch& P = pt[patches[blockIdx.x]];
   const DPatch& Q = pt[P.parent];
   const int* mth = P.mthbc;
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
     const int total = 2 * nf0 + 2 * nf1 + 4;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
       int g[3], s1[3], s2[3], cq[4][3];
-      double w[4];
+      T w[4];
       int nco = 1;
       g[2] = s1[2] = s2[2] = 1;
       cq[0][2] = cq[1][2] = cq[2][2] = cq[3][2] = 1;
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
           cq[0][d] = fi[d][3]; cq[0][t] = kind_c(P, t, k, pos, 0);
         } else {
           nco = 3;
-          for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); w[r] = cB[r]; }
+          for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); w[r] = (T)cB[r]; }
         }
       } else {
         // corners (boundary.f90:155-181): applied whatever mthbc is
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
         if (!(ipass == 0 || parity == ipass - 1)) continue;
         for (int a = 0; a < 2; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
       }
-      interp_store<2>(q, P, Q, nv, g, s1, s2, nco, cq, w);
+      interp_store<2, T>(q, P, Q, nv, g, s1, s2, nco, cq, w);
     }
     return;
   }
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
   const int total = nfaces + nedges + 8;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     int g[3], s1[3], s2[3], cq[4][3];
-    double w[4];
+    T w[4];
     int nco = 1;
     if (e < nfaces) {
       int d = 0, rem = e;
@@ -713,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
         cq[1][t1] = ft1[3]; cq[1][t2] = ft2[4];
         cq[2][t1] = ft1[4]; cq[2][t2] = ft2[3];
         cq[3][t1] = ft1[3]; cq[3][t2] = ft2[3];
-        for (int r = 0; r < 4; r++) w[r] = cC[r];
+        for (int r = 0; r < 4; r++) w[r] = (T)cC[r];
       } else {
         const int parity = (fi[d][0] + a + b) & 1;
         if (!(ipass == 0 || parity == ipass - 1)) continue;
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
             cq[r][d] = fi[d][3];
             cq[r][t1] = kind_c(P, t1, k1, a, q1 ? r : 0);
             cq[r][t2] = kind_c(P, t2, k2, b, q2 ? r : 0);
-            w[r] = cB[r];
+            w[r] = (T)cB[r];
           }
         }
       }
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
         nco = 3;
         for (int r = 0; r < 3; r++) {
           cq[r][fa] = fi[fa][3]; cq[r][fb] = fi[fb][3]; cq[r][ed] = kind_c(P, ed, k, pos, r);
-          w[r] = cB[r];
+          w[r] = (T)cB[r];
         }
       }
     } else {
@@ -771,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
       face_index(P, 2, p3, fi[2]);
       for (int a = 0; a < 3; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
     }
-    interp_store<3>(q, P, Q, nv, g, s1, s2, nco, cq, w);
+    interp_store<3, T>(q, P, Q, nv, g, s1, s2, nco, cq, w);
   }
 }
 
@@ -779,9 +779,12 @@ __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt
 // boundary.f90:359-431), self-periodic wraps (SelfPeriodicBC :480-577),
 // periodic transfers (TransferPeriodicBC :580-611) and physical boundary
 // conditions (SetPhysicalBC + UserBC).  One block per box.
+// `homog` selects the boundary condition of a correction (mixed precision):
+// q_ghost = 2 - q_in becomes e_ghost = -e_in.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt,
-                                                   const BoxDesc* __restrict__ boxes, int nv,
-                                                   double* __restrict__ q) {
+                                                   const BoxDesc* __restrict__ boxes, int nv, int homog,
+                                                   T* __restrict__ q) {
   const BoxDesc B = boxes[blockIdx.x];
   const DPatch& D = pt[B.dp];
   const DPatch& S = pt[B.sp];
@@ -790,12 +793,13 @@ __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt
     const int v = e / n, c = e % n;
     const int a = c % B.ext[0], r = c / B.ext[0];
     const int b = r % B.ext[1], z = r / B.ext[1];
-    const double x = q[qidx(S, B.slo[0] + a, B.slo[1] + b, B.slo[2] + z) + v * S.qvs];
-    double y;
+    const T x = q[qidx(S, B.slo[0] + a, B.slo[1] + b, B.slo[2] + z) + v * S.qvs];
+    const T dir = homog ? -x : (T)2 - x;
+    T y;
     switch (B.op) {
-      case kCopyPlus0: y = x + 0.0; break;
-      case kDirichlet2: y = 2.0 - x; break;
-      case kBgMixed: y = (v == 0) ? x : 2.0 - x; break;
+      case kCopyPlus0: y = x + (T)0; break;
+      case kDirichlet2: y = dir; break;
+      case kBgMixed: y = (v == 0) ? x : dir; break;
       default: y = x;
     }
     q[qidx(D, B.dlo[0] + a, B.dlo[1] + b, B.dlo[2] + z) + v * D.qvs] = y;
@@ -806,9 +810,9 @@ __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt
 // applied x, then y, then z): the net effect is that every ghost cell takes
 // the value of its periodic image, plus 0.0 (the qpo offset of the x wrap)
 // when the cell lies on an x ghost plane.  Domain: the ghost shell.
-template <int DIM>
+template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_wrap_full(const DPatch* __restrict__ pt, int pid, int nv,
-                                                      double* __restrict__ q) {
+                                                      T* __restrict__ q) {
   const DPatch& P = pt[pid];
   const int nx = P.nq[0], ny = P.nq[1], nz = P.nq[2];
   const int g = xcd_block() * kBlock + threadIdx.x;
@@ -841,9 +845,151 @@ __global__ __launch_bounds__(kBlock) void k_wrap_full(const DPatch* __restrict__
   const int sk = (DIM == 3) ? ((k == 0) ? mz : (k == mz + 1 ? 1 : k)) : 0;
   const bool xg = (i == 0 || i == mx + 1);
   for (int v = 0; v < nv; v++) {
-    const double x = q[qidx(P, si, sj, sk) + v * P.qvs];
-    q[qidx(P, i, j, k) + v * P.qvs] = xg ? x + 0.0 : x;
+    const T x = q[qidx(P, si, sj, sk) + v * P.qvs];
+    q[qidx(P, i, j, k) + v * P.qvs] = xg ? x + (T)0 : x;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 correction V-cycles for the CH step linearised about
+// the FP64 iterate q.  With e = (e_phi, e_mu) and c = 3*phi^2 the
+// correction operator is
+//   A1(e) = e_phi - (dt/h^2) ULap(e_mu)
+//   A2(e) = e_mu - c e_phi + (eps^2/h^2) ULap(e_phi),
+// the Jacobian of Operator2D/3D (CH/problem.f90:393-422, 638-669); its
+// pointwise smoother is the 2x2 solve of RelaxGrid2D/3D with the cubic
+// term dropped.  The outer FP64 loop forms r = f - L(q), solves A e = r
+// approximately with one FP32 V-cycle, and updates q += e.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_zero_f(float* __restrict__ p, long n) {
+  long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g < n) p[g] = 0.0f;
+}
+
+// q(FP64) += e(FP32) over a contiguous span (ghost cells included).
+__global__ __launch_bounds__(kBlock) void k_add_corr(double* __restrict__ q, const float* __restrict__ e,
+                                                     long n) {
+  long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g < n) q[g] = q[g] + (double)e[g];
+}
+
+// c = 3 phi^2 over the interior of every patch of a level.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __restrict__ q,
+                                                 float* __restrict__ coef) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const int local = g - L.prefix[pi];
+  const int i = local % P.mx[0] + 1;
+  const int row = local / P.mx[0];
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int o = qidx(P, i, j, k);
+  const double p = q[o];
+  coef[o] = (float)(3.0 * (p * p));
+}
+
+template <int DIM>
+__device__ __forceinline__ float ulap_f(const float* __restrict__ a, const DPatch& P, int i, int j, int k,
+                                        int off) {
+  float s = a[qidx(P, i + 1, j, k) + off] + a[qidx(P, i - 1, j, k) + off] + a[qidx(P, i, j + 1, k) + off] +
+            a[qidx(P, i, j - 1, k) + off];
+  if (DIM == 3) {
+    s = s + a[qidx(P, i, j, k + 1) + off] + a[qidx(P, i, j, k - 1) + off];
+    return s - 6.0f * a[qidx(P, i, j, k) + off];
+  }
+  return s - 4.0f * a[qidx(P, i, j, k) + off];
+}
+template <int DIM>
+__device__ __forceinline__ float ulap_cf(const float* __restrict__ a, const DPatch& P, int i, int j, int k,
+                                         int off) {
+  float s = a[cidx(P, i + 1, j, k) + off] + a[cidx(P, i - 1, j, k) + off] + a[cidx(P, i, j + 1, k) + off] +
+            a[cidx(P, i, j - 1, k) + off];
+  if (DIM == 3) {
+    s = s + a[cidx(P, i, j, k + 1) + off] + a[cidx(P, i, j, k - 1) + off];
+    return s - 6.0f * a[cidx(P, i, j, k) + off];
+  }
+  return s - 4.0f * a[cidx(P, i, j, k) + off];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float tmp1, float tmp2, float tmp3,
+                                                      float a12, float* __restrict__ e,
+                                                      const float* __restrict__ rhs,
+                                                      const float* __restrict__ coef) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const int local = g - L.prefix[pi];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int t = local % cnt;
+  const int row = local / cnt;
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int ib = (c + j + k) & 1;
+  const int i = 2 * t + 2 - ib;
+  if (i > P.mx[0]) return;
+  const int vs = P.qvs;
+  const int o = qidx(P, i, j, k);
+  const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
+  const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
+  float smu = e[xm + vs] + e[xp + vs] + e[ym + vs] + e[yp + vs];
+  float sphi = e[xp] + e[xm] + e[yp] + e[ym];
+  if (DIM == 3) {
+    const int zm = qidx(P, i, j, k - 1), zp = qidx(P, i, j, k + 1);
+    smu = smu + e[zm + vs] + e[zp + vs];
+    sphi = sphi + e[zp] + e[zm];
+  }
+  const float a21 = -coef[o] - tmp3;
+  const float b1 = rhs[o] + tmp1 * smu;
+  const float b2 = rhs[o + vs] - tmp2 * sphi;
+  const float det = 1.0f - a12 * a21;
+  e[o] = (b1 - a12 * b2) / det;
+  e[o + vs] = (b2 - a21 * b1) / det;
+}
+
+// Coarse right-hand side of the correction V-cycle (the FAS form of
+// CoarseLoadingFunction, afasroutines.f90:673-736, for the operator A):
+// parent rhs(mb) = R(rhs - A_h e) + A_2h(ec).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, double eps2,
+                                                        const float* __restrict__ e,
+                                                        const float* __restrict__ rhs,
+                                                        const float* __restrict__ ec,
+                                                        const float* __restrict__ coef,
+                                                        float* __restrict__ rpar) {
+  CoarseCell c;
+  if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
+  const DPatch& P = L.pt[c.pid];
+  const DPatch& Q = L.pt[P.parent];
+  const float t1 = (float)(dt / (P.dx * P.dx)), t2 = (float)(eps2 / (P.dx * P.dx));
+  const int nch = (DIM == 3) ? 8 : 4;
+  float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+  for (int n = 0; n < nch; n++) {
+    const int i = 2 * c.ic - (n & 1), j = 2 * c.jc - ((n >> 1) & 1);
+    const int k = (DIM == 3) ? 2 * c.kc - ((n >> 2) & 1) : 0;
+    const int o = qidx(P, i, j, k);
+    const float ep = e[o], em = e[o + P.qvs];
+    const float a1 = ep - t1 * ulap_f<DIM>(e, P, i, j, k, P.qvs);
+    const float a2 = em - coef[o] * ep + t2 * ulap_f<DIM>(e, P, i, j, k, 0);
+    s0 += rhs[o] - a1;
+    s1 += rhs[o + P.qvs] - a2;
+  }
+  const float w = (DIM == 3) ? 0.125f : 0.25f;
+  const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
+  const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
+  const int opar = qidx(Q, ip, jp, kp);
+  const int oc = cidx(P, c.ic, c.jc, c.kc);
+  const float T1 = (float)(dt / (Q.dx * Q.dx)), T2 = (float)(eps2 / (Q.dx * Q.dx));
+  const float ep = ec[oc], em = ec[oc + P.cvs];
+  const float a1 = ep - T1 * ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, P.cvs);
+  const float a2 = em - coef[opar] * ep + T2 * ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, 0);
+  rpar[opar] = w * s0 + a1;
+  rpar[opar + Q.qvs] = w * s1 + a2;
 }
 
 // ---------------------------------------------------------------------------
@@ -1531,29 +1677,35 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
   return BSAM_OK;
 }
 
-extern "C" int32_t bsam_set_ghost(bsam_ctx* c, int32_t level, int32_t ipass) {
-  RETCHK(check_level(c, level));
-  if (ipass < 0 || ipass > 2) return set_err(BSAM_EINVAL, "ipass must be 0, 1 or 2");
+// SetGhost on any field of the q geometry (FP64 solution or FP32 correction).
+template <typename T>
+static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog) {
   Level& L = c->L(level);
   const int nv = c->p.nrvars;
   if (level > 0) {
     const int np = (int)L.patches.size();
-    if (c->p.ndims == 2) k_interp<2><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, c->q);
-    else k_interp<3><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, c->q);
+    if (c->p.ndims == 2) k_interp<2, T><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, q);
+    else k_interp<3, T><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, q);
     LAUNCH_CHECK();
   }
   if (L.wrap_pid >= 0) {
     const DPatch& P = c->hp[L.wrap_pid];
     const long shell = (long)P.nq[0] * P.nq[1] * P.nq[2] - (long)P.mx[0] * P.mx[1] * P.mx[2];
-    if (c->p.ndims == 2) k_wrap_full<2><<<nblocks(shell), kBlock, 0, c->s>>>(c->d_pt, L.wrap_pid, nv, c->q);
-    else k_wrap_full<3><<<nblocks(shell), kBlock, 0, c->s>>>(c->d_pt, L.wrap_pid, nv, c->q);
+    if (c->p.ndims == 2) k_wrap_full<2, T><<<nblocks(shell), kBlock, 0, c->s>>>(c->d_pt, L.wrap_pid, nv, q);
+    else k_wrap_full<3, T><<<nblocks(shell), kBlock, 0, c->s>>>(c->d_pt, L.wrap_pid, nv, q);
     LAUNCH_CHECK();
   }
   for (const Stage& s : L.stages) {
-    k_boxops<<<s.n, kBlock, 0, c->s>>>(c->d_pt, s.d, nv, c->q);
+    k_boxops<T><<<s.n, kBlock, 0, c->s>>>(c->d_pt, s.d, nv, homog, q);
     LAUNCH_CHECK();
   }
   return BSAM_OK;
+}
+
+extern "C" int32_t bsam_set_ghost(bsam_ctx* c, int32_t level, int32_t ipass) {
+  RETCHK(check_level(c, level));
+  if (ipass < 0 || ipass > 2) return set_err(BSAM_EINVAL, "ipass must be 0, 1 or 2");
+  return set_ghost_t<double>(c, level, ipass, c->q, 0);
 }
 
 extern "C" int32_t bsam_level_relax(bsam_ctx* c, int32_t level) {
@@ -1572,24 +1724,48 @@ static int32_t need_parent(bsam_ctx* c, int level) {
   return BSAM_OK;
 }
 
-extern "C" int32_t bsam_restrict_solution(bsam_ctx* c, int32_t level) {
-  RETCHK(need_parent(c, level));
+template <typename T>
+static int32_t restrict_t(bsam_ctx* c, int level, const T* src, T* dst, T* qc) {
   Level& L = c->L(level);
   LevelArgs a = largs(c, L, kCoarse);
-  DISPATCH_DIM(c->p.ndims, (k_restrict<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->q, c->qc)),
-               (k_restrict<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->q, c->qc)));
+  DISPATCH_DIM(c->p.ndims, (k_restrict<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
+               (k_restrict<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
   LAUNCH_CHECK();
   return BSAM_OK;
 }
 
-extern "C" int32_t bsam_coarse_ghost_points(bsam_ctx* c, int32_t level) {
-  RETCHK(need_parent(c, level));
+template <typename T>
+static int32_t ring_t(bsam_ctx* c, int level, const T* q, T* qc) {
   Level& L = c->L(level);
   LevelArgs a = largs(c, L, kQcBox);
-  DISPATCH_DIM(c->p.ndims, (k_coarse_ring<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)),
-               (k_coarse_ring<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)));
+  DISPATCH_DIM(c->p.ndims, (k_coarse_ring<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)),
+               (k_coarse_ring<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)));
   LAUNCH_CHECK();
   return BSAM_OK;
+}
+
+template <typename T>
+static int32_t correct_t(bsam_ctx* c, int level, T* q, T* qc) {
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kQcBox);
+  LevelArgs b = largs(c, L, kFineG);
+  DISPATCH_DIM(c->p.ndims, (k_correct_qc<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)),
+               (k_correct_qc<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)));
+  LAUNCH_CHECK();
+  DISPATCH_DIM(c->p.ndims, (k_prolong_add<2, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
+               (k_prolong_add<3, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_restrict_solution(bsam_ctx* c, int32_t level) {
+  RETCHK(need_parent(c, level));
+  return restrict_t<double>(c, level, c->q, c->q, c->qc);
+}
+
+extern "C" int32_t bsam_coarse_ghost_points(bsam_ctx* c, int32_t level) {
+  RETCHK(need_parent(c, level));
+  return ring_t<double>(c, level, c->q, c->qc);
 }
 
 extern "C" int32_t bsam_coarse_loading(bsam_ctx* c, int32_t level) {
@@ -1626,29 +1802,16 @@ extern "C" int32_t bsam_coarse_loading(bsam_ctx* c, int32_t level) {
 
 extern "C" int32_t bsam_correct_fine(bsam_ctx* c, int32_t level) {
   RETCHK(need_parent(c, level));
-  Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kQcBox);
-  LevelArgs b = largs(c, L, kFineG);
-  DISPATCH_DIM(c->p.ndims, (k_correct_qc<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)),
-               (k_correct_qc<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), c->q, c->qc)));
-  LAUNCH_CHECK();
-  DISPATCH_DIM(c->p.ndims, (k_prolong_add<2><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), c->q, c->qc)),
-               (k_prolong_add<3><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), c->q, c->qc)));
-  LAUNCH_CHECK();
-  return BSAM_OK;
+  return correct_t<double>(c, level, c->q, c->qc);
 }
 
 extern "C" int32_t bsam_fill_down(bsam_ctx* c, int32_t field) {
   if (!c || !c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
   if (field != BSAM_SOURCEFIELD && field != BSAM_SOLUTIONFIELD) return set_err(BSAM_EINVAL, "bad field %d", field);
+  double* fld = (field == BSAM_SOURCEFIELD) ? c->f : c->q;
   for (int level = c->finest; level >= c->minlevel + 1; level--) {
-    Level& L = c->L(level);
-    LevelArgs a = largs(c, L, kCoarse);
-    double* fld = (field == BSAM_SOURCEFIELD) ? c->f : c->q;
-    DISPATCH_DIM(c->p.ndims, (k_restrict<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), fld, fld, nullptr)),
-                 (k_restrict<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), fld, fld, nullptr)));
-    LAUNCH_CHECK();
-    if (field == BSAM_SOLUTIONFIELD) RETCHK(bsam_set_ghost(c, level - 1, 0));
+    RETCHK(restrict_t<double>(c, level, fld, fld, nullptr));
+    if (field == BSAM_SOLUTIONFIELD) RETCHK(set_ghost_t<double>(c, level - 1, 0, c->q, 0));
   }
   return BSAM_OK;
 }
@@ -1684,10 +1847,10 @@ static int32_t residual_launch(bsam_ctx* c, int level, double* rf, double* parti
   const ModelK m = c->mk();
   if (c->p.model == BSAM_MODEL_CH) {
     DISPATCH_DIM(c->p.ndims,
-                 (k_residual<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)),
-                 (k_residual<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)));
+                 (k_residual<2, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)),
+                 (k_residual<3, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)));
   } else {
-    k_residual<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial);
+    k_residual<2, BSAM_MODEL_BG, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial);
   }
   LAUNCH_CHECK();
   return nb;
@@ -1794,12 +1957,12 @@ static bool kstat_any(bsam_ctx* c) {
   return false;
 }
 
-static int32_t run_iteration(bsam_ctx* c) {
+static int32_t run_iteration(bsam_ctx* c, int32_t (*body)(bsam_ctx*)) {
   if (c->p.use_graph && !kstat_any(c)) {
     if (!c->gexec) {
       HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
       c->capturing = true;
-      int32_t r = enqueue_iteration(c);
+      int32_t r = body(c);
       c->capturing = false;
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(c->s, &g);
@@ -1813,7 +1976,7 @@ static int32_t run_iteration(bsam_ctx* c) {
     }
     HIPCHK(hipGraphLaunch(c->gexec, c->s));
   } else {
-    RETCHK(enqueue_iteration(c));
+    RETCHK(body(c));
   }
   HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
   return BSAM_OK;
@@ -1832,7 +1995,7 @@ extern "C" int32_t bsam_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, doub
   double res = 0.0;
   int it;
   for (it = 1; it <= c->p.maxvcycles; it++) {
-    RETCHK(run_iteration(c));
+    RETCHK(run_iteration(c, enqueue_iteration));
     HIPCHK(hipStreamSynchronize(c->s));
     res = norm_from(c, c->h_norm, c->finest);
     if (history && it - 1 < history_len) history[it - 1] = res;
@@ -1896,20 +2059,194 @@ extern "C" int32_t bsam_kstat_read(bsam_ctx* c, int32_t which, int64_t* launches
 }
 
 // ---------------------------------------------------------------------------
-// mixed precision (filled in below the FP64 path)
+// mixed precision: FP64 defect correction around FP32 correction V-cycles
 // ---------------------------------------------------------------------------
 namespace bsam {
 struct MixedState {
-  int dummy;
+  float* e = nullptr;     // correction (q geometry)
+  float* rhs = nullptr;   // right-hand side of the correction equation
+  float* coef = nullptr;  // 3 phi^2 of the FP64 iterate
+  float* ec = nullptr;    // coarse-under-fine correction (qc geometry)
 };
+
 int32_t mixed_init(bsam_ctx* c) {
-  (void)c;
-  return set_err(BSAM_EINVAL, "mixed precision not available in this build");
+  MixedState* m = new MixedState();
+  c->mixed = m;
+  float** qs[3] = {&m->e, &m->rhs, &m->coef};
+  for (float** pp : qs) {
+    if (hipMalloc(pp, c->nq * sizeof(float)) != hipSuccess) return set_err(BSAM_ENOMEM, "FP32 pools");
+    HIPCHK(hipMemset(*pp, 0, c->nq * sizeof(float)));
+  }
+  if (hipMalloc(&m->ec, c->nc * sizeof(float)) != hipSuccess) return set_err(BSAM_ENOMEM, "FP32 qc pool");
+  HIPCHK(hipMemset(m->ec, 0, c->nc * sizeof(float)));
+  return BSAM_OK;
 }
-void mixed_free(MixedState* m) { delete m; }
+
+void mixed_free(MixedState* m) {
+  if (!m) return;
+  (void)hipFree(m->e);
+  (void)hipFree(m->rhs);
+  (void)hipFree(m->coef);
+  (void)hipFree(m->ec);
+  delete m;
+}
+}  // namespace bsam
+
+namespace {
+
+int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
+  bsam::MixedState* m = c->mixed;
+  Level& L = c->L(level);
+  LevelArgs a = largs(c, L, kHalf);
+  const double h2 = L.h * L.h, eps2 = c->p.model_params[0];
+  const double d2 = 2.0 * c->p.ndims;
+  const float tmp1 = (float)(c->p.dt / h2), tmp2 = (float)(eps2 / h2), tmp3 = (float)(d2 * eps2 / h2);
+  const float a12 = (float)(d2 * (c->p.dt / h2));
+  const bool fin = level == c->finest;
+  if (fin) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
+  DISPATCH_DIM(c->p.ndims,
+               (k_relax_lin<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
+               (k_relax_lin<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+  LAUNCH_CHECK();
+  if (fin) {
+    double N = 0, G = 0;
+    for (int pid : L.patches) {
+      const DPatch& P = c->hp[pid];
+      N += (double)P.mx[0] * P.mx[1] * P.mx[2];
+      G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
+                             : 2.0 * ((double)P.mx[0] + P.mx[1]);
+    }
+    // e_phi over N+G, e_mu of the other colour (N+G)/2, rhs (2 fields) and
+    // coef of the colour N/2 each, the colour's e written N/2 x 2 fields
+    kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + 0.5 * (N + G) + N + 0.5 * N + N));
+  }
+  return BSAM_OK;
+}
+
+int32_t mixed_level_relax(bsam_ctx* c, int level) {
+  for (int s = 0; s < c->p.nsmoothingpasses; s++)
+    for (int rb = 1; rb <= 2; rb++) {
+      RETCHK(mixed_relax(c, level, rb));
+      RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1));
+    }
+  return BSAM_OK;
+}
+
+int32_t mixed_vcycle(bsam_ctx* c, int level) {
+  bsam::MixedState* m = c->mixed;
+  RETCHK(mixed_level_relax(c, level));
+  if (level > c->minlevel) {
+    RETCHK(restrict_t<float>(c, level, m->e, m->e, m->ec));
+    RETCHK(set_ghost_t<float>(c, level - 1, 0, m->e, 1));
+    RETCHK(ring_t<float>(c, level, m->e, m->ec));
+    Level& L = c->L(level);
+    LevelArgs a = largs(c, L, kCoarse);
+    DISPATCH_DIM(c->p.ndims,
+                 (k_fas_rhs_lin<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)),
+                 (k_fas_rhs_lin<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)));
+    LAUNCH_CHECK();
+    RETCHK(mixed_vcycle(c, level - 1));
+    RETCHK(correct_t<float>(c, level, m->e, m->ec));
+    RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1));
+    RETCHK(mixed_level_relax(c, level));
+  }
+  return BSAM_OK;
+}
+
+// FP64 residual of levels >= 0 into the FP32 right-hand side; the finest
+// level's squared sums go to d_norm.
+int32_t mixed_residuals(bsam_ctx* c) {
+  bsam::MixedState* m = c->mixed;
+  const ModelK mk = c->mk();
+  int nbf = 0;
+  for (int level = c->finest; level >= 0; level--) {
+    Level& L = c->L(level);
+    LevelArgs a = largs(c, L, kCell);
+    const int nb = nblocks(a.total);
+    double* part = (level == c->finest) ? c->d_partial : nullptr;
+    const bool fin = level == c->finest;
+    if (fin) kstat_begin(c, BSAM_KSTAT_RESID_FINEST);
+    DISPATCH_DIM(c->p.ndims,
+                 (k_residual<2, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)),
+                 (k_residual<3, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)));
+    LAUNCH_CHECK();
+    if (fin) {
+      double N = 0, G = 0;
+      for (int pid : L.patches) {
+        const DPatch& P = c->hp[pid];
+        N += (double)P.mx[0] * P.mx[1] * P.mx[2];
+        G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
+                               : 2.0 * ((double)P.mx[0] + P.mx[1]);
+      }
+      kstat_end(c, BSAM_KSTAT_RESID_FINEST, 16.0 * (N + G) + 16.0 * N + 8.0 * N);
+      nbf = nb;
+    }
+  }
+  k_reduce_partials<<<1, kBlock, 0, c->s>>>(c->d_partial, nbf, c->d_norm);
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
+// One outer iteration: correction V-cycle, q += e, FillDown, ghost refresh
+// of the finest level, new residual and norm.
+int32_t mixed_iteration(bsam_ctx* c) {
+  bsam::MixedState* m = c->mixed;
+  k_zero_f<<<nblocks(c->nq), kBlock, 0, c->s>>>(m->e, c->nq);
+  LAUNCH_CHECK();
+  RETCHK(mixed_vcycle(c, c->finest));
+  for (int level = 0; level <= c->finest; level++) {
+    Level& L = c->L(level);
+    if (L.qspan_len > 0) {
+      k_add_corr<<<nblocks(L.qspan_len), kBlock, 0, c->s>>>(c->q + L.qspan_off, m->e + L.qspan_off, L.qspan_len);
+      LAUNCH_CHECK();
+    } else {
+      for (int pid : L.patches) {
+        const DPatch& P = c->hp[pid];
+        const long n = (long)P.qvs * c->p.nrvars;
+        k_add_corr<<<nblocks(n), kBlock, 0, c->s>>>(c->q + P.qoff, m->e + P.qoff, n);
+        LAUNCH_CHECK();
+      }
+    }
+  }
+  RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+  RETCHK(set_ghost_t<double>(c, c->finest, 0, c->q, 0));
+  RETCHK(mixed_residuals(c));
+  return BSAM_OK;
+}
+
+}  // namespace
+
+namespace bsam {
 int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residual, double* history,
                                    int32_t history_len) {
-  (void)c; (void)ncycles; (void)residual; (void)history; (void)history_len;
-  return set_err(BSAM_EINVAL, "mixed precision not available in this build");
+  RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_get_source(c, level));
+  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_update_source(c, level));
+  RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
+  for (int level = c->minlevel; level <= c->finest; level++) {
+    Level& L = c->L(level);
+    LevelArgs a = largs(c, L, kCell);
+    DISPATCH_DIM(c->p.ndims, (k_coef<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)),
+                 (k_coef<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)));
+    LAUNCH_CHECK();
+  }
+  RETCHK(mixed_residuals(c));
+  double res = 0.0;
+  int it;
+  for (it = 1; it <= c->p.maxvcycles; it++) {
+    RETCHK(run_iteration(c, mixed_iteration));
+    HIPCHK(hipStreamSynchronize(c->s));
+    res = norm_from(c, c->h_norm, c->finest);
+    if (history && it - 1 < history_len) history[it - 1] = res;
+    if (!std::isfinite(res)) {
+      *ncycles = it;
+      *residual = res;
+      return set_err(BSAM_EDIVERGED, "residual not finite after outer iteration %d", it);
+    }
+    if (res < c->p.qerrortol) break;
+  }
+  *ncycles = it > c->p.maxvcycles ? c->p.maxvcycles : it;
+  *residual = res;
+  return it > c->p.maxvcycles ? BSAM_NOT_CONVERGED : BSAM_OK;
 }
 }  // namespace bsam

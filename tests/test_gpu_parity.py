@@ -102,3 +102,37 @@ def test_graph_replay_amr(gpu_solver_cls):
     _hist_close(hist, st["residuals"])
     for i in st["end"]:
         assert np.array_equal(S.get(i, "q").ravel(), fu.end_state(z, 0, i))
+
+
+@pytest.mark.parametrize("nd,n,steps", [(3, 32, 3), (2, 64, 3)])
+def test_mixed_precision_matches_fp64(nd, n, steps, gpu_solver_cls):
+    """Mixed mode (FP32 correction V-cycles, FP64 residual and update)
+    converges to the FP64 path's solution: after each time step both stop at
+    RMS residual < 1e-9 and the max-norm difference of phi and mu is within
+    a small multiple of the tolerance."""
+    print(f"[gpu] mixed {nd}D {n}", flush=True)
+    minlevel = -(int(np.log2(n)) - 1)
+    hier = hierarchy.uniform(nd, (n,) * nd, minlevel, [2] * (2 * nd))
+    root = len(hier) - 1
+    phi = configs.init_phi(hier[root], nd, 1, 1003, 0.1, 0.05)
+    q0 = np.zeros((2,) + (n + 2,) * nd)
+    q0[(0,) + (slice(1, -1),) * nd] = phi
+    outs = {}
+    for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+        S = gpu_solver_cls(configs.params(nd, 1e-4, 0.05, 1e-9, precision=prec), hier)
+        S.put(root, "q", q0)
+        S.set_ghost(0, 0)
+        S.copy_q_to_qold(0)
+        cyc = []
+        for s in range(steps):
+            rc, ncyc, r = S.time_step(s == 0)
+            assert rc == abi.BSAM_OK and r < 1e-9, (prec, s, rc, r)
+            cyc.append(ncyc)
+        outs[prec] = (S.get(root, "q"), cyc)
+        S.close()
+    qa, ca = outs[abi.PREC_FP64]
+    qb, cb = outs[abi.PREC_MIXED]
+    err = float(np.max(np.abs(qa - qb)))
+    print(f"[gpu] mixed max-norm diff {err:.3e}, cycles fp64 {ca} mixed {cb}", flush=True)
+    assert err < 1e-7
+    assert max(cb) <= max(ca) + 2
