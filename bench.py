@@ -83,10 +83,6 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
-    import torch  # noqa: F401  (HIP runtime warm-up and device check)
-    if not torch.cuda.is_available():
-        print("bench.py: no GPU visible", file=sys.stderr)
-        sys.exit(1)
     from bsam_amd import abi
     n = a.n
     S, hier, p, q0, minlevel = make_state(n, a.precision, a.graph)
