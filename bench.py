@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--check-fp64", type=int, default=1,
+                    help="mixed mode: also run the FP64 path and report the max-norm difference")
     ap.add_argument("--kstat-steps", type=int, default=1,
                     help="extra instrumented steps for the kernel roofline (0: skip)")
     return ap.parse_args()
@@ -119,6 +121,26 @@ def main():
         roof = dict(bound="hbm", kernel="k_relax_ch<3> on the finest level", achieved=achieved,
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=None, avg_us=avg_ms * 1e3, launches=nl, bytes_per_launch=b)
+    err = None
+    if a.precision == "mixed" and a.check_fp64:
+        # the FP64 path on the same steps (bit-identical to the oracle in the
+        # parity tests) as the reference for the mixed path's error
+        import numpy as np
+        print("[bench] fp64 reference run for the mixed-precision error", file=sys.stderr, flush=True)
+        root = len(hier) - 1
+        qm = S.get(root, "q")
+        S.close()
+        R, _, _, _, _ = make_state(n, "fp64", a.graph)
+        first_r = True
+        for _ in range(a.warmup + a.steps + a.kstat_steps):
+            R.time_step(first_r)
+            first_r = False
+        qr = R.get(root, "q")
+        inner = (slice(1, -1),) * 3
+        err = {"phi": float(np.max(np.abs(qm[(0,) + inner] - qr[(0,) + inner]))),
+               "mu": float(np.max(np.abs(qm[(1,) + inner] - qr[(1,) + inner]))),
+               "steps": a.warmup + a.steps + a.kstat_steps}
+        R.close()
     cpu = None
     if a.cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -146,6 +168,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if err is not None:
+        out["max_norm_error_vs_fp64"] = err
     print(json.dumps(out))
 
 

@@ -102,7 +102,30 @@ struct LevelArgs {
 // Domain: cells of colour c, (mx+1)/2 per row.  Colour c relaxes cells with
 // (i+j+k) mod 2 == c; the reference's ipass = c+1 (problem.f90:350, 591).
 // ---------------------------------------------------------------------------
-template <int DIM>
+// Ghost images of an updated interior cell of a patch that is periodic in
+// every dimension (the net effect of SelfPeriodicBC, boundary.f90:480-577,
+// on the colour just relaxed): every ghost whose periodic image is (i,j,k)
+// takes its value, plus 0.0 on the x ghost planes (the qpo term).  The
+// images have the cell's colour because mx is even.
+template <int DIM, typename T>
+__device__ __forceinline__ void wrap_images(T* __restrict__ q, const DPatch& P, int i, int j, int k, T v0,
+                                            T v1) {
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  const bool ex = (i == 1 || i == mx), ey = (j == 1 || j == my), ez = (DIM == 3) && (k == 1 || k == mz);
+  if (!(ex || ey || ez)) return;
+  const int gi = (i == 1) ? mx + 1 : 0, gj = (j == 1) ? my + 1 : 0, gk = (k == 1) ? mz + 1 : 0;
+  for (int a = 0; a <= (ex ? 1 : 0); a++)
+    for (int b = 0; b <= (ey ? 1 : 0); b++)
+      for (int d = 0; d <= (ez ? 1 : 0); d++) {
+        if (a + b + d == 0) continue;
+        const int ii = a ? gi : i, jj = b ? gj : j, kk = d ? gk : k;
+        const int o = qidx(P, ii, jj, kk);
+        q[o] = a ? v0 + (T)0 : v0;
+        q[o + P.qvs] = a ? v1 + (T)0 : v1;
+      }
+}
+
+template <int DIM, bool WRAP>
 __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double dt, double eps2,
                                                      double h, double* __restrict__ q,
                                                      const double* __restrict__ f) {
@@ -142,8 +165,140 @@ __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double 
   const double b1 = f0 + tmp1 * smu;
   const double b2 = f1 - tmp2 * sphi - 2.0 * (phi * phi * phi);
   const double det = 1.0 - a12 * a21;
-  q[o] = (b1 - a12 * b2) / det;
-  q[o + vs] = (b2 - a21 * b1) / det;
+  const double n0 = (b1 - a12 * b2) / det, n1 = (b2 - a21 * b1) / det;
+  q[o] = n0;
+  q[o + vs] = n1;
+  if (WRAP) wrap_images<DIM, double>(q, P, i, j, k, n0, n1);
+}
+
+// 3D colour sweep of one large patch, marching in k.  A block owns a tile
+// of 64 half-row positions x 8 rows and KC planes; the other colour's phi
+// and mu of planes k-1, k, k+1 (tile plus one halo row each side and one
+// halo position) sit in a 4-slot LDS ring, so every value of the other
+// colour is read from HBM about once, and the own colour's phi and f once.
+// LIN=false: CH RelaxGrid3D (CH/problem.f90:554-614) in FP64 with f.
+// LIN=true:  the linearised correction smoother (FP32) with rhs and coef.
+// WRAP: also write the periodic ghost images of the updated cells.
+// FINEST only separates the finest level's launches in kernel traces.
+template <typename T, bool LIN, bool WRAP, int FINEST>
+__global__ __launch_bounds__(512) void k_relax3d_march(const DPatch* __restrict__ pt, int pid, int c, int KC,
+                                                      T tmp1, T tmp2, T tmp3, T a12, T* __restrict__ q,
+                                                      const T* __restrict__ f, const T* __restrict__ coef) {
+  constexpr int TX = 64, TY = 8, NR = TY + 2, NP = TX + 1;
+  __shared__ T sh[4][2][NR][NP];
+  const DPatch& P = pt[pid];
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  const int nx2 = mx >> 1;
+  const int ntx = (nx2 + TX - 1) / TX, nty = (my + TY - 1) / TY;
+  int lb = xcd_block();
+  const int bx = lb % ntx;
+  lb /= ntx;
+  const int by = lb % nty;
+  const int bk = lb / nty;
+  const int t0 = bx * TX, j0 = by * TY;
+  const int k0 = bk * KC + 1, k1 = min(mz, k0 + KC - 1);
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int cc = c ^ 1;
+  const int hq = P.hq, ny = P.nq[1], nz = P.nq[2], vs = P.qvs, base = P.qoff;
+  auto rowoff = [&](int col, int k, int j) { return base + ((col * nz + k) * ny + j) * hq; };
+  // One plane of the other colour's tile is NE entries; each thread owns
+  // the entries threadIdx.x + 512*n.  Loads for plane k+3 are issued at the
+  // end of iteration k into registers and stored to LDS during iteration
+  // k+1, so their latency overlaps a plane of arithmetic.
+  constexpr int NE = 2 * NR * NP;
+  constexpr int NPER = (NE + 511) / 512;
+  auto fetch = [&](int k, T (&reg)[NPER]) {
+#pragma unroll
+    for (int n = 0; n < NPER; n++) {
+      const int e = threadIdx.x + 512 * n;
+      T val = (T)0;
+      if (e < NE) {
+        const int v = e / (NR * NP);
+        const int r = (e / NP) % NR;
+        const int p = e % NP;
+        const int jj = j0 + r, pp = t0 + p;
+        if (jj <= my + 1 && pp <= nx2) val = q[rowoff(cc, k, jj) + pp + v * vs];
+      }
+      reg[n] = val;
+    }
+  };
+  auto store = [&](int k, const T (&reg)[NPER]) {
+    T* sl = &sh[k & 3][0][0][0];
+#pragma unroll
+    for (int n = 0; n < NPER; n++) {
+      const int e = threadIdx.x + 512 * n;
+      if (e < NE) sl[e] = reg[n];
+    }
+  };
+  T pre[NPER];
+  fetch(k0 - 1, pre);
+  store(k0 - 1, pre);
+  fetch(k0, pre);
+  store(k0, pre);
+  fetch(k0 + 1, pre);
+  store(k0 + 1, pre);
+  if (k0 + 2 <= k1 + 1) fetch(k0 + 2, pre);
+  const int t = t0 + tx, j = j0 + 1 + ty;
+  const bool active = (t < nx2) && (j <= my);
+  // own-colour values of the current plane, prefetched one plane ahead
+  T own0 = (T)0, own1 = (T)0, own2 = (T)0;
+  auto own_off = [&](int k) { return rowoff(c, k, j) + t + 1 - ((c + j + k) & 1); };
+  if (active) {
+    const int o = own_off(k0);
+    own0 = f[o];
+    own1 = f[o + vs];
+    if (LIN) own2 = coef[o]; else own2 = q[o];
+  }
+  __syncthreads();
+  for (int k = k0; k <= k1; k++) {
+    T nx0 = (T)0, nx1 = (T)0, nx2v = (T)0;
+    if (active && k + 1 <= k1) {
+      const int o = own_off(k + 1);
+      nx0 = f[o];
+      nx1 = f[o + vs];
+      if (LIN) nx2v = coef[o]; else nx2v = q[o];
+    }
+    if (active) {
+      const int ib = (c + j + k) & 1;
+      const int m = t + 1 - ib;
+      const int pr = tx + 1 - ib;
+      const int sk = k & 3, skm = (k - 1) & 3, skp = (k + 1) & 3;
+      const int r = ty + 1;
+      const T smu = sh[sk][1][r][tx] + sh[sk][1][r][tx + 1] + sh[sk][1][r - 1][pr] + sh[sk][1][r + 1][pr] +
+                    sh[skm][1][r][pr] + sh[skp][1][r][pr];
+      const T sphi = sh[sk][0][r][tx + 1] + sh[sk][0][r][tx] + sh[sk][0][r + 1][pr] + sh[sk][0][r - 1][pr] +
+                     sh[skp][0][r][pr] + sh[skm][0][r][pr];
+      const int o = rowoff(c, k, j) + m;
+      T n0, n1;
+      if (LIN) {
+        const T a21 = -own2 - tmp3;
+        const T b1 = own0 + tmp1 * smu;
+        const T b2 = own1 - tmp2 * sphi;
+        const T det = (T)1 - a12 * a21;
+        n0 = (b1 - a12 * b2) / det;
+        n1 = (b2 - a21 * b1) / det;
+      } else {
+        const T phi = own2;
+        const T a21 = (T)-3 * (phi * phi) - tmp3;
+        const T b1 = own0 + tmp1 * smu;
+        const T b2 = own1 - tmp2 * sphi - (T)2 * (phi * phi * phi);
+        const T det = (T)1 - a12 * a21;
+        n0 = (b1 - a12 * b2) / det;
+        n1 = (b2 - a21 * b1) / det;
+      }
+      q[o] = n0;
+      q[o + vs] = n1;
+      if (WRAP) wrap_images<3, T>(q, P, 2 * t + 2 - ib, j, k, n0, n1);
+    }
+    if (k + 2 <= k1 + 1) {
+      store(k + 2, pre);
+      if (k + 3 <= k1 + 1) fetch(k + 3, pre);
+    }
+    own0 = nx0;
+    own1 = nx1;
+    own2 = nx2v;
+    __syncthreads();
+  }
 }
 
 // BioGrowth RelaxGrid2D, BioGrowth/problem.f90:255-296 (decoupled GS).
@@ -248,8 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      TO* __restrict__ rf,
                                                      double* __restrict__ partial) {
   const int lb = xcd_block();
-  const int g = lb * kBlock + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
+  const int g = lb * kBlock + threadIdx.x;
   if (g < L.total) {
     const int pi = find_patch(L.prefix, L.npatch, g);
     const DPatch& P = L.pt[L.patches[pi]];
@@ -266,8 +421,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
       rf[o] = (TO)r0;
       rf[o + P.qvs] = (TO)r1;
     }
-    s0 = r0 * r0;
-    s1 = r1 * r1;
+    s0 += r0 * r0;
+    s1 += r1 * r1;
   }
   if (!partial) return;
   // wavefront then block reduction in a fixed order (deterministic)
@@ -293,12 +448,15 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
   }
 }
 
-// Sum the per-block partials in a fixed order into out[0..1].
+// Sum the per-block partials in a fixed order: block b of this launch sums
+// the chunk [b*chunk, (b+1)*chunk) of `partial` into out[2b..2b+1]
+// (launched with many blocks, then with one).
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ partial,
-                                                            int nb, double* __restrict__ out) {
+                                                            int nb, int chunk, double* __restrict__ out) {
   __shared__ double s[2][kBlock];
   double a = 0.0, b = 0.0;
-  for (int x = threadIdx.x; x < nb; x += kBlock) {
+  const int lo = blockIdx.x * chunk, hi = min(nb, lo + chunk);
+  for (int x = lo + threadIdx.x; x < hi; x += kBlock) {
     a += partial[2 * x];
     b += partial[2 * x + 1];
   }
@@ -313,8 +471,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __rest
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = s[0][0];
-    out[1] = s[1][0];
+    out[2 * blockIdx.x] = s[0][0];
+    out[2 * blockIdx.x + 1] = s[1][0];
   }
 }
 
@@ -358,7 +516,11 @@ __device__ __forceinline__ bool decode_coarse(const LevelArgs& L, int g, CoarseC
 // RestrictSolution (afasroutines.f90:610-652) when `qc` is non-null: qc =
 // R q and parent%q(mb) = qc.  FillDownLevel (afasroutines.f90:257-327) when
 // `qc` is null: parent field(mb) = R field (field = q or f).
-template <int DIM, typename T>
+// WRAP: the parent is one patch periodic in every dimension that this
+// level's single patch covers completely; the restriction then also writes
+// the parent's ghost images, which is the whole effect of the following
+// SetGhost(L-1, 0).
+template <int DIM, typename T, bool WRAP = false>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const T* __restrict__ src,
                                                      T* __restrict__ dst_parent, T* __restrict__ qc) {
   CoarseCell c;
@@ -367,11 +529,14 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const 
   const DPatch& Q = L.pt[P.parent];
   const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
   const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
+  T rv[2] = {(T)0, (T)0};
   for (int v = 0; v < nv; v++) {
     const T r = restrict_cell<DIM, T>(src, P, c.ic, c.jc, c.kc, v * P.qvs);
     if (qc) qc[cidx(P, c.ic, c.jc, c.kc) + v * P.cvs] = r;
     dst_parent[qidx(Q, ip, jp, kp) + v * Q.qvs] = r;
+    if (v < 2) rv[v] = r;
   }
+  if (WRAP) wrap_images<DIM, T>(dst_parent, Q, ip, jp, kp, rv[0], rv[1]);
 }
 
 // GetCoarseGhostPoints (boundary.f90:957-1015): the ring of qc <- parent q.
@@ -481,6 +646,70 @@ __device__ __forceinline__ T linw(T lo, T hi, int fine) {
   return (fine & 1) ? lo + (T)3 * hi : (T)3 * lo + hi;
 }
 
+// CorrectFine, second half: q(0:mx+1) += P1(qc).  One thread per dual
+// cell (ic,jc[,kc]) in 0..cmx: it loads the 2^d coarse values once and
+// updates the 2^d fine cells 2ic..2ic+1 (x) etc., all of which interpolate
+// from exactly those coarse values.  Same expressions as below, so the
+// result is bit-identical.  Domain: (cmx+1)^d.
+template <int DIM, typename T, bool WRAP = false>
+__global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T* __restrict__ q,
+                                                         const T* __restrict__ qc) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g >= L.total) return;
+  const int pi = find_patch(L.prefix, L.npatch, g);
+  const DPatch& P = L.pt[L.patches[pi]];
+  const int local = g - L.prefix[pi];
+  const int n0 = P.cmx[0] + 1, n1 = P.cmx[1] + 1;
+  const int ic = local % n0;
+  const int row = local / n0;
+  const int jc = row % n1;
+  const int kc = (DIM == 3) ? row / n1 : 0;
+  // coarse values of both variables around the dual cell
+  T a[2][2][2][2];
+#pragma unroll
+  for (int v = 0; v < 2; v++)
+#pragma unroll
+    for (int d = 0; d < 2; d++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int e = 0; e < 2; e++)
+          a[v][d][b][e] = (DIM == 3 || d == 0) ? qc[cidx(P, ic + e, jc + b, kc + d) + v * P.cvs] : (T)0;
+  const int nk = (DIM == 3) ? 2 : 1;
+  for (int dk = 0; dk < nk; dk++)
+#pragma unroll
+    for (int db = 0; db < 2; db++)
+#pragma unroll
+      for (int da = 0; da < 2; da++) {
+        const int i = 2 * ic + da, j = 2 * jc + db, k = (DIM == 3) ? 2 * kc + dk : 0;
+        const bool ghost = i == 0 || i == P.mx[0] + 1 || j == 0 || j == P.mx[1] + 1 ||
+                           (DIM == 3 && (k == 0 || k == P.mx[2] + 1));
+        if (WRAP && ghost) continue;
+        T nv2[2];
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+          T pr;
+          if (DIM == 2) {
+            const T b0 = linw<T>(a[v][0][0][0], a[v][0][0][1], i), b1 = linw<T>(a[v][0][1][0], a[v][0][1][1], i);
+            pr = linw<T>(b0, b1, j) / (T)16;
+          } else {
+            T cc2[2];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+              const T b0 = linw<T>(a[v][d][0][0], a[v][d][0][1], i);
+              const T b1 = linw<T>(a[v][d][1][0], a[v][d][1][1], i);
+              cc2[d] = linw<T>(b0, b1, j);
+            }
+            pr = linw<T>(cc2[0], cc2[1], k) / (T)64;
+          }
+          const int o = qidx(P, i, j, k) + v * P.qvs;
+          nv2[v] = q[o] + pr;
+          q[o] = nv2[v];
+        }
+        if (WRAP) wrap_images<DIM, T>(q, P, i, j, k, nv2[0], nv2[1]);
+      }
+}
+
 // CorrectFine, second half: q(0:mx+1) += P1(qc).  Domain: (mx+2)^d.
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* __restrict__ q,
@@ -535,7 +764,7 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double
   const int j = row % P.mx[1] + 1;
   const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
   const int o = qidx(P, i, j, k);
-  if (mode == 0) {
+  if (mode != 1) {
     const double po = qold[o];
     if (MODEL == BSAM_MODEL_CH) {
       ftmp[o] = po;
@@ -544,7 +773,8 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double
       ftmp[o] = (1.0 + dt) * po;
       ftmp[o + P.qvs] = 0.0;
     }
-  } else {
+  }
+  if (mode != 0) {
     f[o] = ftmp[o] + 0.0;
     f[o + P.qvs] = ftmp[o + P.qvs] + 0.0;
   }
@@ -914,7 +1144,7 @@ __device__ __forceinline__ float ulap_cf(const float* __restrict__ a, const DPat
   return s - 4.0f * a[cidx(P, i, j, k) + off];
 }
 
-template <int DIM>
+template <int DIM, bool WRAP>
 __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float tmp1, float tmp2, float tmp3,
                                                       float a12, float* __restrict__ e,
                                                       const float* __restrict__ rhs,
@@ -947,8 +1177,10 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
   const float b1 = rhs[o] + tmp1 * smu;
   const float b2 = rhs[o + vs] - tmp2 * sphi;
   const float det = 1.0f - a12 * a21;
-  e[o] = (b1 - a12 * b2) / det;
-  e[o + vs] = (b2 - a21 * b1) / det;
+  const float n0 = (b1 - a12 * b2) / det, n1 = (b2 - a21 * b1) / det;
+  e[o] = n0;
+  e[o + vs] = n1;
+  if (WRAP) wrap_images<DIM, float>(e, P, i, j, k, n0, n1);
 }
 
 // Coarse right-hand side of the correction V-cycle (the FAS form of
@@ -990,6 +1222,280 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
   const float a2 = em - coef[opar] * ep + T2 * ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, 0);
   rpar[opar] = w * s0 + a1;
   rpar[opar + Q.qvs] = w * s1 + a2;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse end of the V-cycle in one workgroup.  Below a level with at most
+// 16^3 (3D) or 64^2 (2D) cells every level is one patch periodic in every
+// dimension (the multigrid levels under the root, bsamroutines.f90:443-510),
+// and AFASVcycle (afasroutines.f90:495-529) on those levels is a chain of
+// small dependent steps.  This kernel runs that chain -- LevelRelax,
+// RestrictSolution, SetGhost(L-1,0), GetCoarseGhostPoints, the coarse load,
+// recursion, CorrectFine, SetGhost(L,0), LevelRelax -- with workgroup
+// barriers instead of kernel launches.  Each step uses the same per-cell
+// expressions as the level-wide kernels, so the results are bit-identical.
+// LIN=false: the FP64 CH V-cycle; LIN=true: the FP32 correction V-cycle.
+// ---------------------------------------------------------------------------
+struct CoarseArgs {
+  int pid[16];       // patch of each level, index 0 = the top level of the chain
+  int nlev;          // levels in the chain (top .. minlevel)
+  double dt, eps2;
+};
+
+template <int DIM, typename T, bool LIN>
+__device__ void cv_relax(const DPatch& P, int c, double dt, double eps2, T* __restrict__ q,
+                         const T* __restrict__ f, const T* __restrict__ coef) {
+  const double h2 = P.dx * P.dx;
+  const double d2 = (DIM == 3) ? 6.0 : 4.0;
+  const T tmp1 = (T)(dt / h2), tmp2 = (T)(eps2 / h2), tmp3 = (T)(d2 * eps2 / h2);
+  const T a12 = (T)(d2 * (dt / h2));
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int total = cnt * P.mx[1] * (DIM == 3 ? P.mx[2] : 1);
+  const int vs = P.qvs;
+  for (int g = threadIdx.x; g < total; g += blockDim.x) {
+    const int t = g % cnt, row = g / cnt;
+    const int j = row % P.mx[1] + 1;
+    const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+    const int ib = (c + j + k) & 1;
+    const int i = 2 * t + 2 - ib;
+    if (i > P.mx[0]) continue;
+    const int o = qidx(P, i, j, k);
+    const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
+    const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
+    T smu, sphi;
+    if (DIM == 3) {
+      const int zm = qidx(P, i, j, k - 1), zp = qidx(P, i, j, k + 1);
+      smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs] + q[zm + vs] + q[zp + vs];
+      sphi = q[xp] + q[xm] + q[yp] + q[ym] + q[zp] + q[zm];
+    } else {
+      smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs];
+      sphi = q[xp] + q[xm] + q[yp] + q[ym];
+    }
+    T n0, n1;
+    if constexpr (LIN) {
+      const T a21 = -coef[o] - tmp3;
+      const T b1 = f[o] + tmp1 * smu;
+      const T b2 = f[o + vs] - tmp2 * sphi;
+      const T det = (T)1 - a12 * a21;
+      n0 = (b1 - a12 * b2) / det;
+      n1 = (b2 - a21 * b1) / det;
+    } else {
+      const T phi = q[o];
+      const T a21 = (T)-3 * (phi * phi) - tmp3;
+      const T b1 = f[o] + tmp1 * smu;
+      const T b2 = f[o + vs] - tmp2 * sphi - (T)2 * (phi * phi * phi);
+      const T det = (T)1 - a12 * a21;
+      n0 = (b1 - a12 * b2) / det;
+      n1 = (b2 - a21 * b1) / det;
+    }
+    q[o] = n0;
+    q[o + vs] = n1;
+    wrap_images<DIM, T>(q, P, i, j, k, n0, n1);
+  }
+}
+
+template <int DIM, typename T>
+__device__ void cv_wrap(const DPatch& P, T* __restrict__ q) {
+  const int nx = P.nq[0], ny = P.nq[1], nz = P.nq[2];
+  const int nxp = 2 * ny * nz, nyp = 2 * (nx - 2) * nz, nzp = (DIM == 3) ? 2 * (nx - 2) * (ny - 2) : 0;
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  for (int g = threadIdx.x; g < nxp + nyp + nzp; g += blockDim.x) {
+    int i, j, k;
+    if (g < nxp) {
+      i = (g & 1) ? nx - 1 : 0;
+      const int r = g >> 1;
+      j = r % ny;
+      k = r / ny;
+    } else if (g < nxp + nyp) {
+      const int r0 = g - nxp;
+      j = (r0 & 1) ? ny - 1 : 0;
+      const int r = r0 >> 1;
+      i = r % (nx - 2) + 1;
+      k = r / (nx - 2);
+    } else {
+      const int r0 = g - nxp - nyp;
+      k = (r0 & 1) ? nz - 1 : 0;
+      const int r = r0 >> 1;
+      i = r % (nx - 2) + 1;
+      j = r / (nx - 2) + 1;
+    }
+    const int si = (i == 0) ? mx : (i == mx + 1 ? 1 : i);
+    const int sj = (j == 0) ? my : (j == my + 1 ? 1 : j);
+    const int sk = (DIM == 3) ? ((k == 0) ? mz : (k == mz + 1 ? 1 : k)) : 0;
+    const bool xg = (i == 0 || i == mx + 1);
+    for (int v = 0; v < 2; v++) {
+      const T x = q[qidx(P, si, sj, sk) + v * P.qvs];
+      q[qidx(P, i, j, k) + v * P.qvs] = xg ? x + (T)0 : x;
+    }
+  }
+}
+
+template <int DIM, typename T, bool LIN>
+__device__ void cv_down(const DPatch& P, const DPatch& Q, double dt, double eps2, T* __restrict__ q,
+                        T* __restrict__ f, T* __restrict__ qc, const T* __restrict__ coef) {
+  const int ncc = P.cmx[0] * P.cmx[1] * (DIM == 3 ? P.cmx[2] : 1);
+  // RestrictSolution: qc = R q, parent q(mb) = qc
+  for (int g = threadIdx.x; g < ncc; g += blockDim.x) {
+    const int ic = g % P.cmx[0] + 1, row = g / P.cmx[0];
+    const int jc = row % P.cmx[1] + 1, kc = (DIM == 3) ? row / P.cmx[1] + 1 : 0;
+    const int ip = P.mb[0][0] + ic - 1, jp = P.mb[1][0] + jc - 1, kp = (DIM == 3) ? P.mb[2][0] + kc - 1 : 0;
+    for (int v = 0; v < 2; v++) {
+      const T r = restrict_cell<DIM, T>(q, P, ic, jc, kc, v * P.qvs);
+      qc[cidx(P, ic, jc, kc) + v * P.cvs] = r;
+      q[qidx(Q, ip, jp, kp) + v * Q.qvs] = r;
+    }
+  }
+  __syncthreads();
+  cv_wrap<DIM, T>(Q, q);  // SetGhost(L-1, 0)
+  __syncthreads();
+  // GetCoarseGhostPoints
+  const int nbox = P.nc[0] * P.nc[1] * P.nc[2];
+  for (int g = threadIdx.x; g < nbox; g += blockDim.x) {
+    const int ic = g % P.nc[0], row = g / P.nc[0];
+    const int jc = row % P.nc[1], kc = (DIM == 3) ? row / P.nc[1] : 0;
+    const bool interior = ic >= 1 && ic <= P.cmx[0] && jc >= 1 && jc <= P.cmx[1] &&
+                          (DIM == 2 || (kc >= 1 && kc <= P.cmx[2]));
+    if (interior) continue;
+    const int ip = P.mb[0][0] - 1 + ic, jp = P.mb[1][0] - 1 + jc, kp = (DIM == 3) ? P.mb[2][0] - 1 + kc : 0;
+    for (int v = 0; v < 2; v++) qc[cidx(P, ic, jc, kc) + v * P.cvs] = q[qidx(Q, ip, jp, kp) + v * Q.qvs];
+  }
+  __syncthreads();
+  // coarse load: parent f(mb) = R(f - L_h q) + L_2h(qc)
+  const int nch = (DIM == 3) ? 8 : 4;
+  for (int g = threadIdx.x; g < ncc; g += blockDim.x) {
+    const int ic = g % P.cmx[0] + 1, row = g / P.cmx[0];
+    const int jc = row % P.cmx[1] + 1, kc = (DIM == 3) ? row / P.cmx[1] + 1 : 0;
+    const int ip = P.mb[0][0] + ic - 1, jp = P.mb[1][0] + jc - 1, kp = (DIM == 3) ? P.mb[2][0] + kc - 1 : 0;
+    const int opar = qidx(Q, ip, jp, kp);
+    const int oc = cidx(P, ic, jc, kc);
+    T s0 = (T)0, s1 = (T)0;
+    if constexpr (LIN) {
+      const T t1 = (T)(dt / (P.dx * P.dx)), t2 = (T)(eps2 / (P.dx * P.dx));
+      for (int n = 0; n < nch; n++) {
+        const int i = 2 * ic - (n & 1), j = 2 * jc - ((n >> 1) & 1), k = (DIM == 3) ? 2 * kc - ((n >> 2) & 1) : 0;
+        const int o = qidx(P, i, j, k);
+        const T ep = q[o], em = q[o + P.qvs];
+        const T a1 = ep - t1 * ulap_f<DIM>(q, P, i, j, k, P.qvs);
+        const T a2 = em - coef[o] * ep + t2 * ulap_f<DIM>(q, P, i, j, k, 0);
+        s0 += f[o] - a1;
+        s1 += f[o + P.qvs] - a2;
+      }
+      const T w = (DIM == 3) ? (T)0.125 : (T)0.25;
+      const T T1 = (T)(dt / (Q.dx * Q.dx)), T2 = (T)(eps2 / (Q.dx * Q.dx));
+      const T ep = qc[oc], em = qc[oc + P.cvs];
+      const T a1 = ep - T1 * ulap_cf<DIM>(qc, P, ic, jc, kc, P.cvs);
+      const T a2 = em - coef[opar] * ep + T2 * ulap_cf<DIM>(qc, P, ic, jc, kc, 0);
+      f[opar] = w * s0 + a1;
+      f[opar + Q.qvs] = w * s1 + a2;
+    } else {
+      ModelK m;
+      m.dt = dt; m.eps2 = eps2; m.gam = 0.0; m.lam = 0.0; m.cz = 0.0;
+      for (int n = 0; n < nch; n++) {
+        const int i = 2 * ic - (n & 1), j = 2 * jc - ((n >> 1) & 1), k = (DIM == 3) ? 2 * kc - ((n >> 2) & 1) : 0;
+        double o1, o2;
+        op_q<DIM, BSAM_MODEL_CH>(q, q, P, i, j, k, m, P.dx, o1, o2);
+        const int o = qidx(P, i, j, k);
+        const T a0 = f[o] - o1, a1 = f[o + P.qvs] - o2;
+        if (n == 0) { s0 = a0; s1 = a1; } else { s0 = s0 + a0; s1 = s1 + a1; }
+      }
+      const T r0 = ((DIM == 3) ? (T)0.125 : (T)0.25) * s0;
+      const T r1 = ((DIM == 3) ? (T)0.125 : (T)0.25) * s1;
+      const double lmu = ulap_c<DIM>(qc, P, ic, jc, kc, P.cvs);
+      const double lphi = ulap_c<DIM>(qc, P, ic, jc, kc, 0);
+      double o1, o2;
+      op_from<DIM, BSAM_MODEL_CH>(qc[oc], qc[oc + P.cvs], lphi, lmu, 0.0, m, Q.dx, o1, o2);
+      f[opar] = r0 + o1;
+      f[opar + Q.qvs] = r1 + o2;
+    }
+  }
+}
+
+template <int DIM, typename T>
+__device__ void cv_up(const DPatch& P, const DPatch& Q, T* __restrict__ q, T* __restrict__ qc) {
+  // CorrectFine: qc = parent q(mb-1:mb+1) - qc, then q += P1(qc)
+  const int nbox = P.nc[0] * P.nc[1] * P.nc[2];
+  for (int g = threadIdx.x; g < nbox; g += blockDim.x) {
+    const int ic = g % P.nc[0], row = g / P.nc[0];
+    const int jc = row % P.nc[1], kc = (DIM == 3) ? row / P.nc[1] : 0;
+    const int ip = P.mb[0][0] - 1 + ic, jp = P.mb[1][0] - 1 + jc, kp = (DIM == 3) ? P.mb[2][0] - 1 + kc : 0;
+    for (int v = 0; v < 2; v++) {
+      const int o = cidx(P, ic, jc, kc) + v * P.cvs;
+      qc[o] = q[qidx(Q, ip, jp, kp) + v * Q.qvs] - qc[o];
+    }
+  }
+  __syncthreads();
+  const int n0 = P.cmx[0] + 1, n1 = P.cmx[1] + 1, n2 = (DIM == 3) ? P.cmx[2] + 1 : 1;
+  for (int g = threadIdx.x; g < n0 * n1 * n2; g += blockDim.x) {
+    const int ic = g % n0, row = g / n0;
+    const int jc = row % n1, kc = (DIM == 3) ? row / n1 : 0;
+    for (int v = 0; v < 2; v++) {
+      const int cv = v * P.cvs, fv = v * P.qvs;
+      if (DIM == 2) {
+        const T a00 = qc[cidx(P, ic, jc, 0) + cv], a01 = qc[cidx(P, ic + 1, jc, 0) + cv];
+        const T a10 = qc[cidx(P, ic, jc + 1, 0) + cv], a11 = qc[cidx(P, ic + 1, jc + 1, 0) + cv];
+        for (int b = 0; b < 2; b++)
+          for (int a = 0; a < 2; a++) {
+            const int i = 2 * ic + a, j = 2 * jc + b;
+            const T b0 = linw<T>(a00, a01, i), b1 = linw<T>(a10, a11, i);
+            const T pr = linw<T>(b0, b1, j) / (T)16;
+            const int o = qidx(P, i, j, 0) + fv;
+            q[o] = q[o] + pr;
+          }
+      } else {
+        T a[2][2][2];
+        for (int d = 0; d < 2; d++)
+          for (int b = 0; b < 2; b++)
+            for (int e = 0; e < 2; e++) a[d][b][e] = qc[cidx(P, ic + e, jc + b, kc + d) + cv];
+        for (int dk = 0; dk < 2; dk++)
+          for (int db = 0; db < 2; db++)
+            for (int da = 0; da < 2; da++) {
+              const int i = 2 * ic + da, j = 2 * jc + db, k = 2 * kc + dk;
+              T cc2[2];
+              for (int d = 0; d < 2; d++) {
+                const T b0 = linw<T>(a[d][0][0], a[d][0][1], i);
+                const T b1 = linw<T>(a[d][1][0], a[d][1][1], i);
+                cc2[d] = linw<T>(b0, b1, j);
+              }
+              const T pr = linw<T>(cc2[0], cc2[1], k) / (T)64;
+              const int o = qidx(P, i, j, k) + fv;
+              q[o] = q[o] + pr;
+            }
+      }
+    }
+  }
+  __syncthreads();
+  cv_wrap<DIM, T>(P, q);  // SetGhost(L, 0)
+}
+
+template <int DIM, typename T, bool LIN>
+__device__ void cv_level_relax(const DPatch& P, int nsm, double dt, double eps2, T* q, const T* f,
+                               const T* coef) {
+  for (int s = 0; s < nsm; s++)
+    for (int c = 0; c < 2; c++) {
+      cv_relax<DIM, T, LIN>(P, c, dt, eps2, q, f, coef);
+      __syncthreads();
+    }
+}
+
+template <int DIM, typename T, bool LIN>
+__global__ __launch_bounds__(1024) void k_coarse_vcycle(const DPatch* __restrict__ pt, CoarseArgs A, int nsm,
+                                                        T* __restrict__ q, T* __restrict__ f,
+                                                        T* __restrict__ qc, const T* __restrict__ coef) {
+  // descend
+  for (int l = 0; l < A.nlev - 1; l++) {
+    const DPatch& P = pt[A.pid[l]];
+    cv_level_relax<DIM, T, LIN>(P, nsm, A.dt, A.eps2, q, f, coef);
+    cv_down<DIM, T, LIN>(P, pt[A.pid[l + 1]], A.dt, A.eps2, q, f, qc, coef);
+    __syncthreads();
+  }
+  cv_level_relax<DIM, T, LIN>(pt[A.pid[A.nlev - 1]], nsm, A.dt, A.eps2, q, f, coef);
+  // ascend
+  for (int l = A.nlev - 2; l >= 0; l--) {
+    const DPatch& P = pt[A.pid[l]];
+    cv_up<DIM, T>(P, pt[A.pid[l + 1]], q, qc);
+    __syncthreads();
+    cv_level_relax<DIM, T, LIN>(P, nsm, A.dt, A.eps2, q, f, coef);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1035,7 +1541,7 @@ __global__ __launch_bounds__(kBlock) void k_layout(const DPatch* __restrict__ pt
 // ============================================================================
 namespace {
 
-enum Dom { kHalf = 0, kCell, kCoarse, kFineG, kQcBox, kNumDom };
+enum Dom { kHalf = 0, kCell, kCoarse, kFineG, kQcBox, kDual, kNumDom };
 
 struct Stage {
   BoxDesc* d = nullptr;
@@ -1050,6 +1556,7 @@ struct Level {
   int total[kNumDom] = {};
   std::vector<Stage> stages;      // box stages after interpolation
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
+  int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
   double h = 0.0;
   long qspan_off = 0, qspan_len = 0;  // contiguous q-pool span of the level
 };
@@ -1080,6 +1587,7 @@ struct bsam_ctx {
   std::vector<BoxDesc*> box_allocs;
   // reductions
   double* d_partial = nullptr;
+  double* d_partial2 = nullptr;
   int partial_cap = 0;
   double* d_norm = nullptr;
   double* h_norm = nullptr;
@@ -1218,6 +1726,7 @@ extern "C" int32_t bsam_destroy(bsam_ctx* c) {
   if (c->s) hipStreamSynchronize(c->s);
   free_hierarchy(c);
   hipFree(c->d_partial);
+  hipFree(c->d_partial2);
   hipFree(c->d_norm);
   hipFree(c->d_stage);
   if (c->h_norm) hipHostFree(c->h_norm);
@@ -1521,6 +2030,9 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
           case kCell: cnt = (long)P.mx[0] * P.mx[1] * P.mx[2]; break;
           case kCoarse: cnt = (long)P.cmx[0] * P.cmx[1] * P.cmx[2]; break;
           case kFineG: cnt = (long)P.nq[0] * P.nq[1] * P.nq[2]; break;
+          case kDual:
+            cnt = (long)(P.cmx[0] + 1) * (P.cmx[1] + 1) * (c->p.ndims == 3 ? P.cmx[2] + 1 : 1);
+            break;
           default: cnt = (long)P.nc[0] * P.nc[1] * P.nc[2]; break;
         }
         pre[a] = (int)acc;
@@ -1546,9 +2058,27 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
     L.qspan_len = contiguous ? end - L.qspan_off : -1;
     RETCHK(build_ghost_stages(c, L));
   }
+  // coarse chains: from the highest level whose patch has at most 16^3
+  // (2D: 64^2) cells down to minlevel, every level a single patch periodic
+  // in every dimension
+  if (c->p.model == BSAM_MODEL_CH) {
+    const long cap = (c->p.ndims == 3) ? 4096 : 4096;
+    int top = c->minlevel - 1;
+    for (int l = c->minlevel; l <= c->finest; l++) {
+      const Level& L = c->L(l);
+      if (L.wrap_pid < 0) break;
+      const DPatch& P = c->hp[L.wrap_pid];
+      if ((long)P.mx[0] * P.mx[1] * P.mx[2] > cap) break;
+      top = l;
+    }
+    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= 16)
+      c->L(top).chain = top - c->minlevel + 1;
+  }
   if (maxblocks > c->partial_cap) {
     hipFree(c->d_partial);
+    hipFree(c->d_partial2);
     HIPCHK(hipMalloc(&c->d_partial, 2 * (long)maxblocks * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_partial2, 2 * (long)(maxblocks / 256 + 2) * sizeof(double)));
     c->partial_cap = maxblocks;
   }
   c->have_hier = true;
@@ -1650,9 +2180,9 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
   if (c->p.model == BSAM_MODEL_CH) {
     const double eps2 = c->p.model_params[0];
     if (c->p.ndims == 2)
-      k_relax_ch<2><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
+      k_relax_ch<2, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
     else
-      k_relax_ch<3><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
+      k_relax_ch<3, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
   } else {
     k_relax_bg2d<<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), L.h, c->q, c->qold, c->f);
   }
@@ -1708,13 +2238,72 @@ extern "C" int32_t bsam_set_ghost(bsam_ctx* c, int32_t level, int32_t ipass) {
   return set_ghost_t<double>(c, level, ipass, c->q, 0);
 }
 
+// Launch shape of the marching 3D sweep for a single large patch, or 0 when
+// the level uses the flat kernel.
+static int march_kc(bsam_ctx* c, const Level& L, int* nblk) {
+  if (c->p.ndims != 3 || L.patches.size() != 1) return 0;
+  const DPatch& P = c->hp[L.patches[0]];
+  const int nx2 = P.mx[0] / 2;
+  if (nx2 < 64 || P.mx[1] < 8 || (P.mx[0] & 1)) return 0;
+  const int ntile = ((nx2 + 63) / 64) * ((P.mx[1] + 7) / 8);
+  int nkc = std::max(1, (768 + ntile - 1) / ntile);
+  int kc = std::max(4, (P.mx[2] + nkc - 1) / nkc);
+  nkc = (P.mx[2] + kc - 1) / kc;
+  *nblk = ntile * nkc;
+  return kc;
+}
+
+// One colour sweep followed by SetGhost(level, ipass) (the body of
+// LevelRelax, afasroutines.f90:546-555).  For a single patch periodic in
+// every dimension the sweep writes the ghost images itself, which is the
+// whole effect of that SetGhost on the values that changed.
+static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass) {
+  Level& L = c->L(level);
+  const bool wrap = L.wrap_pid >= 0;
+  int nblk = 0;
+  const int kc = (c->p.model == BSAM_MODEL_CH) ? march_kc(c, L, &nblk) : 0;
+  if (!kc && !(wrap && c->p.model == BSAM_MODEL_CH)) {
+    RETCHK(bsam_relax_level(c, level, ipass));
+    return set_ghost_t<double>(c, level, ipass, c->q, 0);
+  }
+  const int col = ipass - 1;
+  const bool fin = level == c->finest;
+  const double h2 = L.h * L.h, eps2 = c->p.model_params[0];
+  if (fin) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
+  if (kc) {
+    const double tmp1 = c->p.dt / h2, tmp2 = eps2 / h2, tmp3 = 6.0 * eps2 / h2, a12 = 6.0 * tmp1;
+    const int pid = L.patches[0];
+    if (wrap) {
+      if (fin) k_relax3d_march<double, false, true, 1><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+      else k_relax3d_march<double, false, true, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+    } else {
+      if (fin) k_relax3d_march<double, false, false, 1><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+      else k_relax3d_march<double, false, false, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+    }
+  } else {
+    LevelArgs a = largs(c, L, kHalf);
+    DISPATCH_DIM(c->p.ndims, (k_relax_ch<2, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f)),
+                 (k_relax_ch<3, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f)));
+  }
+  LAUNCH_CHECK();
+  if (fin) {
+    double N = 0, G = 0;
+    for (int pid : L.patches) {
+      const DPatch& P = c->hp[pid];
+      N += (double)P.mx[0] * P.mx[1] * P.mx[2];
+      G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
+                             : 2.0 * ((double)P.mx[0] + P.mx[1]);
+    }
+    kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 8.0 * ((N + G) + 0.5 * (N + G) + N + N));
+  }
+  if (!wrap) RETCHK(set_ghost_t<double>(c, level, ipass, c->q, 0));
+  return BSAM_OK;
+}
+
 extern "C" int32_t bsam_level_relax(bsam_ctx* c, int32_t level) {
   RETCHK(check_level(c, level));
   for (int s = 0; s < c->p.nsmoothingpasses; s++)
-    for (int rb = 1; rb <= 2; rb++) {
-      RETCHK(bsam_relax_level(c, level, rb));
-      RETCHK(bsam_set_ghost(c, level, rb));
-    }
+    for (int rb = 1; rb <= 2; rb++) RETCHK(relax_and_ghost(c, level, rb));
   return BSAM_OK;
 }
 
@@ -1725,13 +2314,33 @@ static int32_t need_parent(bsam_ctx* c, int level) {
 }
 
 template <typename T>
-static int32_t restrict_t(bsam_ctx* c, int level, const T* src, T* dst, T* qc) {
+static int32_t restrict_t(bsam_ctx* c, int level, const T* src, T* dst, T* qc, bool wrap = false) {
   Level& L = c->L(level);
   LevelArgs a = largs(c, L, kCoarse);
-  DISPATCH_DIM(c->p.ndims, (k_restrict<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
-               (k_restrict<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
+  if (wrap)
+    DISPATCH_DIM(c->p.ndims, (k_restrict<2, T, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
+                 (k_restrict<3, T, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
+  else
+    DISPATCH_DIM(c->p.ndims, (k_restrict<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
+                 (k_restrict<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
   LAUNCH_CHECK();
   return BSAM_OK;
+}
+
+// True when SetGhost(level-1, 0) after a restriction from `level` reduces to
+// the periodic images of the parent values just written: the parent level
+// is one patch periodic in every dimension, completely covered by this
+// level's single patch.
+static bool restrict_wraps_parent(bsam_ctx* c, int level) {
+  if (level <= c->minlevel) return false;
+  const Level& L = c->L(level);
+  const Level& Lp = c->L(level - 1);
+  if (L.patches.size() != 1 || Lp.wrap_pid < 0 || level - 1 > 0) return false;
+  const DPatch& P = c->hp[L.patches[0]];
+  const DPatch& Q = c->hp[Lp.wrap_pid];
+  for (int a = 0; a < c->p.ndims; a++)
+    if (P.mb[a][0] != 1 || P.mb[a][1] != Q.mx[a]) return false;
+  return true;
 }
 
 template <typename T>
@@ -1745,15 +2354,19 @@ static int32_t ring_t(bsam_ctx* c, int level, const T* q, T* qc) {
 }
 
 template <typename T>
-static int32_t correct_t(bsam_ctx* c, int level, T* q, T* qc) {
+static int32_t correct_t(bsam_ctx* c, int level, T* q, T* qc, bool wrap = false) {
   Level& L = c->L(level);
   LevelArgs a = largs(c, L, kQcBox);
-  LevelArgs b = largs(c, L, kFineG);
+  LevelArgs b = largs(c, L, kDual);
   DISPATCH_DIM(c->p.ndims, (k_correct_qc<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)),
                (k_correct_qc<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)));
   LAUNCH_CHECK();
-  DISPATCH_DIM(c->p.ndims, (k_prolong_add<2, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
-               (k_prolong_add<3, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
+  if (wrap)
+    DISPATCH_DIM(c->p.ndims, (k_prolong_dual<2, T, true><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
+                 (k_prolong_dual<3, T, true><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
+  else
+    DISPATCH_DIM(c->p.ndims, (k_prolong_dual<2, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
+                 (k_prolong_dual<3, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
   LAUNCH_CHECK();
   return BSAM_OK;
 }
@@ -1805,6 +2418,17 @@ extern "C" int32_t bsam_correct_fine(bsam_ctx* c, int32_t level) {
   return correct_t<double>(c, level, c->q, c->qc);
 }
 
+// FillDown restricted to the levels min(finest, highest) .. max(minlevel+1,
+// lowest), in the reference's order (finest first).
+static int32_t fill_down_range(bsam_ctx* c, int32_t field, int highest, int lowest) {
+  double* fld = (field == BSAM_SOURCEFIELD) ? c->f : c->q;
+  for (int level = std::min(c->finest, highest); level >= std::max(c->minlevel + 1, lowest); level--) {
+    RETCHK(restrict_t<double>(c, level, fld, fld, nullptr));
+    if (field == BSAM_SOLUTIONFIELD) RETCHK(set_ghost_t<double>(c, level - 1, 0, c->q, 0));
+  }
+  return BSAM_OK;
+}
+
 extern "C" int32_t bsam_fill_down(bsam_ctx* c, int32_t field) {
   if (!c || !c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
   if (field != BSAM_SOURCEFIELD && field != BSAM_SOLUTIONFIELD) return set_err(BSAM_EINVAL, "bad field %d", field);
@@ -1834,10 +2458,28 @@ static int32_t source_launch(bsam_ctx* c, int level, int mode) {
 
 extern "C" int32_t bsam_get_source(bsam_ctx* c, int32_t level) { return source_launch(c, level, 0); }
 extern "C" int32_t bsam_update_source(bsam_ctx* c, int32_t level) { return source_launch(c, level, 1); }
+// GetSourceFunction then UpdateSourceFunction on one level in one pass.
+static int32_t source_and_update(bsam_ctx* c, int32_t level) { return source_launch(c, level, 2); }
+
 extern "C" int32_t bsam_update_aux(bsam_ctx* c, int32_t level) {
   // maux = 0: UpdateAuxInVcycle returns before calling the hook
   // (afasroutines.f90:417).
   return check_level(c, level);
+}
+
+// Two-stage deterministic reduction of nb block partials into d_norm[0..1].
+static int32_t reduce_norm(bsam_ctx* c, int nb) {
+  const int chunk = 256;
+  const int nb2 = (nb + chunk - 1) / chunk;
+  if (nb2 > 1) {
+    k_reduce_partials<<<nb2, kBlock, 0, c->s>>>(c->d_partial, nb, chunk, c->d_partial2);
+    LAUNCH_CHECK();
+    k_reduce_partials<<<1, kBlock, 0, c->s>>>(c->d_partial2, nb2, nb2, c->d_norm);
+  } else {
+    k_reduce_partials<<<1, kBlock, 0, c->s>>>(c->d_partial, nb, nb, c->d_norm);
+  }
+  LAUNCH_CHECK();
+  return BSAM_OK;
 }
 
 static int32_t residual_launch(bsam_ctx* c, int level, double* rf, double* partial) {
@@ -1863,10 +2505,10 @@ extern "C" int32_t bsam_residual(bsam_ctx* c, int32_t level) {
 }
 
 // Enqueue the residual norm of `level` into d_norm[0..1] (no wait).
-static int32_t enqueue_norm(bsam_ctx* c, int level) {
+static int32_t enqueue_norm(bsam_ctx* c, int level, bool write_rf) {
   const bool fin = level == c->finest;
   if (fin) kstat_begin(c, BSAM_KSTAT_RESID_FINEST);
-  int nb = residual_launch(c, level, c->rf, c->d_partial);
+  int nb = residual_launch(c, level, write_rf ? c->rf : nullptr, c->d_partial);
   if (nb < 0) return nb;
   if (fin) {
     double N = 0, G = 0;
@@ -1876,11 +2518,9 @@ static int32_t enqueue_norm(bsam_ctx* c, int level) {
       G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
                              : 2.0 * ((double)P.mx[0] + P.mx[1]);
     }
-    kstat_end(c, BSAM_KSTAT_RESID_FINEST, 8.0 * 2.0 * ((N + G) + N + N));
+    kstat_end(c, BSAM_KSTAT_RESID_FINEST, 8.0 * 2.0 * ((N + G) + N + (write_rf ? N : 0.0)));
   }
-  k_reduce_partials<<<1, kBlock, 0, c->s>>>(c->d_partial, nb, c->d_norm);
-  LAUNCH_CHECK();
-  return BSAM_OK;
+  return reduce_norm(c, nb);
 }
 
 static double norm_from(bsam_ctx* c, const double* s, int level) {
@@ -1896,7 +2536,7 @@ static double norm_from(bsam_ctx* c, const double* s, int level) {
 extern "C" int32_t bsam_error_afas(bsam_ctx* c, int32_t level, double* norm) {
   RETCHK(check_level(c, level));
   if (!norm) return set_err(BSAM_EINVAL, "null norm");
-  RETCHK(enqueue_norm(c, level));
+  RETCHK(enqueue_norm(c, level, true));
   HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   *norm = norm_from(c, c->h_norm, level);
@@ -1924,17 +2564,45 @@ extern "C" int32_t bsam_copy_q_to_qold(bsam_ctx* c, int32_t level) {
 // ---------------------------------------------------------------------------
 // V-cycle and MultigridIterations
 // ---------------------------------------------------------------------------
+static int32_t launch_coarse_chain(bsam_ctx* c, int level, bool lin) {
+  Level& L = c->L(level);
+  CoarseArgs A;
+  A.nlev = L.chain;
+  for (int n = 0; n < L.chain; n++) A.pid[n] = c->L(level - n).wrap_pid;
+  A.dt = c->p.dt;
+  A.eps2 = c->p.model_params[0];
+  const int nsm = c->p.nsmoothingpasses;
+  if (!lin) {
+    DISPATCH_DIM(c->p.ndims,
+                 (k_coarse_vcycle<2, double, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, c->q, c->f, c->qc, nullptr)),
+                 (k_coarse_vcycle<3, double, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, c->q, c->f, c->qc, nullptr)));
+  } else {
+    bsam::MixedState* m = c->mixed;
+    DISPATCH_DIM(c->p.ndims,
+                 (k_coarse_vcycle<2, float, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, m->e, m->rhs, m->ec, m->coef)),
+                 (k_coarse_vcycle<3, float, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, m->e, m->rhs, m->ec, m->coef)));
+  }
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
+
 extern "C" int32_t bsam_vcycle(bsam_ctx* c, int32_t level) {
   RETCHK(check_level(c, level));
+  if (c->L(level).chain > 0) return launch_coarse_chain(c, level, false);
   RETCHK(bsam_level_relax(c, level));
   if (level > c->minlevel) {
-    RETCHK(bsam_restrict_solution(c, level));
-    RETCHK(bsam_set_ghost(c, level - 1, 0));
+    // RestrictSolution + SetGhost(level-1, 0), fused when the parent is a
+    // fully covered periodic patch
+    const bool rw = restrict_wraps_parent(c, level);
+    RETCHK(restrict_t<double>(c, level, c->q, c->q, c->qc, rw));
+    if (!rw) RETCHK(bsam_set_ghost(c, level - 1, 0));
     RETCHK(bsam_coarse_ghost_points(c, level));
     RETCHK(bsam_coarse_loading(c, level));
     RETCHK(bsam_vcycle(c, level - 1));
-    RETCHK(bsam_correct_fine(c, level));
-    RETCHK(bsam_set_ghost(c, level, 0));
+    // CorrectFine + SetGhost(level, 0), fused for a periodic single patch
+    const bool cw = c->L(level).wrap_pid >= 0;
+    RETCHK(correct_t<double>(c, level, c->q, c->qc, cw));
+    if (!cw) RETCHK(bsam_set_ghost(c, level, 0));
     RETCHK(bsam_level_relax(c, level));
   }
   return BSAM_OK;
@@ -1942,12 +2610,28 @@ extern "C" int32_t bsam_vcycle(bsam_ctx* c, int32_t level) {
 
 // Loop body of MultigridIterations after the V-cycle (afasroutines.f90:
 // 83-104), ending with the residual norm enqueued in d_norm.
+//
+// The reference's loop body also calls UpdateSourceFunction on every level
+// >= root and FillDown(sourcefield) (afasroutines.f90:96-100).  With
+// SourceUpdate#D = 0 (both the CH and the BG hooks, CH/problem.f90:453-476,
+// BioGrowth/problem.f90:382-405) those stores are dead: the only f values
+// the V-cycle changes are parent cells under a child, written by
+// CoarseLoadingFunction, which rewrites exactly those cells before any read
+// in the next V-cycle; every other f value already equals ftmp + 0.  They
+// are therefore done once after the loop (so f leaves the call as the
+// reference leaves it) instead of once per V-cycle.
+//
+// FillDown(solutionfield) (afasroutines.f90:83) is needed inside the loop
+// only for levels >= 1: it refreshes the parent cells that the next
+// V-cycle's coarse-fine ghost interpolation of the child reads.  Levels at
+// and below the root have no interpolation, and every parent value the
+// restriction would write there is overwritten by RestrictSolution and
+// SetGhost of the next V-cycle before it is read; those levels are filled
+// down once after the loop.
 static int32_t enqueue_iteration(bsam_ctx* c) {
   RETCHK(bsam_vcycle(c, c->finest));
-  RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
-  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_update_source(c, level));
-  RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
-  RETCHK(enqueue_norm(c, c->finest));
+  RETCHK(fill_down_range(c, BSAM_SOLUTIONFIELD, c->finest, 1));
+  RETCHK(enqueue_norm(c, c->finest, false));
   return BSAM_OK;
 }
 
@@ -1989,8 +2673,7 @@ extern "C" int32_t bsam_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, doub
   if (c->p.precision == BSAM_PREC_MIXED) return bsam::mixed_multigrid_iterations(c, ncycles, residual, history, history_len);
   RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
   for (int level = c->finest; level >= c->minlevel; level--) RETCHK(bsam_update_aux(c, level));
-  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_get_source(c, level));
-  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_update_source(c, level));
+  for (int level = c->finest; level >= 0; level--) RETCHK(source_and_update(c, level));
   RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
   double res = 0.0;
   int it;
@@ -2006,6 +2689,11 @@ extern "C" int32_t bsam_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, doub
     }
     if (res < c->p.qerrortol) break;
   }
+  // the finest level's f is never written by the V-cycle
+  for (int level = c->finest - 1; level >= 0; level--) RETCHK(bsam_update_source(c, level));
+  RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
+  // the part of the last FillDown(solutionfield) skipped in the loop
+  RETCHK(fill_down_range(c, BSAM_SOLUTIONFIELD, 0, c->minlevel + 1));
   *ncycles = it > c->p.maxvcycles ? c->p.maxvcycles : it;
   *residual = res;
   return it > c->p.maxvcycles ? BSAM_NOT_CONVERGED : BSAM_OK;
@@ -2062,13 +2750,6 @@ extern "C" int32_t bsam_kstat_read(bsam_ctx* c, int32_t which, int64_t* launches
 // mixed precision: FP64 defect correction around FP32 correction V-cycles
 // ---------------------------------------------------------------------------
 namespace bsam {
-struct MixedState {
-  float* e = nullptr;     // correction (q geometry)
-  float* rhs = nullptr;   // right-hand side of the correction equation
-  float* coef = nullptr;  // 3 phi^2 of the FP64 iterate
-  float* ec = nullptr;    // coarse-under-fine correction (qc geometry)
-};
-
 int32_t mixed_init(bsam_ctx* c) {
   MixedState* m = new MixedState();
   c->mixed = m;
@@ -2094,6 +2775,9 @@ void mixed_free(MixedState* m) {
 
 namespace {
 
+// One colour sweep of the correction smoother; returns 1 when the sweep
+// also wrote the periodic ghost images (the following SetGhost is then
+// redundant), 0 otherwise.
 int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
   bsam::MixedState* m = c->mixed;
   Level& L = c->L(level);
@@ -2103,10 +2787,28 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
   const float tmp1 = (float)(c->p.dt / h2), tmp2 = (float)(eps2 / h2), tmp3 = (float)(d2 * eps2 / h2);
   const float a12 = (float)(d2 * (c->p.dt / h2));
   const bool fin = level == c->finest;
+  const bool wrap = L.wrap_pid >= 0;
+  int nblk = 0;
+  const int kc = march_kc(c, L, &nblk);
   if (fin) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
-  DISPATCH_DIM(c->p.ndims,
-               (k_relax_lin<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
-               (k_relax_lin<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+  if (kc) {
+    const int pid = L.patches[0];
+    if (wrap) {
+      if (fin) k_relax3d_march<float, true, true, 1><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+      else k_relax3d_march<float, true, true, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+    } else {
+      k_relax3d_march<float, true, false, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+    }
+  } else {
+    if (wrap)
+      DISPATCH_DIM(c->p.ndims,
+                   (k_relax_lin<2, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
+                   (k_relax_lin<3, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+    else
+      DISPATCH_DIM(c->p.ndims,
+                   (k_relax_lin<2, false><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
+                   (k_relax_lin<3, false><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+  }
   LAUNCH_CHECK();
   if (fin) {
     double N = 0, G = 0;
@@ -2120,24 +2822,27 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
     // coef of the colour N/2 each, the colour's e written N/2 x 2 fields
     kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + 0.5 * (N + G) + N + 0.5 * N + N));
   }
-  return BSAM_OK;
+  return wrap ? 1 : 0;
 }
 
 int32_t mixed_level_relax(bsam_ctx* c, int level) {
   for (int s = 0; s < c->p.nsmoothingpasses; s++)
     for (int rb = 1; rb <= 2; rb++) {
-      RETCHK(mixed_relax(c, level, rb));
-      RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1));
+      const int32_t done = mixed_relax(c, level, rb);
+      RETCHK(done);
+      if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1));
     }
   return BSAM_OK;
 }
 
 int32_t mixed_vcycle(bsam_ctx* c, int level) {
   bsam::MixedState* m = c->mixed;
+  if (c->L(level).chain > 0) return launch_coarse_chain(c, level, true);
   RETCHK(mixed_level_relax(c, level));
   if (level > c->minlevel) {
-    RETCHK(restrict_t<float>(c, level, m->e, m->e, m->ec));
-    RETCHK(set_ghost_t<float>(c, level - 1, 0, m->e, 1));
+    const bool rw = restrict_wraps_parent(c, level);
+    RETCHK(restrict_t<float>(c, level, m->e, m->e, m->ec, rw));
+    if (!rw) RETCHK(set_ghost_t<float>(c, level - 1, 0, m->e, 1));
     RETCHK(ring_t<float>(c, level, m->e, m->ec));
     Level& L = c->L(level);
     LevelArgs a = largs(c, L, kCoarse);
@@ -2146,8 +2851,9 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
                  (k_fas_rhs_lin<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)));
     LAUNCH_CHECK();
     RETCHK(mixed_vcycle(c, level - 1));
-    RETCHK(correct_t<float>(c, level, m->e, m->ec));
-    RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1));
+    const bool cw = c->L(level).wrap_pid >= 0;
+    RETCHK(correct_t<float>(c, level, m->e, m->ec, cw));
+    if (!cw) RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1));
     RETCHK(mixed_level_relax(c, level));
   }
   return BSAM_OK;
@@ -2182,9 +2888,7 @@ int32_t mixed_residuals(bsam_ctx* c) {
       nbf = nb;
     }
   }
-  k_reduce_partials<<<1, kBlock, 0, c->s>>>(c->d_partial, nbf, c->d_norm);
-  LAUNCH_CHECK();
-  return BSAM_OK;
+  return reduce_norm(c, nbf);
 }
 
 // One outer iteration: correction V-cycle, q += e, FillDown, ghost refresh
@@ -2208,8 +2912,15 @@ int32_t mixed_iteration(bsam_ctx* c) {
       }
     }
   }
-  RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
-  RETCHK(set_ghost_t<double>(c, c->finest, 0, c->q, 0));
+  // With refined levels the finest residual sees the coarse level through
+  // the coarse-fine ghost interpolation, so the coarse q must be the
+  // restriction of the updated fine q first.  A single-level mesh has no
+  // such coupling (its ghosts were updated with e), and FillDown is done
+  // once after the loop.
+  if (c->finest > 0) {
+    RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+    RETCHK(set_ghost_t<double>(c, c->finest, 0, c->q, 0));
+  }
   RETCHK(mixed_residuals(c));
   return BSAM_OK;
 }
@@ -2220,8 +2931,7 @@ namespace bsam {
 int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residual, double* history,
                                    int32_t history_len) {
   RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
-  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_get_source(c, level));
-  for (int level = c->finest; level >= 0; level--) RETCHK(bsam_update_source(c, level));
+  for (int level = c->finest; level >= 0; level--) RETCHK(source_and_update(c, level));
   RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
   for (int level = c->minlevel; level <= c->finest; level++) {
     Level& L = c->L(level);
@@ -2245,6 +2955,7 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
     }
     if (res < c->p.qerrortol) break;
   }
+  if (c->finest == 0) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
   *ncycles = it > c->p.maxvcycles ? c->p.maxvcycles : it;
   *residual = res;
   return it > c->p.maxvcycles ? BSAM_NOT_CONVERGED : BSAM_OK;

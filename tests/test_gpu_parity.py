@@ -2,8 +2,11 @@
 
 Every kernel except the residual-norm reduction keeps the reference's
 floating-point operation order (compiled with -ffp-contract=off), so states
-must agree bit for bit; residual norms are sums in a different order and
-must agree to 1e-12 relative.
+must agree bit for bit.  Residual norms are sums of n = nrvars x cells
+squares taken in a different order (the reference sums sequentially, whose
+rounding error is bounded by n*u, u = 1.1e-16); they must agree to 1e-10
+relative, which covers n up to ~1e6 with margin and is far below the 1e-8
+stopping tolerance.
 """
 import numpy as np
 import pytest
@@ -13,7 +16,7 @@ import oracle
 from bsam_amd import abi, configs, hierarchy
 
 pytestmark = pytest.mark.gpu
-RTOL_NORM = 1e-12
+RTOL_NORM = 1e-10
 
 
 def _hist_close(a, b):
@@ -136,3 +139,33 @@ def test_mixed_precision_matches_fp64(nd, n, steps, gpu_solver_cls):
     print(f"[gpu] mixed max-norm diff {err:.3e}, cycles fp64 {ca} mixed {cb}", flush=True)
     assert err < 1e-7
     assert max(cb) <= max(ca) + 2
+
+
+@pytest.mark.parametrize("mthbc", [(2, 2, 2, 2, 2, 2), (2, 2, 2, 2, 10, 10)])
+def test_large_patch_sweep_vs_oracle(mthbc, gpu_solver_cls):
+    """128^3 root: the finest level uses the marching 3D sweep (with the
+    periodic ghost images written by the sweep when the patch is periodic in
+    every direction).  One time step, bit-identical to the oracle."""
+    nd, n = 3, 128
+    print(f"[gpu] 128^3 sweep mthbc={mthbc}", flush=True)
+    minlevel = -(int(np.log2(n)) - 1)
+    hier = hierarchy.uniform(nd, (n,) * nd, minlevel, list(mthbc))
+    root = len(hier) - 1
+    phi = configs.init_phi(hier[root], nd, 1, 1003, 0.1, 0.05)
+    q0 = np.zeros((2,) + (n + 2,) * nd)
+    q0[0, 1:-1, 1:-1, 1:-1] = phi
+    res = []
+    for S in (gpu_solver_cls(configs.params(nd, 5e-5, 0.015, 1e-8), hier),
+              oracle.Oracle(configs.params(nd, 5e-5, 0.015, 1e-8), hier)):
+        S.put(root, "q", q0)
+        S.set_ghost(0, 0)
+        S.copy_q_to_qold(0)
+        S.set_ghost(0, 0)
+        S.fill_down(abi.SOLUTIONFIELD)
+        for lv in range(minlevel, 1):
+            S.copy_q_to_qold(lv)
+        rc, ncyc, r, h = S.multigrid_iterations()
+        res.append((h, S.get(root, "q")))
+    (hg, qg), (ho, qo) = res
+    assert np.array_equal(qg, qo), float(np.max(np.abs(qg - qo)))
+    _hist_close(hg, ho)
