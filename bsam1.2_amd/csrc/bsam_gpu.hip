@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double 
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(512) void k_relax3d_march(const DPatch* __restrict_
                                                       const T* __restrict__ f, const T* __restrict__ coef) {
   constexpr int TX = 64, TY = 8, NR = TY + 2, NP = TX + 1;
   __shared__ T sh[4][2][NR][NP];
-  const DPatch& P = pt[pid];
+  const DPatch P = pt[pid];
   const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
   const int nx2 = mx >> 1;
   const int ntx = (nx2 + TX - 1) / TX, nty = (my + TY - 1) / TY;
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, Model
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
   const int g = lb * kBlock + threadIdx.x;
   if (g < L.total) {
     const int pi = find_patch(L.prefix, L.npatch, g);
-    const DPatch& P = L.pt[L.patches[pi]];
+    const DPatch P = L.pt[L.patches[pi]];
     const int local = g - L.prefix[pi];
     const int i = local % P.mx[0] + 1;
     const int row = local / P.mx[0];
@@ -440,6 +440,250 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
   if (threadIdx.x == 0) {
     double a = 0.0, b = 0.0;
     for (int x = 0; x < kBlock / 64; x++) {
+      a += red[0][x];
+      b += red[1][x];
+    }
+    partial[2 * lb] = a;
+    partial[2 * lb + 1] = b;
+  }
+}
+
+// Residual of one large 3D CH patch, marching in k.  A block owns 64 cell
+// pairs (iA, iB) = (2t+1, 2t+2) x 8 rows x KC planes; a thread keeps its
+// pair's phi and mu on planes k-1, k, k+1 in registers (the z neighbours),
+// and plane k of the block's rows plus a halo row and cell on each side sits
+// in LDS for the x and y neighbours.  The loads of plane k+2 (own pair), of
+// f on plane k+1 and of the halo of plane k+1 are issued before the barrier
+// of plane k, so their latency overlaps a plane of arithmetic.  The
+// expressions are those of op_q / k_residual, so the residual is bit-identical.
+//   MODE 0: r = f - L(q) into `out` (rf, or the FP32 right-hand side of the
+//           mixed path; may be null) and squared sums into `partial`.
+//   MODE 1: k_fas_rhs (CoarseLoadingFunction): the odd row hands its values
+//           of planes 2kc-1 and 2kc to the even row through LDS, which sums
+//           the eight children of coarse cell (t+1, j/2, kc) in k_fas_rhs's
+//           order and adds the coarse operator on qc into the parent's f.
+//           KC is even, so a block holds whole coarse planes.
+template <int MODE, typename TO>
+__global__ __launch_bounds__(512) void k_ch3d_resid_march(const DPatch* __restrict__ pt, int pid, int KC,
+                                                         double tmp1, double tmp2, ModelK m,
+                                                         const double* __restrict__ q,
+                                                         const double* __restrict__ f, TO* __restrict__ out,
+                                                         double* __restrict__ partial,
+                                                         const double* __restrict__ qc,
+                                                         double* __restrict__ fpar) {
+  constexpr int TX = 64, TY = 8, NR = TY + 2, NC = 2 * TX + 2;
+  // halo entries of one plane: rows 0 and NR-1 (NC cells each), cells 0 and
+  // NC-1 of rows 1..TY; two fields
+  constexpr int NH = 2 * (2 * NC + 2 * TY);
+  constexpr int HPER = (NH + 511) / 512;
+  __shared__ double sh[2][2][NR][NC];
+  __shared__ double xb[MODE == 1 ? TY / 2 : 1][8][TX];
+  __shared__ double red[2][8];
+  const DPatch P = pt[pid];
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  const int nx2 = mx >> 1;
+  const int ntx = (nx2 + TX - 1) / TX, nty = (my + TY - 1) / TY;
+  const int lb = xcd_block();
+  int rem = lb;
+  const int bx = rem % ntx;
+  rem /= ntx;
+  const int by = rem % nty;
+  const int bk = rem / nty;
+  const int t0 = bx * TX, j0 = by * TY;
+  const int k0 = bk * KC + 1, k1 = min(mz, k0 + KC - 1);
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int hq = P.hq, ny = P.nq[1], nz = P.nq[2], vs = P.qvs, base = P.qoff;
+  auto cell = [&](int i, int jj, int k) { return base + (((((i + jj + k) & 1) * nz + k) * ny + jj) * hq + (i >> 1)); };
+  const int t = t0 + tx, j = j0 + 1 + ty;
+  const bool active = (t < nx2) && (j <= my);
+  const int iA = 2 * t + 1, iB = 2 * t + 2;
+  // the pair is stored (interior or ghost) for these threads; the others of
+  // a partial tile hold zeros, which no active thread reads
+  const bool inA = iA <= mx + 1 && j <= my + 1, inB = iB <= mx + 1 && j <= my + 1;
+  auto load_pair = [&](int k, double (&a)[2], double (&b)[2]) {
+    const int oa = cell(iA, j, k), ob = cell(iB, j, k);
+    a[0] = inA ? q[oa] : 0.0;
+    a[1] = inA ? q[oa + vs] : 0.0;
+    b[0] = inB ? q[ob] : 0.0;
+    b[1] = inB ? q[ob + vs] : 0.0;
+  };
+  auto load_f = [&](int k, double (&fv)[4]) {
+    if (active) {
+      const int fa = cell(iA, j, k), fb = cell(iB, j, k);
+      fv[0] = f[fa]; fv[1] = f[fa + vs];
+      fv[2] = f[fb]; fv[3] = f[fb + vs];
+    } else {
+      fv[0] = fv[1] = fv[2] = fv[3] = 0.0;
+    }
+  };
+  auto halo_pos = [&](int e, int& v, int& r, int& cx) {
+    v = e / (NH / 2);
+    const int h = e % (NH / 2);
+    if (h < 2 * NC) {
+      r = (h < NC) ? 0 : NR - 1;
+      cx = h % NC;
+    } else {
+      const int h2 = h - 2 * NC;
+      r = 1 + (h2 >> 1);
+      cx = (h2 & 1) ? NC - 1 : 0;
+    }
+  };
+  auto load_halo = [&](int k, double (&hv)[HPER]) {
+#pragma unroll
+    for (int n = 0; n < HPER; n++) {
+      const int e = threadIdx.x + 512 * n;
+      double val = 0.0;
+      if (e < NH) {
+        int v, r, cx;
+        halo_pos(e, v, r, cx);
+        const int ii = 2 * t0 + cx, jj = j0 + r;
+        if (ii <= mx + 1 && jj <= my + 1) val = q[cell(ii, jj, k) + v * vs];
+      }
+      hv[n] = val;
+    }
+  };
+  double A[3][2], B[3][2], An[2], Bn[2], F[4], Fn[4], H[HPER], Hn[HPER];
+  load_pair(k0 - 1, A[1], B[1]);
+  load_pair(k0, A[2], B[2]);
+  load_pair(k0 + 1, An, Bn);
+  load_f(k0, Fn);
+  load_halo(k0, Hn);
+  double s0 = 0.0, s1 = 0.0;
+  // MODE 1 state: residuals of the odd plane, and the coarse cell of the
+  // previous even plane, completed after the next barrier
+  double ro[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, re[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  bool pending = false;
+  int pkc = 0;
+  const bool evenrow = (j & 1) == 0;
+  auto finish = [&]() {
+    const int rr = ty >> 1;
+    double sv[2];
+    for (int v = 0; v < 2; v++) {
+      double a = re[v][1];            // (i2, j2, k2)
+      a = a + re[v][0];               // (i1, j2, k2)
+      a = a + xb[rr][4 * v + 1][tx];  // (i2, j1, k2)
+      a = a + xb[rr][4 * v + 0][tx];  // (i1, j1, k2)
+      a = a + ro[v][1];               // (i2, j2, k1)
+      a = a + ro[v][0];               // (i1, j2, k1)
+      a = a + xb[rr][4 * v + 3][tx];  // (i2, j1, k1)
+      a = a + xb[rr][4 * v + 2][tx];  // (i1, j1, k1)
+      sv[v] = 0.125 * a;
+    }
+    const DPatch& Q = pt[P.parent];
+    const int ic = t + 1, jc = j >> 1, kc = pkc;
+    const double lmu = ulap_c<3>(qc, P, ic, jc, kc, P.cvs);
+    const double lphi = ulap_c<3>(qc, P, ic, jc, kc, 0);
+    const int oc = cidx(P, ic, jc, kc);
+    const int ip = P.mb[0][0] + ic - 1, jp = P.mb[1][0] + jc - 1, kp = P.mb[2][0] + kc - 1;
+    double o1, o2;
+    op_from<3, BSAM_MODEL_CH>(qc[oc], qc[oc + P.cvs], lphi, lmu, 0.0, m, Q.dx, o1, o2);
+    const int opar = qidx(Q, ip, jp, kp);
+    fpar[opar] = sv[0] + o1;
+    fpar[opar + Q.qvs] = sv[1] + o2;
+  };
+  for (int k = k0; k <= k1; k++) {
+    const int sl = k & 1;
+    // window: plane k-1 -> [0], k -> [1], k+1 -> [2]
+    for (int v = 0; v < 2; v++) {
+      A[0][v] = A[1][v]; A[1][v] = A[2][v]; A[2][v] = An[v];
+      B[0][v] = B[1][v]; B[1][v] = B[2][v]; B[2][v] = Bn[v];
+    }
+    for (int n = 0; n < 4; n++) F[n] = Fn[n];
+    for (int n = 0; n < HPER; n++) H[n] = Hn[n];
+    for (int v = 0; v < 2; v++) {
+      sh[sl][v][ty + 1][2 * tx + 1] = A[1][v];
+      sh[sl][v][ty + 1][2 * tx + 2] = B[1][v];
+    }
+#pragma unroll
+    for (int n = 0; n < HPER; n++) {
+      const int e = threadIdx.x + 512 * n;
+      if (e < NH) {
+        int v, r, cx;
+        halo_pos(e, v, r, cx);
+        sh[sl][v][r][cx] = H[n];
+      }
+    }
+    if (k < k1) {
+      load_pair(k + 2, An, Bn);
+      load_f(k + 1, Fn);
+      load_halo(k + 1, Hn);
+    }
+    __syncthreads();
+    if (MODE == 1 && pending) {
+      finish();
+      pending = false;
+    }
+    if (active) {
+      const int r = ty + 1, cA = 2 * tx + 1, cB = 2 * tx + 2;
+      double lapA[2], lapB[2];
+      for (int v = 0; v < 2; v++) {
+        double sA = B[1][v] + sh[sl][v][r][cA - 1] + sh[sl][v][r + 1][cA] + sh[sl][v][r - 1][cA];
+        sA = sA + A[2][v];
+        sA = sA + A[0][v];
+        lapA[v] = sA - 6.0 * A[1][v];
+        double sB = sh[sl][v][r][cB + 1] + A[1][v] + sh[sl][v][r + 1][cB] + sh[sl][v][r - 1][cB];
+        sB = sB + B[2][v];
+        sB = sB + B[0][v];
+        lapB[v] = sB - 6.0 * B[1][v];
+      }
+      // CH Operator3D: o1 = phi - tmp1*lap(mu), o2 = mu - phi^3 + tmp2*lap(phi)
+      const double pa = A[1][0], pb = B[1][0];
+      double ra[2], rb[2];
+      ra[0] = F[0] - (pa - tmp1 * lapA[1]);
+      ra[1] = F[1] - (A[1][1] - (pa * pa * pa) + tmp2 * lapA[0]);
+      rb[0] = F[2] - (pb - tmp1 * lapB[1]);
+      rb[1] = F[3] - (B[1][1] - (pb * pb * pb) + tmp2 * lapB[0]);
+      if (MODE == 0) {
+        s0 += ra[0] * ra[0] + rb[0] * rb[0];
+        s1 += ra[1] * ra[1] + rb[1] * rb[1];
+        if (out) {
+          const int oa = cell(iA, j, k), ob = cell(iB, j, k);
+          out[oa] = (TO)ra[0]; out[oa + vs] = (TO)ra[1];
+          out[ob] = (TO)rb[0]; out[ob + vs] = (TO)rb[1];
+        }
+      } else if (k & 1) {
+        for (int v = 0; v < 2; v++) {
+          ro[v][0] = ra[v];
+          ro[v][1] = rb[v];
+        }
+      } else if (!evenrow) {
+        // odd row: hand this pair's values to the even row after it
+        const int rr = ty >> 1;
+        for (int v = 0; v < 2; v++) {
+          xb[rr][4 * v + 0][tx] = ra[v];
+          xb[rr][4 * v + 1][tx] = rb[v];
+          xb[rr][4 * v + 2][tx] = ro[v][0];
+          xb[rr][4 * v + 3][tx] = ro[v][1];
+        }
+      } else {
+        for (int v = 0; v < 2; v++) {
+          re[v][0] = ra[v];
+          re[v][1] = rb[v];
+        }
+        pending = true;
+        pkc = k >> 1;
+      }
+    }
+  }
+  if (MODE == 1) {
+    __syncthreads();
+    if (pending) finish();
+    return;
+  }
+  if (!partial) return;
+  for (int d = 32; d > 0; d >>= 1) {
+    s0 += __shfl_down(s0, d, 64);
+    s1 += __shfl_down(s1, d, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = s0;
+    red[1][w] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int x = 0; x < 8; x++) {
       a += red[0][x];
       b += red[1][x];
     }
@@ -504,7 +748,7 @@ __device__ __forceinline__ bool decode_coarse(const LevelArgs& L, int g, CoarseC
   if (g >= L.total) return false;
   const int pi = find_patch(L.prefix, L.npatch, g);
   cc.pid = L.patches[pi];
-  const DPatch& P = L.pt[cc.pid];
+  const DPatch P = L.pt[cc.pid];
   const int local = g - L.prefix[pi];
   cc.ic = local % P.cmx[0] + 1;
   const int row = local / P.cmx[0];
@@ -525,8 +769,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const 
                                                      T* __restrict__ dst_parent, T* __restrict__ qc) {
   CoarseCell c;
   if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
-  const DPatch& P = L.pt[c.pid];
-  const DPatch& Q = L.pt[P.parent];
+  const DPatch P = L.pt[c.pid];
+  const DPatch Q = L.pt[P.parent];
   const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
   const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
   T rv[2] = {(T)0, (T)0};
@@ -549,8 +793,8 @@ __global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, con
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
-  const DPatch& Q = L.pt[P.parent];
+  const DPatch P = L.pt[L.patches[pi]];
+  const DPatch Q = L.pt[P.parent];
   const int local = g - L.prefix[pi];
   const int ic = local % P.nc[0];
   const int row = local / P.nc[0];
@@ -576,8 +820,8 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
                                                     double* __restrict__ fpar) {
   CoarseCell c;
   if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
-  const DPatch& P = L.pt[c.pid];
-  const DPatch& Q = L.pt[P.parent];
+  const DPatch P = L.pt[c.pid];
+  const DPatch Q = L.pt[P.parent];
   const double h = P.dx;
   double r0 = 0.0, r1 = 0.0;
   const int i2 = 2 * c.ic, j2 = 2 * c.jc, k2 = 2 * c.kc;
@@ -624,8 +868,8 @@ __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, cons
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
-  const DPatch& Q = L.pt[P.parent];
+  const DPatch P = L.pt[L.patches[pi]];
+  const DPatch Q = L.pt[P.parent];
   const int local = g - L.prefix[pi];
   const int ic = local % P.nc[0];
   const int row = local / P.nc[0];
@@ -657,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T*
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int n0 = P.cmx[0] + 1, n1 = P.cmx[1] + 1;
   const int ic = local % n0;
@@ -675,39 +919,66 @@ __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T*
 #pragma unroll
         for (int e = 0; e < 2; e++)
           a[v][d][b][e] = (DIM == 3 || d == 0) ? qc[cidx(P, ic + e, jc + b, kc + d) + v * P.cvs] : (T)0;
-  const int nk = (DIM == 3) ? 2 : 1;
-  for (int dk = 0; dk < nk; dk++)
+  constexpr int ncell = (DIM == 3) ? 8 : 4;
+  auto fine = [&](int n, int& i, int& j, int& k) {
+    i = 2 * ic + (n & 1);
+    j = 2 * jc + ((n >> 1) & 1);
+    k = (DIM == 3) ? 2 * kc + (n >> 2) : 0;
+  };
+  auto is_ghost = [&](int i, int j, int k) {
+    return i == 0 || i == P.mx[0] + 1 || j == 0 || j == P.mx[1] + 1 || (DIM == 3 && (k == 0 || k == P.mx[2] + 1));
+  };
+  // all loads of the 2^d fine cells first (ghost cells are storage too), so
+  // they are in flight together
+  T old[ncell][2];
 #pragma unroll
-    for (int db = 0; db < 2; db++)
+  for (int n = 0; n < ncell; n++) {
+    int i, j, k;
+    fine(n, i, j, k);
+    const int o = qidx(P, i, j, k);
+    old[n][0] = q[o];
+    old[n][1] = q[o + P.qvs];
+  }
 #pragma unroll
-      for (int da = 0; da < 2; da++) {
-        const int i = 2 * ic + da, j = 2 * jc + db, k = (DIM == 3) ? 2 * kc + dk : 0;
-        const bool ghost = i == 0 || i == P.mx[0] + 1 || j == 0 || j == P.mx[1] + 1 ||
-                           (DIM == 3 && (k == 0 || k == P.mx[2] + 1));
-        if (WRAP && ghost) continue;
-        T nv2[2];
+  for (int n = 0; n < ncell; n++) {
+    int i, j, k;
+    fine(n, i, j, k);
+    if (WRAP && is_ghost(i, j, k)) continue;
 #pragma unroll
-        for (int v = 0; v < 2; v++) {
-          T pr;
-          if (DIM == 2) {
-            const T b0 = linw<T>(a[v][0][0][0], a[v][0][0][1], i), b1 = linw<T>(a[v][0][1][0], a[v][0][1][1], i);
-            pr = linw<T>(b0, b1, j) / (T)16;
-          } else {
-            T cc2[2];
+    for (int v = 0; v < 2; v++) {
+      T pr;
+      if (DIM == 2) {
+        const T b0 = linw<T>(a[v][0][0][0], a[v][0][0][1], i), b1 = linw<T>(a[v][0][1][0], a[v][0][1][1], i);
+        pr = linw<T>(b0, b1, j) / (T)16;
+      } else {
+        T cc2[2];
 #pragma unroll
-            for (int d = 0; d < 2; d++) {
-              const T b0 = linw<T>(a[v][d][0][0], a[v][d][0][1], i);
-              const T b1 = linw<T>(a[v][d][1][0], a[v][d][1][1], i);
-              cc2[d] = linw<T>(b0, b1, j);
-            }
-            pr = linw<T>(cc2[0], cc2[1], k) / (T)64;
-          }
-          const int o = qidx(P, i, j, k) + v * P.qvs;
-          nv2[v] = q[o] + pr;
-          q[o] = nv2[v];
+        for (int d = 0; d < 2; d++) {
+          const T b0 = linw<T>(a[v][d][0][0], a[v][d][0][1], i);
+          const T b1 = linw<T>(a[v][d][1][0], a[v][d][1][1], i);
+          cc2[d] = linw<T>(b0, b1, j);
         }
-        if (WRAP) wrap_images<DIM, T>(q, P, i, j, k, nv2[0], nv2[1]);
+        pr = linw<T>(cc2[0], cc2[1], k) / (T)64;
       }
+      q[qidx(P, i, j, k) + v * P.qvs] = old[n][v] + pr;
+    }
+  }
+  if (WRAP) {
+    // dual cells on the patch boundary also write the periodic images of
+    // their interior fine cells, read back from this thread's own stores
+    const bool edge = ic == 0 || ic == P.cmx[0] || jc == 0 || jc == P.cmx[1] ||
+                      (DIM == 3 && (kc == 0 || kc == P.cmx[2]));
+    if (edge) {
+#pragma unroll 1
+      for (int n = 0; n < ncell; n++) {
+        int i, j, k;
+        fine(n, i, j, k);
+        if (is_ghost(i, j, k)) continue;
+        const int o = qidx(P, i, j, k);
+        wrap_images<DIM, T>(q, P, i, j, k, q[o], q[o + P.qvs]);
+      }
+    }
+  }
 }
 
 // CorrectFine, second half: q(0:mx+1) += P1(qc).  Domain: (mx+2)^d.
@@ -717,7 +988,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* 
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int i = local % P.nq[0];
   const int row = local / P.nq[0];
@@ -757,7 +1028,7 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int i = local % P.mx[0] + 1;
   const int row = local / P.mx[0];
@@ -859,8 +1130,8 @@ template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt,
                                                    const int* __restrict__ patches, int nv, int ipass,
                                                    T* __restrict__ q) {
-  const DPatch& P = pt[patches[blockIdx.x]];
-  const DPatch& Q = pt[P.parent];
+  const DPatch P = pt[patches[blockIdx.x]];
+  const DPatch Q = pt[P.parent];
   const int* mth = P.mthbc;
   int fi[3][5];
   if (DIM == 2) {
@@ -1043,7 +1314,7 @@ __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_wrap_full(const DPatch* __restrict__ pt, int pid, int nv,
                                                       T* __restrict__ q) {
-  const DPatch& P = pt[pid];
+  const DPatch P = pt[pid];
   const int nx = P.nq[0], ny = P.nq[1], nz = P.nq[2];
   const int g = xcd_block() * kBlock + threadIdx.x;
   // shell enumeration: x planes (2*ny*nz), then y planes (2*(nx-2)*nz), then
@@ -1096,11 +1367,14 @@ __global__ __launch_bounds__(kBlock) void k_zero_f(float* __restrict__ p, long n
   if (g < n) p[g] = 0.0f;
 }
 
-// q(FP64) += e(FP32) over a contiguous span (ghost cells included).
-__global__ __launch_bounds__(kBlock) void k_add_corr(double* __restrict__ q, const float* __restrict__ e,
-                                                     long n) {
+// q(FP64) += e(FP32) over a contiguous span (ghost cells included), and
+// e = 0 there for the next correction cycle.
+__global__ __launch_bounds__(kBlock) void k_add_corr(double* __restrict__ q, float* __restrict__ e, long n) {
   long g = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (g < n) q[g] = q[g] + (double)e[g];
+  if (g < n) {
+    q[g] = q[g] + (double)e[g];
+    e[g] = 0.0f;
+  }
 }
 
 // c = 3 phi^2 over the interior of every patch of a level.
@@ -1110,7 +1384,7 @@ __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __re
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int i = local % P.mx[0] + 1;
   const int row = local / P.mx[0];
@@ -1152,7 +1426,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
   const int g = xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
-  const DPatch& P = L.pt[L.patches[pi]];
+  const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -1195,8 +1469,8 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
                                                         float* __restrict__ rpar) {
   CoarseCell c;
   if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
-  const DPatch& P = L.pt[c.pid];
-  const DPatch& Q = L.pt[P.parent];
+  const DPatch P = L.pt[c.pid];
+  const DPatch Q = L.pt[P.parent];
   const float t1 = (float)(dt / (P.dx * P.dx)), t2 = (float)(eps2 / (P.dx * P.dx));
   const int nch = (DIM == 3) ? 8 : 4;
   float s0 = 0.0f, s1 = 0.0f;
@@ -1483,7 +1757,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const DPatch* __restrict
                                                         T* __restrict__ qc, const T* __restrict__ coef) {
   // descend
   for (int l = 0; l < A.nlev - 1; l++) {
-    const DPatch& P = pt[A.pid[l]];
+    const DPatch P = pt[A.pid[l]];
     cv_level_relax<DIM, T, LIN>(P, nsm, A.dt, A.eps2, q, f, coef);
     cv_down<DIM, T, LIN>(P, pt[A.pid[l + 1]], A.dt, A.eps2, q, f, qc, coef);
     __syncthreads();
@@ -1491,7 +1765,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const DPatch* __restrict
   cv_level_relax<DIM, T, LIN>(pt[A.pid[A.nlev - 1]], nsm, A.dt, A.eps2, q, f, coef);
   // ascend
   for (int l = A.nlev - 2; l >= 0; l--) {
-    const DPatch& P = pt[A.pid[l]];
+    const DPatch P = pt[A.pid[l]];
     cv_up<DIM, T>(P, pt[A.pid[l + 1]], q, qc);
     __syncthreads();
     cv_level_relax<DIM, T, LIN>(P, nsm, A.dt, A.eps2, q, f, coef);
@@ -1506,7 +1780,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const DPatch* __restrict
 __global__ __launch_bounds__(kBlock) void k_layout(const DPatch* __restrict__ pt, int pid, int kind,
                                                    int nv, int to_pool, double* __restrict__ stage,
                                                    double* __restrict__ pool) {
-  const DPatch& P = pt[pid];
+  const DPatch P = pt[pid];
   int e0, e1, e2;
   if (kind == 0) { e0 = P.nq[0]; e1 = P.nq[1]; e2 = P.nq[2]; }
   else if (kind == 1) { e0 = P.mx[0]; e1 = P.mx[1]; e2 = P.mx[2]; }
@@ -2389,7 +2663,18 @@ extern "C" int32_t bsam_coarse_loading(bsam_ctx* c, int32_t level) {
   const ModelK m = c->mk();
   const bool fin = level == c->finest;
   if (fin) kstat_begin(c, BSAM_KSTAT_FASRHS_FINEST);
-  if (c->p.model == BSAM_MODEL_CH) {
+  int nblk = 0;
+  int kc = (c->p.model == BSAM_MODEL_CH) ? march_kc(c, L, &nblk) : 0;
+  if (kc) {
+    // whole coarse planes per block
+    const DPatch& P = c->hp[L.patches[0]];
+    kc += kc & 1;
+    const int ntile = ((P.mx[0] / 2 + 63) / 64) * ((P.mx[1] + 7) / 8);
+    nblk = ntile * ((P.mx[2] + kc - 1) / kc);
+    const double h2 = L.h * L.h;
+    k_ch3d_resid_march<1, double><<<nblk, 512, 0, c->s>>>(c->d_pt, L.patches[0], kc, m.dt / h2, m.eps2 / h2, m,
+                                                          c->q, c->f, nullptr, nullptr, c->qc, c->f);
+  } else if (c->p.model == BSAM_MODEL_CH) {
     DISPATCH_DIM(c->p.ndims,
                  (k_fas_rhs<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)),
                  (k_fas_rhs<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)));
@@ -2505,10 +2790,29 @@ extern "C" int32_t bsam_residual(bsam_ctx* c, int32_t level) {
 }
 
 // Enqueue the residual norm of `level` into d_norm[0..1] (no wait).
+// Marching residual for a single large 3D CH patch; returns the number of
+// partial sums written, or 0 when the level does not qualify.
+template <typename TO>
+static int resid_march(bsam_ctx* c, int level, TO* out, double* partial) {
+  Level& L = c->L(level);
+  int nblk = 0;
+  if (c->p.model != BSAM_MODEL_CH) return 0;
+  const int kc = march_kc(c, L, &nblk);
+  if (!kc) return 0;
+  const double h2 = L.h * L.h;
+  k_ch3d_resid_march<0, TO><<<nblk, 512, 0, c->s>>>(c->d_pt, L.patches[0], kc, c->p.dt / h2,
+                                                     c->p.model_params[0] / h2, c->mk(), c->q, c->f, out, partial,
+                                                     nullptr, nullptr);
+  if (hipGetLastError() != hipSuccess) return -1;
+  return nblk;
+}
+
 static int32_t enqueue_norm(bsam_ctx* c, int level, bool write_rf) {
   const bool fin = level == c->finest;
   if (fin) kstat_begin(c, BSAM_KSTAT_RESID_FINEST);
-  int nb = residual_launch(c, level, write_rf ? c->rf : nullptr, c->d_partial);
+  int nb = resid_march<double>(c, level, write_rf ? c->rf : nullptr, c->d_partial);
+  if (nb < 0) return set_err(BSAM_EHIP, "residual kernel launch failed");
+  if (nb == 0) nb = residual_launch(c, level, write_rf ? c->rf : nullptr, c->d_partial);
   if (nb < 0) return nb;
   if (fin) {
     double N = 0, G = 0;
@@ -2872,10 +3176,14 @@ int32_t mixed_residuals(bsam_ctx* c) {
     double* part = (level == c->finest) ? c->d_partial : nullptr;
     const bool fin = level == c->finest;
     if (fin) kstat_begin(c, BSAM_KSTAT_RESID_FINEST);
-    DISPATCH_DIM(c->p.ndims,
-                 (k_residual<2, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)),
-                 (k_residual<3, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)));
-    LAUNCH_CHECK();
+    int nbm = resid_march<float>(c, level, m->rhs, part);
+    if (nbm < 0) return set_err(BSAM_EHIP, "residual kernel launch failed");
+    if (nbm == 0) {
+      DISPATCH_DIM(c->p.ndims,
+                   (k_residual<2, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)),
+                   (k_residual<3, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)));
+      LAUNCH_CHECK();
+    }
     if (fin) {
       double N = 0, G = 0;
       for (int pid : L.patches) {
@@ -2885,7 +3193,7 @@ int32_t mixed_residuals(bsam_ctx* c) {
                                : 2.0 * ((double)P.mx[0] + P.mx[1]);
       }
       kstat_end(c, BSAM_KSTAT_RESID_FINEST, 16.0 * (N + G) + 16.0 * N + 8.0 * N);
-      nbf = nb;
+      nbf = nbm > 0 ? nbm : nb;
     }
   }
   return reduce_norm(c, nbf);
@@ -2893,10 +3201,19 @@ int32_t mixed_residuals(bsam_ctx* c) {
 
 // One outer iteration: correction V-cycle, q += e, FillDown, ghost refresh
 // of the finest level, new residual and norm.
+// The correction e starts every cycle at zero: levels >= 0 are cleared by
+// the q += e update of the previous cycle (or the set-up), the levels below
+// the root here.
 int32_t mixed_iteration(bsam_ctx* c) {
   bsam::MixedState* m = c->mixed;
-  k_zero_f<<<nblocks(c->nq), kBlock, 0, c->s>>>(m->e, c->nq);
-  LAUNCH_CHECK();
+  for (int level = c->minlevel; level < 0 && level <= c->finest; level++) {
+    for (int pid : c->L(level).patches) {
+      const DPatch& P = c->hp[pid];
+      const long n = (long)P.qvs * c->p.nrvars;
+      k_zero_f<<<nblocks(n), kBlock, 0, c->s>>>(m->e + P.qoff, n);
+      LAUNCH_CHECK();
+    }
+  }
   RETCHK(mixed_vcycle(c, c->finest));
   for (int level = 0; level <= c->finest; level++) {
     Level& L = c->L(level);
@@ -2940,6 +3257,8 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
                  (k_coef<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)));
     LAUNCH_CHECK();
   }
+  k_zero_f<<<nblocks(c->nq), kBlock, 0, c->s>>>(c->mixed->e, c->nq);
+  LAUNCH_CHECK();
   RETCHK(mixed_residuals(c));
   double res = 0.0;
   int it;

@@ -141,18 +141,23 @@ def test_mixed_precision_matches_fp64(nd, n, steps, gpu_solver_cls):
     assert max(cb) <= max(ca) + 2
 
 
-@pytest.mark.parametrize("mthbc", [(2, 2, 2, 2, 2, 2), (2, 2, 2, 2, 10, 10)])
-def test_large_patch_sweep_vs_oracle(mthbc, gpu_solver_cls):
-    """128^3 root: the finest level uses the marching 3D sweep (with the
-    periodic ghost images written by the sweep when the patch is periodic in
-    every direction).  One time step, bit-identical to the oracle."""
-    nd, n = 3, 128
-    print(f"[gpu] 128^3 sweep mthbc={mthbc}", flush=True)
-    minlevel = -(int(np.log2(n)) - 1)
-    hier = hierarchy.uniform(nd, (n,) * nd, minlevel, list(mthbc))
+@pytest.mark.parametrize("shape,mthbc", [((128, 128, 128), (2, 2, 2, 2, 2, 2)),
+                                          ((128, 128, 128), (2, 2, 2, 2, 10, 10)),
+                                          ((200, 36, 16), (2, 2, 2, 2, 2, 2)),
+                                          ((200, 36, 16), (2, 2, 10, 10, 2, 2))])
+def test_large_patch_sweep_vs_oracle(shape, mthbc, gpu_solver_cls):
+    """A root large enough for the marching 3D sweep and residual on the
+    finest level (with the periodic ghost images written by the sweep when
+    the patch is periodic in every direction).  200 x 36 x 16 leaves partial
+    tiles in x (100 = 64 + 36 half-rows) and y (36 = 4 x 8 + 4 rows).  One
+    time step, bit-identical to the oracle."""
+    nd = 3
+    print(f"[gpu] {shape} sweep mthbc={mthbc}", flush=True)
+    minlevel = -(int(np.log2(min(shape))) - 2)
+    hier = hierarchy.uniform(nd, shape, minlevel, list(mthbc))
     root = len(hier) - 1
     phi = configs.init_phi(hier[root], nd, 1, 1003, 0.1, 0.05)
-    q0 = np.zeros((2,) + (n + 2,) * nd)
+    q0 = np.zeros(hierarchy.field_shape(hier[root], nd, 2, "q"))
     q0[0, 1:-1, 1:-1, 1:-1] = phi
     res = []
     for S in (gpu_solver_cls(configs.params(nd, 5e-5, 0.015, 1e-8), hier),
@@ -165,7 +170,38 @@ def test_large_patch_sweep_vs_oracle(mthbc, gpu_solver_cls):
         for lv in range(minlevel, 1):
             S.copy_q_to_qold(lv)
         rc, ncyc, r, h = S.multigrid_iterations()
-        res.append((h, S.get(root, "q")))
-    (hg, qg), (ho, qo) = res
+        S.residual(0)
+        res.append((h, S.get(root, "q"), S.get(root, "rf")))
+    (hg, qg, rg), (ho, qo, ro) = res
     assert np.array_equal(qg, qo), float(np.max(np.abs(qg - qo)))
+    assert np.array_equal(rg, ro), float(np.max(np.abs(rg - ro)))
     _hist_close(hg, ho)
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 128), (200, 36, 16)])
+def test_large_patch_mixed_vs_fp64(shape, gpu_solver_cls):
+    """Marching FP32 correction sweep and FP32 residual output on the finest
+    level, full and partial tiles: the mixed path reaches the FP64 path's
+    solution to within 1e-7 in max norm after two time steps."""
+    nd = 3
+    print(f"[gpu] mixed {shape}", flush=True)
+    minlevel = -(int(np.log2(min(shape))) - 2)
+    hier = hierarchy.uniform(nd, shape, minlevel, [2] * 6)
+    root = len(hier) - 1
+    phi = configs.init_phi(hier[root], nd, 1, 1003, 0.1, 0.05)
+    q0 = np.zeros(hierarchy.field_shape(hier[root], nd, 2, "q"))
+    q0[0, 1:-1, 1:-1, 1:-1] = phi
+    outs = []
+    for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+        S = gpu_solver_cls(configs.params(nd, 5e-5, 0.015, 1e-9, precision=prec), hier)
+        S.put(root, "q", q0)
+        S.set_ghost(0, 0)
+        S.copy_q_to_qold(0)
+        for s in range(2):
+            rc, ncyc, r = S.time_step(s == 0)
+            assert r < 1e-9, (prec, s, rc, r)
+        outs.append(S.get(root, "q"))
+        S.close()
+    err = float(np.max(np.abs(outs[0] - outs[1])))
+    print(f"[gpu] mixed {shape} max-norm diff {err:.3e}", flush=True)
+    assert err < 1e-7
