@@ -66,6 +66,52 @@ STOCK_PINS = {
 }
 
 
+# The reference's own PROBLEM/BioGrowth module (2D), run from a seeded
+# restart frame of phi and the nutrient c; model constants as its shipped
+# problemdata.dat (PROBLEM/BioGrowth/problemdata.dat), dx = 0.1 as its
+# griddata.dat (64 cells on [0, 6.4]).
+BG_PINS = {
+    "stock_bg2d_u32_phys": dict(ndims=2, n=32, mthbc=(10, 10, 10, 10), dt=0.1, steps=2, seed=1031,
+                                gamma=1.5, lam=2.0, czero=0.5, eps=0.05),
+    "stock_bg2d_u32_per": dict(ndims=2, n=32, mthbc=(2, 2, 2, 2), dt=0.1, steps=2, seed=1032,
+                               gamma=1.5, lam=2.0, czero=0.5, eps=0.05),
+}
+
+
+def bg_initial(n, seed):
+    """phi = -0.2 + 0.5 U, c = 1 + 0.1 U' (U' keyed by seed + 7), both
+    rounded to the 12 decimals of the frame format, zero ghosts."""
+    hier = hierarchy.uniform(2, (n, n), -1, [2] * 4)
+    root = hier[-1]
+    q = np.zeros((2, n + 2, n + 2))
+    for v, (sd, base, amp) in enumerate(((seed, -0.2, 0.5), (seed + 7, 1.0, 0.1))):
+        x = configs.init_phi(root, 2, 1, sd, base, amp)
+        q[v, 1:-1, 1:-1] = np.array([float("%25.12f" % t) for t in x.ravel()]).reshape(x.shape)
+    return q
+
+
+def make_bg_pin(name, fx, results):
+    nd, n = fx["ndims"], fx["n"]
+    minlevel = -(int(round(math.log2(n))) - 1)
+    h = 0.1
+    q = bg_initial(n, fx["seed"])
+    work = tempfile.mkdtemp(prefix="bsam_bgpin_")
+    try:
+        prob = [("czero", fx["czero"]), ("gamma", fx["gamma"]), ("lambda", fx["lam"]), ("eps", fx["eps"])]
+        refrun.write_inputs(work, ndims=nd, mx=(n,) * nd, maxlevel=0, minlevel=minlevel, dt=fx["dt"],
+                            steps=fx["steps"], qerrortol=1e-9, mthbc=fx["mthbc"], restart=True,
+                            problem=prob, getafterstepstats=False, xupper=h * n, minimumgridpoints=4)
+        refrun.write_restart_frame(work, nd, (n,) * nd, h, q)
+        out = refrun.run("xbsam_bg", work)
+        hist = refrun.parse_residuals(out)
+        end = refrun.parse_frame(os.path.join(work, "OUT", "m00001.dat"), nd)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    assert len(end) == 1 and end[0]["level"] == 0
+    print(f"{name}: BioGrowth histories {[len(x) for x in hist]}")
+    results[name] = dict(fx, minlevel=minlevel, h=h, stock=hist, end_q=end[0]["q"].ravel().tolist())
+
+
 def problem_items(fx, n, dumpsweeps):
     it = [("alpha", 1.0), ("eps", fx["eps"]), ("hroot", 1.0 / n)]
     init = dict(fx["init"])
@@ -225,6 +271,8 @@ def main():
     pins = {}
     for name, fx in STOCK_PINS.items():
         make_stock_pin(name, fx, pins)
+    for name, fx in BG_PINS.items():
+        make_bg_pin(name, fx, pins)
     with open(os.path.join(HERE, "stock_pins.json"), "w") as fh:
         json.dump(pins, fh, indent=1)
 

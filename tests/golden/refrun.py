@@ -16,6 +16,7 @@ binary, and parses
 """
 import os
 import re
+import resource
 import struct
 import subprocess
 
@@ -110,9 +111,15 @@ def write_restart_frame(d, ndims, mx, h, q):
     return path
 
 
+def _unlimited_stack():
+    # the reference keeps whole-patch array temporaries on the stack
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
 def run(binary, d, timeout=600):
     exe = os.path.join(REF_DIR, binary)
-    p = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout)
+    p = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=timeout,
+                       preexec_fn=_unlimited_stack)
     if p.returncode != 0 or "Error" in p.stdout:
         raise RuntimeError(f"reference run failed in {d}:\n{p.stdout[-3000:]}\n{p.stderr[-2000:]}")
     return p.stdout
@@ -153,3 +160,31 @@ def parse_dump(path):
         off += 8 * cnt
         recs.append(dict(tag=tag, ipass=ipass, mx=(m1, m2, m3), h=h, xlower=xl, data=data))
     return recs
+
+
+def parse_frame(path, ndims):
+    """Patches of an output frame written by WriteQ / OutputQ
+    (LIBRARY/bsaminputoutput.f90:60-181): list of dicts with level, mx,
+    mglobal and q as (nrvars, [mz+2,] my+2, mx+2), values as written
+    (F25.12)."""
+    toks = open(path).read().split()
+    pos = 2  # time, finestlevel
+    patches = []
+    while pos < len(toks):
+        level, nd, _two, nv = (int(t) for t in toks[pos:pos + 4])
+        pos += 4
+        pos += 3 * nd  # dx, xlower, xupper
+        mx = [int(t) for t in toks[pos:pos + nd]]
+        pos += nd
+        mg = [int(t) for t in toks[pos:pos + 2 * nd]]
+        pos += 2 * nd
+        ext = [m + 2 for m in mx]
+        cnt = int(np.prod(ext)) * nv
+        vals = np.array([float(t) for t in toks[pos:pos + cnt]])
+        pos += cnt
+        # rows are (i fastest) with the nv values of one cell on a line
+        q = vals.reshape(tuple(ext[::-1]) + (nv,))
+        q = np.moveaxis(q, -1, 0)
+        patches.append(dict(level=level, mx=mx, mglobal=mg, q=q))
+    return patches
+

@@ -205,3 +205,23 @@ def test_large_patch_mixed_vs_fp64(shape, gpu_solver_cls):
     err = float(np.max(np.abs(outs[0] - outs[1])))
     print(f"[gpu] mixed {shape} max-norm diff {err:.3e}", flush=True)
     assert err < 1e-7
+
+
+@pytest.mark.parametrize("name", ["stock_bg2d_u32_phys", "stock_bg2d_u32_per"])
+def test_bg2d_vs_oracle(name, gpu_solver_cls):
+    """BioGrowth 2D (decoupled Gauss-Seidel, BioGrowth/problem.f90:255-353)
+    on the GPU: two time steps from the pinned restart state, bit-identical
+    to the oracle, which matches the reference's own module."""
+    import json
+    import os
+    from test_oracle_pins import bg_setup, bg_steps
+    print(f"[gpu] BioGrowth {name}", flush=True)
+    p = json.load(open(os.path.join(fu.GOLDEN, "stock_pins.json")))[name]
+    G, root = bg_setup(p, gpu_solver_cls)
+    O, _ = bg_setup(p, oracle.Oracle)
+    hg = bg_steps(G, p)
+    ho = bg_steps(O, p)
+    assert len(hg) == len(ho)
+    for a, b in zip(hg, ho):
+        _hist_close(a, b)
+    assert np.array_equal(G.get(root, "q"), O.get(root, "q"))
