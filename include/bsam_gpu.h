@@ -109,7 +109,8 @@ typedef struct bsam_params {
   int32_t use_graph;         /* 1: capture the V-cycle loop body as a hipGraph */
   double  dt;
   double  qerrortol;         /* reference default 1e-6 */
-  double  model_params[8];   /* CH: [0]=eps^2.  BG: [0]=eps^2,[1]=gamma,[2]=lambda,[3]=D */
+  double  model_params[8];   /* CH: [0]=eps^2.  BG: [0]=eps^2,[1]=gamma,[2]=lambda,[3]=czero
+                                (BioGrowth/problemdef.f90:5) */
 } bsam_params;
 
 typedef struct bsam_ctx bsam_ctx;

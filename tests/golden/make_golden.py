@@ -63,6 +63,9 @@ STOCK_PINS = {
     "stock_ch2d_u16_phys": dict(ndims=2, n=16, mthbc=(2, 2, 10, 10), eps=0.05, dt=1e-4, steps=2, seed=1022),
     "stock_ch3d_u8_phys": dict(ndims=3, n=8, mthbc=(10, 10, 10, 10, 10, 10), eps=0.08, dt=1e-4, steps=2,
                                seed=1023),
+    # ErrorAFAS errortype = 2 (max of the component RMS, afasroutines.f90:139-156)
+    "stock_ch2d_u32_err2": dict(ndims=2, n=32, mthbc=(2, 2, 2, 2), eps=0.03, dt=1e-4, steps=2, seed=1024,
+                                errortype=2),
 }
 
 
@@ -252,7 +255,7 @@ def make_stock_pin(name, fx, results):
                 prob = [("alpha", 1.0), ("eps", fx["eps"]), ("hroot", 1.0 / n)]
             refrun.write_inputs(work, ndims=nd, mx=(n,) * nd, maxlevel=0, minlevel=minlevel, dt=fx["dt"],
                                 steps=fx["steps"], qerrortol=1e-9, mthbc=fx["mthbc"], restart=True,
-                                problem=prob, getafterstepstats=False)
+                                problem=prob, getafterstepstats=False, errortype=fx.get("errortype", 1))
             refrun.write_restart_frame(work, nd, (n,) * nd, 1.0 / n, q)
             out = refrun.run(binary, work)
             hists[binary] = refrun.parse_residuals(out)

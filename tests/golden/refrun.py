@@ -46,14 +46,14 @@ def _nml(name, items):
 def write_inputs(d, *, ndims, mx, maxlevel, minlevel, dt, steps, qerrortol,
                  mthbc, xupper=1.0, maxvcycles=50, nsmoothingpasses=2,
                  restart=False, eps=0.03, problem=None, minimumgridpoints=8,
-                 ibuffer=1, desiredfillratio=0.8, getafterstepstats=True):
+                 ibuffer=1, desiredfillratio=0.8, getafterstepstats=True, errortype=1):
     os.makedirs(os.path.join(d, "OUT"), exist_ok=True)
     run = [("restart", restart), ("restartframe", 0), ("syncelliptic", False),
            ("outframes", 1), ("outputuniformmesh", False),
            ("getafterstepstats", getafterstepstats),
            ("timeiterations", steps), ("maxvcycles", maxvcycles),
            ("nsmoothingpasses", nsmoothingpasses), ("omega", 1.0),
-           ("qerrortol", qerrortol), ("dt", dt)]
+           ("qerrortol", qerrortol), ("dt", dt), ("errortype", errortype)]
     grid = [("ndims", ndims), ("maxlevel", maxlevel), ("minlevel", minlevel)]
     for lv in range(0, 6):
         grid += [(f"qtolerance({lv})", 1.0e-2), (f"ibuffer({lv})", ibuffer),

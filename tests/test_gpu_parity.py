@@ -225,3 +225,43 @@ def test_bg2d_vs_oracle(name, gpu_solver_cls):
     for a, b in zip(hg, ho):
         _hist_close(a, b)
     assert np.array_equal(G.get(root, "q"), O.get(root, "q"))
+
+
+@pytest.mark.parametrize("name", ["stock_ch2d_u32", "stock_ch2d_u16_phys", "stock_ch3d_u8_phys",
+                                  "stock_ch2d_u32_err2"])
+def test_reference_ch_module_histories(name, gpu_solver_cls):
+    """The GPU path against the residual histories of the reference's own
+    PROBLEM/CH module (ErrorAFAS errortype 1 and 2), from the same
+    12-decimal restart state, and bit-identical to the oracle's state."""
+    import json
+    import os
+    print(f"[gpu] reference CH module {name}", flush=True)
+    p = json.load(open(os.path.join(fu.GOLDEN, "stock_pins.json")))[name]
+    nd, n = p["ndims"], p["n"]
+    hier = hierarchy.uniform(nd, (n,) * nd, p["minlevel"], p["mthbc"])
+    root = len(hier) - 1
+    phi = configs.init_phi(hier[root], nd, 1, p["seed"], 0.0, 0.05)
+    phi = np.array([float("%25.12f" % v) for v in phi.ravel()]).reshape(phi.shape)
+    q0 = np.zeros(hierarchy.field_shape(hier[root], nd, 2, "q"))
+    q0[(0,) + (slice(1, -1),) * nd] = phi
+    out = []
+    for S in (gpu_solver_cls(configs.params(nd, p["dt"], p["eps"], 1e-9, errortype=p.get("errortype", 1)), hier),
+              oracle.Oracle(configs.params(nd, p["dt"], p["eps"], 1e-9, errortype=p.get("errortype", 1)), hier)):
+        S.put(root, "q", q0)
+        S.set_ghost(0, 0)
+        S.copy_q_to_qold(0)
+        hists = []
+        for s in range(p["steps"]):
+            S.set_ghost(0, 0)
+            if s == 0:
+                S.fill_down(abi.SOLUTIONFIELD)
+            for lv in range(p["minlevel"], 1):
+                S.copy_q_to_qold(lv)
+            rc, ncyc, r, h = S.multigrid_iterations()
+            hists.append(h)
+        out.append((hists, S.get(root, "q")))
+    (hg, qg), (ho, qo) = out
+    assert len(hg) == len(p["stock"])
+    for a, b in zip(hg, p["stock"]):
+        _hist_close(a, b)
+    assert np.array_equal(qg, qo)

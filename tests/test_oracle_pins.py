@@ -44,12 +44,13 @@ def test_probe_hooks_match_reference_ch_module():
     pins = json.load(open(os.path.join(fu.GOLDEN, "stock_pins.json")))
     assert pins
     ch = {k: v for k, v in pins.items() if "probe" in v}
-    assert len(ch) == 3
+    assert len(ch) == 4
     for name, p in ch.items():
         assert p["stock"] == p["probe"], name
 
 
-@pytest.mark.parametrize("name", ["stock_ch2d_u32", "stock_ch2d_u16_phys", "stock_ch3d_u8_phys"])
+@pytest.mark.parametrize("name", ["stock_ch2d_u32", "stock_ch2d_u16_phys", "stock_ch3d_u8_phys",
+                                  "stock_ch2d_u32_err2"])
 def test_oracle_matches_reference_ch_module(name):
     """The oracle, started from the same 12-decimal restart frame, reproduces
     the residual histories of the reference's own PROBLEM/CH module."""
@@ -57,7 +58,7 @@ def test_oracle_matches_reference_ch_module(name):
     nd, n = p["ndims"], p["n"]
     hier = hierarchy.uniform(nd, (n,) * nd, p["minlevel"], p["mthbc"])
     root = len(hier) - 1
-    O = oracle.Oracle(configs.params(nd, p["dt"], p["eps"], 1e-9), hier)
+    O = oracle.Oracle(configs.params(nd, p["dt"], p["eps"], 1e-9, errortype=p.get("errortype", 1)), hier)
     phi = configs.init_phi(hier[root], nd, 1, p["seed"], 0.0, 0.05)
     phi = np.array([float("%25.12f" % v) for v in phi.ravel()]).reshape(phi.shape)
     q = O.field(root, "q")
@@ -70,7 +71,7 @@ def test_oracle_matches_reference_ch_module(name):
         rc, ncyc, r = O.time_step(s == 0)
         hists.append(None)
     # rerun collecting histories through the explicit path
-    O2 = oracle.Oracle(configs.params(nd, p["dt"], p["eps"], 1e-9), hier)
+    O2 = oracle.Oracle(configs.params(nd, p["dt"], p["eps"], 1e-9, errortype=p.get("errortype", 1)), hier)
     q2 = O2.field(root, "q")
     q2[...] = 0.0
     q2[(0,) + (slice(1, -1),) * nd] = phi
