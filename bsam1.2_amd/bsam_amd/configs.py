@@ -47,10 +47,50 @@ def seeded_u(seed, level, i, j, k):
     return 2.0 * (x.astype(np.float64) / 4294967296.0) - 1.0
 
 
+def sphere_radius(seed, dx, dy, dz, radius, pamp=0.02, lmax=6):
+    """Radius of the perturbed sphere in direction (dx, dy, dz) about its
+    centre: radius + pamp * sum_{l=1..lmax, |m|<=l} c_lm Y_lm(theta, psi),
+    orthonormal real spherical harmonics (polar axis z), c_lm = U(-1,1) of
+    the seeded generator keyed by level -32, i = l, j = m+8, k = 1."""
+    r = np.sqrt(dx * dx + dy * dy + dz * dz)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ct = np.where(r > 0.0, dz / np.where(r > 0.0, r, 1.0), 1.0)
+    psi = np.where(r > 0.0, np.arctan2(dy, dx), 0.0)
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    s = np.zeros_like(r)
+    for mm in range(0, lmax + 1):
+        pmm = np.ones_like(r)
+        fact = 1.0
+        for _ in range(mm):
+            pmm = -pmm * fact * st
+            fact += 2.0
+        pa, pb = np.zeros_like(r), pmm
+        for l in range(mm, lmax + 1):
+            if l == mm:
+                pl = pmm
+            else:
+                pl = (ct * float(2 * l - 1) * pb - float(l + mm - 1) * pa) / float(l - mm)
+                pa, pb = pb, pl
+            if l < 1:
+                continue
+            nrm = 1.0
+            for k in range(l - mm + 1, l + mm + 1):
+                nrm = nrm / float(k)
+            nrm = math.sqrt(float(2 * l + 1) / (4.0 * math.pi) * nrm)
+            if mm == 0:
+                s = s + float(seeded_u(seed, -32, l, 8, 1)) * nrm * pl
+            else:
+                ylm = math.sqrt(2.0) * nrm * pl
+                s = s + float(seeded_u(seed, -32, l, mm + 8, 1)) * ylm * np.cos(float(mm) * psi)
+                s = s + float(seeded_u(seed, -32, l, -mm + 8, 1)) * ylm * np.sin(float(mm) * psi)
+    return radius + pamp * s
+
+
 def init_phi(desc, ndims, mode, seed, base=0.0, amp=0.05, radius=0.25, eps=0.03,
-             xc=(0.5, 0.5, 0.5)):
+             xc=(0.5, 0.5, 0.5), pamp=0.02, lmax=6):
     """Interior phi of one patch, shape ([mz,] my, mx), following the probe
-    module's InitPhi."""
+    module's InitPhi (mode 1: base + amp U; mode 2: tanh circle/sphere +
+    amp U; mode 3: tanh perturbed sphere + amp U)."""
     h = desc["dx"]
     lev = desc["level"]
     mx = desc["mx"]
@@ -70,6 +110,9 @@ def init_phi(desc, ndims, mode, seed, base=0.0, amp=0.05, radius=0.25, eps=0.03,
     y = (J - 0.5) * h + desc["xlower"][1]
     z = (K - 0.5) * h + desc["xlower"][2] if ndims == 3 else xc[2]
     r = np.sqrt((x - xc[0]) ** 2 + (y - xc[1]) ** 2 + (z - xc[2]) ** 2)
+    if mode == 3:
+        rs = sphere_radius(seed, x - xc[0], y - xc[1], z - xc[2], radius, pamp, lmax)
+        return np.tanh((rs - r) / (math.sqrt(2.0) * eps)) + amp * u
     return np.tanh((radius - r) / (math.sqrt(2.0) * eps)) + amp * u
 
 

@@ -94,7 +94,8 @@ struct LevelArgs {
   const int* __restrict__ patches;  // descriptor indices of the level
   const int* __restrict__ prefix;   // npatch+1 prefix sums of the kernel's domain
   int npatch;
-  int total;
+  int total;  // end of the launch's range of the domain
+  int gbase;  // start of the range: 0 for a whole level, prefix[p] for patch p alone
 };
 
 // ---------------------------------------------------------------------------
@@ -129,7 +130,7 @@ template <int DIM, bool WRAP>
 __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double dt, double eps2,
                                                      double h, double* __restrict__ q,
                                                      const double* __restrict__ f) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, Model
                                                        double* __restrict__ q,
                                                        const double* __restrict__ qold,
                                                        const double* __restrict__ f) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      double* __restrict__ partial) {
   const int lb = xcd_block();
   double s0 = 0.0, s1 = 0.0;
-  const int g = lb * kBlock + threadIdx.x;
+  const int g = L.gbase + lb * kBlock + threadIdx.x;
   if (g < L.total) {
     const int pi = find_patch(L.prefix, L.npatch, g);
     const DPatch P = L.pt[L.patches[pi]];
@@ -768,7 +769,7 @@ template <int DIM, typename T, bool WRAP = false>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const T* __restrict__ src,
                                                      T* __restrict__ dst_parent, T* __restrict__ qc) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
+  if (!decode_coarse<DIM>(L, L.gbase + xcd_block() * kBlock + threadIdx.x, c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
@@ -790,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const 
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, const T* __restrict__ q,
                                                         T* __restrict__ qc) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -819,7 +820,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
                                                     const double* __restrict__ qc,
                                                     double* __restrict__ fpar) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
+  if (!decode_coarse<DIM>(L, L.gbase + xcd_block() * kBlock + threadIdx.x, c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const double h = P.dx;
@@ -865,7 +866,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, const T* __restrict__ q,
                                                        T* __restrict__ qc) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -898,7 +899,7 @@ __device__ __forceinline__ T linw(T lo, T hi, int fine) {
 template <int DIM, typename T, bool WRAP = false>
 __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T* __restrict__ q,
                                                          const T* __restrict__ qc) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -985,7 +986,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T*
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* __restrict__ q,
                                                         const T* __restrict__ qc) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -1025,7 +1026,7 @@ template <int DIM, int MODEL>
 __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double dt,
                                                    const double* __restrict__ qold,
                                                    double* __restrict__ ftmp, double* __restrict__ f) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -1287,8 +1288,8 @@ __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt
                                                    const BoxDesc* __restrict__ boxes, int nv, int homog,
                                                    T* __restrict__ q) {
   const BoxDesc B = boxes[blockIdx.x];
-  const DPatch& D = pt[B.dp];
-  const DPatch& S = pt[B.sp];
+  const DPatch D = pt[B.dp];
+  const DPatch S = pt[B.sp];
   const int n = B.ext[0] * B.ext[1] * B.ext[2];
   for (int e = threadIdx.x; e < n * nv; e += blockDim.x) {
     const int v = e / n, c = e % n;
@@ -1381,7 +1382,7 @@ __global__ __launch_bounds__(kBlock) void k_add_corr(double* __restrict__ q, flo
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __restrict__ q,
                                                  float* __restrict__ coef) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -1423,7 +1424,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
                                                       float a12, float* __restrict__ e,
                                                       const float* __restrict__ rhs,
                                                       const float* __restrict__ coef) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
   const int pi = find_patch(L.prefix, L.npatch, g);
   const DPatch P = L.pt[L.patches[pi]];
@@ -1468,7 +1469,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
                                                         const float* __restrict__ coef,
                                                         float* __restrict__ rpar) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, xcd_block() * kBlock + threadIdx.x, c)) return;
+  if (!decode_coarse<DIM>(L, L.gbase + xcd_block() * kBlock + threadIdx.x, c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const float t1 = (float)(dt / (P.dx * P.dx)), t2 = (float)(eps2 / (P.dx * P.dx));
@@ -1827,6 +1828,7 @@ struct Level {
   std::vector<int> patches;
   int* d_patches = nullptr;
   int* d_prefix[kNumDom] = {};
+  std::vector<int> hprefix[kNumDom];
   int total[kNumDom] = {};
   std::vector<Stage> stages;      // box stages after interpolation
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
@@ -1901,8 +1903,54 @@ LevelArgs largs(bsam_ctx* c, Level& L, Dom d) {
   a.prefix = L.d_prefix[d];
   a.npatch = (int)L.patches.size();
   a.total = L.total[d];
+  a.gbase = 0;
   return a;
 }
+
+// The launches of one per-level operation: the whole level in one launch
+// (batching = 1), or one launch per patch (batching = 0, the reference's
+// per-patch ApplyOnLevel granularity).  `boff` is the number of blocks
+// launched before the current one, for kernels that write per-block
+// partial results.
+struct LaunchIter {
+  bsam_ctx* c;
+  Level& L;
+  Dom d;
+  size_t i = 0;
+  int boff = 0, nb_prev = 0;
+  bool first = true;
+  LaunchIter(bsam_ctx* c_, Level& L_, Dom d_) : c(c_), L(L_), d(d_) {}
+  bool next(LevelArgs& a, int& nb) {
+    boff += nb_prev;
+    nb_prev = 0;
+    const LevelArgs full = largs(c, L, d);
+    if (c->p.batching || L.patches.size() <= 1) {
+      if (!first) return false;
+      first = false;
+      a = full;
+      nb = nblocks(full.total);
+      nb_prev = nb;
+      return nb > 0;
+    }
+    while (i < L.patches.size()) {
+      a = full;
+      a.gbase = L.hprefix[d][i];
+      a.total = L.hprefix[d][i + 1];
+      i++;
+      nb = nblocks((long)a.total - a.gbase);
+      if (nb > 0) {
+        nb_prev = nb;
+        return true;
+      }
+    }
+    return false;
+  }
+  int blocks() const { return boff + nb_prev; }
+};
+#define FOR_LAUNCHES(c, L, dom, a, nb) \
+  LevelArgs a;                          \
+  int nb = 0;                           \
+  for (LaunchIter it_(c, L, dom); it_.next(a, nb);)
 
 int32_t check_level(bsam_ctx* c, int level) {
   if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
@@ -2315,10 +2363,12 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
       pre[L.patches.size()] = (int)acc;
       if (acc > 0x7fffffffL) return set_err(BSAM_EINVAL, "level %d too large", L.level);
       L.total[d] = (int)acc;
+      L.hprefix[d] = pre;
       HIPCHK(hipMalloc(&L.d_prefix[d], pre.size() * sizeof(int)));
       HIPCHK(hipMemcpy(L.d_prefix[d], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
     }
-    maxblocks = std::max(maxblocks, nblocks(L.total[kCell]));
+    // per-patch launches round every patch up to whole blocks
+    maxblocks = std::max(maxblocks, nblocks(L.total[kCell]) + (int)L.patches.size());
     // contiguous span of the level in the q pools (patches are laid out in
     // descriptor order; levels are spans only if their patches are
     // consecutive -- otherwise copies go patch by patch)
@@ -2446,21 +2496,21 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
   RETCHK(check_level(c, level));
   if (ipass != 1 && ipass != 2) return set_err(BSAM_EINVAL, "ipass must be 1 or 2");
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kHalf);
-  const int nb = nblocks(a.total);
   const int col = ipass - 1;
   const bool fin = level == c->finest;
   if (fin) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
-  if (c->p.model == BSAM_MODEL_CH) {
-    const double eps2 = c->p.model_params[0];
-    if (c->p.ndims == 2)
-      k_relax_ch<2, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
-    else
-      k_relax_ch<3, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
-  } else {
-    k_relax_bg2d<<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), L.h, c->q, c->qold, c->f);
+  FOR_LAUNCHES(c, L, kHalf, a, nb) {
+    if (c->p.model == BSAM_MODEL_CH) {
+      const double eps2 = c->p.model_params[0];
+      if (c->p.ndims == 2)
+        k_relax_ch<2, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
+      else
+        k_relax_ch<3, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
+    } else {
+      k_relax_bg2d<<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), L.h, c->q, c->qold, c->f);
+    }
+    LAUNCH_CHECK();
   }
-  LAUNCH_CHECK();
   if (fin) {
     // algorithmic bytes of one colour sweep (CH): phi over N+G_f, mu of the
     // other colour over (N+G_f)/2, f of the colour (2 fields) N/2, and the
@@ -2487,10 +2537,14 @@ static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog) {
   Level& L = c->L(level);
   const int nv = c->p.nrvars;
   if (level > 0) {
+    // one workgroup per patch; per-patch mode launches them one by one
     const int np = (int)L.patches.size();
-    if (c->p.ndims == 2) k_interp<2, T><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, q);
-    else k_interp<3, T><<<np, kBlock, 0, c->s>>>(c->d_pt, L.d_patches, nv, ipass, q);
-    LAUNCH_CHECK();
+    const int per = c->p.batching ? np : 1;
+    for (int p0 = 0; p0 < np; p0 += per) {
+      if (c->p.ndims == 2) k_interp<2, T><<<per, kBlock, 0, c->s>>>(c->d_pt, L.d_patches + p0, nv, ipass, q);
+      else k_interp<3, T><<<per, kBlock, 0, c->s>>>(c->d_pt, L.d_patches + p0, nv, ipass, q);
+      LAUNCH_CHECK();
+    }
   }
   if (L.wrap_pid >= 0) {
     const DPatch& P = c->hp[L.wrap_pid];
@@ -2500,8 +2554,13 @@ static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog) {
     LAUNCH_CHECK();
   }
   for (const Stage& s : L.stages) {
-    k_boxops<T><<<s.n, kBlock, 0, c->s>>>(c->d_pt, s.d, nv, homog, q);
-    LAUNCH_CHECK();
+    // one workgroup per box; per-patch mode launches one box (one patch
+    // pair or face) at a time, as the reference's pairwise callbacks go
+    const int per = c->p.batching ? s.n : 1;
+    for (int b0 = 0; b0 < s.n; b0 += per) {
+      k_boxops<T><<<per, kBlock, 0, c->s>>>(c->d_pt, s.d + b0, nv, homog, q);
+      LAUNCH_CHECK();
+    }
   }
   return BSAM_OK;
 }
@@ -2557,7 +2616,7 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass) {
   } else {
     LevelArgs a = largs(c, L, kHalf);
     DISPATCH_DIM(c->p.ndims, (k_relax_ch<2, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f)),
-                 (k_relax_ch<3, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f)));
+                 (k_relax_ch<3, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f)));  // one patch
   }
   LAUNCH_CHECK();
   if (fin) {
@@ -2590,14 +2649,15 @@ static int32_t need_parent(bsam_ctx* c, int level) {
 template <typename T>
 static int32_t restrict_t(bsam_ctx* c, int level, const T* src, T* dst, T* qc, bool wrap = false) {
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kCoarse);
-  if (wrap)
-    DISPATCH_DIM(c->p.ndims, (k_restrict<2, T, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
-                 (k_restrict<3, T, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
-  else
-    DISPATCH_DIM(c->p.ndims, (k_restrict<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
-                 (k_restrict<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
-  LAUNCH_CHECK();
+  FOR_LAUNCHES(c, L, kCoarse, a, nb) {
+    if (wrap)
+      DISPATCH_DIM(c->p.ndims, (k_restrict<2, T, true><<<nb, kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
+                   (k_restrict<3, T, true><<<nb, kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
+    else
+      DISPATCH_DIM(c->p.ndims, (k_restrict<2, T><<<nb, kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)),
+                   (k_restrict<3, T><<<nb, kBlock, 0, c->s>>>(a, c->nv(), src, dst, qc)));
+    LAUNCH_CHECK();
+  }
   return BSAM_OK;
 }
 
@@ -2620,28 +2680,33 @@ static bool restrict_wraps_parent(bsam_ctx* c, int level) {
 template <typename T>
 static int32_t ring_t(bsam_ctx* c, int level, const T* q, T* qc) {
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kQcBox);
-  DISPATCH_DIM(c->p.ndims, (k_coarse_ring<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)),
-               (k_coarse_ring<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)));
-  LAUNCH_CHECK();
+  FOR_LAUNCHES(c, L, kQcBox, a, nb) {
+    DISPATCH_DIM(c->p.ndims, (k_coarse_ring<2, T><<<nb, kBlock, 0, c->s>>>(a, c->nv(), q, qc)),
+                 (k_coarse_ring<3, T><<<nb, kBlock, 0, c->s>>>(a, c->nv(), q, qc)));
+    LAUNCH_CHECK();
+  }
   return BSAM_OK;
 }
 
 template <typename T>
 static int32_t correct_t(bsam_ctx* c, int level, T* q, T* qc, bool wrap = false) {
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kQcBox);
-  LevelArgs b = largs(c, L, kDual);
-  DISPATCH_DIM(c->p.ndims, (k_correct_qc<2, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)),
-               (k_correct_qc<3, T><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->nv(), q, qc)));
-  LAUNCH_CHECK();
-  if (wrap)
-    DISPATCH_DIM(c->p.ndims, (k_prolong_dual<2, T, true><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
-                 (k_prolong_dual<3, T, true><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
-  else
-    DISPATCH_DIM(c->p.ndims, (k_prolong_dual<2, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
-                 (k_prolong_dual<3, T><<<nblocks(b.total), kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
-  LAUNCH_CHECK();
+  {
+    FOR_LAUNCHES(c, L, kQcBox, a, nb) {
+      DISPATCH_DIM(c->p.ndims, (k_correct_qc<2, T><<<nb, kBlock, 0, c->s>>>(a, c->nv(), q, qc)),
+                   (k_correct_qc<3, T><<<nb, kBlock, 0, c->s>>>(a, c->nv(), q, qc)));
+      LAUNCH_CHECK();
+    }
+  }
+  FOR_LAUNCHES(c, L, kDual, b, nb) {
+    if (wrap)
+      DISPATCH_DIM(c->p.ndims, (k_prolong_dual<2, T, true><<<nb, kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
+                   (k_prolong_dual<3, T, true><<<nb, kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
+    else
+      DISPATCH_DIM(c->p.ndims, (k_prolong_dual<2, T><<<nb, kBlock, 0, c->s>>>(b, c->nv(), q, qc)),
+                   (k_prolong_dual<3, T><<<nb, kBlock, 0, c->s>>>(b, c->nv(), q, qc)));
+    LAUNCH_CHECK();
+  }
   return BSAM_OK;
 }
 
@@ -2658,8 +2723,6 @@ extern "C" int32_t bsam_coarse_ghost_points(bsam_ctx* c, int32_t level) {
 extern "C" int32_t bsam_coarse_loading(bsam_ctx* c, int32_t level) {
   RETCHK(need_parent(c, level));
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kCoarse);
-  const int nb = nblocks(a.total);
   const ModelK m = c->mk();
   const bool fin = level == c->finest;
   if (fin) kstat_begin(c, BSAM_KSTAT_FASRHS_FINEST);
@@ -2674,12 +2737,17 @@ extern "C" int32_t bsam_coarse_loading(bsam_ctx* c, int32_t level) {
     const double h2 = L.h * L.h;
     k_ch3d_resid_march<1, double><<<nblk, 512, 0, c->s>>>(c->d_pt, L.patches[0], kc, m.dt / h2, m.eps2 / h2, m,
                                                           c->q, c->f, nullptr, nullptr, c->qc, c->f);
-  } else if (c->p.model == BSAM_MODEL_CH) {
-    DISPATCH_DIM(c->p.ndims,
-                 (k_fas_rhs<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)),
-                 (k_fas_rhs<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)));
   } else {
-    k_fas_rhs<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f);
+    FOR_LAUNCHES(c, L, kCoarse, a, nb) {
+      if (c->p.model == BSAM_MODEL_CH) {
+        DISPATCH_DIM(c->p.ndims,
+                     (k_fas_rhs<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)),
+                     (k_fas_rhs<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)));
+      } else {
+        k_fas_rhs<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f);
+      }
+      LAUNCH_CHECK();
+    }
   }
   LAUNCH_CHECK();
   if (fin) {
@@ -2728,16 +2796,16 @@ extern "C" int32_t bsam_fill_down(bsam_ctx* c, int32_t field) {
 static int32_t source_launch(bsam_ctx* c, int level, int mode) {
   RETCHK(check_level(c, level));
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kCell);
-  const int nb = nblocks(a.total);
-  if (c->p.model == BSAM_MODEL_CH) {
-    DISPATCH_DIM(c->p.ndims,
-                 (k_source<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)),
-                 (k_source<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)));
-  } else {
-    k_source<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f);
+  FOR_LAUNCHES(c, L, kCell, a, nb) {
+    if (c->p.model == BSAM_MODEL_CH) {
+      DISPATCH_DIM(c->p.ndims,
+                   (k_source<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)),
+                   (k_source<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)));
+    } else {
+      k_source<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f);
+    }
+    LAUNCH_CHECK();
   }
-  LAUNCH_CHECK();
   return BSAM_OK;
 }
 
@@ -2769,18 +2837,23 @@ static int32_t reduce_norm(bsam_ctx* c, int nb) {
 
 static int32_t residual_launch(bsam_ctx* c, int level, double* rf, double* partial) {
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kCell);
-  const int nb = nblocks(a.total);
   const ModelK m = c->mk();
-  if (c->p.model == BSAM_MODEL_CH) {
-    DISPATCH_DIM(c->p.ndims,
-                 (k_residual<2, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)),
-                 (k_residual<3, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial)));
-  } else {
-    k_residual<2, BSAM_MODEL_BG, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, partial);
+  int total = 0;
+  LevelArgs a;
+  int nb = 0;
+  for (LaunchIter it(c, L, kCell); it.next(a, nb);) {
+    double* part = partial ? partial + 2L * it.boff : nullptr;
+    if (c->p.model == BSAM_MODEL_CH) {
+      DISPATCH_DIM(c->p.ndims,
+                   (k_residual<2, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part)),
+                   (k_residual<3, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part)));
+    } else {
+      k_residual<2, BSAM_MODEL_BG, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part);
+    }
+    LAUNCH_CHECK();
+    total = it.blocks();
   }
-  LAUNCH_CHECK();
-  return nb;
+  return total;
 }
 
 extern "C" int32_t bsam_residual(bsam_ctx* c, int32_t level) {
@@ -2945,8 +3018,12 @@ static bool kstat_any(bsam_ctx* c) {
   return false;
 }
 
+// The iteration body is captured as a graph only with per-level launches:
+// one launch per patch (batching = 0) is the plain per-patch mode, whose
+// launch count grows with the patch count (tens of thousands per V-cycle on
+// large AMR hierarchies) and is not captured.
 static int32_t run_iteration(bsam_ctx* c, int32_t (*body)(bsam_ctx*)) {
-  if (c->p.use_graph && !kstat_any(c)) {
+  if (c->p.use_graph && c->p.batching && !kstat_any(c)) {
     if (!c->gexec) {
       HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
       c->capturing = true;
@@ -3085,7 +3162,6 @@ namespace {
 int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
   bsam::MixedState* m = c->mixed;
   Level& L = c->L(level);
-  LevelArgs a = largs(c, L, kHalf);
   const double h2 = L.h * L.h, eps2 = c->p.model_params[0];
   const double d2 = 2.0 * c->p.ndims;
   const float tmp1 = (float)(c->p.dt / h2), tmp2 = (float)(eps2 / h2), tmp3 = (float)(d2 * eps2 / h2);
@@ -3104,14 +3180,17 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
       k_relax3d_march<float, true, false, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
     }
   } else {
-    if (wrap)
-      DISPATCH_DIM(c->p.ndims,
-                   (k_relax_lin<2, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
-                   (k_relax_lin<3, true><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
-    else
-      DISPATCH_DIM(c->p.ndims,
-                   (k_relax_lin<2, false><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
-                   (k_relax_lin<3, false><<<nblocks(a.total), kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+    FOR_LAUNCHES(c, L, kHalf, a, nb) {
+      if (wrap)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin<2, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
+                     (k_relax_lin<3, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+      else
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin<2, false><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
+                     (k_relax_lin<3, false><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+      LAUNCH_CHECK();
+    }
   }
   LAUNCH_CHECK();
   if (fin) {
@@ -3149,11 +3228,12 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
     if (!rw) RETCHK(set_ghost_t<float>(c, level - 1, 0, m->e, 1));
     RETCHK(ring_t<float>(c, level, m->e, m->ec));
     Level& L = c->L(level);
-    LevelArgs a = largs(c, L, kCoarse);
-    DISPATCH_DIM(c->p.ndims,
-                 (k_fas_rhs_lin<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)),
-                 (k_fas_rhs_lin<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)));
-    LAUNCH_CHECK();
+    FOR_LAUNCHES(c, L, kCoarse, a, nb) {
+      DISPATCH_DIM(c->p.ndims,
+                   (k_fas_rhs_lin<2><<<nb, kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)),
+                   (k_fas_rhs_lin<3><<<nb, kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)));
+      LAUNCH_CHECK();
+    }
     RETCHK(mixed_vcycle(c, level - 1));
     const bool cw = c->L(level).wrap_pid >= 0;
     RETCHK(correct_t<float>(c, level, m->e, m->ec, cw));
@@ -3171,18 +3251,23 @@ int32_t mixed_residuals(bsam_ctx* c) {
   int nbf = 0;
   for (int level = c->finest; level >= 0; level--) {
     Level& L = c->L(level);
-    LevelArgs a = largs(c, L, kCell);
-    const int nb = nblocks(a.total);
     double* part = (level == c->finest) ? c->d_partial : nullptr;
     const bool fin = level == c->finest;
     if (fin) kstat_begin(c, BSAM_KSTAT_RESID_FINEST);
     int nbm = resid_march<float>(c, level, m->rhs, part);
     if (nbm < 0) return set_err(BSAM_EHIP, "residual kernel launch failed");
+    int nb = nbm;
     if (nbm == 0) {
-      DISPATCH_DIM(c->p.ndims,
-                   (k_residual<2, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)),
-                   (k_residual<3, BSAM_MODEL_CH, float><<<nb, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, part)));
-      LAUNCH_CHECK();
+      LevelArgs a;
+      int nbl = 0;
+      for (LaunchIter it(c, L, kCell); it.next(a, nbl);) {
+        double* pp = part ? part + 2L * it.boff : nullptr;
+        DISPATCH_DIM(c->p.ndims,
+                     (k_residual<2, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)),
+                     (k_residual<3, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)));
+        LAUNCH_CHECK();
+        nb = it.blocks();
+      }
     }
     if (fin) {
       double N = 0, G = 0;
@@ -3193,7 +3278,7 @@ int32_t mixed_residuals(bsam_ctx* c) {
                                : 2.0 * ((double)P.mx[0] + P.mx[1]);
       }
       kstat_end(c, BSAM_KSTAT_RESID_FINEST, 16.0 * (N + G) + 16.0 * N + 8.0 * N);
-      nbf = nbm > 0 ? nbm : nb;
+      nbf = nb;
     }
   }
   return reduce_norm(c, nbf);
@@ -3252,10 +3337,11 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
   RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
   for (int level = c->minlevel; level <= c->finest; level++) {
     Level& L = c->L(level);
-    LevelArgs a = largs(c, L, kCell);
-    DISPATCH_DIM(c->p.ndims, (k_coef<2><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)),
-                 (k_coef<3><<<nblocks(a.total), kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)));
-    LAUNCH_CHECK();
+    FOR_LAUNCHES(c, L, kCell, a, nb) {
+      DISPATCH_DIM(c->p.ndims, (k_coef<2><<<nb, kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)),
+                   (k_coef<3><<<nb, kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)));
+      LAUNCH_CHECK();
+    }
   }
   k_zero_f<<<nblocks(c->nq), kBlock, 0, c->s>>>(c->mixed->e, c->nq);
   LAUNCH_CHECK();

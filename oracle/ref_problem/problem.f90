@@ -24,10 +24,11 @@ USE ProblemDef
 IMPLICIT NONE
 INTEGER:: ierror
 NAMELIST/problemdata/alpha, eps, initmode, seed, base, amp, radius, xc, &
-                     hroot, tagtol, dumpsweeps, dumpsrc, dumpend
+                     hroot, tagtol, dumpsweeps, dumpsrc, dumpend, pamp, lmax
 alpha = 1.0_r8; eps = 0.03_r8; initmode = 1; seed = 1001; base = 0.0_r8
 amp = 0.05_r8; radius = 0.25_r8; xc = 0.5_r8; hroot = 1.0_r8/128.0_r8
 tagtol = 0.05_r8; dumpsweeps = 0; dumpsrc = .FALSE.; dumpend = .FALSE.
+pamp = 0.02_r8; lmax = 6
 OPEN(UNIT=75,FILE='problemdata.dat',STATUS='OLD',ACTION='READ',IOSTAT=ierror)
 IF(ierror/=0) THEN
   PRINT *,'Error opening input file problemdata.dat. Program stop.'
@@ -99,6 +100,58 @@ x = Mix32(IEOR(x,INT(k,i8)))
 u = 2.0_r8*(REAL(x,r8)/4294967296.0_r8)-1.0_r8
 END FUNCTION SeededU
 !
+! Radius of the perturbed sphere in the direction (dx,dy,dz), |d| = r:
+! radius + pamp * sum_{l=1..lmax} sum_{m=-l..l} c_lm Y_lm(theta,psi), with
+! orthonormal real spherical harmonics, cos(theta) = dz/r, psi = atan2(dy,dx),
+! and associated Legendre functions from the standard upward recursion
+! (Condon-Shortley phase).
+FUNCTION SphereRadius(dx,dy,dz,r) RESULT(rs)
+USE ProblemDef
+REAL(KIND=r8), INTENT(IN):: dx, dy, dz, r
+REAL(KIND=r8):: rs, ct, st, psi, pmm, pa, pb, pl, fact, nrm, ylm, s
+INTEGER:: l, mm, k
+REAL(KIND=r8), PARAMETER:: pi = 3.14159265358979323846_r8
+IF(r>0.0_r8) THEN
+  ct = dz/r; psi = ATAN2(dy,dx)
+ELSE
+  ct = 1.0_r8; psi = 0.0_r8
+END IF
+st = SQRT(MAX(0.0_r8,1.0_r8-ct*ct))
+s = 0.0_r8
+DO mm = 0, lmax
+  ! P_mm^mm = (-1)^mm (2mm-1)!! st^mm
+  pmm = 1.0_r8; fact = 1.0_r8
+  DO k = 1, mm
+    pmm = -pmm*fact*st; fact = fact+2.0_r8
+  END DO
+  pa = 0.0_r8; pb = pmm
+  DO l = mm, lmax
+    IF(l==mm) THEN
+      pl = pmm
+    ELSE
+      pl = (ct*REAL(2*l-1,r8)*pb-REAL(l+mm-1,r8)*pa)/REAL(l-mm,r8)
+      pa = pb; pb = pl
+    END IF
+    IF(l>=1) THEN
+      ! sqrt((2l+1)/(4 pi) (l-mm)!/(l+mm)!)
+      nrm = 1.0_r8
+      DO k = l-mm+1, l+mm
+        nrm = nrm/REAL(k,r8)
+      END DO
+      nrm = SQRT(REAL(2*l+1,r8)/(4.0_r8*pi)*nrm)
+      IF(mm==0) THEN
+        s = s+SeededU(-32,l,8,1)*nrm*pl
+      ELSE
+        ylm = SQRT(2.0_r8)*nrm*pl
+        s = s+SeededU(-32,l,mm+8,1)*ylm*COS(REAL(mm,r8)*psi)
+        s = s+SeededU(-32,l,-mm+8,1)*ylm*SIN(REAL(mm,r8)*psi)
+      END IF
+    END IF
+  END DO
+END DO
+rs = radius+pamp*s
+END FUNCTION SphereRadius
+!
 FUNCTION InitPhi(lev,ig,jg,kg,x,y,z) RESULT(p)
 USE ProblemDef
 INTEGER, INTENT(IN):: lev, ig, jg, kg
@@ -110,6 +163,10 @@ CASE(1)
 CASE(2)
   r = SQRT((x-xc(1))**2+(y-xc(2))**2+(z-xc(3))**2)
   p = TANH((radius-r)/(SQRT(2.0_r8)*eps))+amp*SeededU(lev,ig,jg,kg)
+CASE(3)
+  r = SQRT((x-xc(1))**2+(y-xc(2))**2+(z-xc(3))**2)
+  p = TANH((SphereRadius(x-xc(1),y-xc(2),z-xc(3),r)-r)/(SQRT(2.0_r8)*eps)) &
+    + amp*SeededU(lev,ig,jg,kg)
 CASE DEFAULT
   p = 0.0_r8
 END SELECT

@@ -8,7 +8,12 @@ REAL(KIND=r8), PUBLIC:: alpha, eps, eps2
 ! Seeded initial data:
 !   initmode 1: phi = base + amp*U(-1,1)
 !   initmode 2: phi = tanh((radius-|x-xc|)/(sqrt(2)*eps)) + amp*U(-1,1)
-REAL(KIND=r8), PUBLIC:: base, amp, radius, hroot, tagtol
+!   initmode 3: phi = tanh((rs(theta,psi)-|x-xc|)/(sqrt(2)*eps)) + amp*U(-1,1),
+!               rs = radius + pamp*sum_{l=1..lmax,|m|<=l} c_lm Y_lm(theta,psi)
+!               (orthonormal real spherical harmonics about xc, polar axis z;
+!               c_lm = U(-1,1) keyed by level -32, i = l, j = m+8, k = 1)
+REAL(KIND=r8), PUBLIC:: base, amp, radius, hroot, tagtol, pamp
+INTEGER, PUBLIC:: lmax
 REAL(KIND=r8), DIMENSION(1:3), PUBLIC:: xc
 INTEGER, PUBLIC:: initmode, seed
 ! Dump controls: sweeps after relaxation (tag 2), state at source build

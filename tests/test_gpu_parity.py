@@ -26,11 +26,15 @@ def _hist_close(a, b):
 
 
 @pytest.mark.parametrize("name", fu.NAMES)
-def test_golden_multigrid_iterations(name, gpu_solver_cls):
+@pytest.mark.parametrize("batching", [1, 0])
+def test_golden_multigrid_iterations(name, batching, gpu_solver_cls):
+    """MultigridIterations on the reference's states: batched per-level
+    launches and one launch per patch give the reference's end states bit
+    for bit."""
     meta, z = fu.load(name)
     for s, st in enumerate(meta["steps"]):
-        print(f"[gpu] {name} step {s}", flush=True)
-        S, hist = fu.run_step(gpu_solver_cls, meta, z, s)
+        print(f"[gpu] {name} step {s} batching {batching}", flush=True)
+        S, hist = fu.run_step(gpu_solver_cls, meta, z, s, batching=batching)
         _hist_close(hist, st["residuals"])
         for i in st["end"]:
             got = S.get(i, "q").ravel()
