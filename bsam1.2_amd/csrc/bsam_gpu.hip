@@ -1064,7 +1064,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_span(const double* __restrict__
 // ---------------------------------------------------------------------------
 // GetFaceIndex (boundary.f90:1018-1045): 0 ghost, 1 second interior,
 // 2 first interior, 3 coarse outside, 4 coarse inside.
-__device__ __forceinline__ void face_index(const DPatch& P, int dim, int parity, int fi[5]) {
+__host__ __device__ inline void face_index(const DPatch& P, int dim, int parity, int fi[5]) {
   if (parity == 1) {
     fi[0] = 0; fi[1] = 2; fi[2] = 1; fi[3] = P.mb[dim][0] - 1; fi[4] = P.mb[dim][0];
   } else {
@@ -1076,204 +1076,55 @@ __device__ __forceinline__ void face_index(const DPatch& P, int dim, int parity,
 // Transverse position kind of Interpolate2D/3D's statements: 1 -> L,
 // mx -> H, even -> E (sources at +2/+1), odd -> O (sources at -2/-1).
 enum { KE = 0, KO = 1, KL = 2, KH = 3 };
-__device__ __forceinline__ int pos_kind(int pos, int m) {
+__host__ __device__ inline int pos_kind(int pos, int m) {
   return pos == 1 ? KL : (pos == m ? KH : ((pos & 1) ? KO : KE));
 }
-__device__ __forceinline__ int kind_s1(int k) { return k == KE ? 2 : (k == KO ? -2 : 0); }
-__device__ __forceinline__ int kind_s2(int k) { return k == KE ? 1 : (k == KO ? -1 : 0); }
+__host__ __device__ inline int kind_s1(int k) { return k == KE ? 2 : (k == KO ? -2 : 0); }
+__host__ __device__ inline int kind_s2(int k) { return k == KE ? 1 : (k == KO ? -1 : 0); }
 // coarse index of an E/O position, or the three quadratic nodes of L/H
-__device__ __forceinline__ int kind_c(const DPatch& P, int dim, int k, int pos, int r) {
+__host__ __device__ inline int kind_c(const DPatch& P, int dim, int k, int pos, int r) {
   const int mb1 = P.mb[dim][0], mb2 = P.mb[dim][1];
   if (k == KE) return mb1 + pos / 2 - 1;
   if (k == KO) return mb1 + (pos - 1) / 2;
   if (k == KL) return mb1 - 1 + r;   // mb1-1, mb1, mb1+1
   return mb2 + 1 - r;                // mb2+1, mb2, mb2-1
 }
-__device__ __forceinline__ bool bc_interp(int code) { return code == 2 || code == 999; }
+__host__ __device__ inline bool bc_interp(int code) { return code == 2 || code == 999; }
 
-template <int DIM>
-__device__ __forceinline__ int st_k(int k) { return DIM == 3 ? k : 0; }
+// One coarse-fine ghost value of InterpolateCoarseFine (boundary.f90:
+// 100-140, Interpolate2D :143-333, Interpolate3D :1048-1550), reduced to a
+// gather: destination and sources are element offsets of variable 0 in the
+// q-geometry pools (pvs / cvs: variable strides of the patch and of its
+// parent).  kind 1: one coarse value; 3: the transverse quadratic
+// (5/32, 15/16, -3/32); 4: the bilinear (9, 3, 3, 1)/16 of face corners.
+struct GRule {
+  int dst, s1, s2;
+  int c[4];
+  int kind;
+  int pvs, cvs;
+};
 
-// One interpolated ghost value from fine sources s1, s2 (patch P) and the
-// coarse term (parent Q), in the reference's order.
-template <int DIM, typename T>
-__device__ __forceinline__ void interp_store(T* __restrict__ q, const DPatch& P, const DPatch& Q, int nv,
-                                             const int g[3], const int s1[3], const int s2[3], int ncoarse,
-                                             const int (*cq)[3], const T* w) {
+// Apply the rules [lo, hi) in the reference's expression order.  Every
+// destination is a ghost cell written by one rule and every source an
+// interior or parent cell, so the rules are independent.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_interp_map(const GRule* __restrict__ rules, int lo, int hi, int nv,
+                                                       T* __restrict__ q) {
+  const int g = lo + xcd_block() * kBlock + threadIdx.x;
+  if (g >= hi) return;
+  const GRule R = rules[g];
   for (int v = 0; v < nv; v++) {
-    const int fv = v * P.qvs, pv = v * Q.qvs;
+    const int fv = v * R.pvs, pv = v * R.cvs;
     T coarse;
-    if (ncoarse == 1) {
-      coarse = q[qidx(Q, cq[0][0], cq[0][1], st_k<DIM>(cq[0][2])) + pv];
-    } else if (ncoarse == 3) {
-      coarse = w[0] * q[qidx(Q, cq[0][0], cq[0][1], st_k<DIM>(cq[0][2])) + pv] +
-               w[1] * q[qidx(Q, cq[1][0], cq[1][1], st_k<DIM>(cq[1][2])) + pv] +
-               w[2] * q[qidx(Q, cq[2][0], cq[2][1], st_k<DIM>(cq[2][2])) + pv];
+    if (R.kind == 1) {
+      coarse = q[R.c[0] + pv];
+    } else if (R.kind == 3) {
+      coarse = (T)cB[0] * q[R.c[0] + pv] + (T)cB[1] * q[R.c[1] + pv] + (T)cB[2] * q[R.c[2] + pv];
     } else {
-      coarse = w[0] * q[qidx(Q, cq[0][0], cq[0][1], st_k<DIM>(cq[0][2])) + pv] +
-               w[1] * q[qidx(Q, cq[1][0], cq[1][1], st_k<DIM>(cq[1][2])) + pv] +
-               w[2] * q[qidx(Q, cq[2][0], cq[2][1], st_k<DIM>(cq[2][2])) + pv] +
-               w[3] * q[qidx(Q, cq[3][0], cq[3][1], st_k<DIM>(cq[3][2])) + pv];
+      coarse = (T)cC[0] * q[R.c[0] + pv] + (T)cC[1] * q[R.c[1] + pv] + (T)cC[2] * q[R.c[2] + pv] +
+               (T)cC[3] * q[R.c[3] + pv];
     }
-    q[qidx(P, g[0], g[1], st_k<DIM>(g[2])) + fv] =
-        (T)cA[0] * q[qidx(P, s1[0], s1[1], st_k<DIM>(s1[2])) + fv] +
-        (T)cA[1] * q[qidx(P, s2[0], s2[1], st_k<DIM>(s2[2])) + fv] + (T)cA[2] * coarse;
-  }
-}
-
-// InterpolateCoarseFine (boundary.f90:100-140) with Interpolate2D (:143-333)
-// and Interpolate3D (:1048-1550).  One block per patch of the level; the
-// threads enumerate the patch's ghost cells (faces, then edges, then
-// corners) and apply the rule the reference applies to that cell, including
-// the colour filter, the physical-face skip for ipass=1 in 3D, and the
-// comparison with 99 instead of 999 at boundary.f90:1104.
-template <int DIM, typename T>
-__global__ __launch_bounds__(kBlock) void k_interp(const DPatch* __restrict__ pt,
-                                                   const int* __restrict__ patches, int nv, int ipass,
-                                                   T* __restrict__ q) {
-  const DPatch P = pt[patches[blockIdx.x]];
-  const DPatch Q = pt[P.parent];
-  const int* mth = P.mthbc;
-  int fi[3][5];
-  if (DIM == 2) {
-    // faces: 4 faces x m_t cells; corners: 4
-    const int nf0 = P.mx[1], nf1 = P.mx[0];
-    const int total = 2 * nf0 + 2 * nf1 + 4;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      int g[3], s1[3], s2[3], cq[4][3];
-      T w[4];
-      int nco = 1;
-      g[2] = s1[2] = s2[2] = 1;
-      cq[0][2] = cq[1][2] = cq[2][2] = cq[3][2] = 1;
-      if (e < 2 * nf0 + 2 * nf1) {
-        int d, p, pos;
-        if (e < 2 * nf0) { d = 0; p = 1 + e / nf0; pos = e % nf0 + 1; }
-        else { int e2 = e - 2 * nf0; d = 1; p = 1 + e2 / nf1; pos = e2 % nf1 + 1; }
-        const int face = 2 * d + p - 1;
-        if (!bc_interp(mth[face])) continue;
-        const int t = 1 - d;
-        face_index(P, d, p, fi[d]);
-        const int parity = (fi[d][0] + pos) & 1;
-        if (!(ipass == 0 || parity == ipass - 1)) continue;
-        const int k = pos_kind(pos, P.mx[t]);
-        g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
-        g[t] = pos; s1[t] = pos + kind_s1(k); s2[t] = pos + kind_s2(k);
-        if (k == KE || k == KO) {
-          cq[0][d] = fi[d][3]; cq[0][t] = kind_c(P, t, k, pos, 0);
-        } else {
-          nco = 3;
-          for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); w[r] = (T)cB[r]; }
-        }
-      } else {
-        // corners (boundary.f90:155-181): applied whatever mthbc is
-        const int cn = e - (2 * nf0 + 2 * nf1);
-        const int p0 = 1 + (cn & 1), p1 = 1 + (cn >> 1);
-        face_index(P, 0, p0, fi[0]);
-        face_index(P, 1, p1, fi[1]);
-        const int parity = (fi[0][0] + fi[1][0]) & 1;
-        if (!(ipass == 0 || parity == ipass - 1)) continue;
-        for (int a = 0; a < 2; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
-      }
-      interp_store<2, T>(q, P, Q, nv, g, s1, s2, nco, cq, w);
-    }
-    return;
-  }
-  // 3D: faces (6), edges (12), corners (8)
-  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
-  const int nface[3] = {my * mz, mx * mz, mx * my};
-  const int nfaces = 2 * (nface[0] + nface[1] + nface[2]);
-  const int nedge[3] = {my, mz, mx};  // edges along dim 1 (e=0), dim 2 (e=1), dim 0 (e=2)
-  const int nedges = 4 * (nedge[0] + nedge[1] + nedge[2]);
-  const int total = nfaces + nedges + 8;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    int g[3], s1[3], s2[3], cq[4][3];
-    T w[4];
-    int nco = 1;
-    if (e < nfaces) {
-      int d = 0, rem = e;
-      while (rem >= 2 * nface[d]) { rem -= 2 * nface[d]; d++; }
-      const int p = 1 + rem / nface[d];
-      const int local = rem % nface[d];
-      const int face = 2 * d + p - 1;
-      if (!bc_interp(mth[face])) continue;
-      const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
-      const int a = local % P.mx[t1] + 1, b = local / P.mx[t1] + 1;
-      const int k1 = pos_kind(a, P.mx[t1]), k2 = pos_kind(b, P.mx[t2]);
-      const bool q1 = k1 >= KL, q2 = k2 >= KL;
-      face_index(P, d, p, fi[d]);
-      g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
-      if (q1 && q2) {
-        // corner grey cells (boundary.f90:1320-1333 and the like): always
-        int ft1[5], ft2[5];
-        face_index(P, t1, a == 1 ? 1 : 2, ft1);
-        face_index(P, t2, b == 1 ? 1 : 2, ft2);
-        g[t1] = s1[t1] = s2[t1] = ft1[2];
-        g[t2] = s1[t2] = s2[t2] = ft2[2];
-        nco = 4;
-        for (int r = 0; r < 4; r++) cq[r][d] = fi[d][3];
-        cq[0][t1] = ft1[4]; cq[0][t2] = ft2[4];
-        cq[1][t1] = ft1[3]; cq[1][t2] = ft2[4];
-        cq[2][t1] = ft1[4]; cq[2][t2] = ft2[3];
-        cq[3][t1] = ft1[3]; cq[3][t2] = ft2[3];
-        for (int r = 0; r < 4; r++) w[r] = (T)cC[r];
-      } else {
-        const int parity = (fi[d][0] + a + b) & 1;
-        if (!(ipass == 0 || parity == ipass - 1)) continue;
-        g[t1] = a; s1[t1] = a + kind_s1(k1); s2[t1] = a + kind_s2(k1);
-        g[t2] = b; s1[t2] = b + kind_s1(k2); s2[t2] = b + kind_s2(k2);
-        if (!q1 && !q2) {
-          cq[0][d] = fi[d][3]; cq[0][t1] = kind_c(P, t1, k1, a, 0); cq[0][t2] = kind_c(P, t2, k2, b, 0);
-        } else {
-          nco = 3;
-          for (int r = 0; r < 3; r++) {
-            cq[r][d] = fi[d][3];
-            cq[r][t1] = kind_c(P, t1, k1, a, q1 ? r : 0);
-            cq[r][t2] = kind_c(P, t2, k2, b, q2 ? r : 0);
-            w[r] = (T)cB[r];
-          }
-        }
-      }
-    } else if (e < nfaces + nedges) {
-      int rem = e - nfaces, ee = 0;
-      while (rem >= 4 * nedge[ee]) { rem -= 4 * nedge[ee]; ee++; }
-      const int ed = (ee == 0) ? 1 : (ee == 1 ? 2 : 0);
-      const int fa = (ed == 0) ? 1 : 0, fb = (ed == 2) ? 1 : 2;
-      const int combo = rem / nedge[ee], pos = rem % nedge[ee] + 1;
-      const int pa = 1 + (combo >> 1), pb = 1 + (combo & 1);
-      if (!bc_interp(mth[2 * fa + pa - 1]) && ipass == 1) continue;
-      const int cb = mth[2 * fb + pb - 1];
-      if (ee == 0) {
-        if (cb != 2 && cb != 99 && ipass == 1) continue;  // boundary.f90:1104
-      } else if (!bc_interp(cb) && ipass == 1) {
-        continue;
-      }
-      face_index(P, fa, pa, fi[fa]);
-      face_index(P, fb, pb, fi[fb]);
-      const int k = pos_kind(pos, P.mx[ed]);
-      g[fa] = fi[fa][0]; s1[fa] = fi[fa][1]; s2[fa] = fi[fa][2];
-      g[fb] = fi[fb][0]; s1[fb] = fi[fb][1]; s2[fb] = fi[fb][2];
-      g[ed] = pos; s1[ed] = pos + kind_s1(k); s2[ed] = pos + kind_s2(k);
-      if (k == KE || k == KO) {
-        cq[0][fa] = fi[fa][3]; cq[0][fb] = fi[fb][3]; cq[0][ed] = kind_c(P, ed, k, pos, 0);
-      } else {
-        nco = 3;
-        for (int r = 0; r < 3; r++) {
-          cq[r][fa] = fi[fa][3]; cq[r][fb] = fi[fb][3]; cq[r][ed] = kind_c(P, ed, k, pos, r);
-          w[r] = (T)cB[r];
-        }
-      }
-    } else {
-      const int cn = e - nfaces - nedges;
-      const int p1 = 1 + (cn & 1), p2 = 1 + ((cn >> 1) & 1), p3 = 1 + (cn >> 2);
-      if (!bc_interp(mth[p1 - 1]) && ipass == 1) continue;
-      if (!bc_interp(mth[p2 + 1]) && ipass == 1) continue;
-      if (!bc_interp(mth[p3 + 3]) && ipass == 1) continue;
-      face_index(P, 0, p1, fi[0]);
-      face_index(P, 1, p2, fi[1]);
-      face_index(P, 2, p3, fi[2]);
-      for (int a = 0; a < 3; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
-    }
-    interp_store<3, T>(q, P, Q, nv, g, s1, s2, nco, cq, w);
+    q[R.dst + fv] = (T)cA[0] * q[R.s1 + fv] + (T)cA[1] * q[R.s2 + fv] + (T)cA[2] * coarse;
   }
 }
 
@@ -1831,6 +1682,12 @@ struct Level {
   std::vector<int> hprefix[kNumDom];
   int total[kNumDom] = {};
   std::vector<Stage> stages;      // box stages after interpolation
+  // coarse-fine interpolation rules (levels > 0), grouped by colour-filter
+  // segment s (0: ipass 0 and 1; 1: every ipass; 2: ipass 0 and 2) and,
+  // within a segment, by patch: rules of segment s, patch p are
+  // [roff[s][p], roff[s][p+1])
+  GRule* d_rules = nullptr;
+  std::vector<int> roff[3];
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
   double h = 0.0;
@@ -2031,6 +1888,7 @@ static void free_hierarchy(bsam_ctx* c) {
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
   for (Level& L : c->lv) {
     hipFree(L.d_patches);
+    hipFree(L.d_rules);
     for (int d = 0; d < kNumDom; d++) hipFree(L.d_prefix[d]);
   }
   c->lv.clear();
@@ -2157,6 +2015,185 @@ int32_t upload_stage(bsam_ctx* c, Level& L, const std::vector<BoxDesc>& boxes) {
   s.d = d;
   s.n = (int)boxes.size();
   L.stages.push_back(s);
+  return BSAM_OK;
+}
+
+// The coarse-fine interpolation of one patch as gather rules, enumerating
+// its ghost cells as InterpolateCoarseFine does: 2D faces then corners
+// (boundary.f90:143-333), 3D faces (with the grey face-corner cells), edges
+// and corners (:1048-1550).  seg receives the rules of the three
+// colour-filter segments: a face cell refreshes only after the colour of
+// its parity (segment 0 or 2); 3D edges and corners refresh after every
+// colour except that ipass = 1 skips those touching a non-interpolated
+// face (segment 2 then, else 1), including the comparison with 99 instead
+// of 999 at boundary.f90:1104; grey cells refresh always (segment 1).
+static void interp_rules(const DPatch& P, const DPatch& Q, int ndims, std::vector<GRule> (&seg)[3]) {
+  const int* mth = P.mthbc;
+  auto k3 = [&](int k) { return ndims == 3 ? k : 0; };
+  auto emit = [&](int sg, const int g[3], const int s1[3], const int s2[3], int nco, const int (*cq)[3]) {
+    GRule r;
+    r.dst = qidx(P, g[0], g[1], k3(g[2]));
+    r.s1 = qidx(P, s1[0], s1[1], k3(s1[2]));
+    r.s2 = qidx(P, s2[0], s2[1], k3(s2[2]));
+    for (int n = 0; n < 4; n++) r.c[n] = (n < nco) ? qidx(Q, cq[n][0], cq[n][1], k3(cq[n][2])) : 0;
+    r.kind = nco;
+    r.pvs = P.qvs;
+    r.cvs = Q.qvs;
+    seg[sg].push_back(r);
+  };
+  int fi[3][5];
+  if (ndims == 2) {
+    const int nf0 = P.mx[1], nf1 = P.mx[0];
+    const int total = 2 * nf0 + 2 * nf1 + 4;
+    for (int e = 0; e < total; e++) {
+      int g[3], s1[3], s2[3], cq[4][3];
+      int nco = 1;
+      g[2] = s1[2] = s2[2] = 1;
+      for (int n = 0; n < 4; n++) cq[n][2] = 1;
+      int sg;
+      if (e < 2 * nf0 + 2 * nf1) {
+        int d, p, pos;
+        if (e < 2 * nf0) { d = 0; p = 1 + e / nf0; pos = e % nf0 + 1; }
+        else { int e2 = e - 2 * nf0; d = 1; p = 1 + e2 / nf1; pos = e2 % nf1 + 1; }
+        if (!bc_interp(mth[2 * d + p - 1])) continue;
+        const int t = 1 - d;
+        face_index(P, d, p, fi[d]);
+        sg = ((fi[d][0] + pos) & 1) ? 2 : 0;
+        const int k = pos_kind(pos, P.mx[t]);
+        g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
+        g[t] = pos; s1[t] = pos + kind_s1(k); s2[t] = pos + kind_s2(k);
+        if (k == KE || k == KO) {
+          cq[0][d] = fi[d][3]; cq[0][t] = kind_c(P, t, k, pos, 0);
+        } else {
+          nco = 3;
+          for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); }
+        }
+      } else {
+        // corners (boundary.f90:155-181): applied whatever mthbc is
+        const int cn = e - (2 * nf0 + 2 * nf1);
+        const int p0 = 1 + (cn & 1), p1 = 1 + (cn >> 1);
+        face_index(P, 0, p0, fi[0]);
+        face_index(P, 1, p1, fi[1]);
+        sg = ((fi[0][0] + fi[1][0]) & 1) ? 2 : 0;
+        for (int a = 0; a < 2; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
+      }
+      emit(sg, g, s1, s2, nco, cq);
+    }
+    return;
+  }
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  const int nface[3] = {my * mz, mx * mz, mx * my};
+  const int nfaces = 2 * (nface[0] + nface[1] + nface[2]);
+  const int nedge[3] = {my, mz, mx};  // edges along dim 1 (ee=0), dim 2 (ee=1), dim 0 (ee=2)
+  const int nedges = 4 * (nedge[0] + nedge[1] + nedge[2]);
+  const int total = nfaces + nedges + 8;
+  for (int e = 0; e < total; e++) {
+    int g[3], s1[3], s2[3], cq[4][3];
+    int nco = 1, sg;
+    if (e < nfaces) {
+      int d = 0, rem = e;
+      while (rem >= 2 * nface[d]) { rem -= 2 * nface[d]; d++; }
+      const int p = 1 + rem / nface[d];
+      const int local = rem % nface[d];
+      if (!bc_interp(mth[2 * d + p - 1])) continue;
+      const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+      const int a = local % P.mx[t1] + 1, b = local / P.mx[t1] + 1;
+      const int k1 = pos_kind(a, P.mx[t1]), k2 = pos_kind(b, P.mx[t2]);
+      const bool q1 = k1 >= KL, q2 = k2 >= KL;
+      face_index(P, d, p, fi[d]);
+      g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
+      if (q1 && q2) {
+        // face-corner grey cells (boundary.f90:1320-1333 and the like)
+        int ft1[5], ft2[5];
+        face_index(P, t1, a == 1 ? 1 : 2, ft1);
+        face_index(P, t2, b == 1 ? 1 : 2, ft2);
+        g[t1] = s1[t1] = s2[t1] = ft1[2];
+        g[t2] = s1[t2] = s2[t2] = ft2[2];
+        nco = 4;
+        for (int r = 0; r < 4; r++) cq[r][d] = fi[d][3];
+        cq[0][t1] = ft1[4]; cq[0][t2] = ft2[4];
+        cq[1][t1] = ft1[3]; cq[1][t2] = ft2[4];
+        cq[2][t1] = ft1[4]; cq[2][t2] = ft2[3];
+        cq[3][t1] = ft1[3]; cq[3][t2] = ft2[3];
+        sg = 1;
+      } else {
+        sg = ((fi[d][0] + a + b) & 1) ? 2 : 0;
+        g[t1] = a; s1[t1] = a + kind_s1(k1); s2[t1] = a + kind_s2(k1);
+        g[t2] = b; s1[t2] = b + kind_s1(k2); s2[t2] = b + kind_s2(k2);
+        if (!q1 && !q2) {
+          cq[0][d] = fi[d][3]; cq[0][t1] = kind_c(P, t1, k1, a, 0); cq[0][t2] = kind_c(P, t2, k2, b, 0);
+        } else {
+          nco = 3;
+          for (int r = 0; r < 3; r++) {
+            cq[r][d] = fi[d][3];
+            cq[r][t1] = kind_c(P, t1, k1, a, q1 ? r : 0);
+            cq[r][t2] = kind_c(P, t2, k2, b, q2 ? r : 0);
+          }
+        }
+      }
+    } else if (e < nfaces + nedges) {
+      int rem = e - nfaces, ee = 0;
+      while (rem >= 4 * nedge[ee]) { rem -= 4 * nedge[ee]; ee++; }
+      const int ed = (ee == 0) ? 1 : (ee == 1 ? 2 : 0);
+      const int fa = (ed == 0) ? 1 : 0, fb = (ed == 2) ? 1 : 2;
+      const int combo = rem / nedge[ee], pos = rem % nedge[ee] + 1;
+      const int pa = 1 + (combo >> 1), pb = 1 + (combo & 1);
+      bool skip1 = !bc_interp(mth[2 * fa + pa - 1]);
+      const int cb = mth[2 * fb + pb - 1];
+      if (ee == 0) skip1 = skip1 || (cb != 2 && cb != 99);  // boundary.f90:1104
+      else skip1 = skip1 || !bc_interp(cb);
+      sg = skip1 ? 2 : 1;
+      face_index(P, fa, pa, fi[fa]);
+      face_index(P, fb, pb, fi[fb]);
+      const int k = pos_kind(pos, P.mx[ed]);
+      g[fa] = fi[fa][0]; s1[fa] = fi[fa][1]; s2[fa] = fi[fa][2];
+      g[fb] = fi[fb][0]; s1[fb] = fi[fb][1]; s2[fb] = fi[fb][2];
+      g[ed] = pos; s1[ed] = pos + kind_s1(k); s2[ed] = pos + kind_s2(k);
+      if (k == KE || k == KO) {
+        cq[0][fa] = fi[fa][3]; cq[0][fb] = fi[fb][3]; cq[0][ed] = kind_c(P, ed, k, pos, 0);
+      } else {
+        nco = 3;
+        for (int r = 0; r < 3; r++) {
+          cq[r][fa] = fi[fa][3]; cq[r][fb] = fi[fb][3]; cq[r][ed] = kind_c(P, ed, k, pos, r);
+        }
+      }
+    } else {
+      const int cn = e - nfaces - nedges;
+      const int p1 = 1 + (cn & 1), p2 = 1 + ((cn >> 1) & 1), p3 = 1 + (cn >> 2);
+      const bool skip1 = !bc_interp(mth[p1 - 1]) || !bc_interp(mth[p2 + 1]) || !bc_interp(mth[p3 + 3]);
+      sg = skip1 ? 2 : 1;
+      face_index(P, 0, p1, fi[0]);
+      face_index(P, 1, p2, fi[1]);
+      face_index(P, 2, p3, fi[2]);
+      for (int a = 0; a < 3; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
+    }
+    emit(sg, g, s1, s2, nco, cq);
+  }
+}
+
+static int32_t build_interp_rules(bsam_ctx* c, Level& L) {
+  if (L.level <= 0) return BSAM_OK;
+  std::vector<GRule> all;
+  const size_t np = L.patches.size();
+  std::vector<std::vector<GRule>> per[3];
+  for (int sg = 0; sg < 3; sg++) per[sg].resize(np);
+  for (size_t a = 0; a < np; a++) {
+    const DPatch& P = c->hp[L.patches[a]];
+    std::vector<GRule> seg[3];
+    interp_rules(P, c->hp[P.parent], c->p.ndims, seg);
+    for (int sg = 0; sg < 3; sg++) per[sg][a].swap(seg[sg]);
+  }
+  for (int sg = 0; sg < 3; sg++) {
+    L.roff[sg].assign(np + 1, 0);
+    for (size_t a = 0; a < np; a++) {
+      L.roff[sg][a] = (int)all.size();
+      all.insert(all.end(), per[sg][a].begin(), per[sg][a].end());
+    }
+    L.roff[sg][np] = (int)all.size();
+  }
+  if (all.empty()) return BSAM_OK;
+  HIPCHK(hipMalloc(&L.d_rules, all.size() * sizeof(GRule)));
+  HIPCHK(hipMemcpy(L.d_rules, all.data(), all.size() * sizeof(GRule), hipMemcpyHostToDevice));
   return BSAM_OK;
 }
 
@@ -2381,6 +2418,7 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
     }
     L.qspan_len = contiguous ? end - L.qspan_off : -1;
     RETCHK(build_ghost_stages(c, L));
+    RETCHK(build_interp_rules(c, L));
   }
   // coarse chains: from the highest level whose patch has at most 16^3
   // (2D: 64^2) cells down to minlevel, every level a single patch periodic
@@ -2536,14 +2574,25 @@ template <typename T>
 static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog) {
   Level& L = c->L(level);
   const int nv = c->p.nrvars;
-  if (level > 0) {
-    // one workgroup per patch; per-patch mode launches them one by one
+  if (level > 0 && L.d_rules) {
+    // colour-filter segments this ipass refreshes: 0 -> all, 1 -> 0..1,
+    // 2 -> 1..2
+    const int s0 = (ipass == 2) ? 1 : 0, s1 = (ipass == 1) ? 1 : 2;
     const int np = (int)L.patches.size();
-    const int per = c->p.batching ? np : 1;
-    for (int p0 = 0; p0 < np; p0 += per) {
-      if (c->p.ndims == 2) k_interp<2, T><<<per, kBlock, 0, c->s>>>(c->d_pt, L.d_patches + p0, nv, ipass, q);
-      else k_interp<3, T><<<per, kBlock, 0, c->s>>>(c->d_pt, L.d_patches + p0, nv, ipass, q);
-      LAUNCH_CHECK();
+    if (c->p.batching) {
+      const int lo = L.roff[s0][0], hi = L.roff[s1][np];
+      if (hi > lo) {
+        k_interp_map<T><<<nblocks(hi - lo), kBlock, 0, c->s>>>(L.d_rules, lo, hi, nv, q);
+        LAUNCH_CHECK();
+      }
+    } else {
+      for (int a = 0; a < np; a++)
+        for (int sg = s0; sg <= s1; sg++) {
+          const int lo = L.roff[sg][a], hi = L.roff[sg][a + 1];
+          if (hi <= lo) continue;
+          k_interp_map<T><<<nblocks(hi - lo), kBlock, 0, c->s>>>(L.d_rules, lo, hi, nv, q);
+          LAUNCH_CHECK();
+        }
     }
   }
   if (L.wrap_pid >= 0) {
