@@ -93,10 +93,21 @@ struct LevelArgs {
   const DPatch* __restrict__ pt;
   const int* __restrict__ patches;  // descriptor indices of the level
   const int* __restrict__ prefix;   // npatch+1 prefix sums of the kernel's domain
+  const int* __restrict__ bp;       // patch holding the first element of each block (whole-level launches)
   int npatch;
   int total;  // end of the launch's range of the domain
   int gbase;  // start of the range: 0 for a whole level, prefix[p] for patch p alone
+  int fixed;  // the patch of a one-patch launch, else -1
 };
+
+// Patch (position in the level's list) of domain element g: the launch's
+// single patch, or a short forward scan from the patch the block starts in.
+__device__ __forceinline__ int level_patch(const LevelArgs& L, int g) {
+  if (L.fixed >= 0) return L.fixed;
+  int pi = L.bp[g / kBlock];
+  while (pi + 1 < L.npatch && L.prefix[pi + 1] <= g) pi++;
+  return pi;
+}
 
 // ---------------------------------------------------------------------------
 // CH relaxation: RelaxGrid2D/3D, CH/problem.f90:318-369, 554-614.
@@ -132,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double 
                                                      const double* __restrict__ f) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
@@ -173,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double 
 }
 
 // 3D colour sweep of one large patch, marching in k.  A block owns a tile
-// of 64 half-row positions x 8 rows and KC planes; the other colour's phi
+// of 64 half-row positions x 16 rows and KC planes; the other colour's phi
 // and mu of planes k-1, k, k+1 (tile plus one halo row each side and one
 // halo position) sit in a 4-slot LDS ring, so every value of the other
 // colour is read from HBM about once, and the own colour's phi and f once.
@@ -181,11 +192,18 @@ __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double 
 // LIN=true:  the linearised correction smoother (FP32) with rhs and coef.
 // WRAP: also write the periodic ghost images of the updated cells.
 // FINEST only separates the finest level's launches in kernel traces.
-template <typename T, bool LIN, bool WRAP, int FINEST>
-__global__ __launch_bounds__(512) void k_relax3d_march(const DPatch* __restrict__ pt, int pid, int c, int KC,
-                                                      T tmp1, T tmp2, T tmp3, T a12, T* __restrict__ q,
-                                                      const T* __restrict__ f, const T* __restrict__ coef) {
-  constexpr int TX = 64, TY = 8, NR = TY + 2, NP = TX + 1;
+// EMODE (LIN only) 1: the correction starts from zero, so the other
+// colour's values are taken as zero and not read; 2: the last sweep of the
+// cycle adds the correction into the FP64 solution q64 for both cells of
+// each thread's pair (q64 += e, ghost images included) instead of storing e.
+constexpr int kSweepTY = 16, kSweepThreads = 64 * kSweepTY;
+template <typename T, bool LIN, bool WRAP, int FINEST, int EMODE = 0>
+__global__ __launch_bounds__(kSweepThreads) void k_relax3d_march(const DPatch* __restrict__ pt, int pid, int c,
+                                                                int KC, T tmp1, T tmp2, T tmp3, T a12,
+                                                                T* __restrict__ q, const T* __restrict__ f,
+                                                                const T* __restrict__ coef,
+                                                                double* __restrict__ q64 = nullptr) {
+  constexpr int TX = 64, TY = kSweepTY, NR = TY + 2, NP = TX + 1, NT = kSweepThreads;
   __shared__ T sh[4][2][NR][NP];
   const DPatch P = pt[pid];
   const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
@@ -203,17 +221,17 @@ __global__ __launch_bounds__(512) void k_relax3d_march(const DPatch* __restrict_
   const int hq = P.hq, ny = P.nq[1], nz = P.nq[2], vs = P.qvs, base = P.qoff;
   auto rowoff = [&](int col, int k, int j) { return base + ((col * nz + k) * ny + j) * hq; };
   // One plane of the other colour's tile is NE entries; each thread owns
-  // the entries threadIdx.x + 512*n.  Loads for plane k+3 are issued at the
+  // the entries threadIdx.x + NT*n.  Loads for plane k+3 are issued at the
   // end of iteration k into registers and stored to LDS during iteration
   // k+1, so their latency overlaps a plane of arithmetic.
   constexpr int NE = 2 * NR * NP;
-  constexpr int NPER = (NE + 511) / 512;
+  constexpr int NPER = (NE + NT - 1) / NT;
   auto fetch = [&](int k, T (&reg)[NPER]) {
 #pragma unroll
     for (int n = 0; n < NPER; n++) {
-      const int e = threadIdx.x + 512 * n;
+      const int e = threadIdx.x + NT * n;
       T val = (T)0;
-      if (e < NE) {
+      if (EMODE != 1 && e < NE) {
         const int v = e / (NR * NP);
         const int r = (e / NP) % NR;
         const int p = e % NP;
@@ -227,7 +245,7 @@ __global__ __launch_bounds__(512) void k_relax3d_march(const DPatch* __restrict_
     T* sl = &sh[k & 3][0][0][0];
 #pragma unroll
     for (int n = 0; n < NPER; n++) {
-      const int e = threadIdx.x + 512 * n;
+      const int e = threadIdx.x + NT * n;
       if (e < NE) sl[e] = reg[n];
     }
   };
@@ -287,9 +305,25 @@ __global__ __launch_bounds__(512) void k_relax3d_march(const DPatch* __restrict_
         n0 = (b1 - a12 * b2) / det;
         n1 = (b2 - a21 * b1) / det;
       }
-      q[o] = n0;
-      q[o + vs] = n1;
-      if (WRAP) wrap_images<3, T>(q, P, 2 * t + 2 - ib, j, k, n0, n1);
+      if (EMODE == 2) {
+        // own cell: q64 += the new correction; the pair's other cell:
+        // q64 += its correction, final since the other colour's sweep
+        const double a0 = q64[o] + (double)n0, a1 = q64[o + vs] + (double)n1;
+        q64[o] = a0;
+        q64[o + vs] = a1;
+        if (WRAP) wrap_images<3, double>(q64, P, 2 * t + 2 - ib, j, k, a0, a1);
+        const int io = 2 * t + 1 + ib;
+        const int oo = rowoff(cc, k, j) + (io >> 1);
+        const double b0 = q64[oo] + (double)sh[sk][0][r][tx + ib];
+        const double b1 = q64[oo + vs] + (double)sh[sk][1][r][tx + ib];
+        q64[oo] = b0;
+        q64[oo + vs] = b1;
+        if (WRAP) wrap_images<3, double>(q64, P, io, j, k, b0, b1);
+      } else {
+        q[o] = n0;
+        q[o + vs] = n1;
+        if (WRAP) wrap_images<3, T>(q, P, 2 * t + 2 - ib, j, k, n0, n1);
+      }
     }
     if (k + 2 <= k1 + 1) {
       store(k + 2, pre);
@@ -309,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, Model
                                                        const double* __restrict__ f) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
@@ -407,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
   double s0 = 0.0, s1 = 0.0;
   const int g = L.gbase + lb * kBlock + threadIdx.x;
   if (g < L.total) {
-    const int pi = find_patch(L.prefix, L.npatch, g);
+    const int pi = level_patch(L, g);
     const DPatch P = L.pt[L.patches[pi]];
     const int local = g - L.prefix[pi];
     const int i = local % P.mx[0] + 1;
@@ -747,7 +781,7 @@ struct CoarseCell {
 template <int DIM>
 __device__ __forceinline__ bool decode_coarse(const LevelArgs& L, int g, CoarseCell& cc) {
   if (g >= L.total) return false;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   cc.pid = L.patches[pi];
   const DPatch P = L.pt[cc.pid];
   const int local = g - L.prefix[pi];
@@ -793,7 +827,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, con
                                                         T* __restrict__ qc) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const DPatch Q = L.pt[P.parent];
   const int local = g - L.prefix[pi];
@@ -868,7 +902,7 @@ __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, cons
                                                        T* __restrict__ qc) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const DPatch Q = L.pt[P.parent];
   const int local = g - L.prefix[pi];
@@ -901,7 +935,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T*
                                                          const T* __restrict__ qc) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int n0 = P.cmx[0] + 1, n1 = P.cmx[1] + 1;
@@ -988,7 +1022,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* 
                                                         const T* __restrict__ qc) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int i = local % P.nq[0];
@@ -1028,7 +1062,7 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double
                                                    double* __restrict__ ftmp, double* __restrict__ f) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int i = local % P.mx[0] + 1;
@@ -1235,7 +1269,7 @@ __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __re
                                                  float* __restrict__ coef) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int i = local % P.mx[0] + 1;
@@ -1277,7 +1311,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
                                                       const float* __restrict__ coef) {
   const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
   if (g >= L.total) return;
-  const int pi = find_patch(L.prefix, L.npatch, g);
+  const int pi = level_patch(L, g);
   const DPatch P = L.pt[L.patches[pi]];
   const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
@@ -1679,6 +1713,7 @@ struct Level {
   std::vector<int> patches;
   int* d_patches = nullptr;
   int* d_prefix[kNumDom] = {};
+  int* d_bp[kNumDom] = {};
   std::vector<int> hprefix[kNumDom];
   int total[kNumDom] = {};
   std::vector<Stage> stages;      // box stages after interpolation
@@ -1758,9 +1793,11 @@ LevelArgs largs(bsam_ctx* c, Level& L, Dom d) {
   a.pt = c->d_pt;
   a.patches = L.d_patches;
   a.prefix = L.d_prefix[d];
+  a.bp = L.d_bp[d];
   a.npatch = (int)L.patches.size();
   a.total = L.total[d];
   a.gbase = 0;
+  a.fixed = -1;
   return a;
 }
 
@@ -1793,6 +1830,7 @@ struct LaunchIter {
       a = full;
       a.gbase = L.hprefix[d][i];
       a.total = L.hprefix[d][i + 1];
+      a.fixed = (int)i;
       i++;
       nb = nblocks((long)a.total - a.gbase);
       if (nb > 0) {
@@ -1889,7 +1927,10 @@ static void free_hierarchy(bsam_ctx* c) {
   for (Level& L : c->lv) {
     hipFree(L.d_patches);
     hipFree(L.d_rules);
-    for (int d = 0; d < kNumDom; d++) hipFree(L.d_prefix[d]);
+    for (int d = 0; d < kNumDom; d++) {
+      hipFree(L.d_prefix[d]);
+      hipFree(L.d_bp[d]);
+    }
   }
   c->lv.clear();
   for (BoxDesc* b : c->box_allocs) hipFree(b);
@@ -2403,6 +2444,16 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
       L.hprefix[d] = pre;
       HIPCHK(hipMalloc(&L.d_prefix[d], pre.size() * sizeof(int)));
       HIPCHK(hipMemcpy(L.d_prefix[d], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+      // block -> first patch, for whole-level launches
+      std::vector<int> bp(std::max(1, nblocks(acc)), 0);
+      size_t pi = 0;
+      for (size_t b = 0; b < bp.size(); b++) {
+        const long g0 = (long)b * kBlock;
+        while (pi + 1 < L.patches.size() && pre[pi + 1] <= g0) pi++;
+        bp[b] = (int)pi;
+      }
+      HIPCHK(hipMalloc(&L.d_bp[d], bp.size() * sizeof(int)));
+      HIPCHK(hipMemcpy(L.d_bp[d], bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     // per-patch launches round every patch up to whole blocks
     maxblocks = std::max(maxblocks, nblocks(L.total[kCell]) + (int)L.patches.size());
@@ -2620,8 +2671,27 @@ extern "C" int32_t bsam_set_ghost(bsam_ctx* c, int32_t level, int32_t ipass) {
   return set_ghost_t<double>(c, level, ipass, c->q, 0);
 }
 
-// Launch shape of the marching 3D sweep for a single large patch, or 0 when
-// the level uses the flat kernel.
+// Launch shape of the marching 3D colour sweep (tiles of 64 half-rows x
+// kSweepTY rows) for a single large patch, or 0 when the level uses the
+// flat kernel.  Two 1024-thread workgroups fit on a CU; the plane count per
+// workgroup is chosen for about 1024 workgroups in FP64 and 512 in FP32,
+// the fastest of the shapes measured on the 256^3 patch.
+static int sweep_kc(bsam_ctx* c, const Level& L, int elem_bytes, int* nblk) {
+  if (c->p.ndims != 3 || L.patches.size() != 1) return 0;
+  const DPatch& P = c->hp[L.patches[0]];
+  const int nx2 = P.mx[0] / 2;
+  if (nx2 < 64 || P.mx[1] < kSweepTY || (P.mx[0] & 1)) return 0;
+  const int ntile = ((nx2 + 63) / 64) * ((P.mx[1] + kSweepTY - 1) / kSweepTY);
+  const int target = (elem_bytes == 8) ? 1024 : 512;
+  int nkc = std::max(1, (target + ntile - 1) / ntile);
+  int kc = std::max(4, (P.mx[2] + nkc - 1) / nkc);
+  nkc = (P.mx[2] + kc - 1) / kc;
+  *nblk = ntile * nkc;
+  return kc;
+}
+
+// Launch shape of the marching residual kernels (tiles of 64 half-rows x 8
+// rows, 512 threads, about 768 workgroups), or 0.
 static int march_kc(bsam_ctx* c, const Level& L, int* nblk) {
   if (c->p.ndims != 3 || L.patches.size() != 1) return 0;
   const DPatch& P = c->hp[L.patches[0]];
@@ -2643,7 +2713,7 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass) {
   Level& L = c->L(level);
   const bool wrap = L.wrap_pid >= 0;
   int nblk = 0;
-  const int kc = (c->p.model == BSAM_MODEL_CH) ? march_kc(c, L, &nblk) : 0;
+  const int kc = (c->p.model == BSAM_MODEL_CH) ? sweep_kc(c, L, 8, &nblk) : 0;
   if (!kc && !(wrap && c->p.model == BSAM_MODEL_CH)) {
     RETCHK(bsam_relax_level(c, level, ipass));
     return set_ghost_t<double>(c, level, ipass, c->q, 0);
@@ -2656,11 +2726,11 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass) {
     const double tmp1 = c->p.dt / h2, tmp2 = eps2 / h2, tmp3 = 6.0 * eps2 / h2, a12 = 6.0 * tmp1;
     const int pid = L.patches[0];
     if (wrap) {
-      if (fin) k_relax3d_march<double, false, true, 1><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
-      else k_relax3d_march<double, false, true, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+      if (fin) k_relax3d_march<double, false, true, 1><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+      else k_relax3d_march<double, false, true, 0><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
     } else {
-      if (fin) k_relax3d_march<double, false, false, 1><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
-      else k_relax3d_march<double, false, false, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+      if (fin) k_relax3d_march<double, false, false, 1><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
+      else k_relax3d_march<double, false, false, 0><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, col, kc, tmp1, tmp2, tmp3, a12, c->q, c->f, nullptr);
     }
   } else {
     LevelArgs a = largs(c, L, kHalf);
@@ -3205,10 +3275,22 @@ void mixed_free(MixedState* m) {
 
 namespace {
 
+// True when the correction cycle's first and last finest-level sweeps take
+// over the zeroing of e and the update q += e: a single level (finest = 0)
+// that is one large patch periodic in every dimension, swept by the
+// marching kernel.
+static bool mixed_fused(bsam_ctx* c) {
+  if (c->finest != 0 || c->p.nsmoothingpasses < 1) return false;
+  Level& L = c->L(0);
+  int nblk = 0;
+  return L.wrap_pid >= 0 && sweep_kc(c, L, 4, &nblk) > 0;
+}
+
 // One colour sweep of the correction smoother; returns 1 when the sweep
 // also wrote the periodic ghost images (the following SetGhost is then
-// redundant), 0 otherwise.
-int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
+// redundant), 0 otherwise.  emode 1: the first sweep from e = 0; emode 2:
+// the last sweep, adding the correction into q (mixed_fused levels only).
+int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
   bsam::MixedState* m = c->mixed;
   Level& L = c->L(level);
   const double h2 = L.h * L.h, eps2 = c->p.model_params[0];
@@ -3218,15 +3300,20 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
   const bool fin = level == c->finest;
   const bool wrap = L.wrap_pid >= 0;
   int nblk = 0;
-  const int kc = march_kc(c, L, &nblk);
-  if (fin) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
+  const int kc = sweep_kc(c, L, 4, &nblk);
+  const bool timed = fin && emode == 0;
+  if (timed) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
   if (kc) {
     const int pid = L.patches[0];
     if (wrap) {
-      if (fin) k_relax3d_march<float, true, true, 1><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
-      else k_relax3d_march<float, true, true, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+      if (emode == 1)
+        k_relax3d_march<float, true, true, 1, 1><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+      else if (emode == 2)
+        k_relax3d_march<float, true, true, 1, 2><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, c->q);
+      else if (fin) k_relax3d_march<float, true, true, 1><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+      else k_relax3d_march<float, true, true, 0><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
     } else {
-      k_relax3d_march<float, true, false, 0><<<nblk, 512, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+      k_relax3d_march<float, true, false, 0><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
     }
   } else {
     FOR_LAUNCHES(c, L, kHalf, a, nb) {
@@ -3242,7 +3329,7 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
     }
   }
   LAUNCH_CHECK();
-  if (fin) {
+  if (timed) {
     double N = 0, G = 0;
     for (int pid : L.patches) {
       const DPatch& P = c->hp[pid];
@@ -3250,17 +3337,24 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass) {
       G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
                              : 2.0 * ((double)P.mx[0] + P.mx[1]);
     }
-    // e_phi over N+G, e_mu of the other colour (N+G)/2, rhs (2 fields) and
-    // coef of the colour N/2 each, the colour's e written N/2 x 2 fields
-    kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + 0.5 * (N + G) + N + 0.5 * N + N));
+    // e_phi and e_mu of the other colour over (N+G)/2 each, rhs (2 fields)
+    // and coef of the colour N/2 each, the colour's e written N/2 x 2
+    // fields (the correction smoother does not read the cell's own e)
+    kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + N + 0.5 * N + N));
   }
   return wrap ? 1 : 0;
 }
 
-int32_t mixed_level_relax(bsam_ctx* c, int level) {
-  for (int s = 0; s < c->p.nsmoothingpasses; s++)
+// nsmoothingpasses red-black passes; phase 0 = pre-smoothing, 1 = post.
+int32_t mixed_level_relax(bsam_ctx* c, int level, int phase) {
+  const bool fused = level == c->finest && mixed_fused(c);
+  const int ns = c->p.nsmoothingpasses;
+  for (int s = 0; s < ns; s++)
     for (int rb = 1; rb <= 2; rb++) {
-      const int32_t done = mixed_relax(c, level, rb);
+      int emode = 0;
+      if (fused && phase == 0 && s == 0 && rb == 1) emode = 1;
+      if (fused && phase == 1 && s == ns - 1 && rb == 2) emode = 2;
+      const int32_t done = mixed_relax(c, level, rb, emode);
       RETCHK(done);
       if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1));
     }
@@ -3270,7 +3364,7 @@ int32_t mixed_level_relax(bsam_ctx* c, int level) {
 int32_t mixed_vcycle(bsam_ctx* c, int level) {
   bsam::MixedState* m = c->mixed;
   if (c->L(level).chain > 0) return launch_coarse_chain(c, level, true);
-  RETCHK(mixed_level_relax(c, level));
+  RETCHK(mixed_level_relax(c, level, 0));
   if (level > c->minlevel) {
     const bool rw = restrict_wraps_parent(c, level);
     RETCHK(restrict_t<float>(c, level, m->e, m->e, m->ec, rw));
@@ -3287,7 +3381,7 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
     const bool cw = c->L(level).wrap_pid >= 0;
     RETCHK(correct_t<float>(c, level, m->e, m->ec, cw));
     if (!cw) RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1));
-    RETCHK(mixed_level_relax(c, level));
+    RETCHK(mixed_level_relax(c, level, 1));
   }
   return BSAM_OK;
 }
@@ -3349,7 +3443,8 @@ int32_t mixed_iteration(bsam_ctx* c) {
     }
   }
   RETCHK(mixed_vcycle(c, c->finest));
-  for (int level = 0; level <= c->finest; level++) {
+  // with the fused finest sweeps, q += e has been done by the last sweep
+  for (int level = 0; level <= c->finest && !mixed_fused(c); level++) {
     Level& L = c->L(level);
     if (L.qspan_len > 0) {
       k_add_corr<<<nblocks(L.qspan_len), kBlock, 0, c->s>>>(c->q + L.qspan_off, m->e + L.qspan_off, L.qspan_len);
