@@ -1142,24 +1142,63 @@ struct GRule {
 // destination is a ghost cell written by one rule and every source an
 // interior or parent cell, so the rules are independent.
 template <typename T>
+__device__ __forceinline__ T interp_value(const GRule& R, int kind, int v, const T* __restrict__ q) {
+  const int fv = v * R.pvs, pv = v * R.cvs;
+  T coarse;
+  if (kind == 1) {
+    coarse = q[R.c[0] + pv];
+  } else if (kind == 3) {
+    coarse = (T)cB[0] * q[R.c[0] + pv] + (T)cB[1] * q[R.c[1] + pv] + (T)cB[2] * q[R.c[2] + pv];
+  } else {
+    coarse = (T)cC[0] * q[R.c[0] + pv] + (T)cC[1] * q[R.c[1] + pv] + (T)cC[2] * q[R.c[2] + pv] +
+             (T)cC[3] * q[R.c[3] + pv];
+  }
+  return (T)cA[0] * q[R.s1 + fv] + (T)cA[1] * q[R.s2 + fv] + (T)cA[2] * coarse;
+}
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_interp_map(const GRule* __restrict__ rules, int lo, int hi, int nv,
                                                        T* __restrict__ q) {
   const int g = lo + xcd_block() * kBlock + threadIdx.x;
   if (g >= hi) return;
   const GRule R = rules[g];
-  for (int v = 0; v < nv; v++) {
-    const int fv = v * R.pvs, pv = v * R.cvs;
-    T coarse;
-    if (R.kind == 1) {
-      coarse = q[R.c[0] + pv];
-    } else if (R.kind == 3) {
-      coarse = (T)cB[0] * q[R.c[0] + pv] + (T)cB[1] * q[R.c[1] + pv] + (T)cB[2] * q[R.c[2] + pv];
-    } else {
-      coarse = (T)cC[0] * q[R.c[0] + pv] + (T)cC[1] * q[R.c[1] + pv] + (T)cC[2] * q[R.c[2] + pv] +
-               (T)cC[3] * q[R.c[3] + pv];
-    }
-    q[R.dst + fv] = (T)cA[0] * q[R.s1 + fv] + (T)cA[1] * q[R.s2 + fv] + (T)cA[2] * coarse;
+  for (int v = 0; v < nv; v++) q[R.dst + v * R.pvs] = interp_value<T>(R, R.kind, v, q);
+}
+
+// Post-operations of a flattened ghost rule, two bits each from the low
+// end, applied in that order: 1 adds +0.0 (the qpo term of an x wrap), 2
+// takes the Dirichlet image 2 - x (-x for a correction, `homog`), 3 does
+// that on variables >= 1 only (the BioGrowth boundary condition).
+template <typename T>
+__device__ __forceinline__ T ghost_post(T x, int post, int v, int homog) {
+  for (; post; post >>= 2) {
+    const int op = post & 3;
+    if (op == 1) x = x + (T)0;
+    else if (op == 2 || (op == 3 && v >= 1)) x = homog ? -x : (T)2 - x;
   }
+  return x;
+}
+
+// One whole SetGhost of a level from its flattened rules: every rule
+// writes one ghost cell (all variables) from interior or parent cells, so
+// the rules are independent and one launch applies them all.  Copy rules
+// are (dst, src, dst variable stride, src variable stride); interpolation
+// rules carry their post-operations in bits 8-15 of `kind`.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ cr, const uint8_t* __restrict__ cpost,
+                                                      int nc, const GRule* __restrict__ ir, int ni, int nv,
+                                                      int homog, T* __restrict__ q) {
+  const int g = xcd_block() * kBlock + threadIdx.x;
+  if (g < nc) {
+    const int4 r = cr[g];
+    const int post = cpost ? cpost[g] : 0;
+    for (int v = 0; v < nv; v++) q[r.x + v * r.z] = ghost_post<T>(q[r.y + v * r.w], post, v, homog);
+    return;
+  }
+  if (g >= nc + ni) return;
+  const GRule R = ir[g - nc];
+  const int kind = R.kind & 0xff, post = R.kind >> 8;
+  for (int v = 0; v < nv; v++) q[R.dst + v * R.pvs] = ghost_post<T>(interp_value<T>(R, kind, v, q), post, v, homog);
 }
 
 // Box copies of one ghost-fill stage: sibling transfers (GhostOverlap,
@@ -1708,6 +1747,17 @@ struct Stage {
   int n = 0;
 };
 
+// A flattened ghost fill on the device: copy rules (dst, src, dst variable
+// stride, src variable stride) with optional post-operation chains, then
+// interpolation rules.
+struct DevMap {
+  int4* cr = nullptr;
+  uint8_t* cpost = nullptr;
+  GRule* ir = nullptr;
+  int nc = 0, ni = 0;
+  int nc1 = 0, ni1 = 0;  // first launch: rules reading ghost values from before the call
+};
+
 struct Level {
   int level = 0;
   std::vector<int> patches;
@@ -1723,6 +1773,12 @@ struct Level {
   // [roff[s][p], roff[s][p+1])
   GRule* d_rules = nullptr;
   std::vector<int> roff[3];
+  // the same ghost fill flattened to one rule per ghost cell (levels > 0,
+  // per-level launches): gmap[ipass] is SetGhost(level, ipass) in full,
+  // gmap[2 + ipass] (ipass 1, 2) its face ghosts a red-black sweep
+  // sequence needs (see plan_ghost_maps)
+  DevMap gmap[5];
+  bool maps_ok = false;
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
   double h = 0.0;
@@ -1927,6 +1983,11 @@ static void free_hierarchy(bsam_ctx* c) {
   for (Level& L : c->lv) {
     hipFree(L.d_patches);
     hipFree(L.d_rules);
+    for (DevMap& m : L.gmap) {
+      hipFree(m.cr);
+      hipFree(m.cpost);
+      hipFree(m.ir);
+    }
     for (int d = 0; d < kNumDom; d++) {
       hipFree(L.d_prefix[d]);
       hipFree(L.d_bp[d]);
@@ -2046,18 +2107,24 @@ void periodic_offsets(const bsam_ctx* c, int level, std::vector<std::array<int, 
   }
 }
 
-int32_t upload_stage(bsam_ctx* c, Level& L, const std::vector<BoxDesc>& boxes) {
-  if (boxes.empty()) return BSAM_OK;
-  BoxDesc* d = nullptr;
-  HIPCHK(hipMalloc(&d, boxes.size() * sizeof(BoxDesc)));
-  c->box_allocs.push_back(d);
-  HIPCHK(hipMemcpy(d, boxes.data(), boxes.size() * sizeof(BoxDesc), hipMemcpyHostToDevice));
-  Stage s;
-  s.d = d;
-  s.n = (int)boxes.size();
-  L.stages.push_back(s);
-  return BSAM_OK;
-}
+// Host plan of one level's SetGhost, built by plan_ghost_* and uploaded by
+// upload_ghost: the staged form (interpolation rules, then box stages in
+// the reference's order) and, where it is valid, the flattened maps.
+struct HostMap {
+  std::vector<int4> copies;    // dst, src, dst variable stride, src variable stride
+  std::vector<uint8_t> cpost;  // post-operation chain per copy (empty: none anywhere)
+  std::vector<GRule> interps;  // kind | post << 8
+  int nc1 = 0, ni1 = 0;        // leading rules that read ghost values from before the call
+};
+struct GhostPlan {
+  int wrap_pid = -1;
+  std::vector<std::vector<BoxDesc>> stages;
+  std::vector<GRule> rules;
+  std::vector<int> roff[3];
+  bool maps_ok = false;
+  HostMap map[5];
+  std::string why;  // why the flattened maps are not used
+};
 
 // The coarse-fine interpolation of one patch as gather rules, enumerating
 // its ghost cells as InterpolateCoarseFine does: 2D faces then corners
@@ -2212,10 +2279,11 @@ static void interp_rules(const DPatch& P, const DPatch& Q, int ndims, std::vecto
   }
 }
 
-static int32_t build_interp_rules(bsam_ctx* c, Level& L) {
-  if (L.level <= 0) return BSAM_OK;
-  std::vector<GRule> all;
+static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const size_t np = L.patches.size();
+  for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(np + 1, 0);
+  if (L.level <= 0) return;
+  std::vector<GRule>& all = G.rules;
   std::vector<std::vector<GRule>> per[3];
   for (int sg = 0; sg < 3; sg++) per[sg].resize(np);
   for (size_t a = 0; a < np; a++) {
@@ -2225,20 +2293,15 @@ static int32_t build_interp_rules(bsam_ctx* c, Level& L) {
     for (int sg = 0; sg < 3; sg++) per[sg][a].swap(seg[sg]);
   }
   for (int sg = 0; sg < 3; sg++) {
-    L.roff[sg].assign(np + 1, 0);
     for (size_t a = 0; a < np; a++) {
-      L.roff[sg][a] = (int)all.size();
+      G.roff[sg][a] = (int)all.size();
       all.insert(all.end(), per[sg][a].begin(), per[sg][a].end());
     }
-    L.roff[sg][np] = (int)all.size();
+    G.roff[sg][np] = (int)all.size();
   }
-  if (all.empty()) return BSAM_OK;
-  HIPCHK(hipMalloc(&L.d_rules, all.size() * sizeof(GRule)));
-  HIPCHK(hipMemcpy(L.d_rules, all.data(), all.size() * sizeof(GRule), hipMemcpyHostToDevice));
-  return BSAM_OK;
 }
 
-int32_t build_ghost_stages(bsam_ctx* c, Level& L) {
+int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const int nd = c->p.ndims;
   std::vector<std::array<int, 3>> offs;
   bool periodic = false;
@@ -2311,37 +2374,351 @@ int32_t build_ghost_stages(bsam_ctx* c, Level& L) {
     }
   }
   if (has_phys) all_wrap = false;
-  L.stages.clear();
-  L.wrap_pid = -1;
+  G.stages.clear();
+  G.wrap_pid = -1;
   if (all_wrap && L.level <= 0) {
-    L.wrap_pid = ps[0];
+    G.wrap_pid = ps[0];
     return BSAM_OK;
   }
+  auto add = [&](std::vector<BoxDesc>& v) {
+    if (!v.empty()) G.stages.push_back(std::move(v));
+  };
   const bool no_self = px.empty() && py.empty() && pz.empty();
   if (no_self) {
-    std::vector<BoxDesc> m = sib;
-    m.insert(m.end(), pp.begin(), pp.end());
-    RETCHK(upload_stage(c, L, m));
+    sib.insert(sib.end(), pp.begin(), pp.end());
+    add(sib);
   } else {
-    RETCHK(upload_stage(c, L, sib));
-    RETCHK(upload_stage(c, L, px));
-    RETCHK(upload_stage(c, L, py));
-    RETCHK(upload_stage(c, L, pz));
-    RETCHK(upload_stage(c, L, pp));
+    add(sib);
+    add(px);
+    add(py);
+    add(pz);
+    add(pp);
   }
-  for (int dim = 0; dim < 3; dim++) RETCHK(upload_stage(c, L, phys[dim]));
+  for (int dim = 0; dim < 3; dim++) add(phys[dim]);
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Flattened ghost fill.  SetGhost(level, ipass) runs its stages in order
+// (interpolation, sibling transfers, self-periodic, periodic pairs,
+// physical conditions), later stages overwriting earlier ones.  Executing
+// the stages symbolically on the host gives, for every ghost cell the call
+// writes, the expression its final value is: a copy of one cell or one
+// interpolation rule, followed by a chain of +0.0 / Dirichlet images when
+// a stage read a ghost cell that an earlier stage of the same call wrote.
+// When every expression reads only interior cells of the level and parent
+// cells (which no stage writes), the expressions are independent and one
+// gather launch evaluates them all with the bits of the staged order.
+// ---------------------------------------------------------------------------
+struct FExpr {
+  int kind;   // 0: copy of cell `src`; 1: interpolation rule `src` of the plan
+  int src;
+  int svs;    // copy: source variable stride
+  int dvs;    // destination variable stride
+  int spid;   // copy: source patch
+  int dpid;   // destination patch
+  int post;   // post-operation chain (ghost_post)
+  int npost;
+  int early;  // reads a face ghost as it was before the call
+};
+
+__host__ inline void qdecode(const DPatch& P, int o, int& i, int& j, int& k) {
+  int r = o - P.qoff;
+  const int m = r % P.hq;
+  r /= P.hq;
+  j = r % P.nq[1];
+  r /= P.nq[1];
+  k = r % P.nq[2];
+  const int col = r / P.nq[2];
+  i = 2 * m + ((col + j + k) & 1);
+}
+__host__ inline int n_outside(const DPatch& P, int nd, int i, int j, int k) {
+  return (i < 1 || i > P.mx[0]) + (j < 1 || j > P.mx[1]) + (nd == 3 && (k < 1 || k > P.mx[2]));
+}
+
+// Expressions of one flattened call, indexed by the variable-0 cell of a
+// level patch: dense(pid, off) = base[pos[pid]] + off - qoff(pid).
+struct LevelCells {
+  std::vector<int> pos;     // patch -> position in the level, or -1
+  std::vector<long> base;
+  long n = 0;
+  LevelCells(const bsam_ctx* c, const Level& L) : pos(c->npatch, -1), base(L.patches.size()) {
+    for (size_t a = 0; a < L.patches.size(); a++) {
+      pos[L.patches[a]] = (int)a;
+      base[a] = n;
+      n += c->hp[L.patches[a]].qvs;
+    }
+  }
+  long at(const bsam_ctx* c, int pid, int off) const { return base[pos[pid]] + (off - c->hp[pid].qoff); }
+};
+struct FlatCall {
+  std::vector<int> at;     // dense cell -> index into ex, or -1
+  std::vector<FExpr> ex;
+  std::vector<int> dst;    // pool offset of the cell of ex[n]
+  const FExpr* find(long cell) const { return at[cell] < 0 ? nullptr : &ex[at[cell]]; }
+  void set(long cell, int d, const FExpr& e) {
+    if (at[cell] < 0) {
+      at[cell] = (int)ex.size();
+      ex.push_back(e);
+      dst.push_back(d);
+    } else {
+      ex[at[cell]] = e;
+    }
+  }
+};
+
+static bool flatten_setghost(const bsam_ctx* c, const Level& L, const GhostPlan& G, const LevelCells& LC,
+                             int ipass, FlatCall& E, std::string& why) {
+  const int nd = c->p.ndims;
+  E.at.assign(LC.n, -1);
+  E.ex.clear();
+  E.dst.clear();
+  const int s0 = (ipass == 2) ? 1 : 0, s1 = (ipass == 1) ? 1 : 2;
+  const int np = (int)L.patches.size();
+  for (int sg = s0; sg <= s1; sg++)
+    for (int a = 0; a < np; a++)
+      for (int r = G.roff[sg][a]; r < G.roff[sg][a + 1]; r++) {
+        const FExpr e{1, r, 0, G.rules[r].pvs, -1, L.patches[a], 0, 0, 0};
+        E.set(LC.at(c, L.patches[a], G.rules[r].dst), G.rules[r].dst, e);
+      }
+  for (const auto& st : G.stages)
+    for (const BoxDesc& B : st) {
+      const DPatch& D = c->hp[B.dp];
+      const DPatch& S = c->hp[B.sp];
+      const int code = (B.op == kCopy) ? 0 : (B.op == kCopyPlus0 ? 1 : (B.op == kDirichlet2 ? 2 : 3));
+      for (int z = 0; z < B.ext[2]; z++)
+        for (int y = 0; y < B.ext[1]; y++)
+          for (int x = 0; x < B.ext[0]; x++) {
+            const int si = B.slo[0] + x, sj = B.slo[1] + y, sk = B.slo[2] + z;
+            const int s = qidx(S, si, sj, sk);
+            const int d = qidx(D, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z);
+            if (!n_outside(D, nd, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z)) {
+              why = "a ghost-fill stage writes an interior cell (overlapping patches)";
+              return false;
+            }
+            FExpr e;
+            const FExpr* prev = E.find(LC.at(c, B.sp, s));
+            if (prev) {
+              e = *prev;
+              if (e.kind == 1 && e.dpid != B.dp) {
+                why = "an interpolated ghost value is copied into another patch";
+                return false;
+              }
+            } else {
+              // a ghost not written so far in this call is read as it was
+              // before the call (physical conditions over ranges that
+              // include the ghost rows, boundary.f90:642-802)
+              const int out = n_outside(S, nd, si, sj, sk);
+              if (out > 1) {
+                why = "a ghost-fill stage reads an edge or corner ghost the call has not written";
+                return false;
+              }
+              e = FExpr{0, s, S.qvs, 0, B.sp, 0, 0, 0, out};
+            }
+            e.dvs = D.qvs;
+            e.dpid = B.dp;
+            if (code) {
+              if (e.npost >= 4) {
+                why = "post-operation chain longer than four";
+                return false;
+              }
+              e.post |= code << (2 * e.npost);
+              e.npost++;
+            }
+            E.set(LC.at(c, B.dp, d), d, e);
+          }
+    }
+  return true;
+}
+
+static bool same_expr(const GhostPlan& G, const FExpr& a, const FExpr& b) {
+  if (a.kind != b.kind || a.post != b.post || a.npost != b.npost) return false;
+  if (a.kind == 0) return a.src == b.src && a.svs == b.svs;
+  const GRule &r = G.rules[a.src], &s = G.rules[b.src];
+  if (r.kind != s.kind || r.s1 != s.s1 || r.s2 != s.s2 || r.cvs != s.cvs) return false;
+  for (int n = 0; n < r.kind; n++)
+    if (r.c[n] != s.c[n]) return false;
+  return true;
+}
+
+// The flattened maps of a level (levels > 0 only: below them SetGhost is a
+// periodic wrap or a physical condition on one patch).
+//
+// map[p], p = 0, 1, 2: SetGhost(level, p) in full.
+// map[2 + p], p = 1, 2: the face ghosts X that SetGhost(level, p) writes
+// and for which (a) X has the colour p-1 just relaxed, or (b) X's
+// expression reads a cell of colour p-1, or (c) X's expression differs
+// from its expression in SetGhost(level, 3-p); and the edge and corner
+// ghosts SetGhost(level, p) writes but SetGhost(level, 3-p) does not (2D
+// corners are colour-filtered, boundary.f90:155-181).
+//
+// Use of map[2 + p] inside a red-black sweep sequence (LevelRelax: sweep
+// colour 0, SetGhost(1), sweep colour 1, SetGhost(2), ..., the last
+// SetGhost in full) leaves every value a 7-point sweep, residual or FAS
+// right-hand side reads equal to the staged fill's: a colour-c sweep reads
+// only face ghosts of colour 1-c.  A face ghost X the full call would
+// write but map[2 + p] skips has colour 2-p, its expression E is the same
+// at ipass 1 and 2 and reads nothing of colour p-1: after the first
+// SetGhost(2) of the sequence (which writes every colour-1 face ghost of
+// map[2]) X holds E evaluated after the previous call, whose sources the
+// sweep in between did not change.  Before that point only colour-1 face
+// ghosts can differ, and the colour-1 sweep does not read them.  Edge and
+// corner ghosts are not read by 7-point stencils; those both ipasses write
+// are restored by the last SetGhost of the sequence, which is full, the
+// others are rewritten at every call of their ipass, and SetGhost(level,
+// 0) (after CorrectFine) writes every cell any ipass writes.
+int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
+  G.maps_ok = false;
+  if (L.level <= 0) return BSAM_OK;
+  const int nd = c->p.ndims;
+  const LevelCells LC(c, L);
+  FlatCall E[3];
+  for (int p = 0; p < 3; p++) {
+    if (!flatten_setghost(c, L, G, LC, p, E[p], G.why)) return BSAM_OK;
+    // the rules that read ghost values from before the call run in a first
+    // launch; none of them may write a cell another of them reads
+    for (size_t n = 0; n < E[p].ex.size(); n++) {
+      const FExpr& e = E[p].ex[n];
+      if (!e.early) continue;
+      const FExpr* w = E[p].find(LC.at(c, e.spid, e.src));
+      if (w && w->early) {
+        G.why = "a ghost read from before the call is rewritten by another such rule";
+        return BSAM_OK;
+      }
+    }
+  }
+  // rules in destination order
+  auto emit = [&](const FlatCall& M, const std::vector<int>& which, HostMap& H) {
+    H.copies.clear();
+    H.cpost.clear();
+    H.interps.clear();
+    H.nc1 = H.ni1 = 0;
+    bool anypost = false;
+    for (int phase = 1; phase >= 0; phase--)
+      for (int n : which) {
+        const FExpr& e = M.ex[n];
+        if ((e.early != 0) != (phase == 1)) continue;
+        const int d = M.dst[n];
+        if (e.kind == 0) {
+          H.copies.push_back(make_int4(d, e.src, e.dvs, e.svs));
+          H.cpost.push_back((uint8_t)e.post);
+          anypost = anypost || e.post;
+          if (phase) H.nc1++;
+        } else {
+          GRule r = G.rules[e.src];
+          r.dst = d;
+          r.kind = (r.kind & 0xff) | (e.post << 8);
+          H.interps.push_back(r);
+          if (phase) H.ni1++;
+        }
+      }
+    if (!anypost) H.cpost.clear();
+  };
+  for (int p = 0; p < 3; p++) {
+    std::vector<int> order(E[p].ex.size());
+    for (size_t n = 0; n < order.size(); n++) order[n] = (int)n;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return E[p].dst[x] < E[p].dst[y]; });
+    emit(E[p], order, G.map[p]);
+    if (p == 0) continue;
+    const int col = p - 1;
+    const FlatCall& other = E[3 - p];
+    std::vector<int> lean;
+    for (int n : order) {
+      const FExpr& e = E[p].ex[n];
+      const int d = E[p].dst[n];
+      const DPatch& D = c->hp[e.dpid];
+      int i, j, k;
+      qdecode(D, d, i, j, k);
+      const FExpr* o = other.find(LC.at(c, e.dpid, d));
+      if (n_outside(D, nd, i, j, k) != 1) {
+        // an edge or corner ghost: only if the other ipass does not write it
+        if (!o && e.early) {
+          G.why = "a colour-filtered edge or corner ghost reads a ghost from before the call";
+          return BSAM_OK;
+        }
+        if (!o) lean.push_back(n);
+        continue;
+      }
+      bool need = ((i + j + k) & 1) == col;
+      if (!need) {
+        int si, sj, sk;
+        if (e.kind == 0) {
+          qdecode(c->hp[e.spid], e.src, si, sj, sk);
+          need = ((si + sj + sk) & 1) == col;
+        } else {
+          const GRule& r = G.rules[e.src];
+          qdecode(D, r.s1, si, sj, sk);
+          need = ((si + sj + sk) & 1) == col;
+          qdecode(D, r.s2, si, sj, sk);
+          need = need || ((si + sj + sk) & 1) == col;
+        }
+      }
+      if (!need) need = !o || !same_expr(G, e, *o);
+      if (need && e.early) {
+        G.why = "a face ghost of a reduced table reads a ghost from before the call";
+        return BSAM_OK;
+      }
+      if (need) lean.push_back(n);
+    }
+    emit(E[p], lean, G.map[2 + p]);
+  }
+  G.maps_ok = true;
+  return BSAM_OK;
+}
+
+int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
+  L.wrap_pid = G.wrap_pid;
+  L.stages.clear();
+  for (const auto& boxes : G.stages) {
+    BoxDesc* d = nullptr;
+    HIPCHK(hipMalloc(&d, boxes.size() * sizeof(BoxDesc)));
+    c->box_allocs.push_back(d);
+    HIPCHK(hipMemcpy(d, boxes.data(), boxes.size() * sizeof(BoxDesc), hipMemcpyHostToDevice));
+    Stage s;
+    s.d = d;
+    s.n = (int)boxes.size();
+    L.stages.push_back(s);
+  }
+  for (int sg = 0; sg < 3; sg++) L.roff[sg] = G.roff[sg];
+  if (!G.rules.empty()) {
+    HIPCHK(hipMalloc(&L.d_rules, G.rules.size() * sizeof(GRule)));
+    HIPCHK(hipMemcpy(L.d_rules, G.rules.data(), G.rules.size() * sizeof(GRule), hipMemcpyHostToDevice));
+  }
+  L.maps_ok = G.maps_ok;
+  if (!G.maps_ok) return BSAM_OK;
+  for (int m = 0; m < 5; m++) {
+    const HostMap& H = G.map[m];
+    DevMap& D = L.gmap[m];
+    D.nc = (int)H.copies.size();
+    D.ni = (int)H.interps.size();
+    D.nc1 = H.nc1;
+    D.ni1 = H.ni1;
+    if (D.nc) {
+      HIPCHK(hipMalloc(&D.cr, H.copies.size() * sizeof(int4)));
+      HIPCHK(hipMemcpy(D.cr, H.copies.data(), H.copies.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    if (!H.cpost.empty()) {
+      HIPCHK(hipMalloc(&D.cpost, H.cpost.size()));
+      HIPCHK(hipMemcpy(D.cpost, H.cpost.data(), H.cpost.size(), hipMemcpyHostToDevice));
+    }
+    if (D.ni) {
+      HIPCHK(hipMalloc(&D.ir, H.interps.size() * sizeof(GRule)));
+      HIPCHK(hipMemcpy(D.ir, H.interps.data(), H.interps.size() * sizeof(GRule), hipMemcpyHostToDevice));
+    }
+  }
   return BSAM_OK;
 }
 
 }  // namespace
 
-extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc,
-                                      int32_t minlevel, int32_t finestlevel) {
-  if (!c || !desc || npatch < 1) return set_err(BSAM_EINVAL, "bad arguments");
+namespace {
+// The host half of bsam_set_hierarchy: descriptor validation, the device
+// patch records with their pool offsets, and each level's patch list.
+int32_t plan_patches(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel,
+                     int32_t finestlevel) {
+  if (!desc || npatch < 1) return set_err(BSAM_EINVAL, "bad arguments");
   if (minlevel > 0 || finestlevel < 0 || minlevel < -30 || finestlevel > 30)
     return set_err(BSAM_EINVAL, "level range [%d, %d] invalid", minlevel, finestlevel);
-  HIPCHK(hipStreamSynchronize(c->s));
-  free_hierarchy(c);
   const int nd = c->p.ndims, nv = c->p.nrvars;
   c->desc.assign(desc, desc + npatch);
   c->npatch = npatch;
@@ -2399,6 +2776,25 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
   if (nroot != 1) return set_err(BSAM_EINVAL, "exactly one root-level patch required (found %d)", nroot);
   c->nq = qo;
   c->nc = co;
+  c->lv.assign(finestlevel - minlevel + 1, Level());
+  for (int l = minlevel; l <= finestlevel; l++) c->L(l).level = l;
+  for (int n = 0; n < npatch; n++) c->L(desc[n].level).patches.push_back(n);
+  for (Level& L : c->lv) {
+    if (L.patches.empty()) return set_err(BSAM_EINVAL, "level %d has no patches", L.level);
+    L.h = c->hp[L.patches[0]].dx;
+  }
+  return BSAM_OK;
+}
+}  // namespace
+
+extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc,
+                                      int32_t minlevel, int32_t finestlevel) {
+  if (!c) return set_err(BSAM_EINVAL, "bad arguments");
+  HIPCHK(hipStreamSynchronize(c->s));
+  free_hierarchy(c);
+  const int nv = c->p.nrvars;
+  RETCHK(plan_patches(c, npatch, desc, minlevel, finestlevel));
+  const long qo = c->nq, co = c->nc;
   HIPCHK(hipMalloc(&c->d_pt, npatch * sizeof(DPatch)));
   HIPCHK(hipMemcpy(c->d_pt, c->hp.data(), npatch * sizeof(DPatch), hipMemcpyHostToDevice));
   double** pools[5] = {&c->q, &c->qold, &c->f, &c->rf, &c->ftmp};
@@ -2410,13 +2806,8 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
   if (hipMalloc(&c->qc, co * sizeof(double)) != hipSuccess) return set_err(BSAM_ENOMEM, "qc pool");
   HIPCHK(hipMemset(c->qc, 0, co * sizeof(double)));
   // levels
-  c->lv.assign(finestlevel - minlevel + 1, Level());
-  for (int l = minlevel; l <= finestlevel; l++) c->L(l).level = l;
-  for (int n = 0; n < npatch; n++) c->L(desc[n].level).patches.push_back(n);
   int maxblocks = 1;
   for (Level& L : c->lv) {
-    if (L.patches.empty()) return set_err(BSAM_EINVAL, "level %d has no patches", L.level);
-    L.h = c->hp[L.patches[0]].dx;
     HIPCHK(hipMalloc(&L.d_patches, L.patches.size() * sizeof(int)));
     HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), L.patches.size() * sizeof(int), hipMemcpyHostToDevice));
     for (int d = 0; d < kNumDom; d++) {
@@ -2468,8 +2859,11 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
       end = c->hp[pid].qoff + (long)c->hp[pid].qvs * nv;
     }
     L.qspan_len = contiguous ? end - L.qspan_off : -1;
-    RETCHK(build_ghost_stages(c, L));
-    RETCHK(build_interp_rules(c, L));
+    GhostPlan G;
+    RETCHK(plan_ghost_stages(c, L, G));
+    plan_interp_rules(c, L, G);
+    if (c->p.batching) RETCHK(plan_ghost_maps(c, L, G));
+    RETCHK(upload_ghost(c, L, G));
   }
   // coarse chains: from the highest level whose patch has at most 16^3
   // (2D: 64^2) cells down to minlevel, every level a single patch periodic
@@ -2497,6 +2891,186 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
   c->have_hier = true;
   if (c->p.precision == BSAM_PREC_MIXED) {
     RETCHK(bsam::mixed_init(c));
+  }
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host self-test of the flattened ghost-fill tables
+// ---------------------------------------------------------------------------
+namespace {
+
+// Host evaluation of both forms of SetGhost on one pool, in FP64, with the
+// interpolation constants of boundary.f90:54-63.
+const double hA[3] = {-1.0 / 5.0, 2.0 / 3.0, 8.0 / 15.0};
+const double hB[3] = {5.0 / 32.0, 15.0 / 16.0, -3.0 / 32.0};
+const double hC[4] = {9.0 / 16.0, 3.0 / 16.0, 3.0 / 16.0, 1.0 / 16.0};
+
+double host_interp(const GRule& R, int kind, int v, const std::vector<double>& q) {
+  const int fv = v * R.pvs, pv = v * R.cvs;
+  double coarse;
+  if (kind == 1) coarse = q[R.c[0] + pv];
+  else if (kind == 3) coarse = hB[0] * q[R.c[0] + pv] + hB[1] * q[R.c[1] + pv] + hB[2] * q[R.c[2] + pv];
+  else
+    coarse = hC[0] * q[R.c[0] + pv] + hC[1] * q[R.c[1] + pv] + hC[2] * q[R.c[2] + pv] + hC[3] * q[R.c[3] + pv];
+  return hA[0] * q[R.s1 + fv] + hA[1] * q[R.s2 + fv] + hA[2] * coarse;
+}
+double host_post(double x, int post, int v, int homog) {
+  for (; post; post >>= 2) {
+    const int op = post & 3;
+    if (op == 1) x = x + 0.0;
+    else if (op == 2 || (op == 3 && v >= 1)) x = homog ? -x : 2.0 - x;
+  }
+  return x;
+}
+
+// The staged order: interpolation rules of the ipass's segments, then the
+// box stages one box after another (the reference's sequential order).
+void host_staged(const bsam_ctx* c, const Level& L, const GhostPlan& G, int ipass, int homog,
+                 std::vector<double>& q) {
+  const int nv = c->p.nrvars;
+  const int s0 = (ipass == 2) ? 1 : 0, s1 = (ipass == 1) ? 1 : 2;
+  const int np = (int)L.patches.size();
+  for (int sg = s0; sg <= s1; sg++)
+    for (int r = G.roff[sg][0]; r < G.roff[sg][np]; r++)
+      for (int v = 0; v < nv; v++) q[G.rules[r].dst + v * G.rules[r].pvs] = host_interp(G.rules[r], G.rules[r].kind, v, q);
+  for (const auto& st : G.stages)
+    for (const BoxDesc& B : st) {
+      const DPatch& D = c->hp[B.dp];
+      const DPatch& S = c->hp[B.sp];
+      for (int v = 0; v < nv; v++)
+        for (int z = 0; z < B.ext[2]; z++)
+          for (int y = 0; y < B.ext[1]; y++)
+            for (int x = 0; x < B.ext[0]; x++) {
+              const double a = q[qidx(S, B.slo[0] + x, B.slo[1] + y, B.slo[2] + z) + v * S.qvs];
+              const double dir = homog ? -a : 2.0 - a;
+              double y0;
+              switch (B.op) {
+                case kCopyPlus0: y0 = a + 0.0; break;
+                case kDirichlet2: y0 = dir; break;
+                case kBgMixed: y0 = (v == 0) ? a : dir; break;
+                default: y0 = a;
+              }
+              q[qidx(D, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z) + v * D.qvs] = y0;
+            }
+    }
+}
+
+// The flattened form: all right-hand sides first, then the stores (the
+// device evaluates the rules concurrently).
+void host_map(const bsam_ctx* c, const HostMap& H, int homog, std::vector<double>& q) {
+  const int nv = c->p.nrvars;
+  const int cs[3] = {0, H.nc1, (int)H.copies.size()}, is[3] = {0, H.ni1, (int)H.interps.size()};
+  for (int ph = 0; ph < 2; ph++) {
+    std::vector<double> val;
+    for (int n = cs[ph]; n < cs[ph + 1]; n++)
+      for (int v = 0; v < nv; v++)
+        val.push_back(host_post(q[H.copies[n].y + v * H.copies[n].w], H.cpost.empty() ? 0 : H.cpost[n], v, homog));
+    for (int n = is[ph]; n < is[ph + 1]; n++) {
+      const GRule& R = H.interps[n];
+      for (int v = 0; v < nv; v++) val.push_back(host_post(host_interp(R, R.kind & 0xff, v, q), R.kind >> 8, v, homog));
+    }
+    size_t t = 0;
+    for (int n = cs[ph]; n < cs[ph + 1]; n++)
+      for (int v = 0; v < nv; v++) q[H.copies[n].x + v * H.copies[n].z] = val[t++];
+    for (int n = is[ph]; n < is[ph + 1]; n++)
+      for (int v = 0; v < nv; v++) q[H.interps[n].dst + v * H.interps[n].pvs] = val[t++];
+  }
+}
+
+struct Rng {
+  uint64_t s;
+  double next() {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    return (double)(s >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+  }
+};
+
+}  // namespace
+
+extern "C" int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t npatch, const bsam_patch_desc* desc,
+                                            int32_t minlevel, int32_t finestlevel, uint64_t seed, int64_t* checked,
+                                            int32_t* levels_flat) {
+  if (!params || !checked || !levels_flat) return set_err(BSAM_EINVAL, "null argument");
+  *checked = 0;
+  *levels_flat = 0;
+  bsam_ctx tmp;
+  bsam_ctx* c = &tmp;
+  c->p = *params;
+  c->p.batching = 1;
+  RETCHK(plan_patches(c, npatch, desc, minlevel, finestlevel));
+  const int nd = c->p.ndims, nv = c->p.nrvars;
+  Rng rng{seed * 2654435761u + 12345u};
+  for (Level& L : c->lv) {
+    if (L.level <= 0) continue;
+    GhostPlan G;
+    RETCHK(plan_ghost_stages(c, L, G));
+    plan_interp_rules(c, L, G);
+    RETCHK(plan_ghost_maps(c, L, G));
+    if (!G.maps_ok) continue;  // the level keeps the staged fill
+    ++*levels_flat;
+    // cells of the level by kind
+    std::vector<int> face[2], inner[2];
+    for (int pid : L.patches) {
+      const DPatch& P = c->hp[pid];
+      for (int k = (nd == 3 ? 0 : 0); k < P.nq[2]; k++)
+        for (int j = 0; j < P.nq[1]; j++)
+          for (int i = 0; i < P.nq[0]; i++) {
+            const int n = n_outside(P, nd, i, j, k);
+            const int o = qidx(P, i, j, k), col = (i + j + k) & 1;
+            for (int v = 0; v < nv; v++) {
+              if (n == 0) inner[col].push_back(o + v * P.qvs);
+              if (n == 1) face[col].push_back(o + v * P.qvs);
+            }
+          }
+    }
+    for (int homog = 0; homog <= 1; homog++) {
+      std::vector<double> a(c->nq), b;
+      for (double& x : a) x = rng.next();
+      // (1) every full table against the staged call
+      for (int p = 0; p < 3; p++) {
+        b = a;
+        std::vector<double> s = a;
+        host_staged(c, L, G, p, homog, s);
+        host_map(c, G.map[p], homog, b);
+        for (long n = 0; n < c->nq; n++)
+          if (memcmp(&s[n], &b[n], sizeof(double)))
+            return set_err(BSAM_EINVAL, "level %d ipass %d homog %d: table differs from the staged fill at %ld",
+                           L.level, p, homog, n);
+        *checked += c->nq;
+      }
+      // (2) a red-black sequence (three passes, the last SetGhost full)
+      // from an arbitrary state, each sweep replaced by new values of its
+      // colour's interior cells
+      b = a;
+      const int ns = 3;
+      for (int sweep = 0; sweep < 2 * ns; sweep++) {
+        const int col = sweep & 1, ipass = col + 1;
+        const bool last = sweep == 2 * ns - 1;
+        for (int o : inner[col]) a[o] = b[o] = rng.next();
+        host_staged(c, L, G, ipass, homog, a);
+        host_map(c, G.map[last ? ipass : 2 + ipass], homog, b);
+        const std::vector<int>& f0 = face[col];
+        for (int o : f0)
+          if (memcmp(&a[o], &b[o], sizeof(double)))
+            return set_err(BSAM_EINVAL, "level %d homog %d sweep %d: colour-%d face ghost %d differs", L.level,
+                           homog, sweep, col, o);
+        *checked += f0.size();
+        if (ipass == 2) {
+          for (int o : face[0])
+            if (memcmp(&a[o], &b[o], sizeof(double)))
+              return set_err(BSAM_EINVAL, "level %d homog %d sweep %d: face ghost %d differs", L.level, homog,
+                             sweep, o);
+          *checked += face[0].size();
+        }
+      }
+      for (long n = 0; n < c->nq; n++)
+        if (memcmp(&a[n], &b[n], sizeof(double)))
+          return set_err(BSAM_EINVAL, "level %d homog %d: state after the sequence differs at %ld", L.level, homog, n);
+      *checked += c->nq;
+    }
   }
   return BSAM_OK;
 }
@@ -2621,10 +3195,27 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
 }
 
 // SetGhost on any field of the q geometry (FP64 solution or FP32 correction).
+// `lean` (ipass 1, 2 inside a red-black sweep sequence whose last SetGhost
+// is not lean) refreshes only the face ghosts such a sequence reads.
 template <typename T>
-static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog) {
+static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, bool lean = false) {
   Level& L = c->L(level);
   const int nv = c->p.nrvars;
+  if (L.maps_ok && c->p.batching) {
+    const DevMap& M = L.gmap[(lean && ipass > 0) ? 2 + ipass : ipass];
+    // rules reading ghosts from before the call first, then the rest
+    if (M.nc1 + M.ni1 > 0) {
+      k_ghost_map<T><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q);
+      LAUNCH_CHECK();
+    }
+    const int nc = M.nc - M.nc1, ni = M.ni - M.ni1;
+    if (nc + ni > 0) {
+      k_ghost_map<T><<<nblocks(nc + ni), kBlock, 0, c->s>>>(M.cr + M.nc1, M.cpost ? M.cpost + M.nc1 : nullptr, nc,
+                                                             M.ir + M.ni1, ni, nv, homog, q);
+      LAUNCH_CHECK();
+    }
+    return BSAM_OK;
+  }
   if (level > 0 && L.d_rules) {
     // colour-filter segments this ipass refreshes: 0 -> all, 1 -> 0..1,
     // 2 -> 1..2
@@ -2709,14 +3300,14 @@ static int march_kc(bsam_ctx* c, const Level& L, int* nblk) {
 // LevelRelax, afasroutines.f90:546-555).  For a single patch periodic in
 // every dimension the sweep writes the ghost images itself, which is the
 // whole effect of that SetGhost on the values that changed.
-static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass) {
+static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass, bool lean) {
   Level& L = c->L(level);
   const bool wrap = L.wrap_pid >= 0;
   int nblk = 0;
   const int kc = (c->p.model == BSAM_MODEL_CH) ? sweep_kc(c, L, 8, &nblk) : 0;
   if (!kc && !(wrap && c->p.model == BSAM_MODEL_CH)) {
     RETCHK(bsam_relax_level(c, level, ipass));
-    return set_ghost_t<double>(c, level, ipass, c->q, 0);
+    return set_ghost_t<double>(c, level, ipass, c->q, 0, lean);
   }
   const int col = ipass - 1;
   const bool fin = level == c->finest;
@@ -2748,14 +3339,14 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass) {
     }
     kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 8.0 * ((N + G) + 0.5 * (N + G) + N + N));
   }
-  if (!wrap) RETCHK(set_ghost_t<double>(c, level, ipass, c->q, 0));
+  if (!wrap) RETCHK(set_ghost_t<double>(c, level, ipass, c->q, 0, lean));
   return BSAM_OK;
 }
 
 extern "C" int32_t bsam_level_relax(bsam_ctx* c, int32_t level) {
   RETCHK(check_level(c, level));
   for (int s = 0; s < c->p.nsmoothingpasses; s++)
-    for (int rb = 1; rb <= 2; rb++) RETCHK(relax_and_ghost(c, level, rb));
+    for (int rb = 1; rb <= 2; rb++) RETCHK(relax_and_ghost(c, level, rb, !(s == c->p.nsmoothingpasses - 1 && rb == 2)));
   return BSAM_OK;
 }
 
@@ -3356,7 +3947,7 @@ int32_t mixed_level_relax(bsam_ctx* c, int level, int phase) {
       if (fused && phase == 1 && s == ns - 1 && rb == 2) emode = 2;
       const int32_t done = mixed_relax(c, level, rb, emode);
       RETCHK(done);
-      if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1));
+      if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, !(s == ns - 1 && rb == 2)));
     }
   return BSAM_OK;
 }

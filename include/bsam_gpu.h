@@ -210,6 +210,26 @@ int32_t bsam_kstat_enable(bsam_ctx* ctx, int32_t on);
 int32_t bsam_kstat_read(bsam_ctx* ctx, int32_t which, int64_t* launches,
                         double* total_ms, double* bytes_per_launch);
 
+/* ---- diagnostics (no counterpart in the reference) ----------------------
+ * Host-only check of the flattened ghost-fill tables that per-level
+ * launches use in place of the staged SetGhost (boundary.f90:67-97); needs
+ * no GPU.  For every level > 0 of the hierarchy, on seeded random host
+ * data and for the solution and correction forms of the boundary
+ * conditions: (1) each table SetGhost(level, ipass), ipass = 0, 1, 2,
+ * gives the staged fill's values bit for bit; (2) a red-black sweep
+ * sequence using the reduced tables, whose last SetGhost is full, leaves
+ * every face ghost the next sweep reads, and at its end every cell, equal
+ * to the staged fill's.  A level whose fill cannot be flattened (a stage
+ * reading an edge or corner ghost's value from before the call, as 3D
+ * physical corners do) keeps the staged fill and is skipped.  *checked
+ * receives the number of values compared, *levels_flat the number of
+ * levels checked.  Returns BSAM_EINVAL with a message naming the first
+ * difference. */
+int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t npatch,
+                                 const bsam_patch_desc* desc, int32_t minlevel,
+                                 int32_t finestlevel, uint64_t seed,
+                                 int64_t* checked, int32_t* levels_flat);
+
 #ifdef __cplusplus
 }
 #endif
