@@ -89,24 +89,31 @@ __device__ __forceinline__ int find_patch(const int* __restrict__ prefix, int n,
   return lo;
 }
 
+// A kernel's domain over one level (half-rows, cells, coarse cells, ...):
+// patch p owns the flat range [prefix[p], prefix[p] + cnt[p]), and every
+// range starts on a block boundary, so a block of a whole-level launch lies
+// in one patch and reads that patch's record with scalar loads.
 struct LevelArgs {
   const DPatch* __restrict__ pt;
   const int* __restrict__ patches;  // descriptor indices of the level
-  const int* __restrict__ prefix;   // npatch+1 prefix sums of the kernel's domain
-  const int* __restrict__ bp;       // patch holding the first element of each block (whole-level launches)
+  const int* __restrict__ prefix;   // block-aligned start of each patch's range
+  const int* __restrict__ cnt;      // elements of each patch
+  const int* __restrict__ bp;       // patch of each block (whole-level launches)
   int npatch;
   int total;  // end of the launch's range of the domain
   int gbase;  // start of the range: 0 for a whole level, prefix[p] for patch p alone
   int fixed;  // the patch of a one-patch launch, else -1
 };
 
-// Patch (position in the level's list) of domain element g: the launch's
-// single patch, or a short forward scan from the patch the block starts in.
-__device__ __forceinline__ int level_patch(const LevelArgs& L, int g) {
-  if (L.fixed >= 0) return L.fixed;
-  int pi = L.bp[g / kBlock];
-  while (pi + 1 < L.npatch && L.prefix[pi + 1] <= g) pi++;
-  return pi;
+// Patch (position in the level's list) and local index of element
+// gbase + lb*kBlock + threadIdx.x; false for the padding after a patch's
+// range and past the launch's end.
+__device__ __forceinline__ bool level_elem(const LevelArgs& L, int lb, int& pi, int& local) {
+  const int g = L.gbase + lb * kBlock + threadIdx.x;
+  if (g >= L.total) return false;
+  pi = (L.fixed >= 0) ? L.fixed : L.bp[lb];
+  local = g - L.prefix[pi];
+  return local < L.cnt[pi];
 }
 
 // ---------------------------------------------------------------------------
@@ -141,11 +148,9 @@ template <int DIM, bool WRAP>
 __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double dt, double eps2,
                                                      double h, double* __restrict__ q,
                                                      const double* __restrict__ f) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
   const int row = local / cnt;
@@ -341,11 +346,9 @@ __global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, Model
                                                        double* __restrict__ q,
                                                        const double* __restrict__ qold,
                                                        const double* __restrict__ f) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
   const int j = local / cnt + 1;
@@ -439,11 +442,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      double* __restrict__ partial) {
   const int lb = xcd_block();
   double s0 = 0.0, s1 = 0.0;
-  const int g = L.gbase + lb * kBlock + threadIdx.x;
-  if (g < L.total) {
-    const int pi = level_patch(L, g);
+  int pi, local;
+  if (level_elem(L, lb, pi, local)) {
     const DPatch P = L.pt[L.patches[pi]];
-    const int local = g - L.prefix[pi];
     const int i = local % P.mx[0] + 1;
     const int row = local / P.mx[0];
     const int j = row % P.mx[1] + 1;
@@ -779,12 +780,11 @@ struct CoarseCell {
   int pid, ic, jc, kc;
 };
 template <int DIM>
-__device__ __forceinline__ bool decode_coarse(const LevelArgs& L, int g, CoarseCell& cc) {
-  if (g >= L.total) return false;
-  const int pi = level_patch(L, g);
+__device__ __forceinline__ bool decode_coarse(const LevelArgs& L, int lb, CoarseCell& cc) {
+  int pi, local;
+  if (!level_elem(L, lb, pi, local)) return false;
   cc.pid = L.patches[pi];
   const DPatch P = L.pt[cc.pid];
-  const int local = g - L.prefix[pi];
   cc.ic = local % P.cmx[0] + 1;
   const int row = local / P.cmx[0];
   cc.jc = row % P.cmx[1] + 1;
@@ -803,7 +803,7 @@ template <int DIM, typename T, bool WRAP = false>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const T* __restrict__ src,
                                                      T* __restrict__ dst_parent, T* __restrict__ qc) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, L.gbase + xcd_block() * kBlock + threadIdx.x, c)) return;
+  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
@@ -825,12 +825,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const 
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, const T* __restrict__ q,
                                                         T* __restrict__ qc) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const DPatch Q = L.pt[P.parent];
-  const int local = g - L.prefix[pi];
   const int ic = local % P.nc[0];
   const int row = local / P.nc[0];
   const int jc = row % P.nc[1];
@@ -854,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
                                                     const double* __restrict__ qc,
                                                     double* __restrict__ fpar) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, L.gbase + xcd_block() * kBlock + threadIdx.x, c)) return;
+  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const double h = P.dx;
@@ -900,12 +898,10 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, const T* __restrict__ q,
                                                        T* __restrict__ qc) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const DPatch Q = L.pt[P.parent];
-  const int local = g - L.prefix[pi];
   const int ic = local % P.nc[0];
   const int row = local / P.nc[0];
   const int jc = row % P.nc[1];
@@ -933,11 +929,9 @@ __device__ __forceinline__ T linw(T lo, T hi, int fine) {
 template <int DIM, typename T, bool WRAP = false>
 __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T* __restrict__ q,
                                                          const T* __restrict__ qc) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int local = g - L.prefix[pi];
   const int n0 = P.cmx[0] + 1, n1 = P.cmx[1] + 1;
   const int ic = local % n0;
   const int row = local / n0;
@@ -1020,11 +1014,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T*
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* __restrict__ q,
                                                         const T* __restrict__ qc) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int local = g - L.prefix[pi];
   const int i = local % P.nq[0];
   const int row = local / P.nq[0];
   const int j = row % P.nq[1];
@@ -1060,11 +1052,9 @@ template <int DIM, int MODEL>
 __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double dt,
                                                    const double* __restrict__ qold,
                                                    double* __restrict__ ftmp, double* __restrict__ f) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int local = g - L.prefix[pi];
   const int i = local % P.mx[0] + 1;
   const int row = local / P.mx[0];
   const int j = row % P.mx[1] + 1;
@@ -1306,11 +1296,9 @@ __global__ __launch_bounds__(kBlock) void k_add_corr(double* __restrict__ q, flo
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __restrict__ q,
                                                  float* __restrict__ coef) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int local = g - L.prefix[pi];
   const int i = local % P.mx[0] + 1;
   const int row = local / P.mx[0];
   const int j = row % P.mx[1] + 1;
@@ -1348,11 +1336,9 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
                                                       float a12, float* __restrict__ e,
                                                       const float* __restrict__ rhs,
                                                       const float* __restrict__ coef) {
-  const int g = L.gbase + xcd_block() * kBlock + threadIdx.x;
-  if (g >= L.total) return;
-  const int pi = level_patch(L, g);
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int local = g - L.prefix[pi];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
   const int row = local / cnt;
@@ -1393,7 +1379,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
                                                         const float* __restrict__ coef,
                                                         float* __restrict__ rpar) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, L.gbase + xcd_block() * kBlock + threadIdx.x, c)) return;
+  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const float t1 = (float)(dt / (P.dx * P.dx)), t2 = (float)(eps2 / (P.dx * P.dx));
@@ -1764,7 +1750,9 @@ struct Level {
   int* d_patches = nullptr;
   int* d_prefix[kNumDom] = {};
   int* d_bp[kNumDom] = {};
-  std::vector<int> hprefix[kNumDom];
+  std::vector<int> hprefix[kNumDom];  // block-aligned starts of each patch's range
+  std::vector<int> hcnt[kNumDom];     // elements of each patch
+  int* d_cnt[kNumDom] = {};
   int total[kNumDom] = {};
   std::vector<Stage> stages;      // box stages after interpolation
   // coarse-fine interpolation rules (levels > 0), grouped by colour-filter
@@ -1849,6 +1837,7 @@ LevelArgs largs(bsam_ctx* c, Level& L, Dom d) {
   a.pt = c->d_pt;
   a.patches = L.d_patches;
   a.prefix = L.d_prefix[d];
+  a.cnt = L.d_cnt[d];
   a.bp = L.d_bp[d];
   a.npatch = (int)L.patches.size();
   a.total = L.total[d];
@@ -1885,7 +1874,7 @@ struct LaunchIter {
     while (i < L.patches.size()) {
       a = full;
       a.gbase = L.hprefix[d][i];
-      a.total = L.hprefix[d][i + 1];
+      a.total = L.hprefix[d][i] + L.hcnt[d][i];
       a.fixed = (int)i;
       i++;
       nb = nblocks((long)a.total - a.gbase);
@@ -1990,6 +1979,7 @@ static void free_hierarchy(bsam_ctx* c) {
     }
     for (int d = 0; d < kNumDom; d++) {
       hipFree(L.d_prefix[d]);
+      hipFree(L.d_cnt[d]);
       hipFree(L.d_bp[d]);
     }
   }
@@ -2811,7 +2801,7 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
     HIPCHK(hipMalloc(&L.d_patches, L.patches.size() * sizeof(int)));
     HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), L.patches.size() * sizeof(int), hipMemcpyHostToDevice));
     for (int d = 0; d < kNumDom; d++) {
-      std::vector<int> pre(L.patches.size() + 1, 0);
+      std::vector<int> pre(L.patches.size() + 1, 0), cnts(L.patches.size(), 0);
       long acc = 0;
       for (size_t a = 0; a < L.patches.size(); a++) {
         const DPatch& P = c->hp[L.patches[a]];
@@ -2826,16 +2816,21 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
             break;
           default: cnt = (long)P.nc[0] * P.nc[1] * P.nc[2]; break;
         }
+        // every patch's range starts on a block boundary
         pre[a] = (int)acc;
-        acc += cnt;
+        cnts[a] = (int)cnt;
+        acc += (cnt + kBlock - 1) / kBlock * kBlock;
       }
       pre[L.patches.size()] = (int)acc;
       if (acc > 0x7fffffffL) return set_err(BSAM_EINVAL, "level %d too large", L.level);
       L.total[d] = (int)acc;
       L.hprefix[d] = pre;
+      L.hcnt[d] = cnts;
       HIPCHK(hipMalloc(&L.d_prefix[d], pre.size() * sizeof(int)));
       HIPCHK(hipMemcpy(L.d_prefix[d], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
-      // block -> first patch, for whole-level launches
+      HIPCHK(hipMalloc(&L.d_cnt[d], cnts.size() * sizeof(int)));
+      HIPCHK(hipMemcpy(L.d_cnt[d], cnts.data(), cnts.size() * sizeof(int), hipMemcpyHostToDevice));
+      // block -> patch, for whole-level launches
       std::vector<int> bp(std::max(1, nblocks(acc)), 0);
       size_t pi = 0;
       for (size_t b = 0; b < bp.size(); b++) {
