@@ -1763,9 +1763,9 @@ struct Level {
   std::vector<int> roff[3];
   // the same ghost fill flattened to one rule per ghost cell (levels > 0,
   // per-level launches): gmap[ipass] is SetGhost(level, ipass) in full,
-  // gmap[2 + ipass] (ipass 1, 2) its face ghosts a red-black sweep
+  // gmap[2 + ipass] (ipass 1, 2) and gmap[5] (ipass 0) the ghosts a red-black sweep
   // sequence needs (see plan_ghost_maps)
-  DevMap gmap[5];
+  DevMap gmap[6];
   bool maps_ok = false;
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
@@ -2112,7 +2112,7 @@ struct GhostPlan {
   std::vector<GRule> rules;
   std::vector<int> roff[3];
   bool maps_ok = false;
-  HostMap map[5];
+  HostMap map[6];
   std::string why;  // why the flattened maps are not used
 };
 
@@ -2555,8 +2555,17 @@ static bool same_expr(const GhostPlan& G, const FExpr& a, const FExpr& b) {
 // ghosts can differ, and the colour-1 sweep does not read them.  Edge and
 // corner ghosts are not read by 7-point stencils; those both ipasses write
 // are restored by the last SetGhost of the sequence, which is full, the
-// others are rewritten at every call of their ipass, and SetGhost(level,
-// 0) (after CorrectFine) writes every cell any ipass writes.
+// others are rewritten at every call of their ipass.
+//
+// map[5], the colour-1 face ghosts of SetGhost(level, 0), may stand for it
+// when a sweep sequence follows (the post-smoothing after CorrectFine):
+// the first sweep (colour 0) reads exactly those ghosts.  Every colour-0
+// face ghost is rewritten by the SetGhost(1) after it; a colour-1 face
+// ghost that SetGhost(1) writes has the same expression at ipass 0, 1 and
+// 2 (a copy, since copies follow interpolation), so after map[5] it holds
+// what the full calls would give until SetGhost(2) rewrites it; and
+// SetGhost(level, 0) writes no cell that neither ipass 1 nor 2 writes, so
+// the sequence ends with every cell as the staged fills leave it.
 int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   G.maps_ok = false;
   if (L.level <= 0) return BSAM_OK;
@@ -2609,7 +2618,25 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
     for (size_t n = 0; n < order.size(); n++) order[n] = (int)n;
     std::sort(order.begin(), order.end(), [&](int x, int y) { return E[p].dst[x] < E[p].dst[y]; });
     emit(E[p], order, G.map[p]);
-    if (p == 0) continue;
+    if (p == 0) {
+      // map[5]: the colour-1 face ghosts of SetGhost(level, 0)
+      std::vector<int> sel;
+      for (int n : order) {
+        const FExpr& e = E[0].ex[n];
+        const DPatch& D = c->hp[e.dpid];
+        int i, j, k;
+        qdecode(D, E[0].dst[n], i, j, k);
+        if (n_outside(D, nd, i, j, k) == 1 && ((i + j + k) & 1) == 1) {
+          if (e.early) {
+            G.why = "a colour-1 face ghost of SetGhost(level, 0) reads a ghost from before the call";
+            return BSAM_OK;
+          }
+          sel.push_back(n);
+        }
+      }
+      emit(E[0], sel, G.map[5]);
+      continue;
+    }
     const int col = p - 1;
     const FlatCall& other = E[3 - p];
     std::vector<int> lean;
@@ -2676,7 +2703,7 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
   }
   L.maps_ok = G.maps_ok;
   if (!G.maps_ok) return BSAM_OK;
-  for (int m = 0; m < 5; m++) {
+  for (int m = 0; m < 6; m++) {
     const HostMap& H = G.map[m];
     DevMap& D = L.gmap[m];
     D.nc = (int)H.copies.size();
@@ -3038,8 +3065,20 @@ extern "C" int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t n
       }
       // (2) a red-black sequence (three passes, the last SetGhost full)
       // from an arbitrary state, each sweep replaced by new values of its
-      // colour's interior cells
+      // colour's interior cells; (3) the same after SetGhost(level, 0)
+      // against its reduced table map[5]
+      for (int variant = 0; variant < 2; variant++) {
+      for (long n = 0; n < c->nq; n++) a[n] = rng.next();
       b = a;
+      if (variant == 1) {
+        host_staged(c, L, G, 0, homog, a);
+        host_map(c, G.map[5], homog, b);
+        for (int o : face[1])
+          if (memcmp(&a[o], &b[o], sizeof(double)))
+            return set_err(BSAM_EINVAL, "level %d homog %d: colour-1 face ghost %d differs after map[5]", L.level,
+                           homog, o);
+        *checked += face[1].size();
+      }
       const int ns = 3;
       for (int sweep = 0; sweep < 2 * ns; sweep++) {
         const int col = sweep & 1, ipass = col + 1;
@@ -3063,8 +3102,10 @@ extern "C" int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t n
       }
       for (long n = 0; n < c->nq; n++)
         if (memcmp(&a[n], &b[n], sizeof(double)))
-          return set_err(BSAM_EINVAL, "level %d homog %d: state after the sequence differs at %ld", L.level, homog, n);
+          return set_err(BSAM_EINVAL, "level %d homog %d variant %d: state after the sequence differs at %ld",
+                         L.level, homog, variant, n);
       *checked += c->nq;
+      }
     }
   }
   return BSAM_OK;
@@ -3197,7 +3238,7 @@ static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, b
   Level& L = c->L(level);
   const int nv = c->p.nrvars;
   if (L.maps_ok && c->p.batching) {
-    const DevMap& M = L.gmap[(lean && ipass > 0) ? 2 + ipass : ipass];
+    const DevMap& M = L.gmap[lean ? (ipass > 0 ? 2 + ipass : 5) : ipass];
     // rules reading ghosts from before the call first, then the rest
     if (M.nc1 + M.ni1 > 0) {
       k_ghost_map<T><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q);
@@ -3684,7 +3725,9 @@ extern "C" int32_t bsam_vcycle(bsam_ctx* c, int32_t level) {
     // CorrectFine + SetGhost(level, 0), fused for a periodic single patch
     const bool cw = c->L(level).wrap_pid >= 0;
     RETCHK(correct_t<double>(c, level, c->q, c->qc, cw));
-    if (!cw) RETCHK(bsam_set_ghost(c, level, 0));
+    // SetGhost(level, 0); the post-smoothing sweeps that follow read only
+    // the reduced table's ghosts before rewriting the rest
+    if (!cw) RETCHK(set_ghost_t<double>(c, level, 0, c->q, 0, true));
     RETCHK(bsam_level_relax(c, level));
   }
   return BSAM_OK;
@@ -3966,7 +4009,7 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
     RETCHK(mixed_vcycle(c, level - 1));
     const bool cw = c->L(level).wrap_pid >= 0;
     RETCHK(correct_t<float>(c, level, m->e, m->ec, cw));
-    if (!cw) RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1));
+    if (!cw) RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1, true));
     RETCHK(mixed_level_relax(c, level, 1));
   }
   return BSAM_OK;
