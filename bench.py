@@ -1,29 +1,34 @@
-"""Benchmark: Cahn-Hilliard time steps per second on one MI355X.
+"""Benchmark: AMR Cahn-Hilliard time steps per second on one MI355X, mixed
+FP32/FP64 with per-level batched launches against plain FP64.
 
 Workloads (--workload):
-  ch3d_uniform_256 (default): 256^3 uniform periodic root on [0,1]^3 with
-      multigrid levels down to 2^3, eps = 0.015, dt = 5e-5,
-      phi = 0.1 + 0.05*U(-1,1) (seed 1003), RMS residual below 1e-8.
+  ch3d_amr_64_l3 (default): 3D, 64^3 periodic root with three refined levels
+      (finest effective 512^3, 1883 patches of 16^3..64^3, 15.8 M cells),
+      eps = 0.008, dt = 1e-5, a sphere of radius 0.3 perturbed by real
+      spherical harmonics l <= 6 (seed 1004), RMS residual below 1e-8.
+  ch3d_uniform_256: 256^3 uniform periodic root on [0,1]^3 with multigrid
+      levels down to 2^3, eps = 0.015, dt = 5e-5, phi = 0.1 + 0.05*U(-1,1)
+      (seed 1003), RMS residual below 1e-8.
   ch2d_amr_128_l2: 2D, 128^2 root with two refined levels, eps = 0.01,
       dt = 1e-5, a circle of radius 0.25 plus 0.01 U(-1,1) (seed 1002),
       residual below 1e-9.
-  ch3d_amr_64_l3: 3D, 64^3 root with three refined levels (finest
-      effective 512^3), eps = 0.008, dt = 1e-5, a sphere of radius 0.3
-      perturbed by real spherical harmonics l <= 6 (seed 1004), residual
-      below 1e-8.
 The AMR meshes are the reference's own regrid result for these initial
 data (bsam_amd/workloads/*.json), frozen: regrid is not part of the path.
 Every patch is seeded with the same formula at its own level.
 
-Each time step is solved by FAS V(2,2) cycles until the RMS residual is
-below the tolerance.  A "step" is one time step of the reference's time loop
-(bsamroutines.f90:673-758) after its regrid: SetGhost(root), on the first
-step FillDown, CopyQToQold on every level, MultigridIterations.  Inputs are
-resident in HBM before timing starts.
+Each time step is solved by FAS V(2,2) cycles until the RMS residual
+(ErrorAFAS) is below the tolerance.  A "step" is one time step of the
+reference's time loop (bsamroutines.f90:673-758) after its regrid:
+SetGhost(root), on the first step FillDown, CopyQToQold on every level,
+MultigridIterations.  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line with the throughput, the roofline of the dominant
-kernel (the finest-level colour sweep), and a CPU baseline (the C oracle of
-oracle/, one thread, on a bounded sample of the same workload).
+--precision both (default) times the mixed path (`value`) and the plain
+FP64 path on the same steps, reports their ratio, and measures both
+against a FP64 run converged to 1e-12 (max-norm over every level's
+interior).  The JSON line also carries the roofline of the dominant kernel
+(the finest-level colour sweep of the headline path) and a CPU baseline
+(the C oracle of oracle/, one thread, on a bounded sample of the same
+workload).
 """
 import argparse
 import json
@@ -49,15 +54,18 @@ def _cpu_model():
 
 
 # The finest-level colour sweep is the dominant kernel (8 launches per
-# V-cycle); its name in kernel traces per precision.
-DOMINANT = {"fp64": "k_relax3d_march<double, false, true, 1>",
-            "mixed": "k_relax3d_march<float, true, true, 1>"}
+# V-cycle); its name in kernel traces per workload kind and precision.
+DOMINANT = {("uniform", "fp64"): "k_relax3d_march<double, false, true, 1>",
+            ("uniform", "mixed"): "k_relax3d_march<float, true, true, 1>",
+            ("amr", "fp64"): "k_relax_ch<3, false>",
+            ("amr", "mixed"): "k_relax_lin<3, false>"}
 
 
 def measured_traffic(workload, precision, kernel):
-    """HBM bytes per launch of `kernel` from the committed counter digest
-    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the copy kernel), or
-    None when no digest for this workload and precision is present."""
+    """HBM bytes per launch of `kernel` at its largest grid (the finest
+    level) from the committed counter digest (rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs, scaled by the copy kernel), or None when no digest for
+    this workload and precision is present."""
     path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_hbm_traffic.json")
     if not os.path.exists(path):
         return None
@@ -74,16 +82,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="ch3d_uniform_256",
-                    choices=["ch3d_uniform_256", "ch2d_amr_128_l2", "ch3d_amr_64_l3"])
+    ap.add_argument("--workload", default="ch3d_amr_64_l3",
+                    choices=["ch3d_amr_64_l3", "ch3d_uniform_256", "ch2d_amr_128_l2"])
     ap.add_argument("--n", type=int, default=256, help="root cells per side of the uniform workload")
     ap.add_argument("--batching", type=int, default=1,
                     help="1: one launch per level (default); 0: one launch per patch")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--check-fp64", type=int, default=1,
-                    help="mixed mode: also run the FP64 path and report the max-norm difference")
+    ap.add_argument("--check", type=int, default=1,
+                    help="measure the paths' max-norm error against FP64 converged to 1e-12")
     ap.add_argument("--kstat-steps", type=int, default=1,
                     help="extra instrumented steps for the kernel roofline (0: skip)")
     return ap.parse_args()
@@ -143,57 +151,30 @@ class Workload:
             S.copy_q_to_qold(lv)
 
 
-def make_state(wl, precision, use_graph, batching):
+def make_state(wl, precision, use_graph, batching, tol=None):
     from bsam_amd import gpu
-    S = gpu.Solver(wl.params(precision, use_graph, batching), wl.hier)
+    p = wl.params(precision, use_graph, batching)
+    if tol is not None:
+        p.qerrortol = tol
+    S = gpu.Solver(p, wl.hier)
     wl.load(S)
     return S
 
 
-def cpu_baseline(wl):
-    """The second time step of the same workload on the C oracle (one
-    thread); the first step, untimed, brings it to the state the timed GPU
-    steps start from."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    O = oracle.Oracle(wl.params("fp64", 0, 1), wl.hier)
-    wl.load(O)
-    O.time_step(True)
-    # time steps 2, 3, ... until about 10 s of CPU work or the workload's
-    # step count
-    t0 = time.perf_counter()
-    nsteps = ncyc = 0
-    while True:
-        rc, nc, r = O.time_step(False)
-        nsteps += 1
-        ncyc += nc
-        dt = time.perf_counter() - t0
-        if dt >= 10.0 or nsteps + 1 >= wl.steps:
-            break
-    return dict(value=nsteps / dt, unit="time-steps/s", cores=1, kind="port",
-                sample=f"time steps 2..{nsteps + 1} of the {wl.name} workload ({ncyc} V-cycles, {dt:.2f} s, "
-                       f"{1e3 * dt / max(ncyc, 1):.1f} ms per V-cycle) on the C oracle "
-                       f"(oracle/bsam_oracle.c, gcc -O2, 1 thread, {_cpu_model()})")
-
-
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
-        sys.exit(2)
+def timed_path(wl, a, precision):
+    """Warm-up, then exactly a.steps timed time steps of one path, then the
+    instrumented steps.  Returns (solver, elapsed s, V-cycles per timed
+    step, roofline or None)."""
     from bsam_amd import abi
-    wl = Workload(a.workload, a.n)
-    S = make_state(wl, a.precision, a.graph, a.batching)
-    print(f"[bench] {wl.name} {a.precision} graph={a.graph} batching={a.batching}: warmup {a.warmup}",
+    S = make_state(wl, precision, a.graph, a.batching)
+    print(f"[bench] {wl.name} {precision} graph={a.graph} batching={a.batching}: warmup {a.warmup}",
           file=sys.stderr, flush=True)
     first = True
-    cycles = []
     for _ in range(a.warmup):
-        rc, ncyc, r = S.time_step(first)
+        S.time_step(first)
         first = False
     S.synchronize()
-    print(f"[bench] timing {a.steps} steps", file=sys.stderr, flush=True)
+    cycles = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         rc, ncyc, r = S.time_step(first)
@@ -202,11 +183,9 @@ def main():
         if rc != abi.BSAM_OK:
             print(f"[bench] step did not converge (rc={rc}, residual {r})", file=sys.stderr)
     S.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms_step = 1000.0 * elapsed / a.steps
-    print(f"[bench] {ms_step:.2f} ms/step, V-cycles {cycles}", file=sys.stderr, flush=True)
-
+    elapsed = time.perf_counter() - t0
+    print(f"[bench] {precision}: {1e3 * elapsed / a.steps:.2f} ms/step, V-cycles {cycles}", file=sys.stderr,
+          flush=True)
     roof = None
     if a.kstat_steps > 0:
         S.kstat_enable(True)
@@ -216,64 +195,117 @@ def main():
         S.kstat_enable(False)
         avg_ms = ms / max(nl, 1)
         achieved = b / (avg_ms * 1e-3) / 1e9 if nl else None
-        kname = DOMINANT[a.precision] if wl.name.startswith("ch3d_uniform") else \
-            ("k_relax_ch<%d, false> / k_relax_lin (finest level, all patches)" % wl.nd)
-        roof = dict(bound="hbm", kernel=f"{kname} (finest-level colour sweep)", achieved=achieved,
+        kind = "uniform" if wl.name.startswith("ch3d_uniform") else "amr"
+        kname = DOMINANT.get((kind, precision), "k_relax_ch / k_relax_lin")
+        roof = dict(bound="hbm", kernel=f"{kname} (finest-level colour sweep, {precision})", achieved=achieved,
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=(achieved / HBM_PEAK_GBS) if achieved else None,
-                    traffic=measured_traffic(wl.name, a.precision, kname),
+                    traffic=measured_traffic(wl.name, precision, kname),
                     avg_us=avg_ms * 1e3, launches=nl, bytes_per_launch=b)
-    err = None
-    if a.precision == "mixed" and a.check_fp64:
-        # the FP64 path on the same steps (bit-identical to the oracle in the
-        # parity tests) as the reference for the mixed path's error
-        import numpy as np
-        print("[bench] fp64 reference run for the mixed-precision error", file=sys.stderr, flush=True)
-        keep = [k for k, d in enumerate(wl.hier) if d["level"] >= 0]
-        qm = {k: S.get(k, "q") for k in keep}
+    return S, elapsed, sum(cycles) / a.steps, roof
+
+
+def fields(wl, S):
+    return {k: S.get(k, "q") for k, d in enumerate(wl.hier) if d["level"] >= 0}
+
+
+def max_err(wl, qa, qb):
+    import numpy as np
+    inner = (slice(1, -1),) * wl.nd
+    e = [0.0, 0.0]
+    for k in qa:
+        for v in range(2):
+            e[v] = max(e[v], float(np.max(np.abs(qa[k][(v,) + inner] - qb[k][(v,) + inner]))))
+    return {"phi": e[0], "mu": e[1]}
+
+
+def cpu_baseline(wl, vcycles_per_step, ncyc=3):
+    """The C oracle (one thread) on the same workload: the first `ncyc`
+    V-cycles of the first time step are timed (about 20 s on the 3D AMR
+    mesh), and the time per step is that per V-cycle times the V-cycles per
+    step of the FP64 path, which is bit-identical to the oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    p = wl.params("fp64", 0, 1)
+    p.maxvcycles = ncyc
+    O = oracle.Oracle(p, wl.hier)
+    wl.load(O)
+    t0 = time.perf_counter()
+    rc, nc, r = O.time_step(True)
+    dt = time.perf_counter() - t0
+    per_cycle = dt / max(nc, 1)
+    return dict(value=1.0 / (per_cycle * vcycles_per_step), unit="time-steps/s", cores=1, kind="port",
+                sample=f"first {nc} V-cycles of time step 1 of the {wl.name} workload ({dt:.2f} s, "
+                       f"{1e3 * per_cycle:.1f} ms per V-cycle incl. the step set-up) scaled by "
+                       f"{vcycles_per_step:.2f} V-cycles per step, on the C oracle "
+                       f"(oracle/bsam_oracle.c, gcc -O2, 1 thread, {_cpu_model()})")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    wl = Workload(a.workload, a.n)
+    paths = ["mixed", "fp64"] if a.precision == "both" else [a.precision]
+    res = {}
+    for pr in paths:
+        S, elapsed, vc, roof = timed_path(wl, a, pr)
+        res[pr] = dict(elapsed=elapsed, vc=vc, roof=roof, q=fields(wl, S) if a.check else None)
         S.close()
-        R = make_state(wl, "fp64", a.graph, a.batching)
-        first_r = True
-        for _ in range(a.warmup + a.steps + a.kstat_steps):
-            R.time_step(first_r)
-            first_r = False
-        inner = (slice(1, -1),) * wl.nd
-        ephi = emu = 0.0
-        for k in keep:
-            qr = R.get(k, "q")
-            ephi = max(ephi, float(np.max(np.abs(qm[k][(0,) + inner] - qr[(0,) + inner]))))
-            emu = max(emu, float(np.max(np.abs(qm[k][(1,) + inner] - qr[(1,) + inner]))))
-        err = {"phi": ephi, "mu": emu, "steps": a.warmup + a.steps + a.kstat_steps}
+    head = paths[0]
+    check = None
+    if a.check:
+        # FP64 converged to 1e-12 on the same number of steps: the error
+        # scale of stopping at the tolerance, for both paths
+        nst = a.warmup + a.steps + a.kstat_steps
+        print(f"[bench] FP64 to 1e-12 over {nst} steps for the error check", file=sys.stderr, flush=True)
+        R = make_state(wl, "fp64", a.graph, a.batching, tol=1e-12)
+        for s in range(nst):
+            R.time_step(s == 0)
+        qr = fields(wl, R)
         R.close()
+        check = {"steps": nst, "reference": "FP64 path converged to RMS residual 1e-12"}
+        for pr in paths:
+            check[f"{pr}_vs_converged"] = max_err(wl, res[pr]["q"], qr)
+        if len(paths) == 2:
+            check["mixed_vs_fp64"] = max_err(wl, res["mixed"]["q"], res["fp64"]["q"])
     cpu = None
     if a.cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(wl)
-    vc = sum(cycles)
+        cpu = cpu_baseline(wl, res["fp64" if "fp64" in res else head]["vc"])
+    el = res[head]["elapsed"]
     out = {
-        "metric": f"AMR CH time-steps/s at ||r||<={wl.tol:g} ({wl.nd}D CH, "
-                  + ("plain FP64 path)" if a.precision == "fp64" else "mixed FP32/FP64 path)"),
-        "value": a.steps / elapsed,
+        "metric": "AMR CH time-steps/s at ||r||<=1e-8: mixed+batched vs plain FP64; HBM GB/s"
+                  if wl.tol == 1e-8 else f"AMR CH time-steps/s at ||r||<={wl.tol:g}: mixed+batched vs plain FP64",
+        "value": a.steps / el,
         "unit": "time-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1000.0 * el / a.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": wl.name, "mesh": wl.label,
                    "eps": wl.eps, "dt": wl.dt, "tol": wl.tol, "cycle": "FAS V(2,2)",
-                   "precision": a.precision, "hipgraph": bool(a.graph),
+                   "path": ("mixed FP32/FP64 defect correction" if head == "mixed" else "plain FP64"),
+                   "hipgraph": bool(a.graph),
                    "launches": "one per level" if a.batching else "one per patch"},
-        "vcycles_per_step": vc / a.steps,
-        "vcycles_per_s": vc / elapsed,
-        "roofline": roof,
+        "vcycles_per_step": res[head]["vc"],
+        "vcycles_per_s": res[head]["vc"] * a.steps / el,
+        "roofline": res[head]["roof"],
         "cpu_baseline": cpu,
     }
-    if err is not None:
-        out["max_norm_error_vs_fp64"] = err
+    if len(paths) == 2:
+        f = res["fp64"]
+        out["plain_fp64"] = {"value": a.steps / f["elapsed"], "ms_per_step": 1000.0 * f["elapsed"] / a.steps,
+                             "vcycles_per_step": f["vc"], "roofline": f["roof"]}
+        out["speedup_vs_fp64"] = f["elapsed"] / el
+    if check is not None:
+        out["max_norm_error"] = check
     print(json.dumps(out))
 
 
