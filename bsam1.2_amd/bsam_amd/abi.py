@@ -17,6 +17,7 @@ BSAM_ENODEV = -6
 
 MODEL_CH = 1
 MODEL_BG = 2
+MODEL_TUMOUR = 3
 
 FIELD_Q, FIELD_QOLD, FIELD_QC, FIELD_F, FIELD_RF, FIELD_FTMP = range(6)
 FIELD_NAMES = {"q": FIELD_Q, "qold": FIELD_QOLD, "qc": FIELD_QC,

@@ -118,9 +118,12 @@ def init_phi(desc, ndims, mode, seed, base=0.0, amp=0.05, radius=0.25, eps=0.03,
 
 def params(ndims, dt, eps, qerrortol, maxvcycles=50, nsmoothingpasses=2, precision=abi.PREC_FP64,
            model=abi.MODEL_CH, batching=1, use_graph=0, errortype=1, extra=()):
+    """Run parameters.  `extra` are the model constants after eps^2: BG
+    (gamma, lambda, czero); tumour (proliferation, apoptosis, nutrient
+    consumption)."""
     p = abi.Params()
     p.ndims = ndims
-    p.nrvars = 2
+    p.nrvars = 4 if model == abi.MODEL_TUMOUR else 2
     p.mbc = 1
     p.maux = 0
     p.model = model

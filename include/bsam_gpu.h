@@ -47,6 +47,10 @@ extern "C" {
 /* ---- models (the PROBLEM-module hook sets the library implements) ---- */
 #define BSAM_MODEL_CH        1   /* PROBLEM/CH/problem.f90 hooks */
 #define BSAM_MODEL_BG        2   /* PROBLEM/BioGrowth/problem.f90 2D hooks */
+#define BSAM_MODEL_TUMOUR    3   /* tumour growth (phi, mu, nutrient, pressure; nrvars = 4):
+                                    a model the reference does not ship, written with
+                                    the hook signatures of PROBLEM/CH (the oracle's probe
+                                    PROBLEM module runs it inside the reference LIBRARY) */
 
 /* ---- field identifiers for upload/download (nodeinfodef.f90:196-214) - */
 #define BSAM_FIELD_Q         0   /* q   (1-mbc:mx+mbc)^d x nrvars */
@@ -96,7 +100,7 @@ typedef struct bsam_patch_desc {
  */
 typedef struct bsam_params {
   int32_t ndims;             /* 2 or 3 */
-  int32_t nrvars;            /* 2 for CH and BG */
+  int32_t nrvars;            /* 2 for CH and BG, 4 for the tumour model */
   int32_t mbc;               /* must be 1 */
   int32_t maux;              /* must be 0 (CH and BG use no aux fields) */
   int32_t model;             /* BSAM_MODEL_* */
@@ -110,7 +114,8 @@ typedef struct bsam_params {
   double  dt;
   double  qerrortol;         /* reference default 1e-6 */
   double  model_params[8];   /* CH: [0]=eps^2.  BG: [0]=eps^2,[1]=gamma,[2]=lambda,[3]=czero
-                                (BioGrowth/problemdef.f90:5) */
+                                (BioGrowth/problemdef.f90:5).  Tumour: [0]=eps^2,
+                                [1]=proliferation, [2]=apoptosis, [3]=nutrient consumption */
 } bsam_params;
 
 typedef struct bsam_ctx bsam_ctx;

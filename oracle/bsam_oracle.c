@@ -238,6 +238,47 @@ double* orc_field_ptr(orc_ctx* c, int32_t n, int32_t field) {
 
 /* ---- model hooks --------------------------------------------------------- */
 
+/* The tumour model's relaxation (the probe PROBLEM module's TumourRelax2D/
+ * 3D, oracle/ref_problem/problem.f90): per cell of the colour, the nutrient
+ * n from -ULap(n)/h^2 + kappa t0 n = f3, the CH pair by RelaxGrid's Newton
+ * step with dt*lamp*t0*n added to its first right-hand side, then the
+ * pressure from -ULap(p)/h^2 - lamp t0 n = f4; t0 = (1 + phi_old)/2. */
+static void relax_tumour(orc_ctx* c, opatch* P, int ipass) {
+  const int nd = c->ndims;
+  const double h = P->d.dx, dt = c->p.dt, eps2 = c->p.model_params[0];
+  const double lamp = c->p.model_params[1], kappa = c->p.model_params[3];
+  const double h2 = h * h, dd = nd == 3 ? 6.0 : 4.0;
+  const double cdt = dt / h2, ceps = eps2 / h2, cdiag = dd * eps2 / h2, c12 = dd * cdt;
+  const double chh = 1.0 / h2, cd = dd / h2;
+  const int mz = nd == 3 ? P->mx[2] : 1;
+  for (int k = 1; k <= mz; k++)
+    for (int j = 1; j <= P->mx[1]; j++)
+      for (int i = 1 + ((nd == 3 ? k : 0) + j + ipass) % 2; i <= P->mx[0]; i += 2) {
+        const int kk = nd == 3 ? k : 1;
+#define NB(v, op) ((nd == 3) ? (Q(P, i - 1, j, kk, v) op Q(P, i + 1, j, kk, v) op Q(P, i, j - 1, kk, v) op \
+                                Q(P, i, j + 1, kk, v) op Q(P, i, j, kk - 1, v) op Q(P, i, j, kk + 1, v)) \
+                             : (Q(P, i - 1, j, kk, v) op Q(P, i + 1, j, kk, v) op Q(P, i, j - 1, kk, v) op \
+                                Q(P, i, j + 1, kk, v)))
+#define NBP(v) ((nd == 3) ? (Q(P, i + 1, j, kk, v) + Q(P, i - 1, j, kk, v) + Q(P, i, j + 1, kk, v) + \
+                             Q(P, i, j - 1, kk, v) + Q(P, i, j, kk + 1, v) + Q(P, i, j, kk - 1, v)) \
+                          : (Q(P, i + 1, j, kk, v) + Q(P, i - 1, j, kk, v) + Q(P, i, j + 1, kk, v) + \
+                             Q(P, i, j - 1, kk, v)))
+        const double t0 = 0.5 * (QOLD(P, i, j, kk, 0) + 1.0);
+        const double n = (F(P, i, j, kk, 2) + chh * NB(2, +)) / (cd + kappa * t0);
+        Q(P, i, j, kk, 2) = n;
+        const double p = Q(P, i, j, kk, 0);
+        const double c21 = -3.0 * (p * p) - cdiag;
+        const double r1 = F(P, i, j, kk, 0) + cdt * NB(1, +) + dt * lamp * t0 * n;
+        const double r2 = F(P, i, j, kk, 1) - ceps * NBP(0) - 2.0 * (p * p * p);
+        const double d = 1.0 - c12 * c21;
+        Q(P, i, j, kk, 0) = (r1 - c12 * r2) / d;
+        Q(P, i, j, kk, 1) = (r2 - c21 * r1) / d;
+        Q(P, i, j, kk, 3) = (F(P, i, j, kk, 3) + chh * NB(3, +) + lamp * t0 * n) / cd;
+#undef NB
+#undef NBP
+      }
+}
+
 /* CH RelaxGrid2D (CH/problem.f90:318-369) and BG RelaxGrid2D
  * (BioGrowth/problem.f90:255-296). */
 static void relax_patch_2d(orc_ctx* c, opatch* P, int ipass) {
@@ -260,6 +301,8 @@ static void relax_patch_2d(orc_ctx* c, opatch* P, int ipass) {
         Q(P, i, j, 1, 0) = (b1 - a12 * b2) / det;
         Q(P, i, j, 1, 1) = (b2 - a21 * b1) / det;
       }
+  } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+    relax_tumour(c, P, ipass);
   } else { /* BSAM_MODEL_BG */
     const double gam = c->p.model_params[1], lam = c->p.model_params[2], cz = c->p.model_params[3];
     const double tmp1 = dt * eps2 / h2, tmp3 = 1.0 / h2;
@@ -282,6 +325,10 @@ static void relax_patch_2d(orc_ctx* c, opatch* P, int ipass) {
 
 /* CH RelaxGrid3D (CH/problem.f90:554-614). */
 static void relax_patch_3d(orc_ctx* c, opatch* P, int ipass) {
+  if (c->p.model == BSAM_MODEL_TUMOUR) {
+    relax_tumour(c, P, ipass);
+    return;
+  }
   const double h = P->d.dx, dt = c->p.dt, eps2 = c->p.model_params[0];
   const double h2 = h * h;
   const double tmp1 = dt / h2, tmp2 = eps2 / h2, tmp3 = 6.0 * eps2 / h2;
@@ -323,8 +370,9 @@ static inline double ulap(orc_ctx* c, qview a, int i, int j, int k, int v) {
          QV(a, i, j - 1, k, v) + QV(a, i, j, k + 1, v) + QV(a, i, j, k - 1, v) -
          6.0 * QV(a, i, j, k, v);
 }
-static inline void op_cell(orc_ctx* c, qview q, qview qo, double h, int i, int j, int k,
-                           double* o1, double* o2) {
+static inline void op_cell(orc_ctx* c, qview q, qview qo, double h, int i, int j, int k, double* o) {
+  double* o1 = &o[0];
+  double* o2 = &o[1];
   const double dt = c->p.dt, eps2 = c->p.model_params[0];
   const double h2 = h * h;
   if (c->p.model == BSAM_MODEL_CH) {
@@ -332,6 +380,18 @@ static inline void op_cell(orc_ctx* c, qview q, qview qo, double h, int i, int j
     double p = QV(q, i, j, k, 0);
     *o1 = QV(q, i, j, k, 0) - tmp1 * ulap(c, q, i, j, k, 1);
     *o2 = QV(q, i, j, k, 1) - (p * p * p) + tmp2 * ulap(c, q, i, j, k, 0);
+  } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+    /* the probe module's Operator2D/3D with model = 3 */
+    const double lamp = c->p.model_params[1], kappa = c->p.model_params[3];
+    const double cdt = dt / h2, ceps = eps2 / h2, chh = 1.0 / h2;
+    const double l2 = ulap(c, q, i, j, k, 1), l1 = ulap(c, q, i, j, k, 0);
+    const double l3 = ulap(c, q, i, j, k, 2), l4 = ulap(c, q, i, j, k, 3);
+    const double p = QV(q, i, j, k, 0), n = QV(q, i, j, k, 2);
+    const double t0 = 0.5 * (QV(qo, i, j, k, 0) + 1.0);
+    o[0] = p - cdt * l2 - dt * lamp * t0 * n;
+    o[1] = QV(q, i, j, k, 1) - (p * p * p) + ceps * l1;
+    o[2] = -chh * l3 + kappa * t0 * n;
+    o[3] = -chh * l4 - lamp * t0 * n;
   } else {
     const double gam = c->p.model_params[1], lam = c->p.model_params[2], cz = c->p.model_params[3];
     const double tmp1 = dt * eps2 / h2, tmp3 = 1.0 / h2;
@@ -350,10 +410,9 @@ static void residual_patch(orc_ctx* c, opatch* P) {
   for (int k = 1; k <= P->mx[2]; k++)
     for (int j = 1; j <= P->mx[1]; j++)
       for (int i = 1; i <= P->mx[0]; i++) {
-        double o1, o2;
-        op_cell(c, q, qo, P->d.dx, i, j, k, &o1, &o2);
-        RF(P, i, j, k, 0) = F(P, i, j, k, 0) - o1;
-        RF(P, i, j, k, 1) = F(P, i, j, k, 1) - o2;
+        double o[4];
+        op_cell(c, q, qo, P->d.dx, i, j, k, o);
+        for (int v = 0; v < c->nv; v++) RF(P, i, j, k, v) = F(P, i, j, k, v) - o[v];
       }
 }
 
@@ -429,18 +488,17 @@ static void coarse_loading_patch(orc_ctx* c, opatch* P) {
   for (int kc = 1; kc <= kc_hi(c, P); kc++)
     for (int jc = 1; jc <= P->cmx[1]; jc++)
       for (int ic = 1; ic <= P->cmx[0]; ic++) {
-        double o1, o2, r0, r1;
-        op_cell(c, qc, qo, ch, ic, jc, kc, &o1, &o2);
-#define G0(i, j, k) RF(P, i, j, k, 0)
-#define G1(i, j, k) RF(P, i, j, k, 1)
-        RESTRICT_CELL(c->ndims, G0, ic, jc, kc, r0);
-        RESTRICT_CELL(c->ndims, G1, ic, jc, kc, r1);
-#undef G0
-#undef G1
+        double o[4];
+        op_cell(c, qc, qo, ch, ic, jc, kc, o);
         int ip = mb[0][0] + ic - 1, jp = mb[1][0] + jc - 1;
         int kp = c->ndims == 3 ? mb[2][0] + kc - 1 : 1;
-        F(Par, ip, jp, kp, 0) = r0 + o1;
-        F(Par, ip, jp, kp, 1) = r1 + o2;
+        for (int v = 0; v < c->nv; v++) {
+          double r;
+#define GV(i, j, k) RF(P, i, j, k, v)
+          RESTRICT_CELL(c->ndims, GV, ic, jc, kc, r);
+#undef GV
+          F(Par, ip, jp, kp, v) = r + o[v];
+        }
       }
 }
 
@@ -536,6 +594,13 @@ static void get_source_patch(orc_ctx* c, opatch* P) {
         if (c->p.model == BSAM_MODEL_CH) {
           FT(P, i, j, k, 0) = po;
           FT(P, i, j, k, 1) = -po;
+        } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+          /* the probe module's Source2D/3D with model = 3 */
+          const double lama = c->p.model_params[2];
+          FT(P, i, j, k, 0) = po - c->p.dt * lama * (0.5 * (po + 1.0));
+          FT(P, i, j, k, 1) = -po;
+          FT(P, i, j, k, 2) = 0.0;
+          FT(P, i, j, k, 3) = -lama * (0.5 * (po + 1.0));
         } else {
           FT(P, i, j, k, 0) = (1.0 + c->p.dt) * po;
           FT(P, i, j, k, 1) = 0.0;
@@ -925,8 +990,10 @@ static int physical_bc(orc_ctx* c, opatch* P) {
     int na = P->nq[ta], nb = P->nq[tb];
     int m = P->mx[dim];
     for (int v = 0; v < c->nv; v++) {
-      /* CH 2D reflects both fields, CH 3D sets 2-q, BG reflects phi and sets 2-c */
+      /* CH 2D reflects both fields, CH 3D sets 2-q, BG reflects phi and sets
+       * 2-c; the tumour model reflects phi and mu, sets 2-n and -p */
       int dirichlet = (c->p.model == BSAM_MODEL_CH) ? (nd == 3) : (v == 1);
+      if (c->p.model == BSAM_MODEL_TUMOUR) dirichlet = (v == 2) ? 1 : (v == 3 ? 2 : 0);
       for (int b = 0; b < nb; b++)
         for (int a = 0; a < na; a++) {
           int g[3], s[3];
@@ -935,7 +1002,7 @@ static int physical_bc(orc_ctx* c, opatch* P) {
           g[dim] = hi ? m + 1 : 0;
           s[dim] = hi ? m : 1;
           double x = Q(P, s[0], s[1], s[2], v);
-          Q(P, g[0], g[1], g[2], v) = dirichlet ? 2.0 - x : x;
+          Q(P, g[0], g[1], g[2], v) = dirichlet == 2 ? -x : (dirichlet ? 2.0 - x : x);
         }
     }
   }

@@ -10,6 +10,11 @@
 ! UserBC3D 821-881) in the same floating-point evaluation order; the
 ! residual histories of this module and of the unmodified CH module are
 ! compared bit for bit in tests/test_oracle_pins.py.
+!
+! With model = 3 the same hooks carry the tumour-growth system described in
+! problemdef.f90 (nrvars = 4), a model the reference does not ship; it is
+! written here with the hook signatures of PROBLEM/CH so that the
+! reference's LIBRARY runs it unmodified.
 MODULE Problem
 USE NodeInfoDef, ONLY: r8
 IMPLICIT NONE
@@ -24,7 +29,9 @@ USE ProblemDef
 IMPLICIT NONE
 INTEGER:: ierror
 NAMELIST/problemdata/alpha, eps, initmode, seed, base, amp, radius, xc, &
-                     hroot, tagtol, dumpsweeps, dumpsrc, dumpend, pamp, lmax
+                     hroot, tagtol, dumpsweeps, dumpsrc, dumpend, pamp, lmax, &
+                     model, lamp, lama, kappa
+model = 1; lamp = 1.0_r8; lama = 0.1_r8; kappa = 10.0_r8
 alpha = 1.0_r8; eps = 0.03_r8; initmode = 1; seed = 1001; base = 0.0_r8
 amp = 0.05_r8; radius = 0.25_r8; xc = 0.5_r8; hroot = 1.0_r8/128.0_r8
 tagtol = 0.05_r8; dumpsweeps = 0; dumpsrc = .FALSE.; dumpend = .FALSE.
@@ -190,6 +197,9 @@ DO j = 1, mx(2)
     x = (REAL(i,r8)-0.5_r8)*h+xlower(1)
     q(i,j,1) = InitPhi(lev,i0+i,j0+j,1,x,y,xc(3))
     q(i,j,2) = 0.0_r8
+    IF(model==3) THEN
+      q(i,j,3) = 1.0_r8; q(i,j,4) = 0.0_r8
+    END IF
   END DO
 END DO
 END SUBROUTINE QInit2D
@@ -214,6 +224,9 @@ DO k = 1, mx(3)
       x = (REAL(i,r8)-0.5_r8)*h+xlower(1)
       q(i,j,k,1) = InitPhi(lev,i0+i,j0+j,k0+k,x,y,z)
       q(i,j,k,2) = 0.0_r8
+      IF(model==3) THEN
+        q(i,j,k,3) = 1.0_r8; q(i,j,k,4) = 0.0_r8
+      END IF
     END DO
   END DO
 END DO
@@ -317,6 +330,9 @@ REAL(KIND=r8), INTENT(IN):: h
 REAL(KIND=r8), DIMENSION(1:2), INTENT(IN):: xlower
 INTEGER:: i, j
 REAL(KIND=r8):: cdt, ceps, cdiag, c12, c21, r1, r2, p, d
+IF(model==3) THEN
+  CALL TumourRelax2D(qnew,qold,f,mx,h,ipass)
+ELSE
 cdt = dt/(h*h); ceps = eps2/(h*h); cdiag = 4.0_r8*eps2/(h*h)
 c12 = 4.0_r8*cdt
 DO j = 1, mx(2)
@@ -331,6 +347,7 @@ DO j = 1, mx(2)
     qnew(i,j,2) = (r2-c21*r1)/d
   END DO
 END DO
+END IF
 IF(ndumpedsweeps<dumpsweeps) THEN
   ndumpedsweeps = ndumpedsweeps+1
   CALL DumpHeader(2,ipass,mx,h,xlower,2,INT(SIZE(qnew),i8))
@@ -351,6 +368,9 @@ REAL(KIND=r8), INTENT(IN):: h
 REAL(KIND=r8), DIMENSION(1:3), INTENT(IN):: xlower
 INTEGER:: i, j, k
 REAL(KIND=r8):: cdt, ceps, cdiag, c12, c21, r1, r2, p, d
+IF(model==3) THEN
+  CALL TumourRelax3D(qnew,qold,f,mx,h,ipass)
+ELSE
 cdt = dt/(h*h); ceps = eps2/(h*h); cdiag = 6.0_r8*eps2/(h*h)
 c12 = 6.0_r8*cdt
 DO k = 1, mx(3)
@@ -368,6 +388,7 @@ DO k = 1, mx(3)
     END DO
   END DO
 END DO
+END IF
 IF(ndumpedsweeps<dumpsweeps) THEN
   ndumpedsweeps = ndumpedsweeps+1
   CALL DumpHeader(2,ipass,mx,h,xlower,3,INT(SIZE(qnew),i8))
@@ -405,15 +426,26 @@ REAL(KIND=r8), INTENT(IN):: h
 REAL(KIND=r8), DIMENSION(1:2), INTENT(IN):: xlower
 REAL(KIND=r8), DIMENSION(1:mx(1),1:mx(2),1:nrvars):: op
 INTEGER:: i, j
-REAL(KIND=r8):: cdt, ceps, l1, l2, p
-cdt = dt/(h*h); ceps = eps2/(h*h)
+REAL(KIND=r8):: cdt, ceps, l1, l2, l3, l4, p, ch, t0, n
+cdt = dt/(h*h); ceps = eps2/(h*h); ch = 1.0_r8/(h*h)
 DO j = 1, mx(2)
   DO i = 1, mx(1)
     l2 = qnew(i+1,j,2)+qnew(i-1,j,2)+qnew(i,j+1,2)+qnew(i,j-1,2)-4.0_r8*qnew(i,j,2)
     l1 = qnew(i+1,j,1)+qnew(i-1,j,1)+qnew(i,j+1,1)+qnew(i,j-1,1)-4.0_r8*qnew(i,j,1)
     p = qnew(i,j,1)
-    op(i,j,1) = p-cdt*l2
-    op(i,j,2) = qnew(i,j,2)-(p*p*p)+ceps*l1
+    IF(model==3) THEN
+      l3 = qnew(i+1,j,3)+qnew(i-1,j,3)+qnew(i,j+1,3)+qnew(i,j-1,3)-4.0_r8*qnew(i,j,3)
+      l4 = qnew(i+1,j,4)+qnew(i-1,j,4)+qnew(i,j+1,4)+qnew(i,j-1,4)-4.0_r8*qnew(i,j,4)
+      t0 = 0.5_r8*(qold(i,j,1)+1.0_r8)
+      n = qnew(i,j,3)
+      op(i,j,1) = p-cdt*l2-dt*lamp*t0*n
+      op(i,j,2) = qnew(i,j,2)-(p*p*p)+ceps*l1
+      op(i,j,3) = -ch*l3+kappa*t0*n
+      op(i,j,4) = -ch*l4-lamp*t0*n
+    ELSE
+      op(i,j,1) = p-cdt*l2
+      op(i,j,2) = qnew(i,j,2)-(p*p*p)+ceps*l1
+    END IF
   END DO
 END DO
 END FUNCTION Operator2D
@@ -428,8 +460,8 @@ REAL(KIND=r8), INTENT(IN):: h
 REAL(KIND=r8), DIMENSION(1:3), INTENT(IN):: xlower
 REAL(KIND=r8), DIMENSION(1:mx(1),1:mx(2),1:mx(3),1:nrvars):: op
 INTEGER:: i, j, k
-REAL(KIND=r8):: cdt, ceps, l1, l2, p
-cdt = dt/(h*h); ceps = eps2/(h*h)
+REAL(KIND=r8):: cdt, ceps, l1, l2, l3, l4, p, ch, t0, n
+cdt = dt/(h*h); ceps = eps2/(h*h); ch = 1.0_r8/(h*h)
 DO k = 1, mx(3)
   DO j = 1, mx(2)
     DO i = 1, mx(1)
@@ -438,14 +470,28 @@ DO k = 1, mx(3)
       l1 = qnew(i+1,j,k,1)+qnew(i-1,j,k,1)+qnew(i,j+1,k,1)+qnew(i,j-1,k,1) &
          + qnew(i,j,k+1,1)+qnew(i,j,k-1,1)-6.0_r8*qnew(i,j,k,1)
       p = qnew(i,j,k,1)
-      op(i,j,k,1) = p-cdt*l2
-      op(i,j,k,2) = qnew(i,j,k,2)-(p*p*p)+ceps*l1
+      IF(model==3) THEN
+        l3 = qnew(i+1,j,k,3)+qnew(i-1,j,k,3)+qnew(i,j+1,k,3)+qnew(i,j-1,k,3) &
+           + qnew(i,j,k+1,3)+qnew(i,j,k-1,3)-6.0_r8*qnew(i,j,k,3)
+        l4 = qnew(i+1,j,k,4)+qnew(i-1,j,k,4)+qnew(i,j+1,k,4)+qnew(i,j-1,k,4) &
+           + qnew(i,j,k+1,4)+qnew(i,j,k-1,4)-6.0_r8*qnew(i,j,k,4)
+        t0 = 0.5_r8*(qold(i,j,k,1)+1.0_r8)
+        n = qnew(i,j,k,3)
+        op(i,j,k,1) = p-cdt*l2-dt*lamp*t0*n
+        op(i,j,k,2) = qnew(i,j,k,2)-(p*p*p)+ceps*l1
+        op(i,j,k,3) = -ch*l3+kappa*t0*n
+        op(i,j,k,4) = -ch*l4-lamp*t0*n
+      ELSE
+        op(i,j,k,1) = p-cdt*l2
+        op(i,j,k,2) = qnew(i,j,k,2)-(p*p*p)+ceps*l1
+      END IF
     END DO
   END DO
 END DO
 END FUNCTION Operator3D
 !
 FUNCTION Source2D(qnew,qold,aux,ll,mx,nrvars,maux,mbc,h,xlower) RESULT(s)
+USE NodeInfoDef, ONLY: dt
 USE ProblemDef
 INTEGER, INTENT(IN):: ll
 REAL(KIND=r8), DIMENSION(ll:,ll:,1:), INTENT(IN):: qnew, qold, aux
@@ -458,11 +504,19 @@ IF(dumpsrc) THEN
   CALL DumpHeader(1,0,mx,h,xlower,2,INT(SIZE(qnew)+SIZE(qold),i8))
   WRITE(91) qnew, qold
 END IF
-s(:,:,1) = qold(1:mx(1),1:mx(2),1)
-s(:,:,2) = -qold(1:mx(1),1:mx(2),1)
+IF(model==3) THEN
+  s(:,:,1) = qold(1:mx(1),1:mx(2),1)-dt*lama*(0.5_r8*(qold(1:mx(1),1:mx(2),1)+1.0_r8))
+  s(:,:,2) = -qold(1:mx(1),1:mx(2),1)
+  s(:,:,3) = 0.0_r8
+  s(:,:,4) = -lama*(0.5_r8*(qold(1:mx(1),1:mx(2),1)+1.0_r8))
+ELSE
+  s(:,:,1) = qold(1:mx(1),1:mx(2),1)
+  s(:,:,2) = -qold(1:mx(1),1:mx(2),1)
+END IF
 END FUNCTION Source2D
 !
 FUNCTION Source3D(qnew,qold,aux,ll,mx,nrvars,maux,mbc,h,xlower) RESULT(s)
+USE NodeInfoDef, ONLY: dt
 USE ProblemDef
 INTEGER, INTENT(IN):: ll
 REAL(KIND=r8), DIMENSION(ll:,ll:,ll:,1:), INTENT(IN):: qnew, qold, aux
@@ -475,8 +529,16 @@ IF(dumpsrc) THEN
   CALL DumpHeader(1,0,mx,h,xlower,3,INT(SIZE(qnew)+SIZE(qold),i8))
   WRITE(91) qnew, qold
 END IF
-s(:,:,:,1) = qold(1:mx(1),1:mx(2),1:mx(3),1)
-s(:,:,:,2) = -qold(1:mx(1),1:mx(2),1:mx(3),1)
+IF(model==3) THEN
+  s(:,:,:,1) = qold(1:mx(1),1:mx(2),1:mx(3),1) &
+             - dt*lama*(0.5_r8*(qold(1:mx(1),1:mx(2),1:mx(3),1)+1.0_r8))
+  s(:,:,:,2) = -qold(1:mx(1),1:mx(2),1:mx(3),1)
+  s(:,:,:,3) = 0.0_r8
+  s(:,:,:,4) = -lama*(0.5_r8*(qold(1:mx(1),1:mx(2),1:mx(3),1)+1.0_r8))
+ELSE
+  s(:,:,:,1) = qold(1:mx(1),1:mx(2),1:mx(3),1)
+  s(:,:,:,2) = -qold(1:mx(1),1:mx(2),1:mx(3),1)
+END IF
 END FUNCTION Source3D
 !
 FUNCTION SourceUpdate2D(qnew,qold,aux,ll,mx,nrvars,maux,mbc,h,xlower) RESULT(s)
@@ -503,6 +565,7 @@ END FUNCTION SourceUpdate3D
 !
 ! Physical boundaries (mthbc=10): 2D reflection, 3D q_ghost = 2 - q_in.
 SUBROUTINE UserBC2D(q,ll,mx,nrvars,mbc,bcnow)
+USE ProblemDef, ONLY: model
 INTEGER, INTENT(IN):: ll
 REAL(KIND=r8), DIMENSION(ll:,ll:,1:), INTENT(IN OUT):: q
 INTEGER, DIMENSION(1:2), INTENT(IN):: mx
@@ -515,15 +578,51 @@ DO b = 1, mbc
   CASE(3); q(:,1-b,:) = q(:,b,:)
   CASE(4); q(:,mx(2)+b,:) = q(:,mx(2)-b+1,:)
   END SELECT
+  IF(model==3) THEN
+    ! the tumour system: phi, mu reflected; n = 1 and p = 0 on the face
+    SELECT CASE(bcnow)
+    CASE(1); q(1-b,:,3) = 2.0_r8-q(b,:,3); q(1-b,:,4) = -q(b,:,4)
+    CASE(2); q(mx(1)+b,:,3) = 2.0_r8-q(mx(1)-b+1,:,3); q(mx(1)+b,:,4) = -q(mx(1)-b+1,:,4)
+    CASE(3); q(:,1-b,3) = 2.0_r8-q(:,b,3); q(:,1-b,4) = -q(:,b,4)
+    CASE(4); q(:,mx(2)+b,3) = 2.0_r8-q(:,mx(2)-b+1,3); q(:,mx(2)+b,4) = -q(:,mx(2)-b+1,4)
+    END SELECT
+  END IF
 END DO
 END SUBROUTINE UserBC2D
 !
 SUBROUTINE UserBC3D(q,ll,mx,nrvars,mbc,bcnow)
+USE ProblemDef, ONLY: model
 INTEGER, INTENT(IN):: ll
 REAL(KIND=r8), DIMENSION(ll:,ll:,ll:,1:), INTENT(IN OUT):: q
 INTEGER, DIMENSION(1:3), INTENT(IN):: mx
 INTEGER, INTENT(IN):: nrvars, mbc, bcnow
 INTEGER:: b
+IF(model==3) THEN
+  ! the tumour system: phi, mu reflected; n = 1 and p = 0 on the face
+  DO b = 1, mbc
+    SELECT CASE(bcnow)
+    CASE(1)
+      q(1-b,:,:,1:2) = q(b,:,:,1:2)
+      q(1-b,:,:,3) = 2.0_r8-q(b,:,:,3); q(1-b,:,:,4) = -q(b,:,:,4)
+    CASE(2)
+      q(mx(1)+b,:,:,1:2) = q(mx(1)-b+1,:,:,1:2)
+      q(mx(1)+b,:,:,3) = 2.0_r8-q(mx(1)-b+1,:,:,3); q(mx(1)+b,:,:,4) = -q(mx(1)-b+1,:,:,4)
+    CASE(3)
+      q(:,1-b,:,1:2) = q(:,b,:,1:2)
+      q(:,1-b,:,3) = 2.0_r8-q(:,b,:,3); q(:,1-b,:,4) = -q(:,b,:,4)
+    CASE(4)
+      q(:,mx(2)+b,:,1:2) = q(:,mx(2)-b+1,:,1:2)
+      q(:,mx(2)+b,:,3) = 2.0_r8-q(:,mx(2)-b+1,:,3); q(:,mx(2)+b,:,4) = -q(:,mx(2)-b+1,:,4)
+    CASE(5)
+      q(:,:,1-b,1:2) = q(:,:,b,1:2)
+      q(:,:,1-b,3) = 2.0_r8-q(:,:,b,3); q(:,:,1-b,4) = -q(:,:,b,4)
+    CASE(6)
+      q(:,:,mx(3)+b,1:2) = q(:,:,mx(3)-b+1,1:2)
+      q(:,:,mx(3)+b,3) = 2.0_r8-q(:,:,mx(3)-b+1,3); q(:,:,mx(3)+b,4) = -q(:,:,mx(3)-b+1,4)
+    END SELECT
+  END DO
+  RETURN
+END IF
 DO b = 1, mbc
   SELECT CASE(bcnow)
   CASE(1); q(1-b,:,:,:) = 2.0_r8-q(b,:,:,:)
@@ -535,5 +634,77 @@ DO b = 1, mbc
   END SELECT
 END DO
 END SUBROUTINE UserBC3D
+!
+! ---- the tumour-growth system (model 3) ----------------------------------
+! Pointwise Gauss-Seidel in the cell: the nutrient from its own equation,
+! then the Cahn-Hilliard pair by the Newton step of RelaxGrid (with the new
+! nutrient in the proliferation term), then the pressure.
+SUBROUTINE TumourRelax2D(qnew,qold,f,mx,h,ipass)
+USE NodeInfoDef
+USE ProblemDef
+REAL(KIND=r8), DIMENSION(0:,0:,1:), INTENT(IN OUT):: qnew
+REAL(KIND=r8), DIMENSION(0:,0:,1:), INTENT(IN):: qold
+REAL(KIND=r8), DIMENSION(1:,1:,1:), INTENT(IN):: f
+INTEGER, DIMENSION(1:2), INTENT(IN):: mx
+REAL(KIND=r8), INTENT(IN):: h
+INTEGER, INTENT(IN):: ipass
+INTEGER:: i, j
+REAL(KIND=r8):: cdt, ceps, cdiag, c12, c21, r1, r2, p, d, ch, cd, t0, n
+cdt = dt/(h*h); ceps = eps2/(h*h); cdiag = 4.0_r8*eps2/(h*h)
+c12 = 4.0_r8*cdt; ch = 1.0_r8/(h*h); cd = 4.0_r8/(h*h)
+DO j = 1, mx(2)
+  DO i = 1+MODULO(j+ipass,2), mx(1), 2
+    t0 = 0.5_r8*(qold(i,j,1)+1.0_r8)
+    n = (f(i,j,3)+ch*(qnew(i-1,j,3)+qnew(i+1,j,3)+qnew(i,j-1,3)+qnew(i,j+1,3)))/(cd+kappa*t0)
+    qnew(i,j,3) = n
+    p = qnew(i,j,1)
+    c21 = -3.0_r8*(p*p)-cdiag
+    r1 = f(i,j,1)+cdt*(qnew(i-1,j,2)+qnew(i+1,j,2)+qnew(i,j-1,2)+qnew(i,j+1,2))+dt*lamp*t0*n
+    r2 = f(i,j,2)-ceps*(qnew(i+1,j,1)+qnew(i-1,j,1)+qnew(i,j+1,1)+qnew(i,j-1,1)) &
+       - 2.0_r8*(p*p*p)
+    d = 1.0_r8-c12*c21
+    qnew(i,j,1) = (r1-c12*r2)/d
+    qnew(i,j,2) = (r2-c21*r1)/d
+    qnew(i,j,4) = (f(i,j,4)+ch*(qnew(i-1,j,4)+qnew(i+1,j,4)+qnew(i,j-1,4)+qnew(i,j+1,4)) &
+                + lamp*t0*n)/cd
+  END DO
+END DO
+END SUBROUTINE TumourRelax2D
+!
+SUBROUTINE TumourRelax3D(qnew,qold,f,mx,h,ipass)
+USE NodeInfoDef
+USE ProblemDef
+REAL(KIND=r8), DIMENSION(0:,0:,0:,1:), INTENT(IN OUT):: qnew
+REAL(KIND=r8), DIMENSION(0:,0:,0:,1:), INTENT(IN):: qold
+REAL(KIND=r8), DIMENSION(1:,1:,1:,1:), INTENT(IN):: f
+INTEGER, DIMENSION(1:3), INTENT(IN):: mx
+REAL(KIND=r8), INTENT(IN):: h
+INTEGER, INTENT(IN):: ipass
+INTEGER:: i, j, k
+REAL(KIND=r8):: cdt, ceps, cdiag, c12, c21, r1, r2, p, d, ch, cd, t0, n
+cdt = dt/(h*h); ceps = eps2/(h*h); cdiag = 6.0_r8*eps2/(h*h)
+c12 = 6.0_r8*cdt; ch = 1.0_r8/(h*h); cd = 6.0_r8/(h*h)
+DO k = 1, mx(3)
+  DO j = 1, mx(2)
+    DO i = 1+MODULO(k+j+ipass,2), mx(1), 2
+      t0 = 0.5_r8*(qold(i,j,k,1)+1.0_r8)
+      n = (f(i,j,k,3)+ch*(qnew(i-1,j,k,3)+qnew(i+1,j,k,3)+qnew(i,j-1,k,3) &
+        + qnew(i,j+1,k,3)+qnew(i,j,k-1,3)+qnew(i,j,k+1,3)))/(cd+kappa*t0)
+      qnew(i,j,k,3) = n
+      p = qnew(i,j,k,1)
+      c21 = -3.0_r8*(p*p)-cdiag
+      r1 = f(i,j,k,1)+cdt*(qnew(i-1,j,k,2)+qnew(i+1,j,k,2)+qnew(i,j-1,k,2) &
+         + qnew(i,j+1,k,2)+qnew(i,j,k-1,2)+qnew(i,j,k+1,2))+dt*lamp*t0*n
+      r2 = f(i,j,k,2)-ceps*(qnew(i+1,j,k,1)+qnew(i-1,j,k,1)+qnew(i,j+1,k,1) &
+         + qnew(i,j-1,k,1)+qnew(i,j,k+1,1)+qnew(i,j,k-1,1))-2.0_r8*(p*p*p)
+      d = 1.0_r8-c12*c21
+      qnew(i,j,k,1) = (r1-c12*r2)/d
+      qnew(i,j,k,2) = (r2-c21*r1)/d
+      qnew(i,j,k,4) = (f(i,j,k,4)+ch*(qnew(i-1,j,k,4)+qnew(i+1,j,k,4)+qnew(i,j-1,k,4) &
+                    + qnew(i,j+1,k,4)+qnew(i,j,k-1,4)+qnew(i,j,k+1,4))+lamp*t0*n)/cd
+    END DO
+  END DO
+END DO
+END SUBROUTINE TumourRelax3D
 !
 END MODULE Problem

@@ -16,6 +16,16 @@ REAL(KIND=r8), PUBLIC:: base, amp, radius, hroot, tagtol, pamp
 INTEGER, PUBLIC:: lmax
 REAL(KIND=r8), DIMENSION(1:3), PUBLIC:: xc
 INTEGER, PUBLIC:: initmode, seed
+! Model: 1 Cahn-Hilliard (the hooks of PROBLEM/CH); 3 the tumour-growth
+! system (phi, mu, nutrient n, pressure p; nrvars = 4): with the lagged
+! tumour fraction t0 = (1+phi_old)/2 as BioGrowth's Growth/Uptake use it,
+!   phi - dt Lap(mu) - dt lamp t0 n = phi_old - dt lama t0
+!   mu - phi^3 + eps^2 Lap(phi)     = -phi_old
+!   -Lap(n) + kappa t0 n            = 0,  n = 1 on physical boundaries
+!   -Lap(p) - lamp t0 n             = -lama t0,  p = 0 on physical boundaries
+! (proliferation lamp, apoptosis lama, nutrient consumption kappa).
+INTEGER, PUBLIC:: model
+REAL(KIND=r8), PUBLIC:: lamp, lama, kappa
 ! Dump controls: sweeps after relaxation (tag 2), state at source build
 ! (tag 1), state after each time step (tag 3).
 INTEGER, PUBLIC:: dumpsweeps, ndumpedsweeps

@@ -15,6 +15,9 @@ from bsam_amd import abi, configs
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 NAMES = ["ch2d_u32", "ch3d_u16", "ch2d_u16_phys", "ch3d_u8_phys", "ch2d_amr", "ch2d_amr_phys",
          "ch3d_amr", "ch3d_amr_phys"]
+# the tumour-growth model (4 fields) run by the reference LIBRARY through the
+# probe PROBLEM module (model = 3)
+TUMOUR_NAMES = ["tm2d_u16_phys", "tm3d_u8_phys", "tm2d_amr_phys", "tm3d_amr_phys"]
 
 
 def load(name):
@@ -24,6 +27,9 @@ def load(name):
 
 
 def step_params(meta, **kw):
+    tm = meta.get("tumour")
+    if tm:
+        kw = dict(kw, model=abi.MODEL_TUMOUR, extra=(tm["lamp"], tm["lama"], tm["kappa"]))
     return configs.params(meta["ndims"], meta["dt"], meta["eps"], meta["tol"], **kw)
 
 
@@ -33,6 +39,26 @@ def load_state(S, meta, z, s):
         shp = S.get(i, "q").shape
         S.put(i, "q", z[f"s{s}_q_p{i}"].reshape(shp))
         S.put(i, "qold", z[f"s{s}_qold_p{i}"].reshape(shp))
+    if meta.get("tumour"):
+        below_root_qold(S, meta, z, s)
+
+
+def below_root_qold(S, meta, z, s):
+    """qold of the levels below the root as the reference holds it at
+    MultigridIterations entry: CopyQToQold on every level copies q there,
+    and q below the root is the FillDown of the root's q at the end of the
+    previous step (or of the initial data, bsamroutines.f90:693-702), which
+    is this step's root qold.  Only hooks that read qold on coarse levels
+    (the tumour model's lagged tumour fraction) see it."""
+    st = meta["steps"][s]
+    shp = {i: S.get(i, "q").shape for i in st["src"]}
+    for i in st["src"]:
+        S.put(i, "q", z[f"s{s}_qold_p{i}"].reshape(shp[i]))
+    S.fill_down(abi.SOLUTIONFIELD)
+    for lv in range(meta["minlevel"], 0):
+        S.copy_q_to_qold(lv)
+    for i in st["src"]:
+        S.put(i, "q", z[f"s{s}_q_p{i}"].reshape(shp[i]))
 
 
 def run_step(make_solver, meta, z, s, **kw):

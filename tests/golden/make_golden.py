@@ -57,6 +57,23 @@ FIXTURES = {
                                                        xc=(0.5, 0.5, 0.3)), sweeps=24, mgp=4),
 }
 
+# The tumour-growth model (model 3 of the probe module, nrvars = 4: phi, mu,
+# nutrient, pressure) on physical boundaries, where the nutrient is held at
+# 1 and the pressure at 0; proliferation 1.0, apoptosis 0.1, consumption 10.
+TUMOUR = dict(model=3, lamp=1.0, lama=0.1, kappa=10.0)
+FIXTURES.update({
+    "tm2d_u16_phys": dict(ndims=2, n=16, maxlevel=0, mthbc=(10,) * 4, eps=0.06, dt=1e-3, steps=2, tol=1e-9,
+                          init=dict(initmode=2, seed=1051, amp=0.01, radius=0.3), sweeps=20, tumour=TUMOUR),
+    "tm3d_u8_phys": dict(ndims=3, n=8, maxlevel=0, mthbc=(10,) * 6, eps=0.1, dt=1e-3, steps=2, tol=1e-9,
+                         init=dict(initmode=2, seed=1052, amp=0.01, radius=0.3), sweeps=12, tumour=TUMOUR),
+    "tm2d_amr_phys": dict(ndims=2, n=32, maxlevel=2, mthbc=(10,) * 4, eps=0.02, dt=1e-4, steps=2, tol=1e-9,
+                          init=dict(initmode=2, seed=1053, amp=0.01, radius=0.2), sweeps=120, mgp=4,
+                          tumour=TUMOUR),
+    "tm3d_amr_phys": dict(ndims=3, n=16, maxlevel=1, mthbc=(10,) * 6, eps=0.04, dt=1e-4, steps=1, tol=1e-9,
+                          init=dict(initmode=2, seed=1054, amp=0.01, radius=0.2), sweeps=24, mgp=4,
+                          tumour=TUMOUR),
+})
+
 # Probe module vs the reference's own PROBLEM/CH module from one restart frame.
 STOCK_PINS = {
     "stock_ch2d_u32": dict(ndims=2, n=32, mthbc=(2, 2, 2, 2), eps=0.03, dt=1e-4, steps=2, seed=1021),
@@ -125,6 +142,8 @@ def problem_items(fx, n, dumpsweeps):
         for a in range(3):
             it.append((f"xc({a+1})", float(xc[a])))
     it += [("tagtol", 0.05), ("dumpsweeps", dumpsweeps), ("dumpsrc", True), ("dumpend", True)]
+    for k, v in fx.get("tumour", {}).items():
+        it.append((k, v))
     return it
 
 
@@ -161,7 +180,7 @@ def make_fixture(name, fx, outdir):
         refrun.write_inputs(work, ndims=nd, mx=(n,) * nd, maxlevel=fx["maxlevel"], minlevel=minlevel,
                             dt=fx["dt"], steps=fx["steps"], qerrortol=fx["tol"], mthbc=fx["mthbc"],
                             minimumgridpoints=fx.get("mgp", 8),
-                            problem=problem_items(fx, n, fx["sweeps"]))
+                            problem=problem_items(fx, n, fx["sweeps"]), nrvars=4 if "tumour" in fx else 2)
         out = refrun.run("xbsam_probe", work)
         hist = refrun.parse_residuals(out)
         recs = refrun.parse_dump(os.path.join(work, "dump.bin"))
@@ -193,6 +212,8 @@ def make_fixture(name, fx, outdir):
     arrays = {}
     meta = dict(name=name, ndims=nd, n=n, minlevel=minlevel, eps=fx["eps"], dt=fx["dt"], tol=fx["tol"],
                 mthbc=list(fx["mthbc"]), steps=[])
+    if "tumour" in fx:
+        meta["tumour"] = dict(fx["tumour"])
     for s, st in enumerate(steps):
         hier = build_step_hierarchy(base, st["src"], hroot, nd)
         finest = max(d["level"] for d in hier)

@@ -46,7 +46,7 @@ def _nml(name, items):
 def write_inputs(d, *, ndims, mx, maxlevel, minlevel, dt, steps, qerrortol,
                  mthbc, xupper=1.0, maxvcycles=50, nsmoothingpasses=2,
                  restart=False, eps=0.03, problem=None, minimumgridpoints=8,
-                 ibuffer=1, desiredfillratio=0.8, getafterstepstats=True, errortype=1):
+                 ibuffer=1, desiredfillratio=0.8, getafterstepstats=True, errortype=1, nrvars=2):
     os.makedirs(os.path.join(d, "OUT"), exist_ok=True)
     run = [("restart", restart), ("restartframe", 0), ("syncelliptic", False),
            ("outframes", 1), ("outputuniformmesh", False),
@@ -67,7 +67,7 @@ def write_inputs(d, *, ndims, mx, maxlevel, minlevel, dt, steps, qerrortol,
     grid += [("mbc", 1)]
     for f in range(2 * ndims):
         grid += [(f"mthbc({f+1})", mthbc[f])]
-    grid += [("nrvars", 2), ("maux", 0)]
+    grid += [("nrvars", nrvars), ("maux", 0)]
     with open(os.path.join(d, "rundata.dat"), "w") as fh:
         fh.write(_nml("rundata", run))
     with open(os.path.join(d, "griddata.dat"), "w") as fh:

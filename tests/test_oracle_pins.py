@@ -5,7 +5,9 @@ make_golden.py); the oracle must reproduce them bit for bit: residual
 histories (afasroutines.f90:105), the state after each time step, and the
 state after every recorded relaxation sweep.  stock_pins.json pins the
 probe PROBLEM module used for the golden runs to the reference's own
-PROBLEM/CH hooks.
+PROBLEM/CH hooks.  The tumour fixtures (tm*) are the reference LIBRARY
+running the probe module's 4-field tumour model (model = 3), which has no
+counterpart among the reference's PROBLEM modules.
 """
 import json
 import os
@@ -18,7 +20,7 @@ import oracle
 from bsam_amd import configs, hierarchy
 
 
-@pytest.mark.parametrize("name", fu.NAMES)
+@pytest.mark.parametrize("name", fu.NAMES + fu.TUMOUR_NAMES)
 def test_multigrid_iterations_bitwise(name):
     meta, z = fu.load(name)
     for s, st in enumerate(meta["steps"]):
@@ -29,7 +31,7 @@ def test_multigrid_iterations_bitwise(name):
             assert np.array_equal(got, fu.end_state(z, s, i)), (name, s, i)
 
 
-@pytest.mark.parametrize("name", fu.NAMES)
+@pytest.mark.parametrize("name", fu.NAMES + fu.TUMOUR_NAMES)
 def test_sweeps_bitwise(name):
     meta, z = fu.load(name)
 
