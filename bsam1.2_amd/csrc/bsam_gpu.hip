@@ -66,6 +66,7 @@ extern "C" const char* bsam_last_error(void) { return g_err.c_str(); }
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kNvMax = 4;  // fields of the largest model (the tumour model)
 
 // Interpolation constants, boundary.f90:54-63.
 __constant__ double cA[3] = {-1.0 / 5.0, 2.0 / 3.0, 8.0 / 15.0};
@@ -341,6 +342,56 @@ __global__ __launch_bounds__(kSweepThreads) void k_relax3d_march(const DPatch* _
   }
 }
 
+// The tumour model's relaxation (the probe PROBLEM module's TumourRelax2D/
+// 3D): per cell of colour c, the nutrient n from its equation, the CH pair
+// by RelaxGrid's Newton step with dt*lamp*t0*n in its first right-hand
+// side, then the pressure; t0 = (1 + phi_old)/2.  ModelK: lamp in `gam`,
+// kappa in `cz`.  Domain: cells of colour c, (mx+1)/2 per row.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_relax_tm(LevelArgs L, int c, ModelK m, double* __restrict__ q,
+                                                     const double* __restrict__ qold,
+                                                     const double* __restrict__ f) {
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int t = local % cnt;
+  const int row = local / cnt;
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int ib = (c + j + k) & 1;
+  const int i = 2 * t + 2 - ib;
+  if (i > P.mx[0]) return;
+  const double h = P.dx, h2 = h * h, dd = (DIM == 3) ? 6.0 : 4.0;
+  const double cdt = m.dt / h2, ceps = m.eps2 / h2, cdiag = dd * m.eps2 / h2, c12 = dd * cdt;
+  const double chh = 1.0 / h2, cd = dd / h2;
+  const int vs = P.qvs;
+  const int o = qidx(P, i, j, k);
+  const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
+  const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
+  const int zm = (DIM == 3) ? qidx(P, i, j, k - 1) : 0, zp = (DIM == 3) ? qidx(P, i, j, k + 1) : 0;
+  // neighbour sums in the order i-1, i+1, j-1, j+1, k-1, k+1 (and i+1,
+  // i-1, ... for phi), as TumourRelax writes them
+  auto nb = [&](int off) {
+    double s0 = q[xm + off] + q[xp + off] + q[ym + off] + q[yp + off];
+    if (DIM == 3) s0 = s0 + q[zm + off] + q[zp + off];
+    return s0;
+  };
+  double sphi = q[xp] + q[xm] + q[yp] + q[ym];
+  if (DIM == 3) sphi = sphi + q[zp] + q[zm];
+  const double t0 = 0.5 * (qold[o] + 1.0);
+  const double n = (f[o + 2 * vs] + chh * nb(2 * vs)) / (cd + m.cz * t0);
+  q[o + 2 * vs] = n;
+  const double p = q[o];
+  const double c21 = -3.0 * (p * p) - cdiag;
+  const double r1 = f[o] + cdt * nb(vs) + m.dt * m.gam * t0 * n;
+  const double r2 = f[o + vs] - ceps * sphi - 2.0 * (p * p * p);
+  const double d = 1.0 - c12 * c21;
+  q[o] = (r1 - c12 * r2) / d;
+  q[o + vs] = (r2 - c21 * r1) / d;
+  q[o + 3 * vs] = (f[o + 3 * vs] + chh * nb(3 * vs) + m.gam * t0 * n) / cd;
+}
+
 // BioGrowth RelaxGrid2D, BioGrowth/problem.f90:255-296 (decoupled GS).
 __global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, ModelK m, double h,
                                                        double* __restrict__ q,
@@ -429,6 +480,40 @@ __device__ __forceinline__ void op_q(const double* __restrict__ q, const double*
   op_from<DIM, MODEL>(q[o], q[o + P.qvs], lphi, lmu, po, m, h, o1, o2);
 }
 
+// The tumour model's operator (the probe PROBLEM module's Operator2D/3D
+// with model = 3): with t0 = (1 + phi_old)/2,
+//   o1 = phi - (dt/h^2) ULap(mu) - dt lamp t0 n
+//   o2 = mu - phi^3 + (eps^2/h^2) ULap(phi)
+//   o3 = -ULap(n)/h^2 + kappa t0 n
+//   o4 = -ULap(p)/h^2 - lamp t0 n
+// ModelK carries lamp in `gam`, lama in `lam` and kappa in `cz`.
+template <int DIM>
+__device__ __forceinline__ void op_tm_from(double p, double mu, double n, double l1, double l2, double l3,
+                                           double l4, double po, const ModelK& m, double h, double* o) {
+  const double h2 = h * h;
+  const double cdt = m.dt / h2, ceps = m.eps2 / h2, chh = 1.0 / h2;
+  const double t0 = 0.5 * (po + 1.0);
+  o[0] = p - cdt * l2 - m.dt * m.gam * t0 * n;
+  o[1] = mu - (p * p * p) + ceps * l1;
+  o[2] = -chh * l3 + m.cz * t0 * n;
+  o[3] = -chh * l4 - m.gam * t0 * n;
+}
+
+// Operator of any model at cell (i,j,k): nrvars outputs.
+template <int DIM, int MODEL>
+__device__ __forceinline__ void op_cell(const double* __restrict__ q, const double* __restrict__ qold,
+                                        const DPatch& P, int i, int j, int k, const ModelK& m, double h,
+                                        double* o) {
+  if (MODEL == BSAM_MODEL_TUMOUR) {
+    const int c0 = qidx(P, i, j, k), vs = P.qvs;
+    const double l2 = ulap<DIM>(q, P, i, j, k, vs), l1 = ulap<DIM>(q, P, i, j, k, 0);
+    const double l3 = ulap<DIM>(q, P, i, j, k, 2 * vs), l4 = ulap<DIM>(q, P, i, j, k, 3 * vs);
+    op_tm_from<DIM>(q[c0], q[c0 + vs], q[c0 + 2 * vs], l1, l2, l3, l4, qold[c0], m, h, o);
+  } else {
+    op_q<DIM, MODEL>(q, qold, P, i, j, k, m, h, o[0], o[1]);
+  }
+}
+
 // Residual (afasroutines.f90:817-857): rf = f - L(q) over the interior.
 // Also accumulates sum(rf^2) per variable into per-block partials when
 // `partial` is non-null (ErrorAFAS / L2Error / SquareSum, :113-233), and
@@ -440,8 +525,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      const double* __restrict__ f,
                                                      TO* __restrict__ rf,
                                                      double* __restrict__ partial) {
+  constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
   const int lb = xcd_block();
-  double s0 = 0.0, s1 = 0.0;
+  double sv[kNvMax] = {};
   int pi, local;
   if (level_elem(L, lb, pi, local)) {
     const DPatch P = L.pt[L.patches[pi]];
@@ -449,38 +535,33 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
     const int row = local / P.mx[0];
     const int j = row % P.mx[1] + 1;
     const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
-    double o1, o2;
-    op_q<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o1, o2);
-    const int o = qidx(P, i, j, k);
-    const double r0 = f[o] - o1, r1 = f[o + P.qvs] - o2;
-    if (rf) {
-      rf[o] = (TO)r0;
-      rf[o + P.qvs] = (TO)r1;
+    double o[kNvMax];
+    op_cell<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o);
+    const int c0 = qidx(P, i, j, k);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const double r = f[c0 + v * P.qvs] - o[v];
+      if (rf) rf[c0 + v * P.qvs] = (TO)r;
+      sv[v] += r * r;
     }
-    s0 += r0 * r0;
-    s1 += r1 * r1;
   }
   if (!partial) return;
   // wavefront then block reduction in a fixed order (deterministic)
-  for (int d = 32; d > 0; d >>= 1) {
-    s0 += __shfl_down(s0, d, 64);
-    s1 += __shfl_down(s1, d, 64);
-  }
-  __shared__ double red[2][kBlock / 64];
+  for (int d = 32; d > 0; d >>= 1)
+#pragma unroll
+    for (int v = 0; v < NV; v++) sv[v] += __shfl_down(sv[v], d, 64);
+  __shared__ double red[kNvMax][kBlock / 64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) {
-    red[0][w] = s0;
-    red[1][w] = s1;
-  }
+  if (lane == 0)
+#pragma unroll
+    for (int v = 0; v < kNvMax; v++) red[v][w] = sv[v];
   __syncthreads();
   if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int x = 0; x < kBlock / 64; x++) {
-      a += red[0][x];
-      b += red[1][x];
+    for (int v = 0; v < kNvMax; v++) {
+      double a = 0.0;
+      for (int x = 0; x < kBlock / 64; x++) a += red[v][x];
+      partial[kNvMax * lb + v] = a;
     }
-    partial[2 * lb] = a;
-    partial[2 * lb + 1] = b;
   }
 }
 
@@ -723,8 +804,10 @@ __global__ __launch_bounds__(512) void k_ch3d_resid_march(const DPatch* __restri
       a += red[0][x];
       b += red[1][x];
     }
-    partial[2 * lb] = a;
-    partial[2 * lb + 1] = b;
+    partial[kNvMax * lb] = a;
+    partial[kNvMax * lb + 1] = b;
+    partial[kNvMax * lb + 2] = 0.0;
+    partial[kNvMax * lb + 3] = 0.0;
   }
 }
 
@@ -733,27 +816,20 @@ __global__ __launch_bounds__(512) void k_ch3d_resid_march(const DPatch* __restri
 // (launched with many blocks, then with one).
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ partial,
                                                             int nb, int chunk, double* __restrict__ out) {
-  __shared__ double s[2][kBlock];
-  double a = 0.0, b = 0.0;
+  __shared__ double s[kNvMax][kBlock];
+  double a[kNvMax] = {};
   const int lo = blockIdx.x * chunk, hi = min(nb, lo + chunk);
-  for (int x = lo + threadIdx.x; x < hi; x += kBlock) {
-    a += partial[2 * x];
-    b += partial[2 * x + 1];
-  }
-  s[0][threadIdx.x] = a;
-  s[1][threadIdx.x] = b;
+  for (int x = lo + threadIdx.x; x < hi; x += kBlock)
+    for (int v = 0; v < kNvMax; v++) a[v] += partial[kNvMax * x + v];
+  for (int v = 0; v < kNvMax; v++) s[v][threadIdx.x] = a[v];
   __syncthreads();
   for (int d = kBlock / 2; d > 0; d >>= 1) {
-    if ((int)threadIdx.x < d) {
-      s[0][threadIdx.x] += s[0][threadIdx.x + d];
-      s[1][threadIdx.x] += s[1][threadIdx.x + d];
-    }
+    if ((int)threadIdx.x < d)
+      for (int v = 0; v < kNvMax; v++) s[v][threadIdx.x] += s[v][threadIdx.x + d];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = s[0][0];
-    out[2 * blockIdx.x + 1] = s[1][0];
-  }
+  if (threadIdx.x == 0)
+    for (int v = 0; v < kNvMax; v++) out[kNvMax * blockIdx.x + v] = s[v][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -851,46 +927,51 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
                                                     const double* __restrict__ f,
                                                     const double* __restrict__ qc,
                                                     double* __restrict__ fpar) {
+  constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
   CoarseCell c;
   if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const double h = P.dx;
-  double r0 = 0.0, r1 = 0.0;
   const int i2 = 2 * c.ic, j2 = 2 * c.jc, k2 = 2 * c.kc;
   const int nch = (DIM == 3) ? 8 : 4;
-  double s0 = 0.0, s1 = 0.0;
+  double s[NV];
 #pragma unroll
   for (int n = 0; n < nch; n++) {
     // order: (e,e,e),(o,e,e),(e,o,e),(o,o,e),(e,e,o),(o,e,o),(e,o,o),(o,o,o)
     const int i = i2 - (n & 1), j = j2 - ((n >> 1) & 1);
     const int k = (DIM == 3) ? k2 - ((n >> 2) & 1) : 0;
-    double o1, o2;
-    op_q<DIM, MODEL>(q, qold, P, i, j, k, m, h, o1, o2);
-    const int o = qidx(P, i, j, k);
-    const double a0 = f[o] - o1, a1 = f[o + P.qvs] - o2;
-    if (n == 0) {
-      s0 = a0;
-      s1 = a1;
-    } else {
-      s0 = s0 + a0;
-      s1 = s1 + a1;
+    double o[kNvMax];
+    op_cell<DIM, MODEL>(q, qold, P, i, j, k, m, h, o);
+    const int ofs = qidx(P, i, j, k);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const double a = f[ofs + v * P.qvs] - o[v];
+      s[v] = (n == 0) ? a : s[v] + a;
     }
   }
-  r0 = ((DIM == 3) ? 0.125 : 0.25) * s0;
-  r1 = ((DIM == 3) ? 0.125 : 0.25) * s1;
-  // coarse operator on qc with ch = parent%dx(1); BG reads parent%qold.
+  // coarse operator on qc with ch = parent%dx(1); BG and the tumour model
+  // read parent%qold.
   const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
   const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
   const int oc = cidx(P, c.ic, c.jc, c.kc);
-  const double lmu = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, P.cvs);
-  const double lphi = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, 0);
   const int opar = qidx(Q, ip, jp, kp);
-  const double po = (MODEL == BSAM_MODEL_BG) ? qold[opar] : 0.0;
-  double o1, o2;
-  op_from<DIM, MODEL>(qc[oc], qc[oc + P.cvs], lphi, lmu, po, m, Q.dx, o1, o2);
-  fpar[opar] = r0 + o1;
-  fpar[opar + Q.qvs] = r1 + o2;
+  double o[kNvMax];
+  if (MODEL == BSAM_MODEL_TUMOUR) {
+    const int vs = P.cvs;
+    const double l2 = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, vs), l1 = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, 0);
+    const double l3 = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, 2 * vs);
+    const double l4 = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, 3 * vs);
+    op_tm_from<DIM>(qc[oc], qc[oc + vs], qc[oc + 2 * vs], l1, l2, l3, l4, qold[opar], m, Q.dx, o);
+  } else {
+    const double lmu = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, P.cvs);
+    const double lphi = ulap_c<DIM>(qc, P, c.ic, c.jc, c.kc, 0);
+    const double po = (MODEL == BSAM_MODEL_BG) ? qold[opar] : 0.0;
+    op_from<DIM, MODEL>(qc[oc], qc[oc + P.cvs], lphi, lmu, po, m, Q.dx, o[0], o[1]);
+  }
+  const double w = (DIM == 3) ? 0.125 : 0.25;
+#pragma unroll
+  for (int v = 0; v < NV; v++) fpar[opar + v * Q.qvs] = w * s[v] + o[v];
 }
 
 // CorrectFine, first half (afasroutines.f90:892-895, 934-938):
@@ -937,17 +1018,6 @@ __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T*
   const int row = local / n0;
   const int jc = row % n1;
   const int kc = (DIM == 3) ? row / n1 : 0;
-  // coarse values of both variables around the dual cell
-  T a[2][2][2][2];
-#pragma unroll
-  for (int v = 0; v < 2; v++)
-#pragma unroll
-    for (int d = 0; d < 2; d++)
-#pragma unroll
-      for (int b = 0; b < 2; b++)
-#pragma unroll
-        for (int e = 0; e < 2; e++)
-          a[v][d][b][e] = (DIM == 3 || d == 0) ? qc[cidx(P, ic + e, jc + b, kc + d) + v * P.cvs] : (T)0;
   constexpr int ncell = (DIM == 3) ? 8 : 4;
   auto fine = [&](int n, int& i, int& j, int& k) {
     i = 2 * ic + (n & 1);
@@ -957,39 +1027,53 @@ __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T*
   auto is_ghost = [&](int i, int j, int k) {
     return i == 0 || i == P.mx[0] + 1 || j == 0 || j == P.mx[1] + 1 || (DIM == 3 && (k == 0 || k == P.mx[2] + 1));
   };
-  // all loads of the 2^d fine cells first (ghost cells are storage too), so
-  // they are in flight together
-  T old[ncell][2];
+  // variables in pairs (CH and BG have one pair, the tumour model two)
+  for (int vp = 0; vp < nv; vp += 2) {
+    // coarse values of the pair around the dual cell
+    T a[2][2][2][2];
 #pragma unroll
-  for (int n = 0; n < ncell; n++) {
-    int i, j, k;
-    fine(n, i, j, k);
-    const int o = qidx(P, i, j, k);
-    old[n][0] = q[o];
-    old[n][1] = q[o + P.qvs];
-  }
+    for (int v = 0; v < 2; v++)
 #pragma unroll
-  for (int n = 0; n < ncell; n++) {
-    int i, j, k;
-    fine(n, i, j, k);
-    if (WRAP && is_ghost(i, j, k)) continue;
+      for (int d = 0; d < 2; d++)
 #pragma unroll
-    for (int v = 0; v < 2; v++) {
-      T pr;
-      if (DIM == 2) {
-        const T b0 = linw<T>(a[v][0][0][0], a[v][0][0][1], i), b1 = linw<T>(a[v][0][1][0], a[v][0][1][1], i);
-        pr = linw<T>(b0, b1, j) / (T)16;
-      } else {
-        T cc2[2];
+        for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int d = 0; d < 2; d++) {
-          const T b0 = linw<T>(a[v][d][0][0], a[v][d][0][1], i);
-          const T b1 = linw<T>(a[v][d][1][0], a[v][d][1][1], i);
-          cc2[d] = linw<T>(b0, b1, j);
+          for (int e = 0; e < 2; e++)
+            a[v][d][b][e] = (DIM == 3 || d == 0) ? qc[cidx(P, ic + e, jc + b, kc + d) + (vp + v) * P.cvs] : (T)0;
+    // all loads of the 2^d fine cells first (ghost cells are storage too), so
+    // they are in flight together
+    T old[ncell][2];
+#pragma unroll
+    for (int n = 0; n < ncell; n++) {
+      int i, j, k;
+      fine(n, i, j, k);
+      const int o = qidx(P, i, j, k) + vp * P.qvs;
+      old[n][0] = q[o];
+      old[n][1] = q[o + P.qvs];
+    }
+#pragma unroll
+    for (int n = 0; n < ncell; n++) {
+      int i, j, k;
+      fine(n, i, j, k);
+      if (WRAP && is_ghost(i, j, k)) continue;
+#pragma unroll
+      for (int v = 0; v < 2; v++) {
+        T pr;
+        if (DIM == 2) {
+          const T b0 = linw<T>(a[v][0][0][0], a[v][0][0][1], i), b1 = linw<T>(a[v][0][1][0], a[v][0][1][1], i);
+          pr = linw<T>(b0, b1, j) / (T)16;
+        } else {
+          T cc2[2];
+#pragma unroll
+          for (int d = 0; d < 2; d++) {
+            const T b0 = linw<T>(a[v][d][0][0], a[v][d][0][1], i);
+            const T b1 = linw<T>(a[v][d][1][0], a[v][d][1][1], i);
+            cc2[d] = linw<T>(b0, b1, j);
+          }
+          pr = linw<T>(cc2[0], cc2[1], k) / (T)64;
         }
-        pr = linw<T>(cc2[0], cc2[1], k) / (T)64;
+        q[qidx(P, i, j, k) + (vp + v) * P.qvs] = old[n][v] + pr;
       }
-      q[qidx(P, i, j, k) + v * P.qvs] = old[n][v] + pr;
     }
   }
   if (WRAP) {
@@ -1049,9 +1133,11 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* 
 // into ftmp, then f = ftmp + SourceUpdate#D (= 0 for both models)
 // (afasroutines.f90:330-381, 438-492).  mode 0: source only, 1: update only.
 template <int DIM, int MODEL>
-__global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double dt,
+__global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, ModelK m,
                                                    const double* __restrict__ qold,
                                                    double* __restrict__ ftmp, double* __restrict__ f) {
+  constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
+  const double dt = m.dt;
   int pi, local;
   if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
@@ -1065,14 +1151,20 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, double
     if (MODEL == BSAM_MODEL_CH) {
       ftmp[o] = po;
       ftmp[o + P.qvs] = -po;
+    } else if (MODEL == BSAM_MODEL_TUMOUR) {
+      // the probe PROBLEM module's Source2D/3D with model = 3
+      ftmp[o] = po - dt * m.lam * (0.5 * (po + 1.0));
+      ftmp[o + P.qvs] = -po;
+      ftmp[o + 2 * P.qvs] = 0.0;
+      ftmp[o + 3 * P.qvs] = -m.lam * (0.5 * (po + 1.0));
     } else {
       ftmp[o] = (1.0 + dt) * po;
       ftmp[o + P.qvs] = 0.0;
     }
   }
   if (mode != 0) {
-    f[o] = ftmp[o] + 0.0;
-    f[o + P.qvs] = ftmp[o + P.qvs] + 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; v++) f[o + v * P.qvs] = ftmp[o + v * P.qvs] + 0.0;
   }
 }
 
@@ -1155,16 +1247,18 @@ __global__ __launch_bounds__(kBlock) void k_interp_map(const GRule* __restrict__
   for (int v = 0; v < nv; v++) q[R.dst + v * R.pvs] = interp_value<T>(R, R.kind, v, q);
 }
 
-// Post-operations of a flattened ghost rule, two bits each from the low
+// Post-operations of a flattened ghost rule, three bits each from the low
 // end, applied in that order: 1 adds +0.0 (the qpo term of an x wrap), 2
 // takes the Dirichlet image 2 - x (-x for a correction, `homog`), 3 does
-// that on variables >= 1 only (the BioGrowth boundary condition).
+// that on variables >= 1 only (the BioGrowth boundary condition), 4 is the
+// tumour model's condition (2 - x on variable 2, -x on variable 3).
 template <typename T>
-__device__ __forceinline__ T ghost_post(T x, int post, int v, int homog) {
-  for (; post; post >>= 2) {
-    const int op = post & 3;
+__host__ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog) {
+  for (; post; post >>= 3) {
+    const int op = post & 7;
     if (op == 1) x = x + (T)0;
-    else if (op == 2 || (op == 3 && v >= 1)) x = homog ? -x : (T)2 - x;
+    else if (op == 2 || (op == 3 && v >= 1) || (op == 4 && v == 2)) x = homog ? -x : (T)2 - x;
+    else if (op == 4 && v == 3) x = -x;
   }
   return x;
 }
@@ -1175,7 +1269,7 @@ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog) {
 // are (dst, src, dst variable stride, src variable stride); interpolation
 // rules carry their post-operations in bits 8-15 of `kind`.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ cr, const uint8_t* __restrict__ cpost,
+__global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ cr, const uint16_t* __restrict__ cpost,
                                                       int nc, const GRule* __restrict__ ir, int ni, int nv,
                                                       int homog, T* __restrict__ q) {
   const int g = xcd_block() * kBlock + threadIdx.x;
@@ -1216,6 +1310,7 @@ __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt
       case kCopyPlus0: y = x + (T)0; break;
       case kDirichlet2: y = dir; break;
       case kBgMixed: y = (v == 0) ? x : dir; break;
+      case kTumourBC: y = (v == 2) ? dir : ((v == 3) ? -x : x); break;
       default: y = x;
     }
     q[qidx(D, B.dlo[0] + a, B.dlo[1] + b, B.dlo[2] + z) + v * D.qvs] = y;
@@ -1738,7 +1833,7 @@ struct Stage {
 // interpolation rules.
 struct DevMap {
   int4* cr = nullptr;
-  uint8_t* cpost = nullptr;
+  uint16_t* cpost = nullptr;
   GRule* ir = nullptr;
   int nc = 0, ni = 0;
   int nc1 = 0, ni1 = 0;  // first launch: rules reading ghost values from before the call
@@ -1933,11 +2028,12 @@ extern "C" int32_t bsam_create(const bsam_params* params, bsam_ctx** out) {
   *out = nullptr;
   const bsam_params& p = *params;
   if (p.ndims != 2 && p.ndims != 3) return set_err(BSAM_EINVAL, "ndims must be 2 or 3");
-  if (p.nrvars != 2) return set_err(BSAM_EINVAL, "nrvars must be 2 (CH and BG are 2-field models)");
+  if (p.model != BSAM_MODEL_CH && p.model != BSAM_MODEL_BG && p.model != BSAM_MODEL_TUMOUR)
+    return set_err(BSAM_EINVAL, "unknown model %d", p.model);
+  if (p.nrvars != (p.model == BSAM_MODEL_TUMOUR ? 4 : 2))
+    return set_err(BSAM_EINVAL, "nrvars must be 2 for CH and BG, 4 for the tumour model");
   if (p.mbc != 1) return set_err(BSAM_EINVAL, "only mbc=1 is supported");
   if (p.maux != 0) return set_err(BSAM_EINVAL, "maux must be 0");
-  if (p.model != BSAM_MODEL_CH && p.model != BSAM_MODEL_BG)
-    return set_err(BSAM_EINVAL, "unknown model %d", p.model);
   if (p.model == BSAM_MODEL_BG && p.ndims != 2)
     return set_err(BSAM_EINVAL, "the BioGrowth hooks exist in 2D only (their 3D forms are stubs)");
   if (p.errortype != 1 && p.errortype != 2)
@@ -2102,7 +2198,7 @@ void periodic_offsets(const bsam_ctx* c, int level, std::vector<std::array<int, 
 // the reference's order) and, where it is valid, the flattened maps.
 struct HostMap {
   std::vector<int4> copies;    // dst, src, dst variable stride, src variable stride
-  std::vector<uint8_t> cpost;  // post-operation chain per copy (empty: none anywhere)
+  std::vector<uint16_t> cpost;  // post-operation chain per copy (empty: none anywhere)
   std::vector<GRule> interps;  // kind | post << 8
   int nc1 = 0, ni1 = 0;        // leading rules that read ghost values from before the call
 };
@@ -2352,6 +2448,7 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       BoxDesc b{};
       b.dp = b.sp = pid;
       if (c->p.model == BSAM_MODEL_CH) b.op = (nd == 3) ? kDirichlet2 : kCopy;
+      else if (c->p.model == BSAM_MODEL_TUMOUR) b.op = kTumourBC;
       else b.op = kBgMixed;
       for (int n = 0; n < 3; n++) {
         b.dlo[n] = b.slo[n] = 0;
@@ -2475,7 +2572,8 @@ static bool flatten_setghost(const bsam_ctx* c, const Level& L, const GhostPlan&
     for (const BoxDesc& B : st) {
       const DPatch& D = c->hp[B.dp];
       const DPatch& S = c->hp[B.sp];
-      const int code = (B.op == kCopy) ? 0 : (B.op == kCopyPlus0 ? 1 : (B.op == kDirichlet2 ? 2 : 3));
+      const int code = (B.op == kCopy) ? 0
+                       : (B.op == kCopyPlus0 ? 1 : (B.op == kDirichlet2 ? 2 : (B.op == kBgMixed ? 3 : 4)));
       for (int z = 0; z < B.ext[2]; z++)
         for (int y = 0; y < B.ext[1]; y++)
           for (int x = 0; x < B.ext[0]; x++) {
@@ -2508,11 +2606,11 @@ static bool flatten_setghost(const bsam_ctx* c, const Level& L, const GhostPlan&
             e.dvs = D.qvs;
             e.dpid = B.dp;
             if (code) {
-              if (e.npost >= 4) {
-                why = "post-operation chain longer than four";
+              if (e.npost >= 5) {
+                why = "post-operation chain longer than five";
                 return false;
               }
-              e.post |= code << (2 * e.npost);
+              e.post |= code << (3 * e.npost);
               e.npost++;
             }
             E.set(LC.at(c, B.dp, d), d, e);
@@ -2600,7 +2698,7 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
         const int d = M.dst[n];
         if (e.kind == 0) {
           H.copies.push_back(make_int4(d, e.src, e.dvs, e.svs));
-          H.cpost.push_back((uint8_t)e.post);
+          H.cpost.push_back((uint16_t)e.post);
           anypost = anypost || e.post;
           if (phase) H.nc1++;
         } else {
@@ -2715,8 +2813,8 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
       HIPCHK(hipMemcpy(D.cr, H.copies.data(), H.copies.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     if (!H.cpost.empty()) {
-      HIPCHK(hipMalloc(&D.cpost, H.cpost.size()));
-      HIPCHK(hipMemcpy(D.cpost, H.cpost.data(), H.cpost.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMalloc(&D.cpost, H.cpost.size() * sizeof(uint16_t)));
+      HIPCHK(hipMemcpy(D.cpost, H.cpost.data(), H.cpost.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     }
     if (D.ni) {
       HIPCHK(hipMalloc(&D.ir, H.interps.size() * sizeof(GRule)));
@@ -2906,8 +3004,8 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
   if (maxblocks > c->partial_cap) {
     hipFree(c->d_partial);
     hipFree(c->d_partial2);
-    HIPCHK(hipMalloc(&c->d_partial, 2 * (long)maxblocks * sizeof(double)));
-    HIPCHK(hipMalloc(&c->d_partial2, 2 * (long)(maxblocks / 256 + 2) * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_partial, kNvMax * (long)maxblocks * sizeof(double)));
+    HIPCHK(hipMalloc(&c->d_partial2, kNvMax * (long)(maxblocks / 256 + 2) * sizeof(double)));
     c->partial_cap = maxblocks;
   }
   c->have_hier = true;
@@ -2937,14 +3035,7 @@ double host_interp(const GRule& R, int kind, int v, const std::vector<double>& q
     coarse = hC[0] * q[R.c[0] + pv] + hC[1] * q[R.c[1] + pv] + hC[2] * q[R.c[2] + pv] + hC[3] * q[R.c[3] + pv];
   return hA[0] * q[R.s1 + fv] + hA[1] * q[R.s2 + fv] + hA[2] * coarse;
 }
-double host_post(double x, int post, int v, int homog) {
-  for (; post; post >>= 2) {
-    const int op = post & 3;
-    if (op == 1) x = x + 0.0;
-    else if (op == 2 || (op == 3 && v >= 1)) x = homog ? -x : 2.0 - x;
-  }
-  return x;
-}
+double host_post(double x, int post, int v, int homog) { return ghost_post<double>(x, post, v, homog); }
 
 // The staged order: interpolation rules of the ipass's segments, then the
 // box stages one box after another (the reference's sequential order).
@@ -2971,6 +3062,7 @@ void host_staged(const bsam_ctx* c, const Level& L, const GhostPlan& G, int ipas
                 case kCopyPlus0: y0 = a + 0.0; break;
                 case kDirichlet2: y0 = dir; break;
                 case kBgMixed: y0 = (v == 0) ? a : dir; break;
+                case kTumourBC: y0 = (v == 2) ? dir : ((v == 3) ? -a : a); break;
                 default: y0 = a;
               }
               q[qidx(D, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z) + v * D.qvs] = y0;
@@ -3205,6 +3297,9 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
         k_relax_ch<2, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
       else
         k_relax_ch<3, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
+    } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+      DISPATCH_DIM(c->p.ndims, (k_relax_tm<2><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)),
+                   (k_relax_tm<3><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)));
     } else {
       k_relax_bg2d<<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), L.h, c->q, c->qold, c->f);
     }
@@ -3489,6 +3584,10 @@ extern "C" int32_t bsam_coarse_loading(bsam_ctx* c, int32_t level) {
         DISPATCH_DIM(c->p.ndims,
                      (k_fas_rhs<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)),
                      (k_fas_rhs<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)));
+      } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+        DISPATCH_DIM(c->p.ndims,
+                     (k_fas_rhs<2, BSAM_MODEL_TUMOUR><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)),
+                     (k_fas_rhs<3, BSAM_MODEL_TUMOUR><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f)));
       } else {
         k_fas_rhs<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, c->qc, c->f);
       }
@@ -3543,12 +3642,17 @@ static int32_t source_launch(bsam_ctx* c, int level, int mode) {
   RETCHK(check_level(c, level));
   Level& L = c->L(level);
   FOR_LAUNCHES(c, L, kCell, a, nb) {
+    const ModelK m = c->mk();
     if (c->p.model == BSAM_MODEL_CH) {
       DISPATCH_DIM(c->p.ndims,
-                   (k_source<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)),
-                   (k_source<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f)));
+                   (k_source<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, m, c->qold, c->ftmp, c->f)),
+                   (k_source<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, mode, m, c->qold, c->ftmp, c->f)));
+    } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+      DISPATCH_DIM(c->p.ndims,
+                   (k_source<2, BSAM_MODEL_TUMOUR><<<nb, kBlock, 0, c->s>>>(a, mode, m, c->qold, c->ftmp, c->f)),
+                   (k_source<3, BSAM_MODEL_TUMOUR><<<nb, kBlock, 0, c->s>>>(a, mode, m, c->qold, c->ftmp, c->f)));
     } else {
-      k_source<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, mode, c->p.dt, c->qold, c->ftmp, c->f);
+      k_source<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, mode, m, c->qold, c->ftmp, c->f);
     }
     LAUNCH_CHECK();
   }
@@ -3588,11 +3692,15 @@ static int32_t residual_launch(bsam_ctx* c, int level, double* rf, double* parti
   LevelArgs a;
   int nb = 0;
   for (LaunchIter it(c, L, kCell); it.next(a, nb);) {
-    double* part = partial ? partial + 2L * it.boff : nullptr;
+    double* part = partial ? partial + (long)kNvMax * it.boff : nullptr;
     if (c->p.model == BSAM_MODEL_CH) {
       DISPATCH_DIM(c->p.ndims,
                    (k_residual<2, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part)),
                    (k_residual<3, BSAM_MODEL_CH, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part)));
+    } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+      DISPATCH_DIM(c->p.ndims,
+                   (k_residual<2, BSAM_MODEL_TUMOUR, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part)),
+                   (k_residual<3, BSAM_MODEL_TUMOUR, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part)));
     } else {
       k_residual<2, BSAM_MODEL_BG, double><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->f, rf, part);
     }
@@ -3652,15 +3760,22 @@ static double norm_from(bsam_ctx* c, const double* s, int level) {
     const DPatch& P = c->hp[pid];
     cells += (double)P.mx[0] * P.mx[1] * P.mx[2];
   }
-  if (c->p.errortype == 1) return std::sqrt((s[0] + s[1]) / (c->p.nrvars * cells));
-  return std::max(std::sqrt(s[0] / cells), std::sqrt(s[1] / cells));
+  const int nv = c->p.nrvars;
+  if (c->p.errortype == 1) {
+    double t = 0.0;
+    for (int v = 0; v < nv; v++) t += s[v];
+    return std::sqrt(t / (nv * cells));
+  }
+  double e = 0.0;
+  for (int v = 0; v < nv; v++) e = std::max(e, std::sqrt(s[v] / cells));
+  return e;
 }
 
 extern "C" int32_t bsam_error_afas(bsam_ctx* c, int32_t level, double* norm) {
   RETCHK(check_level(c, level));
   if (!norm) return set_err(BSAM_EINVAL, "null norm");
   RETCHK(enqueue_norm(c, level, true));
-  HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, kNvMax * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   *norm = norm_from(c, c->h_norm, level);
   return BSAM_OK;
@@ -3791,7 +3906,7 @@ static int32_t run_iteration(bsam_ctx* c, int32_t (*body)(bsam_ctx*)) {
   } else {
     RETCHK(body(c));
   }
-  HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(c->h_norm, c->d_norm, kNvMax * sizeof(double), hipMemcpyDeviceToHost, c->s));
   return BSAM_OK;
 }
 
@@ -4033,7 +4148,7 @@ int32_t mixed_residuals(bsam_ctx* c) {
       LevelArgs a;
       int nbl = 0;
       for (LaunchIter it(c, L, kCell); it.next(a, nbl);) {
-        double* pp = part ? part + 2L * it.boff : nullptr;
+        double* pp = part ? part + (long)kNvMax * it.boff : nullptr;
         DISPATCH_DIM(c->p.ndims,
                      (k_residual<2, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)),
                      (k_residual<3, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)));

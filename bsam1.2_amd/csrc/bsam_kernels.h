@@ -47,7 +47,8 @@ __host__ __device__ __forceinline__ int cidx(const DPatch& P, int i, int j, int 
 
 // Ghost-fill box operation: dst(patch dp, box dlo+ext) <- op(src(patch sp,
 // box slo+ext)), all variables.  Indices are storage indices (k = 0 in 2D).
-enum BoxOp : int { kCopy = 0, kCopyPlus0 = 1, kDirichlet2 = 2, kBgMixed = 3 };
+// kTumourBC: variables 0, 1 copied, 2 <- 2 - x, 3 <- -x.
+enum BoxOp : int { kCopy = 0, kCopyPlus0 = 1, kDirichlet2 = 2, kBgMixed = 3, kTumourBC = 4 };
 struct BoxDesc {
   int dp, sp, op, pad;
   int dlo[3], slo[3], ext[3];

@@ -36,10 +36,13 @@ def _selftest(params, hier, seed=7):
     return n.value, nl.value, hi
 
 
-@pytest.mark.parametrize("name", ["ch2d_amr", "ch2d_amr_phys", "ch3d_amr", "ch3d_amr_phys"])
+@pytest.mark.parametrize("name", ["ch2d_amr", "ch2d_amr_phys", "ch3d_amr", "ch3d_amr_phys", "tm2d_amr_phys",
+                                  "tm3d_amr_phys"])
 def test_golden_hierarchies(name):
     """Every refined level is flattened except where a 3D physical corner
-    reads an edge ghost from before the call (the staged fill stays)."""
+    reads an edge ghost from before the call (the staged fill stays).  The
+    tumour fixtures exercise four fields and their mixed boundary
+    condition."""
     meta, _ = fu.load(name)
     for st in meta["steps"]:
         n, nl, hi = _selftest(fu.step_params(meta), st["hierarchy"])

@@ -25,7 +25,7 @@ def _hist_close(a, b):
         assert abs(x - y) <= RTOL_NORM * abs(y), (x, y)
 
 
-@pytest.mark.parametrize("name", fu.NAMES)
+@pytest.mark.parametrize("name", fu.NAMES + fu.TUMOUR_NAMES)
 @pytest.mark.parametrize("batching", [1, 0])
 def test_golden_multigrid_iterations(name, batching, gpu_solver_cls):
     """MultigridIterations on the reference's states: batched per-level
@@ -43,7 +43,7 @@ def test_golden_multigrid_iterations(name, batching, gpu_solver_cls):
         S.close()
 
 
-@pytest.mark.parametrize("name", fu.NAMES)
+@pytest.mark.parametrize("name", fu.NAMES + fu.TUMOUR_NAMES)
 def test_golden_sweeps(name, gpu_solver_cls):
     meta, z = fu.load(name)
     print(f"[gpu] sweeps {name}", flush=True)
