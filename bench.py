@@ -12,6 +12,12 @@ Workloads (--workload):
   ch2d_amr_128_l2: 2D, 128^2 root with two refined levels, eps = 0.01,
       dt = 1e-5, a circle of radius 0.25 plus 0.01 U(-1,1) (seed 1002),
       residual below 1e-9.
+  tm3d_amr_64_l3: the 3D tumour-growth system (phi, mu, nutrient n,
+      pressure p; proliferation 1.0, apoptosis 0.1, consumption 10), 64^3
+      root with three refined levels (435 patches, 4.4 M cells), eps = 0.01,
+      dt = 1e-3, a tumour of radius 0.15 perturbed by real spherical
+      harmonics l <= 6 (seed 1005), n = 1 and p = 0 on every face, residual
+      below 1e-8 (FP64 path).
 The AMR meshes are the reference's own regrid result for these initial
 data (bsam_amd/workloads/*.json), frozen: regrid is not part of the path.
 Every patch is seeded with the same formula at its own level.
@@ -58,7 +64,8 @@ def _cpu_model():
 DOMINANT = {("uniform", "fp64"): "k_relax3d_march<double, false, true, 1>",
             ("uniform", "mixed"): "k_relax3d_march<float, true, true, 1>",
             ("amr", "fp64"): "k_relax_ch<3, false>",
-            ("amr", "mixed"): "k_relax_lin<3, false>"}
+            ("amr", "mixed"): "k_relax_lin<3, false>",
+            ("tumour", "fp64"): "k_relax_tm<3>"}
 
 
 def measured_traffic(workload, precision, kernel):
@@ -83,7 +90,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ch3d_amr_64_l3",
-                    choices=["ch3d_amr_64_l3", "ch3d_uniform_256", "ch2d_amr_128_l2"])
+                    choices=["ch3d_amr_64_l3", "ch3d_uniform_256", "ch2d_amr_128_l2", "tm3d_amr_64_l3"])
     ap.add_argument("--n", type=int, default=256, help="root cells per side of the uniform workload")
     ap.add_argument("--batching", type=int, default=1,
                     help="1: one launch per level (default); 0: one launch per patch")
@@ -104,6 +111,7 @@ class Workload:
         import numpy as np
         from bsam_amd import configs, hierarchy
         self.name = name
+        self.tumour = None
         if name == "ch3d_uniform_256":
             w = configs.WORKLOADS["ch3d_uniform_256"]
             self.nd, self.n = 3, n
@@ -119,26 +127,34 @@ class Workload:
             base = hierarchy.uniform(self.nd, (self.n,) * self.nd, self.minlevel, w["mthbc"])
             self.hier = hierarchy.from_patch_boxes(base, w["boxes"], self.nd)
             self.eps, self.dt, self.tol, self.steps = w["eps"], w["dt"], w["tol"], w["steps"]
+            self.tumour = w.get("tumour")
             i = w["init"]
             self.init = dict(mode=i["initmode"], seed=i["seed"], amp=i["amp"], radius=i["radius"],
                              pamp=i.get("pamp", 0.02), lmax=i.get("lmax", 6))
             self.label = (f"{self.n}^{self.nd} periodic root, levels {self.minlevel}..{w['maxlevel']}, "
                           f"patches per level {w['patches_per_level']}, cells per level {w['cells_per_level']}")
+        self.nv = 4 if self.tumour else 2
         self.root = [k for k, d in enumerate(self.hier) if d["level"] == 0][0]
         self.q0 = {}
         for k, d in enumerate(self.hier):
             if d["level"] < 0:
                 continue
-            q = np.zeros(hierarchy.field_shape(d, self.nd, 2, "q"))
+            q = np.zeros(hierarchy.field_shape(d, self.nd, self.nv, "q"))
             q[(0,) + (slice(1, -1),) * self.nd] = configs.init_phi(d, self.nd, eps=self.eps, **self.init)
+            if self.tumour:
+                q[(2,) + (slice(1, -1),) * self.nd] = 1.0  # nutrient at its far-field value, pressure 0
             self.q0[k] = q
         self.finest = max(d["level"] for d in self.hier)
 
     def params(self, precision, use_graph, batching):
         from bsam_amd import abi, configs
         prec = abi.PREC_MIXED if precision == "mixed" else abi.PREC_FP64
+        kw = {}
+        if self.tumour:
+            t = self.tumour
+            kw = dict(model=abi.MODEL_TUMOUR, extra=(t["lamp"], t["lama"], t["kappa"]))
         return configs.params(self.nd, self.dt, self.eps, self.tol, precision=prec, use_graph=use_graph,
-                              batching=batching)
+                              batching=batching, **kw)
 
     def load(self, S):
         """Initial fields into solver S (GPU or oracle), ghosts set level by
@@ -195,7 +211,7 @@ def timed_path(wl, a, precision):
         S.kstat_enable(False)
         avg_ms = ms / max(nl, 1)
         achieved = b / (avg_ms * 1e-3) / 1e9 if nl else None
-        kind = "uniform" if wl.name.startswith("ch3d_uniform") else "amr"
+        kind = "uniform" if wl.name.startswith("ch3d_uniform") else ("tumour" if wl.tumour else "amr")
         kname = DOMINANT.get((kind, precision), "k_relax_ch / k_relax_lin")
         roof = dict(bound="hbm", kernel=f"{kname} (finest-level colour sweep, {precision})", achieved=achieved,
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=(achieved / HBM_PEAK_GBS) if achieved else None,
@@ -211,11 +227,11 @@ def fields(wl, S):
 def max_err(wl, qa, qb):
     import numpy as np
     inner = (slice(1, -1),) * wl.nd
-    e = [0.0, 0.0]
+    e = [0.0] * wl.nv
     for k in qa:
-        for v in range(2):
+        for v in range(wl.nv):
             e[v] = max(e[v], float(np.max(np.abs(qa[k][(v,) + inner] - qb[k][(v,) + inner]))))
-    return {"phi": e[0], "mu": e[1]}
+    return dict(zip(["phi", "mu", "n", "p"], e))
 
 
 def cpu_baseline(wl, vcycles_per_step, ncyc=3):
@@ -248,6 +264,9 @@ def main():
         sys.exit(2)
     wl = Workload(a.workload, a.n)
     paths = ["mixed", "fp64"] if a.precision == "both" else [a.precision]
+    if wl.tumour and "mixed" in paths:
+        print("[bench] the tumour model runs the FP64 path only; mixed is skipped", file=sys.stderr)
+        paths = ["fp64"]
     res = {}
     for pr in paths:
         S, elapsed, vc, roof = timed_path(wl, a, pr)
@@ -276,8 +295,9 @@ def main():
         cpu = cpu_baseline(wl, res["fp64" if "fp64" in res else head]["vc"])
     el = res[head]["elapsed"]
     out = {
-        "metric": "AMR CH time-steps/s at ||r||<=1e-8: mixed+batched vs plain FP64; HBM GB/s"
-                  if wl.tol == 1e-8 else f"AMR CH time-steps/s at ||r||<={wl.tol:g}: mixed+batched vs plain FP64",
+        "metric": ("AMR tumour-growth time-steps/s at ||r||<=1e-8 (plain FP64, batched)" if wl.tumour else
+                   "AMR CH time-steps/s at ||r||<=1e-8: mixed+batched vs plain FP64; HBM GB/s"
+                   if wl.tol == 1e-8 else f"AMR CH time-steps/s at ||r||<={wl.tol:g}: mixed+batched vs plain FP64"),
         "value": a.steps / el,
         "unit": "time-steps/s",
         "n_gpus": 1,

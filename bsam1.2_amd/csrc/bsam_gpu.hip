@@ -67,6 +67,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kNvMax = 4;  // fields of the largest model (the tumour model)
+constexpr int kSeqBoxElems = 8192;  // box values up to which a level's stages run in one workgroup
 
 // Interpolation constants, boundary.f90:54-63.
 __constant__ double cA[3] = {-1.0 / 5.0, 2.0 / 3.0, 8.0 / 15.0};
@@ -1288,9 +1289,30 @@ __global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ c
 // Box copies of one ghost-fill stage: sibling transfers (GhostOverlap,
 // boundary.f90:359-431), self-periodic wraps (SelfPeriodicBC :480-577),
 // periodic transfers (TransferPeriodicBC :580-611) and physical boundary
-// conditions (SetPhysicalBC + UserBC).  One block per box.
+// conditions (SetPhysicalBC + UserBC).  Element e of a box runs variable
+// by variable over the box in x-fastest order.
 // `homog` selects the boundary condition of a correction (mixed precision):
 // q_ghost = 2 - q_in becomes e_ghost = -e_in.
+template <typename T>
+__device__ __forceinline__ void box_elem(const DPatch& D, const DPatch& S, const BoxDesc& B, int n, int e,
+                                         int homog, T* __restrict__ q) {
+  const int v = e / n, c = e % n;
+  const int a = c % B.ext[0], r = c / B.ext[0];
+  const int b = r % B.ext[1], z = r / B.ext[1];
+  const T x = q[qidx(S, B.slo[0] + a, B.slo[1] + b, B.slo[2] + z) + v * S.qvs];
+  const T dir = homog ? -x : (T)2 - x;
+  T y;
+  switch (B.op) {
+    case kCopyPlus0: y = x + (T)0; break;
+    case kDirichlet2: y = dir; break;
+    case kBgMixed: y = (v == 0) ? x : dir; break;
+    case kTumourBC: y = (v == 2) ? dir : ((v == 3) ? -x : x); break;
+    default: y = x;
+  }
+  q[qidx(D, B.dlo[0] + a, B.dlo[1] + b, B.dlo[2] + z) + v * D.qvs] = y;
+}
+
+// One stage: box blockIdx.x, split over gridDim.y workgroups.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt,
                                                    const BoxDesc* __restrict__ boxes, int nv, int homog,
@@ -1299,22 +1321,31 @@ __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt
   const DPatch D = pt[B.dp];
   const DPatch S = pt[B.sp];
   const int n = B.ext[0] * B.ext[1] * B.ext[2];
-  for (int e = threadIdx.x; e < n * nv; e += blockDim.x) {
-    const int v = e / n, c = e % n;
-    const int a = c % B.ext[0], r = c / B.ext[0];
-    const int b = r % B.ext[1], z = r / B.ext[1];
-    const T x = q[qidx(S, B.slo[0] + a, B.slo[1] + b, B.slo[2] + z) + v * S.qvs];
-    const T dir = homog ? -x : (T)2 - x;
-    T y;
-    switch (B.op) {
-      case kCopyPlus0: y = x + (T)0; break;
-      case kDirichlet2: y = dir; break;
-      case kBgMixed: y = (v == 0) ? x : dir; break;
-      case kTumourBC: y = (v == 2) ? dir : ((v == 3) ? -x : x); break;
-      default: y = x;
+  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < n * nv; e += blockDim.x * gridDim.y)
+    box_elem<T>(D, S, B, n, e, homog, q);
+}
+
+// Every stage of a small level's fill in one workgroup, box after box with
+// a barrier after each: the reference's sequential order in one launch
+// (the physical conditions of the small levels at and below the root).
+constexpr int kMaxStages = 8;
+struct BoxSeq {
+  const BoxDesc* d[kMaxStages];
+  int n[kMaxStages];
+  int ns;
+};
+template <typename T>
+__global__ __launch_bounds__(1024) void k_boxops_seq(const DPatch* __restrict__ pt, BoxSeq S, int nv, int homog,
+                                                     T* __restrict__ q) {
+  for (int st = 0; st < S.ns; st++)
+    for (int bx = 0; bx < S.n[st]; bx++) {
+      const BoxDesc B = S.d[st][bx];
+      const DPatch D = pt[B.dp];
+      const DPatch Sp = pt[B.sp];
+      const int n = B.ext[0] * B.ext[1] * B.ext[2];
+      for (int e = threadIdx.x; e < n * nv; e += blockDim.x) box_elem<T>(D, Sp, B, n, e, homog, q);
+      __syncthreads();
     }
-    q[qidx(D, B.dlo[0] + a, B.dlo[1] + b, B.dlo[2] + z) + v * D.qvs] = y;
-  }
 }
 
 // Self-periodic wrap of a patch periodic in every dimension (SelfPeriodicBC
@@ -1826,6 +1857,7 @@ enum Dom { kHalf = 0, kCell, kCoarse, kFineG, kQcBox, kDual, kNumDom };
 struct Stage {
   BoxDesc* d = nullptr;
   int n = 0;
+  int maxel = 0;  // cells of the largest box
 };
 
 // A flattened ghost fill on the device: copy rules (dst, src, dst variable
@@ -2792,6 +2824,7 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
     Stage s;
     s.d = d;
     s.n = (int)boxes.size();
+    for (const BoxDesc& b : boxes) s.maxel = std::max(s.maxel, b.ext[0] * b.ext[1] * b.ext[2]);
     L.stages.push_back(s);
   }
   for (int sg = 0; sg < 3; sg++) L.roff[sg] = G.roff[sg];
@@ -3317,9 +3350,13 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
       G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
                              : 2.0 * ((double)P.mx[0] + P.mx[1]);
     }
+    // tumour: the other colour's four fields over (N+G)/2 each, the cell's
+    // own phi and qold N/2 each, f (four fields) N/2 each, four fields
+    // written N/2 each
     const double bytes = (c->p.model == BSAM_MODEL_CH)
                              ? 8.0 * ((N + G) + 0.5 * (N + G) + N + N)
-                             : 8.0 * (2.0 * (N + G) + 0.5 * N + N + N);
+                             : (c->p.model == BSAM_MODEL_TUMOUR) ? 8.0 * (2.0 * (N + G) + 5.0 * N)
+                                                                 : 8.0 * (2.0 * (N + G) + 0.5 * N + N + N);
     kstat_end(c, BSAM_KSTAT_RELAX_FINEST, bytes);
   }
   return BSAM_OK;
@@ -3375,12 +3412,30 @@ static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, b
     else k_wrap_full<3, T><<<nblocks(shell), kBlock, 0, c->s>>>(c->d_pt, L.wrap_pid, nv, q);
     LAUNCH_CHECK();
   }
+  // a small level whose fill is box stages only: one workgroup, in order
+  if (c->p.batching && !L.d_rules && !L.stages.empty() && (int)L.stages.size() <= kMaxStages) {
+    int maxel = 0;
+    for (const Stage& s : L.stages) maxel = std::max(maxel, s.maxel);
+    if (maxel * nv <= kSeqBoxElems) {
+      BoxSeq S{};
+      S.ns = (int)L.stages.size();
+      for (int st = 0; st < S.ns; st++) {
+        S.d[st] = L.stages[st].d;
+        S.n[st] = L.stages[st].n;
+      }
+      k_boxops_seq<T><<<1, 1024, 0, c->s>>>(c->d_pt, S, nv, homog, q);
+      LAUNCH_CHECK();
+      return BSAM_OK;
+    }
+  }
   for (const Stage& s : L.stages) {
-    // one workgroup per box; per-patch mode launches one box (one patch
-    // pair or face) at a time, as the reference's pairwise callbacks go
+    // each box over as many workgroups as its size needs; per-patch mode
+    // launches one box (one patch pair or face) at a time, as the
+    // reference's pairwise callbacks go
     const int per = c->p.batching ? s.n : 1;
+    const int chunks = std::max(1, std::min(64, (s.maxel * nv + kBlock - 1) / kBlock));
     for (int b0 = 0; b0 < s.n; b0 += per) {
-      k_boxops<T><<<per, kBlock, 0, c->s>>>(c->d_pt, s.d + b0, nv, homog, q);
+      k_boxops<T><<<dim3(per, chunks), kBlock, 0, c->s>>>(c->d_pt, s.d + b0, nv, homog, q);
       LAUNCH_CHECK();
     }
   }
