@@ -1457,11 +1457,19 @@ __device__ __forceinline__ float ulap_cf(const float* __restrict__ a, const DPat
   return s - 4.0f * a[cidx(P, i, j, k) + off];
 }
 
-template <int DIM, bool WRAP>
+// EMODE 1: the correction starts from zero, so the other colour's values
+// are taken as zero and not read, and the other cell of the thread's pair
+// (2t+1, 2t+2) is set to zero, since the coarse-fine interpolation after
+// this sweep reads it; 2 (the finest level's last sweep): the
+// new correction and that of the other cell of the thread's pair (2t+1,
+// 2t+2), final since the previous sweep, are added into the FP64 solution
+// q64 (interior cells; even mx).
+template <int DIM, bool WRAP, int EMODE = 0>
 __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float tmp1, float tmp2, float tmp3,
                                                       float a12, float* __restrict__ e,
                                                       const float* __restrict__ rhs,
-                                                      const float* __restrict__ coef) {
+                                                      const float* __restrict__ coef,
+                                                      double* __restrict__ q64 = nullptr) {
   int pi, local;
   if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
@@ -1475,14 +1483,17 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
   if (i > P.mx[0]) return;
   const int vs = P.qvs;
   const int o = qidx(P, i, j, k);
-  const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
-  const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
-  float smu = e[xm + vs] + e[xp + vs] + e[ym + vs] + e[yp + vs];
-  float sphi = e[xp] + e[xm] + e[yp] + e[ym];
-  if (DIM == 3) {
-    const int zm = qidx(P, i, j, k - 1), zp = qidx(P, i, j, k + 1);
-    smu = smu + e[zm + vs] + e[zp + vs];
-    sphi = sphi + e[zp] + e[zm];
+  float smu = 0.0f, sphi = 0.0f;
+  if (EMODE != 1) {
+    const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
+    const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
+    smu = e[xm + vs] + e[xp + vs] + e[ym + vs] + e[yp + vs];
+    sphi = e[xp] + e[xm] + e[yp] + e[ym];
+    if (DIM == 3) {
+      const int zm = qidx(P, i, j, k - 1), zp = qidx(P, i, j, k + 1);
+      smu = smu + e[zm + vs] + e[zp + vs];
+      sphi = sphi + e[zp] + e[zm];
+    }
   }
   const float a21 = -coef[o] - tmp3;
   const float b1 = rhs[o] + tmp1 * smu;
@@ -1492,6 +1503,18 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
   e[o] = n0;
   e[o + vs] = n1;
   if (WRAP) wrap_images<DIM, float>(e, P, i, j, k, n0, n1);
+  if (EMODE == 1) {
+    const int po = qidx(P, 2 * t + 1 + ib, j, k);
+    e[po] = 0.0f;
+    e[po + vs] = 0.0f;
+  }
+  if (EMODE == 2) {
+    q64[o] = q64[o] + (double)n0;
+    q64[o + vs] = q64[o + vs] + (double)n1;
+    const int po = qidx(P, 2 * t + 1 + ib, j, k);
+    q64[po] = q64[po] + (double)e[po];
+    q64[po + vs] = q64[po + vs] + (double)e[po + vs];
+  }
 }
 
 // Coarse right-hand side of the correction V-cycle (the FAS form of
@@ -4085,10 +4108,25 @@ static bool mixed_fused(bsam_ctx* c) {
   return L.wrap_pid >= 0 && sweep_kc(c, L, 4, &nblk) > 0;
 }
 
+// A refined mesh whose finest level runs the flat correction sweeps: its
+// first sweep starts from a zero correction and its last adds the
+// correction into q (k_relax_lin EMODE 1, 2), so the finest level's q += e
+// and the clearing of e need no pass of their own.
+static bool mixed_fused_amr(bsam_ctx* c) {
+  if (c->finest == 0 || c->p.nsmoothingpasses < 1) return false;
+  Level& L = c->L(c->finest);
+  int nblk = 0;
+  if (L.wrap_pid >= 0 || sweep_kc(c, L, 4, &nblk) > 0) return false;
+  for (int pid : L.patches)
+    if (c->hp[pid].mx[0] % 2) return false;
+  return true;
+}
+
 // One colour sweep of the correction smoother; returns 1 when the sweep
 // also wrote the periodic ghost images (the following SetGhost is then
 // redundant), 0 otherwise.  emode 1: the first sweep from e = 0; emode 2:
-// the last sweep, adding the correction into q (mixed_fused levels only).
+// the last sweep, adding the correction into q (the finest level of
+// mixed_fused or mixed_fused_amr meshes).
 int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
   bsam::MixedState* m = c->mixed;
   Level& L = c->L(level);
@@ -4120,6 +4158,14 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
         DISPATCH_DIM(c->p.ndims,
                      (k_relax_lin<2, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
                      (k_relax_lin<3, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+      else if (emode == 1)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin<2, false, 1><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
+                     (k_relax_lin<3, false, 1><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)));
+      else if (emode == 2)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin<2, false, 2><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, c->q)),
+                     (k_relax_lin<3, false, 2><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, c->q)));
       else
         DISPATCH_DIM(c->p.ndims,
                      (k_relax_lin<2, false><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef)),
@@ -4146,7 +4192,7 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
 
 // nsmoothingpasses red-black passes; phase 0 = pre-smoothing, 1 = post.
 int32_t mixed_level_relax(bsam_ctx* c, int level, int phase) {
-  const bool fused = level == c->finest && mixed_fused(c);
+  const bool fused = level == c->finest && (mixed_fused(c) || mixed_fused_amr(c));
   const int ns = c->p.nsmoothingpasses;
   for (int s = 0; s < ns; s++)
     for (int rb = 1; rb <= 2; rb++) {
@@ -4155,6 +4201,9 @@ int32_t mixed_level_relax(bsam_ctx* c, int level, int phase) {
       if (fused && phase == 1 && s == ns - 1 && rb == 2) emode = 2;
       const int32_t done = mixed_relax(c, level, rb, emode);
       RETCHK(done);
+      // after the finest level's last sweep nothing reads its correction's
+      // ghosts: q has been updated and the next cycle starts from zero
+      if (emode == 2) continue;
       if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, !(s == ns - 1 && rb == 2)));
     }
   return BSAM_OK;
@@ -4243,7 +4292,9 @@ int32_t mixed_iteration(bsam_ctx* c) {
   }
   RETCHK(mixed_vcycle(c, c->finest));
   // with the fused finest sweeps, q += e has been done by the last sweep
-  for (int level = 0; level <= c->finest && !mixed_fused(c); level++) {
+  // (on a refined mesh for the finest level only)
+  const int top = mixed_fused(c) ? -1 : (mixed_fused_amr(c) ? c->finest - 1 : c->finest);
+  for (int level = 0; level <= top; level++) {
     Level& L = c->L(level);
     if (L.qspan_len > 0) {
       k_add_corr<<<nblocks(L.qspan_len), kBlock, 0, c->s>>>(c->q + L.qspan_off, m->e + L.qspan_off, L.qspan_len);

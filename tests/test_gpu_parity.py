@@ -145,6 +145,32 @@ def test_mixed_precision_matches_fp64(nd, n, steps, gpu_solver_cls):
     assert max(cb) <= max(ca) + 2
 
 
+@pytest.mark.parametrize("name", ["ch2d_amr", "ch3d_amr", "ch2d_amr_phys", "ch3d_amr_phys"])
+def test_mixed_precision_on_amr_matches_fp64(name, gpu_solver_cls):
+    """Mixed mode on the reference's AMR hierarchies (flattened ghost fill
+    of the correction, q += e fused into the finest level's last sweep,
+    staged fill where a level cannot be flattened): from the reference's
+    state at the first step, both paths stop below the fixture's tolerance
+    and agree on every level to a small multiple of it."""
+    meta, z = fu.load(name)
+    print(f"[gpu] mixed amr {name}", flush=True)
+    outs = {}
+    for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+        st = meta["steps"][0]
+        S = gpu_solver_cls(fu.step_params(meta, precision=prec), st["hierarchy"])
+        fu.load_state(S, meta, z, 0)
+        rc, ncyc, r, hist = S.multigrid_iterations()
+        assert rc == abi.BSAM_OK and r < meta["tol"], (prec, rc, r)
+        outs[prec] = ({i: S.get(i, "q") for i in st["end"]}, ncyc)
+        S.close()
+    qa, ca = outs[abi.PREC_FP64]
+    qb, cb = outs[abi.PREC_MIXED]
+    err = max(float(np.max(np.abs(qa[i] - qb[i]))) for i in qa)
+    print(f"[gpu] mixed amr {name} max-norm diff {err:.3e}, cycles fp64 {ca} mixed {cb}", flush=True)
+    assert err < 1e-7
+    assert cb <= ca + 2
+
+
 @pytest.mark.parametrize("shape,mthbc", [((128, 128, 128), (2, 2, 2, 2, 2, 2)),
                                           ((128, 128, 128), (2, 2, 2, 2, 10, 10)),
                                           ((200, 36, 16), (2, 2, 2, 2, 2, 2)),
