@@ -1418,10 +1418,11 @@ __global__ __launch_bounds__(kBlock) void k_add_corr(double* __restrict__ q, flo
   }
 }
 
-// c = 3 phi^2 over the interior of every patch of a level.
-template <int DIM>
+// c = 3 phi^2 over the interior of every patch of a level; with TM (the
+// tumour model) also t0 = (1 + phi_old)/2 in variable 1.
+template <int DIM, bool TM = false>
 __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __restrict__ q,
-                                                 float* __restrict__ coef) {
+                                                 const double* __restrict__ qold, float* __restrict__ coef) {
   int pi, local;
   if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
@@ -1432,6 +1433,7 @@ __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __re
   const int o = qidx(P, i, j, k);
   const double p = q[o];
   coef[o] = (float)(3.0 * (p * p));
+  if (TM) coef[o + P.qvs] = (float)(0.5 * (qold[o] + 1.0));
 }
 
 template <int DIM>
@@ -1556,6 +1558,117 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
   const float a2 = em - coef[opar] * ep + T2 * ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, 0);
   rpar[opar] = w * s0 + a1;
   rpar[opar + Q.qvs] = w * s1 + a2;
+}
+
+// The tumour model's correction operator: the Jacobian of its Operator,
+//   A1(e) = e_phi - (dt/h^2) ULap(e_mu) - dt lamp t0 e_n,
+//   A2(e) = e_mu - c e_phi + (eps^2/h^2) ULap(e_phi),
+//   A3(e) = -ULap(e_n)/h^2 + kappa t0 e_n,
+//   A4(e) = -ULap(e_p)/h^2 - lamp t0 e_n,
+// with the frozen c = 3 phi^2 (coef variable 0) and the lagged tumour
+// fraction t0 = (1 + phi_old)/2 (coef variable 1) of the cell.  CG selects
+// the qc geometry.
+template <int DIM, bool CG>
+__device__ __forceinline__ void lin_tm(const float* __restrict__ a, const DPatch& P, int i, int j, int k,
+                                       float cc, float t0, double h, const ModelK& m, float out[4]) {
+  const int vs = CG ? P.cvs : P.qvs;
+  auto lap = [&](int off) { return CG ? ulap_cf<DIM>(a, P, i, j, k, off) : ulap_f<DIM>(a, P, i, j, k, off); };
+  const int o = CG ? cidx(P, i, j, k) : qidx(P, i, j, k);
+  const double h2 = h * h;
+  const float t1 = (float)(m.dt / h2), t2 = (float)(m.eps2 / h2), hh = (float)(1.0 / h2);
+  const float e0 = a[o], e1 = a[o + vs], e2 = a[o + 2 * vs];
+  out[0] = e0 - t1 * lap(vs) - (float)(m.dt * m.gam) * t0 * e2;
+  out[1] = e1 - cc * e0 + t2 * lap(0);
+  out[2] = -hh * lap(2 * vs) + (float)m.cz * t0 * e2;
+  out[3] = -hh * lap(3 * vs) - (float)m.gam * t0 * e2;
+}
+
+// The tumour model's correction smoother: the pointwise update of
+// k_relax_tm (nutrient first, then the 2x2 phi/mu solve, then pressure) for
+// the operator A above.  EMODE as in k_relax_lin, over all four variables.
+template <int DIM, int EMODE = 0>
+__global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, ModelK m, float* __restrict__ e,
+                                                         const float* __restrict__ rhs,
+                                                         const float* __restrict__ coef,
+                                                         double* __restrict__ q64 = nullptr) {
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int t = local % cnt;
+  const int row = local / cnt;
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int ib = (c + j + k) & 1;
+  const int i = 2 * t + 2 - ib;
+  if (i > P.mx[0]) return;
+  const double h2 = P.dx * P.dx, dd = (DIM == 3) ? 6.0 : 4.0;
+  const float cdt = (float)(m.dt / h2), ceps = (float)(m.eps2 / h2), cdiag = (float)(dd * m.eps2 / h2);
+  const float c12 = (float)(dd * (m.dt / h2)), chh = (float)(1.0 / h2), cd = (float)(dd / h2);
+  const int vs = P.qvs;
+  const int o = qidx(P, i, j, k);
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (EMODE != 1) {
+    const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
+    const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int off = v * vs;
+      s[v] = e[xm + off] + e[xp + off] + e[ym + off] + e[yp + off];
+      if (DIM == 3) s[v] = s[v] + e[qidx(P, i, j, k - 1) + off] + e[qidx(P, i, j, k + 1) + off];
+    }
+  }
+  const float t0 = coef[o + vs];
+  const float en = (rhs[o + 2 * vs] + chh * s[2]) / (cd + (float)m.cz * t0);
+  const float c21 = -coef[o] - cdiag;
+  const float r1 = rhs[o] + cdt * s[1] + (float)(m.dt * m.gam) * t0 * en;
+  const float r2 = rhs[o + vs] - ceps * s[0];
+  const float d = 1.0f - c12 * c21;
+  const float n[4] = {(r1 - c12 * r2) / d, (r2 - c21 * r1) / d, en,
+                      (rhs[o + 3 * vs] + chh * s[3] + (float)m.gam * t0 * en) / cd};
+  const int po = qidx(P, 2 * t + 1 + ib, j, k);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    e[o + v * vs] = n[v];
+    if (EMODE == 1) e[po + v * vs] = 0.0f;
+    if (EMODE == 2) {
+      q64[o + v * vs] = q64[o + v * vs] + (double)n[v];
+      q64[po + v * vs] = q64[po + v * vs] + (double)e[po + v * vs];
+    }
+  }
+}
+
+// Coarse right-hand side of the tumour model's correction V-cycle:
+// parent rhs(mb) = R(rhs - A_h e) + A_2h(ec), as k_fas_rhs_lin.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_fas_rhs_lin_tm(LevelArgs L, ModelK m, const float* __restrict__ e,
+                                                           const float* __restrict__ rhs,
+                                                           const float* __restrict__ ec,
+                                                           const float* __restrict__ coef,
+                                                           float* __restrict__ rpar) {
+  CoarseCell c;
+  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
+  const DPatch P = L.pt[c.pid];
+  const DPatch Q = L.pt[P.parent];
+  const int nch = (DIM == 3) ? 8 : 4;
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int n = 0; n < nch; n++) {
+    const int i = 2 * c.ic - (n & 1), j = 2 * c.jc - ((n >> 1) & 1);
+    const int k = (DIM == 3) ? 2 * c.kc - ((n >> 2) & 1) : 0;
+    const int o = qidx(P, i, j, k);
+    float a[4];
+    lin_tm<DIM, false>(e, P, i, j, k, coef[o], coef[o + P.qvs], P.dx, m, a);
+#pragma unroll
+    for (int v = 0; v < 4; v++) s[v] += rhs[o + v * P.qvs] - a[v];
+  }
+  const float w = (DIM == 3) ? 0.125f : 0.25f;
+  const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
+  const int kp = (DIM == 3) ? P.mb[2][0] + c.kc - 1 : 0;
+  const int opar = qidx(Q, ip, jp, kp);
+  float a[4];
+  lin_tm<DIM, true>(ec, P, c.ic, c.jc, c.kc, coef[opar], coef[opar + Q.qvs], Q.dx, m, a);
+#pragma unroll
+  for (int v = 0; v < 4; v++) rpar[opar + v * Q.qvs] = w * s[v] + a[v];
 }
 
 // ---------------------------------------------------------------------------
@@ -2096,8 +2209,8 @@ extern "C" int32_t bsam_create(const bsam_params* params, bsam_ctx** out) {
   if (p.nsmoothingpasses < 1 || p.maxvcycles < 1) return set_err(BSAM_EINVAL, "bad V-cycle knobs");
   if (p.precision != BSAM_PREC_FP64 && p.precision != BSAM_PREC_MIXED)
     return set_err(BSAM_EINVAL, "unknown precision mode %d", p.precision);
-  if (p.precision == BSAM_PREC_MIXED && p.model != BSAM_MODEL_CH)
-    return set_err(BSAM_EINVAL, "mixed precision is implemented for the CH model only");
+  if (p.precision == BSAM_PREC_MIXED && p.model != BSAM_MODEL_CH && p.model != BSAM_MODEL_TUMOUR)
+    return set_err(BSAM_EINVAL, "mixed precision is implemented for the CH and tumour models only");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(BSAM_ENODEV, "no HIP device visible");
@@ -4102,7 +4215,7 @@ namespace {
 // that is one large patch periodic in every dimension, swept by the
 // marching kernel.
 static bool mixed_fused(bsam_ctx* c) {
-  if (c->finest != 0 || c->p.nsmoothingpasses < 1) return false;
+  if (c->finest != 0 || c->p.nsmoothingpasses < 1 || c->p.model != BSAM_MODEL_CH) return false;
   Level& L = c->L(0);
   int nblk = 0;
   return L.wrap_pid >= 0 && sweep_kc(c, L, 4, &nblk) > 0;
@@ -4116,7 +4229,8 @@ static bool mixed_fused_amr(bsam_ctx* c) {
   if (c->finest == 0 || c->p.nsmoothingpasses < 1) return false;
   Level& L = c->L(c->finest);
   int nblk = 0;
-  if (L.wrap_pid >= 0 || sweep_kc(c, L, 4, &nblk) > 0) return false;
+  if (L.wrap_pid >= 0) return false;
+  if (c->p.model == BSAM_MODEL_CH && sweep_kc(c, L, 4, &nblk) > 0) return false;
   for (int pid : L.patches)
     if (c->hp[pid].mx[0] % 2) return false;
   return true;
@@ -4135,12 +4249,31 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
   const float tmp1 = (float)(c->p.dt / h2), tmp2 = (float)(eps2 / h2), tmp3 = (float)(d2 * eps2 / h2);
   const float a12 = (float)(d2 * (c->p.dt / h2));
   const bool fin = level == c->finest;
-  const bool wrap = L.wrap_pid >= 0;
+  const bool tm = c->p.model == BSAM_MODEL_TUMOUR;
+  // the tumour sweep writes no periodic images: its SetGhost follows
+  const bool wrap = L.wrap_pid >= 0 && !tm;
   int nblk = 0;
-  const int kc = sweep_kc(c, L, 4, &nblk);
+  const int kc = tm ? 0 : sweep_kc(c, L, 4, &nblk);
   const bool timed = fin && emode == 0;
   if (timed) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
-  if (kc) {
+  if (tm) {
+    const ModelK mk = c->mk();
+    FOR_LAUNCHES(c, L, kHalf, a, nb) {
+      if (emode == 1)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin_tm<2, 1><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)),
+                     (k_relax_lin_tm<3, 1><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)));
+      else if (emode == 2)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin_tm<2, 2><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef, c->q)),
+                     (k_relax_lin_tm<3, 2><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef, c->q)));
+      else
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin_tm<2><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)),
+                     (k_relax_lin_tm<3><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)));
+      LAUNCH_CHECK();
+    }
+  } else if (kc) {
     const int pid = L.patches[0];
     if (wrap) {
       if (emode == 1)
@@ -4184,8 +4317,11 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
     }
     // e_phi and e_mu of the other colour over (N+G)/2 each, rhs (2 fields)
     // and coef of the colour N/2 each, the colour's e written N/2 x 2
-    // fields (the correction smoother does not read the cell's own e)
-    kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + N + 0.5 * N + N));
+    // fields (the correction smoother does not read the cell's own e); the
+    // tumour model: e (4 fields) of the other colour, rhs (4) and coef (2)
+    // of the colour, e (4) written
+    if (tm) kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * (2.0 * (N + G) + 2.0 * N + N + 2.0 * N));
+    else kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + N + 0.5 * N + N));
   }
   return wrap ? 1 : 0;
 }
@@ -4219,10 +4355,16 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
     if (!rw) RETCHK(set_ghost_t<float>(c, level - 1, 0, m->e, 1));
     RETCHK(ring_t<float>(c, level, m->e, m->ec));
     Level& L = c->L(level);
+    const ModelK mk = c->mk();
     FOR_LAUNCHES(c, L, kCoarse, a, nb) {
-      DISPATCH_DIM(c->p.ndims,
-                   (k_fas_rhs_lin<2><<<nb, kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)),
-                   (k_fas_rhs_lin<3><<<nb, kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)));
+      if (c->p.model == BSAM_MODEL_TUMOUR)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_fas_rhs_lin_tm<2><<<nb, kBlock, 0, c->s>>>(a, mk, m->e, m->rhs, m->ec, m->coef, m->rhs)),
+                     (k_fas_rhs_lin_tm<3><<<nb, kBlock, 0, c->s>>>(a, mk, m->e, m->rhs, m->ec, m->coef, m->rhs)));
+      else
+        DISPATCH_DIM(c->p.ndims,
+                     (k_fas_rhs_lin<2><<<nb, kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)),
+                     (k_fas_rhs_lin<3><<<nb, kBlock, 0, c->s>>>(a, c->p.dt, c->p.model_params[0], m->e, m->rhs, m->ec, m->coef, m->rhs)));
       LAUNCH_CHECK();
     }
     RETCHK(mixed_vcycle(c, level - 1));
@@ -4253,9 +4395,14 @@ int32_t mixed_residuals(bsam_ctx* c) {
       int nbl = 0;
       for (LaunchIter it(c, L, kCell); it.next(a, nbl);) {
         double* pp = part ? part + (long)kNvMax * it.boff : nullptr;
-        DISPATCH_DIM(c->p.ndims,
-                     (k_residual<2, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)),
-                     (k_residual<3, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)));
+        if (c->p.model == BSAM_MODEL_TUMOUR)
+          DISPATCH_DIM(c->p.ndims,
+                       (k_residual<2, BSAM_MODEL_TUMOUR, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)),
+                       (k_residual<3, BSAM_MODEL_TUMOUR, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)));
+        else
+          DISPATCH_DIM(c->p.ndims,
+                       (k_residual<2, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)),
+                       (k_residual<3, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)));
         LAUNCH_CHECK();
         nb = it.blocks();
       }
@@ -4268,7 +4415,11 @@ int32_t mixed_residuals(bsam_ctx* c) {
         G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
                                : 2.0 * ((double)P.mx[0] + P.mx[1]);
       }
-      kstat_end(c, BSAM_KSTAT_RESID_FINEST, 16.0 * (N + G) + 16.0 * N + 8.0 * N);
+      // q over the interior and face ghosts, f, the FP32 right-hand side
+      // (all fields) and, for the tumour model, qold of phi
+      const double nv = c->p.nrvars;
+      kstat_end(c, BSAM_KSTAT_RESID_FINEST,
+                8.0 * nv * (N + G) + 8.0 * nv * N + 4.0 * nv * N + (c->p.model == BSAM_MODEL_TUMOUR ? 8.0 * N : 0.0));
       nbf = nb;
     }
   }
@@ -4332,8 +4483,12 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
   for (int level = c->minlevel; level <= c->finest; level++) {
     Level& L = c->L(level);
     FOR_LAUNCHES(c, L, kCell, a, nb) {
-      DISPATCH_DIM(c->p.ndims, (k_coef<2><<<nb, kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)),
-                   (k_coef<3><<<nb, kBlock, 0, c->s>>>(a, c->q, c->mixed->coef)));
+      if (c->p.model == BSAM_MODEL_TUMOUR)
+        DISPATCH_DIM(c->p.ndims, (k_coef<2, true><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)),
+                     (k_coef<3, true><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)));
+      else
+        DISPATCH_DIM(c->p.ndims, (k_coef<2><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)),
+                     (k_coef<3><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)));
       LAUNCH_CHECK();
     }
   }

@@ -62,7 +62,8 @@ extern "C" {
 
 /* ---- precision modes -------------------------------------------------- */
 #define BSAM_PREC_FP64       0   /* FP64 everywhere (the reference's r8) */
-#define BSAM_PREC_MIXED      1   /* FP64 outer defect correction, FP32 V-cycles */
+#define BSAM_PREC_MIXED      1   /* FP64 outer defect correction, FP32 V-cycles
+                                    (CH and tumour models; BSAM_EINVAL for BG) */
 
 /* ---- per-field selector for bsam_fill_down (nodeinfodef.f90:69-70) --- */
 #define BSAM_SOURCEFIELD     1

@@ -69,3 +69,17 @@ def test_create_without_gpu_fails_cleanly():
     rc = lib.bsam_create(C.byref(p), C.byref(ctx))
     assert rc == abi.BSAM_ENODEV
     assert b"device" in lib.bsam_last_error()
+
+
+def test_mixed_precision_model_check():
+    """Mixed precision is built for the CH and tumour models; asking for it
+    with the BioGrowth model is refused as an invalid argument before any
+    device is touched."""
+    lib = gpu.load()
+    ctx = C.c_void_p()
+    p = abi.Params()
+    p.ndims, p.nrvars, p.mbc, p.model, p.nsmoothingpasses, p.maxvcycles, p.errortype = 2, 2, 1, abi.MODEL_BG, 2, 5, 1
+    p.precision = abi.PREC_MIXED
+    rc = lib.bsam_create(C.byref(p), C.byref(ctx))
+    assert rc == abi.BSAM_EINVAL
+    assert b"mixed precision" in lib.bsam_last_error()

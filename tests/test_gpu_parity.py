@@ -145,13 +145,14 @@ def test_mixed_precision_matches_fp64(nd, n, steps, gpu_solver_cls):
     assert max(cb) <= max(ca) + 2
 
 
-@pytest.mark.parametrize("name", ["ch2d_amr", "ch3d_amr", "ch2d_amr_phys", "ch3d_amr_phys"])
+@pytest.mark.parametrize("name", ["ch2d_amr", "ch3d_amr", "ch2d_amr_phys", "ch3d_amr_phys"] + fu.TUMOUR_NAMES)
 def test_mixed_precision_on_amr_matches_fp64(name, gpu_solver_cls):
     """Mixed mode on the reference's AMR hierarchies (flattened ghost fill
     of the correction, q += e fused into the finest level's last sweep,
     staged fill where a level cannot be flattened): from the reference's
     state at the first step, both paths stop below the fixture's tolerance
-    and agree on every level to a small multiple of it."""
+    and agree on every level to a small multiple of it.  The tumour cases
+    run the four-field correction cycle (linearised phi^3, lagged t0)."""
     meta, z = fu.load(name)
     print(f"[gpu] mixed amr {name}", flush=True)
     outs = {}
