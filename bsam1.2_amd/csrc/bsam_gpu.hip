@@ -2027,8 +2027,9 @@ struct Level {
   // the same ghost fill flattened to one rule per ghost cell (levels > 0,
   // per-level launches): gmap[ipass] is SetGhost(level, ipass) in full,
   // gmap[2 + ipass] (ipass 1, 2) and gmap[5] (ipass 0) the ghosts a red-black sweep
-  // sequence needs (see plan_ghost_maps)
-  DevMap gmap[6];
+  // sequence needs, gmap[6] those of gmap[5] a correction does not leave
+  // exact (see plan_ghost_maps)
+  DevMap gmap[7];
   bool maps_ok = false;
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
@@ -2376,7 +2377,7 @@ struct GhostPlan {
   std::vector<GRule> rules;
   std::vector<int> roff[3];
   bool maps_ok = false;
-  HostMap map[6];
+  HostMap map[7];
   std::string why;  // why the flattened maps are not used
 };
 
@@ -2832,6 +2833,36 @@ static bool same_expr(const GhostPlan& G, const FExpr& a, const FExpr& b) {
 // what the full calls would give until SetGhost(2) rewrites it; and
 // SetGhost(level, 0) writes no cell that neither ipass 1 nor 2 writes, so
 // the sequence ends with every cell as the staged fills leave it.
+// Cells along dimension a of the whole domain at `level` (the root's extent
+// refined level times).
+static int level_period(const bsam_ctx* c, int level, int a) {
+  for (const bsam_patch_desc& d : c->desc)
+    if (d.level == 0) return level >= 0 ? d.mx[a] << level : d.mx[a] >> -level;
+  return 1;
+}
+
+// Cells of the whole domain at `level` (x fastest) that lie inside one of
+// the level's patches.
+static std::vector<uint8_t> parent_interior(const bsam_ctx* c, int level) {
+  const int nd = c->p.ndims;
+  long n = 1;
+  int per[3] = {1, 1, 1};
+  for (int a = 0; a < nd; a++) {
+    per[a] = level_period(c, level, a);
+    n *= per[a];
+  }
+  std::vector<uint8_t> occ(n, 0);
+  for (const bsam_patch_desc& d : c->desc) {
+    if (d.level != level) continue;
+    const int k0 = nd == 3 ? d.mglobal[2][0] : 1, k1 = nd == 3 ? d.mglobal[2][1] : 1;
+    for (int k = k0; k <= k1; k++)
+      for (int j = d.mglobal[1][0]; j <= d.mglobal[1][1]; j++)
+        for (int i = d.mglobal[0][0]; i <= d.mglobal[0][1]; i++)
+          occ[((long)(k - 1) * per[1] + (j - 1)) * per[0] + (i - 1)] = 1;
+  }
+  return occ;
+}
+
 int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   G.maps_ok = false;
   if (L.level <= 0) return BSAM_OK;
@@ -2901,6 +2932,72 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
         }
       }
       emit(E[0], sel, G.map[5]);
+      // map[6]: map[5] less the copies that CorrectFine leaves exact.  A
+      // ghost whose value is a plain copy (no post-operation) of an interior
+      // cell at the same global position (modulo the period) held that
+      // cell's value before the correction (the last SetGhost of the
+      // pre-smoothing wrote it), and both receive the prolongation of the
+      // same coarse correction, by the same expression, provided every
+      // coarse cell of the prolongation stencil holds one value for both
+      // patches: a cell inside a patch of the parent level (its ghost
+      // images in other parents are copies of it, and the coarse-under-fine
+      // values are restrictions of the same fine cells).  A coarse cell
+      // outside every parent-level patch is a coarse-fine ghost that each
+      // parent interpolates from its own cells; such ghosts, and
+      // reflecting copies (mirrored position), stay in the table.
+      const std::vector<uint8_t> occ = parent_interior(c, L.level - 1);
+      int cper[3] = {1, 1, 1}, cwrap[3] = {0, 0, 0};
+      const bsam_patch_desc* root = nullptr;
+      for (const bsam_patch_desc& d : c->desc)
+        if (d.level == 0) root = &d;
+      for (int a = 0; a < nd; a++) {
+        cper[a] = level_period(c, L.level - 1, a);
+        cwrap[a] = root->mthbc[2 * a] == 2;
+      }
+      auto covered = [&](const int* g) {
+        long idx = 0;
+        for (int a = nd - 1; a >= 0; a--) {
+          int x = g[a] - 1;
+          if (x < 0 || x >= cper[a]) {
+            if (!cwrap[a]) return false;
+            x = ((x % cper[a]) + cper[a]) % cper[a];
+          }
+          idx = idx * cper[a] + x;
+        }
+        return occ[idx] != 0;
+      };
+      std::vector<int> sel6;
+      for (int n : sel) {
+        const FExpr& e = E[0].ex[n];
+        bool exact = e.kind == 0 && e.post == 0 && !e.early;
+        if (exact) {
+          const DPatch& S = c->hp[e.spid];
+          int s3[3], d3[3];
+          qdecode(S, e.src, s3[0], s3[1], s3[2]);
+          qdecode(c->hp[e.dpid], E[0].dst[n], d3[0], d3[1], d3[2]);
+          exact = n_outside(S, nd, s3[0], s3[1], s3[2]) == 0;
+          for (int a = 0; a < nd && exact; a++) {
+            const int per = level_period(c, L.level, a);
+            const int ds = (c->desc[e.spid].mglobal[a][0] + s3[a]) - (c->desc[e.dpid].mglobal[a][0] + d3[a]);
+            exact = ((ds % per) + per) % per == 0;
+          }
+          // the coarse cells of the prolongation of the destination: along
+          // each dimension an odd fine index g takes coarse (g+1)/2 - 1 and
+          // (g+1)/2, an even one g/2 and g/2 + 1 (BiLinProlongationP1,
+          // TriLinProlongationP1, gridutilities.f90:156-195, 510-566)
+          for (int corner = 0; corner < (1 << nd) && exact; corner++) {
+            int g[3] = {1, 1, 1};
+            for (int a = 0; a < nd; a++) {
+              const int gf = c->desc[e.dpid].mglobal[a][0] + d3[a] - 1;
+              const int lo = (gf & 1) ? (gf + 1) / 2 - 1 : gf / 2;
+              g[a] = lo + ((corner >> a) & 1);
+            }
+            exact = covered(g);
+          }
+        }
+        if (!exact) sel6.push_back(n);
+      }
+      emit(E[0], sel6, G.map[6]);
       continue;
     }
     const int col = p - 1;
@@ -2970,7 +3067,7 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
   }
   L.maps_ok = G.maps_ok;
   if (!G.maps_ok) return BSAM_OK;
-  for (int m = 0; m < 6; m++) {
+  for (int m = 0; m < 7; m++) {
     const HostMap& H = G.map[m];
     DevMap& D = L.gmap[m];
     D.nc = (int)H.copies.size();
@@ -3327,17 +3424,43 @@ extern "C" int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t n
       // (2) a red-black sequence (three passes, the last SetGhost full)
       // from an arbitrary state, each sweep replaced by new values of its
       // colour's interior cells; (3) the same after SetGhost(level, 0)
-      // against its reduced table map[5]
-      for (int variant = 0; variant < 2; variant++) {
+      // against its reduced table map[5]; (4) the same after a correction:
+      // from a state SetGhost(level, 0) has made consistent, every cell
+      // (ghosts included) shifted by one value per variable and global
+      // position modulo the period, as the prolongation of a coarse
+      // correction does, then SetGhost(level, 0) against map[6]
+      for (int variant = 0; variant < 3; variant++) {
       for (long n = 0; n < c->nq; n++) a[n] = rng.next();
-      b = a;
-      if (variant == 1) {
+      if (variant == 2) {
         host_staged(c, L, G, 0, homog, a);
-        host_map(c, G.map[5], homog, b);
+        for (int pid : L.patches) {
+          const DPatch& P = c->hp[pid];
+          const bsam_patch_desc& d = c->desc[pid];
+          for (int k = 0; k < P.nq[2]; k++)
+            for (int j = 0; j < P.nq[1]; j++)
+              for (int i = 0; i < P.nq[0]; i++) {
+                const int loc[3] = {i, j, k};
+                uint64_t h = 1469598103934665603ull;
+                for (int ax = 0; ax < nd; ax++) {
+                  const int per = level_period(c, L.level, ax);
+                  const int g = (((d.mglobal[ax][0] + loc[ax]) % per) + per) % per;
+                  h = (h ^ (uint64_t)g) * 1099511628211ull;
+                }
+                for (int v = 0; v < nv; v++) {
+                  const uint64_t hv = (h ^ (uint64_t)(v + 1)) * 0x9E3779B97F4A7C15ull;
+                  a[qidx(P, i, j, k) + v * P.qvs] += (double)(hv >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+                }
+              }
+        }
+      }
+      b = a;
+      if (variant >= 1) {
+        host_staged(c, L, G, 0, homog, a);
+        host_map(c, G.map[variant == 1 ? 5 : 6], homog, b);
         for (int o : face[1])
           if (memcmp(&a[o], &b[o], sizeof(double)))
-            return set_err(BSAM_EINVAL, "level %d homog %d: colour-1 face ghost %d differs after map[5]", L.level,
-                           homog, o);
+            return set_err(BSAM_EINVAL, "level %d homog %d: colour-1 face ghost %d differs after map[%d]", L.level,
+                           homog, o, variant == 1 ? 5 : 6);
         *checked += face[1].size();
       }
       const int ns = 3;
@@ -3499,14 +3622,19 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
 }
 
 // SetGhost on any field of the q geometry (FP64 solution or FP32 correction).
-// `lean` (ipass 1, 2 inside a red-black sweep sequence whose last SetGhost
-// is not lean) refreshes only the face ghosts such a sequence reads.
+// form kLean (ipass 1, 2 inside a red-black sweep sequence whose last
+// SetGhost is not lean) refreshes only the face ghosts such a sequence
+// reads; kAfterCorrection (ipass 0, right after CorrectFine, before such a
+// sequence) also leaves out the ghosts the correction left exact.  Levels
+// without flattened tables do the full SetGhost.
+enum GhostForm : int { kFull = 0, kLean = 1, kAfterCorrection = 2 };
 template <typename T>
-static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, bool lean = false) {
+static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, int form = kFull) {
   Level& L = c->L(level);
   const int nv = c->p.nrvars;
   if (L.maps_ok && c->p.batching) {
-    const DevMap& M = L.gmap[lean ? (ipass > 0 ? 2 + ipass : 5) : ipass];
+    const int mi = form == kAfterCorrection ? 6 : (form == kLean ? (ipass > 0 ? 2 + ipass : 5) : ipass);
+    const DevMap& M = L.gmap[mi];
     // rules reading ghosts from before the call first, then the rest
     if (M.nc1 + M.ni1 > 0) {
       k_ghost_map<T><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q);
@@ -3629,7 +3757,7 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass, bool lean) {
   const int kc = (c->p.model == BSAM_MODEL_CH) ? sweep_kc(c, L, 8, &nblk) : 0;
   if (!kc && !(wrap && c->p.model == BSAM_MODEL_CH)) {
     RETCHK(bsam_relax_level(c, level, ipass));
-    return set_ghost_t<double>(c, level, ipass, c->q, 0, lean);
+    return set_ghost_t<double>(c, level, ipass, c->q, 0, lean ? kLean : kFull);
   }
   const int col = ipass - 1;
   const bool fin = level == c->finest;
@@ -3661,7 +3789,7 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass, bool lean) {
     }
     kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 8.0 * ((N + G) + 0.5 * (N + G) + N + N));
   }
-  if (!wrap) RETCHK(set_ghost_t<double>(c, level, ipass, c->q, 0, lean));
+  if (!wrap) RETCHK(set_ghost_t<double>(c, level, ipass, c->q, 0, lean ? kLean : kFull));
   return BSAM_OK;
 }
 
@@ -4033,7 +4161,7 @@ extern "C" int32_t bsam_vcycle(bsam_ctx* c, int32_t level) {
     RETCHK(correct_t<double>(c, level, c->q, c->qc, cw));
     // SetGhost(level, 0); the post-smoothing sweeps that follow read only
     // the reduced table's ghosts before rewriting the rest
-    if (!cw) RETCHK(set_ghost_t<double>(c, level, 0, c->q, 0, true));
+    if (!cw) RETCHK(set_ghost_t<double>(c, level, 0, c->q, 0, kAfterCorrection));
     RETCHK(bsam_level_relax(c, level));
   }
   return BSAM_OK;
@@ -4338,9 +4466,16 @@ int32_t mixed_level_relax(bsam_ctx* c, int level, int phase) {
       const int32_t done = mixed_relax(c, level, rb, emode);
       RETCHK(done);
       // after the finest level's last sweep nothing reads its correction's
-      // ghosts: q has been updated and the next cycle starts from zero
+      // ghosts: q has been updated and the next cycle starts from zero.
+      // The pre-smoothing's last fill is lean too when the cycle descends:
+      // after two lean fills every face ghost is current, and until the
+      // post-correction fill only face ghosts are read (the coarse right-
+      // hand side); edge and corner ghosts are rewritten by the post-
+      // smoothing's last, full, fill.
       if (emode == 2) continue;
-      if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, !(s == ns - 1 && rb == 2)));
+      const bool last = s == ns - 1 && rb == 2;
+      const bool lean = !last || (phase == 0 && level > c->minlevel);
+      if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, lean ? kLean : kFull));
     }
   return BSAM_OK;
 }
@@ -4370,7 +4505,7 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
     RETCHK(mixed_vcycle(c, level - 1));
     const bool cw = c->L(level).wrap_pid >= 0;
     RETCHK(correct_t<float>(c, level, m->e, m->ec, cw));
-    if (!cw) RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1, true));
+    if (!cw) RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1, kAfterCorrection));
     RETCHK(mixed_level_relax(c, level, 1));
   }
   return BSAM_OK;
@@ -4461,11 +4596,12 @@ int32_t mixed_iteration(bsam_ctx* c) {
   }
   // With refined levels the finest residual sees the coarse level through
   // the coarse-fine ghost interpolation, so the coarse q must be the
-  // restriction of the updated fine q first.  A single-level mesh has no
-  // such coupling (its ghosts were updated with e), and FillDown is done
-  // once after the loop.
+  // restriction of the updated fine q first (levels >= 1 into their
+  // parents; nothing in the loop reads q below the root).  A single-level
+  // mesh has no such coupling (its ghosts were updated with e).  The rest
+  // of FillDown is done once after the loop.
   if (c->finest > 0) {
-    RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+    RETCHK(fill_down_range(c, BSAM_SOLUTIONFIELD, c->finest, 1));
     RETCHK(set_ghost_t<double>(c, c->finest, 0, c->q, 0));
   }
   RETCHK(mixed_residuals(c));
@@ -4509,7 +4645,7 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
     }
     if (res < c->p.qerrortol) break;
   }
-  if (c->finest == 0) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+  RETCHK(fill_down_range(c, BSAM_SOLUTIONFIELD, 0, c->minlevel + 1));
   *ncycles = it > c->p.maxvcycles ? c->p.maxvcycles : it;
   *residual = res;
   return it > c->p.maxvcycles ? BSAM_NOT_CONVERGED : BSAM_OK;
