@@ -107,6 +107,23 @@ struct LevelArgs {
   int fixed;  // the patch of a one-patch launch, else -1
 };
 
+// Cell (i, j, k) of element `local` of a patch's kCell range, which runs
+// colour by colour, then row by row (j fastest, then k), then along the
+// colour's half-row, so that consecutive elements are consecutive cells of
+// one colour array; false for the padding element of an odd row length.
+template <int DIM>
+__device__ __forceinline__ bool cell_of(const DPatch& P, int local, int& i, int& j, int& k) {
+  const int hx = (P.mx[0] + 1) >> 1;
+  const int per = hx * P.mx[1] * (DIM == 3 ? P.mx[2] : 1);
+  const int c = local >= per;
+  const int r = local - c * per;
+  const int t = r % hx, row = r / hx;
+  j = row % P.mx[1] + 1;
+  k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  i = 2 * t + 2 - ((c + j + k) & 1);
+  return i <= P.mx[0];
+}
+
 // Patch (position in the level's list) and local index of element
 // gbase + lb*kBlock + threadIdx.x; false for the padding after a patch's
 // range and past the launch's end.
@@ -532,10 +549,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
   int pi, local;
   if (level_elem(L, lb, pi, local)) {
     const DPatch P = L.pt[L.patches[pi]];
-    const int i = local % P.mx[0] + 1;
-    const int row = local / P.mx[0];
-    const int j = row % P.mx[1] + 1;
-    const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+    int i, j, k;
+    if (cell_of<DIM>(P, local, i, j, k)) {
     double o[kNvMax];
     op_cell<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o);
     const int c0 = qidx(P, i, j, k);
@@ -544,6 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
       const double r = f[c0 + v * P.qvs] - o[v];
       if (rf) rf[c0 + v * P.qvs] = (TO)r;
       sv[v] += r * r;
+    }
     }
   }
   if (!partial) return;
@@ -1142,10 +1158,8 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, ModelK
   int pi, local;
   if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int i = local % P.mx[0] + 1;
-  const int row = local / P.mx[0];
-  const int j = row % P.mx[1] + 1;
-  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  int i, j, k;
+  if (!cell_of<DIM>(P, local, i, j, k)) return;
   const int o = qidx(P, i, j, k);
   if (mode != 1) {
     const double po = qold[o];
@@ -1426,10 +1440,8 @@ __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __re
   int pi, local;
   if (!level_elem(L, xcd_block(), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int i = local % P.mx[0] + 1;
-  const int row = local / P.mx[0];
-  const int j = row % P.mx[1] + 1;
-  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  int i, j, k;
+  if (!cell_of<DIM>(P, local, i, j, k)) return;
   const int o = qidx(P, i, j, k);
   const double p = q[o];
   coef[o] = (float)(3.0 * (p * p));
@@ -3199,7 +3211,7 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
         long cnt;
         switch (d) {
           case kHalf: cnt = (long)((P.mx[0] + 1) / 2) * P.mx[1] * P.mx[2]; break;
-          case kCell: cnt = (long)P.mx[0] * P.mx[1] * P.mx[2]; break;
+          case kCell: cnt = 2L * ((P.mx[0] + 1) / 2) * P.mx[1] * P.mx[2]; break;
           case kCoarse: cnt = (long)P.cmx[0] * P.cmx[1] * P.cmx[2]; break;
           case kFineG: cnt = (long)P.nq[0] * P.nq[1] * P.nq[2]; break;
           case kDual:
