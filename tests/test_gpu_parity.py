@@ -209,6 +209,37 @@ def test_large_patch_sweep_vs_oracle(shape, mthbc, gpu_solver_cls):
     _hist_close(hg, ho)
 
 
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16), (48, 24, 20), (96, 32), (40, 24)])
+def test_coarse_chain_vs_oracle(shape, gpu_solver_cls):
+    """Roots whose coarse levels run in the one-workgroup V-cycle chain:
+    cubic, non-cubic (64 x 32 x 16: levels 32 x 16 x 8 and 16 x 8 x 4), an
+    odd coarsest extent (48 x 24 x 20 ends at 12 x 6 x 5), and 2D.  One time
+    step, bit-identical to the oracle."""
+    nd = len(shape)
+    print(f"[gpu] chain {shape}", flush=True)
+    minlevel = -(int(np.log2(min(shape))) - 2)
+    hier = hierarchy.uniform(nd, shape, minlevel, [2] * (2 * nd))
+    root = len(hier) - 1
+    phi = configs.init_phi(hier[root], nd, 1, 1005, 0.1, 0.05)
+    q0 = np.zeros(hierarchy.field_shape(hier[root], nd, 2, "q"))
+    q0[(0,) + (slice(1, -1),) * nd] = phi
+    res = []
+    for S in (gpu_solver_cls(configs.params(nd, 5e-5, 0.015, 1e-9), hier),
+              oracle.Oracle(configs.params(nd, 5e-5, 0.015, 1e-9), hier)):
+        S.put(root, "q", q0)
+        S.set_ghost(0, 0)
+        S.copy_q_to_qold(0)
+        S.fill_down(abi.SOLUTIONFIELD)
+        for lv in range(minlevel, 1):
+            S.copy_q_to_qold(lv)
+        rc, ncyc, r, h = S.multigrid_iterations()
+        res.append((h, [S.get(k, "q") for k in range(len(hier))]))
+    (hg, qg), (ho, qo) = res
+    for k in range(len(hier)):
+        assert np.array_equal(qg[k], qo[k]), (k, float(np.max(np.abs(qg[k] - qo[k]))))
+    _hist_close(hg, ho)
+
+
 @pytest.mark.parametrize("shape", [(128, 128, 128), (200, 36, 16)])
 def test_large_patch_mixed_vs_fp64(shape, gpu_solver_cls):
     """Marching FP32 correction sweep and FP32 residual output on the finest
