@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bsam_gpu.h"
@@ -360,26 +361,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_relax3d_march(const DPatch* _
   }
 }
 
-// The tumour model's relaxation (the probe PROBLEM module's TumourRelax2D/
-// 3D): per cell of colour c, the nutrient n from its equation, the CH pair
-// by RelaxGrid's Newton step with dt*lamp*t0*n in its first right-hand
-// side, then the pressure; t0 = (1 + phi_old)/2.  ModelK: lamp in `gam`,
-// kappa in `cz`.  Domain: cells of colour c, (mx+1)/2 per row.
+// One cell of the tumour model's relaxation (the probe PROBLEM module's
+// TumourRelax2D/3D): nutrient, then the phi/mu 2x2 Newton solve, then
+// pressure, each with the updated nutrient.
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_relax_tm(LevelArgs L, int c, ModelK m, double* __restrict__ q,
-                                                     const double* __restrict__ qold,
-                                                     const double* __restrict__ f) {
-  int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
-  const DPatch P = L.pt[L.patches[pi]];
-  const int cnt = (P.mx[0] + 1) >> 1;
-  const int t = local % cnt;
-  const int row = local / cnt;
-  const int j = row % P.mx[1] + 1;
-  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
-  const int ib = (c + j + k) & 1;
-  const int i = 2 * t + 2 - ib;
-  if (i > P.mx[0]) return;
+__device__ __forceinline__ void relax_tm_cell(const DPatch& P, int i, int j, int k, const ModelK& m,
+                                              double* __restrict__ q, const double* __restrict__ qold,
+                                              const double* __restrict__ f) {
   const double h = P.dx, h2 = h * h, dd = (DIM == 3) ? 6.0 : 4.0;
   const double cdt = m.dt / h2, ceps = m.eps2 / h2, cdiag = dd * m.eps2 / h2, c12 = dd * cdt;
   const double chh = 1.0 / h2, cd = dd / h2;
@@ -409,6 +397,30 @@ __global__ __launch_bounds__(kBlock) void k_relax_tm(LevelArgs L, int c, ModelK 
   q[o + vs] = (r2 - c21 * r1) / d;
   q[o + 3 * vs] = (f[o + 3 * vs] + chh * nb(3 * vs) + m.gam * t0 * n) / cd;
 }
+
+// The tumour model's relaxation (the probe PROBLEM module's TumourRelax2D/
+// 3D): per cell of colour c, the nutrient n from its equation, the CH pair
+// by RelaxGrid's Newton step with dt*lamp*t0*n in its first right-hand
+// side, then the pressure; t0 = (1 + phi_old)/2.  ModelK: lamp in `gam`,
+// kappa in `cz`.  Domain: cells of colour c, (mx+1)/2 per row.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_relax_tm(LevelArgs L, int c, ModelK m, double* __restrict__ q,
+                                                     const double* __restrict__ qold,
+                                                     const double* __restrict__ f) {
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int t = local % cnt;
+  const int row = local / cnt;
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int ib = (c + j + k) & 1;
+  const int i = 2 * t + 2 - ib;
+  if (i > P.mx[0]) return;
+  relax_tm_cell<DIM>(P, i, j, k, m, q, qold, f);
+}
+
 
 // BioGrowth RelaxGrid2D, BioGrowth/problem.f90:255-296 (decoupled GS).
 __global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, ModelK m, double h,
@@ -1349,8 +1361,7 @@ struct BoxSeq {
   int ns;
 };
 template <typename T>
-__global__ __launch_bounds__(1024) void k_boxops_seq(const DPatch* __restrict__ pt, BoxSeq S, int nv, int homog,
-                                                     T* __restrict__ q) {
+__device__ void run_boxseq(const DPatch* __restrict__ pt, const BoxSeq& S, int nv, int homog, T* __restrict__ q) {
   for (int st = 0; st < S.ns; st++)
     for (int bx = 0; bx < S.n[st]; bx++) {
       const BoxDesc B = S.d[st][bx];
@@ -1360,6 +1371,11 @@ __global__ __launch_bounds__(1024) void k_boxops_seq(const DPatch* __restrict__ 
       for (int e = threadIdx.x; e < n * nv; e += blockDim.x) box_elem<T>(D, Sp, B, n, e, homog, q);
       __syncthreads();
     }
+}
+template <typename T>
+__global__ __launch_bounds__(1024) void k_boxops_seq(const DPatch* __restrict__ pt, BoxSeq S, int nv, int homog,
+                                                     T* __restrict__ q) {
+  run_boxseq<T>(pt, S, nv, homog, q);
 }
 
 // Self-periodic wrap of a patch periodic in every dimension (SelfPeriodicBC
@@ -1595,25 +1611,12 @@ __device__ __forceinline__ void lin_tm(const float* __restrict__ a, const DPatch
   out[3] = -hh * lap(3 * vs) - (float)m.gam * t0 * e2;
 }
 
-// The tumour model's correction smoother: the pointwise update of
-// k_relax_tm (nutrient first, then the 2x2 phi/mu solve, then pressure) for
-// the operator A above.  EMODE as in k_relax_lin, over all four variables.
-template <int DIM, int EMODE = 0>
-__global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, ModelK m, float* __restrict__ e,
-                                                         const float* __restrict__ rhs,
-                                                         const float* __restrict__ coef,
-                                                         double* __restrict__ q64 = nullptr) {
-  int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
-  const DPatch P = L.pt[L.patches[pi]];
-  const int cnt = (P.mx[0] + 1) >> 1;
-  const int t = local % cnt;
-  const int row = local / cnt;
-  const int j = row % P.mx[1] + 1;
-  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
-  const int ib = (c + j + k) & 1;
-  const int i = 2 * t + 2 - ib;
-  if (i > P.mx[0]) return;
+// One cell (i, j, k) of the tumour correction smoother; `ip` is the other
+// cell of the thread's pair (EMODE 1, 2).
+template <int DIM, int EMODE>
+__device__ __forceinline__ void relax_lin_tm_cell(const DPatch& P, int i, int j, int k, int ip, const ModelK& m,
+                                                  float* __restrict__ e, const float* __restrict__ rhs,
+                                                  const float* __restrict__ coef, double* __restrict__ q64) {
   const double h2 = P.dx * P.dx, dd = (DIM == 3) ? 6.0 : 4.0;
   const float cdt = (float)(m.dt / h2), ceps = (float)(m.eps2 / h2), cdiag = (float)(dd * m.eps2 / h2);
   const float c12 = (float)(dd * (m.dt / h2)), chh = (float)(1.0 / h2), cd = (float)(dd / h2);
@@ -1638,7 +1641,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, Mod
   const float d = 1.0f - c12 * c21;
   const float n[4] = {(r1 - c12 * r2) / d, (r2 - c21 * r1) / d, en,
                       (rhs[o + 3 * vs] + chh * s[3] + (float)m.gam * t0 * en) / cd};
-  const int po = qidx(P, 2 * t + 1 + ib, j, k);
+  const int po = qidx(P, ip, j, k);
 #pragma unroll
   for (int v = 0; v < 4; v++) {
     e[o + v * vs] = n[v];
@@ -1649,6 +1652,29 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, Mod
     }
   }
 }
+
+// The tumour model's correction smoother: the pointwise update of
+// k_relax_tm (nutrient first, then the 2x2 phi/mu solve, then pressure) for
+// the operator A above.  EMODE as in k_relax_lin, over all four variables.
+template <int DIM, int EMODE = 0>
+__global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, ModelK m, float* __restrict__ e,
+                                                         const float* __restrict__ rhs,
+                                                         const float* __restrict__ coef,
+                                                         double* __restrict__ q64 = nullptr) {
+  int pi, local;
+  if (!level_elem(L, xcd_block(), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int t = local % cnt;
+  const int row = local / cnt;
+  const int j = row % P.mx[1] + 1;
+  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  const int ib = (c + j + k) & 1;
+  const int i = 2 * t + 2 - ib;
+  if (i > P.mx[0]) return;
+  relax_lin_tm_cell<DIM, EMODE>(P, i, j, k, 2 * t + 1 + ib, m, e, rhs, coef, q64);
+}
+
 
 // Coarse right-hand side of the tumour model's correction V-cycle:
 // parent rhs(mb) = R(rhs - A_h e) + A_2h(ec), as k_fas_rhs_lin.
@@ -1954,6 +1980,198 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const DPatch* __restrict
     cv_up<DIM, T>(P, pt[A.pid[l + 1]], q, qc);
     __syncthreads();
     cv_level_relax<DIM, T, LIN>(P, nsm, A.dt, A.eps2, q, f, coef);
+  }
+}
+
+// The coarse end of the V-cycle in one workgroup for levels whose ghost
+// fill is a short sequence of box stages (physical boundaries), with the
+// tumour model: the steps of k_coarse_vcycle -- LevelRelax, RestrictSolution,
+// SetGhost(L-1,0), GetCoarseGhostPoints, the coarse load, recursion,
+// CorrectFine, SetGhost(L,0), LevelRelax -- each with the per-cell
+// expressions of the level-wise kernels (relax_tm_cell / relax_lin_tm_cell,
+// restrict_cell, op_cell and op_tm_from or lin_tm, the prolongation of
+// k_prolong_dual) and the box stages of k_boxops_seq, so the results are
+// bit-identical.  LIN=false: FP64; LIN=true: the FP32 correction cycle.
+constexpr int kMaxChain = 12;
+struct TmChainArgs {
+  int pid[kMaxChain];
+  BoxSeq seq[kMaxChain];
+  int nlev;
+  ModelK m;
+};
+
+template <int DIM, bool LIN>
+__device__ void tc_level_relax(const DPatch* __restrict__ pt, const TmChainArgs& A, int l, int nsm,
+                               typename std::conditional<LIN, float, double>::type* __restrict__ q,
+                               const typename std::conditional<LIN, float, double>::type* __restrict__ f,
+                               const double* __restrict__ qold, const float* __restrict__ coef) {
+  using T = typename std::conditional<LIN, float, double>::type;
+  const DPatch P = pt[A.pid[l]];
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int total = cnt * P.mx[1] * (DIM == 3 ? P.mx[2] : 1);
+  for (int s = 0; s < nsm; s++)
+    for (int c = 0; c < 2; c++) {
+      for (int g = threadIdx.x; g < total; g += blockDim.x) {
+        const int t = g % cnt, row = g / cnt;
+        const int j = row % P.mx[1] + 1;
+        const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+        const int ib = (c + j + k) & 1;
+        const int i = 2 * t + 2 - ib;
+        if (i > P.mx[0]) continue;
+        if constexpr (LIN) relax_lin_tm_cell<DIM, 0>(P, i, j, k, 0, A.m, q, f, coef, nullptr);
+        else relax_tm_cell<DIM>(P, i, j, k, A.m, q, qold, f);
+      }
+      __syncthreads();
+      run_boxseq<T>(pt, A.seq[l], 4, LIN ? 1 : 0, q);
+      __syncthreads();
+    }
+}
+
+template <int DIM, bool LIN>
+__global__ __launch_bounds__(1024) void k_coarse_vcycle_tm(
+    const DPatch* __restrict__ pt, TmChainArgs A, int nsm, typename std::conditional<LIN, float, double>::type* __restrict__ q,
+    typename std::conditional<LIN, float, double>::type* __restrict__ f,
+    typename std::conditional<LIN, float, double>::type* __restrict__ qc, const double* __restrict__ qold,
+    const float* __restrict__ coef) {
+  using T = typename std::conditional<LIN, float, double>::type;
+  constexpr int NV = 4;
+  const ModelK m = A.m;
+  // descend
+  for (int l = 0; l < A.nlev - 1; l++) {
+    tc_level_relax<DIM, LIN>(pt, A, l, nsm, q, f, qold, coef);
+    const DPatch P = pt[A.pid[l]];
+    const DPatch Q = pt[A.pid[l + 1]];
+    const int ncc = P.cmx[0] * P.cmx[1] * (DIM == 3 ? P.cmx[2] : 1);
+    // RestrictSolution: qc = R q, parent q(mb) = qc
+    for (int g = threadIdx.x; g < ncc; g += blockDim.x) {
+      const int ic = g % P.cmx[0] + 1, row = g / P.cmx[0];
+      const int jc = row % P.cmx[1] + 1, kc = (DIM == 3) ? row / P.cmx[1] + 1 : 0;
+      const int ip = P.mb[0][0] + ic - 1, jp = P.mb[1][0] + jc - 1, kp = (DIM == 3) ? P.mb[2][0] + kc - 1 : 0;
+      for (int v = 0; v < NV; v++) {
+        const T r = restrict_cell<DIM, T>(q, P, ic, jc, kc, v * P.qvs);
+        qc[cidx(P, ic, jc, kc) + v * P.cvs] = r;
+        q[qidx(Q, ip, jp, kp) + v * Q.qvs] = r;
+      }
+    }
+    __syncthreads();
+    run_boxseq<T>(pt, A.seq[l + 1], NV, LIN ? 1 : 0, q);  // SetGhost(L-1, 0)
+    __syncthreads();
+    // GetCoarseGhostPoints
+    const int nbox = P.nc[0] * P.nc[1] * P.nc[2];
+    for (int g = threadIdx.x; g < nbox; g += blockDim.x) {
+      const int ic = g % P.nc[0], row = g / P.nc[0];
+      const int jc = row % P.nc[1], kc = (DIM == 3) ? row / P.nc[1] : 0;
+      const bool interior = ic >= 1 && ic <= P.cmx[0] && jc >= 1 && jc <= P.cmx[1] &&
+                            (DIM == 2 || (kc >= 1 && kc <= P.cmx[2]));
+      if (interior) continue;
+      const int ip = P.mb[0][0] - 1 + ic, jp = P.mb[1][0] - 1 + jc, kp = (DIM == 3) ? P.mb[2][0] - 1 + kc : 0;
+      for (int v = 0; v < NV; v++) qc[cidx(P, ic, jc, kc) + v * P.cvs] = q[qidx(Q, ip, jp, kp) + v * Q.qvs];
+    }
+    __syncthreads();
+    // the coarse load: parent f(mb) = R(f - L_h q) + L_2h(qc)
+    const int nch = (DIM == 3) ? 8 : 4;
+    for (int g = threadIdx.x; g < ncc; g += blockDim.x) {
+      const int ic = g % P.cmx[0] + 1, row = g / P.cmx[0];
+      const int jc = row % P.cmx[1] + 1, kc = (DIM == 3) ? row / P.cmx[1] + 1 : 0;
+      const int ip = P.mb[0][0] + ic - 1, jp = P.mb[1][0] + jc - 1, kp = (DIM == 3) ? P.mb[2][0] + kc - 1 : 0;
+      const int opar = qidx(Q, ip, jp, kp);
+      if constexpr (LIN) {
+        float s[NV] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int n = 0; n < nch; n++) {
+          const int i = 2 * ic - (n & 1), j = 2 * jc - ((n >> 1) & 1), k = (DIM == 3) ? 2 * kc - ((n >> 2) & 1) : 0;
+          const int o = qidx(P, i, j, k);
+          float a[NV];
+          lin_tm<DIM, false>(q, P, i, j, k, coef[o], coef[o + P.qvs], P.dx, m, a);
+          for (int v = 0; v < NV; v++) s[v] += f[o + v * P.qvs] - a[v];
+        }
+        const float w = (DIM == 3) ? 0.125f : 0.25f;
+        float a[NV];
+        lin_tm<DIM, true>(qc, P, ic, jc, kc, coef[opar], coef[opar + Q.qvs], Q.dx, m, a);
+        for (int v = 0; v < NV; v++) f[opar + v * Q.qvs] = w * s[v] + a[v];
+      } else {
+        double s[NV];
+        for (int n = 0; n < nch; n++) {
+          const int i = 2 * ic - (n & 1), j = 2 * jc - ((n >> 1) & 1), k = (DIM == 3) ? 2 * kc - ((n >> 2) & 1) : 0;
+          double o[kNvMax];
+          op_cell<DIM, BSAM_MODEL_TUMOUR>(q, qold, P, i, j, k, m, P.dx, o);
+          const int ofs = qidx(P, i, j, k);
+          for (int v = 0; v < NV; v++) {
+            const double a = f[ofs + v * P.qvs] - o[v];
+            s[v] = (n == 0) ? a : s[v] + a;
+          }
+        }
+        const int oc = cidx(P, ic, jc, kc), vs = P.cvs;
+        const double l2 = ulap_c<DIM>(qc, P, ic, jc, kc, vs), l1 = ulap_c<DIM>(qc, P, ic, jc, kc, 0);
+        const double l3 = ulap_c<DIM>(qc, P, ic, jc, kc, 2 * vs);
+        const double l4 = ulap_c<DIM>(qc, P, ic, jc, kc, 3 * vs);
+        double o[kNvMax];
+        op_tm_from<DIM>(qc[oc], qc[oc + vs], qc[oc + 2 * vs], l1, l2, l3, l4, qold[opar], m, Q.dx, o);
+        const double w = (DIM == 3) ? 0.125 : 0.25;
+        for (int v = 0; v < NV; v++) f[opar + v * Q.qvs] = w * s[v] + o[v];
+      }
+    }
+    __syncthreads();
+  }
+  tc_level_relax<DIM, LIN>(pt, A, A.nlev - 1, nsm, q, f, qold, coef);
+  // ascend
+  for (int l = A.nlev - 2; l >= 0; l--) {
+    const DPatch P = pt[A.pid[l]];
+    const DPatch Q = pt[A.pid[l + 1]];
+    // CorrectFine: qc = parent q(mb-1:mb+1) - qc, then q += P1(qc)
+    const int nbox = P.nc[0] * P.nc[1] * P.nc[2];
+    for (int g = threadIdx.x; g < nbox; g += blockDim.x) {
+      const int ic = g % P.nc[0], row = g / P.nc[0];
+      const int jc = row % P.nc[1], kc = (DIM == 3) ? row / P.nc[1] : 0;
+      const int ip = P.mb[0][0] - 1 + ic, jp = P.mb[1][0] - 1 + jc, kp = (DIM == 3) ? P.mb[2][0] - 1 + kc : 0;
+      for (int v = 0; v < NV; v++) {
+        const int o = cidx(P, ic, jc, kc) + v * P.cvs;
+        qc[o] = q[qidx(Q, ip, jp, kp) + v * Q.qvs] - qc[o];
+      }
+    }
+    __syncthreads();
+    const int n0 = P.cmx[0] + 1, n1 = P.cmx[1] + 1, n2 = (DIM == 3) ? P.cmx[2] + 1 : 1;
+    for (int g = threadIdx.x; g < n0 * n1 * n2; g += blockDim.x) {
+      const int ic = g % n0, row = g / n0;
+      const int jc = row % n1, kc = (DIM == 3) ? row / n1 : 0;
+      for (int v = 0; v < NV; v++) {
+        const int cv = v * P.cvs, fv = v * P.qvs;
+        if (DIM == 2) {
+          const T a00 = qc[cidx(P, ic, jc, 0) + cv], a01 = qc[cidx(P, ic + 1, jc, 0) + cv];
+          const T a10 = qc[cidx(P, ic, jc + 1, 0) + cv], a11 = qc[cidx(P, ic + 1, jc + 1, 0) + cv];
+          for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) {
+              const int i = 2 * ic + a, j = 2 * jc + b;
+              const T b0 = linw<T>(a00, a01, i), b1 = linw<T>(a10, a11, i);
+              const T pr = linw<T>(b0, b1, j) / (T)16;
+              const int o = qidx(P, i, j, 0) + fv;
+              q[o] = q[o] + pr;
+            }
+        } else {
+          T a[2][2][2];
+          for (int d = 0; d < 2; d++)
+            for (int b = 0; b < 2; b++)
+              for (int e = 0; e < 2; e++) a[d][b][e] = qc[cidx(P, ic + e, jc + b, kc + d) + cv];
+          for (int dk = 0; dk < 2; dk++)
+            for (int db = 0; db < 2; db++)
+              for (int da = 0; da < 2; da++) {
+                const int i = 2 * ic + da, j = 2 * jc + db, k = 2 * kc + dk;
+                T cc2[2];
+                for (int d = 0; d < 2; d++) {
+                  const T b0 = linw<T>(a[d][0][0], a[d][0][1], i);
+                  const T b1 = linw<T>(a[d][1][0], a[d][1][1], i);
+                  cc2[d] = linw<T>(b0, b1, j);
+                }
+                const T pr = linw<T>(cc2[0], cc2[1], k) / (T)64;
+                const int o = qidx(P, i, j, k) + fv;
+                q[o] = q[o] + pr;
+              }
+        }
+      }
+    }
+    __syncthreads();
+    run_boxseq<T>(pt, A.seq[l], NV, LIN ? 1 : 0, q);  // SetGhost(L, 0)
+    __syncthreads();
+    tc_level_relax<DIM, LIN>(pt, A, l, nsm, q, f, qold, coef);
   }
 }
 
@@ -3278,6 +3496,23 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
     }
     if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= 16)
       c->L(top).chain = top - c->minlevel + 1;
+  } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+    // the tumour model: levels of one patch each whose fill is a short
+    // sequence of box stages that one workgroup runs in order
+    int top = c->minlevel - 1;
+    for (int l = c->minlevel; l <= c->finest && l <= 0; l++) {
+      const Level& L = c->L(l);
+      if (L.patches.size() != 1 || L.wrap_pid >= 0 || L.d_rules || L.stages.empty() ||
+          (int)L.stages.size() > kMaxStages)
+        break;
+      int maxel = 0;
+      for (const Stage& st : L.stages) maxel = std::max(maxel, st.maxel);
+      const DPatch& P = c->hp[L.patches[0]];
+      if (maxel * c->p.nrvars > kSeqBoxElems || (long)P.mx[0] * P.mx[1] * P.mx[2] > 4096) break;
+      top = l;
+    }
+    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= kMaxChain)
+      c->L(top).chain = top - c->minlevel + 1;
   }
   if (maxblocks > c->partial_cap) {
     hipFree(c->d_partial);
@@ -4135,6 +4370,34 @@ extern "C" int32_t bsam_copy_q_to_qold(bsam_ctx* c, int32_t level) {
 // ---------------------------------------------------------------------------
 static int32_t launch_coarse_chain(bsam_ctx* c, int level, bool lin) {
   Level& L = c->L(level);
+  if (c->p.model == BSAM_MODEL_TUMOUR) {
+    TmChainArgs A;
+    A.nlev = L.chain;
+    A.m = c->mk();
+    for (int n = 0; n < L.chain; n++) {
+      const Level& Ln = c->L(level - n);
+      A.pid[n] = Ln.patches[0];
+      A.seq[n] = BoxSeq{};
+      A.seq[n].ns = (int)Ln.stages.size();
+      for (int st = 0; st < A.seq[n].ns; st++) {
+        A.seq[n].d[st] = Ln.stages[st].d;
+        A.seq[n].n[st] = Ln.stages[st].n;
+      }
+    }
+    const int nsm = c->p.nsmoothingpasses;
+    if (!lin) {
+      DISPATCH_DIM(c->p.ndims,
+                   (k_coarse_vcycle_tm<2, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, c->q, c->f, c->qc, c->qold, nullptr)),
+                   (k_coarse_vcycle_tm<3, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, c->q, c->f, c->qc, c->qold, nullptr)));
+    } else {
+      bsam::MixedState* m = c->mixed;
+      DISPATCH_DIM(c->p.ndims,
+                   (k_coarse_vcycle_tm<2, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, m->e, m->rhs, m->ec, c->qold, m->coef)),
+                   (k_coarse_vcycle_tm<3, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, m->e, m->rhs, m->ec, c->qold, m->coef)));
+    }
+    LAUNCH_CHECK();
+    return BSAM_OK;
+  }
   CoarseArgs A;
   A.nlev = L.chain;
   for (int n = 0; n < L.chain; n++) A.pid[n] = c->L(level - n).wrap_pid;
