@@ -97,6 +97,8 @@ def parse():
                     help="1: one launch per level (default); 0: one launch per patch")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="time the candidate launch shapes once per hierarchy before the warm-up (bsam_autotune)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--check", type=int, default=1,
                     help="measure the paths' max-norm error against FP64 converged to 1e-12")
@@ -185,6 +187,7 @@ def timed_path(wl, a, precision):
     step, roofline or None)."""
     from bsam_amd import abi
     S = make_state(wl, precision, a.graph, a.batching)
+    tune = S.autotune() if a.autotune else None
     print(f"[bench] {wl.name} {precision} graph={a.graph} batching={a.batching}: warmup {a.warmup}",
           file=sys.stderr, flush=True)
     first = True
@@ -219,7 +222,14 @@ def timed_path(wl, a, precision):
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=measured_traffic(wl.name, precision, kname),
                     avg_us=avg_ms * 1e3, launches=nl, bytes_per_launch=b)
-    return S, elapsed, sum(cycles) / a.steps, roof
+    return S, elapsed, sum(cycles) / a.steps, roof, tune
+
+
+def tune_choices(tune):
+    """The chosen launch shapes per level of an autotune report."""
+    if not tune:
+        return None
+    return {str(L["level"]): {k[:-7]: v for k, v in L.items() if k.endswith("_choice")} for L in tune["levels"]}
 
 
 def fields(wl, S):
@@ -268,8 +278,8 @@ def main():
     paths = ["mixed", "fp64"] if a.precision == "both" else [a.precision]
     res = {}
     for pr in paths:
-        S, elapsed, vc, roof = timed_path(wl, a, pr)
-        res[pr] = dict(elapsed=elapsed, vc=vc, roof=roof, q=fields(wl, S) if a.check else None)
+        S, elapsed, vc, roof, tune = timed_path(wl, a, pr)
+        res[pr] = dict(elapsed=elapsed, vc=vc, roof=roof, q=fields(wl, S) if a.check else None, tune=tune)
         S.close()
     head = paths[0]
     check = None
@@ -312,7 +322,8 @@ def main():
                    "eps": wl.eps, "dt": wl.dt, "tol": wl.tol, "cycle": "FAS V(2,2)",
                    "path": ("mixed FP32/FP64 defect correction" if head == "mixed" else "plain FP64"),
                    "hipgraph": bool(a.graph),
-                   "launches": "one per level" if a.batching else "one per patch"},
+                   "launches": "one per level" if a.batching else "one per patch",
+                   "autotuned": tune_choices(res[head]["tune"])},
         "vcycles_per_step": res[head]["vc"],
         "vcycles_per_s": res[head]["vc"] * a.steps / el,
         "roofline": res[head]["roof"],
@@ -321,10 +332,13 @@ def main():
     if len(paths) == 2:
         f = res["fp64"]
         out["plain_fp64"] = {"value": a.steps / f["elapsed"], "ms_per_step": 1000.0 * f["elapsed"] / a.steps,
-                             "vcycles_per_step": f["vc"], "roofline": f["roof"]}
+                             "vcycles_per_step": f["vc"], "roofline": f["roof"],
+                             "autotuned": tune_choices(f["tune"])}
         out["speedup_vs_fp64"] = f["elapsed"] / el
     if check is not None:
         out["max_norm_error"] = check
+    if res[head]["tune"]:
+        out["autotune_report"] = {pr: res[pr]["tune"] for pr in paths}
     print(json.dumps(out))
 
 

@@ -106,6 +106,8 @@ ENTRY_POINTS = {
     "bsam_last_error": ([], C.c_char_p),
     "bsam_kstat_enable": ([_P, _I], _I),
     "bsam_kstat_read": ([_P, _I, _LP, _DP, _DP], _I),
+    "bsam_autotune": ([_P], _I),
+    "bsam_tuning_report": ([_P, C.c_char_p, _I], _I),
     "bsam_selftest_ghost_maps": ([C.POINTER(Params), _I, C.POINTER(PatchDesc), _I, _I, C.c_uint64, _LP, _IP], _I),
 }
 

@@ -151,6 +151,19 @@ class Solver:
     def synchronize(self):
         self._chk(self.lib.bsam_synchronize(self.ctx))
 
+    # ---- launch-shape autotuning --------------------------------------------
+    def autotune(self):
+        """Time the candidate launch shapes on scratch copies and keep the
+        fastest (bit-identical fields either way); returns the report."""
+        self._chk(self.lib.bsam_autotune(self.ctx))
+        return self.tuning_report()
+
+    def tuning_report(self):
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self._chk(self.lib.bsam_tuning_report(self.ctx, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     # ---- instrumentation ----------------------------------------------------
     def kstat_enable(self, on=True):
         self._chk(self.lib.bsam_kstat_enable(self.ctx, 1 if on else 0))

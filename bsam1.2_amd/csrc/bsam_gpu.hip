@@ -106,7 +106,11 @@ struct LevelArgs {
   int total;  // end of the launch's range of the domain
   int gbase;  // start of the range: 0 for a whole level, prefix[p] for patch p alone
   int fixed;  // the patch of a one-patch launch, else -1
+  int xcd;    // 1: consecutive blocks on one XCD (xcd_block), 0: the hardware's order
 };
+
+// Logical block index of a level launch (the level's tuned block order).
+__device__ __forceinline__ int level_block(const LevelArgs& L) { return L.xcd ? xcd_block() : (int)blockIdx.x; }
 
 // Cell (i, j, k) of element `local` of a patch's kCell range, which runs
 // colour by colour, then row by row (j fastest, then k), then along the
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double 
                                                      double h, double* __restrict__ q,
                                                      const double* __restrict__ f) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_tm(LevelArgs L, int c, ModelK 
                                                      const double* __restrict__ qold,
                                                      const double* __restrict__ f) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -428,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_bg2d(LevelArgs L, int c, Model
                                                        const double* __restrict__ qold,
                                                        const double* __restrict__ f) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -556,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      TO* __restrict__ rf,
                                                      double* __restrict__ partial) {
   constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
-  const int lb = xcd_block();
+  const int lb = level_block(L);
   double sv[kNvMax] = {};
   int pi, local;
   if (level_elem(L, lb, pi, local)) {
@@ -908,7 +912,7 @@ template <int DIM, typename T, bool WRAP = false>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelArgs L, int nv, const T* __restrict__ src,
                                                      T* __restrict__ dst_parent, T* __restrict__ qc) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
+  if (!decode_coarse<DIM>(L, level_block(L), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const int ip = P.mb[0][0] + c.ic - 1, jp = P.mb[1][0] + c.jc - 1;
@@ -931,7 +935,7 @@ template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_coarse_ring(LevelArgs L, int nv, const T* __restrict__ q,
                                                         T* __restrict__ qc) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const DPatch Q = L.pt[P.parent];
   const int ic = local % P.nc[0];
@@ -958,7 +962,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
                                                     double* __restrict__ fpar) {
   constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
+  if (!decode_coarse<DIM>(L, level_block(L), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const double h = P.dx;
@@ -1009,7 +1013,7 @@ template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, const T* __restrict__ q,
                                                        T* __restrict__ qc) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const DPatch Q = L.pt[P.parent];
   const int ic = local % P.nc[0];
@@ -1040,7 +1044,7 @@ template <int DIM, typename T, bool WRAP = false>
 __global__ __launch_bounds__(kBlock) void k_prolong_dual(LevelArgs L, int nv, T* __restrict__ q,
                                                          const T* __restrict__ qc) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const int n0 = P.cmx[0] + 1, n1 = P.cmx[1] + 1;
   const int ic = local % n0;
@@ -1128,7 +1132,7 @@ template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(LevelArgs L, int nv, T* __restrict__ q,
                                                         const T* __restrict__ qc) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const int i = local % P.nq[0];
   const int row = local / P.nq[0];
@@ -1168,7 +1172,7 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, ModelK
   constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
   const double dt = m.dt;
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   int i, j, k;
   if (!cell_of<DIM>(P, local, i, j, k)) return;
@@ -1298,8 +1302,8 @@ __host__ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ cr, const uint16_t* __restrict__ cpost,
                                                       int nc, const GRule* __restrict__ ir, int ni, int nv,
-                                                      int homog, T* __restrict__ q) {
-  const int g = xcd_block() * kBlock + threadIdx.x;
+                                                      int homog, T* __restrict__ q, int xcd) {
+  const int g = (xcd ? xcd_block() : (int)blockIdx.x) * kBlock + threadIdx.x;
   if (g < nc) {
     const int4 r = cr[g];
     const int post = cpost ? cpost[g] : 0;
@@ -1454,7 +1458,7 @@ template <int DIM, bool TM = false>
 __global__ __launch_bounds__(kBlock) void k_coef(LevelArgs L, const double* __restrict__ q,
                                                  const double* __restrict__ qold, float* __restrict__ coef) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   int i, j, k;
   if (!cell_of<DIM>(P, local, i, j, k)) return;
@@ -1501,7 +1505,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
                                                       const float* __restrict__ coef,
                                                       double* __restrict__ q64 = nullptr) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -1558,7 +1562,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
                                                         const float* __restrict__ coef,
                                                         float* __restrict__ rpar) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
+  if (!decode_coarse<DIM>(L, level_block(L), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const float t1 = (float)(dt / (P.dx * P.dx)), t2 = (float)(eps2 / (P.dx * P.dx));
@@ -1662,7 +1666,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, Mod
                                                          const float* __restrict__ coef,
                                                          double* __restrict__ q64 = nullptr) {
   int pi, local;
-  if (!level_elem(L, xcd_block(), pi, local)) return;
+  if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
   const int cnt = (P.mx[0] + 1) >> 1;
   const int t = local % cnt;
@@ -1685,7 +1689,7 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin_tm(LevelArgs L, ModelK m
                                                            const float* __restrict__ coef,
                                                            float* __restrict__ rpar) {
   CoarseCell c;
-  if (!decode_coarse<DIM>(L, xcd_block(), c)) return;
+  if (!decode_coarse<DIM>(L, level_block(L), c)) return;
   const DPatch P = L.pt[c.pid];
   const DPatch Q = L.pt[P.parent];
   const int nch = (DIM == 3) ? 8 : 4;
@@ -2263,6 +2267,11 @@ struct Level {
   bool maps_ok = false;
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
+  // launch shapes chosen by bsam_autotune: block order of the level kernels
+  // (xcd) and of the ghost gather (gxcd), planes per workgroup of the
+  // marching sweeps (FP64, FP32) and residuals (0: the default rule)
+  int xcd = 1, gxcd = 1;
+  int kc_sweep[2] = {0, 0}, kc_resid = 0;
   double h = 0.0;
   long qspan_off = 0, qspan_len = 0;  // contiguous q-pool span of the level
 };
@@ -2304,6 +2313,7 @@ struct bsam_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool capturing = false;
+  std::string tune_report;  // JSON of the last bsam_autotune
   // instrumentation
   KStat ks[BSAM_KSTAT_COUNT];
   // mixed precision state
@@ -2337,6 +2347,7 @@ LevelArgs largs(bsam_ctx* c, Level& L, Dom d) {
   a.total = L.total[d];
   a.gbase = 0;
   a.fixed = -1;
+  a.xcd = L.xcd;
   return a;
 }
 
@@ -3884,13 +3895,14 @@ static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, i
     const DevMap& M = L.gmap[mi];
     // rules reading ghosts from before the call first, then the rest
     if (M.nc1 + M.ni1 > 0) {
-      k_ghost_map<T><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q);
+      k_ghost_map<T><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q,
+                                                                  L.gxcd);
       LAUNCH_CHECK();
     }
     const int nc = M.nc - M.nc1, ni = M.ni - M.ni1;
     if (nc + ni > 0) {
       k_ghost_map<T><<<nblocks(nc + ni), kBlock, 0, c->s>>>(M.cr + M.nc1, M.cpost ? M.cpost + M.nc1 : nullptr, nc,
-                                                             M.ir + M.ni1, ni, nv, homog, q);
+                                                             M.ir + M.ni1, ni, nv, homog, q, L.gxcd);
       LAUNCH_CHECK();
     }
     return BSAM_OK;
@@ -3970,11 +3982,15 @@ static int sweep_kc(bsam_ctx* c, const Level& L, int elem_bytes, int* nblk) {
   const int nx2 = P.mx[0] / 2;
   if (nx2 < 64 || P.mx[1] < kSweepTY || (P.mx[0] & 1)) return 0;
   const int ntile = ((nx2 + 63) / 64) * ((P.mx[1] + kSweepTY - 1) / kSweepTY);
-  const int target = (elem_bytes == 8) ? 1024 : 512;
-  int nkc = std::max(1, (target + ntile - 1) / ntile);
-  int kc = std::max(4, (P.mx[2] + nkc - 1) / nkc);
-  nkc = (P.mx[2] + kc - 1) / kc;
-  *nblk = ntile * nkc;
+  // the tuned planes per workgroup, else about 1024 (FP64) / 512 (FP32)
+  // workgroups of at least 4 planes
+  int kc = L.kc_sweep[elem_bytes == 8 ? 0 : 1];
+  if (kc <= 0) {
+    const int target = (elem_bytes == 8) ? 1024 : 512;
+    const int nkc = std::max(1, (target + ntile - 1) / ntile);
+    kc = std::max(4, (P.mx[2] + nkc - 1) / nkc);
+  }
+  *nblk = ntile * ((P.mx[2] + kc - 1) / kc);
   return kc;
 }
 
@@ -3986,10 +4002,12 @@ static int march_kc(bsam_ctx* c, const Level& L, int* nblk) {
   const int nx2 = P.mx[0] / 2;
   if (nx2 < 64 || P.mx[1] < 8 || (P.mx[0] & 1)) return 0;
   const int ntile = ((nx2 + 63) / 64) * ((P.mx[1] + 7) / 8);
-  int nkc = std::max(1, (768 + ntile - 1) / ntile);
-  int kc = std::max(4, (P.mx[2] + nkc - 1) / nkc);
-  nkc = (P.mx[2] + kc - 1) / kc;
-  *nblk = ntile * nkc;
+  int kc = L.kc_resid;
+  if (kc <= 0) {
+    const int nkc = std::max(1, (768 + ntile - 1) / ntile);
+    kc = std::max(4, (P.mx[2] + nkc - 1) / nkc);
+  }
+  *nblk = ntile * ((P.mx[2] + kc - 1) / kc);
   return kc;
 }
 
@@ -4926,3 +4944,174 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
   return it > c->p.maxvcycles ? BSAM_NOT_CONVERGED : BSAM_OK;
 }
 }  // namespace bsam
+
+// ---------------------------------------------------------------------------
+// Autotuning of launch shapes
+// ---------------------------------------------------------------------------
+namespace {
+// Mean duration (us) of `reps` calls of fn on the context's stream, after
+// one untimed call.
+template <typename F>
+static float time_launches(bsam_ctx* c, int reps, F&& fn) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return -1.0f;
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return -1.0f;
+  }
+  float ms = -1.0f;
+  if (fn() >= 0 && hipEventRecord(a, c->s) == hipSuccess) {
+    bool ok = true;
+    for (int r = 0; r < reps && ok; r++) ok = fn() >= 0;
+    if (ok && hipEventRecord(b, c->s) == hipSuccess && hipEventSynchronize(b) == hipSuccess)
+      hipEventElapsedTime(&ms, a, b);
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ms < 0.0f ? -1.0f : 1e3f * ms / reps;
+}
+
+// Best of the candidate values: sets *knob to each, times fn, keeps the
+// fastest; appends "name":[[value, us], ...] to the report.
+template <typename F>
+static int32_t tune_knob(bsam_ctx* c, const char* name, int* knob, const std::vector<int>& cand, F&& fn,
+                         std::string& rep) {
+  int best = *knob;
+  float tbest = 0.0f;
+  char buf[64];
+  rep += std::string(",\"") + name + "\":[";
+  for (size_t n = 0; n < cand.size(); n++) {
+    *knob = cand[n];
+    const float t = time_launches(c, 3, fn);
+    if (t < 0.0f) return set_err(BSAM_EHIP, "autotune: %s candidate %d failed", name, cand[n]);
+    snprintf(buf, sizeof buf, "%s[%d,%.1f]", n ? "," : "", cand[n], t);
+    rep += buf;
+    if (n == 0 || t < tbest) {
+      tbest = t;
+      best = cand[n];
+    }
+  }
+  *knob = best;
+  snprintf(buf, sizeof buf, "],\"%s_choice\":%d", name, best);
+  rep += buf;
+  return BSAM_OK;
+}
+}  // namespace
+
+// Times the candidate launch shapes of every level with at least 32768
+// cells on scratch copies of the fields and keeps the fastest (all shapes
+// give bit-identical fields): the block order of the level kernels (one
+// XCD per run of consecutive blocks, or the hardware's order), the block
+// order of the ghost gather, and the planes per workgroup of the marching
+// sweeps and residuals on one large patch.  The report lists each level's
+// patch-size histogram with the timings.
+extern "C" int32_t bsam_autotune(bsam_ctx* c) {
+  if (!c || !c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
+  if (c->capturing) return set_err(BSAM_ESTATE, "bsam_autotune inside a graph capture");
+  std::string rep = "{\"levels\":[";
+  bool firstlev = true;
+  if (c->p.batching) {
+    double* sq = nullptr;
+    float* se = nullptr;
+    if (hipMalloc(&sq, c->nq * sizeof(double)) != hipSuccess) return set_err(BSAM_ENOMEM, "autotune scratch");
+    HIPCHK(hipMemcpyAsync(sq, c->q, c->nq * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    if (c->mixed) {
+      if (hipMalloc(&se, c->nq * sizeof(float)) != hipSuccess) {
+        hipFree(sq);
+        return set_err(BSAM_ENOMEM, "autotune scratch");
+      }
+      HIPCHK(hipMemcpyAsync(se, c->mixed->e, c->nq * sizeof(float), hipMemcpyDeviceToDevice, c->s));
+    }
+    double* q0 = c->q;
+    float* e0 = c->mixed ? c->mixed->e : nullptr;
+    int32_t rc = BSAM_OK;
+    for (int level = c->minlevel; level <= c->finest && rc == BSAM_OK; level++) {
+      Level& L = c->L(level);
+      long cells = 0;
+      std::vector<int> hist(8, 0);
+      for (int pid : L.patches) {
+        const DPatch& P = c->hp[pid];
+        cells += (long)P.mx[0] * P.mx[1] * P.mx[2];
+        const int ext = std::max(P.mx[0], std::max(P.mx[1], c->p.ndims == 3 ? P.mx[2] : 1));
+        int b = 0;
+        while (b < 7 && (8 << b) < ext) b++;
+        hist[b]++;
+      }
+      if (cells < 32768) continue;
+      char buf[160];
+      snprintf(buf, sizeof buf, "%s{\"level\":%d,\"patches\":%d,\"cells\":%ld,\"extent_hist\":{", firstlev ? "" : ",",
+               level, (int)L.patches.size(), cells);
+      rep += buf;
+      firstlev = false;
+      bool fh = true;
+      for (int b = 0; b < 8; b++)
+        if (hist[b]) {
+          snprintf(buf, sizeof buf, "%s\"<=%d\":%d", fh ? "" : ",", 8 << b, hist[b]);
+          rep += buf;
+          fh = false;
+        }
+      rep += "}";
+      c->q = sq;
+      int nb = 0;
+      const bool march = c->p.model == BSAM_MODEL_CH && sweep_kc(c, L, 8, &nb) > 0;
+      const bool wrap = L.wrap_pid >= 0;
+      // the level's colour sweep (with the periodic images it writes on a
+      // fully periodic patch)
+      if (!march)
+        rc = tune_knob(c, "xcd", &L.xcd, {1, 0}, [&]() {
+          return (wrap && c->p.model == BSAM_MODEL_CH) ? relax_and_ghost(c, level, 1, false)
+                                                         : bsam_relax_level(c, level, 1);
+        }, rep);
+      if (rc == BSAM_OK && L.maps_ok)
+        rc = tune_knob(c, "gather_xcd", &L.gxcd, {1, 0},
+                       [&]() { return set_ghost_t<double>(c, level, 0, sq, 0); }, rep);
+      if (rc == BSAM_OK && march) {
+        std::vector<int> cand;
+        for (int k : {4, 8, 16, 32})
+          if (k <= c->hp[L.patches[0]].mx[2]) cand.push_back(k);
+        rc = tune_knob(c, "sweep_planes_fp64", &L.kc_sweep[0], cand,
+                       [&]() { return relax_and_ghost(c, level, 1, false); }, rep);
+        if (rc == BSAM_OK && c->mixed) {
+          c->mixed->e = se;
+          rc = tune_knob(c, "sweep_planes_fp32", &L.kc_sweep[1], cand,
+                         [&]() { return mixed_relax(c, level, 1); }, rep);
+          c->mixed->e = e0;
+        }
+      }
+      if (rc == BSAM_OK && c->p.model == BSAM_MODEL_CH && march_kc(c, L, &nb) > 0) {
+        std::vector<int> cand;
+        for (int k : {4, 8, 16, 32})
+          if (k <= c->hp[L.patches[0]].mx[2]) cand.push_back(k);
+        rc = tune_knob(c, "resid_planes", &L.kc_resid, cand,
+                       [&]() { return resid_march<double>(c, level, c->rf, nullptr); }, rep);
+      }
+      c->q = q0;
+      rep += "}";
+    }
+    c->q = q0;
+    if (c->mixed) c->mixed->e = e0;
+    hipStreamSynchronize(c->s);
+    hipFree(sq);
+    hipFree(se);
+    RETCHK(rc);
+  }
+  rep += "]}";
+  c->tune_report = rep;
+  // launches captured with the old shapes are rebuilt at the next call
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_tuning_report(const bsam_ctx* c, char* buf, int32_t len) {
+  if (!c || !buf || len <= 0) return set_err(BSAM_EINVAL, "null argument");
+  const std::string& r = c->tune_report.empty() ? std::string("{}") : c->tune_report;
+  snprintf(buf, (size_t)len, "%s", r.c_str());
+  return (int32_t)r.size() < len ? BSAM_OK : set_err(BSAM_EINVAL, "report needs %d bytes", (int)r.size() + 1);
+}

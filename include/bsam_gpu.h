@@ -216,6 +216,24 @@ int32_t bsam_kstat_enable(bsam_ctx* ctx, int32_t on);
 int32_t bsam_kstat_read(bsam_ctx* ctx, int32_t which, int64_t* launches,
                         double* total_ms, double* bytes_per_launch);
 
+/* ---- launch-shape autotuning (no counterpart in the reference, whose
+ * per-patch loops have no launch shape) --------------------------------
+ * Times the candidate launch shapes of every level with at least 32768
+ * cells on scratch copies of the fields and keeps the fastest; every shape
+ * gives bit-identical fields, so tuning changes speed only.  Knobs: the
+ * block order of a level's kernels (consecutive blocks on one XCD, or the
+ * hardware's round-robin order), the block order of its ghost gather, and
+ * on one large patch the planes per workgroup of the marching sweeps (FP64,
+ * FP32) and residuals.  Call after bsam_set_hierarchy (and bsam_upload),
+ * outside graph capture; a captured iteration is re-captured at the next
+ * call.  Synchronous.  The fields are not changed. */
+int32_t bsam_autotune(bsam_ctx* ctx);
+/* JSON report of the last bsam_autotune: per tuned level the patch count,
+ * cells, a histogram of the largest patch extent, each knob's candidates
+ * with their mean launch time in microseconds, and the choice ("{}" before
+ * any tuning).  BSAM_EINVAL if `len` bytes do not hold it. */
+int32_t bsam_tuning_report(const bsam_ctx* ctx, char* buf, int32_t len);
+
 /* ---- diagnostics (no counterpart in the reference) ----------------------
  * Host-only check of the flattened ghost-fill tables that per-level
  * launches use in place of the staged SetGhost (boundary.f90:67-97); needs

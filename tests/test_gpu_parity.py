@@ -209,6 +209,44 @@ def test_large_patch_sweep_vs_oracle(shape, mthbc, gpu_solver_cls):
     _hist_close(hg, ho)
 
 
+@pytest.mark.parametrize("shape,mthbc", [((128, 128, 128), (2, 2, 2, 2, 2, 2)),
+                                          ((96, 64, 48), (2, 2, 10, 10, 2, 2))])
+def test_autotuned_shapes_vs_oracle(shape, mthbc, gpu_solver_cls):
+    """bsam_autotune changes launch shapes only: after tuning (block orders
+    of the level kernels and ghost gather, planes per workgroup of the
+    marching sweeps and residuals) a time step is still bit-identical to the
+    oracle, in FP64; the report lists the tuned levels and their choices."""
+    nd = 3
+    print(f"[gpu] autotune {shape} mthbc={mthbc}", flush=True)
+    minlevel = -(int(np.log2(min(shape))) - 2)
+    hier = hierarchy.uniform(nd, shape, minlevel, list(mthbc))
+    root = len(hier) - 1
+    phi = configs.init_phi(hier[root], nd, 1, 1007, 0.1, 0.05)
+    q0 = np.zeros(hierarchy.field_shape(hier[root], nd, 2, "q"))
+    q0[0, 1:-1, 1:-1, 1:-1] = phi
+    res = []
+    for S in (gpu_solver_cls(configs.params(nd, 5e-5, 0.015, 1e-8), hier),
+              oracle.Oracle(configs.params(nd, 5e-5, 0.015, 1e-8), hier)):
+        S.put(root, "q", q0)
+        S.set_ghost(0, 0)
+        S.copy_q_to_qold(0)
+        S.fill_down(abi.SOLUTIONFIELD)
+        for lv in range(minlevel, 1):
+            S.copy_q_to_qold(lv)
+        if hasattr(S, "autotune"):
+            rep = S.autotune()
+            assert rep["levels"] and rep["levels"][0]["cells"] >= 32768
+            assert all(any(k.endswith("_choice") for k in L) for L in rep["levels"]), rep
+            # the fields are untouched by the tuning
+            assert np.array_equal(S.get(root, "q")[(slice(None),) + (slice(1, -1),) * nd],
+                                  q0[(slice(None),) + (slice(1, -1),) * nd])
+        rc, ncyc, r, h = S.multigrid_iterations()
+        res.append((h, S.get(root, "q")))
+    (hg, qg), (ho, qo) = res
+    assert np.array_equal(qg, qo), float(np.max(np.abs(qg - qo)))
+    _hist_close(hg, ho)
+
+
 @pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16), (48, 24, 20), (96, 32), (40, 24)])
 def test_coarse_chain_vs_oracle(shape, gpu_solver_cls):
     """Roots whose coarse levels run in the one-workgroup V-cycle chain:
