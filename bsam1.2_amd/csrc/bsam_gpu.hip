@@ -83,6 +83,19 @@ __device__ __forceinline__ int xcd_block() {
   return (x < r) ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
 
+// Logical block index in one of the block orders the autotuner picks from:
+// order 0 is xcd_block (one contiguous range per XCD); order c >= 1 gives
+// each XCD runs of c consecutive logical blocks in turn (1 is the hardware's
+// round-robin order), over whole groups of 8c blocks, the rest in order.
+__device__ __forceinline__ int ordered_block(int order) {
+  if (order == 0) return xcd_block();
+  const int b = blockIdx.x, g8 = 8 * order;
+  const int full = ((int)gridDim.x / g8) * g8;
+  if (b >= full) return b;
+  const int grp = b / g8, r = b - grp * g8;
+  return grp * g8 + (r & 7) * order + (r >> 3);
+}
+
 __device__ __forceinline__ int find_patch(const int* __restrict__ prefix, int n, int g) {
   int lo = 0, hi = n;
   while (hi - lo > 1) {
@@ -106,11 +119,11 @@ struct LevelArgs {
   int total;  // end of the launch's range of the domain
   int gbase;  // start of the range: 0 for a whole level, prefix[p] for patch p alone
   int fixed;  // the patch of a one-patch launch, else -1
-  int xcd;    // 1: consecutive blocks on one XCD (xcd_block), 0: the hardware's order
+  int order;  // block order (ordered_block)
 };
 
 // Logical block index of a level launch (the level's tuned block order).
-__device__ __forceinline__ int level_block(const LevelArgs& L) { return L.xcd ? xcd_block() : (int)blockIdx.x; }
+__device__ __forceinline__ int level_block(const LevelArgs& L) { return ordered_block(L.order); }
 
 // Cell (i, j, k) of element `local` of a patch's kCell range, which runs
 // colour by colour, then row by row (j fastest, then k), then along the
@@ -1302,8 +1315,8 @@ __host__ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ cr, const uint16_t* __restrict__ cpost,
                                                       int nc, const GRule* __restrict__ ir, int ni, int nv,
-                                                      int homog, T* __restrict__ q, int xcd) {
-  const int g = (xcd ? xcd_block() : (int)blockIdx.x) * kBlock + threadIdx.x;
+                                                      int homog, T* __restrict__ q, int order) {
+  const int g = ordered_block(order) * kBlock + threadIdx.x;
   if (g < nc) {
     const int4 r = cr[g];
     const int post = cpost ? cpost[g] : 0;
@@ -2267,10 +2280,10 @@ struct Level {
   bool maps_ok = false;
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
-  // launch shapes chosen by bsam_autotune: block order of the level kernels
-  // (xcd) and of the ghost gather (gxcd), planes per workgroup of the
+  // launch shapes chosen by bsam_autotune: block order (ordered_block) of
+  // the level kernels and of the ghost gather, planes per workgroup of the
   // marching sweeps (FP64, FP32) and residuals (0: the default rule)
-  int xcd = 1, gxcd = 1;
+  int order = 0, gorder[2] = {0, 0};  // gather: FP64, FP32
   int kc_sweep[2] = {0, 0}, kc_resid = 0;
   double h = 0.0;
   long qspan_off = 0, qspan_len = 0;  // contiguous q-pool span of the level
@@ -2347,7 +2360,7 @@ LevelArgs largs(bsam_ctx* c, Level& L, Dom d) {
   a.total = L.total[d];
   a.gbase = 0;
   a.fixed = -1;
-  a.xcd = L.xcd;
+  a.order = L.order;
   return a;
 }
 
@@ -3896,13 +3909,14 @@ static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, i
     // rules reading ghosts from before the call first, then the rest
     if (M.nc1 + M.ni1 > 0) {
       k_ghost_map<T><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q,
-                                                                  L.gxcd);
+                                                                  L.gorder[sizeof(T) == 8 ? 0 : 1]);
       LAUNCH_CHECK();
     }
     const int nc = M.nc - M.nc1, ni = M.ni - M.ni1;
     if (nc + ni > 0) {
       k_ghost_map<T><<<nblocks(nc + ni), kBlock, 0, c->s>>>(M.cr + M.nc1, M.cpost ? M.cpost + M.nc1 : nullptr, nc,
-                                                             M.ir + M.ni1, ni, nv, homog, q, L.gxcd);
+                                                             M.ir + M.ni1, ni, nv, homog, q,
+                                                             L.gorder[sizeof(T) == 8 ? 0 : 1]);
       LAUNCH_CHECK();
     }
     return BSAM_OK;
@@ -5000,10 +5014,10 @@ static int32_t tune_knob(bsam_ctx* c, const char* name, int* knob, const std::ve
 
 // Times the candidate launch shapes of every level with at least 32768
 // cells on scratch copies of the fields and keeps the fastest (all shapes
-// give bit-identical fields): the block order of the level kernels (one
-// XCD per run of consecutive blocks, or the hardware's order), the block
-// order of the ghost gather, and the planes per workgroup of the marching
-// sweeps and residuals on one large patch.  The report lists each level's
+// give bit-identical fields): the block order of the level kernels and of
+// the ghost gather (ordered_block: one contiguous range per XCD, or runs of
+// 1 to 256 blocks per XCD in turn), and the planes per workgroup of the
+// marching sweeps and residuals on one large patch.  The report lists each level's
 // patch-size histogram with the timings.
 extern "C" int32_t bsam_autotune(bsam_ctx* c) {
   if (!c || !c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
@@ -5058,13 +5072,16 @@ extern "C" int32_t bsam_autotune(bsam_ctx* c) {
       // the level's colour sweep (with the periodic images it writes on a
       // fully periodic patch)
       if (!march)
-        rc = tune_knob(c, "xcd", &L.xcd, {1, 0}, [&]() {
+        rc = tune_knob(c, "order", &L.order, {0, 1, 4, 16, 64}, [&]() {
           return (wrap && c->p.model == BSAM_MODEL_CH) ? relax_and_ghost(c, level, 1, false)
                                                          : bsam_relax_level(c, level, 1);
         }, rep);
       if (rc == BSAM_OK && L.maps_ok)
-        rc = tune_knob(c, "gather_xcd", &L.gxcd, {1, 0},
+        rc = tune_knob(c, "gather_order_fp64", &L.gorder[0], {0, 1, 16, 64, 256},
                        [&]() { return set_ghost_t<double>(c, level, 0, sq, 0); }, rep);
+      if (rc == BSAM_OK && L.maps_ok && se)
+        rc = tune_knob(c, "gather_order_fp32", &L.gorder[1], {0, 1, 16, 64, 256},
+                       [&]() { return set_ghost_t<float>(c, level, 0, se, 1); }, rep);
       if (rc == BSAM_OK && march) {
         std::vector<int> cand;
         for (int k : {4, 8, 16, 32})

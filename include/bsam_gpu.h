@@ -221,10 +221,12 @@ int32_t bsam_kstat_read(bsam_ctx* ctx, int32_t which, int64_t* launches,
  * Times the candidate launch shapes of every level with at least 32768
  * cells on scratch copies of the fields and keeps the fastest; every shape
  * gives bit-identical fields, so tuning changes speed only.  Knobs: the
- * block order of a level's kernels (consecutive blocks on one XCD, or the
- * hardware's round-robin order), the block order of its ghost gather, and
- * on one large patch the planes per workgroup of the marching sweeps (FP64,
- * FP32) and residuals.  Call after bsam_set_hierarchy (and bsam_upload),
+ * block order of a level's kernels and of its ghost gather in FP64 and
+ * FP32 (one contiguous range of blocks per XCD, or runs of 1 to 256
+ * consecutive blocks per XCD in turn; 1 is the hardware's round-robin
+ * order), and on one large
+ * patch the planes per workgroup of the marching sweeps (FP64, FP32) and
+ * residuals.  Call after bsam_set_hierarchy (and bsam_upload),
  * outside graph capture; a captured iteration is re-captured at the next
  * call.  Synchronous.  The fields are not changed. */
 int32_t bsam_autotune(bsam_ctx* ctx);
