@@ -61,10 +61,10 @@ def _cpu_model():
 
 # The finest-level colour sweep is the dominant kernel (8 launches per
 # V-cycle); its name in kernel traces per workload kind and precision.
-DOMINANT = {("uniform", "fp64"): "k_relax3d_march<double, false, true, 1>",
-            ("uniform", "mixed"): "k_relax3d_march<float, true, true, 1>",
+DOMINANT = {("uniform", "fp64"): "k_relax3d_march<double, false, true, 1, 0>",
+            ("uniform", "mixed"): "k_relax3d_march<float, true, true, 1, 0>",
             ("amr", "fp64"): "k_relax_ch<3, false>",
-            ("amr", "mixed"): "k_relax_lin<3, false>",
+            ("amr", "mixed"): "k_relax_lin<3, false, 0>",
             ("tumour", "fp64"): "k_relax_tm<3>",
             ("tumour", "mixed"): "k_relax_lin_tm<3, 0>"}
 

@@ -100,6 +100,8 @@ rc = bsam_set_ghost(ctx, 0, 0)
 CALL check('bsam_set_ghost')
 rc = bsam_copy_q_to_qold(ctx, 0)
 CALL check('bsam_copy_q_to_qold')
+rc = bsam_autotune(ctx)
+CALL check('bsam_autotune')
 !
 DO s = 1, steps
   rc = bsam_time_step(ctx, MERGE(1,0,s==1), ncyc, res)

@@ -66,7 +66,7 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_fill_down, bsam_get_source, bsam_update_source, bsam_update_aux, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
          bsam_multigrid_iterations, bsam_time_step, bsam_synchronize, &
-         bsam_last_error
+         bsam_autotune, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
 !
 INTERFACE
@@ -203,6 +203,10 @@ INTERFACE
     REAL(C_DOUBLE), INTENT(OUT):: residual
   END FUNCTION
   INTEGER(C_INT32_T) FUNCTION bsam_synchronize(ctx) BIND(C, NAME='bsam_synchronize')
+    IMPORT:: C_INT32_T, C_PTR
+    TYPE(C_PTR), VALUE:: ctx
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_autotune(ctx) BIND(C, NAME='bsam_autotune')
     IMPORT:: C_INT32_T, C_PTR
     TYPE(C_PTR), VALUE:: ctx
   END FUNCTION
