@@ -564,7 +564,7 @@ __device__ __forceinline__ void op_cell(const double* __restrict__ q, const doub
 // Residual (afasroutines.f90:817-857): rf = f - L(q) over the interior.
 // Also accumulates sum(rf^2) per variable into per-block partials when
 // `partial` is non-null (ErrorAFAS / L2Error / SquareSum, :113-233), and
-// writes rf when `rf` is non-null.
+// writes rf when `rf` is non-null.  Domain: kCell.
 template <int DIM, int MODEL, typename TO>
 __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      const double* __restrict__ q,
@@ -580,15 +580,15 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
     const DPatch P = L.pt[L.patches[pi]];
     int i, j, k;
     if (cell_of<DIM>(P, local, i, j, k)) {
-    double o[kNvMax];
-    op_cell<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o);
-    const int c0 = qidx(P, i, j, k);
+      double o[kNvMax];
+      op_cell<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o);
+      const int c0 = qidx(P, i, j, k);
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      const double r = f[c0 + v * P.qvs] - o[v];
-      if (rf) rf[c0 + v * P.qvs] = (TO)r;
-      sv[v] += r * r;
-    }
+      for (int v = 0; v < NV; v++) {
+        const double r = f[c0 + v * P.qvs] - o[v];
+        if (rf) rf[c0 + v * P.qvs] = (TO)r;
+        sv[v] += r * r;
+      }
     }
   }
   if (!partial) return;
@@ -4870,19 +4870,13 @@ int32_t mixed_residuals(bsam_ctx* c) {
 
 // One outer iteration: correction V-cycle, q += e, FillDown, ghost refresh
 // of the finest level, new residual and norm.
-// The correction e starts every cycle at zero: levels >= 0 are cleared by
-// the q += e update of the previous cycle (or the set-up), the levels below
-// the root here.
+// The correction e starts every cycle at zero on the levels >= 0: they are
+// cleared by the q += e update of the previous cycle (or the set-up).  A
+// level below the root lies wholly under the next finer level, so the
+// restriction writes every interior cell of it and the SetGhost after it
+// every ghost before anything reads them: its old values need no clearing.
 int32_t mixed_iteration(bsam_ctx* c) {
   bsam::MixedState* m = c->mixed;
-  for (int level = c->minlevel; level < 0 && level <= c->finest; level++) {
-    for (int pid : c->L(level).patches) {
-      const DPatch& P = c->hp[pid];
-      const long n = (long)P.qvs * c->p.nrvars;
-      k_zero_f<<<nblocks(n), kBlock, 0, c->s>>>(m->e + P.qoff, n);
-      LAUNCH_CHECK();
-    }
-  }
   RETCHK(mixed_vcycle(c, c->finest));
   // with the fused finest sweeps, q += e has been done by the last sweep
   // (on a refined mesh for the finest level only)
