@@ -186,8 +186,17 @@ def timed_path(wl, a, precision):
     instrumented steps.  Returns (solver, elapsed s, V-cycles per timed
     step, roofline or None)."""
     from bsam_amd import abi
+    t_set = time.perf_counter()
     S = make_state(wl, precision, a.graph, a.batching)
+    t_tune = time.perf_counter()
     tune = S.autotune() if a.autotune else None
+    t_end = time.perf_counter()
+    if tune is not None:
+        # host set-up of the hierarchy (bsam_set_hierarchy: work lists,
+        # sibling discovery, flattened ghost tables) with the upload of the
+        # initial fields, and the autotuning, in seconds
+        tune["setup_s"] = round(t_tune - t_set, 3)
+        tune["autotune_s"] = round(t_end - t_tune, 3)
     print(f"[bench] {wl.name} {precision} graph={a.graph} batching={a.batching}: warmup {a.warmup}",
           file=sys.stderr, flush=True)
     first = True

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -2788,6 +2789,12 @@ static void interp_rules(const DPatch& P, const DPatch& Q, int ndims, std::vecto
   }
 }
 
+// Host threads for the hierarchy set-up (at most 16).
+static size_t host_threads() {
+  const unsigned n = std::thread::hardware_concurrency();
+  return n ? std::min(16u, n) : 4u;
+}
+
 static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const size_t np = L.patches.size();
   for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(np + 1, 0);
@@ -2795,12 +2802,23 @@ static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   std::vector<GRule>& all = G.rules;
   std::vector<std::vector<GRule>> per[3];
   for (int sg = 0; sg < 3; sg++) per[sg].resize(np);
-  for (size_t a = 0; a < np; a++) {
-    const DPatch& P = c->hp[L.patches[a]];
-    std::vector<GRule> seg[3];
-    interp_rules(P, c->hp[P.parent], c->p.ndims, seg);
-    for (int sg = 0; sg < 3; sg++) per[sg][a].swap(seg[sg]);
-  }
+  // patches are independent: interleaved over the host's threads
+  const int nth = (int)std::max<size_t>(1, std::min<size_t>(np, host_threads()));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; t++)
+    th.emplace_back([&, t]() {
+      for (size_t a = t; a < np; a += nth) {
+        const DPatch& P = c->hp[L.patches[a]];
+        std::vector<GRule> seg[3];
+        interp_rules(P, c->hp[P.parent], c->p.ndims, seg);
+        for (int sg = 0; sg < 3; sg++) per[sg][a].swap(seg[sg]);
+      }
+    });
+  for (auto& x : th) x.join();
+  size_t total = 0;
+  for (int sg = 0; sg < 3; sg++)
+    for (size_t a = 0; a < np; a++) total += per[sg][a].size();
+  all.reserve(all.size() + total);
   for (int sg = 0; sg < 3; sg++) {
     for (size_t a = 0; a < np; a++) {
       G.roff[sg][a] = (int)all.size();
@@ -2810,6 +2828,82 @@ static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   }
 }
 
+// For each patch a of a level, the patches b > a whose ghost-extended box
+// can meet a's, directly or through one of the periodic offsets, in
+// ascending order: a uniform grid of bins over the level's index space,
+// each listing the patches whose box it touches, replaces the test of
+// every pair (ApplyOnLevelPairs, treeops.f90:555-590), so the work grows
+// with the number of neighbours rather than with the square of the patch
+// count.  Pairs whose boxes do not meet produce no transfer boxes, so the
+// boxes come out as from the all-pairs loop, in the same order.
+static std::vector<std::vector<int>> neighbour_candidates(const bsam_ctx* c, const std::vector<int>& ps,
+                                                          const std::vector<std::array<int, 3>>& offs) {
+  const int nd = c->p.ndims;
+  const size_t np = ps.size();
+  std::vector<std::vector<int>> cand(np);
+  if (np < 2) return cand;
+  // bins of the largest ghost-extended patch extent, over the bounding box
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, ext = 1;
+  for (size_t a = 0; a < np; a++) {
+    const bsam_patch_desc& d = c->desc[ps[a]];
+    for (int n = 0; n < nd; n++) {
+      lo[n] = a ? std::min(lo[n], d.mglobal[n][0] - 1) : d.mglobal[n][0] - 1;
+      hi[n] = a ? std::max(hi[n], d.mglobal[n][1] + 1) : d.mglobal[n][1] + 1;
+      ext = std::max(ext, d.mglobal[n][1] - d.mglobal[n][0] + 3);
+    }
+  }
+  int nb[3] = {1, 1, 1};
+  for (int n = 0; n < nd; n++) nb[n] = (hi[n] - lo[n]) / ext + 1;
+  const long nbins = (long)nb[0] * nb[1] * nb[2];
+  std::vector<std::vector<int>> bins(nbins);
+  auto bin_range = [&](const int (*box)[2], int* b0, int* b1) {
+    for (int n = 0; n < 3; n++) {
+      if (n >= nd) { b0[n] = b1[n] = 0; continue; }
+      b0[n] = std::max(0, (box[n][0] - lo[n]) / ext);
+      b1[n] = std::min(nb[n] - 1, (box[n][1] - lo[n]) / ext);
+      if (box[n][1] < lo[n] || box[n][0] > hi[n]) return false;
+    }
+    return true;
+  };
+  for (size_t a = 0; a < np; a++) {
+    const bsam_patch_desc& d = c->desc[ps[a]];
+    int box[3][2], b0[3], b1[3];
+    for (int n = 0; n < 3; n++) {
+      box[n][0] = n < nd ? d.mglobal[n][0] - 1 : 0;
+      box[n][1] = n < nd ? d.mglobal[n][1] + 1 : 0;
+    }
+    bin_range(box, b0, b1);
+    for (int z = b0[2]; z <= b1[2]; z++)
+      for (int y = b0[1]; y <= b1[1]; y++)
+        for (int x = b0[0]; x <= b1[0]; x++) bins[((long)z * nb[1] + y) * nb[0] + x].push_back((int)a);
+  }
+  std::vector<int> seen(np, -1);
+  for (size_t a = 0; a < np; a++) {
+    const bsam_patch_desc& d = c->desc[ps[a]];
+    // a's box shifted by -off meets b's box exactly when b's box shifted by
+    // +off meets a's: both polarities of every offset, and no offset
+    for (int q = -1; q < 2 * (int)offs.size(); q++) {
+      int box[3][2], b0[3], b1[3];
+      for (int n = 0; n < 3; n++) {
+        const int sh = q < 0 ? 0 : (q & 1 ? 1 : -1) * offs[q >> 1][n];
+        box[n][0] = n < nd ? d.mglobal[n][0] - 1 - sh : 0;
+        box[n][1] = n < nd ? d.mglobal[n][1] + 1 - sh : 0;
+      }
+      if (!bin_range(box, b0, b1)) continue;
+      for (int z = b0[2]; z <= b1[2]; z++)
+        for (int y = b0[1]; y <= b1[1]; y++)
+          for (int x = b0[0]; x <= b1[0]; x++)
+            for (int b : bins[((long)z * nb[1] + y) * nb[0] + x])
+              if (b > (int)a && seen[b] != (int)a) {
+                seen[b] = (int)a;
+                cand[a].push_back(b);
+              }
+    }
+    std::sort(cand[a].begin(), cand[a].end());
+  }
+  return cand;
+}
+
 int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const int nd = c->p.ndims;
   std::vector<std::array<int, 3>> offs;
@@ -2817,9 +2911,12 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   periodic_offsets(c, L.level, offs, periodic);
   std::vector<BoxDesc> sib, px, py, pz, pp, phys[3];
   const auto& ps = L.patches;
+  const std::vector<std::vector<int>> cand =
+      L.level > 0 ? neighbour_candidates(c, ps, periodic ? offs : std::vector<std::array<int, 3>>{})
+                  : std::vector<std::vector<int>>();
   if (L.level > 0)
     for (size_t a = 0; a < ps.size(); a++)
-      for (size_t b = a + 1; b < ps.size(); b++) ghost_overlap_boxes(c, ps[a], ps[b], nullptr, kCopy, sib);
+      for (int b : cand[a]) ghost_overlap_boxes(c, ps[a], ps[b], nullptr, kCopy, sib);
   bool all_wrap = periodic && ps.size() == 1;
   if (periodic) {
     for (int pid : ps) {
@@ -2848,7 +2945,7 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
     }
     if (L.level > 0)
       for (size_t a = 0; a < ps.size(); a++)
-        for (size_t b = a + 1; b < ps.size(); b++)
+        for (int b : cand[a])
           for (int pol = -1; pol <= 1; pol += 2)
             for (auto& o : offs) {
               int off[3] = {pol * o[0], pol * o[1], pol * o[2]};
@@ -3123,8 +3220,20 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const int nd = c->p.ndims;
   const LevelCells LC(c, L);
   FlatCall E[3];
+  // the three calls are independent: flatten them on three host threads
+  bool ok[3] = {false, false, false};
+  std::string why[3];
+  {
+    std::vector<std::thread> th;
+    for (int p = 0; p < 3; p++)
+      th.emplace_back([&, p]() { ok[p] = flatten_setghost(c, L, G, LC, p, E[p], why[p]); });
+    for (auto& t : th) t.join();
+  }
   for (int p = 0; p < 3; p++) {
-    if (!flatten_setghost(c, L, G, LC, p, E[p], G.why)) return BSAM_OK;
+    if (!ok[p]) {
+      G.why = why[p];
+      return BSAM_OK;
+    }
     // the rules that read ghost values from before the call run in a first
     // launch; none of them may write a cell another of them reads
     for (size_t n = 0; n < E[p].ex.size(); n++) {
@@ -3164,10 +3273,14 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       }
     if (!anypost) H.cpost.clear();
   };
-  for (int p = 0; p < 3; p++) {
-    std::vector<int> order(E[p].ex.size());
-    for (size_t n = 0; n < order.size(); n++) order[n] = (int)n;
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return E[p].dst[x] < E[p].dst[y]; });
+  // the three ipasses' tables are independent: one host thread each
+  auto plan_p = [&](int p, std::string& why) -> bool {
+    // destination order: a level's patches are in ascending pool offset, so
+    // the dense cell index ascends with the destination offset
+    std::vector<int> order;
+    order.reserve(E[p].ex.size());
+    for (long cell = 0; cell < LC.n; cell++)
+      if (E[p].at[cell] >= 0) order.push_back(E[p].at[cell]);
     emit(E[p], order, G.map[p]);
     if (p == 0) {
       // map[5]: the colour-1 face ghosts of SetGhost(level, 0)
@@ -3179,8 +3292,8 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
         qdecode(D, E[0].dst[n], i, j, k);
         if (n_outside(D, nd, i, j, k) == 1 && ((i + j + k) & 1) == 1) {
           if (e.early) {
-            G.why = "a colour-1 face ghost of SetGhost(level, 0) reads a ghost from before the call";
-            return BSAM_OK;
+            why = "a colour-1 face ghost of SetGhost(level, 0) reads a ghost from before the call";
+            return false;
           }
           sel.push_back(n);
         }
@@ -3252,7 +3365,7 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
         if (!exact) sel6.push_back(n);
       }
       emit(E[0], sel6, G.map[6]);
-      continue;
+      return true;
     }
     const int col = p - 1;
     const FlatCall& other = E[3 - p];
@@ -3267,8 +3380,8 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       if (n_outside(D, nd, i, j, k) != 1) {
         // an edge or corner ghost: only if the other ipass does not write it
         if (!o && e.early) {
-          G.why = "a colour-filtered edge or corner ghost reads a ghost from before the call";
-          return BSAM_OK;
+          why = "a colour-filtered edge or corner ghost reads a ghost from before the call";
+          return false;
         }
         if (!o) lean.push_back(n);
         continue;
@@ -3289,12 +3402,25 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       }
       if (!need) need = !o || !same_expr(G, e, *o);
       if (need && e.early) {
-        G.why = "a face ghost of a reduced table reads a ghost from before the call";
-        return BSAM_OK;
+        why = "a face ghost of a reduced table reads a ghost from before the call";
+        return false;
       }
       if (need) lean.push_back(n);
     }
     emit(E[p], lean, G.map[2 + p]);
+    return true;
+  };
+  {
+    bool ok[3] = {false, false, false};
+    std::string why[3];
+    std::vector<std::thread> th;
+    for (int p = 0; p < 3; p++) th.emplace_back([&, p]() { ok[p] = plan_p(p, why[p]); });
+    for (auto& t : th) t.join();
+    for (int p = 0; p < 3; p++)
+      if (!ok[p]) {
+        G.why = why[p];
+        return BSAM_OK;
+      }
   }
   G.maps_ok = true;
   return BSAM_OK;
