@@ -2795,6 +2795,20 @@ static size_t host_threads() {
   return n ? std::min(16u, n) : 4u;
 }
 
+// fn(t) for t = 0 .. n-1 on host threads; the calls a thread could not be
+// started for run on the calling thread.
+template <typename F>
+static void run_threads(int n, F&& fn) {
+  std::vector<std::thread> th;
+  int started = 0;
+  try {
+    for (; started < n; started++) th.emplace_back([&fn, started]() { fn(started); });
+  } catch (...) {
+  }
+  for (int t = started; t < n; t++) fn(t);
+  for (auto& x : th) x.join();
+}
+
 static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const size_t np = L.patches.size();
   for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(np + 1, 0);
@@ -2804,17 +2818,14 @@ static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   for (int sg = 0; sg < 3; sg++) per[sg].resize(np);
   // patches are independent: interleaved over the host's threads
   const int nth = (int)std::max<size_t>(1, std::min<size_t>(np, host_threads()));
-  std::vector<std::thread> th;
-  for (int t = 0; t < nth; t++)
-    th.emplace_back([&, t]() {
-      for (size_t a = t; a < np; a += nth) {
-        const DPatch& P = c->hp[L.patches[a]];
-        std::vector<GRule> seg[3];
-        interp_rules(P, c->hp[P.parent], c->p.ndims, seg);
-        for (int sg = 0; sg < 3; sg++) per[sg][a].swap(seg[sg]);
-      }
-    });
-  for (auto& x : th) x.join();
+  run_threads(nth, [&](int t) {
+    for (size_t a = t; a < np; a += nth) {
+      const DPatch& P = c->hp[L.patches[a]];
+      std::vector<GRule> seg[3];
+      interp_rules(P, c->hp[P.parent], c->p.ndims, seg);
+      for (int sg = 0; sg < 3; sg++) per[sg][a].swap(seg[sg]);
+    }
+  });
   size_t total = 0;
   for (int sg = 0; sg < 3; sg++)
     for (size_t a = 0; a < np; a++) total += per[sg][a].size();
@@ -3223,12 +3234,7 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   // the three calls are independent: flatten them on three host threads
   bool ok[3] = {false, false, false};
   std::string why[3];
-  {
-    std::vector<std::thread> th;
-    for (int p = 0; p < 3; p++)
-      th.emplace_back([&, p]() { ok[p] = flatten_setghost(c, L, G, LC, p, E[p], why[p]); });
-    for (auto& t : th) t.join();
-  }
+  run_threads(3, [&](int p) { ok[p] = flatten_setghost(c, L, G, LC, p, E[p], why[p]); });
   for (int p = 0; p < 3; p++) {
     if (!ok[p]) {
       G.why = why[p];
@@ -3413,9 +3419,7 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   {
     bool ok[3] = {false, false, false};
     std::string why[3];
-    std::vector<std::thread> th;
-    for (int p = 0; p < 3; p++) th.emplace_back([&, p]() { ok[p] = plan_p(p, why[p]); });
-    for (auto& t : th) t.join();
+    run_threads(3, [&](int p) { ok[p] = plan_p(p, why[p]); });
     for (int p = 0; p < 3; p++)
       if (!ok[p]) {
         G.why = why[p];
@@ -5117,7 +5121,10 @@ static int32_t tune_knob(bsam_ctx* c, const char* name, int* knob, const std::ve
   for (size_t n = 0; n < cand.size(); n++) {
     *knob = cand[n];
     const float t = time_launches(c, 3, fn);
-    if (t < 0.0f) return set_err(BSAM_EHIP, "autotune: %s candidate %d failed", name, cand[n]);
+    if (t < 0.0f) {
+      *knob = best;
+      return set_err(BSAM_EHIP, "autotune: %s candidate %d failed", name, cand[n]);
+    }
     snprintf(buf, sizeof buf, "%s[%d,%.1f]", n ? "," : "", cand[n], t);
     rep += buf;
     if (n == 0 || t < tbest) {
