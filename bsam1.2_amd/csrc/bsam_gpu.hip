@@ -1333,27 +1333,31 @@ __global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ c
 // Box copies of one ghost-fill stage: sibling transfers (GhostOverlap,
 // boundary.f90:359-431), self-periodic wraps (SelfPeriodicBC :480-577),
 // periodic transfers (TransferPeriodicBC :580-611) and physical boundary
-// conditions (SetPhysicalBC + UserBC).  Element e of a box runs variable
-// by variable over the box in x-fastest order.
+// conditions (SetPhysicalBC + UserBC).  Cell c of a box (x-fastest) is
+// done for every variable, with the index arithmetic once per cell; the
+// cells of one box are independent (its sources are interior or
+// earlier-written cells, its destinations distinct).
 // `homog` selects the boundary condition of a correction (mixed precision):
 // q_ghost = 2 - q_in becomes e_ghost = -e_in.
 template <typename T>
-__device__ __forceinline__ void box_elem(const DPatch& D, const DPatch& S, const BoxDesc& B, int n, int e,
+__device__ __forceinline__ T box_value(int op, int v, int homog, T x) {
+  const T dir = homog ? -x : (T)2 - x;
+  switch (op) {
+    case kCopyPlus0: return x + (T)0;
+    case kDirichlet2: return dir;
+    case kBgMixed: return (v == 0) ? x : dir;
+    case kTumourBC: return (v == 2) ? dir : ((v == 3) ? -x : x);
+    default: return x;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void box_cell(const DPatch& D, const DPatch& S, const BoxDesc& B, int c, int nv,
                                          int homog, T* __restrict__ q) {
-  const int v = e / n, c = e % n;
   const int a = c % B.ext[0], r = c / B.ext[0];
   const int b = r % B.ext[1], z = r / B.ext[1];
-  const T x = q[qidx(S, B.slo[0] + a, B.slo[1] + b, B.slo[2] + z) + v * S.qvs];
-  const T dir = homog ? -x : (T)2 - x;
-  T y;
-  switch (B.op) {
-    case kCopyPlus0: y = x + (T)0; break;
-    case kDirichlet2: y = dir; break;
-    case kBgMixed: y = (v == 0) ? x : dir; break;
-    case kTumourBC: y = (v == 2) ? dir : ((v == 3) ? -x : x); break;
-    default: y = x;
-  }
-  q[qidx(D, B.dlo[0] + a, B.dlo[1] + b, B.dlo[2] + z) + v * D.qvs] = y;
+  const int so = qidx(S, B.slo[0] + a, B.slo[1] + b, B.slo[2] + z);
+  const int dof = qidx(D, B.dlo[0] + a, B.dlo[1] + b, B.dlo[2] + z);
+  for (int v = 0; v < nv; v++) q[dof + v * D.qvs] = box_value<T>(B.op, v, homog, q[so + v * S.qvs]);
 }
 
 // One stage: box blockIdx.x, split over gridDim.y workgroups.
@@ -1365,30 +1369,109 @@ __global__ __launch_bounds__(kBlock) void k_boxops(const DPatch* __restrict__ pt
   const DPatch D = pt[B.dp];
   const DPatch S = pt[B.sp];
   const int n = B.ext[0] * B.ext[1] * B.ext[2];
-  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < n * nv; e += blockDim.x * gridDim.y)
-    box_elem<T>(D, S, B, n, e, homog, q);
+  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < n; e += blockDim.x * gridDim.y)
+    box_cell<T>(D, S, B, e, nv, homog, q);
 }
 
-// Every stage of a small level's fill in one workgroup, box after box with
-// a barrier after each: the reference's sequential order in one launch
-// (the physical conditions of the small levels at and below the root).
+// Every stage of a small level's fill in one workgroup, with a barrier
+// after each stage: the reference's sequential order in one launch (the
+// physical conditions of the small levels at and below the root).  The
+// boxes of one stage are independent (as the one-launch-per-stage path
+// also relies on), so a stage is one flat range of cells; each thread loads
+// up to kSeqCells cells of it, every variable, before it stores any, so a
+// stage costs about one memory round trip and one barrier.
 constexpr int kMaxStages = 8;
 struct BoxSeq {
   const BoxDesc* d[kMaxStages];
   int n[kMaxStages];
   int ns;
 };
+constexpr int kSeqMaxBoxes = 48;  // boxes of a sequence staged in LDS
+constexpr int kSeqCells = 4;
+constexpr int kSeqMaxVars = 4;
 template <typename T>
 __device__ void run_boxseq(const DPatch* __restrict__ pt, const BoxSeq& S, int nv, int homog, T* __restrict__ q) {
-  for (int st = 0; st < S.ns; st++)
-    for (int bx = 0; bx < S.n[st]; bx++) {
-      const BoxDesc B = S.d[st][bx];
-      const DPatch D = pt[B.dp];
-      const DPatch Sp = pt[B.sp];
-      const int n = B.ext[0] * B.ext[1] * B.ext[2];
-      for (int e = threadIdx.x; e < n * nv; e += blockDim.x) box_elem<T>(D, Sp, B, n, e, homog, q);
-      __syncthreads();
+  __shared__ BoxDesc sbox[kSeqMaxBoxes];
+  __shared__ int scell[kSeqMaxBoxes];  // first cell of each box within its stage
+  __shared__ int stot[kMaxStages];     // cells of each stage
+  __shared__ DPatch spat[2];
+  int nbox = 0;
+  for (int st = 0; st < S.ns; st++) nbox += S.n[st];
+  if (nbox > kSeqMaxBoxes) {
+    for (int st = 0; st < S.ns; st++)
+      for (int bx = 0; bx < S.n[st]; bx++) {
+        const BoxDesc B = S.d[st][bx];
+        const DPatch D = pt[B.dp];
+        const DPatch Sp = pt[B.sp];
+        const int n = B.ext[0] * B.ext[1] * B.ext[2];
+        for (int e = threadIdx.x; e < n; e += blockDim.x) box_cell<T>(D, Sp, B, e, nv, homog, q);
+        __syncthreads();
+      }
+    return;
+  }
+  if ((int)threadIdx.x < nbox) {
+    int t = threadIdx.x, st = 0;
+    while (t >= S.n[st]) {
+      t -= S.n[st];
+      st++;
     }
+    sbox[threadIdx.x] = S.d[st][t];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int b = 0;
+    for (int st = 0; st < S.ns; st++) {
+      int acc = 0;
+      for (int i = 0; i < S.n[st]; i++, b++) {
+        scell[b] = acc;
+        acc += sbox[b].ext[0] * sbox[b].ext[1] * sbox[b].ext[2];
+      }
+      stot[st] = acc;
+    }
+  }
+  if (nbox > 0 && threadIdx.x >= 64 && threadIdx.x < 66)
+    spat[threadIdx.x - 64] = pt[threadIdx.x == 65 ? sbox[0].sp : sbox[0].dp];
+  __syncthreads();
+  const int dp0 = nbox > 0 ? sbox[0].dp : -1, sp0 = nbox > 0 ? sbox[0].sp : -1;
+  int b0 = 0;
+  for (int st = 0; st < S.ns; st++) {
+    const int nb = S.n[st], tot = stot[st];
+    for (int base = threadIdx.x; base < tot; base += kSeqCells * blockDim.x) {
+      T y[kSeqCells][kSeqMaxVars];
+      int dof[kSeqCells], dvs[kSeqCells];
+#pragma unroll
+      for (int u = 0; u < kSeqCells; u++) {
+        const int e = base + u * (int)blockDim.x;
+        dof[u] = -1;
+        dvs[u] = 0;
+        if (e < tot) {
+          int b = b0;
+          while (b + 1 < b0 + nb && scell[b + 1] <= e) b++;
+          const BoxDesc B = sbox[b];
+          const DPatch D = B.dp == dp0 ? spat[0] : (B.dp == sp0 ? spat[1] : pt[B.dp]);
+          const DPatch Sp = B.sp == sp0 ? spat[1] : (B.sp == dp0 ? spat[0] : pt[B.sp]);
+          const int c = e - scell[b];
+          const int a = c % B.ext[0], r = c / B.ext[0];
+          const int bb = r % B.ext[1], z = r / B.ext[1];
+          const int so = qidx(Sp, B.slo[0] + a, B.slo[1] + bb, B.slo[2] + z);
+          dof[u] = qidx(D, B.dlo[0] + a, B.dlo[1] + bb, B.dlo[2] + z);
+          dvs[u] = D.qvs;
+#pragma unroll
+          for (int v = 0; v < kSeqMaxVars; v++)
+            if (v < nv) y[u][v] = box_value<T>(B.op, v, homog, q[so + v * Sp.qvs]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kSeqCells; u++)
+        if (dof[u] >= 0) {
+#pragma unroll
+          for (int v = 0; v < kSeqMaxVars; v++)
+            if (v < nv) q[dof[u] + v * dvs[u]] = y[u][v];
+        }
+    }
+    __syncthreads();
+    b0 += nb;
+  }
 }
 template <typename T>
 __global__ __launch_bounds__(1024) void k_boxops_seq(const DPatch* __restrict__ pt, BoxSeq S, int nv, int homog,
