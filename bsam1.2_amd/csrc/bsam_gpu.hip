@@ -3112,7 +3112,9 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
 // gather launch evaluates them all with the bits of the staged order.
 // ---------------------------------------------------------------------------
 struct FExpr {
-  int kind;   // 0: copy of cell `src`; 1: interpolation rule `src` of the plan
+  int kind;   // 0: copy of cell `src`; 1: interpolation rule `src` of the plan;
+              // 2: a value read from an edge or corner ghost the call has not
+              // written (allowed only where a later stage overwrites it)
   int src;
   int svs;    // copy: source variable stride
   int dvs;    // destination variable stride
@@ -3209,13 +3211,12 @@ static bool flatten_setghost(const bsam_ctx* c, const Level& L, const GhostPlan&
             } else {
               // a ghost not written so far in this call is read as it was
               // before the call (physical conditions over ranges that
-              // include the ghost rows, boundary.f90:642-802)
+              // include the ghost rows, boundary.f90:642-802); an edge or
+              // corner ghost read so is only tolerated in values that a
+              // later stage overwrites (the x faces of a 3D box read the
+              // y-z edges, whose images the z faces rewrite)
               const int out = n_outside(S, nd, si, sj, sk);
-              if (out > 1) {
-                why = "a ghost-fill stage reads an edge or corner ghost the call has not written";
-                return false;
-              }
-              e = FExpr{0, s, S.qvs, 0, B.sp, 0, 0, 0, out};
+              e = FExpr{out > 1 ? 2 : 0, s, S.qvs, 0, B.sp, 0, 0, 0, out};
             }
             e.dvs = D.qvs;
             e.dpid = B.dp;
@@ -3230,6 +3231,11 @@ static bool flatten_setghost(const bsam_ctx* c, const Level& L, const GhostPlan&
             E.set(LC.at(c, B.dp, d), d, e);
           }
     }
+  for (const FExpr& e : E.ex)
+    if (e.kind == 2) {
+      why = "a ghost value is read from an edge or corner ghost the call has not written";
+      return false;
+    }
   return true;
 }
 
@@ -3243,8 +3249,9 @@ static bool same_expr(const GhostPlan& G, const FExpr& a, const FExpr& b) {
   return true;
 }
 
-// The flattened maps of a level (levels > 0 only: below them SetGhost is a
-// periodic wrap or a physical condition on one patch).
+// The flattened maps of a level (at and below the root only where the fill
+// is box stages: a single patch periodic in every dimension keeps its wrap
+// kernel).
 //
 // map[p], p = 0, 1, 2: SetGhost(level, p) in full.
 // map[2 + p], p = 1, 2: the face ghosts X that SetGhost(level, p) writes
@@ -3310,7 +3317,7 @@ static std::vector<uint8_t> parent_interior(const bsam_ctx* c, int level) {
 
 int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   G.maps_ok = false;
-  if (L.level <= 0) return BSAM_OK;
+  if (G.wrap_pid >= 0 || (L.level <= 0 && G.stages.empty())) return BSAM_OK;
   const int nd = c->p.ndims;
   const LevelCells LC(c, L);
   FlatCall E[3];
@@ -3868,7 +3875,6 @@ extern "C" int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t n
   const int nd = c->p.ndims, nv = c->p.nrvars;
   Rng rng{seed * 2654435761u + 12345u};
   for (Level& L : c->lv) {
-    if (L.level <= 0) continue;
     GhostPlan G;
     RETCHK(plan_ghost_stages(c, L, G));
     plan_interp_rules(c, L, G);

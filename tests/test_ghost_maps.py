@@ -26,40 +26,49 @@ WORKLOADS = os.path.join(ROOT, "bsam1.2_amd", "bsam_amd", "workloads")
 
 
 def _selftest(params, hier, seed=7):
-    """(values compared, levels checked, levels > 0)"""
+    """(values compared, levels checked, lowest level, finest level)"""
     lib = gpu.load()
     lo, hi = hierarchy.levels(hier)
     descs = hierarchy.to_ctypes(hier)
     n, nl = C.c_int64(0), C.c_int32(0)
     rc = lib.bsam_selftest_ghost_maps(C.byref(params), len(hier), descs, lo, hi, seed, C.byref(n), C.byref(nl))
     assert rc == abi.BSAM_OK, lib.bsam_last_error().decode()
-    return n.value, nl.value, hi
+    return n.value, nl.value, lo, hi
 
 
 @pytest.mark.parametrize("name", ["ch2d_amr", "ch2d_amr_phys", "ch3d_amr", "ch3d_amr_phys", "tm2d_amr_phys",
                                   "tm3d_amr_phys"])
 def test_golden_hierarchies(name):
-    """Every refined level is flattened except where a 3D physical corner
-    reads an edge ghost from before the call (the staged fill stays).  The
-    tumour fixtures exercise four fields and their mixed boundary
-    condition."""
+    """Every refined level is flattened; with physical boundaries so are the
+    root and the levels below it, whose fill is the physical condition
+    face by face (a 3D face reads edge ghosts from before the call, which
+    the later faces overwrite).  A root periodic in every dimension keeps
+    its wrap kernel.  The tumour fixtures exercise four fields and their
+    mixed boundary condition."""
     meta, _ = fu.load(name)
     for st in meta["steps"]:
-        n, nl, hi = _selftest(fu.step_params(meta), st["hierarchy"])
-        if name == "ch3d_amr_phys":
-            continue
-        assert nl == hi and n > 0
+        n, nl, lo, hi = _selftest(fu.step_params(meta), st["hierarchy"])
+        expect = hi - lo + 1 if name.endswith("_phys") else hi
+        assert nl == expect and n > 0
 
 
-@pytest.mark.parametrize("name", ["ch2d_amr_128_l2", "ch3d_amr_64_l3"])
+@pytest.mark.parametrize("name", ["ch2d_amr_128_l2", "ch3d_amr_64_l3", "tm3d_amr_64_l3"])
 def test_bench_meshes(name):
+    """The AMR bench meshes: the periodic CH meshes flatten every refined
+    level, the tumour mesh (physical conditions on every face) every
+    level."""
     w = json.load(open(os.path.join(WORKLOADS, name + ".json")))
     nd = w["ndims"]
     base = hierarchy.uniform(nd, (w["n"],) * nd, w["minlevel"], w["mthbc"])
     hier = hierarchy.from_patch_boxes(base, w["boxes"], nd)
-    p = configs.params(nd, w["dt"], w["eps"], w["tol"])
-    n, nl, hi = _selftest(p, hier)
-    assert nl == hi == w["maxlevel"] and n > 0
+    kw = {}
+    if "tumour" in w:
+        t = w["tumour"]
+        kw = dict(model=abi.MODEL_TUMOUR, extra=(t["lamp"], t["lama"], t["kappa"]))
+    p = configs.params(nd, w["dt"], w["eps"], w["tol"], **kw)
+    n, nl, lo, hi = _selftest(p, hier)
+    assert hi == w["maxlevel"] and lo == w["minlevel"] and n > 0
+    assert nl == (hi - lo + 1 if "tumour" in w else hi)
 
 
 def test_overlapping_patches_keep_the_staged_fill():
@@ -68,5 +77,5 @@ def test_overlapping_patches_keep_the_staged_fill():
     the fill cannot be flattened and the level keeps the staged order."""
     base = hierarchy.uniform(2, (16, 16), -3, [2, 2, 2, 2])
     hier = hierarchy.from_patch_boxes(base, [[1, [[1, 8], [1, 8]]], [1, [[7, 14], [1, 8]]]], 2)
-    n, nl, hi = _selftest(configs.params(2, 1e-4, 0.03, 1e-9), hier)
+    n, nl, lo, hi = _selftest(configs.params(2, 1e-4, 0.03, 1e-9), hier)
     assert hi == 1 and nl == 0
