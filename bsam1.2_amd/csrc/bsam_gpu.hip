@@ -1340,7 +1340,7 @@ __global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ c
 // `homog` selects the boundary condition of a correction (mixed precision):
 // q_ghost = 2 - q_in becomes e_ghost = -e_in.
 template <typename T>
-__device__ __forceinline__ T box_value(int op, int v, int homog, T x) {
+__host__ __device__ __forceinline__ T box_value(int op, int v, int homog, T x) {
   const T dir = homog ? -x : (T)2 - x;
   switch (op) {
     case kCopyPlus0: return x + (T)0;
@@ -1473,6 +1473,89 @@ __device__ void run_boxseq(const DPatch* __restrict__ pt, const BoxSeq& S, int n
     b0 += nb;
   }
 }
+// SetPhysicalBC of one patch with condition 10 on every face, in one pass
+// of the workgroup over the ghost shell.  The staged fill runs the x faces,
+// then y, then z, each face box spanning the ghost rows of the other
+// dimensions and reading the cells one step inside; so a ghost cell ends
+// as the face operation applied once per dimension in which it lies
+// outside, in x, y, z order, to the interior cell mirrored into the patch
+// (0 -> 1, mx+1 -> mx) -- the same operations on the same value.  Shell
+// cells: the two full z planes (3D), then per interior plane the two full
+// y rows and the x ends of the interior rows.  Sources are interior cells
+// only, so the cells are independent; each thread loads up to kSeqCells
+// cells before it stores.
+template <int DIM>
+__host__ __device__ __forceinline__ int shell_cells(const DPatch& P) {
+  const int caps = DIM == 3 ? 2 * P.nq[0] * P.nq[1] : 0;
+  return caps + (DIM == 3 ? P.mx[2] : 1) * (2 * P.nq[0] + 2 * P.mx[1]);
+}
+// Shell cell e: its storage indices and its mirrored interior source; the
+// value is the face operation applied once per outside dimension, x first.
+template <int DIM>
+__host__ __device__ __forceinline__ void shell_cell(const DPatch& P, int e, int& i, int& j, int& k, int& so,
+                                                    int out[3]) {
+  const int n0 = P.nq[0], n1 = P.nq[1], n2 = DIM == 3 ? P.nq[2] : 1;
+  const int plane = n0 * n1, caps = DIM == 3 ? 2 * plane : 0;
+  const int ring = 2 * n0 + 2 * P.mx[1];  // shell cells of one interior plane
+  if (e < caps) {
+    const int r = e % plane;
+    k = e < plane ? 0 : n2 - 1;
+    i = r % n0;
+    j = r / n0;
+  } else {
+    const int e2 = e - caps, r = e2 % ring;
+    k = DIM == 3 ? e2 / ring + 1 : 0;
+    if (r < 2 * n0) {
+      j = r < n0 ? 0 : n1 - 1;
+      i = r % n0;
+    } else {
+      j = (r - 2 * n0) / 2 + 1;
+      i = (r & 1) ? n0 - 1 : 0;
+    }
+  }
+  out[0] = i == 0 || i == n0 - 1;
+  out[1] = j == 0 || j == n1 - 1;
+  out[2] = DIM == 3 && (k == 0 || k == n2 - 1);
+  const int si = i == 0 ? 1 : (i == n0 - 1 ? P.mx[0] : i);
+  const int sj = j == 0 ? 1 : (j == n1 - 1 ? P.mx[1] : j);
+  const int sk = DIM == 3 ? (k == 0 ? 1 : (k == n2 - 1 ? P.mx[2] : k)) : 0;
+  so = qidx(P, si, sj, sk);
+}
+template <int DIM, typename T>
+__device__ void phys_shell(const DPatch& P, int op, int nv, int homog, T* __restrict__ q) {
+  const int nshell = shell_cells<DIM>(P);
+  for (int base = threadIdx.x; base < nshell; base += kSeqCells * blockDim.x) {
+    T y[kSeqCells][kSeqMaxVars];
+    int dof[kSeqCells];
+#pragma unroll
+    for (int u = 0; u < kSeqCells; u++) {
+      const int e = base + u * (int)blockDim.x;
+      dof[u] = -1;
+      if (e < nshell) {
+        int i, j, k, so, out[3];
+        shell_cell<DIM>(P, e, i, j, k, so, out);
+        dof[u] = qidx(P, i, j, k);
+#pragma unroll
+        for (int v = 0; v < kSeqMaxVars; v++)
+          if (v < nv) {
+            T x = q[so + v * P.qvs];
+            if (out[0]) x = box_value<T>(op, v, homog, x);
+            if (out[1]) x = box_value<T>(op, v, homog, x);
+            if (out[2]) x = box_value<T>(op, v, homog, x);
+            y[u][v] = x;
+          }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSeqCells; u++)
+      if (dof[u] >= 0) {
+#pragma unroll
+        for (int v = 0; v < kSeqMaxVars; v++)
+          if (v < nv) q[dof[u] + v * P.qvs] = y[u][v];
+      }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(1024) void k_boxops_seq(const DPatch* __restrict__ pt, BoxSeq S, int nv, int homog,
                                                      T* __restrict__ q) {
@@ -2097,9 +2180,24 @@ constexpr int kMaxChain = 12;
 struct TmChainArgs {
   int pid[kMaxChain];
   BoxSeq seq[kMaxChain];
+  int physop[kMaxChain];  // face operation when every face has condition 10, else -1
   int nlev;
   ModelK m;
 };
+
+// SetGhost(level, ipass) of a chain level: the face stages in order, or,
+// for a patch with condition 10 on every face, one pass over the shell.
+template <int DIM, typename T>
+__device__ __forceinline__ void tc_fill(const DPatch* __restrict__ pt, const TmChainArgs& A, int l, int homog,
+                                        T* __restrict__ q) {
+  if (A.physop[l] >= 0) {
+    const DPatch P = pt[A.pid[l]];
+    phys_shell<DIM, T>(P, A.physop[l], 4, homog, q);
+    __syncthreads();
+  } else {
+    run_boxseq<T>(pt, A.seq[l], 4, homog, q);
+  }
+}
 
 template <int DIM, bool LIN>
 __device__ void tc_level_relax(const DPatch* __restrict__ pt, const TmChainArgs& A, int l, int nsm,
@@ -2123,8 +2221,7 @@ __device__ void tc_level_relax(const DPatch* __restrict__ pt, const TmChainArgs&
         else relax_tm_cell<DIM>(P, i, j, k, A.m, q, qold, f);
       }
       __syncthreads();
-      run_boxseq<T>(pt, A.seq[l], 4, LIN ? 1 : 0, q);
-      __syncthreads();
+      tc_fill<DIM, T>(pt, A, l, LIN ? 1 : 0, q);
     }
 }
 
@@ -2155,8 +2252,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle_tm(
       }
     }
     __syncthreads();
-    run_boxseq<T>(pt, A.seq[l + 1], NV, LIN ? 1 : 0, q);  // SetGhost(L-1, 0)
-    __syncthreads();
+    tc_fill<DIM, T>(pt, A, l + 1, LIN ? 1 : 0, q);  // SetGhost(L-1, 0)
     // GetCoarseGhostPoints
     const int nbox = P.nc[0] * P.nc[1] * P.nc[2];
     for (int g = threadIdx.x; g < nbox; g += blockDim.x) {
@@ -2270,8 +2366,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle_tm(
       }
     }
     __syncthreads();
-    run_boxseq<T>(pt, A.seq[l], NV, LIN ? 1 : 0, q);  // SetGhost(L, 0)
-    __syncthreads();
+    tc_fill<DIM, T>(pt, A, l, LIN ? 1 : 0, q);  // SetGhost(L, 0)
     tc_level_relax<DIM, LIN>(pt, A, l, nsm, q, f, qold, coef);
   }
 }
@@ -2998,6 +3093,16 @@ static std::vector<std::vector<int>> neighbour_candidates(const bsam_ctx* c, con
   return cand;
 }
 
+// Operation of a physical face (condition 10, SetPhysicalBC + UserBC):
+// 3D CH 2 - q_in, 2D CH a copy; the tumour model copies variables 0 and 1,
+// sets 2 - q_in for variable 2 and -q_in for variable 3; the BG model
+// copies variable 0 and sets 2 - q_in for the others.
+static int phys_face_op(const bsam_ctx* c) {
+  if (c->p.model == BSAM_MODEL_CH) return c->p.ndims == 3 ? kDirichlet2 : kCopy;
+  if (c->p.model == BSAM_MODEL_TUMOUR) return kTumourBC;
+  return kBgMixed;
+}
+
 int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const int nd = c->p.ndims;
   std::vector<std::array<int, 3>> offs;
@@ -3061,9 +3166,7 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       const int dim = face / 2, hi = face % 2;
       BoxDesc b{};
       b.dp = b.sp = pid;
-      if (c->p.model == BSAM_MODEL_CH) b.op = (nd == 3) ? kDirichlet2 : kCopy;
-      else if (c->p.model == BSAM_MODEL_TUMOUR) b.op = kTumourBC;
-      else b.op = kBgMixed;
+      b.op = phys_face_op(c);
       for (int n = 0; n < 3; n++) {
         b.dlo[n] = b.slo[n] = 0;
         b.ext[n] = P.nq[n];
@@ -3911,6 +4014,46 @@ extern "C" int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t n
                            L.level, p, homog, n);
         *checked += c->nq;
       }
+      // (1b) a single patch with condition 10 on every face: the one-pass
+      // shell fill (phys_shell) against the staged call
+      bool allphys = L.patches.size() == 1 && G.wrap_pid < 0 && (int)G.stages.size() == nd;
+      for (int f = 0; f < 2 * nd && allphys; f++) allphys = c->desc[L.patches[0]].mthbc[f] == 10;
+      if (allphys) {
+        const DPatch& P = c->hp[L.patches[0]];
+        std::vector<double> s = a;
+        b = a;
+        host_staged(c, L, G, 0, homog, s);
+        const int ns = nd == 3 ? shell_cells<3>(P) : shell_cells<2>(P);
+        std::vector<uint8_t> seen((size_t)P.qvs, 0);
+        for (int e = 0; e < ns; e++) {
+          int i, j, k, so, out[3];
+          if (nd == 3) shell_cell<3>(P, e, i, j, k, so, out);
+          else shell_cell<2>(P, e, i, j, k, so, out);
+          const int d = qidx(P, i, j, k);
+          if (!n_outside(P, nd, i, j, k) || n_outside(P, nd, i, j, k) != out[0] + out[1] + out[2] ||
+              seen[d - P.qoff]++)
+            return set_err(BSAM_EINVAL, "level %d: shell cell %d (%d,%d,%d) is not a new ghost cell", L.level, e,
+                           i, j, k);
+          for (int v = 0; v < nv; v++) {
+            double x = a[so + v * P.qvs];
+            for (int ax = 0; ax < 3; ax++)
+              if (out[ax]) x = box_value<double>(phys_face_op(c), v, homog, x);
+            b[d + v * P.qvs] = x;
+          }
+        }
+        long nghost = 1, ninner = 1;
+        for (int ax = 0; ax < nd; ax++) {
+          nghost *= P.nq[ax];
+          ninner *= P.mx[ax];
+        }
+        if (ns != nghost - ninner)
+          return set_err(BSAM_EINVAL, "level %d: %d shell cells, %ld ghost cells", L.level, ns, nghost - ninner);
+        for (long n = 0; n < c->nq; n++)
+          if (memcmp(&s[n], &b[n], sizeof(double)))
+            return set_err(BSAM_EINVAL, "level %d homog %d: shell fill differs from the staged fill at %ld",
+                           L.level, homog, n);
+        *checked += c->nq;
+      }
       // (2) a red-black sequence (three passes, the last SetGhost full)
       // from an arbitrary state, each sweep replaced by new values of its
       // colour's interior cells; (3) the same after SetGhost(level, 0)
@@ -4634,6 +4777,11 @@ static int32_t launch_coarse_chain(bsam_ctx* c, int level, bool lin) {
         A.seq[n].d[st] = Ln.stages[st].d;
         A.seq[n].n[st] = Ln.stages[st].n;
       }
+      // one patch, condition 10 on every face: its stages are the faces
+      // x, y, z, which phys_shell evaluates in one pass
+      bool allphys = Ln.wrap_pid < 0 && (int)Ln.stages.size() == c->p.ndims;
+      for (int f = 0; f < 2 * c->p.ndims; f++) allphys = allphys && c->desc[A.pid[n]].mthbc[f] == 10;
+      A.physop[n] = allphys ? phys_face_op(c) : -1;
     }
     const int nsm = c->p.nsmoothingpasses;
     if (!lin) {
