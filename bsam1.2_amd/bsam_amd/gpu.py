@@ -124,6 +124,16 @@ class Solver:
     def copy_q_to_qold(self, level):
         self._chk(self.lib.bsam_copy_q_to_qold(self.ctx, level))
 
+    def estimate_level_errors(self, level, errflagopt, qtolerance, ncells):
+        """EstimateLevelErrors + ErrFlag tagging: the level's flags (patches
+        in descriptor order, each column-major over 1..mx) and the number
+        flagged."""
+        flags = np.zeros(ncells, dtype=np.int32)
+        nt = C.c_int64()
+        self._chk(self.lib.bsam_estimate_level_errors(self.ctx, level, errflagopt, qtolerance,
+                                                      flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nt)))
+        return flags, nt.value
+
     def vcycle(self, level):
         self._chk(self.lib.bsam_vcycle(self.ctx, level))
 

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1019,6 +1020,99 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs(LevelArgs L, ModelK m, const
   const double w = (DIM == 3) ? 0.125 : 0.25;
 #pragma unroll
   for (int v = 0; v < NV; v++) fpar[opar + v * Q.qvs] = w * s[v] + o[v];
+}
+
+// Regrid tagging, errflagopt 1: RelativeTruncationError
+// (afasroutines.f90:739-814), qrte = R(L_h q) - L_2h(qc) with the coarse
+// operator on parent%qold and ch, and SetErrFlags2D/3D
+// (bsamroutines.f90:1134-1203): the 2^d children of a coarse cell whose
+// largest |qrte| over the variables exceeds `tol` are flagged.  The fine
+// operator values are summed in the restriction's child order, as in
+// k_fas_rhs.  Domain: coarse cells.  flags: per patch (foff[position in
+// the level]) column-major over 1..mx, cleared beforehand.
+template <int DIM, int MODEL>
+__global__ __launch_bounds__(kBlock) void k_rte_flags(LevelArgs L, ModelK m, const double* __restrict__ q,
+                                                      const double* __restrict__ qold,
+                                                      const double* __restrict__ qc, double tol,
+                                                      const long* __restrict__ foff, int32_t* __restrict__ flags) {
+  constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
+  int pi, local;
+  if (!level_elem(L, level_block(L), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  const DPatch Q = L.pt[P.parent];
+  const int ic = local % P.cmx[0] + 1, row = local / P.cmx[0];
+  const int jc = row % P.cmx[1] + 1, kc = (DIM == 3) ? row / P.cmx[1] + 1 : 0;
+  const int i2 = 2 * ic, j2 = 2 * jc, k2 = 2 * kc;
+  const int nch = (DIM == 3) ? 8 : 4;
+  double s[NV];
+#pragma unroll
+  for (int n = 0; n < nch; n++) {
+    const int i = i2 - (n & 1), j = j2 - ((n >> 1) & 1);
+    const int k = (DIM == 3) ? k2 - ((n >> 2) & 1) : 0;
+    double o[kNvMax];
+    op_cell<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o);
+#pragma unroll
+    for (int v = 0; v < NV; v++) s[v] = (n == 0) ? o[v] : s[v] + o[v];
+  }
+  const int ip = P.mb[0][0] + ic - 1, jp = P.mb[1][0] + jc - 1;
+  const int kp = (DIM == 3) ? P.mb[2][0] + kc - 1 : 0;
+  const int oc = cidx(P, ic, jc, kc);
+  const int opar = qidx(Q, ip, jp, kp);
+  double o[kNvMax];
+  if (MODEL == BSAM_MODEL_TUMOUR) {
+    const int vs = P.cvs;
+    const double l2 = ulap_c<DIM>(qc, P, ic, jc, kc, vs), l1 = ulap_c<DIM>(qc, P, ic, jc, kc, 0);
+    const double l3 = ulap_c<DIM>(qc, P, ic, jc, kc, 2 * vs);
+    const double l4 = ulap_c<DIM>(qc, P, ic, jc, kc, 3 * vs);
+    op_tm_from<DIM>(qc[oc], qc[oc + vs], qc[oc + 2 * vs], l1, l2, l3, l4, qold[opar], m, Q.dx, o);
+  } else {
+    const double lmu = ulap_c<DIM>(qc, P, ic, jc, kc, P.cvs);
+    const double lphi = ulap_c<DIM>(qc, P, ic, jc, kc, 0);
+    const double po = (MODEL == BSAM_MODEL_BG) ? qold[opar] : 0.0;
+    op_from<DIM, MODEL>(qc[oc], qc[oc + P.cvs], lphi, lmu, po, m, Q.dx, o[0], o[1]);
+  }
+  const double w = (DIM == 3) ? 0.125 : 0.25;
+  double mv = -DBL_MAX;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const double a = fabs(w * s[v] - o[v]);
+    if (a > mv) mv = a;
+  }
+  if (!(mv > tol)) return;
+  int32_t* fl = flags + foff[pi];
+  const int mxx = P.mx[0], myy = P.mx[1];
+#pragma unroll
+  for (int n = 0; n < nch; n++) {
+    const int i = i2 - (n & 1), j = j2 - ((n >> 1) & 1);
+    const int k = (DIM == 3) ? k2 - ((n >> 2) & 1) : 1;
+    fl[((long)(k - 1) * myy + (j - 1)) * mxx + (i - 1)] = 1;
+  }
+}
+
+// Regrid tagging, errflagopt 10: the second-difference test of the CH and
+// BioGrowth SetErrFlagsUser2D/3D (CH/problem.f90:248-307, BioGrowth/
+// problem.f90:185-240) on variable 1:
+//   |q(i+1)-2q(i)+q(i-1)| + |..j..| (+ |..k..|) > thr (= ndims*qtolerance).
+// Domain: cells; every flag of the level is written.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_user_flags(LevelArgs L, const double* __restrict__ q, double thr,
+                                                       const long* __restrict__ foff, int32_t* __restrict__ flags) {
+  int pi, local;
+  if (!level_elem(L, level_block(L), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  int i, j, k;
+  if (!cell_of<DIM>(P, local, i, j, k)) return;
+  const double c0 = q[qidx(P, i, j, k)];
+  double d = fabs(q[qidx(P, i + 1, j, k)] - 2.0 * c0 + q[qidx(P, i - 1, j, k)]) +
+             fabs(q[qidx(P, i, j + 1, k)] - 2.0 * c0 + q[qidx(P, i, j - 1, k)]);
+  if (DIM == 3) d = d + fabs(q[qidx(P, i, j, k + 1)] - 2.0 * c0 + q[qidx(P, i, j, k - 1)]);
+  const int kk = (DIM == 3) ? k - 1 : 0;
+  flags[foff[pi] + ((long)kk * P.mx[1] + (j - 1)) * P.mx[0] + (i - 1)] = d > thr ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_i32(int32_t* __restrict__ a, long n) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g < n) a[g] = 0;
 }
 
 // CorrectFine, first half (afasroutines.f90:892-895, 934-938):
@@ -2501,6 +2595,9 @@ struct bsam_ctx {
   // staging
   double* d_stage = nullptr;
   long stage_cap = 0;
+  // regrid tagging: per-patch flag offsets, then the level's flags
+  char* d_tag = nullptr;
+  size_t tag_cap = 0;
   // graph of the V-cycle loop body
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -2699,6 +2796,7 @@ extern "C" int32_t bsam_destroy(bsam_ctx* c) {
   hipFree(c->d_partial2);
   hipFree(c->d_norm);
   hipFree(c->d_stage);
+  hipFree(c->d_tag);
   if (c->h_norm) hipHostFree(c->h_norm);
   for (auto& k : c->ks)
     for (hipEvent_t e : k.ev) hipEventDestroy(e);
@@ -4756,6 +4854,81 @@ extern "C" int32_t bsam_copy_q_to_qold(bsam_ctx* c, int32_t level) {
       LAUNCH_CHECK();
     }
   }
+  return BSAM_OK;
+}
+
+// EstimateLevelErrors (bsamroutines.f90:994-1040) and the tagging of
+// ErrFlag (:1059-1131); BufferAndList, InflateEdgeTags and the new grids
+// stay with the caller.  errflagopt 1 runs RestrictSolution, SetGhost(L-1,
+// 0), GetCoarseGhostPoints and CopyQToQold on L and L-1 (the reference's
+// order, with their effect on q, qc and qold), then the truncation-error
+// flags; errflagopt 10 applies the second-difference test to q as it is.
+extern "C" int32_t bsam_estimate_level_errors(bsam_ctx* c, int32_t level, int32_t errflagopt, double qtolerance,
+                                              int32_t* flags, int64_t* ntagged) {
+  if (!c || !flags || !ntagged) return set_err(BSAM_EINVAL, "null argument");
+  RETCHK(check_level(c, level));
+  if (level <= c->minlevel) return set_err(BSAM_EINVAL, "level %d has no coarser level to estimate against", level);
+  if (errflagopt != 1 && errflagopt != 10)
+    return set_err(BSAM_EINVAL, "errflagopt must be 1 (errflagdefault) or 10 (errflaguser), not %d", errflagopt);
+  Level& L = c->L(level);
+  const int nd = c->p.ndims;
+  std::vector<long> foff(L.patches.size());
+  long n = 0;
+  for (size_t a = 0; a < L.patches.size(); a++) {
+    foff[a] = n;
+    const DPatch& P = c->hp[L.patches[a]];
+    n += (long)P.mx[0] * P.mx[1] * P.mx[2];
+  }
+  const size_t obytes = ((foff.size() * sizeof(long) + 255) / 256) * 256;
+  const size_t need = obytes + (size_t)n * sizeof(int32_t);
+  if (need > c->tag_cap) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    hipFree(c->d_tag);
+    c->d_tag = nullptr;
+    c->tag_cap = 0;
+    HIPCHK(hipMalloc(&c->d_tag, need));
+    c->tag_cap = need;
+  }
+  long* d_foff = reinterpret_cast<long*>(c->d_tag);
+  int32_t* d_flags = reinterpret_cast<int32_t*>(c->d_tag + obytes);
+  HIPCHK(hipMemcpyAsync(d_foff, foff.data(), foff.size() * sizeof(long), hipMemcpyHostToDevice, c->s));
+  if (errflagopt == 1) {
+    RETCHK(bsam_restrict_solution(c, level));
+    RETCHK(bsam_set_ghost(c, level - 1, 0));
+    RETCHK(bsam_coarse_ghost_points(c, level));
+    RETCHK(bsam_copy_q_to_qold(c, level));
+    RETCHK(bsam_copy_q_to_qold(c, level - 1));
+    if (n > 0) {
+      k_zero_i32<<<nblocks(n), kBlock, 0, c->s>>>(d_flags, n);
+      LAUNCH_CHECK();
+    }
+    const double h = c->hp[L.patches[0]].dx;
+    const double tol = nd == 3 ? qtolerance / h / h / h : qtolerance / h / h;
+    const ModelK m = c->mk();
+    FOR_LAUNCHES(c, L, kCoarse, a, nb) {
+      if (c->p.model == BSAM_MODEL_CH)
+        DISPATCH_DIM(nd, (k_rte_flags<2, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->qc, tol, d_foff, d_flags)),
+                     (k_rte_flags<3, BSAM_MODEL_CH><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->qc, tol, d_foff, d_flags)));
+      else if (c->p.model == BSAM_MODEL_TUMOUR)
+        DISPATCH_DIM(nd, (k_rte_flags<2, BSAM_MODEL_TUMOUR><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->qc, tol, d_foff, d_flags)),
+                     (k_rte_flags<3, BSAM_MODEL_TUMOUR><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->qc, tol, d_foff, d_flags)));
+      else
+        k_rte_flags<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->qc, tol, d_foff, d_flags);
+      LAUNCH_CHECK();
+    }
+  } else {
+    const double thr = (nd == 3 ? 3.0 : 2.0) * qtolerance;
+    FOR_LAUNCHES(c, L, kCell, a, nb) {
+      DISPATCH_DIM(nd, (k_user_flags<2><<<nb, kBlock, 0, c->s>>>(a, c->q, thr, d_foff, d_flags)),
+                   (k_user_flags<3><<<nb, kBlock, 0, c->s>>>(a, c->q, thr, d_foff, d_flags)));
+      LAUNCH_CHECK();
+    }
+  }
+  if (n > 0) HIPCHK(hipMemcpyAsync(flags, d_flags, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  int64_t nt = 0;
+  for (long e = 0; e < n; e++) nt += flags[e] != 0;
+  *ntagged = nt;
   return BSAM_OK;
 }
 

@@ -65,6 +65,7 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_coarse_ghost_points, bsam_coarse_loading, bsam_correct_fine, &
          bsam_fill_down, bsam_get_source, bsam_update_source, bsam_update_aux, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
+         bsam_estimate_level_errors, &
          bsam_multigrid_iterations, bsam_time_step, bsam_synchronize, &
          bsam_autotune, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
@@ -180,6 +181,18 @@ INTERFACE
     IMPORT:: C_INT32_T, C_PTR
     TYPE(C_PTR), VALUE:: ctx
     INTEGER(C_INT32_T), VALUE:: level
+  END FUNCTION
+  ! EstimateLevelErrors + ErrFlag tagging (bsamroutines.f90:994-1131):
+  ! flags receives errorflags(1:mx) of every patch of the level, patches in
+  ! descriptor order, each column-major.
+  INTEGER(C_INT32_T) FUNCTION bsam_estimate_level_errors(ctx, level, errflagopt, qtolerance, flags, ntagged) &
+      BIND(C, NAME='bsam_estimate_level_errors')
+    IMPORT:: C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE:: ctx
+    INTEGER(C_INT32_T), VALUE:: level, errflagopt
+    REAL(C_DOUBLE), VALUE:: qtolerance
+    INTEGER(C_INT32_T), DIMENSION(*), INTENT(OUT):: flags
+    INTEGER(C_INT64_T), INTENT(OUT):: ntagged
   END FUNCTION
   INTEGER(C_INT32_T) FUNCTION bsam_vcycle(ctx, level) BIND(C, NAME='bsam_vcycle')
     IMPORT:: C_INT32_T, C_PTR

@@ -179,6 +179,24 @@ int32_t bsam_residual(bsam_ctx* ctx, int32_t level);
 int32_t bsam_error_afas(bsam_ctx* ctx, int32_t level, double* norm);
 /* CopyQToQold over the level (bsamroutines.f90:592-605). */
 int32_t bsam_copy_q_to_qold(bsam_ctx* ctx, int32_t level);
+/* Regrid tagging of one level: EstimateLevelErrors (bsamroutines.f90:
+ * 994-1040) and ErrFlag (:1059-1131) up to BufferAndList, which, with
+ * InflateEdgeTags and the new grids, stays with the caller.
+ *   errflagopt 1 (errflagdefault): RestrictSolution(level),
+ *     SetGhost(level-1, 0), GetCoarseGhostPoints(level), CopyQToQold on
+ *     level and level-1 (changing q, qc and qold as the reference does),
+ *     then RelativeTruncationError (afasroutines.f90:739-814) and
+ *     SetErrFlags2D/3D (bsamroutines.f90:1134-1203) with tolerance
+ *     qtolerance/h^ndims;
+ *   errflagopt 10 (errflaguser): the second-difference test of the CH and
+ *     BioGrowth SetErrFlagsUser2D/3D (CH/problem.f90:248-307,
+ *     BioGrowth/problem.f90:185-240) on variable 1 of q as it is, against
+ *     ndims*qtolerance.
+ * flags (host, int32) receives info%errorflags(1:mx) of every patch of the
+ * level, patches in descriptor order, each column-major; *ntagged the
+ * number of flagged cells.  level must be above minlevel.  Synchronous. */
+int32_t bsam_estimate_level_errors(bsam_ctx* ctx, int32_t level, int32_t errflagopt, double qtolerance,
+                                   int32_t* flags, int64_t* ntagged);
 
 /* ---- coarse-grained entry points --------------------------------------- */
 /* AFASVcycle(level) (afasroutines.f90:495-529). */

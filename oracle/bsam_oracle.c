@@ -38,6 +38,7 @@
  */
 #include "bsam_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -402,6 +403,14 @@ static inline void op_cell(orc_ctx* c, qview q, qview qo, double h, int i, int j
           dt * grow * (QV(q, i, j, k, 1) - cz);
     *o2 = -tmp3 * ulap(c, q, i, j, k, 1) + upt * QV(q, i, j, k, 1);
   }
+}
+
+/* Variable v of Operator at fine cell (i,j,k) (the restriction of an
+ * operator array reads it cell by cell). */
+static inline double op_var(orc_ctx* c, qview q, qview qo, double h, int i, int j, int k, int v) {
+  double o[4];
+  op_cell(c, q, qo, h, i, j, k, o);
+  return o[v];
 }
 
 /* Residual (afasroutines.f90:817-857): rf = f - Operator(q). */
@@ -1096,6 +1105,87 @@ int32_t orc_copy_q_to_qold(orc_ctx* c, int32_t level) {
     size_t n = (size_t)P->nq[0] * P->nq[1] * P->nq[2] * c->nv;
     memcpy(P->qold, P->q, n * 8);
   }
+  return BSAM_OK;
+}
+
+/* Regrid tagging of one level: EstimateLevelErrors (bsamroutines.f90:
+ * 994-1040) and ErrFlag (:1059-1131) up to BufferAndList, which stays with
+ * the caller.  errflagopt 1 (errflagdefault): RestrictSolution,
+ * SetGhost(level-1, 0), GetCoarseGhostPoints, CopyQToQold on the level and
+ * its parent level, RelativeTruncationError (afasroutines.f90:739-814)
+ *   qrte = Restriction(Operator(q, qold, h)) - Operator(qc, parent%qold, ch)
+ * and SetErrFlags2D/3D (bsamroutines.f90:1134-1203): the 2^d children of a
+ * coarse cell whose largest |qrte| over the variables exceeds
+ * qtolerance/h/h (2D) or qtolerance/h/h/h (3D) are flagged.  errflagopt 10
+ * (errflaguser): the second-difference test of the CH and BioGrowth
+ * SetErrFlagsUser2D/3D (CH/problem.f90:248-307, BioGrowth/problem.f90:
+ * 185-240) on variable 1 of q as it is:
+ *   |q(i+1)-2q(i)+q(i-1)| + |..j..| (+ |..k..|) > ndims*qtolerance.
+ * flags: the level's patches in level order, each column-major over 1..mx;
+ * *ntagged counts the flagged cells. */
+int32_t orc_estimate_level_errors(orc_ctx* c, int32_t level, int32_t errflagopt, double qtolerance,
+                                  int32_t* flags, int64_t* ntagged) {
+  if (!flags || !ntagged) return fail("null argument");
+  if (level <= c->minlevel || level > c->finestlevel) return fail("bad level %d", level);
+  if (errflagopt != 1 && errflagopt != 10) return fail("errflagopt must be 1 or 10");
+  if (errflagopt == 1) {
+    orc_restrict_solution(c, level);
+    int r = orc_set_ghost(c, level - 1, 0);
+    if (r) return r;
+    orc_coarse_ghost_points(c, level);
+    orc_copy_q_to_qold(c, level);
+    orc_copy_q_to_qold(c, level - 1);
+  }
+  int64_t off = 0, nt = 0;
+  FOR_LEVEL(c, level, P) {
+    const int mx = P->mx[0], my = P->mx[1], mz = P->mx[2];
+    int32_t* fl = flags + off;
+#define FL(i, j, k) fl[((size_t)((k) - 1) * my + ((j) - 1)) * mx + ((i) - 1)]
+    if (errflagopt == 1) {
+      opatch* Par = &c->pt[P->d.parent];
+      const int (*mb)[2] = P->d.mbounds;
+      const double h = P->d.dx, ch = Par->d.dx;
+      const double tol = c->ndims == 3 ? qtolerance / h / h / h : qtolerance / h / h;
+      qview q = {P->q, P->nq, {0, 0, 0}}, qo = {P->qold, P->nq, {0, 0, 0}};
+      qview qc = {P->qc, P->nc, {0, 0, 0}};
+      qview pqo = {Par->qold, Par->nq, {mb[0][0] - 1, mb[1][0] - 1, c->ndims == 3 ? mb[2][0] - 1 : 0}};
+      for (size_t n = 0; n < (size_t)mx * my * mz; n++) fl[n] = 0;
+      for (int kc = 1; kc <= kc_hi(c, P); kc++)
+        for (int jc = 1; jc <= P->cmx[1]; jc++)
+          for (int ic = 1; ic <= P->cmx[0]; ic++) {
+            double oc[4], mv = -DBL_MAX;
+            op_cell(c, qc, pqo, ch, ic, jc, kc, oc);
+            for (int v = 0; v < c->nv; v++) {
+              double r;
+#define GO(i, j, k) op_var(c, q, qo, h, i, j, k, v)
+              RESTRICT_CELL(c->ndims, GO, ic, jc, kc, r);
+#undef GO
+              const double a = fabs(r - oc[v]);
+              if (a > mv) mv = a;
+            }
+            if (mv > tol) {
+              const int k0 = c->ndims == 3 ? 2 * kc - 1 : 1, k1 = c->ndims == 3 ? 2 * kc : 1;
+              for (int k = k0; k <= k1; k++)
+                for (int j = 2 * jc - 1; j <= 2 * jc; j++)
+                  for (int i = 2 * ic - 1; i <= 2 * ic; i++) FL(i, j, k) = 1;
+            }
+          }
+    } else {
+      const double thr = (c->ndims == 3 ? 3.0 : 2.0) * qtolerance;
+      for (int k = 1; k <= mz; k++)
+        for (int j = 1; j <= my; j++)
+          for (int i = 1; i <= mx; i++) {
+            double d = fabs(Q(P, i + 1, j, k, 0) - 2.0 * Q(P, i, j, k, 0) + Q(P, i - 1, j, k, 0)) +
+                       fabs(Q(P, i, j + 1, k, 0) - 2.0 * Q(P, i, j, k, 0) + Q(P, i, j - 1, k, 0));
+            if (c->ndims == 3) d = d + fabs(Q(P, i, j, k + 1, 0) - 2.0 * Q(P, i, j, k, 0) + Q(P, i, j, k - 1, 0));
+            FL(i, j, k) = d > thr ? 1 : 0;
+          }
+    }
+#undef FL
+    for (size_t n = 0; n < (size_t)mx * my * mz; n++) nt += fl[n];
+    off += (int64_t)mx * my * mz;
+  }
+  *ntagged = nt;
   return BSAM_OK;
 }
 

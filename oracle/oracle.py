@@ -46,6 +46,7 @@ def load():
             "orc_residual": ([_P, _I], _I),
             "orc_error_afas": ([_P, _I, C.POINTER(_D)], _I),
             "orc_copy_q_to_qold": ([_P, _I], _I),
+            "orc_estimate_level_errors": ([_P, _I, _I, _D, C.POINTER(_I), C.POINTER(C.c_int64)], _I),
             "orc_vcycle": ([_P, _I], _I),
             "orc_multigrid_iterations": ([_P, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D), _I], _I),
             "orc_time_step": ([_P, _I, C.POINTER(_I), C.POINTER(_D)], _I),
@@ -111,6 +112,15 @@ class Oracle:
         r = C.c_double()
         self._chk(self.lib.orc_error_afas(self.ctx, level, C.byref(r)))
         return r.value
+
+    def estimate_level_errors(self, level, errflagopt, qtolerance, ncells):
+        """Flags of the level's cells (patches in level order, each
+        column-major over 1..mx) and the number flagged."""
+        flags = np.zeros(ncells, dtype=np.int32)
+        nt = C.c_int64()
+        self._chk(self.lib.orc_estimate_level_errors(self.ctx, level, errflagopt, qtolerance,
+                                                     flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nt)))
+        return flags, nt.value
 
     def multigrid_iterations(self, history_len=64):
         n = C.c_int32()

@@ -1,0 +1,177 @@
+"""Regrid tagging: EstimateLevelErrors (bsamroutines.f90:994-1040) and the
+tagging of ErrFlag (:1059-1131), before BufferAndList.
+
+The oracle (orc_estimate_level_errors) is pinned by closed forms:
+
+  * errflagopt 10, the second-difference test of the CH and BioGrowth
+    SetErrFlagsUser2D/3D (CH/problem.f90:248-307, BioGrowth/problem.f90:
+    185-240): phi = (gx - 16.5)^3 + (gy - 7)^2 (+ (gz - 3)^2) has second
+    differences 6 (gx - 16.5), 2 and 2 exactly, so the flagged cells are
+    known from the global x index alone.  mu is a large cubic, which would
+    flag every cell if the test read it; the threshold is ndims*qtolerance.
+  * errflagopt 1, RelativeTruncationError (afasroutines.f90:739-814) and
+    SetErrFlags2D/3D (bsamroutines.f90:1134-1203): with phi = a + b gx on
+    the refined patch (ghosts included), phi = a + b (2 GX - 1/2) on its
+    parent and mu = 0, every undivided Laplacian vanishes, so
+    qrte_phi = R(phi) - phi_c = 0 and qrte_mu = R(-phi^3) + phi_c^3 =
+    -3 phi_c (b/2)^2 exactly.  The 2^d children of a coarse cell are flagged
+    iff 3 |phi_c| b^2 / 4 > qtolerance / h^ndims; phi_c changes sign across
+    the patch.
+
+The HIP path (bsam_estimate_level_errors) is compared with the oracle bit
+for bit on the reference's AMR states (flags, and q, qold and qc after the
+errflagopt-1 sequence), for CH, the tumour model and BioGrowth, batched and
+per patch.
+"""
+import numpy as np
+import pytest
+
+import fixture_util as fu
+import oracle
+from bsam_amd import abi, configs, hierarchy
+
+
+def _hier(nd, mthbc):
+    """16^nd root, a 16^nd level-1 patch over coarse cells 5..12."""
+    base = hierarchy.uniform(nd, (16,) * nd, -3, [mthbc] * (2 * nd))
+    return hierarchy.from_patch_boxes(base, [[1, [[9, 24]] * nd]], nd)
+
+
+def _global(d, nd):
+    """Global indices (ghosts included) of a patch, numpy (k, j, i) order."""
+    ax = [d["mglobal"][a][0] - 1 + np.arange(d["mx"][a] + 2) for a in range(nd)]
+    return np.meshgrid(*ax[::-1], indexing="ij")[::-1]
+
+
+def _level_cells(hier, level):
+    ids = [k for k, d in enumerate(hier) if d["level"] == level]
+    return ids, sum(int(np.prod(hier[k]["mx"][:3])) for k in ids)
+
+
+def _split(flags, hier, ids, nd):
+    out, off = {}, 0
+    for k in ids:
+        mx = hier[k]["mx"][:nd]
+        n = int(np.prod(mx))
+        out[k] = flags[off:off + n].reshape(mx[::-1])
+        off += n
+    return out
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_user_flags_closed_form(nd):
+    hier = _hier(nd, 2)
+    O = oracle.Oracle(configs.params(nd, 1e-3, 0.1, 1e-9), hier)
+    for k, d in enumerate(hier):
+        if d["level"] < 0:
+            continue
+        g = _global(d, nd)
+        q = O.field(k, "q")
+        q[0] = (g[0] - 16.5) ** 3 + (g[1] - 7.0) ** 2 + ((g[2] - 3.0) ** 2 if nd == 3 else 0.0)
+        q[1] = 1000.0 * g[0] ** 3
+    # d = 6|gx - 16.5| + 2 (nd - 1) takes the values 6 m + 3 + 2 (nd - 1);
+    # the threshold nd * qtol = 34 falls between them
+    qtol = 34.0 / nd
+    for level in (0, 1):
+        ids, n = _level_cells(hier, level)
+        flags, nt = O.estimate_level_errors(level, 10, qtol, n)
+        got = _split(flags, hier, ids, nd)
+        for k in ids:
+            d = hier[k]
+            gx = _global(d, nd)[0][(slice(1, -1),) * nd]
+            want = (6.0 * np.abs(gx - 16.5) + 2.0 * (nd - 1) > 34.0).astype(np.int32)
+            assert np.array_equal(got[k], want), (nd, level, k)
+        assert nt == int(flags.sum()) and 0 < nt < n
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_truncation_error_flags_closed_form(nd):
+    hier = _hier(nd, 2)
+    O = oracle.Oracle(configs.params(nd, 1e-3, 0.1, 1e-9), hier)
+    a, b = -3.0, 0.25
+    for k, d in enumerate(hier):
+        q = O.field(k, "q")
+        q[...] = 0.0
+        if d["level"] == 1:
+            q[0] = a + b * _global(d, nd)[0]
+        elif d["level"] == 0:
+            q[0] = a + b * (2.0 * _global(d, nd)[0] - 0.5)
+    ids, n = _level_cells(hier, 1)
+    h = hier[ids[0]]["dx"]
+    T = 0.5  # flag iff |phi_c| > T
+    qtol = T * 3.0 * b * b / 4.0 * h ** nd
+    flags, nt = O.estimate_level_errors(1, 1, qtol, n)
+    got = _split(flags, hier, ids, nd)[ids[0]]
+    d = hier[ids[0]]
+    gx = _global(d, nd)[0][(slice(1, -1),) * nd]
+    phic = a + b * (2.0 * ((gx + 1) // 2) - 0.5)
+    want = (np.abs(phic) > T).astype(np.int32)
+    assert np.array_equal(got, want)
+    assert 0 < nt < n and np.any(phic[want == 1] < 0)
+    # the sequence before the error: parent q under the patch is R(q) (the
+    # same linear values), qold = q on both levels
+    for k, dd in enumerate(hier):
+        if dd["level"] in (0, 1):
+            assert np.array_equal(O.get(k, "qold"), O.get(k, "q"))
+
+
+def _fixture_pair(name, gpu_cls, batching):
+    meta, z = fu.load(name)
+    st = meta["steps"][0]
+    p = fu.step_params(meta, batching=batching)
+    O = oracle.Oracle(fu.step_params(meta), st["hierarchy"])
+    S = gpu_cls(p, st["hierarchy"])
+    fu.load_state(O, meta, z, 0)
+    fu.load_state(S, meta, z, 0)
+    return meta, st["hierarchy"], O, S
+
+
+def _bg_pair(gpu_cls, batching):
+    meta, z = fu.load("ch2d_amr")
+    hier = meta["steps"][0]["hierarchy"]
+    kw = dict(model=abi.MODEL_BG, extra=(1.5, 2.0, 0.5))
+    O = oracle.Oracle(configs.params(2, 1e-3, 0.05, 1e-9, **kw), hier)
+    S = gpu_cls(configs.params(2, 1e-3, 0.05, 1e-9, batching=batching, **kw), hier)
+    rng = np.random.default_rng(11)
+    for k, d in enumerate(hier):
+        if d["level"] < 0:
+            continue
+        g = _global(d, 2)
+        q = np.stack([np.tanh((np.hypot(g[0] / 2.0 ** d["level"] - 40, g[1] / 2.0 ** d["level"] - 70) - 20) / 3.0),
+                      0.5 + 0.1 * rng.standard_normal(g[0].shape)])
+        for X in (O, S):
+            X.put(k, "q", q)
+            X.put(k, "qold", q)
+    return hier, O, S
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ch3d_amr", "ch2d_amr_phys", "tm3d_amr_phys", "bg2d"])
+@pytest.mark.parametrize("batching", [1, 0])
+def test_estimate_level_errors_vs_oracle(name, batching, gpu_solver_cls):
+    if name == "bg2d":
+        hier, O, S = _bg_pair(gpu_solver_cls, batching)
+    else:
+        _, hier, O, S = _fixture_pair(name, gpu_solver_cls, batching)
+    lo, hi = hierarchy.levels(hier)
+    nd = 3 if any(d["mx"][2] > 1 for d in hier) else 2
+    nontrivial = 0
+    for level in range(max(lo + 1, 0), hi + 1):
+        for X in (O, S):
+            X.set_ghost(level, 0)
+        ids, n = _level_cells(hier, level)
+        for opt, qtol in ((10, 1e-2), (10, 1e-3), (1, 1e-2), (1, 1e-4)):
+            print(f"[gpu] tagging {name} batching {batching} level {level} opt {opt} qtol {qtol}", flush=True)
+            fo, no = O.estimate_level_errors(level, opt, qtol, n)
+            fs, ns = S.estimate_level_errors(level, opt, qtol, n)
+            assert np.array_equal(fs, fo) and ns == no, (name, level, opt, qtol, int(np.sum(fs != fo)))
+            nontrivial += 0 < no < n
+            if opt == 1:
+                for k, d in enumerate(hier):
+                    if d["level"] in (level, level - 1):
+                        for f in ("q", "qold"):
+                            assert np.array_equal(S.get(k, f), O.get(k, f)), (name, level, k, f)
+                    if d["level"] == level:
+                        assert np.array_equal(S.get(k, "qc"), O.get(k, "qc")), (name, level, k)
+    assert nontrivial > 0
+    S.close()
