@@ -100,6 +100,8 @@ def parse():
     ap.add_argument("--autotune", type=int, default=1,
                     help="time the candidate launch shapes once per hierarchy before the warm-up (bsam_autotune)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--tagging", type=int, default=1,
+                    help="time the regrid tagging entry point on the AMR levels after the timed steps")
     ap.add_argument("--check", type=int, default=1,
                     help="measure the paths' max-norm error against FP64 converged to 1e-12")
     ap.add_argument("--kstat-steps", type=int, default=1,
@@ -255,6 +257,36 @@ def max_err(wl, qa, qb):
     return dict(zip(["phi", "mu", "n", "p"], e))
 
 
+QTOL_TAG = 1e-2  # qtolerance(0:5) of the reference's CH and BioGrowth griddata.dat
+
+
+def tagging(wl, X, reps):
+    """Regrid tagging (bsam_estimate_level_errors, EstimateLevelErrors) on
+    the levels below the finest, as the reference's AMR(level) runs it for
+    level < maxlevel after SetGhost(level, 0), from the initial state, for
+    errflagopt 10 (the CH configuration's) and 1.  Returns per option the
+    mean wall time of the calls over all levels (ms, flags copied to the
+    host included) and the cells flagged."""
+    out = {}
+    levels = list(range(0, wl.finest))
+    for opt in (10, 1):
+        wl.load(X)
+        total, tagged = 0.0, 0
+        for lv in levels:
+            n = sum(d["mx"][0] * d["mx"][1] * d["mx"][2] for d in wl.hier if d["level"] == lv)
+            X.set_ghost(lv, 0)
+            if hasattr(X, "synchronize"):
+                X.synchronize()
+            X.estimate_level_errors(lv, opt, QTOL_TAG, n)  # untimed first call
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _, k = X.estimate_level_errors(lv, opt, QTOL_TAG, n)
+            total += (time.perf_counter() - t0) / reps
+            tagged += k
+        out[str(opt)] = dict(ms=round(1e3 * total, 3), tagged=tagged)
+    return dict(levels=levels, qtolerance=QTOL_TAG, errflagopt=out)
+
+
 def cpu_baseline(wl, vcycles_per_step, ncyc=3):
     """The C oracle (one thread) on the same workload: the first `ncyc`
     V-cycles of the first time step are timed (about 20 s on the 3D AMR
@@ -270,7 +302,12 @@ def cpu_baseline(wl, vcycles_per_step, ncyc=3):
     rc, nc, r = O.time_step(True)
     dt = time.perf_counter() - t0
     per_cycle = dt / max(nc, 1)
+    tag = None
+    if wl.finest > 0:
+        O2 = oracle.Oracle(wl.params("fp64", 0, 1), wl.hier)
+        tag = tagging(wl, O2, 1)
     return dict(value=1.0 / (per_cycle * vcycles_per_step), unit="time-steps/s", cores=1, kind="port",
+                tagging=tag,
                 sample=f"first {nc} V-cycles of time step 1 of the {wl.name} workload ({dt:.2f} s, "
                        f"{1e3 * per_cycle:.1f} ms per V-cycle incl. the step set-up) scaled by "
                        f"{vcycles_per_step:.2f} V-cycles per step, on the C oracle "
@@ -307,6 +344,12 @@ def main():
             check[f"{pr}_vs_converged"] = max_err(wl, res[pr]["q"], qr)
         if len(paths) == 2:
             check["mixed_vs_fp64"] = max_err(wl, res["mixed"]["q"], res["fp64"]["q"])
+    tag = None
+    if a.tagging and wl.finest > 0:
+        print("[bench] regrid tagging", file=sys.stderr, flush=True)
+        T = make_state(wl, "fp64", 0, a.batching)
+        tag = tagging(wl, T, 5)
+        T.close()
     cpu = None
     if a.cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -337,6 +380,7 @@ def main():
         "vcycles_per_s": res[head]["vc"] * a.steps / el,
         "roofline": res[head]["roof"],
         "cpu_baseline": cpu,
+        "regrid_tagging": tag,
     }
     if len(paths) == 2:
         f = res["fp64"]
