@@ -287,6 +287,25 @@ def tagging(wl, X, reps):
     return dict(levels=levels, qtolerance=QTOL_TAG, errflagopt=out)
 
 
+def frame_write(wl, X):
+    """One output frame (WriteQ) of the initial state, written to the null
+    device so that the edit and the copies are timed and not the disk: wall
+    time (s) and the frame's bytes (cell records of levels >= 0; headers
+    are a few hundred bytes per patch)."""
+    try:
+        wl.load(X)
+        if hasattr(X, "synchronize"):
+            X.synchronize()
+        nb = sum((27 * wl.nv - 1) * (d["mx"][0] + 2) * (d["mx"][1] + 2) * ((d["mx"][2] + 2) if wl.nd == 3 else 1)
+                 for d in wl.hier if d["level"] >= 0)
+        t0 = time.perf_counter()
+        X.write_frame(os.devnull, 0.0)
+        dt = time.perf_counter() - t0
+        return dict(s=round(dt, 4), bytes=nb, gb_per_s=round(nb / dt / 1e9, 3), target=os.devnull)
+    except Exception as e:  # a failed frame must not cost the bench line
+        return dict(error=str(e))
+
+
 def cpu_baseline(wl, vcycles_per_step, ncyc=3):
     """The C oracle (one thread) on the same workload: the first `ncyc`
     V-cycles of the first time step are timed (about 20 s on the 3D AMR
@@ -302,12 +321,13 @@ def cpu_baseline(wl, vcycles_per_step, ncyc=3):
     rc, nc, r = O.time_step(True)
     dt = time.perf_counter() - t0
     per_cycle = dt / max(nc, 1)
-    tag = None
+    tag = frame = None
     if wl.finest > 0:
         O2 = oracle.Oracle(wl.params("fp64", 0, 1), wl.hier)
         tag = tagging(wl, O2, 1)
+        frame = frame_write(wl, O2)
     return dict(value=1.0 / (per_cycle * vcycles_per_step), unit="time-steps/s", cores=1, kind="port",
-                tagging=tag,
+                tagging=tag, frame=frame,
                 sample=f"first {nc} V-cycles of time step 1 of the {wl.name} workload ({dt:.2f} s, "
                        f"{1e3 * per_cycle:.1f} ms per V-cycle incl. the step set-up) scaled by "
                        f"{vcycles_per_step:.2f} V-cycles per step, on the C oracle "
@@ -349,6 +369,7 @@ def main():
         print("[bench] regrid tagging", file=sys.stderr, flush=True)
         T = make_state(wl, "fp64", 0, a.batching)
         tag = tagging(wl, T, 5)
+        tag["frame"] = frame_write(wl, T)
         T.close()
     cpu = None
     if a.cpu_baseline:

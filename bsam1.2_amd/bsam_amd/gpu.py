@@ -124,6 +124,12 @@ class Solver:
     def copy_q_to_qold(self, level):
         self._chk(self.lib.bsam_copy_q_to_qold(self.ctx, level))
 
+    def write_frame(self, path, time, geom=None):
+        """WriteQ: the frame of levels 0..finest to `path`; geom per patch
+        (dx[3], xlower[3], xupper[3]) or None."""
+        g = None if geom is None else np.ascontiguousarray(geom, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+        self._chk(self.lib.bsam_write_frame(self.ctx, path.encode(), time, g))
+
     def estimate_level_errors(self, level, errflagopt, qtolerance, ncells):
         """EstimateLevelErrors + ErrFlag tagging: the level's flags (patches
         in descriptor order, each column-major over 1..mx) and the number

@@ -1115,6 +1115,120 @@ __global__ __launch_bounds__(kBlock) void k_zero_i32(int32_t* __restrict__ a, lo
   if (g < n) a[g] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Output frame (WriteQ / OutputQ / WriteQ2D/3D, bsaminputoutput.f90:60-181)
+// ---------------------------------------------------------------------------
+// Fortran F25.12 of x as the amdflang runtime edits it, into o[0..24]: the
+// value correctly rounded to twelve decimals (halfway cases to even), the
+// sign kept when a negative value rounds to zero, 25 asterisks when the
+// edited value needs more than 25 characters, NaN / Inf / -Inf
+// right-justified.  Exact by integer arithmetic on the binary value
+// x = m 2^e: the fraction f / 2^s (s = -e) times 10^12 is formed in 128
+// bits (f < 2^53, so f 10^12 < 2^93) and shifted, the bits shifted out
+// deciding the rounding; no floating-point operation is involved.
+__host__ __device__ inline void f25_edit(double x, char* o) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const bool neg = b >> 63;
+  const int E = (int)((b >> 52) & 0x7ff);
+  uint64_t m = b & ((1ull << 52) - 1);
+  for (int t = 0; t < 25; t++) o[t] = ' ';
+  if (E == 0x7ff) {
+    const char* w = m ? "NaN" : (neg ? "-Inf" : "Inf");
+    int n = 0;
+    while (w[n]) n++;
+    for (int t = 0; t < n; t++) o[25 - n + t] = w[t];
+    return;
+  }
+  int e;
+  if (E == 0) {
+    e = -1074;
+  } else {
+    m |= 1ull << 52;
+    e = E - 1075;
+  }
+  if (e >= 0) {  // |x| >= 2^52: more than 25 characters
+    for (int t = 0; t < 25; t++) o[t] = '*';
+    return;
+  }
+  const int s = -e;
+  uint64_t ip = s >= 64 ? 0 : (m >> s);
+  const uint64_t f = s >= 64 ? m : (m & ((1ull << s) - 1));
+  const unsigned __int128 P = (unsigned __int128)f * 1000000000000ull;
+  uint64_t q = 0;
+  if (s < 128) {
+    q = (uint64_t)(P >> s);
+    const unsigned __int128 r = P & ((((unsigned __int128)1) << s) - 1);
+    const unsigned __int128 half = ((unsigned __int128)1) << (s - 1);
+    if (r > half || (r == half && (q & 1))) {
+      if (++q == 1000000000000ull) {
+        q = 0;
+        ip++;
+      }
+    }
+  }
+  char d[32];
+  int n = 0;
+  for (int t = 0; t < 12; t++) {
+    d[n++] = (char)('0' + q % 10);
+    q /= 10;
+  }
+  d[n++] = '.';
+  do {
+    d[n++] = (char)('0' + ip % 10);
+    ip /= 10;
+  } while (ip && n < 30);
+  if (neg) d[n++] = '-';
+  if (n > 25 || ip) {
+    for (int t = 0; t < 25; t++) o[t] = '*';
+    return;
+  }
+  for (int t = 0; t < n; t++) o[24 - t] = d[t];
+}
+
+// One patch's share of a frame chunk: its first cell in the chunk's flat
+// cell range and the byte where its body starts.
+struct FramePatch {
+  int pid;
+  long cell0, byte0;
+};
+
+// The body of WriteQ2D/3D (bsaminputoutput.f90:132-181) for the patches of
+// a chunk: cell (i, j, k) of q(0:mx+1, ...), i fastest, is one record of
+// its nv values edited 10(F25.12,2X) -- fields of 25 characters two blanks
+// apart, no trailing blanks -- at a fixed length of 27 nv - 1 bytes with
+// the newline.  Domain: the cells of the chunk.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_frame_body(const DPatch* __restrict__ pt,
+                                                       const FramePatch* __restrict__ fp, int np, long ncell,
+                                                       int nv, const double* __restrict__ q,
+                                                       char* __restrict__ out) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ncell) return;
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (fp[mid].cell0 <= g) lo = mid; else hi = mid;
+  }
+  const FramePatch F = fp[lo];
+  const DPatch P = pt[F.pid];
+  const long c = g - F.cell0;
+  const int i = (int)(c % P.nq[0]);
+  const long r = c / P.nq[0];
+  const int j = (int)(r % P.nq[1]);
+  const int k = (DIM == 3) ? (int)(r / P.nq[1]) : 0;
+  const int len = 27 * nv - 1;
+  char* o = out + F.byte0 + c * len;
+  const int base = qidx(P, i, j, k);
+  for (int v = 0; v < nv; v++) {
+    f25_edit(q[base + v * P.qvs], o + 27 * v);
+    if (v < nv - 1) {
+      o[27 * v + 25] = ' ';
+      o[27 * v + 26] = ' ';
+    }
+  }
+  o[len - 1] = '\n';
+}
+
 // CorrectFine, first half (afasroutines.f90:892-895, 934-938):
 // qc(0:cmx+1) = parent%q(mb-1:mb+1) - qc.  Domain: the qc box.
 template <int DIM, typename T>
@@ -4855,6 +4969,119 @@ extern "C" int32_t bsam_copy_q_to_qold(bsam_ctx* c, int32_t level) {
     }
   }
   return BSAM_OK;
+}
+
+// Iw edit: right-justified in w characters, w asterisks when it does not fit.
+static std::string frame_int(int v, int w) {
+  char b[32];
+  snprintf(b, sizeof b, "%*d", w, v);
+  std::string t(b);
+  return (int)t.size() > w ? std::string(w, '*') : t;
+}
+static std::string frame_f25(double x) {
+  char b[25];
+  f25_edit(x, b);
+  return std::string(b, 25);
+}
+
+// WriteQ (bsaminputoutput.f90:60-82) with OutputQ (:85-129) and
+// WriteQ2D/3D (:132-181): the header is edited on the host, the patch
+// bodies on the device in chunks of whole patches (at most 256 MB, or one
+// patch), copied back through pinned memory and written in level order.
+extern "C" int32_t bsam_write_frame(bsam_ctx* c, const char* path, double time, const double* geom) {
+  if (!c || !path) return set_err(BSAM_EINVAL, "null argument");
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
+  const int nd = c->p.ndims, nv = c->p.nrvars;
+  const long len = 27L * nv - 1;
+  std::vector<int> order;  // patches of levels 0..finest in level order
+  for (int level = 0; level <= c->finest; level++)
+    for (int pid : c->L(level).patches) order.push_back(pid);
+  auto cells = [&](int pid) {
+    const DPatch& P = c->hp[pid];
+    return (long)P.nq[0] * P.nq[1] * (nd == 3 ? P.nq[2] : 1);
+  };
+  long cap = 256L << 20;
+  for (int pid : order) cap = std::max(cap, cells(pid) * len);
+  FILE* fh = fopen(path, "wb");
+  if (!fh) return set_err(BSAM_EINVAL, "cannot open %s for writing", path);
+  char* d_out = nullptr;
+  char* h_out = nullptr;
+  FramePatch* d_fp = nullptr;
+  int32_t rc = BSAM_OK;
+  auto fail = [&](int32_t r) {
+    rc = r;
+    return r;
+  };
+  do {
+    if (hipMalloc(&d_out, cap) != hipSuccess || hipHostMalloc(&h_out, cap) != hipSuccess ||
+        hipMalloc(&d_fp, order.size() * sizeof(FramePatch)) != hipSuccess) {
+      fail(set_err(BSAM_EHIP, "frame buffers of %ld bytes could not be allocated", cap));
+      break;
+    }
+    std::string head = frame_f25(time) + "\n" + frame_int(c->finest, 3) + "\n";
+    if (fwrite(head.data(), 1, head.size(), fh) != head.size()) {
+      fail(set_err(BSAM_EINVAL, "cannot write %s", path));
+      break;
+    }
+    for (size_t a0 = 0; a0 < order.size() && rc == BSAM_OK;) {
+      // one chunk of whole patches
+      std::vector<FramePatch> fp;
+      long ncell = 0, nbyte = 0;
+      size_t a1 = a0;
+      while (a1 < order.size() && (fp.empty() || nbyte + cells(order[a1]) * len <= cap)) {
+        fp.push_back(FramePatch{order[a1], ncell, nbyte});
+        ncell += cells(order[a1]);
+        nbyte += cells(order[a1]) * len;
+        a1++;
+      }
+      if (hipMemcpyAsync(d_fp, fp.data(), fp.size() * sizeof(FramePatch), hipMemcpyHostToDevice, c->s) !=
+          hipSuccess) {
+        fail(set_err(BSAM_EHIP, "frame: patch table upload failed"));
+        break;
+      }
+      if (nd == 2) k_frame_body<2><<<nblocks(ncell), kBlock, 0, c->s>>>(c->d_pt, d_fp, (int)fp.size(), ncell, nv, c->q, d_out);
+      else k_frame_body<3><<<nblocks(ncell), kBlock, 0, c->s>>>(c->d_pt, d_fp, (int)fp.size(), ncell, nv, c->q, d_out);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(h_out, d_out, nbyte, hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+          hipStreamSynchronize(c->s) != hipSuccess) {
+        fail(set_err(BSAM_EHIP, "frame: body kernel or copy failed"));
+        break;
+      }
+      for (const FramePatch& F : fp) {
+        const int pid = F.pid;
+        const DPatch& P = c->hp[pid];
+        const bsam_patch_desc& D = c->desc[pid];
+        double g[9];
+        for (int x = 0; x < 3; x++) {
+          g[x] = geom ? geom[9L * pid + x] : D.dx;
+          g[3 + x] = geom ? geom[9L * pid + 3 + x] : D.xlower[x];
+          g[6 + x] = geom ? geom[9L * pid + 6 + x] : D.xlower[x] + P.mx[x] * D.dx;
+        }
+        std::string h = "  \n" + frame_int(D.level, 3) + " " + frame_int(nd, 3) + " " + frame_int(2, 3) + " " +
+                        frame_int(nv, 3) + "\n";
+        for (int row = 0; row < 3; row++) {
+          for (int x = 0; x < nd; x++) h += (x ? " " : "") + frame_f25(g[3 * row + x]);
+          h += "\n";
+        }
+        for (int x = 0; x < nd; x++) h += (x ? " " : "") + frame_int(P.mx[x], 8);
+        h += "\n";
+        for (int x = 0; x < nd; x++)
+          for (int e = 0; e < 2; e++) h += ((x || e) ? " " : "") + frame_int(D.mglobal[x][e], 8);
+        h += "\n";
+        const size_t nb = (size_t)(cells(pid) * len);
+        if (fwrite(h.data(), 1, h.size(), fh) != h.size() || fwrite(h_out + F.byte0, 1, nb, fh) != nb) {
+          fail(set_err(BSAM_EINVAL, "cannot write %s", path));
+          break;
+        }
+      }
+      a0 = a1;
+    }
+  } while (false);
+  hipFree(d_out);
+  hipFree(d_fp);
+  if (h_out) hipHostFree(h_out);
+  if (fclose(fh) != 0 && rc == BSAM_OK) rc = set_err(BSAM_EINVAL, "cannot write %s", path);
+  return rc;
 }
 
 // EstimateLevelErrors (bsamroutines.f90:994-1040) and the tagging of

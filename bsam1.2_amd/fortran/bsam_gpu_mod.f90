@@ -65,7 +65,7 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_coarse_ghost_points, bsam_coarse_loading, bsam_correct_fine, &
          bsam_fill_down, bsam_get_source, bsam_update_source, bsam_update_aux, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
-         bsam_estimate_level_errors, &
+         bsam_estimate_level_errors, bsam_write_frame, &
          bsam_multigrid_iterations, bsam_time_step, bsam_synchronize, &
          bsam_autotune, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
@@ -193,6 +193,15 @@ INTERFACE
     REAL(C_DOUBLE), VALUE:: qtolerance
     INTEGER(C_INT32_T), DIMENSION(*), INTENT(OUT):: flags
     INTEGER(C_INT64_T), INTENT(OUT):: ntagged
+  END FUNCTION
+  ! WriteQ (bsaminputoutput.f90:60-181) to path (a NUL-terminated string);
+  ! geom: dx(1:3), xlower(1:3), xupper(1:3) per descriptor
+  INTEGER(C_INT32_T) FUNCTION bsam_write_frame(ctx, path, time, geom) BIND(C, NAME='bsam_write_frame')
+    IMPORT:: C_INT32_T, C_DOUBLE, C_PTR, C_CHAR
+    TYPE(C_PTR), VALUE:: ctx
+    CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN):: path
+    REAL(C_DOUBLE), VALUE:: time
+    REAL(C_DOUBLE), DIMENSION(*), INTENT(IN):: geom
   END FUNCTION
   INTEGER(C_INT32_T) FUNCTION bsam_vcycle(ctx, level) BIND(C, NAME='bsam_vcycle')
     IMPORT:: C_INT32_T, C_PTR

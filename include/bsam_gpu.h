@@ -22,7 +22,8 @@
  *     on the calling thread.
  *   - one host thread drives one context; calls are stream-ordered on the
  *     context's HIP stream.  Only bsam_error_afas, bsam_multigrid_iterations,
- *     bsam_download and bsam_synchronize wait for the device.
+ *     bsam_download, bsam_estimate_level_errors, bsam_write_frame and
+ *     bsam_synchronize wait for the device.
  *   - the library keeps no host pointer past the call that received it.
  */
 #ifndef BSAM_GPU_H
@@ -197,6 +198,15 @@ int32_t bsam_copy_q_to_qold(bsam_ctx* ctx, int32_t level);
  * number of flagged cells.  level must be above minlevel.  Synchronous. */
 int32_t bsam_estimate_level_errors(bsam_ctx* ctx, int32_t level, int32_t errflagopt, double qtolerance,
                                    int32_t* flags, int64_t* ntagged);
+/* Output frame: WriteQ (bsaminputoutput.f90:60-82) with OutputQ (:85-129)
+ * and WriteQ2D/3D (:132-181), written to `path` (the reference's
+ * './OUT/m%05d.dat' is the caller's choice) byte for byte as the reference
+ * writes it: time, finestlevel, then every patch of levels 0..finest in
+ * descriptor order with its header and q(0:mx+1, ...), values edited F25.12
+ * as the Fortran runtime does.  geom holds per descriptor dx[3], xlower[3],
+ * xupper[3] as nodeinfo has them (nodeinfodef.f90:118-220); NULL takes dx
+ * and xlower from the descriptors and xupper = xlower + mx*dx.  Synchronous. */
+int32_t bsam_write_frame(bsam_ctx* ctx, const char* path, double time, const double* geom);
 
 /* ---- coarse-grained entry points --------------------------------------- */
 /* AFASVcycle(level) (afasroutines.f90:495-529). */

@@ -1283,3 +1283,103 @@ int32_t orc_time_step(orc_ctx* c, int32_t first_step, int32_t* ncycles, double* 
   for (int level = c->minlevel; level <= c->finestlevel; level++) orc_copy_q_to_qold(c, level);
   return orc_multigrid_iterations(c, ncycles, residual, NULL, 0);
 }
+
+/* ---- output frame ------------------------------------------------------ */
+/* Fortran F25.12 as the amdflang runtime edits it: correctly rounded
+ * (glibc's %f is exact, halfway cases to even), the sign kept when a
+ * negative value rounds to zero, 25 asterisks when the edited value needs
+ * more than 25 characters, NaN / Inf / -Inf right-justified. */
+static void f25(char* o, double x) {
+  char b[512];
+  if (isnan(x)) snprintf(b, sizeof b, "%25s", "NaN");
+  else if (isinf(x)) snprintf(b, sizeof b, "%25s", x < 0 ? "-Inf" : "Inf");
+  else snprintf(b, sizeof b, "%25.12f", x);
+  if (strlen(b) > 25) memset(b, '*', 25);
+  memcpy(o, b, 25);
+  o[25] = 0;
+}
+/* Iw: right-justified, w asterisks when it does not fit. */
+static void iw(char* o, int w, int v) {
+  char b[32];
+  snprintf(b, sizeof b, "%*d", w, v);
+  if ((int)strlen(b) > w) memset(b, '*', w);
+  b[w] = 0;
+  strcpy(o, b);
+}
+
+/* WriteQ (bsaminputoutput.f90:60-82) with OutputQ (:85-129) and
+ * WriteQ2D/3D (:132-181): time (F25.12), finestlevel (I3), then every patch
+ * of levels 0..finest in level order: a list-directed blank line, level,
+ * ndims, 2, nrvars (I3,3(1x,I3)), dx, xlower, xupper (F25.12 with 1x),
+ * mx and mglobal (I8 with 1X), and q(0:mx+1, 0:my+1[, 0:mz+1]) i fastest,
+ * one cell per record with its nrvars values edited 10(F25.12,2X) (no
+ * trailing blanks).  geom: per patch dx[3], xlower[3], xupper[3]; NULL
+ * takes dx and xlower from the descriptors and xupper = xlower + mx*dx. */
+int32_t orc_write_frame(orc_ctx* c, const char* path, double time, const double* geom) {
+  FILE* fh = fopen(path, "wb");
+  if (!fh) return fail("cannot open %s", path);
+  char t[64], u[64];
+  f25(t, time);
+  fprintf(fh, "%s\n", t);
+  iw(t, 3, c->finestlevel);
+  fprintf(fh, "%s\n", t);
+  const int nd = c->ndims, nv = c->nv;
+  for (int level = 0; level <= c->finestlevel; level++) {
+    FOR_LEVEL(c, level, P) {
+      const int pid = (int)(P - c->pt);
+      double g[9];
+      for (int a = 0; a < 3; a++) {
+        g[a] = geom ? geom[9 * pid + a] : P->d.dx;
+        g[3 + a] = geom ? geom[9 * pid + 3 + a] : P->d.xlower[a];
+        g[6 + a] = geom ? geom[9 * pid + 6 + a] : P->d.xlower[a] + P->mx[a] * P->d.dx;
+      }
+      fprintf(fh, "  \n");
+      int hv[4] = {level, nd, 2, nv};
+      for (int n = 0; n < 4; n++) {
+        iw(t, 3, hv[n]);
+        fprintf(fh, n ? " %s" : "%s", t);
+      }
+      fprintf(fh, "\n");
+      for (int row = 0; row < 3; row++) {
+        for (int a = 0; a < nd; a++) {
+          f25(u, g[3 * row + a]);
+          fprintf(fh, a ? " %s" : "%s", u);
+        }
+        fprintf(fh, "\n");
+      }
+      for (int a = 0; a < nd; a++) {
+        iw(t, 8, P->mx[a]);
+        fprintf(fh, a ? " %s" : "%s", t);
+      }
+      fprintf(fh, "\n");
+      for (int a = 0; a < nd; a++)
+        for (int e = 0; e < 2; e++) {
+          iw(t, 8, P->d.mglobal[a][e]);
+          fprintf(fh, (a || e) ? " %s" : "%s", t);
+        }
+      fprintf(fh, "\n");
+      const int k1 = nd == 3 ? P->mx[2] + 1 : 1, k0 = nd == 3 ? 0 : 1;
+      for (int k = k0; k <= k1; k++)
+        for (int j = 0; j <= P->mx[1] + 1; j++)
+          for (int i = 0; i <= P->mx[0] + 1; i++) {
+            for (int v = 0; v < nv; v++) {
+              f25(u, Q(P, i, j, k, v));
+              fprintf(fh, v ? "  %s" : "%s", u);
+            }
+            fprintf(fh, "\n");
+          }
+    }
+  }
+  if (fclose(fh)) return fail("cannot write %s", path);
+  return BSAM_OK;
+}
+
+/* F25.12 of n values into out (25 bytes each): the oracle's edit, for
+ * checking it against the Fortran runtime's. */
+void orc_format_f25(const double* x, int64_t n, char* out) {
+  char t[32];
+  for (int64_t e = 0; e < n; e++) {
+    f25(t, x[e]);
+    memcpy(out + 25 * e, t, 25);
+  }
+}

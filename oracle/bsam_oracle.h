@@ -51,6 +51,8 @@ int32_t  orc_update_source(orc_ctx* c, int32_t level);
 int32_t  orc_residual(orc_ctx* c, int32_t level);
 int32_t  orc_error_afas(orc_ctx* c, int32_t level, double* norm);
 int32_t  orc_copy_q_to_qold(orc_ctx* c, int32_t level);
+int32_t  orc_write_frame(orc_ctx* c, const char* path, double time, const double* geom);
+void     orc_format_f25(const double* x, int64_t n, char* out);
 int32_t  orc_estimate_level_errors(orc_ctx* c, int32_t level, int32_t errflagopt, double qtolerance,
                                    int32_t* flags, int64_t* ntagged);
 int32_t  orc_vcycle(orc_ctx* c, int32_t level);

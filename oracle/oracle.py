@@ -46,6 +46,8 @@ def load():
             "orc_residual": ([_P, _I], _I),
             "orc_error_afas": ([_P, _I, C.POINTER(_D)], _I),
             "orc_copy_q_to_qold": ([_P, _I], _I),
+            "orc_write_frame": ([_P, C.c_char_p, _D, C.POINTER(_D)], _I),
+            "orc_format_f25": ([C.POINTER(_D), C.c_int64, C.c_char_p], None),
             "orc_estimate_level_errors": ([_P, _I, _I, _D, C.POINTER(_I), C.POINTER(C.c_int64)], _I),
             "orc_vcycle": ([_P, _I], _I),
             "orc_multigrid_iterations": ([_P, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D), _I], _I),
@@ -113,6 +115,10 @@ class Oracle:
         self._chk(self.lib.orc_error_afas(self.ctx, level, C.byref(r)))
         return r.value
 
+    def write_frame(self, path, time, geom=None):
+        g = None if geom is None else np.ascontiguousarray(geom, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+        self._chk(self.lib.orc_write_frame(self.ctx, path.encode(), time, g))
+
     def estimate_level_errors(self, level, errflagopt, qtolerance, ncells):
         """Flags of the level's cells (patches in level order, each
         column-major over 1..mx) and the number flagged."""
@@ -134,3 +140,11 @@ class Oracle:
         r = C.c_double()
         rc = self._chk(self.lib.orc_time_step(self.ctx, 1 if first else 0, C.byref(n), C.byref(r)))
         return rc, n.value, r.value
+
+
+def format_f25(values):
+    """The oracle's F25.12 edit of each value (25 bytes each)."""
+    x = np.ascontiguousarray(values, dtype=np.float64)
+    buf = C.create_string_buffer(25 * len(x) + 1)
+    load().orc_format_f25(x.ctypes.data_as(C.POINTER(C.c_double)), len(x), buf)
+    return buf.raw[:25 * len(x)]
