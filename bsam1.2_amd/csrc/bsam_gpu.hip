@@ -2658,8 +2658,10 @@ struct Level {
   // [roff[s][p], roff[s][p+1])
   GRule* d_rules = nullptr;
   std::vector<int> roff[3];
-  // the same ghost fill flattened to one rule per ghost cell (levels > 0,
-  // per-level launches): gmap[ipass] is SetGhost(level, ipass) in full,
+  // the same ghost fill flattened to one rule per ghost cell (per-level
+  // launches; not for a single patch periodic in every dimension at or
+  // below the root, which keeps its wrap kernel, nor where the symbolic
+  // run of the stages refuses): gmap[ipass] is SetGhost(level, ipass) in full,
   // gmap[2 + ipass] (ipass 1, 2) and gmap[5] (ipass 0) the ghosts a red-black sweep
   // sequence needs, gmap[6] those of gmap[5] a correction does not leave
   // exact (see plan_ghost_maps)
