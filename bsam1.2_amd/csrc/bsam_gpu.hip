@@ -71,6 +71,11 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kNvMax = 4;  // fields of the largest model (the tumour model)
 constexpr int kSeqBoxElems = 8192;  // box values up to which a level's stages run in one workgroup
+// Largest level (cells) at which the tumour model's one-workgroup coarse
+// chain starts.  On the 3D tumour AMR mesh (64^3 root) starting at 8^3 ran
+// 65.6 (mixed) / 54.3 (FP64) time-steps/s, at 16^3 64.8 / 53.1 and at 4^3
+// 64.6 / 53.6: the 16^3 level's sweeps are faster as level launches.
+constexpr long kTmChainCells = 512;
 
 // Interpolation constants, boundary.f90:54-63.
 __constant__ double cA[3] = {-1.0 / 5.0, 2.0 / 3.0, 8.0 / 15.0};
@@ -4059,7 +4064,8 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
       c->L(top).chain = top - c->minlevel + 1;
   } else if (c->p.model == BSAM_MODEL_TUMOUR) {
     // the tumour model: levels of one patch each whose fill is a short
-    // sequence of box stages that one workgroup runs in order
+    // sequence of box stages that one workgroup runs in order, from the
+    // highest such level of at most kTmChainCells cells down
     int top = c->minlevel - 1;
     for (int l = c->minlevel; l <= c->finest && l <= 0; l++) {
       const Level& L = c->L(l);
@@ -4069,7 +4075,7 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
       int maxel = 0;
       for (const Stage& st : L.stages) maxel = std::max(maxel, st.maxel);
       const DPatch& P = c->hp[L.patches[0]];
-      if (maxel * c->p.nrvars > kSeqBoxElems || (long)P.mx[0] * P.mx[1] * P.mx[2] > 4096) break;
+      if (maxel * c->p.nrvars > kSeqBoxElems || (long)P.mx[0] * P.mx[1] * P.mx[2] > kTmChainCells) break;
       top = l;
     }
     if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= kMaxChain)
