@@ -267,6 +267,13 @@ def tagging(wl, X, reps):
     errflagopt 10 (the CH configuration's) and 1.  Returns per option the
     mean wall time of the calls over all levels (ms, flags copied to the
     host included) and the cells flagged."""
+    try:
+        return _tagging(wl, X, reps)
+    except Exception as e:  # a failed measurement must not cost the bench line
+        return dict(error=str(e))
+
+
+def _tagging(wl, X, reps):
     out = {}
     levels = list(range(0, wl.finest))
     for opt in (10, 1):
