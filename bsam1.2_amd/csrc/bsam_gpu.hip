@@ -4979,6 +4979,14 @@ extern "C" int32_t bsam_copy_q_to_qold(bsam_ctx* c, int32_t level) {
   return BSAM_OK;
 }
 
+// The frame writer's F25.12 edit of n values on the host (25 bytes each):
+// the same function the device runs, for checking without a GPU.
+extern "C" int32_t bsam_selftest_f25(const double* x, int64_t n, char* out) {
+  if (!x || !out || n < 0) return set_err(BSAM_EINVAL, "null argument");
+  for (int64_t e = 0; e < n; e++) f25_edit(x[e], out + 25 * e);
+  return BSAM_OK;
+}
+
 // Iw edit: right-justified in w characters, w asterisks when it does not fit.
 static std::string frame_int(int v, int w) {
   char b[32];

@@ -207,6 +207,10 @@ int32_t bsam_estimate_level_errors(bsam_ctx* ctx, int32_t level, int32_t errflag
  * xupper[3] as nodeinfo has them (nodeinfodef.f90:118-220); NULL takes dx
  * and xlower from the descriptors and xupper = xlower + mx*dx.  Synchronous. */
 int32_t bsam_write_frame(bsam_ctx* ctx, const char* path, double time, const double* geom);
+/* Host self-test of the frame writer's F25.12 edit (no counterpart in the
+ * reference): out receives 25 bytes per value, as bsam_write_frame writes
+ * them; no GPU is used. */
+int32_t bsam_selftest_f25(const double* x, int64_t n, char* out);
 
 /* ---- coarse-grained entry points --------------------------------------- */
 /* AFASVcycle(level) (afasroutines.f90:495-529). */

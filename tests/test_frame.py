@@ -7,7 +7,8 @@ hook dumped it (nothing changes q in between), and the Fortran runtime's
 F25.12 edit of a list of edge values (halfway cases at the twelfth
 decimal, signed zeros, tiny negatives, overflowing and non-finite values).
 
-  * the oracle's F25.12 edit equals the runtime's on every edge value;
+  * the oracle's F25.12 edit equals the runtime's on every edge value, and
+    the library's own edit (run on the host) equals both;
   * the oracle's frame (orc_write_frame) equals the reference's frame byte
     for byte, 2D and 3D, CH and the tumour model (four fields);
   * the HIP writer (bsam_write_frame) equals the reference's frame byte
@@ -80,6 +81,31 @@ def test_oracle_f25_matches_fortran_runtime():
     bad = [(float(v), got[25 * i:25 * i + 25], want[25 * i:25 * i + 25]) for i, v in enumerate(z["values"])
            if got[25 * i:25 * i + 25] != want[25 * i:25 * i + 25]]
     assert not bad, bad[:5]
+
+
+def test_library_f25_edit_on_the_host():
+    """The library's own F25.12 edit (f25_edit, the function the frame
+    kernel runs, here on the host through bsam_selftest_f25) against the
+    Fortran runtime's on the edge values and against the oracle on 200 000
+    values of every magnitude, halfway cases at the twelfth decimal and
+    arbitrary bit patterns."""
+    import ctypes as C
+    from bsam_amd import abi, gpu
+    lib = gpu.load()
+
+    def edit(v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        buf = C.create_string_buffer(25 * len(v) + 1)
+        assert lib.bsam_selftest_f25(v.ctypes.data_as(C.POINTER(C.c_double)), len(v), buf) == abi.BSAM_OK
+        return buf.raw[:25 * len(v)]
+
+    z = np.load(os.path.join(fu.GOLDEN, "f25_12.npz"))
+    assert edit(z["values"]) == bytes(z["text"])
+    rng = np.random.default_rng(17)
+    v = np.concatenate([rng.uniform(-1, 1, 100000) * 10.0 ** rng.uniform(-20, 16, 100000),
+                        np.arange(-20000, 20001) * 2.0 ** -13,
+                        rng.integers(0, 2 ** 63, 60000, dtype=np.int64).view(np.float64)])
+    assert edit(v) == oracle.format_f25(v)
 
 
 @pytest.mark.parametrize("name", CASES)
