@@ -16,7 +16,11 @@
  *   - closed forms: linear fields reproduced by every ghost interpolation
  *     and by prolongation, Fourier eigenvalues of the undivided Laplacian;
  *   - invariants: a relaxation sweep is stationary at f = L(q); mass
- *     conservation of a converged CH step on a periodic mesh.
+ *     conservation of a converged CH step on a periodic mesh;
+ *   - regrid tagging (orc_estimate_level_errors): closed forms of the
+ *     second-difference test and of the truncation error on linear fields;
+ *   - output frame (orc_write_frame): the reference's own frames byte for
+ *     byte and the Fortran runtime's F25.12 edit of edge values.
  */
 #ifndef BSAM_ORACLE_H
 #define BSAM_ORACLE_H
