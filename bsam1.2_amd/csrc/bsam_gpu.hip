@@ -427,7 +427,51 @@ __device__ __forceinline__ void relax_tm_cell(const DPatch& P, int i, int j, int
 // by RelaxGrid's Newton step with dt*lamp*t0*n in its first right-hand
 // side, then the pressure; t0 = (1 + phi_old)/2.  ModelK: lamp in `gam`,
 // kappa in `cz`.  Domain: cells of colour c, (mx+1)/2 per row.
-template <int DIM>
+template <typename T>
+__host__ __device__ __forceinline__ T box_value(int op, int v, int homog, T x);
+
+// The ghost images of interior cell (i, j, k) of a single patch with
+// condition 10 on every face: every ghost cell whose mirrored interior
+// cell (0 -> 1, mx+1 -> mx per dimension) is this one, set to the face
+// operation applied once per dimension in which it lies outside, x first --
+// the value SetPhysicalBC's staged faces give it.  A sweep that calls this
+// for every cell it updates leaves every ghost as a full SetGhost would,
+// provided the ghosts were consistent before: a ghost whose mirror cell the
+// sweep did not touch still holds the image of its unchanged value.
+template <int DIM, typename T>
+__device__ __forceinline__ void phys_cell_images(const DPatch& P, int op, int i, int j, int k, int nv, int homog,
+                                                 T* __restrict__ q) {
+  int opt[3][3], no[3];
+  const int c3[3] = {i, j, k};
+  for (int a = 0; a < 3; a++) {
+    no[a] = 1;
+    opt[a][0] = c3[a];
+    if (a < DIM) {
+      if (c3[a] == 1) opt[a][no[a]++] = 0;
+      if (c3[a] == P.mx[a]) opt[a][no[a]++] = P.mx[a] + 1;
+    }
+  }
+  if (no[0] * no[1] * no[2] == 1) return;
+  const int src = qidx(P, i, j, k);
+  for (int z = 0; z < no[2]; z++)
+    for (int y = 0; y < no[1]; y++)
+      for (int x = 0; x < no[0]; x++) {
+        if (x == 0 && y == 0 && z == 0) continue;
+        const int d = qidx(P, opt[0][x], opt[1][y], opt[2][z]);
+        for (int v = 0; v < nv; v++) {
+          T w = q[src + v * P.qvs];
+          if (x) w = box_value<T>(op, v, homog, w);
+          if (y) w = box_value<T>(op, v, homog, w);
+          if (z) w = box_value<T>(op, v, homog, w);
+          q[d + v * P.qvs] = w;
+        }
+      }
+}
+
+// IMG: the patch is a single one with condition 10 on every face, and the
+// sweep also writes the ghost images of the cells it updates
+// (phys_cell_images), so no SetGhost need follow.
+template <int DIM, bool IMG = false>
 __global__ __launch_bounds__(kBlock) void k_relax_tm(LevelArgs L, int c, ModelK m, double* __restrict__ q,
                                                      const double* __restrict__ qold,
                                                      const double* __restrict__ f) {
@@ -443,6 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_tm(LevelArgs L, int c, ModelK 
   const int i = 2 * t + 2 - ib;
   if (i > P.mx[0]) return;
   relax_tm_cell<DIM>(P, i, j, k, m, q, qold, f);
+  if (IMG) phys_cell_images<DIM, double>(P, kTumourBC, i, j, k, 4, 0, q);
 }
 
 
@@ -2053,7 +2098,7 @@ __device__ __forceinline__ void relax_lin_tm_cell(const DPatch& P, int i, int j,
 // The tumour model's correction smoother: the pointwise update of
 // k_relax_tm (nutrient first, then the 2x2 phi/mu solve, then pressure) for
 // the operator A above.  EMODE as in k_relax_lin, over all four variables.
-template <int DIM, int EMODE = 0>
+template <int DIM, int EMODE = 0, bool IMG = false>
 __global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, ModelK m, float* __restrict__ e,
                                                          const float* __restrict__ rhs,
                                                          const float* __restrict__ coef,
@@ -2070,6 +2115,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin_tm(LevelArgs L, int c, Mod
   const int i = 2 * t + 2 - ib;
   if (i > P.mx[0]) return;
   relax_lin_tm_cell<DIM, EMODE>(P, i, j, k, 2 * t + 1 + ib, m, e, rhs, coef, q64);
+  if (IMG) phys_cell_images<DIM, float>(P, kTumourBC, i, j, k, 4, 1, e);
 }
 
 
@@ -4428,7 +4474,23 @@ extern "C" int32_t bsam_download(bsam_ctx* c, int32_t patch, int32_t field, doub
     if ((nd) == 2) { KCALL2; } else { KCALL3; } \
   } while (0)
 
+// A tumour level that is one patch with condition 10 on every face: its
+// sweeps write their ghost images (phys_cell_images) instead of a SetGhost
+// after each colour.
+static bool phys_images_level(const bsam_ctx* c, const Level& L) {
+  if (c->p.model != BSAM_MODEL_TUMOUR || L.patches.size() != 1 || L.wrap_pid >= 0 || L.d_rules ||
+      (int)L.stages.size() != c->p.ndims)
+    return false;
+  for (int f = 0; f < 2 * c->p.ndims; f++)
+    if (c->desc[L.patches[0]].mthbc[f] != 10) return false;
+  return true;
+}
+
+static int32_t relax_level_impl(bsam_ctx* c, int32_t level, int32_t ipass, bool img);
 extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
+  return relax_level_impl(c, level, ipass, false);
+}
+static int32_t relax_level_impl(bsam_ctx* c, int32_t level, int32_t ipass, bool img) {
   RETCHK(check_level(c, level));
   if (ipass != 1 && ipass != 2) return set_err(BSAM_EINVAL, "ipass must be 1 or 2");
   Level& L = c->L(level);
@@ -4443,8 +4505,12 @@ extern "C" int32_t bsam_relax_level(bsam_ctx* c, int32_t level, int32_t ipass) {
       else
         k_relax_ch<3, false><<<nb, kBlock, 0, c->s>>>(a, col, c->p.dt, eps2, L.h, c->q, c->f);
     } else if (c->p.model == BSAM_MODEL_TUMOUR) {
-      DISPATCH_DIM(c->p.ndims, (k_relax_tm<2><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)),
-                   (k_relax_tm<3><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)));
+      if (img)
+        DISPATCH_DIM(c->p.ndims, (k_relax_tm<2, true><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)),
+                     (k_relax_tm<3, true><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)));
+      else
+        DISPATCH_DIM(c->p.ndims, (k_relax_tm<2><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)),
+                     (k_relax_tm<3><<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), c->q, c->qold, c->f)));
     } else {
       k_relax_bg2d<<<nb, kBlock, 0, c->s>>>(a, col, c->mk(), L.h, c->q, c->qold, c->f);
     }
@@ -4616,6 +4682,7 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass, bool lean) {
   const bool wrap = L.wrap_pid >= 0;
   int nblk = 0;
   const int kc = (c->p.model == BSAM_MODEL_CH) ? sweep_kc(c, L, 8, &nblk) : 0;
+  if (phys_images_level(c, L)) return relax_level_impl(c, level, ipass, true);  // no SetGhost
   if (!kc && !(wrap && c->p.model == BSAM_MODEL_CH)) {
     RETCHK(bsam_relax_level(c, level, ipass));
     return set_ghost_t<double>(c, level, ipass, c->q, 0, lean ? kLean : kFull);
@@ -5251,7 +5318,11 @@ extern "C" int32_t bsam_vcycle(bsam_ctx* c, int32_t level) {
     RETCHK(correct_t<double>(c, level, c->q, c->qc, cw));
     // SetGhost(level, 0); the post-smoothing sweeps that follow read only
     // the reduced table's ghosts before rewriting the rest
-    if (!cw) RETCHK(set_ghost_t<double>(c, level, 0, c->q, 0, kAfterCorrection));
+    // (a level whose sweeps write their ghost images needs every ghost
+    // consistent before them: the full call)
+    if (!cw)
+      RETCHK(set_ghost_t<double>(c, level, 0, c->q, 0,
+                                 phys_images_level(c, c->L(level)) ? kFull : kAfterCorrection));
     RETCHK(bsam_level_relax(c, level));
   }
   return BSAM_OK;
@@ -5474,10 +5545,15 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
   const int kc = tm ? 0 : sweep_kc(c, L, 4, &nblk);
   const bool timed = fin && emode == 0;
   if (timed) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
+  const bool img = tm && emode == 0 && phys_images_level(c, L);
   if (tm) {
     const ModelK mk = c->mk();
     FOR_LAUNCHES(c, L, kHalf, a, nb) {
-      if (emode == 1)
+      if (img)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin_tm<2, 0, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)),
+                     (k_relax_lin_tm<3, 0, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)));
+      else if (emode == 1)
         DISPATCH_DIM(c->p.ndims,
                      (k_relax_lin_tm<2, 1><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)),
                      (k_relax_lin_tm<3, 1><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, mk, m->e, m->rhs, m->coef)));
@@ -5541,7 +5617,7 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
     if (tm) kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * (2.0 * (N + G) + 2.0 * N + N + 2.0 * N));
     else kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + N + 0.5 * N + N));
   }
-  return wrap ? 1 : 0;
+  return (wrap || img) ? 1 : 0;
 }
 
 // nsmoothingpasses red-black passes; phase 0 = pre-smoothing, 1 = post.
@@ -5595,7 +5671,8 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
     RETCHK(mixed_vcycle(c, level - 1));
     const bool cw = c->L(level).wrap_pid >= 0;
     RETCHK(correct_t<float>(c, level, m->e, m->ec, cw));
-    if (!cw) RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1, kAfterCorrection));
+    if (!cw)
+      RETCHK(set_ghost_t<float>(c, level, 0, m->e, 1, phys_images_level(c, c->L(level)) ? kFull : kAfterCorrection));
     RETCHK(mixed_level_relax(c, level, 1));
   }
   return BSAM_OK;
