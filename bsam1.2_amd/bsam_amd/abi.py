@@ -34,6 +34,14 @@ KSTAT_RESID_FINEST = 1
 KSTAT_FASRHS_FINEST = 2
 KSTAT_COUNT = 3
 
+KNOB_ORDER = 0
+KNOB_GATHER_ORDER_FP64 = 1
+KNOB_GATHER_ORDER_FP32 = 2
+KNOB_SWEEP_PLANES_FP64 = 3
+KNOB_SWEEP_PLANES_FP32 = 4
+KNOB_RESID_PLANES = 5
+KNOB_CHAIN_CELLS = 6
+
 
 class PatchDesc(C.Structure):
     """bsam_patch_desc (include/bsam_gpu.h)."""
@@ -111,6 +119,8 @@ ENTRY_POINTS = {
     "bsam_kstat_read": ([_P, _I, _LP, _DP, _DP], _I),
     "bsam_autotune": ([_P], _I),
     "bsam_tuning_report": ([_P, C.c_char_p, _I], _I),
+    "bsam_set_knob": ([_P, _I, _I, _I], _I),
+    "bsam_get_knob": ([_P, _I, _I, _IP], _I),
     "bsam_selftest_ghost_maps": ([C.POINTER(Params), _I, C.POINTER(PatchDesc), _I, _I, C.c_uint64, _LP, _IP], _I),
 }
 

@@ -130,11 +130,16 @@ class Solver:
         g = None if geom is None else np.ascontiguousarray(geom, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
         self._chk(self.lib.bsam_write_frame(self.ctx, path.encode(), time, g))
 
-    def estimate_level_errors(self, level, errflagopt, qtolerance, ncells):
+    def estimate_level_errors(self, level, errflagopt, qtolerance, ncells=None):
         """EstimateLevelErrors + ErrFlag tagging: the level's flags (patches
         in descriptor order, each column-major over 1..mx) and the number
-        flagged."""
-        flags = np.zeros(ncells, dtype=np.int32)
+        flagged.  The flag buffer is sized here from the installed
+        hierarchy, since the library writes prod(mx) values per patch of the
+        level; a caller's `ncells` must agree with that count."""
+        n = sum(int(np.prod(d["mx"][:self.ndims])) for d in self.hier if d["level"] == level)
+        if ncells is not None and ncells != n:
+            raise ValueError(f"level {level} has {n} cells, not {ncells}")
+        flags = np.zeros(n, dtype=np.int32)
         nt = C.c_int64()
         self._chk(self.lib.bsam_estimate_level_errors(self.ctx, level, errflagopt, qtolerance,
                                                       flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nt)))
@@ -179,6 +184,16 @@ class Solver:
         buf = C.create_string_buffer(1 << 16)
         self._chk(self.lib.bsam_tuning_report(self.ctx, buf, len(buf)))
         return json.loads(buf.value.decode())
+
+    def set_knob(self, knob, level, value):
+        """One launch-shape knob (abi.KNOB_*); bit-identical fields for
+        every value."""
+        self._chk(self.lib.bsam_set_knob(self.ctx, knob, level, value))
+
+    def get_knob(self, knob, level=0):
+        v = C.c_int32()
+        self._chk(self.lib.bsam_get_knob(self.ctx, knob, level, C.byref(v)))
+        return v.value
 
     # ---- instrumentation ----------------------------------------------------
     def kstat_enable(self, on=True):

@@ -76,6 +76,9 @@ constexpr int kSeqBoxElems = 8192;  // box values up to which a level's stages r
 // 65.6 (mixed) / 54.3 (FP64) time-steps/s, at 16^3 64.8 / 53.1 and at 4^3
 // 64.6 / 53.6: the 16^3 level's sweeps are faster as level launches.
 constexpr long kTmChainCells = 512;
+// The same for CH: 16^3 (2D: 64^2).  Starting at 8^3 ran the 3D AMR bench
+// at 74.5 instead of 75.0 (mixed) time-steps/s.
+constexpr long kChChainCells = 4096;
 
 // Interpolation constants, boundary.f90:54-63.
 __constant__ double cA[3] = {-1.0 / 5.0, 2.0 / 3.0, 8.0 / 15.0};
@@ -2170,9 +2173,11 @@ struct CoarseArgs {
   double dt, eps2;
 };
 
+// img: also write the periodic ghost images of the updated cells (only for
+// even extents, where every image has the cell's colour; see wrap_images).
 template <int DIM, typename T, bool LIN>
 __device__ void cv_relax(const DPatch& P, int c, double dt, double eps2, T* __restrict__ q,
-                         const T* __restrict__ f, const T* __restrict__ coef) {
+                         const T* __restrict__ f, const T* __restrict__ coef, bool img) {
   const double h2 = P.dx * P.dx;
   const double d2 = (DIM == 3) ? 6.0 : 4.0;
   const T tmp1 = (T)(dt / h2), tmp2 = (T)(eps2 / h2), tmp3 = (T)(d2 * eps2 / h2);
@@ -2218,7 +2223,7 @@ __device__ void cv_relax(const DPatch& P, int c, double dt, double eps2, T* __re
     }
     q[o] = n0;
     q[o + vs] = n1;
-    wrap_images<DIM, T>(q, P, i, j, k, n0, n1);
+    if (img) wrap_images<DIM, T>(q, P, i, j, k, n0, n1);
   }
 }
 
@@ -2395,13 +2400,22 @@ __device__ void cv_up(const DPatch& P, const DPatch& Q, T* __restrict__ q, T* __
   cv_wrap<DIM, T>(P, q);  // SetGhost(L, 0)
 }
 
+// LevelRelax on a chain level: each colour sweep followed by SetGhost, which
+// on these levels is the periodic wrap.  With even extents the sweep writes
+// the images itself; with an odd extent (a coarse level such as 50x9x4 or
+// 12x6x5) the wrap runs after the sweep, as SelfPeriodicBC does.
 template <int DIM, typename T, bool LIN>
 __device__ void cv_level_relax(const DPatch& P, int nsm, double dt, double eps2, T* q, const T* f,
                                const T* coef) {
+  const bool img = !(P.mx[0] & 1) && !(P.mx[1] & 1) && (DIM == 2 || !(P.mx[2] & 1));
   for (int s = 0; s < nsm; s++)
     for (int c = 0; c < 2; c++) {
-      cv_relax<DIM, T, LIN>(P, c, dt, eps2, q, f, coef);
+      cv_relax<DIM, T, LIN>(P, c, dt, eps2, q, f, coef, img);
       __syncthreads();
+      if (!img) {
+        cv_wrap<DIM, T>(P, q);
+        __syncthreads();
+      }
     }
 }
 
@@ -2719,7 +2733,13 @@ struct Level {
   DevMap gmap[7];
   bool maps_ok = false;
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
-  int chain = 0;                  // >0: AFASVcycle from here down runs in one workgroup (chain length)
+  // wrap_pid's extents are all even: a colour sweep may write the ghost
+  // images of its cells (wrap_images), since every image then has the
+  // cell's colour and only the other colour reads it.  With an odd extent
+  // the image of cell 1 is read by cell mx of the same colour in the same
+  // sweep, so the sweep runs without images and the wrap follows it.
+  bool wrap_img = false;
+  int chain = 0;                 // >0: AFASVcycle from here down runs in one workgroup (chain length)
   // launch shapes chosen by bsam_autotune: block order (ordered_block) of
   // the level kernels and of the ghost gather, planes per workgroup of the
   // marching sweeps (FP64, FP32) and residuals (0: the default rule)
@@ -2770,6 +2790,7 @@ struct bsam_ctx {
   hipGraphExec_t gexec = nullptr;
   bool capturing = false;
   std::string tune_report;  // JSON of the last bsam_autotune
+  long chain_cells = 0;     // largest level (cells) where the coarse chain starts
   // instrumentation
   KStat ks[BSAM_KSTAT_COUNT];
   // mixed precision state
@@ -2914,6 +2935,7 @@ extern "C" int32_t bsam_create(const bsam_params* params, bsam_ctx** out) {
     return set_err(BSAM_ENODEV, "no HIP device visible");
   bsam_ctx* c = new bsam_ctx();
   c->p = p;
+  c->chain_cells = (p.model == BSAM_MODEL_TUMOUR) ? kTmChainCells : kChChainCells;
   hipGetDevice(&c->dev);
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -2926,6 +2948,19 @@ extern "C" int32_t bsam_create(const bsam_params* params, bsam_ctx** out) {
   }
   *out = c;
   return BSAM_OK;
+}
+
+// Destroys the captured V-cycle graph; the next call captures it again
+// (after a change of launch shapes, dt or hierarchy).
+static void drop_graph(bsam_ctx* c) {
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+  }
 }
 
 static void free_hierarchy(bsam_ctx* c) {
@@ -3890,6 +3925,11 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
 
 int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
   L.wrap_pid = G.wrap_pid;
+  L.wrap_img = false;
+  if (L.wrap_pid >= 0) {
+    const DPatch& P = c->hp[L.wrap_pid];
+    L.wrap_img = !(P.mx[0] & 1) && !(P.mx[1] & 1) && (c->p.ndims == 2 || !(P.mx[2] & 1));
+  }
   L.stages.clear();
   for (const auto& boxes : G.stages) {
     BoxDesc* d = nullptr;
@@ -3935,6 +3975,42 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
 }  // namespace
 
 namespace {
+// Coarse chains: from the highest level with at most c->chain_cells cells
+// down to minlevel, AFASVcycle runs in one workgroup.  CH: every level of
+// the chain a single patch periodic in every dimension (k_coarse_vcycle);
+// the tumour model: levels of one patch each whose fill is a short
+// sequence of box stages that one workgroup runs in order
+// (k_coarse_vcycle_tm).  A chain has at least two levels.
+void plan_chain(bsam_ctx* c) {
+  for (Level& L : c->lv) L.chain = 0;
+  int top = c->minlevel - 1;
+  if (c->p.model == BSAM_MODEL_CH) {
+    for (int l = c->minlevel; l <= c->finest; l++) {
+      const Level& L = c->L(l);
+      if (L.wrap_pid < 0) break;
+      const DPatch& P = c->hp[L.wrap_pid];
+      if ((long)P.mx[0] * P.mx[1] * P.mx[2] > c->chain_cells) break;
+      top = l;
+    }
+    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= 16)
+      c->L(top).chain = top - c->minlevel + 1;
+  } else if (c->p.model == BSAM_MODEL_TUMOUR) {
+    for (int l = c->minlevel; l <= c->finest && l <= 0; l++) {
+      const Level& L = c->L(l);
+      if (L.patches.size() != 1 || L.wrap_pid >= 0 || L.d_rules || L.stages.empty() ||
+          (int)L.stages.size() > kMaxStages)
+        break;
+      int maxel = 0;
+      for (const Stage& st : L.stages) maxel = std::max(maxel, st.maxel);
+      const DPatch& P = c->hp[L.patches[0]];
+      if (maxel * c->p.nrvars > kSeqBoxElems || (long)P.mx[0] * P.mx[1] * P.mx[2] > c->chain_cells) break;
+      top = l;
+    }
+    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= kMaxChain)
+      c->L(top).chain = top - c->minlevel + 1;
+  }
+}
+
 // The host half of bsam_set_hierarchy: descriptor validation, the device
 // patch records with their pool offsets, and each level's patch list.
 int32_t plan_patches(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel,
@@ -4093,40 +4169,7 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
     if (c->p.batching) RETCHK(plan_ghost_maps(c, L, G));
     RETCHK(upload_ghost(c, L, G));
   }
-  // coarse chains: from the highest level whose patch has at most 16^3
-  // (2D: 64^2) cells down to minlevel, every level a single patch periodic
-  // in every dimension
-  if (c->p.model == BSAM_MODEL_CH) {
-    const long cap = (c->p.ndims == 3) ? 4096 : 4096;
-    int top = c->minlevel - 1;
-    for (int l = c->minlevel; l <= c->finest; l++) {
-      const Level& L = c->L(l);
-      if (L.wrap_pid < 0) break;
-      const DPatch& P = c->hp[L.wrap_pid];
-      if ((long)P.mx[0] * P.mx[1] * P.mx[2] > cap) break;
-      top = l;
-    }
-    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= 16)
-      c->L(top).chain = top - c->minlevel + 1;
-  } else if (c->p.model == BSAM_MODEL_TUMOUR) {
-    // the tumour model: levels of one patch each whose fill is a short
-    // sequence of box stages that one workgroup runs in order, from the
-    // highest such level of at most kTmChainCells cells down
-    int top = c->minlevel - 1;
-    for (int l = c->minlevel; l <= c->finest && l <= 0; l++) {
-      const Level& L = c->L(l);
-      if (L.patches.size() != 1 || L.wrap_pid >= 0 || L.d_rules || L.stages.empty() ||
-          (int)L.stages.size() > kMaxStages)
-        break;
-      int maxel = 0;
-      for (const Stage& st : L.stages) maxel = std::max(maxel, st.maxel);
-      const DPatch& P = c->hp[L.patches[0]];
-      if (maxel * c->p.nrvars > kSeqBoxElems || (long)P.mx[0] * P.mx[1] * P.mx[2] > kTmChainCells) break;
-      top = l;
-    }
-    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= kMaxChain)
-      c->L(top).chain = top - c->minlevel + 1;
-  }
+  plan_chain(c);
   if (maxblocks > c->partial_cap) {
     hipFree(c->d_partial);
     hipFree(c->d_partial2);
@@ -4679,7 +4722,7 @@ static int march_kc(bsam_ctx* c, const Level& L, int* nblk) {
 // whole effect of that SetGhost on the values that changed.
 static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass, bool lean) {
   Level& L = c->L(level);
-  const bool wrap = L.wrap_pid >= 0;
+  const bool wrap = L.wrap_img;
   int nblk = 0;
   const int kc = (c->p.model == BSAM_MODEL_CH) ? sweep_kc(c, L, 8, &nblk) : 0;
   if (phys_images_level(c, L)) return relax_level_impl(c, level, ipass, true);  // no SetGhost
@@ -5180,6 +5223,11 @@ extern "C" int32_t bsam_estimate_level_errors(bsam_ctx* c, int32_t level, int32_
   if (level <= c->minlevel) return set_err(BSAM_EINVAL, "level %d has no coarser level to estimate against", level);
   if (errflagopt != 1 && errflagopt != 10)
     return set_err(BSAM_EINVAL, "errflagopt must be 1 (errflagdefault) or 10 (errflaguser), not %d", errflagopt);
+  // errflaguser is the second-difference test of the CH and BioGrowth
+  // modules; the tumour model defines its own user tagging (a gradient
+  // test), which is not implemented here
+  if (errflagopt == 10 && c->p.model == BSAM_MODEL_TUMOUR)
+    return set_err(BSAM_EINVAL, "errflagopt 10 (errflaguser) is not implemented for the tumour model");
   Level& L = c->L(level);
   const int nd = c->p.ndims;
   std::vector<long> foff(L.patches.size());
@@ -5507,7 +5555,7 @@ static bool mixed_fused(bsam_ctx* c) {
   if (c->finest != 0 || c->p.nsmoothingpasses < 1 || c->p.model != BSAM_MODEL_CH) return false;
   Level& L = c->L(0);
   int nblk = 0;
-  return L.wrap_pid >= 0 && sweep_kc(c, L, 4, &nblk) > 0;
+  return L.wrap_img && sweep_kc(c, L, 4, &nblk) > 0;
 }
 
 // A refined mesh whose finest level runs the flat correction sweeps: its
@@ -5540,7 +5588,7 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
   const bool fin = level == c->finest;
   const bool tm = c->p.model == BSAM_MODEL_TUMOUR;
   // the tumour sweep writes no periodic images: its SetGhost follows
-  const bool wrap = L.wrap_pid >= 0 && !tm;
+  const bool wrap = L.wrap_img && !tm;
   int nblk = 0;
   const int kc = tm ? 0 : sweep_kc(c, L, 4, &nblk);
   const bool timed = fin && emode == 0;
@@ -5925,14 +5973,19 @@ extern "C" int32_t bsam_autotune(bsam_ctx* c) {
       c->q = sq;
       int nb = 0;
       const bool march = c->p.model == BSAM_MODEL_CH && sweep_kc(c, L, 8, &nb) > 0;
-      const bool wrap = L.wrap_pid >= 0;
+      const bool wrap = L.wrap_img;
       // the level's colour sweep (with the periodic images it writes on a
-      // fully periodic patch)
-      if (!march)
+      // fully periodic patch): in mixed mode the FP32 correction sweep,
+      // which is the sweep that path runs
+      if (!march) {
+        if (c->mixed) c->mixed->e = se;
         rc = tune_knob(c, "order", &L.order, {0, 1, 4, 16, 64}, [&]() {
+          if (c->mixed) return mixed_relax(c, level, 1);
           return (wrap && c->p.model == BSAM_MODEL_CH) ? relax_and_ghost(c, level, 1, false)
                                                          : bsam_relax_level(c, level, 1);
         }, rep);
+        if (c->mixed) c->mixed->e = e0;
+      }
       if (rc == BSAM_OK && L.maps_ok)
         rc = tune_knob(c, "gather_order_fp64", &L.gorder[0], {0, 1, 16, 64, 256},
                        [&]() { return set_ghost_t<double>(c, level, 0, sq, 0); }, rep);
@@ -5972,14 +6025,56 @@ extern "C" int32_t bsam_autotune(bsam_ctx* c) {
   rep += "]}";
   c->tune_report = rep;
   // launches captured with the old shapes are rebuilt at the next call
-  if (c->gexec) {
-    hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
+  drop_graph(c);
+  return BSAM_OK;
+}
+
+// Per-level knob storage, or nullptr.
+static int* level_knob(bsam_ctx* c, int32_t knob, Level& L) {
+  switch (knob) {
+    case BSAM_KNOB_ORDER: return &L.order;
+    case BSAM_KNOB_GATHER_ORDER_FP64: return &L.gorder[0];
+    case BSAM_KNOB_GATHER_ORDER_FP32: return &L.gorder[1];
+    case BSAM_KNOB_SWEEP_PLANES_FP64: return &L.kc_sweep[0];
+    case BSAM_KNOB_SWEEP_PLANES_FP32: return &L.kc_sweep[1];
+    case BSAM_KNOB_RESID_PLANES: return &L.kc_resid;
+    default: return nullptr;
   }
-  if (c->graph) {
-    hipGraphDestroy(c->graph);
-    c->graph = nullptr;
+}
+
+extern "C" int32_t bsam_set_knob(bsam_ctx* c, int32_t knob, int32_t level, int32_t value) {
+  if (!c) return set_err(BSAM_EINVAL, "null context");
+  if (c->capturing) return set_err(BSAM_ESTATE, "bsam_set_knob inside a graph capture");
+  if (knob == BSAM_KNOB_CHAIN_CELLS) {
+    if (value < 0) return set_err(BSAM_EINVAL, "chain cells must be >= 0");
+    c->chain_cells = value;
+    if (c->have_hier) plan_chain(c);
+    drop_graph(c);
+    return BSAM_OK;
   }
+  if (knob < 0 || knob > BSAM_KNOB_RESID_PLANES) return set_err(BSAM_EINVAL, "unknown knob %d", knob);
+  RETCHK(check_level(c, level));
+  Level& L = c->L(level);
+  if (value < 0 || value > 4096) return set_err(BSAM_EINVAL, "knob value %d out of range", value);
+  if (knob >= BSAM_KNOB_SWEEP_PLANES_FP64 && value > 0) {
+    // planes per workgroup: one large 3D patch, at most its z extent
+    if (c->p.ndims != 3 || L.patches.size() != 1 || value > c->hp[L.patches[0]].mx[2])
+      return set_err(BSAM_EINVAL, "plane count %d does not fit level %d", value, level);
+  }
+  *level_knob(c, knob, L) = value;
+  drop_graph(c);
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_get_knob(bsam_ctx* c, int32_t knob, int32_t level, int32_t* value) {
+  if (!c || !value) return set_err(BSAM_EINVAL, "null argument");
+  if (knob == BSAM_KNOB_CHAIN_CELLS) {
+    *value = (int32_t)c->chain_cells;
+    return BSAM_OK;
+  }
+  if (knob < 0 || knob > BSAM_KNOB_RESID_PLANES) return set_err(BSAM_EINVAL, "unknown knob %d", knob);
+  RETCHK(check_level(c, level));
+  *value = *level_knob(c, knob, c->L(level));
   return BSAM_OK;
 }
 

@@ -21,11 +21,15 @@ PRIVATE
 INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_OK = 0, BSAM_NOT_CONVERGED = 1, &
   BSAM_EINVAL = -1, BSAM_EHIP = -2, BSAM_ENOMEM = -3, BSAM_ESTATE = -4, &
   BSAM_EDIVERGED = -5, BSAM_ENODEV = -6
-INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_MODEL_CH = 1, BSAM_MODEL_BG = 2
+INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_MODEL_CH = 1, BSAM_MODEL_BG = 2, &
+  BSAM_MODEL_TUMOUR = 3
 INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_FIELD_Q = 0, BSAM_FIELD_QOLD = 1, &
   BSAM_FIELD_QC = 2, BSAM_FIELD_F = 3, BSAM_FIELD_RF = 4, BSAM_FIELD_FTMP = 5
 INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_PREC_FP64 = 0, BSAM_PREC_MIXED = 1
 INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_SOURCEFIELD = 1, BSAM_SOLUTIONFIELD = 2
+INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_KNOB_ORDER = 0, BSAM_KNOB_GATHER_ORDER_FP64 = 1, &
+  BSAM_KNOB_GATHER_ORDER_FP32 = 2, BSAM_KNOB_SWEEP_PLANES_FP64 = 3, BSAM_KNOB_SWEEP_PLANES_FP32 = 4, &
+  BSAM_KNOB_RESID_PLANES = 5, BSAM_KNOB_CHAIN_CELLS = 6
 !
 ! bsam_patch_desc.  C declares mglobal[3][2] and mbounds[3][2] (dimension
 ! first); in Fortran order that is (1:2, 1:3), the transpose of the
@@ -67,7 +71,7 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
          bsam_estimate_level_errors, bsam_write_frame, &
          bsam_multigrid_iterations, bsam_time_step, bsam_synchronize, &
-         bsam_autotune, bsam_last_error
+         bsam_autotune, bsam_set_knob, bsam_get_knob, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
 !
 INTERFACE
@@ -231,6 +235,17 @@ INTERFACE
   INTEGER(C_INT32_T) FUNCTION bsam_autotune(ctx) BIND(C, NAME='bsam_autotune')
     IMPORT:: C_INT32_T, C_PTR
     TYPE(C_PTR), VALUE:: ctx
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_set_knob(ctx, knob, level, value) BIND(C, NAME='bsam_set_knob')
+    IMPORT:: C_INT32_T, C_PTR
+    TYPE(C_PTR), VALUE:: ctx
+    INTEGER(C_INT32_T), VALUE:: knob, level, value
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_get_knob(ctx, knob, level, value) BIND(C, NAME='bsam_get_knob')
+    IMPORT:: C_INT32_T, C_PTR
+    TYPE(C_PTR), VALUE:: ctx
+    INTEGER(C_INT32_T), VALUE:: knob, level
+    INTEGER(C_INT32_T), INTENT(OUT):: value
   END FUNCTION
   TYPE(C_PTR) FUNCTION bsam_last_error() BIND(C, NAME='bsam_last_error')
     IMPORT:: C_PTR

@@ -192,7 +192,8 @@ int32_t bsam_copy_q_to_qold(bsam_ctx* ctx, int32_t level);
  *   errflagopt 10 (errflaguser): the second-difference test of the CH and
  *     BioGrowth SetErrFlagsUser2D/3D (CH/problem.f90:248-307,
  *     BioGrowth/problem.f90:185-240) on variable 1 of q as it is, against
- *     ndims*qtolerance.
+ *     ndims*qtolerance (BSAM_EINVAL for the tumour model, whose user
+ *     tagging is a different test).
  * flags (host, int32) receives info%errorflags(1:mx) of every patch of the
  * level, patches in descriptor order, each column-major; *ntagged the
  * number of flagged cells.  level must be above minlevel.  Synchronous. */
@@ -267,6 +268,26 @@ int32_t bsam_autotune(bsam_ctx* ctx);
  * with their mean launch time in microseconds, and the choice ("{}" before
  * any tuning).  BSAM_EINVAL if `len` bytes do not hold it. */
 int32_t bsam_tuning_report(const bsam_ctx* ctx, char* buf, int32_t len);
+
+/* The same knobs set or read one at a time (no counterpart in the
+ * reference).  A knob only moves work between workgroups, so every value
+ * gives bit-identical fields; tests use these calls to check a value the
+ * tuner may pick against the oracle.  Per-level knobs take the level; a
+ * captured graph is re-captured at the next call.  Outside graph capture. */
+#define BSAM_KNOB_ORDER              0  /* block order of the level kernels: 0 one range of
+                                           blocks per XCD, c >= 1 runs of c blocks per XCD */
+#define BSAM_KNOB_GATHER_ORDER_FP64  1  /* block order of the level's FP64 ghost gather */
+#define BSAM_KNOB_GATHER_ORDER_FP32  2  /* block order of the level's FP32 ghost gather */
+#define BSAM_KNOB_SWEEP_PLANES_FP64  3  /* one large 3D patch: planes per workgroup of the
+                                           marching FP64 sweep (0: the default rule) */
+#define BSAM_KNOB_SWEEP_PLANES_FP32  4  /* the same for the FP32 correction sweep */
+#define BSAM_KNOB_RESID_PLANES       5  /* the same for the marching residual */
+#define BSAM_KNOB_CHAIN_CELLS        6  /* whole hierarchy (level ignored): the largest level,
+                                           in cells, at which the one-workgroup coarse V-cycle
+                                           chain starts (0: no chain); defaults 4096 (CH) and
+                                           512 (tumour model) */
+int32_t bsam_set_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t value);
+int32_t bsam_get_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t* value);
 
 /* ---- diagnostics (no counterpart in the reference) ----------------------
  * Host-only check of the flattened ghost-fill tables that per-level

@@ -1128,6 +1128,8 @@ int32_t orc_estimate_level_errors(orc_ctx* c, int32_t level, int32_t errflagopt,
   if (!flags || !ntagged) return fail("null argument");
   if (level <= c->minlevel || level > c->finestlevel) return fail("bad level %d", level);
   if (errflagopt != 1 && errflagopt != 10) return fail("errflagopt must be 1 or 10");
+  if (errflagopt == 10 && c->p.model == BSAM_MODEL_TUMOUR)
+    return fail("errflagopt 10 is not implemented for the tumour model");
   if (errflagopt == 1) {
     orc_restrict_solution(c, level);
     int r = orc_set_ghost(c, level - 1, 0);

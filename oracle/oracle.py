@@ -119,10 +119,14 @@ class Oracle:
         g = None if geom is None else np.ascontiguousarray(geom, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
         self._chk(self.lib.orc_write_frame(self.ctx, path.encode(), time, g))
 
-    def estimate_level_errors(self, level, errflagopt, qtolerance, ncells):
+    def estimate_level_errors(self, level, errflagopt, qtolerance, ncells=None):
         """Flags of the level's cells (patches in level order, each
-        column-major over 1..mx) and the number flagged."""
-        flags = np.zeros(ncells, dtype=np.int32)
+        column-major over 1..mx) and the number flagged; the buffer is sized
+        from the hierarchy (a caller's `ncells` must agree)."""
+        n = sum(int(np.prod(d["mx"][:self.ndims])) for d in self.hier if d["level"] == level)
+        if ncells is not None and ncells != n:
+            raise ValueError(f"level {level} has {n} cells, not {ncells}")
+        flags = np.zeros(n, dtype=np.int32)
         nt = C.c_int64()
         self._chk(self.lib.orc_estimate_level_errors(self.ctx, level, errflagopt, qtolerance,
                                                      flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nt)))

@@ -172,18 +172,22 @@ def test_mixed_precision_on_amr_matches_fp64(name, gpu_solver_cls):
     assert cb <= ca + 2
 
 
-@pytest.mark.parametrize("shape,mthbc", [((128, 128, 128), (2, 2, 2, 2, 2, 2)),
-                                          ((128, 128, 128), (2, 2, 2, 2, 10, 10)),
-                                          ((200, 36, 16), (2, 2, 2, 2, 2, 2)),
-                                          ((200, 36, 16), (2, 2, 10, 10, 2, 2))])
-def test_large_patch_sweep_vs_oracle(shape, mthbc, gpu_solver_cls):
+@pytest.mark.parametrize("shape,mthbc,chain", [((128, 128, 128), (2, 2, 2, 2, 2, 2), None),
+                                                ((128, 128, 128), (2, 2, 2, 2, 10, 10), None),
+                                                ((200, 36, 16), (2, 2, 2, 2, 2, 2), None),
+                                                ((200, 36, 16), (2, 2, 2, 2, 2, 2), 32768),
+                                                ((200, 36, 16), (2, 2, 10, 10, 2, 2), None)])
+def test_large_patch_sweep_vs_oracle(shape, mthbc, chain, gpu_solver_cls):
     """A root large enough for the marching 3D sweep and residual on the
     finest level (with the periodic ghost images written by the sweep when
     the patch is periodic in every direction).  200 x 36 x 16 leaves partial
-    tiles in x (100 = 64 + 36 half-rows) and y (36 = 4 x 8 + 4 rows).  One
-    time step, bit-identical to the oracle."""
+    tiles in x (100 = 64 + 36 half-rows) and y (36 = 4 x 8 + 4 rows); its
+    coarse levels 100 x 18 x 8 and 50 x 9 x 4 end on an odd extent, whose
+    sweeps wrap after the sweep instead of writing images.  With the coarse
+    chain starting at 32^3 cells (chain=32768) both levels run in the
+    one-workgroup chain.  One time step, bit-identical to the oracle."""
     nd = 3
-    print(f"[gpu] {shape} sweep mthbc={mthbc}", flush=True)
+    print(f"[gpu] {shape} sweep mthbc={mthbc} chain={chain}", flush=True)
     minlevel = -(int(np.log2(min(shape))) - 2)
     hier = hierarchy.uniform(nd, shape, minlevel, list(mthbc))
     root = len(hier) - 1
@@ -193,6 +197,8 @@ def test_large_patch_sweep_vs_oracle(shape, mthbc, gpu_solver_cls):
     res = []
     for S in (gpu_solver_cls(configs.params(nd, 5e-5, 0.015, 1e-8), hier),
               oracle.Oracle(configs.params(nd, 5e-5, 0.015, 1e-8), hier)):
+        if chain is not None and hasattr(S, "set_knob"):
+            S.set_knob(abi.KNOB_CHAIN_CELLS, 0, chain)
         S.put(root, "q", q0)
         S.set_ghost(0, 0)
         S.copy_q_to_qold(0)
@@ -247,14 +253,18 @@ def test_autotuned_shapes_vs_oracle(shape, mthbc, gpu_solver_cls):
     _hist_close(hg, ho)
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16), (48, 24, 20), (96, 32), (40, 24)])
-def test_coarse_chain_vs_oracle(shape, gpu_solver_cls):
+@pytest.mark.parametrize("shape,chain", [((32, 32, 32), None), ((64, 32, 16), None), ((48, 24, 20), None),
+                                         ((48, 24, 20), 32768), ((96, 32), None), ((40, 24), None),
+                                         ((40, 24), 0)])
+def test_coarse_chain_vs_oracle(shape, chain, gpu_solver_cls):
     """Roots whose coarse levels run in the one-workgroup V-cycle chain:
     cubic, non-cubic (64 x 32 x 16: levels 32 x 16 x 8 and 16 x 8 x 4), an
-    odd coarsest extent (48 x 24 x 20 ends at 12 x 6 x 5), and 2D.  One time
-    step, bit-identical to the oracle."""
+    odd coarsest extent (48 x 24 x 20 ends at 12 x 6 x 5, whose sweeps wrap
+    after the sweep), the whole 48 x 24 x 20 hierarchy in one chain (cap
+    32^3 cells), 2D, and no chain at all (cap 0).  One time step,
+    bit-identical to the oracle."""
     nd = len(shape)
-    print(f"[gpu] chain {shape}", flush=True)
+    print(f"[gpu] chain {shape} cap={chain}", flush=True)
     minlevel = -(int(np.log2(min(shape))) - 2)
     hier = hierarchy.uniform(nd, shape, minlevel, [2] * (2 * nd))
     root = len(hier) - 1
@@ -264,6 +274,9 @@ def test_coarse_chain_vs_oracle(shape, gpu_solver_cls):
     res = []
     for S in (gpu_solver_cls(configs.params(nd, 5e-5, 0.015, 1e-9), hier),
               oracle.Oracle(configs.params(nd, 5e-5, 0.015, 1e-9), hier)):
+        if chain is not None and hasattr(S, "set_knob"):
+            S.set_knob(abi.KNOB_CHAIN_CELLS, 0, chain)
+            assert S.get_knob(abi.KNOB_CHAIN_CELLS) == chain
         S.put(root, "q", q0)
         S.set_ghost(0, 0)
         S.copy_q_to_qold(0)
