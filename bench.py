@@ -276,7 +276,10 @@ def tagging(wl, X, reps):
 def _tagging(wl, X, reps):
     out = {}
     levels = list(range(0, wl.finest))
-    for opt in (10, 1):
+    # errflaguser (10) is the CH/BioGrowth test; the tumour model has only 1
+    from bsam_amd import abi
+    opts = (1,) if X.params.model == abi.MODEL_TUMOUR else (10, 1)
+    for opt in opts:
         wl.load(X)
         total, tagged = 0.0, 0
         for lv in levels:
