@@ -5473,7 +5473,11 @@ extern "C" int32_t bsam_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, doub
 
 extern "C" int32_t bsam_time_step(bsam_ctx* c, int32_t first_step, int32_t* ncycles, double* residual) {
   if (!c || !c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed");
-  RETCHK(bsam_set_ghost(c, 0, 0));
+  // the ghost fills of AMR(rootlevel) on an unchanged mesh: SetGhost(level,
+  // 0) from the root up (bsamroutines.f90:978-989); its tagging changes
+  // nothing the step reads (RestrictSolution rewrites the values the last
+  // FillDown left, and qold, qc are rewritten below and in the V-cycle)
+  for (int level = 0; level <= c->finest; level++) RETCHK(bsam_set_ghost(c, level, 0));
   if (first_step) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
   for (int level = c->minlevel; level <= c->finest; level++) RETCHK(bsam_copy_q_to_qold(c, level));
   return bsam_multigrid_iterations(c, ncycles, residual, nullptr, 0);

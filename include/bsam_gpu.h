@@ -224,9 +224,12 @@ int32_t bsam_vcycle(bsam_ctx* ctx, int32_t level);
 int32_t bsam_multigrid_iterations(bsam_ctx* ctx, int32_t* ncycles,
                                   double* residual, double* history,
                                   int32_t history_len);
-/* One time step of the uniform-mesh time loop without regrid
- * (bsamroutines.f90:673-758 with maxlevel=0): SetGhost(root,0), FillDown on
- * the first step, CopyQToQold on every level, MultigridIterations. */
+/* One time step of the time loop on a frozen mesh (bsamroutines.f90:
+ * 673-758): the ghost fills the regrid AMR(rootlevel) (:966-991) does on an
+ * unchanged mesh, SetGhost(level, 0) from the root to the finest level;
+ * FillDown on the first step; CopyQToQold on every level;
+ * MultigridIterations.  With maxlevel = 0 this is the reference's uniform
+ * time step exactly. */
 int32_t bsam_time_step(bsam_ctx* ctx, int32_t first_step, int32_t* ncycles,
                        double* residual);
 

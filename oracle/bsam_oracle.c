@@ -1278,9 +1278,15 @@ int32_t orc_multigrid_iterations(orc_ctx* c, int32_t* ncycles, double* residual,
   return (it > c->p.maxvcycles) ? BSAM_NOT_CONVERGED : BSAM_OK;
 }
 
+/* One time step on a frozen mesh, TakeTimeSteps (bsamroutines.f90:673-758)
+ * with the regrid reduced to what AMR(rootlevel) (:966-991) does on an
+ * unchanged mesh: SetGhost(level, 0) from the root up.  Then FillDown on the
+ * first step (:693-697), CopyQToQold on every level (:700-702) and
+ * MultigridIterations (:758). */
 int32_t orc_time_step(orc_ctx* c, int32_t first_step, int32_t* ncycles, double* residual) {
   int r;
-  if ((r = orc_set_ghost(c, 0, 0))) return r;
+  for (int level = 0; level <= c->finestlevel; level++)
+    if ((r = orc_set_ghost(c, level, 0))) return r;
   if (first_step && (r = orc_fill_down(c, BSAM_SOLUTIONFIELD))) return r;
   for (int level = c->minlevel; level <= c->finestlevel; level++) orc_copy_q_to_qold(c, level);
   return orc_multigrid_iterations(c, ncycles, residual, NULL, 0);

@@ -14,7 +14,7 @@ from bsam_amd import abi, configs
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 NAMES = ["ch2d_u32", "ch3d_u16", "ch2d_u16_phys", "ch3d_u8_phys", "ch2d_amr", "ch2d_amr_phys",
-         "ch3d_amr", "ch3d_amr_phys"]
+         "ch3d_amr", "ch3d_amr_phys", "ch3d_amr2", "ch3d_amr2_phys"]
 # the tumour-growth model (4 fields) run by the reference LIBRARY through the
 # probe PROBLEM module (model = 3)
 TUMOUR_NAMES = ["tm2d_u16_phys", "tm3d_u8_phys", "tm2d_amr_phys", "tm3d_amr_phys"]

@@ -55,6 +55,15 @@ FIXTURES = {
     "ch3d_amr_phys": dict(ndims=3, n=16, maxlevel=1, mthbc=(2, 2, 2, 2, 10, 10), eps=0.04, dt=1e-5,
                           steps=1, tol=1e-9, init=dict(initmode=2, seed=1007, amp=0.01, radius=0.3,
                                                        xc=(0.5, 0.5, 0.3)), sweeps=24, mgp=4),
+    # two refined levels in 3D: level-2 patches whose parents are refined
+    # patches themselves, as on levels 2 and 3 of the 3D AMR workload
+    # (a higher tagging threshold keeps the refined band, and the fixture,
+    # small)
+    "ch3d_amr2": dict(ndims=3, n=16, maxlevel=2, mthbc=(2,) * 6, eps=0.04, dt=1e-5, steps=1, tol=1e-9,
+                      init=dict(initmode=2, seed=1008, amp=0.01, radius=0.22), sweeps=32, mgp=4, tagtol=0.3),
+    "ch3d_amr2_phys": dict(ndims=3, n=8, maxlevel=2, mthbc=(2, 2, 2, 2, 10, 10), eps=0.04, dt=1e-5,
+                           steps=1, tol=1e-9, init=dict(initmode=2, seed=1009, amp=0.01, radius=0.2,
+                                                        xc=(0.4, 0.5, 0.45)), sweeps=32, mgp=4, tagtol=0.3),
 }
 
 # The tumour-growth model (model 3 of the probe module, nrvars = 4: phi, mu,
@@ -141,7 +150,7 @@ def problem_items(fx, n, dumpsweeps):
     if xc is not None:
         for a in range(3):
             it.append((f"xc({a+1})", float(xc[a])))
-    it += [("tagtol", 0.05), ("dumpsweeps", dumpsweeps), ("dumpsrc", True), ("dumpend", True)]
+    it += [("tagtol", fx.get("tagtol", 0.05)), ("dumpsweeps", dumpsweeps), ("dumpsrc", True), ("dumpend", True)]
     for k, v in fx.get("tumour", {}).items():
         it.append((k, v))
     return it

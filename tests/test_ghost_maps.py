@@ -36,19 +36,24 @@ def _selftest(params, hier, seed=7):
     return n.value, nl.value, lo, hi
 
 
-@pytest.mark.parametrize("name", ["ch2d_amr", "ch2d_amr_phys", "ch3d_amr", "ch3d_amr_phys", "tm2d_amr_phys",
-                                  "tm3d_amr_phys"])
+@pytest.mark.parametrize("name", ["ch2d_amr", "ch2d_amr_phys", "ch3d_amr", "ch3d_amr_phys", "ch3d_amr2",
+                                  "ch3d_amr2_phys", "tm2d_amr_phys", "tm3d_amr_phys"])
 def test_golden_hierarchies(name):
     """Every refined level is flattened; with physical boundaries so are the
     root and the levels below it, whose fill is the physical condition
     face by face (a 3D face reads edge ghosts from before the call, which
     the later faces overwrite).  A root periodic in every dimension keeps
     its wrap kernel.  The tumour fixtures exercise four fields and their
-    mixed boundary condition."""
+    mixed boundary condition.  On ch3d_amr2_phys the finest level keeps the
+    staged fill: a level-2 patch on a physical face next to a coarse-fine
+    edge reads an edge ghost the call has not written, which no later stage
+    overwrites, so its final value depends on the state before the call."""
     meta, _ = fu.load(name)
     for st in meta["steps"]:
         n, nl, lo, hi = _selftest(fu.step_params(meta), st["hierarchy"])
         expect = hi - lo + 1 if name.endswith("_phys") else hi
+        if name == "ch3d_amr2_phys":
+            expect -= 1
         assert nl == expect and n > 0
 
 

@@ -145,7 +145,8 @@ def test_mixed_precision_matches_fp64(nd, n, steps, gpu_solver_cls):
     assert max(cb) <= max(ca) + 2
 
 
-@pytest.mark.parametrize("name", ["ch2d_amr", "ch3d_amr", "ch2d_amr_phys", "ch3d_amr_phys"] + fu.TUMOUR_NAMES)
+@pytest.mark.parametrize("name", ["ch2d_amr", "ch3d_amr", "ch2d_amr_phys", "ch3d_amr_phys", "ch3d_amr2",
+                                  "ch3d_amr2_phys"] + fu.TUMOUR_NAMES)
 def test_mixed_precision_on_amr_matches_fp64(name, gpu_solver_cls):
     """Mixed mode on the reference's AMR hierarchies (flattened ghost fill
     of the correction, q += e fused into the finest level's last sweep,

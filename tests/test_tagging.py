@@ -146,7 +146,7 @@ def _bg_pair(gpu_cls, batching):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["ch3d_amr", "ch2d_amr_phys", "tm3d_amr_phys", "bg2d"])
+@pytest.mark.parametrize("name", ["ch3d_amr", "ch3d_amr2", "ch2d_amr_phys", "tm3d_amr_phys", "bg2d"])
 @pytest.mark.parametrize("batching", [1, 0])
 def test_estimate_level_errors_vs_oracle(name, batching, gpu_solver_cls):
     if name == "bg2d":
