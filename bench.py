@@ -146,12 +146,14 @@ def timed_path(wl, a, precision):
         first = False
     S.synchronize()
     cycles = []
+    bad = 0  # timed steps that ended at maxvcycles above the tolerance
     t0 = time.perf_counter()
     for _ in range(a.steps):
         rc, ncyc, r = S.time_step(first)
         first = False
         cycles.append(ncyc)
         if rc != abi.BSAM_OK:
+            bad += 1
             print(f"[bench] step did not converge (rc={rc}, residual {r})", file=sys.stderr)
     S.synchronize()
     elapsed = time.perf_counter() - t0
@@ -172,7 +174,7 @@ def timed_path(wl, a, precision):
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=(achieved / HBM_PEAK_GBS) if achieved else None,
                     traffic=measured_traffic(wl.name, precision, kname),
                     avg_us=avg_ms * 1e3, launches=nl, bytes_per_launch=b)
-    return S, elapsed, sum(cycles) / a.steps, roof, tune
+    return S, elapsed, sum(cycles) / a.steps, roof, tune, bad
 
 
 def tune_choices(tune):
@@ -293,8 +295,9 @@ def main():
     paths = ["mixed", "fp64"] if a.precision == "both" else [a.precision]
     res = {}
     for pr in paths:
-        S, elapsed, vc, roof, tune = timed_path(wl, a, pr)
-        res[pr] = dict(elapsed=elapsed, vc=vc, roof=roof, q=fields(wl, S) if a.check else None, tune=tune)
+        S, elapsed, vc, roof, tune, bad = timed_path(wl, a, pr)
+        res[pr] = dict(elapsed=elapsed, vc=vc, roof=roof, q=fields(wl, S) if a.check else None, tune=tune,
+                       bad=bad)
         S.close()
     head = paths[0]
     check = None
@@ -358,6 +361,11 @@ def main():
                              "vcycles_per_step": f["vc"], "roofline": f["roof"],
                              "autotuned": tune_choices(f["tune"])}
         out["speedup_vs_fp64"] = f["elapsed"] / el
+    # a timed step that stopped at maxvcycles did not reach the metric's
+    # tolerance: the line says so and is marked invalid
+    out["nonconverged_steps"] = {pr: res[pr]["bad"] for pr in paths}
+    if any(res[pr]["bad"] for pr in paths):
+        out["valid"] = False
     if check is not None:
         out["max_norm_error"] = check
     if res[head]["tune"]:

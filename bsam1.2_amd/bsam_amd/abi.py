@@ -121,6 +121,8 @@ ENTRY_POINTS = {
     "bsam_tuning_report": ([_P, C.c_char_p, _I], _I),
     "bsam_set_knob": ([_P, _I, _I, _I], _I),
     "bsam_get_knob": ([_P, _I, _I, _IP], _I),
+    "bsam_set_mixed_fallback": ([_P, _D], _I),
+    "bsam_mixed_fallbacks": ([_P, _LP], _I),
     "bsam_selftest_ghost_maps": ([C.POINTER(Params), _I, C.POINTER(PatchDesc), _I, _I, C.c_uint64, _LP, _IP], _I),
 }
 

@@ -195,6 +195,16 @@ class Solver:
         self._chk(self.lib.bsam_get_knob(self.ctx, knob, level, C.byref(v)))
         return v.value
 
+    def set_mixed_fallback(self, stall_ratio):
+        """Mixed path: hand the step to FP64 V-cycles when an outer
+        iteration reduces the residual by less than this ratio."""
+        self._chk(self.lib.bsam_set_mixed_fallback(self.ctx, stall_ratio))
+
+    def mixed_fallbacks(self):
+        n = C.c_int64()
+        self._chk(self.lib.bsam_mixed_fallbacks(self.ctx, C.byref(n)))
+        return n.value
+
     # ---- instrumentation ----------------------------------------------------
     def kstat_enable(self, on=True):
         self._chk(self.lib.bsam_kstat_enable(self.ctx, 1 if on else 0))

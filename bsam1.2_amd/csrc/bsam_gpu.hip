@@ -2791,6 +2791,10 @@ struct bsam_ctx {
   bool capturing = false;
   std::string tune_report;  // JSON of the last bsam_autotune
   long chain_cells = 0;     // largest level (cells) where the coarse chain starts
+  // mixed precision: an outer iteration whose residual exceeds mixed_stall
+  // times the previous one ends the step in FP64 V-cycles
+  double mixed_stall = 0.9;
+  int64_t mixed_fallbacks = 0;  // MultigridIterations calls that fell back
   // instrumentation
   KStat ks[BSAM_KSTAT_COUNT];
   // mixed precision state
@@ -5413,8 +5417,11 @@ static bool kstat_any(bsam_ctx* c) {
 // one launch per patch (batching = 0) is the plain per-patch mode, whose
 // launch count grows with the patch count (tens of thousands per V-cycle on
 // large AMR hierarchies) and is not captured.
-static int32_t run_iteration(bsam_ctx* c, int32_t (*body)(bsam_ctx*)) {
-  if (c->p.use_graph && c->p.batching && !kstat_any(c)) {
+// One loop body on the context's stream, then the norm's copy to the host.
+// With use_graph the body is captured once and replayed; `eager` runs it
+// without the graph (the context keeps one graph, that of its own path).
+static int32_t run_iteration(bsam_ctx* c, int32_t (*body)(bsam_ctx*), bool eager = false) {
+  if (c->p.use_graph && c->p.batching && !kstat_any(c) && !eager) {
     if (!c->gexec) {
       HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
       c->capturing = true;
@@ -5844,10 +5851,12 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
   k_zero_f<<<nblocks(c->nq), kBlock, 0, c->s>>>(c->mixed->e, c->nq);
   LAUNCH_CHECK();
   RETCHK(mixed_residuals(c));
-  double res = 0.0;
+  double res = 0.0, prev = 0.0;
+  bool fp64 = false;  // fell back to the FP64 FAS V-cycle
   int it;
   for (it = 1; it <= c->p.maxvcycles; it++) {
-    RETCHK(run_iteration(c, mixed_iteration));
+    if (fp64) RETCHK(run_iteration(c, enqueue_iteration, true));
+    else RETCHK(run_iteration(c, mixed_iteration));
     HIPCHK(hipStreamSynchronize(c->s));
     res = norm_from(c, c->h_norm, c->finest);
     if (history && it - 1 < history_len) history[it - 1] = res;
@@ -5857,6 +5866,21 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
       return set_err(BSAM_EDIVERGED, "residual not finite after outer iteration %d", it);
     }
     if (res < c->p.qerrortol) break;
+    // an outer iteration that does not reduce the residual by the stall
+    // ratio hands the rest of the step to the FP64 V-cycle, which starts
+    // from the current q with f as the source (the state every outer
+    // iteration leaves: q updated, filled down to level 1, the finest
+    // level's ghosts set)
+    if (!fp64 && it >= 2 && res > c->mixed_stall * prev) {
+      fp64 = true;
+      c->mixed_fallbacks++;
+    }
+    prev = res;
+  }
+  if (fp64) {
+    // as the FP64 path leaves f: the source, not the tau terms
+    for (int level = c->finest - 1; level >= 0; level--) RETCHK(bsam_update_source(c, level));
+    RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
   }
   RETCHK(fill_down_range(c, BSAM_SOLUTIONFIELD, 0, c->minlevel + 1));
   *ncycles = it > c->p.maxvcycles ? c->p.maxvcycles : it;
@@ -6067,6 +6091,19 @@ extern "C" int32_t bsam_set_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
   }
   *level_knob(c, knob, L) = value;
   drop_graph(c);
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_set_mixed_fallback(bsam_ctx* c, double stall_ratio) {
+  if (!c) return set_err(BSAM_EINVAL, "null context");
+  if (!(stall_ratio >= 0.0)) return set_err(BSAM_EINVAL, "stall ratio must be >= 0");
+  c->mixed_stall = stall_ratio;
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_mixed_fallbacks(bsam_ctx* c, int64_t* count) {
+  if (!c || !count) return set_err(BSAM_EINVAL, "null argument");
+  *count = c->mixed_fallbacks;
   return BSAM_OK;
 }
 

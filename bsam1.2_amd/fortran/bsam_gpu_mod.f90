@@ -71,7 +71,8 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
          bsam_estimate_level_errors, bsam_write_frame, &
          bsam_multigrid_iterations, bsam_time_step, bsam_synchronize, &
-         bsam_autotune, bsam_set_knob, bsam_get_knob, bsam_last_error
+         bsam_autotune, bsam_set_knob, bsam_get_knob, bsam_set_mixed_fallback, &
+         bsam_mixed_fallbacks, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
 !
 INTERFACE
@@ -246,6 +247,16 @@ INTERFACE
     TYPE(C_PTR), VALUE:: ctx
     INTEGER(C_INT32_T), VALUE:: knob, level
     INTEGER(C_INT32_T), INTENT(OUT):: value
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_set_mixed_fallback(ctx, stall_ratio) BIND(C, NAME='bsam_set_mixed_fallback')
+    IMPORT:: C_INT32_T, C_PTR, C_DOUBLE
+    TYPE(C_PTR), VALUE:: ctx
+    REAL(C_DOUBLE), VALUE:: stall_ratio
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_mixed_fallbacks(ctx, count) BIND(C, NAME='bsam_mixed_fallbacks')
+    IMPORT:: C_INT32_T, C_INT64_T, C_PTR
+    TYPE(C_PTR), VALUE:: ctx
+    INTEGER(C_INT64_T), INTENT(OUT):: count
   END FUNCTION
   TYPE(C_PTR) FUNCTION bsam_last_error() BIND(C, NAME='bsam_last_error')
     IMPORT:: C_PTR

@@ -292,6 +292,19 @@ int32_t bsam_tuning_report(const bsam_ctx* ctx, char* buf, int32_t len);
 int32_t bsam_set_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t value);
 int32_t bsam_get_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t* value);
 
+/* ---- failure handling of the mixed path (no counterpart in the reference,
+ * whose loop is FP64 only and exits silently at maxvcycles,
+ * afasroutines.f90:79-108) ---------------------------------------------
+ * With BSAM_PREC_MIXED, from the second outer iteration on, an iteration
+ * whose residual exceeds stall_ratio times the previous one (the FP32
+ * correction cycle has stopped converging) hands the rest of that
+ * MultigridIterations call to the FP64 FAS V-cycle, from the current
+ * solution.  Default 0.9; 0 forces the hand-over after the second
+ * iteration (a test hook).  bsam_mixed_fallbacks counts the calls that
+ * handed over since the context was created. */
+int32_t bsam_set_mixed_fallback(bsam_ctx* ctx, double stall_ratio);
+int32_t bsam_mixed_fallbacks(bsam_ctx* ctx, int64_t* count);
+
 /* ---- diagnostics (no counterpart in the reference) ----------------------
  * Host-only check of the flattened ghost-fill tables that per-level
  * launches use in place of the staged SetGhost (boundary.f90:67-97); needs
