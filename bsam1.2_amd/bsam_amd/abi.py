@@ -108,6 +108,7 @@ ENTRY_POINTS = {
     "bsam_error_afas": ([_P, _I, _DP], _I),
     "bsam_copy_q_to_qold": ([_P, _I], _I),
     "bsam_estimate_level_errors": ([_P, _I, _I, _D, C.POINTER(_I), _LP], _I),
+    "bsam_estimate_level_errors_noalloc": ([_P, _I, _I, _D, C.POINTER(_I), _LP], _I),
     "bsam_write_frame": ([_P, C.c_char_p, _D, _DP], _I),
     "bsam_selftest_f25": ([_DP, C.c_int64, C.c_char_p], _I),
     "bsam_vcycle": ([_P, _I], _I),

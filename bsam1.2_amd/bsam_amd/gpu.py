@@ -145,6 +145,17 @@ class Solver:
                                                       flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nt)))
         return flags, nt.value
 
+    def estimate_level_errors_noalloc(self, level, errflagopt, qtolerance):
+        """The same tagging through the allocation-free entry point (flags
+        staged in the library's fixed device scratch array)."""
+        n = sum(int(np.prod(d["mx"][:self.ndims])) for d in self.hier if d["level"] == level)
+        flags = np.zeros(n, dtype=np.int32)
+        nt = C.c_int64()
+        self._chk(self.lib.bsam_estimate_level_errors_noalloc(self.ctx, level, errflagopt, qtolerance,
+                                                              flags.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                              C.byref(nt)))
+        return flags, nt.value
+
     def vcycle(self, level):
         self._chk(self.lib.bsam_vcycle(self.ctx, level))
 

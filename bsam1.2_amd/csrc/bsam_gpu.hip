@@ -80,6 +80,15 @@ constexpr long kTmChainCells = 512;
 // at 74.5 instead of 75.0 (mixed) time-steps/s.
 constexpr long kChChainCells = 4096;
 
+// Fixed device scratch of the allocation-free tagging call
+// (bsam_estimate_level_errors_noalloc): per-patch flag offsets and the
+// level's flags, 16 MiB (a level of about 4.19 M cells).  Each device that
+// loads the library holds one copy until the process ends; host code uses
+// the address hipGetSymbolAddress returns for the current device, never
+// the name.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+__device__ __align__(256) char scratch[kScratchBytes];
+
 // Interpolation constants, boundary.f90:54-63.
 __constant__ double cA[3] = {-1.0 / 5.0, 2.0 / 3.0, 8.0 / 15.0};
 __constant__ double cB[3] = {5.0 / 32.0, 15.0 / 16.0, -3.0 / 32.0};
@@ -2747,6 +2756,8 @@ struct Level {
   int kc_sweep[2] = {0, 0}, kc_resid = 0;
   double h = 0.0;
   long qspan_off = 0, qspan_len = 0;  // contiguous q-pool span of the level
+  long ncell = 0;                     // interior cells of the level
+  long* d_foff = nullptr;             // start of each patch's cells in a dense level array (tagging)
 };
 
 struct KStat {
@@ -2785,6 +2796,9 @@ struct bsam_ctx {
   // regrid tagging: per-patch flag offsets, then the level's flags
   char* d_tag = nullptr;
   size_t tag_cap = 0;
+  // this device's copy of the file-scope `scratch` array (its device
+  // address, looked up once in bsam_create with the device current)
+  char* scratch = nullptr;
   // graph of the V-cycle loop body
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -2945,6 +2959,15 @@ extern "C" int32_t bsam_create(const bsam_params* params, bsam_ctx** out) {
     delete c;
     return set_err(BSAM_EHIP, "hipStreamCreate failed");
   }
+  {
+    void* sp = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      bsam_destroy(c);
+      return set_err(BSAM_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
+    c->scratch = static_cast<char*>(sp);
+  }
   if (hipMalloc(&c->d_norm, 4 * sizeof(double)) != hipSuccess ||
       hipHostMalloc(&c->h_norm, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     bsam_destroy(c);
@@ -2972,6 +2995,7 @@ static void free_hierarchy(bsam_ctx* c) {
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
   for (Level& L : c->lv) {
     hipFree(L.d_patches);
+    hipFree(L.d_foff);
     hipFree(L.d_rules);
     for (DevMap& m : L.gmap) {
       hipFree(m.cr);
@@ -4167,6 +4191,19 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
       end = c->hp[pid].qoff + (long)c->hp[pid].qvs * nv;
     }
     L.qspan_len = contiguous ? end - L.qspan_off : -1;
+    // dense cell offsets of the level's patches (the tagging flags)
+    {
+      std::vector<long> foff(L.patches.size());
+      long n = 0;
+      for (size_t a = 0; a < L.patches.size(); a++) {
+        foff[a] = n;
+        const DPatch& P = c->hp[L.patches[a]];
+        n += (long)P.mx[0] * P.mx[1] * P.mx[2];
+      }
+      L.ncell = n;
+      HIPCHK(hipMalloc(&L.d_foff, foff.size() * sizeof(long)));
+      HIPCHK(hipMemcpy(L.d_foff, foff.data(), foff.size() * sizeof(long), hipMemcpyHostToDevice));
+    }
     GhostPlan G;
     RETCHK(plan_ghost_stages(c, L, G));
     plan_interp_rules(c, L, G);
@@ -5220,8 +5257,8 @@ extern "C" int32_t bsam_write_frame(bsam_ctx* c, const char* path, double time, 
 // 0), GetCoarseGhostPoints and CopyQToQold on L and L-1 (the reference's
 // order, with their effect on q, qc and qold), then the truncation-error
 // flags; errflagopt 10 applies the second-difference test to q as it is.
-extern "C" int32_t bsam_estimate_level_errors(bsam_ctx* c, int32_t level, int32_t errflagopt, double qtolerance,
-                                              int32_t* flags, int64_t* ntagged) {
+static int32_t check_tag_args(bsam_ctx* c, int32_t level, int32_t errflagopt, const int32_t* flags,
+                              const int64_t* ntagged) {
   if (!c || !flags || !ntagged) return set_err(BSAM_EINVAL, "null argument");
   RETCHK(check_level(c, level));
   if (level <= c->minlevel) return set_err(BSAM_EINVAL, "level %d has no coarser level to estimate against", level);
@@ -5232,28 +5269,19 @@ extern "C" int32_t bsam_estimate_level_errors(bsam_ctx* c, int32_t level, int32_
   // test), which is not implemented here
   if (errflagopt == 10 && c->p.model == BSAM_MODEL_TUMOUR)
     return set_err(BSAM_EINVAL, "errflagopt 10 (errflaguser) is not implemented for the tumour model");
+  return BSAM_OK;
+}
+
+// The tagging of one level with the level's flags staged in `d_flags`
+// (device memory of at least L.ncell int32); shared by the entry point that
+// grows a library buffer and the one that uses the fixed scratch array.
+// The per-patch flag offsets were uploaded with the hierarchy.
+static int32_t estimate_level_errors_in(bsam_ctx* c, int32_t level, int32_t errflagopt, double qtolerance,
+                                        int32_t* flags, int64_t* ntagged, int32_t* d_flags) {
   Level& L = c->L(level);
   const int nd = c->p.ndims;
-  std::vector<long> foff(L.patches.size());
-  long n = 0;
-  for (size_t a = 0; a < L.patches.size(); a++) {
-    foff[a] = n;
-    const DPatch& P = c->hp[L.patches[a]];
-    n += (long)P.mx[0] * P.mx[1] * P.mx[2];
-  }
-  const size_t obytes = ((foff.size() * sizeof(long) + 255) / 256) * 256;
-  const size_t need = obytes + (size_t)n * sizeof(int32_t);
-  if (need > c->tag_cap) {
-    HIPCHK(hipStreamSynchronize(c->s));
-    hipFree(c->d_tag);
-    c->d_tag = nullptr;
-    c->tag_cap = 0;
-    HIPCHK(hipMalloc(&c->d_tag, need));
-    c->tag_cap = need;
-  }
-  long* d_foff = reinterpret_cast<long*>(c->d_tag);
-  int32_t* d_flags = reinterpret_cast<int32_t*>(c->d_tag + obytes);
-  HIPCHK(hipMemcpyAsync(d_foff, foff.data(), foff.size() * sizeof(long), hipMemcpyHostToDevice, c->s));
+  const long n = L.ncell;
+  const long* d_foff = L.d_foff;
   if (errflagopt == 1) {
     RETCHK(bsam_restrict_solution(c, level));
     RETCHK(bsam_set_ghost(c, level - 1, 0));
@@ -5292,6 +5320,36 @@ extern "C" int32_t bsam_estimate_level_errors(bsam_ctx* c, int32_t level, int32_
   for (long e = 0; e < n; e++) nt += flags[e] != 0;
   *ntagged = nt;
   return BSAM_OK;
+}
+
+extern "C" int32_t bsam_estimate_level_errors(bsam_ctx* c, int32_t level, int32_t errflagopt, double qtolerance,
+                                              int32_t* flags, int64_t* ntagged) {
+  RETCHK(check_tag_args(c, level, errflagopt, flags, ntagged));
+  const size_t need = (size_t)c->L(level).ncell * sizeof(int32_t);
+  if (need > c->tag_cap) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    hipFree(c->d_tag);
+    c->d_tag = nullptr;
+    c->tag_cap = 0;
+    HIPCHK(hipMalloc(&c->d_tag, need));
+    c->tag_cap = need;
+  }
+  return estimate_level_errors_in(c, level, errflagopt, qtolerance, flags, ntagged,
+                                  reinterpret_cast<int32_t*>(c->d_tag));
+}
+
+extern "C" int32_t bsam_estimate_level_errors_noalloc(bsam_ctx* c, int32_t level, int32_t errflagopt,
+                                                      double qtolerance, int32_t* flags, int64_t* ntagged) {
+  RETCHK(check_tag_args(c, level, errflagopt, flags, ntagged));
+  if (!c->scratch) return set_err(BSAM_ESTATE, "the scratch array of device %d has no address", c->dev);
+  const long n = c->L(level).ncell;
+  if ((size_t)n * sizeof(int32_t) > kScratchBytes)
+    return set_err(BSAM_EINVAL,
+                   "level %d has %ld cells; the allocation-free tagging call stages at most %zu flags "
+                   "(%zu bytes of scratch): use bsam_estimate_level_errors",
+                   level, n, kScratchBytes / sizeof(int32_t), kScratchBytes);
+  return estimate_level_errors_in(c, level, errflagopt, qtolerance, flags, ntagged,
+                                  reinterpret_cast<int32_t*>(c->scratch));
 }
 
 // ---------------------------------------------------------------------------

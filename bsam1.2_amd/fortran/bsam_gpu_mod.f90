@@ -69,7 +69,7 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_coarse_ghost_points, bsam_coarse_loading, bsam_correct_fine, &
          bsam_fill_down, bsam_get_source, bsam_update_source, bsam_update_aux, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
-         bsam_estimate_level_errors, bsam_write_frame, &
+         bsam_estimate_level_errors, bsam_estimate_level_errors_noalloc, bsam_write_frame, &
          bsam_multigrid_iterations, bsam_time_step, bsam_synchronize, &
          bsam_autotune, bsam_set_knob, bsam_get_knob, bsam_set_mixed_fallback, &
          bsam_mixed_fallbacks, bsam_last_error
@@ -192,6 +192,18 @@ INTERFACE
   ! descriptor order, each column-major.
   INTEGER(C_INT32_T) FUNCTION bsam_estimate_level_errors(ctx, level, errflagopt, qtolerance, flags, ntagged) &
       BIND(C, NAME='bsam_estimate_level_errors')
+    IMPORT:: C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE:: ctx
+    INTEGER(C_INT32_T), VALUE:: level, errflagopt
+    REAL(C_DOUBLE), VALUE:: qtolerance
+    INTEGER(C_INT32_T), DIMENSION(*), INTENT(OUT):: flags
+    INTEGER(C_INT64_T), INTENT(OUT):: ntagged
+  END FUNCTION
+  ! The same without any device allocation (flags staged in a fixed 16 MiB
+  ! device array: levels of at most 4194304 cells); for hosts that cannot
+  ! allocate inside their time loop.  Overlapping calls on one GPU race.
+  INTEGER(C_INT32_T) FUNCTION bsam_estimate_level_errors_noalloc(ctx, level, errflagopt, qtolerance, flags, &
+      ntagged) BIND(C, NAME='bsam_estimate_level_errors_noalloc')
     IMPORT:: C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
     TYPE(C_PTR), VALUE:: ctx
     INTEGER(C_INT32_T), VALUE:: level, errflagopt

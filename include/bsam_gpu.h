@@ -199,6 +199,18 @@ int32_t bsam_copy_q_to_qold(bsam_ctx* ctx, int32_t level);
  * number of flagged cells.  level must be above minlevel.  Synchronous. */
 int32_t bsam_estimate_level_errors(bsam_ctx* ctx, int32_t level, int32_t errflagopt, double qtolerance,
                                    int32_t* flags, int64_t* ntagged);
+/* The same tagging without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory during the
+ * time loop (HIP graph capture around it, tight serving loops, harnesses
+ * that forbid hipMalloc).  bsam_estimate_level_errors grows a device buffer
+ * for the level's flags on first use; this call stages them in a fixed
+ * device array of 16 MiB instead, so a level may hold at most 4194304 cells
+ * (BSAM_EINVAL beyond that, with nothing run).  Same arguments, results and
+ * side effects.  The array is one per GPU, shared by every context on it
+ * and kept until the process ends: calls that overlap on one GPU (two host
+ * threads, two contexts) race and must be serialised by the caller. */
+int32_t bsam_estimate_level_errors_noalloc(bsam_ctx* ctx, int32_t level, int32_t errflagopt,
+                                           double qtolerance, int32_t* flags, int64_t* ntagged);
 /* Output frame: WriteQ (bsaminputoutput.f90:60-82) with OutputQ (:85-129)
  * and WriteQ2D/3D (:132-181), written to `path` (the reference's
  * './OUT/m%05d.dat' is the caller's choice) byte for byte as the reference
