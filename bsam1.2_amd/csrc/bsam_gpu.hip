@@ -200,6 +200,36 @@ __device__ __forceinline__ void wrap_images(T* __restrict__ q, const DPatch& P, 
       }
 }
 
+// One cell of the CH correction smoother: RelaxGrid2D/3D's pointwise Newton
+// step (CH/problem.f90:318-369, 554-614) written for the correction e of
+// the base state q0 (phi0 = p0, the FP64 iterate the cycle started from).
+// With phi = p0 + e_phi the Newton coefficient is -3 phi^2 and the cubic
+// part of its right-hand side, -2 phi^3 minus what the residual of q0
+// already holds, is -e_phi^2 (3 p0 + 2 e_phi); so the FP32 sweep takes the
+// same step as the FP64 sweep on q, up to rounding of the correction.
+// ec0 is the cell's own e_phi, rhs the residual f - L(q0).
+__device__ __forceinline__ void ch_corr_update(float p0, float ec0, float r0, float r1, float smu, float sphi,
+                                               float tmp1, float tmp2, float tmp3, float a12, float& n0,
+                                               float& n1) {
+  const float ph = p0 + ec0;
+  const float a21 = -3.0f * (ph * ph) - tmp3;
+  const float b1 = r0 + tmp1 * smu;
+  const float b2 = r1 - tmp2 * sphi - (ec0 * ec0) * (3.0f * p0 + 2.0f * ec0);
+  const float det = 1.0f - a12 * a21;
+  n0 = (b1 - a12 * b2) / det;
+  n1 = (b2 - a21 * b1) / det;
+}
+
+// The CH operator's action on the correction, N(e) = L(q0 + e) - L(q0):
+// N1 = e_phi - t1 ULap(e_mu), N2 = e_mu - e_phi (3 p0^2 + 3 p0 e_phi +
+// e_phi^2) + t2 ULap(e_phi), the cubic difference expanded so that no
+// cancellation of O(1) terms happens in FP32.
+__device__ __forceinline__ void ch_corr_op(float p0, float ep, float em, float lphi, float lmu, float t1,
+                                           float t2, float& a1, float& a2) {
+  a1 = ep - t1 * lmu;
+  a2 = em - ep * (3.0f * (p0 * p0) + ep * (3.0f * p0 + ep)) + t2 * lphi;
+}
+
 template <int DIM, bool WRAP>
 __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double dt, double eps2,
                                                      double h, double* __restrict__ q,
@@ -320,23 +350,35 @@ __global__ __launch_bounds__(kSweepThreads) void k_relax3d_march(const DPatch* _
   if (k0 + 2 <= k1 + 1) fetch(k0 + 2, pre);
   const int t = t0 + tx, j = j0 + 1 + ty;
   const bool active = (t < nx2) && (j <= my);
-  // own-colour values of the current plane, prefetched one plane ahead
-  T own0 = (T)0, own1 = (T)0, own2 = (T)0;
+  // own-colour values of the current plane, prefetched one plane ahead:
+  // f (or the correction's right-hand side), phi (or the base phi0) and,
+  // for the correction, the cell's own e_phi
+  T own0 = (T)0, own1 = (T)0, own2 = (T)0, own3 = (T)0;
   auto own_off = [&](int k) { return rowoff(c, k, j) + t + 1 - ((c + j + k) & 1); };
   if (active) {
     const int o = own_off(k0);
     own0 = f[o];
     own1 = f[o + vs];
-    if (LIN) own2 = coef[o]; else own2 = q[o];
+    if (LIN) {
+      own2 = coef[o];
+      if (EMODE != 1) own3 = q[o];
+    } else {
+      own2 = q[o];
+    }
   }
   __syncthreads();
   for (int k = k0; k <= k1; k++) {
-    T nx0 = (T)0, nx1 = (T)0, nx2v = (T)0;
+    T nx0 = (T)0, nx1 = (T)0, nx2v = (T)0, nx3 = (T)0;
     if (active && k + 1 <= k1) {
       const int o = own_off(k + 1);
       nx0 = f[o];
       nx1 = f[o + vs];
-      if (LIN) nx2v = coef[o]; else nx2v = q[o];
+      if (LIN) {
+        nx2v = coef[o];
+        if (EMODE != 1) nx3 = q[o];
+      } else {
+        nx2v = q[o];
+      }
     }
     if (active) {
       const int ib = (c + j + k) & 1;
@@ -350,13 +392,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_relax3d_march(const DPatch* _
                      sh[skp][0][r][pr] + sh[skm][0][r][pr];
       const int o = rowoff(c, k, j) + m;
       T n0, n1;
-      if (LIN) {
-        const T a21 = -own2 - tmp3;
-        const T b1 = own0 + tmp1 * smu;
-        const T b2 = own1 - tmp2 * sphi;
-        const T det = (T)1 - a12 * a21;
-        n0 = (b1 - a12 * b2) / det;
-        n1 = (b2 - a21 * b1) / det;
+      if constexpr (LIN) {
+        ch_corr_update(own2, own3, own0, own1, smu, sphi, tmp1, tmp2, tmp3, a12, n0, n1);
       } else {
         const T phi = own2;
         const T a21 = (T)-3 * (phi * phi) - tmp3;
@@ -393,6 +430,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_relax3d_march(const DPatch* _
     own0 = nx0;
     own1 = nx1;
     own2 = nx2v;
+    own3 = nx3;
     __syncthreads();
   }
 }
@@ -635,7 +673,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
                                                      const double* __restrict__ qold,
                                                      const double* __restrict__ f,
                                                      TO* __restrict__ rf,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial,
+                                                     float* __restrict__ base = nullptr,
+                                                     const float* __restrict__ eg = nullptr) {
   constexpr int NV = (MODEL == BSAM_MODEL_TUMOUR) ? 4 : 2;
   const int lb = level_block(L);
   double sv[kNvMax] = {};
@@ -647,12 +687,35 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
       double o[kNvMax];
       op_cell<DIM, MODEL>(q, qold, P, i, j, k, m, P.dx, o);
       const int c0 = qidx(P, i, j, k);
+      // eg (CH, mixed path): the norm is that of the residual with the
+      // ghosts the FAS cycle holds, q + eg on the ghost neighbours
+      double dg[2] = {0.0, 0.0};
+      if (MODEL == BSAM_MODEL_CH && eg) {
+        const double h2 = P.dx * P.dx;
+        double smu = 0.0, sphi = 0.0;
+        auto add = [&](int ii, int jj, int kk) {
+          const int x = qidx(P, ii, jj, kk);
+          smu += (double)eg[x + P.qvs];
+          sphi += (double)eg[x];
+        };
+        if (i == 1) add(0, j, k);
+        if (i == P.mx[0]) add(i + 1, j, k);
+        if (j == 1) add(i, 0, k);
+        if (j == P.mx[1]) add(i, j + 1, k);
+        if (DIM == 3 && k == 1) add(i, j, 0);
+        if (DIM == 3 && k == P.mx[2]) add(i, j, k + 1);
+        dg[0] = (m.dt / h2) * smu;
+        dg[1] = -(m.eps2 / h2) * sphi;
+      }
 #pragma unroll
       for (int v = 0; v < NV; v++) {
         const double r = f[c0 + v * P.qvs] - o[v];
         if (rf) rf[c0 + v * P.qvs] = (TO)r;
-        sv[v] += r * r;
+        const double rn = (MODEL == BSAM_MODEL_CH && eg && v < 2) ? r + dg[v] : r;
+        sv[v] += rn * rn;
       }
+      // the base state phi0 of the next correction cycle (mixed path)
+      if (base) base[c0] = (float)q[c0];
     }
   }
   if (!partial) return;
@@ -697,7 +760,8 @@ __global__ __launch_bounds__(512) void k_ch3d_resid_march(const DPatch* __restri
                                                          const double* __restrict__ f, TO* __restrict__ out,
                                                          double* __restrict__ partial,
                                                          const double* __restrict__ qc,
-                                                         double* __restrict__ fpar) {
+                                                         double* __restrict__ fpar,
+                                                         float* __restrict__ phi0 = nullptr) {
   constexpr int TX = 64, TY = 8, NR = TY + 2, NC = 2 * TX + 2;
   // halo entries of one plane: rows 0 and NR-1 (NC cells each), cells 0 and
   // NC-1 of rows 1..TY; two fields
@@ -867,6 +931,10 @@ __global__ __launch_bounds__(512) void k_ch3d_resid_march(const DPatch* __restri
           const int oa = cell(iA, j, k), ob = cell(iB, j, k);
           out[oa] = (TO)ra[0]; out[oa + vs] = (TO)ra[1];
           out[ob] = (TO)rb[0]; out[ob + vs] = (TO)rb[1];
+          if (phi0) {
+            phi0[oa] = (float)pa;
+            phi0[ob] = (float)pb;
+          }
         }
       } else if (k & 1) {
         for (int v = 0; v < 2; v++) {
@@ -1583,21 +1651,53 @@ __host__ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog
 // the rules are independent and one launch applies them all.  Copy rules
 // are (dst, src, dst variable stride, src variable stride); interpolation
 // rules carry their post-operations in bits 8-15 of `kind`.
-template <typename T>
+//
+// REBASE (mixed path, FP64 solution): the ghost values before the call plus
+// the correction's ghosts `eg` are the ghosts the FAS cycle would hold (see
+// rebase_ghosts); each rule writes the fresh value into q and the
+// difference (old + eg - new) into eg.
+template <typename T, bool REBASE = false>
 __global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ cr, const uint16_t* __restrict__ cpost,
                                                       int nc, const GRule* __restrict__ ir, int ni, int nv,
-                                                      int homog, T* __restrict__ q, int order) {
+                                                      int homog, T* __restrict__ q, int order,
+                                                      float* __restrict__ eg = nullptr) {
   const int g = ordered_block(order) * kBlock + threadIdx.x;
+  auto put = [&](int d, T x) {
+    if (REBASE) eg[d] = (float)((q[d] + (T)eg[d]) - x);
+    q[d] = x;
+  };
   if (g < nc) {
     const int4 r = cr[g];
     const int post = cpost ? cpost[g] : 0;
-    for (int v = 0; v < nv; v++) q[r.x + v * r.z] = ghost_post<T>(q[r.y + v * r.w], post, v, homog);
+    for (int v = 0; v < nv; v++) put(r.x + v * r.z, ghost_post<T>(q[r.y + v * r.w], post, v, homog));
     return;
   }
   if (g >= nc + ni) return;
   const GRule R = ir[g - nc];
   const int kind = R.kind & 0xff, post = R.kind >> 8;
-  for (int v = 0; v < nv; v++) q[R.dst + v * R.pvs] = ghost_post<T>(interp_value<T>(R, kind, v, q), post, v, homog);
+  for (int v = 0; v < nv; v++) put(R.dst + v * R.pvs, ghost_post<T>(interp_value<T>(R, kind, v, q), post, v, homog));
+}
+
+// The ghost shell of every patch of a level (domain kFineG: the
+// (mx+2)^d box, interior skipped).  STASH 1: save = q + eg (FP64);
+// STASH 0: eg = save - q.  The staged-fill form of REBASE.
+template <int DIM, int STASH>
+__global__ __launch_bounds__(kBlock) void k_ghost_stash(LevelArgs L, int nv, double* __restrict__ q,
+                                                        float* __restrict__ eg, double* __restrict__ save) {
+  int pi, local;
+  if (!level_elem(L, level_block(L), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  const int i = local % P.nq[0], r = local / P.nq[0];
+  const int j = r % P.nq[1], k = (DIM == 3) ? r / P.nq[1] : 0;
+  const bool ghost = i == 0 || i == P.mx[0] + 1 || j == 0 || j == P.mx[1] + 1 ||
+                     (DIM == 3 && (k == 0 || k == P.mx[2] + 1));
+  if (!ghost) return;
+  const int o = qidx(P, i, j, k);
+  for (int v = 0; v < nv; v++) {
+    const int d = o + v * P.qvs;
+    if (STASH) save[d] = q[d] + (double)eg[d];
+    else eg[d] = (float)(save[d] - q[d]);
+  }
 }
 
 // Box copies of one ghost-fill stage: sibling transfers (GhostOverlap,
@@ -1968,7 +2068,22 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
   const int vs = P.qvs;
   const int o = qidx(P, i, j, k);
   float smu = 0.0f, sphi = 0.0f;
-  if (EMODE != 1) {
+  if (EMODE == 1) {
+    // no interior neighbour holds a correction yet; a ghost neighbour holds
+    // the offset of the base state's ghosts from a fresh fill (the values
+    // the FAS cycle's first sweep reads, rebase_ghosts)
+    auto add = [&](int ii, int jj, int kk) {
+      const int x = qidx(P, ii, jj, kk);
+      smu += e[x + vs];
+      sphi += e[x];
+    };
+    if (i == 1) add(0, j, k);
+    if (i == P.mx[0]) add(i + 1, j, k);
+    if (j == 1) add(i, 0, k);
+    if (j == P.mx[1]) add(i, j + 1, k);
+    if (DIM == 3 && k == 1) add(i, j, 0);
+    if (DIM == 3 && k == P.mx[2]) add(i, j, k + 1);
+  } else {
     const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
     const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
     smu = e[xm + vs] + e[xp + vs] + e[ym + vs] + e[yp + vs];
@@ -1979,11 +2094,9 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
       sphi = sphi + e[zp] + e[zm];
     }
   }
-  const float a21 = -coef[o] - tmp3;
-  const float b1 = rhs[o] + tmp1 * smu;
-  const float b2 = rhs[o + vs] - tmp2 * sphi;
-  const float det = 1.0f - a12 * a21;
-  const float n0 = (b1 - a12 * b2) / det, n1 = (b2 - a21 * b1) / det;
+  float n0, n1;
+  ch_corr_update(coef[o], EMODE == 1 ? 0.0f : e[o], rhs[o], rhs[o + vs], smu, sphi, tmp1, tmp2, tmp3, a12, n0,
+                 n1);
   e[o] = n0;
   e[o + vs] = n1;
   if (WRAP) wrap_images<DIM, float>(e, P, i, j, k, n0, n1);
@@ -2023,9 +2136,9 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
     const int i = 2 * c.ic - (n & 1), j = 2 * c.jc - ((n >> 1) & 1);
     const int k = (DIM == 3) ? 2 * c.kc - ((n >> 2) & 1) : 0;
     const int o = qidx(P, i, j, k);
-    const float ep = e[o], em = e[o + P.qvs];
-    const float a1 = ep - t1 * ulap_f<DIM>(e, P, i, j, k, P.qvs);
-    const float a2 = em - coef[o] * ep + t2 * ulap_f<DIM>(e, P, i, j, k, 0);
+    float a1, a2;
+    ch_corr_op(coef[o], e[o], e[o + P.qvs], ulap_f<DIM>(e, P, i, j, k, 0), ulap_f<DIM>(e, P, i, j, k, P.qvs), t1,
+               t2, a1, a2);
     s0 += rhs[o] - a1;
     s1 += rhs[o + P.qvs] - a2;
   }
@@ -2035,9 +2148,9 @@ __global__ __launch_bounds__(kBlock) void k_fas_rhs_lin(LevelArgs L, double dt, 
   const int opar = qidx(Q, ip, jp, kp);
   const int oc = cidx(P, c.ic, c.jc, c.kc);
   const float T1 = (float)(dt / (Q.dx * Q.dx)), T2 = (float)(eps2 / (Q.dx * Q.dx));
-  const float ep = ec[oc], em = ec[oc + P.cvs];
-  const float a1 = ep - T1 * ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, P.cvs);
-  const float a2 = em - coef[opar] * ep + T2 * ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, 0);
+  float a1, a2;
+  ch_corr_op(coef[opar], ec[oc], ec[oc + P.cvs], ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, 0),
+             ulap_cf<DIM>(ec, P, c.ic, c.jc, c.kc, P.cvs), T1, T2, a1, a2);
   rpar[opar] = w * s0 + a1;
   rpar[opar + Q.qvs] = w * s1 + a2;
 }
@@ -2215,12 +2328,7 @@ __device__ void cv_relax(const DPatch& P, int c, double dt, double eps2, T* __re
     }
     T n0, n1;
     if constexpr (LIN) {
-      const T a21 = -coef[o] - tmp3;
-      const T b1 = f[o] + tmp1 * smu;
-      const T b2 = f[o + vs] - tmp2 * sphi;
-      const T det = (T)1 - a12 * a21;
-      n0 = (b1 - a12 * b2) / det;
-      n1 = (b2 - a21 * b1) / det;
+      ch_corr_update(coef[o], q[o], f[o], f[o + vs], smu, sphi, tmp1, tmp2, tmp3, a12, n0, n1);
     } else {
       const T phi = q[o];
       const T a21 = (T)-3 * (phi * phi) - tmp3;
@@ -2274,7 +2382,7 @@ __device__ void cv_wrap(const DPatch& P, T* __restrict__ q) {
 
 template <int DIM, typename T, bool LIN>
 __device__ void cv_down(const DPatch& P, const DPatch& Q, double dt, double eps2, T* __restrict__ q,
-                        T* __restrict__ f, T* __restrict__ qc, const T* __restrict__ coef) {
+                        T* __restrict__ f, T* __restrict__ qc, T* __restrict__ coef) {
   const int ncc = P.cmx[0] * P.cmx[1] * (DIM == 3 ? P.cmx[2] : 1);
   // RestrictSolution: qc = R q, parent q(mb) = qc
   for (int g = threadIdx.x; g < ncc; g += blockDim.x) {
@@ -2286,6 +2394,9 @@ __device__ void cv_down(const DPatch& P, const DPatch& Q, double dt, double eps2
       qc[cidx(P, ic, jc, kc) + v * P.cvs] = r;
       q[qidx(Q, ip, jp, kp) + v * Q.qvs] = r;
     }
+    // the correction's base state follows the restriction: phi0 of the
+    // coarse level is R(phi0) of this one
+    if (LIN) coef[qidx(Q, ip, jp, kp)] = restrict_cell<DIM, T>(coef, P, ic, jc, kc, 0);
   }
   __syncthreads();
   cv_wrap<DIM, T>(Q, q);  // SetGhost(L-1, 0)
@@ -2316,17 +2427,17 @@ __device__ void cv_down(const DPatch& P, const DPatch& Q, double dt, double eps2
       for (int n = 0; n < nch; n++) {
         const int i = 2 * ic - (n & 1), j = 2 * jc - ((n >> 1) & 1), k = (DIM == 3) ? 2 * kc - ((n >> 2) & 1) : 0;
         const int o = qidx(P, i, j, k);
-        const T ep = q[o], em = q[o + P.qvs];
-        const T a1 = ep - t1 * ulap_f<DIM>(q, P, i, j, k, P.qvs);
-        const T a2 = em - coef[o] * ep + t2 * ulap_f<DIM>(q, P, i, j, k, 0);
+        T a1, a2;
+        ch_corr_op(coef[o], q[o], q[o + P.qvs], ulap_f<DIM>(q, P, i, j, k, 0), ulap_f<DIM>(q, P, i, j, k, P.qvs), t1,
+                   t2, a1, a2);
         s0 += f[o] - a1;
         s1 += f[o + P.qvs] - a2;
       }
       const T w = (DIM == 3) ? (T)0.125 : (T)0.25;
       const T T1 = (T)(dt / (Q.dx * Q.dx)), T2 = (T)(eps2 / (Q.dx * Q.dx));
-      const T ep = qc[oc], em = qc[oc + P.cvs];
-      const T a1 = ep - T1 * ulap_cf<DIM>(qc, P, ic, jc, kc, P.cvs);
-      const T a2 = em - coef[opar] * ep + T2 * ulap_cf<DIM>(qc, P, ic, jc, kc, 0);
+      T a1, a2;
+      ch_corr_op(coef[opar], qc[oc], qc[oc + P.cvs], ulap_cf<DIM>(qc, P, ic, jc, kc, 0),
+                 ulap_cf<DIM>(qc, P, ic, jc, kc, P.cvs), T1, T2, a1, a2);
       f[opar] = w * s0 + a1;
       f[opar + Q.qvs] = w * s1 + a2;
     } else {
@@ -2431,7 +2542,7 @@ __device__ void cv_level_relax(const DPatch& P, int nsm, double dt, double eps2,
 template <int DIM, typename T, bool LIN>
 __global__ __launch_bounds__(1024) void k_coarse_vcycle(const DPatch* __restrict__ pt, CoarseArgs A, int nsm,
                                                         T* __restrict__ q, T* __restrict__ f,
-                                                        T* __restrict__ qc, const T* __restrict__ coef) {
+                                                        T* __restrict__ qc, T* __restrict__ coef) {
   // descend
   for (int l = 0; l < A.nlev - 1; l++) {
     const DPatch P = pt[A.pid[l]];
@@ -5051,7 +5162,7 @@ extern "C" int32_t bsam_residual(bsam_ctx* c, int32_t level) {
 // Marching residual for a single large 3D CH patch; returns the number of
 // partial sums written, or 0 when the level does not qualify.
 template <typename TO>
-static int resid_march(bsam_ctx* c, int level, TO* out, double* partial) {
+static int resid_march(bsam_ctx* c, int level, TO* out, double* partial, float* base = nullptr) {
   Level& L = c->L(level);
   int nblk = 0;
   if (c->p.model != BSAM_MODEL_CH) return 0;
@@ -5060,7 +5171,7 @@ static int resid_march(bsam_ctx* c, int level, TO* out, double* partial) {
   const double h2 = L.h * L.h;
   k_ch3d_resid_march<0, TO><<<nblk, 512, 0, c->s>>>(c->d_pt, L.patches[0], kc, c->p.dt / h2,
                                                      c->p.model_params[0] / h2, c->mk(), c->q, c->f, out, partial,
-                                                     nullptr, nullptr);
+                                                     nullptr, nullptr, base);
   if (hipGetLastError() != hipSuccess) return -1;
   return nblk;
 }
@@ -5755,7 +5866,13 @@ int32_t mixed_level_relax(bsam_ctx* c, int level, int phase) {
       // post-correction fill only face ghosts are read (the coarse right-
       // hand side); edge and corner ghosts are rewritten by the post-
       // smoothing's last, full, fill.
-      if (emode == 2) continue;
+      // (CH: the ghosts it leaves are part of the state the next cycle and
+      // the norm read, those of the FAS cycle after its last SetGhost)
+      if (emode == 2) {
+        if (c->p.model == BSAM_MODEL_CH && c->finest > 0)
+          RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, kLean));
+        continue;
+      }
       const bool last = s == ns - 1 && rb == 2;
       const bool lean = !last || (phase == 0 && level > c->minlevel);
       if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, lean ? kLean : kFull));
@@ -5769,6 +5886,17 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
   RETCHK(mixed_level_relax(c, level, 0));
   if (level > c->minlevel) {
     const bool rw = restrict_wraps_parent(c, level);
+    if (level <= 0 && c->p.model == BSAM_MODEL_CH) {
+      // a level below the root is wholly covered: its base state phi0 is
+      // the restriction of this level's (the FP64 state there is stale
+      // inside the loop)
+      Level& L = c->L(level);
+      FOR_LAUNCHES(c, L, kCoarse, a, nb) {
+        DISPATCH_DIM(c->p.ndims, (k_restrict<2, float><<<nb, kBlock, 0, c->s>>>(a, 1, m->coef, m->coef, nullptr)),
+                     (k_restrict<3, float><<<nb, kBlock, 0, c->s>>>(a, 1, m->coef, m->coef, nullptr)));
+        LAUNCH_CHECK();
+      }
+    }
     RETCHK(restrict_t<float>(c, level, m->e, m->e, m->ec, rw));
     if (!rw) RETCHK(set_ghost_t<float>(c, level - 1, 0, m->e, 1));
     RETCHK(ring_t<float>(c, level, m->e, m->ec));
@@ -5795,6 +5923,59 @@ int32_t mixed_vcycle(bsam_ctx* c, int level) {
   return BSAM_OK;
 }
 
+// A level's ghosts at a new base state of the correction cycle (levels
+// 0..finest, root first).  The FAS cycle does not refill ghosts after
+// FillDown: until a level's next SetGhost, whatever reads its ghosts -- the
+// level's own first sweep and the norm (afasroutines.f90:104) on the
+// finest level, the coarse-fine interpolation of the level above during
+// that level's pre-smoothing -- sees the ghosts of the level's last
+// SetGhost in the previous cycle, filled before FillDown changed coarse
+// cells under finer patches (stale ghosts G).  In correction form those
+// are q's ghosts plus the correction's, q_g + e_g.  Every fill of e is
+// linear and reads the parent's e, so for the fills of e to give the FAS
+// cycle's values, q's ghosts must be a fresh fill F of the base state, each
+// reading the parent's fresh fill: this writes F into q and G - F into e_g
+// (a read before the level's next fill of e sees G; that fill replaces the
+// offset, and on the level above it carries the parent's G).  One gather
+// (REBASE) on flattened levels; else the shell is saved in rf, filled, and
+// differenced.
+int32_t rebase_ghosts(bsam_ctx* c, int level) {
+  Level& L = c->L(level);
+  float* e = c->mixed->e;
+  const int nv = c->p.nrvars;
+  if (L.maps_ok && c->p.batching) {
+    const DevMap& M = L.gmap[0];
+    if (M.nc1 + M.ni1 > 0) {
+      k_ghost_map<double, true><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv,
+                                                                            0, c->q, L.gorder[0], e);
+      LAUNCH_CHECK();
+    }
+    const int nc = M.nc - M.nc1, ni = M.ni - M.ni1;
+    if (nc + ni > 0) {
+      k_ghost_map<double, true><<<nblocks(nc + ni), kBlock, 0, c->s>>>(
+          M.cr + M.nc1, M.cpost ? M.cpost + M.nc1 : nullptr, nc, M.ir + M.ni1, ni, nv, 0, c->q, L.gorder[0], e);
+      LAUNCH_CHECK();
+    }
+    return BSAM_OK;
+  }
+  {
+    FOR_LAUNCHES(c, L, kFineG, a, nb) {
+      DISPATCH_DIM(c->p.ndims, (k_ghost_stash<2, 1><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, e, c->rf)),
+                   (k_ghost_stash<3, 1><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, e, c->rf)));
+      LAUNCH_CHECK();
+    }
+  }
+  RETCHK(set_ghost_t<double>(c, level, 0, c->q, 0));
+  {
+    FOR_LAUNCHES(c, L, kFineG, a, nb) {
+      DISPATCH_DIM(c->p.ndims, (k_ghost_stash<2, 0><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, e, c->rf)),
+                   (k_ghost_stash<3, 0><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, e, c->rf)));
+      LAUNCH_CHECK();
+    }
+  }
+  return BSAM_OK;
+}
+
 // FP64 residual of levels >= 0 into the FP32 right-hand side; the finest
 // level's squared sums go to d_norm.
 int32_t mixed_residuals(bsam_ctx* c) {
@@ -5806,7 +5987,12 @@ int32_t mixed_residuals(bsam_ctx* c) {
     double* part = (level == c->finest) ? c->d_partial : nullptr;
     const bool fin = level == c->finest;
     if (fin) kstat_begin(c, BSAM_KSTAT_RESID_FINEST);
-    int nbm = resid_march<float>(c, level, m->rhs, part);
+    // CH: the base state phi0 of the next correction cycle goes to coef;
+    // on a refined mesh the finest norm uses the FAS cycle's ghosts
+    // (rebase_ghosts)
+    float* base = (c->p.model == BSAM_MODEL_CH) ? m->coef : nullptr;
+    const float* eg = (c->p.model == BSAM_MODEL_CH && fin && c->finest > 0) ? m->e : nullptr;
+    int nbm = resid_march<float>(c, level, m->rhs, part, base);
     if (nbm < 0) return set_err(BSAM_EHIP, "residual kernel launch failed");
     int nb = nbm;
     if (nbm == 0) {
@@ -5820,8 +6006,8 @@ int32_t mixed_residuals(bsam_ctx* c) {
                        (k_residual<3, BSAM_MODEL_TUMOUR, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)));
         else
           DISPATCH_DIM(c->p.ndims,
-                       (k_residual<2, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)),
-                       (k_residual<3, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp)));
+                       (k_residual<2, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp, base, eg)),
+                       (k_residual<3, BSAM_MODEL_CH, float><<<nbl, kBlock, 0, c->s>>>(a, mk, c->q, c->qold, c->f, m->rhs, pp, base, eg)));
         LAUNCH_CHECK();
         nb = it.blocks();
       }
@@ -5880,7 +6066,11 @@ int32_t mixed_iteration(bsam_ctx* c) {
   // of FillDown is done once after the loop.
   if (c->finest > 0) {
     RETCHK(fill_down_range(c, BSAM_SOLUTIONFIELD, c->finest, 1));
-    RETCHK(set_ghost_t<double>(c, c->finest, 0, c->q, 0));
+    if (c->p.model == BSAM_MODEL_CH) {
+      for (int level = 0; level <= c->finest; level++) RETCHK(rebase_ghosts(c, level));
+    } else {
+      RETCHK(set_ghost_t<double>(c, c->finest, 0, c->q, 0));
+    }
   }
   RETCHK(mixed_residuals(c));
   return BSAM_OK;
@@ -5894,20 +6084,22 @@ int32_t mixed_multigrid_iterations(bsam_ctx* c, int32_t* ncycles, double* residu
   RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
   for (int level = c->finest; level >= 0; level--) RETCHK(source_and_update(c, level));
   RETCHK(bsam_fill_down(c, BSAM_SOURCEFIELD));
-  for (int level = c->minlevel; level <= c->finest; level++) {
-    Level& L = c->L(level);
-    FOR_LAUNCHES(c, L, kCell, a, nb) {
-      if (c->p.model == BSAM_MODEL_TUMOUR)
+  // the tumour model's correction operator is linearised about the step's
+  // start (3 phi^2 and t0 frozen); the CH correction works on the base
+  // state phi0 that every FP64 residual evaluation writes
+  if (c->p.model == BSAM_MODEL_TUMOUR)
+    for (int level = c->minlevel; level <= c->finest; level++) {
+      Level& L = c->L(level);
+      FOR_LAUNCHES(c, L, kCell, a, nb) {
         DISPATCH_DIM(c->p.ndims, (k_coef<2, true><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)),
                      (k_coef<3, true><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)));
-      else
-        DISPATCH_DIM(c->p.ndims, (k_coef<2><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)),
-                     (k_coef<3><<<nb, kBlock, 0, c->s>>>(a, c->q, c->qold, c->mixed->coef)));
-      LAUNCH_CHECK();
+        LAUNCH_CHECK();
+      }
     }
-  }
   k_zero_f<<<nblocks(c->nq), kBlock, 0, c->s>>>(c->mixed->e, c->nq);
   LAUNCH_CHECK();
+  if (c->p.model == BSAM_MODEL_CH && c->finest > 0)
+    for (int level = 0; level <= c->finest; level++) RETCHK(rebase_ghosts(c, level));
   RETCHK(mixed_residuals(c));
   double res = 0.0, prev = 0.0;
   bool fp64 = false;  // fell back to the FP64 FAS V-cycle
