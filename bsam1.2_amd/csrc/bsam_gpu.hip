@@ -144,6 +144,62 @@ struct LevelArgs {
 // Logical block index of a level launch (the level's tuned block order).
 __device__ __forceinline__ int level_block(const LevelArgs& L) { return ordered_block(L.order); }
 
+// floor(a / b) for 0 <= a < 2^22 and b >= 1: the hardware reciprocal's
+// estimate, corrected by one step (its error is below one), in place of
+// the integer division sequence; the flat level kernels are bound by their
+// address arithmetic.
+__device__ __forceinline__ int fdiv(int a, int b) {
+  int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+  const int r = a - q * b;
+  if (r < 0) q--;
+  else if (r >= b) q++;
+  return q;
+}
+
+// Cell (i, j, k) of element `local` of a patch's kHalf range for colour c
+// (half-row position t, then j, then k), with t; false for the padding
+// element of an odd row length.
+template <int DIM>
+__device__ __forceinline__ bool half_cell(const DPatch& P, int c, int local, int& i, int& j, int& k, int& t) {
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int row = fdiv(local, cnt);
+  t = local - row * cnt;
+  if (DIM == 3) {
+    const int kk = fdiv(row, P.mx[1]);
+    j = row - kk * P.mx[1] + 1;
+    k = kk + 1;
+  } else {
+    j = row + 1;
+    k = 0;
+  }
+  i = 2 * t + 2 - ((c + j + k) & 1);
+  return i <= P.mx[0];
+}
+
+// Offsets of a colour-c cell's stencil in the colour-split layout: the
+// other colour's row of (j, k) starts at ro; the x-neighbours are ro + t
+// and ro + t + 1, the y- and z-neighbours the own half index m = i >> 1 in
+// the rows +-1 (+-hq) and planes +-1 (+-ny hq); the cell itself is in the
+// colour-c block, one colour block (nz ny hq) away.
+struct Stencil {
+  int o, xm, xp, ym, yp, zm, zp;
+};
+template <int DIM>
+__device__ __forceinline__ Stencil stencil_of(const DPatch& P, int c, int i, int j, int k, int t) {
+  const int hq = P.hq, ny = P.nq[1], blk = P.nq[2] * ny * hq;
+  const int cc = c ^ 1, m = i >> 1;
+  const int ro = P.qoff + ((cc * P.nq[2] + k) * ny + j) * hq;
+  Stencil S;
+  S.o = ro + (c - cc) * blk + m;
+  S.xm = ro + t;
+  S.xp = ro + t + 1;
+  S.ym = ro - hq + m;
+  S.yp = ro + hq + m;
+  S.zm = (DIM == 3) ? ro - ny * hq + m : 0;
+  S.zp = (DIM == 3) ? ro + ny * hq + m : 0;
+  return S;
+}
+
 // Cell (i, j, k) of element `local` of a patch's kCell range, which runs
 // colour by colour, then row by row (j fastest, then k), then along the
 // colour's half-row, so that consecutive elements are consecutive cells of
@@ -153,12 +209,8 @@ __device__ __forceinline__ bool cell_of(const DPatch& P, int local, int& i, int&
   const int hx = (P.mx[0] + 1) >> 1;
   const int per = hx * P.mx[1] * (DIM == 3 ? P.mx[2] : 1);
   const int c = local >= per;
-  const int r = local - c * per;
-  const int t = r % hx, row = r / hx;
-  j = row % P.mx[1] + 1;
-  k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
-  i = 2 * t + 2 - ((c + j + k) & 1);
-  return i <= P.mx[0];
+  int t;
+  return half_cell<DIM>(P, c, local - c * per, i, j, k, t);
 }
 
 // Patch (position in the level's list) and local index of element
@@ -237,27 +289,21 @@ __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double 
   int pi, local;
   if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int cnt = (P.mx[0] + 1) >> 1;
-  const int t = local % cnt;
-  const int row = local / cnt;
-  const int j = row % P.mx[1] + 1;
-  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
-  const int ib = (c + j + k) & 1;
-  const int i = 2 * t + 2 - ib;
-  if (i > P.mx[0]) return;
+  int i, j, k, t;
+  if (!half_cell<DIM>(P, c, local, i, j, k, t)) return;
   const double h2 = h * h;
   const double tmp1 = dt / h2, tmp2 = eps2 / h2;
   const double tmp3 = (DIM == 3 ? 6.0 : 4.0) * eps2 / h2;
   const double a12 = (DIM == 3 ? 6.0 : 4.0) * tmp1;
   const int vs = P.qvs;
-  const int o = qidx(P, i, j, k);
+  const Stencil S = stencil_of<DIM>(P, c, i, j, k, t);
+  const int o = S.o;
   const double phi = q[o];
   const double f0 = f[o], f1 = f[o + vs];
-  const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
-  const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
+  const int xm = S.xm, xp = S.xp, ym = S.ym, yp = S.yp;
   double smu, sphi;
   if (DIM == 3) {
-    const int zm = qidx(P, i, j, k - 1), zp = qidx(P, i, j, k + 1);
+    const int zm = S.zm, zp = S.zp;
     smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs] + q[zm + vs] + q[zp + vs];
     sphi = q[xp] + q[xm] + q[yp] + q[ym] + q[zp] + q[zm];
   } else {
@@ -691,21 +737,26 @@ __global__ __launch_bounds__(kBlock) void k_residual(LevelArgs L, ModelK m,
       // ghosts the FAS cycle holds, q + eg on the ghost neighbours
       double dg[2] = {0.0, 0.0};
       if (MODEL == BSAM_MODEL_CH && eg) {
-        const double h2 = P.dx * P.dx;
-        double smu = 0.0, sphi = 0.0;
-        auto add = [&](int ii, int jj, int kk) {
-          const int x = qidx(P, ii, jj, kk);
-          smu += (double)eg[x + P.qvs];
-          sphi += (double)eg[x];
-        };
-        if (i == 1) add(0, j, k);
-        if (i == P.mx[0]) add(i + 1, j, k);
-        if (j == 1) add(i, 0, k);
-        if (j == P.mx[1]) add(i, j + 1, k);
-        if (DIM == 3 && k == 1) add(i, j, 0);
-        if (DIM == 3 && k == P.mx[2]) add(i, j, k + 1);
-        dg[0] = (m.dt / h2) * smu;
-        dg[1] = -(m.eps2 / h2) * sphi;
+        const bool edge = i == 1 || i == P.mx[0] || j == 1 || j == P.mx[1] ||
+                          (DIM == 3 && (k == 1 || k == P.mx[2]));
+        if (edge) {
+          const double h2 = P.dx * P.dx;
+          const int c = (i + j + k) & 1;
+          const Stencil S = stencil_of<DIM>(P, c, i, j, k, (i - 1) >> 1);
+          double smu = 0.0, sphi = 0.0;
+          auto add = [&](int x) {
+            smu += (double)eg[x + P.qvs];
+            sphi += (double)eg[x];
+          };
+          if (i == 1) add(S.xm);
+          if (i == P.mx[0]) add(S.xp);
+          if (j == 1) add(S.ym);
+          if (j == P.mx[1]) add(S.yp);
+          if (DIM == 3 && k == 1) add(S.zm);
+          if (DIM == 3 && k == P.mx[2]) add(S.zp);
+          dg[0] = (m.dt / h2) * smu;
+          dg[1] = -(m.eps2 / h2) * sphi;
+        }
       }
 #pragma unroll
       for (int v = 0; v < NV; v++) {
@@ -1669,7 +1720,21 @@ __global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ c
   if (g < nc) {
     const int4 r = cr[g];
     const int post = cpost ? cpost[g] : 0;
-    for (int v = 0; v < nv; v++) put(r.x + v * r.z, ghost_post<T>(q[r.y + v * r.w], post, v, homog));
+    // a plain copy (or a copy + 0.0) of an interior cell: the ghost the
+    // FAS cycle holds is the same copy of the same value, the source having
+    // been updated by the same FP64 addition, so the offset is exactly zero
+    // and neither old value needs reading
+    const bool plain = REBASE && (post == 0 || post == 1);
+    for (int v = 0; v < nv; v++) {
+      const int d = r.x + v * r.z;
+      const T x = ghost_post<T>(q[r.y + v * r.w], post, v, homog);
+      if (plain) {
+        eg[d] = 0.0f;
+        q[d] = x;
+      } else {
+        put(d, x);
+      }
+    }
     return;
   }
   if (g >= nc + ni) return;
@@ -2057,41 +2122,33 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
   int pi, local;
   if (!level_elem(L, level_block(L), pi, local)) return;
   const DPatch P = L.pt[L.patches[pi]];
-  const int cnt = (P.mx[0] + 1) >> 1;
-  const int t = local % cnt;
-  const int row = local / cnt;
-  const int j = row % P.mx[1] + 1;
-  const int k = (DIM == 3) ? row / P.mx[1] + 1 : 0;
+  int i, j, k, t;
+  if (!half_cell<DIM>(P, c, local, i, j, k, t)) return;
   const int ib = (c + j + k) & 1;
-  const int i = 2 * t + 2 - ib;
-  if (i > P.mx[0]) return;
   const int vs = P.qvs;
-  const int o = qidx(P, i, j, k);
+  const Stencil S = stencil_of<DIM>(P, c, i, j, k, t);
+  const int o = S.o;
   float smu = 0.0f, sphi = 0.0f;
   if (EMODE == 1) {
     // no interior neighbour holds a correction yet; a ghost neighbour holds
     // the offset of the base state's ghosts from a fresh fill (the values
     // the FAS cycle's first sweep reads, rebase_ghosts)
-    auto add = [&](int ii, int jj, int kk) {
-      const int x = qidx(P, ii, jj, kk);
+    auto add = [&](int x) {
       smu += e[x + vs];
       sphi += e[x];
     };
-    if (i == 1) add(0, j, k);
-    if (i == P.mx[0]) add(i + 1, j, k);
-    if (j == 1) add(i, 0, k);
-    if (j == P.mx[1]) add(i, j + 1, k);
-    if (DIM == 3 && k == 1) add(i, j, 0);
-    if (DIM == 3 && k == P.mx[2]) add(i, j, k + 1);
+    if (i == 1) add(S.xm);
+    if (i == P.mx[0]) add(S.xp);
+    if (j == 1) add(S.ym);
+    if (j == P.mx[1]) add(S.yp);
+    if (DIM == 3 && k == 1) add(S.zm);
+    if (DIM == 3 && k == P.mx[2]) add(S.zp);
   } else {
-    const int xm = qidx(P, i - 1, j, k), xp = qidx(P, i + 1, j, k);
-    const int ym = qidx(P, i, j - 1, k), yp = qidx(P, i, j + 1, k);
-    smu = e[xm + vs] + e[xp + vs] + e[ym + vs] + e[yp + vs];
-    sphi = e[xp] + e[xm] + e[yp] + e[ym];
+    smu = e[S.xm + vs] + e[S.xp + vs] + e[S.ym + vs] + e[S.yp + vs];
+    sphi = e[S.xp] + e[S.xm] + e[S.yp] + e[S.ym];
     if (DIM == 3) {
-      const int zm = qidx(P, i, j, k - 1), zp = qidx(P, i, j, k + 1);
-      smu = smu + e[zm + vs] + e[zp + vs];
-      sphi = sphi + e[zp] + e[zm];
+      smu = smu + e[S.zm + vs] + e[S.zp + vs];
+      sphi = sphi + e[S.zp] + e[S.zm];
     }
   }
   float n0, n1;
@@ -2100,15 +2157,17 @@ __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float 
   e[o] = n0;
   e[o + vs] = n1;
   if (WRAP) wrap_images<DIM, float>(e, P, i, j, k, n0, n1);
+  // the other cell of the thread's pair (2t+1, 2t+2) is the other colour's
+  // element t + ib of its row
   if (EMODE == 1) {
-    const int po = qidx(P, 2 * t + 1 + ib, j, k);
+    const int po = S.xm + ib;
     e[po] = 0.0f;
     e[po + vs] = 0.0f;
   }
   if (EMODE == 2) {
     q64[o] = q64[o] + (double)n0;
     q64[o + vs] = q64[o + vs] + (double)n1;
-    const int po = qidx(P, 2 * t + 1 + ib, j, k);
+    const int po = S.xm + ib;
     q64[po] = q64[po] + (double)e[po];
     q64[po + vs] = q64[po + vs] + (double)e[po + vs];
   }
