@@ -34,7 +34,10 @@
  *   BioGrowth/problem.f90  RelaxGrid2D 255-296, Operator2D 320-353,
  *                      Source2D 356-379, UserBC2D 623-673, Growth/Uptake
  *                      743-766
- *   bsamroutines.f90   CopyQToQold 592-605, time step 673-758
+ *   bsamroutines.f90   CopyQToQold 592-605, time step 673-758, InitFields
+ *                      1787-1856, TransferValues 1859-1951
+ *   gridutilities.f90  BiLinProlongationP1MC 242-290, TriLinProlongationP1MC
+ *                      628-705
  */
 #include "bsam_oracle.h"
 
@@ -1290,6 +1293,148 @@ int32_t orc_time_step(orc_ctx* c, int32_t first_step, int32_t* ncycles, double* 
   if (first_step && (r = orc_fill_down(c, BSAM_SOLUTIONFIELD))) return r;
   for (int level = c->minlevel; level <= c->finestlevel; level++) orc_copy_q_to_qold(c, level);
   return orc_multigrid_iterations(c, ncycles, residual, NULL, 0);
+}
+
+/* ---- regrid: fields onto a new hierarchy ---------------------------------
+ * InitFields (bsamroutines.f90:1787-1856): a new patch's qc(0:cmx+1) is the
+ * parent's q(mb-1:mb+1), and q(0:mx+1) the mass-corrected prolongation of
+ * it, BiLinProlongationP1MC / TriLinProlongationP1MC (gridutilities.f90:
+ * 242-290, 628-705): the P1 prolongation (as in CorrectFine), then each
+ * interior coarse cell's value minus the mean of its 2^d children, summed
+ * in the reference's order, added to those children.  MakeNewGrid
+ * (:1773-1779) zeroes f and rf and sets qold = q.  TransferValues
+ * (:1859-1951) then copies q over the interior overlap of every old patch
+ * of the same level.  The levels are done from the root up, each followed
+ * by its SetGhost(level, 0) (AMR, :978-989), since the next level's
+ * InitFields reads the parent's ghosts.  Levels <= 0 keep their q and qold. */
+static void init_fields_patch(orc_ctx* c, opatch* P) {
+  const opatch* Par = &c->pt[P->d.parent];
+  const int (*mb)[2] = P->d.mbounds;
+  const int cx = P->cmx[0], cy = P->cmx[1], cz = P->cmx[2];
+  const int nd = c->ndims;
+  for (int v = 0; v < c->nv; v++) {
+    if (nd == 2) {
+      for (int jc = 0; jc <= cy + 1; jc++)
+        for (int ic = 0; ic <= cx + 1; ic++)
+          QC(P, ic, jc, 1, v) = Q(Par, mb[0][0] - 1 + ic, mb[1][0] - 1 + jc, 1, v);
+      for (int j = 0; j <= P->mx[1] + 1; j++)
+        for (int i = 0; i <= P->mx[0] + 1; i++) {
+          int il = i / 2, jl = j / 2;
+          double b0 = lin_w(QC(P, il, jl, 1, v), QC(P, il + 1, jl, 1, v), i);
+          double b1 = lin_w(QC(P, il, jl + 1, 1, v), QC(P, il + 1, jl + 1, 1, v), i);
+          Q(P, i, j, 1, v) = lin_w(b0, b1, j) / 16.0;
+        }
+      for (int jc = 1; jc <= cy; jc++)
+        for (int ic = 1; ic <= cx; ic++) {
+          const int i2 = 2 * ic, j2 = 2 * jc;
+          double s = Q(P, i2, j2, 1, v);
+          s = s + Q(P, i2 - 1, j2, 1, v);
+          s = s + Q(P, i2, j2 - 1, 1, v);
+          s = s + Q(P, i2 - 1, j2 - 1, 1, v);
+          const double cor = QC(P, ic, jc, 1, v) - 0.25 * s;
+          Q(P, i2, j2, 1, v) += cor;
+          Q(P, i2 - 1, j2, 1, v) += cor;
+          Q(P, i2, j2 - 1, 1, v) += cor;
+          Q(P, i2 - 1, j2 - 1, 1, v) += cor;
+        }
+    } else {
+      for (int kc = 0; kc <= cz + 1; kc++)
+        for (int jc = 0; jc <= cy + 1; jc++)
+          for (int ic = 0; ic <= cx + 1; ic++)
+            QC(P, ic, jc, kc, v) = Q(Par, mb[0][0] - 1 + ic, mb[1][0] - 1 + jc, mb[2][0] - 1 + kc, v);
+      for (int k = 0; k <= P->mx[2] + 1; k++)
+        for (int j = 0; j <= P->mx[1] + 1; j++)
+          for (int i = 0; i <= P->mx[0] + 1; i++) {
+            int il = i / 2, jl = j / 2, kl = k / 2;
+            double cc[2];
+            for (int dk = 0; dk < 2; dk++) {
+              double b0 = lin_w(QC(P, il, jl, kl + dk, v), QC(P, il + 1, jl, kl + dk, v), i);
+              double b1 = lin_w(QC(P, il, jl + 1, kl + dk, v), QC(P, il + 1, jl + 1, kl + dk, v), i);
+              cc[dk] = lin_w(b0, b1, j);
+            }
+            Q(P, i, j, k, v) = lin_w(cc[0], cc[1], k) / 64.0;
+          }
+      for (int kc = 1; kc <= cz; kc++)
+        for (int jc = 1; jc <= cy; jc++)
+          for (int ic = 1; ic <= cx; ic++) {
+            const int i2 = 2 * ic, j2 = 2 * jc, k2 = 2 * kc;
+            double s = Q(P, i2, j2, k2, v);
+            s = s + Q(P, i2 - 1, j2, k2, v);
+            s = s + Q(P, i2, j2 - 1, k2, v);
+            s = s + Q(P, i2 - 1, j2 - 1, k2, v);
+            s = s + Q(P, i2, j2, k2 - 1, v);
+            s = s + Q(P, i2 - 1, j2, k2 - 1, v);
+            s = s + Q(P, i2, j2 - 1, k2 - 1, v);
+            s = s + Q(P, i2 - 1, j2 - 1, k2 - 1, v);
+            const double cor = QC(P, ic, jc, kc, v) - 0.125 * s;
+            for (int n = 0; n < 8; n++) Q(P, i2 - (n & 1), j2 - ((n >> 1) & 1), k2 - (n >> 2), v) += cor;
+          }
+    }
+  }
+  memcpy(P->qold, P->q, sizeof(double) * (size_t)P->nq[0] * P->nq[1] * P->nq[2] * c->nv);
+}
+
+/* Transferq / TransferOverlap: q over the interior overlap of mglobal. */
+static void transfer_patch(orc_ctx* c, const opatch* S, opatch* T) {
+  int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+  for (int a = 0; a < c->ndims; a++) {
+    lo[a] = S->d.mglobal[a][0] > T->d.mglobal[a][0] ? S->d.mglobal[a][0] : T->d.mglobal[a][0];
+    hi[a] = S->d.mglobal[a][1] < T->d.mglobal[a][1] ? S->d.mglobal[a][1] : T->d.mglobal[a][1];
+    if (hi[a] < lo[a]) return;
+  }
+  for (int v = 0; v < c->nv; v++)
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          const int ks = c->ndims == 3 ? k - S->d.mglobal[2][0] + 1 : 1;
+          const int kt = c->ndims == 3 ? k - T->d.mglobal[2][0] + 1 : 1;
+          Q(T, i - T->d.mglobal[0][0] + 1, j - T->d.mglobal[1][0] + 1, kt, v) =
+              Q(S, i - S->d.mglobal[0][0] + 1, j - S->d.mglobal[1][0] + 1, ks, v);
+        }
+}
+
+int32_t orc_regrid(orc_ctx* c, int32_t npatch, const bsam_patch_desc* d, int32_t minlevel, int32_t finestlevel) {
+  /* detach the old hierarchy */
+  opatch* old = c->pt;
+  const int oldn = c->npatch, oldmin = c->minlevel;
+  int* olist[MAXLEVELS];
+  int on[MAXLEVELS];
+  for (int l = 0; l < MAXLEVELS; l++) {
+    olist[l] = c->lvl_list[l];
+    on[l] = c->lvl_n[l];
+    c->lvl_list[l] = NULL;
+    c->lvl_n[l] = 0;
+  }
+  c->pt = NULL;
+  c->npatch = 0;
+  int r = orc_set_hierarchy(c, npatch, d, minlevel, finestlevel);
+  if (r == BSAM_OK && minlevel != oldmin) r = fail("regrid: minlevel %d differs from %d", minlevel, oldmin);
+  for (int L = minlevel; r == BSAM_OK && L <= 0; L++) {
+    const int l = lidx(L);
+    if (c->lvl_n[l] != 1 || on[l] != 1) { r = fail("regrid: level %d must hold one patch", L); break; }
+    opatch* P = &c->pt[c->lvl_list[l][0]];
+    const opatch* O = &old[olist[l][0]];
+    for (int a = 0; a < 3; a++)
+      if (P->mx[a] != O->mx[a]) r = fail("regrid: level %d changed its extent", L);
+    if (r) break;
+    const size_t n = (size_t)P->nq[0] * P->nq[1] * P->nq[2] * c->nv;
+    memcpy(P->q, O->q, n * sizeof(double));
+    memcpy(P->qold, O->qold, n * sizeof(double));
+  }
+  if (r == BSAM_OK) r = orc_set_ghost(c, 0, 0);
+  for (int L = 1; r == BSAM_OK && L <= finestlevel; L++) {
+    const int l = lidx(L);
+    for (int a = 0; a < c->lvl_n[l]; a++) init_fields_patch(c, &c->pt[c->lvl_list[l][a]]);
+    for (int a = 0; a < c->lvl_n[l]; a++)
+      for (int b = 0; b < on[l]; b++) transfer_patch(c, &old[olist[l][b]], &c->pt[c->lvl_list[l][a]]);
+    r = orc_set_ghost(c, L, 0);
+  }
+  for (int n = 0; n < oldn; n++) {
+    free(old[n].q); free(old[n].qold); free(old[n].qc); free(old[n].f); free(old[n].rf); free(old[n].ftmp);
+  }
+  free(old);
+  for (int l = 0; l < MAXLEVELS; l++) free(olist[l]);
+  return r;
 }
 
 /* ---- output frame ------------------------------------------------------ */

@@ -62,6 +62,8 @@ int32_t  orc_estimate_level_errors(orc_ctx* c, int32_t level, int32_t errflagopt
 int32_t  orc_vcycle(orc_ctx* c, int32_t level);
 int32_t  orc_multigrid_iterations(orc_ctx* c, int32_t* ncycles, double* residual,
                                   double* history, int32_t history_len);
+int32_t  orc_regrid(orc_ctx* c, int32_t npatch, const bsam_patch_desc* d,
+                    int32_t minlevel, int32_t finestlevel);
 int32_t  orc_time_step(orc_ctx* c, int32_t first_step, int32_t* ncycles,
                        double* residual);
 const char* orc_last_error(void);

@@ -52,6 +52,7 @@ def load():
             "orc_vcycle": ([_P, _I], _I),
             "orc_multigrid_iterations": ([_P, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D), _I], _I),
             "orc_time_step": ([_P, _I, C.POINTER(_I), C.POINTER(_D)], _I),
+            "orc_regrid": ([_P, _I, C.POINTER(abi.PatchDesc), _I, _I], _I),
             "orc_last_error": ([], C.c_char_p),
         }
         for n, (a, r) in sig.items():
@@ -144,6 +145,13 @@ class Oracle:
         r = C.c_double()
         rc = self._chk(self.lib.orc_time_step(self.ctx, 1 if first else 0, C.byref(n), C.byref(r)))
         return rc, n.value, r.value
+
+    def regrid(self, hier):
+        """Carry the fields onto a new hierarchy (InitFields, TransferValues)."""
+        descs = hierarchy.to_ctypes(hier)
+        lo, hi = hierarchy.levels(hier)
+        self._chk(self.lib.orc_regrid(self.ctx, len(hier), descs, lo, hi))
+        self.hier, self._descs, self.minlevel, self.finestlevel = hier, descs, lo, hi
 
 
 def format_f25(values):

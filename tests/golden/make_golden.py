@@ -62,7 +62,7 @@ FIXTURES = {
     "ch3d_amr2": dict(ndims=3, n=16, maxlevel=2, mthbc=(2,) * 6, eps=0.04, dt=1e-5, steps=1, tol=1e-9,
                       init=dict(initmode=2, seed=1008, amp=0.01, radius=0.22), sweeps=32, mgp=4, tagtol=0.3),
     "ch3d_amr2_phys": dict(ndims=3, n=8, maxlevel=2, mthbc=(2, 2, 2, 2, 10, 10), eps=0.04, dt=1e-5,
-                           steps=1, tol=1e-9, init=dict(initmode=2, seed=1009, amp=0.01, radius=0.2,
+                           steps=2, tol=1e-9, init=dict(initmode=2, seed=1009, amp=0.01, radius=0.2,
                                                         xc=(0.4, 0.5, 0.45)), sweeps=32, mgp=4, tagtol=0.3),
 }
 
