@@ -55,6 +55,13 @@ class Solver:
         self._chk(self.lib.bsam_set_hierarchy(self.ctx, len(hier), self._descs, self.minlevel,
                                               self.finestlevel))
 
+    def regrid(self, hier):
+        """Carry the fields onto a new hierarchy (bsam_regrid)."""
+        lo, hi = hierarchy.levels(hier)
+        descs = hierarchy.to_ctypes(hier)
+        self._chk(self.lib.bsam_regrid(self.ctx, len(hier), descs, lo, hi))
+        self.hier, self._descs, self.minlevel, self.finestlevel = hier, descs, lo, hi
+
     def close(self):
         if getattr(self, "ctx", None) and self.ctx.value:
             self.lib.bsam_destroy(self.ctx)

@@ -1411,8 +1411,10 @@ __global__ __launch_bounds__(kBlock) void k_frame_body(const DPatch* __restrict_
 }
 
 // CorrectFine, first half (afasroutines.f90:892-895, 934-938):
-// qc(0:cmx+1) = parent%q(mb-1:mb+1) - qc.  Domain: the qc box.
-template <int DIM, typename T>
+// qc(0:cmx+1) = parent%q(mb-1:mb+1) - qc.  ASSIGN: qc(0:cmx+1) =
+// parent%q(mb-1:mb+1), InitFields' coarse box (bsamroutines.f90:1819-1826).
+// Domain: the qc box.
+template <int DIM, typename T, bool ASSIGN = false>
 __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, const T* __restrict__ q,
                                                        T* __restrict__ qc) {
   int pi, local;
@@ -1427,7 +1429,8 @@ __global__ __launch_bounds__(kBlock) void k_correct_qc(LevelArgs L, int nv, cons
   const int kp = (DIM == 3) ? P.mb[2][0] - 1 + kc : 0;
   for (int v = 0; v < nv; v++) {
     const int o = cidx(P, ic, jc, kc) + v * P.cvs;
-    qc[o] = q[qidx(Q, ip, jp, kp) + v * Q.qvs] - qc[o];
+    const T pq = q[qidx(Q, ip, jp, kp) + v * Q.qvs];
+    qc[o] = ASSIGN ? pq : pq - qc[o];
   }
 }
 
@@ -1600,6 +1603,117 @@ __global__ __launch_bounds__(kBlock) void k_source(LevelArgs L, int mode, ModelK
 #pragma unroll
     for (int v = 0; v < NV; v++) f[o + v * P.qvs] = ftmp[o + v * P.qvs] + 0.0;
   }
+}
+
+// InitFields on the patches of a new level (bsamroutines.f90:1787-1856),
+// after their qc boxes hold the parent's values (k_correct_qc<ASSIGN>):
+// q(0:mx+1) = P1(qc) (Bi/TriLinProlongationP1, the expressions of
+// k_prolong_dual), then the 2^d children of every interior coarse cell are
+// shifted by one common correction, qc - 2^-d (their sum), summed in the
+// order of the P1MC loop (gridutilities.f90:628-705), and qold = q.  One
+// thread per interior coarse cell (ic, jc[, kc]): its children 2ic-1..2ic
+// per dimension, and the ghost cells 0 and mx+1 next to them when the cell
+// is on the patch boundary (P1 only), so every fine cell has one writer.
+// Domain: kCoarse.
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_init_fields(LevelArgs L, int nv, T* __restrict__ q,
+                                                        T* __restrict__ qold, const T* __restrict__ qc) {
+  CoarseCell cc;
+  if (!decode_coarse<DIM>(L, level_block(L), cc)) return;
+  const DPatch P = L.pt[cc.pid];
+  const int cx[3] = {cc.ic, cc.jc, cc.kc};
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int a = 0; a < DIM; a++) {
+    lo[a] = cx[a] == 1 ? 0 : 2 * cx[a] - 1;
+    hi[a] = cx[a] == P.cmx[a] ? P.mx[a] + 1 : 2 * cx[a];
+  }
+  auto p1 = [&](const T* a, int i, int j, int k) -> T {
+    const int il = i >> 1, jl = j >> 1, kl = k >> 1;
+    if (DIM == 2) {
+      const T b0 = linw<T>(a[cidx(P, il, jl, 0)], a[cidx(P, il + 1, jl, 0)], i);
+      const T b1 = linw<T>(a[cidx(P, il, jl + 1, 0)], a[cidx(P, il + 1, jl + 1, 0)], i);
+      return linw<T>(b0, b1, j) / (T)16;
+    }
+    T c2[2];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      const T b0 = linw<T>(a[cidx(P, il, jl, kl + d)], a[cidx(P, il + 1, jl, kl + d)], i);
+      const T b1 = linw<T>(a[cidx(P, il, jl + 1, kl + d)], a[cidx(P, il + 1, jl + 1, kl + d)], i);
+      c2[d] = linw<T>(b0, b1, j);
+    }
+    return linw<T>(c2[0], c2[1], k) / (T)64;
+  };
+  constexpr int nch = (DIM == 3) ? 8 : 4;
+  for (int v = 0; v < nv; v++) {
+    const T* a = qc + v * P.cvs;
+    // children (2ic - n&1, 2jc - (n>>1)&1, 2kc - n>>2) in the loop's order
+    T ch[nch];
+    T sum = (T)0;
+#pragma unroll
+    for (int n = 0; n < nch; n++) {
+      ch[n] = p1(a, 2 * cc.ic - (n & 1), 2 * cc.jc - ((n >> 1) & 1), DIM == 3 ? 2 * cc.kc - (n >> 2) : 0);
+      sum = n == 0 ? ch[0] : sum + ch[n];
+    }
+    const T cor = a[cidx(P, cc.ic, cc.jc, cc.kc)] - (DIM == 3 ? (T)0.125 : (T)0.25) * sum;
+#pragma unroll
+    for (int n = 0; n < nch; n++) {
+      const int o = qidx(P, 2 * cc.ic - (n & 1), 2 * cc.jc - ((n >> 1) & 1), DIM == 3 ? 2 * cc.kc - (n >> 2) : 0) +
+                    v * P.qvs;
+      const T x = ch[n] + cor;
+      q[o] = x;
+      qold[o] = x;
+    }
+    // the ghost cells this coarse cell owns
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          const bool child = i >= 2 * cc.ic - 1 && i <= 2 * cc.ic && j >= 2 * cc.jc - 1 && j <= 2 * cc.jc &&
+                             (DIM == 2 || (k >= 2 * cc.kc - 1 && k <= 2 * cc.kc));
+          if (child) continue;
+          const int o = qidx(P, i, j, k) + v * P.qvs;
+          const T x = p1(a, i, j, k);
+          q[o] = x;
+          qold[o] = x;
+        }
+  }
+}
+
+// TransferValues (bsamroutines.f90:1859-1951): a new patch takes q over the
+// interior overlap of its global index box with an old patch of its level.
+// One box per (new, old) pair that overlaps; the old patch's descriptor
+// travels in the box, since the old hierarchy's table is gone.
+struct XferBox {
+  DPatch S;       // old patch, offsets into the old q pool
+  int dp;         // new patch (descriptor index)
+  int slo[3];     // first overlap cell in the old patch's local indices
+  int dlo[3];     // ... and in the new patch's
+  int ext[3];     // overlap extents (1 in unused dimensions)
+  long cell0;     // first element of the box in the launch's range
+};
+
+// Domain: the cells of the launch's boxes, box after box.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_transfer(const DPatch* __restrict__ pt,
+                                                     const XferBox* __restrict__ bx, int nbox, long ncell,
+                                                     int nv, const double* __restrict__ oq,
+                                                     double* __restrict__ q) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ncell) return;
+  int lo = 0, hi = nbox;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bx[mid].cell0 <= g) lo = mid; else hi = mid;
+  }
+  const XferBox& B = bx[lo];
+  const DPatch D = pt[B.dp];
+  const long c = g - B.cell0;
+  const int x = (int)(c % B.ext[0]);
+  const long r = c / B.ext[0];
+  const int y = (int)(r % B.ext[1]);
+  const int z = (DIM == 3) ? (int)(r / B.ext[1]) : 0;
+  const int so = qidx(B.S, B.slo[0] + x, B.slo[1] + y, B.slo[2] + z);
+  const int d0 = qidx(D, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z);
+  for (int v = 0; v < nv; v++) q[d0 + v * D.qvs] = oq[so + v * B.S.qvs];
 }
 
 // CopyQToQold (bsamroutines.f90:592-605): whole arrays including ghosts.
@@ -5716,6 +5830,150 @@ extern "C" int32_t bsam_time_step(bsam_ctx* c, int32_t first_step, int32_t* ncyc
   if (first_step) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
   for (int level = c->minlevel; level <= c->finest; level++) RETCHK(bsam_copy_q_to_qold(c, level));
   return bsam_multigrid_iterations(c, ncycles, residual, nullptr, 0);
+}
+
+// GridAdapt's field set-up for a given new hierarchy (bsamroutines.f90:
+// 1663-1951): the levels at and below the root keep their one patch and its
+// q, qold; then level by level from 1 up, every new patch gets MakeNewGrid's
+// zero f and rf, InitFields' prolongation of its parent (qold = q), and
+// TransferValues' copy of q from the old patches of its level, and
+// SetGhost(level, 0) runs (AMR(level) at bsamroutines.f90:978-989 starts
+// with it, before the next level is made).  The tagging and clustering of
+// AMR change no field, so for the same new hierarchy the fields equal the
+// reference's after its regrid.
+namespace {
+struct OldPools {
+  bsam_ctx* c;
+  double *q = nullptr, *qold = nullptr;
+  XferBox* d_box = nullptr;
+  ~OldPools() {
+    hipStreamSynchronize(c->s);
+    hipFree(q);
+    hipFree(qold);
+    hipFree(d_box);
+  }
+};
+}  // namespace
+
+extern "C" int32_t bsam_regrid(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel,
+                               int32_t finestlevel) {
+  if (!c || !desc || npatch <= 0) return set_err(BSAM_EINVAL, "bad arguments");
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
+  if (minlevel != c->minlevel)
+    return set_err(BSAM_EINVAL, "regrid keeps the coarsest level %d (got %d)", c->minlevel, minlevel);
+  if (finestlevel < 0) return set_err(BSAM_EINVAL, "finest level %d below the root", finestlevel);
+  for (int lvl = minlevel; lvl <= 0; lvl++) {
+    int n = 0, id = -1;
+    for (int i = 0; i < npatch; i++)
+      if (desc[i].level == lvl) n++, id = i;
+    if (n != 1) return set_err(BSAM_EINVAL, "regrid: level %d must hold one patch (has %d)", lvl, n);
+    const DPatch& O = c->hp[c->L(lvl).patches[0]];
+    for (int a = 0; a < 3; a++)
+      if (desc[id].mx[a] != O.mx[a]) return set_err(BSAM_EINVAL, "regrid: level %d changed its extent", lvl);
+  }
+  const int nv = c->p.nrvars, nd = c->p.ndims;
+  HIPCHK(hipStreamSynchronize(c->s));
+  OldPools old{c, c->q, c->qold};
+  c->q = c->qold = nullptr;
+  const std::vector<DPatch> ohp = c->hp;
+  const std::vector<bsam_patch_desc> odesc = c->desc;
+  const int ofinest = c->finest;
+  // the old levels' patches and launch shapes (kept for the levels that remain)
+  std::vector<Level> okn;
+  for (int lvl = minlevel; lvl <= ofinest; lvl++) {
+    Level k;
+    const Level& L = c->L(lvl);
+    k.patches = L.patches;
+    k.order = L.order;
+    k.gorder[0] = L.gorder[0], k.gorder[1] = L.gorder[1];
+    k.kc_sweep[0] = L.kc_sweep[0], k.kc_sweep[1] = L.kc_sweep[1];
+    k.kc_resid = L.kc_resid;
+    okn.push_back(std::move(k));
+  }
+  RETCHK(bsam_set_hierarchy(c, npatch, desc, minlevel, finestlevel));
+  for (int lvl = minlevel; lvl <= std::min(ofinest, finestlevel); lvl++) {
+    Level& L = c->L(lvl);
+    const Level& k = okn[lvl - minlevel];
+    L.order = k.order;
+    L.gorder[0] = k.gorder[0], L.gorder[1] = k.gorder[1];
+    L.kc_sweep[0] = k.kc_sweep[0], L.kc_sweep[1] = k.kc_sweep[1];
+    L.kc_resid = k.kc_resid;
+  }
+  for (int lvl = minlevel; lvl <= 0; lvl++) {
+    const DPatch& P = c->hp[c->L(lvl).patches[0]];
+    const DPatch& O = ohp[okn[lvl - minlevel].patches[0]];
+    const size_t n = (size_t)P.qvs * nv * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->q + P.qoff, old.q + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->qold + P.qoff, old.qold + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
+  }
+  RETCHK(bsam_set_ghost(c, 0, 0));
+  // the overlap boxes of every level, level by level
+  std::vector<XferBox> boxes;
+  std::vector<size_t> b0(finestlevel + 2, 0);
+  std::vector<long> ncell(finestlevel + 1, 0);
+  for (int lvl = 1; lvl <= finestlevel; lvl++) {
+    b0[lvl] = boxes.size();
+    if (lvl <= ofinest)
+      for (int pid : c->L(lvl).patches)
+        for (int op : okn[lvl - minlevel].patches) {
+          const bsam_patch_desc& D = desc[pid];
+          const bsam_patch_desc& S = odesc[op];
+          XferBox B{};
+          B.S = ohp[op];
+          B.dp = pid;
+          bool hit = true;
+          for (int a = 0; a < 3; a++) {
+            if (a >= nd) {
+              B.slo[a] = B.dlo[a] = 0;
+              B.ext[a] = 1;
+              continue;
+            }
+            const int lo = std::max(D.mglobal[a][0], S.mglobal[a][0]);
+            const int hi = std::min(D.mglobal[a][1], S.mglobal[a][1]);
+            if (hi < lo) { hit = false; break; }
+            B.slo[a] = lo - S.mglobal[a][0] + 1;
+            B.dlo[a] = lo - D.mglobal[a][0] + 1;
+            B.ext[a] = hi - lo + 1;
+          }
+          if (!hit) continue;
+          B.cell0 = ncell[lvl];
+          ncell[lvl] += (long)B.ext[0] * B.ext[1] * B.ext[2];
+          boxes.push_back(B);
+        }
+  }
+  b0[finestlevel + 1] = boxes.size();
+  if (!boxes.empty()) {
+    HIPCHK(hipMalloc(&old.d_box, boxes.size() * sizeof(XferBox)));
+    HIPCHK(hipMemcpyAsync(old.d_box, boxes.data(), boxes.size() * sizeof(XferBox), hipMemcpyHostToDevice, c->s));
+  }
+  for (int lvl = 1; lvl <= finestlevel; lvl++) {
+    Level& L = c->L(lvl);
+    {
+      FOR_LAUNCHES(c, L, kQcBox, a, nb) {
+        DISPATCH_DIM(nd, (k_correct_qc<2, double, true><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qc)),
+                     (k_correct_qc<3, double, true><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qc)));
+        LAUNCH_CHECK();
+      }
+    }
+    {
+      FOR_LAUNCHES(c, L, kCoarse, a, nb) {
+        DISPATCH_DIM(nd, (k_init_fields<2, double><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qold, c->qc)),
+                     (k_init_fields<3, double><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qold, c->qc)));
+        LAUNCH_CHECK();
+      }
+    }
+    const int nbox = (int)(b0[lvl + 1] - b0[lvl]);
+    if (nbox > 0) {
+      const XferBox* bx = old.d_box + b0[lvl];
+      DISPATCH_DIM(nd, (k_transfer<2><<<nblocks(ncell[lvl]), kBlock, 0, c->s>>>(c->d_pt, bx, nbox, ncell[lvl], nv,
+                                                                                   old.q, c->q)),
+                   (k_transfer<3><<<nblocks(ncell[lvl]), kBlock, 0, c->s>>>(c->d_pt, bx, nbox, ncell[lvl], nv,
+                                                                             old.q, c->q)));
+      LAUNCH_CHECK();
+    }
+    RETCHK(bsam_set_ghost(c, lvl, 0));
+  }
+  return BSAM_OK;
 }
 
 extern "C" int32_t bsam_synchronize(bsam_ctx* c) {

@@ -245,6 +245,21 @@ int32_t bsam_multigrid_iterations(bsam_ctx* ctx, int32_t* ncycles,
 int32_t bsam_time_step(bsam_ctx* ctx, int32_t first_step, int32_t* ncycles,
                        double* residual);
 
+/* The field set-up of a regrid onto a given new hierarchy (GridAdapt,
+ * bsamroutines.f90:1663-1951, called from AMR at :966-991): replaces the
+ * installed hierarchy like bsam_set_hierarchy, but carries the fields.  The
+ * levels at and below the root must keep their one patch and extent (their
+ * q, qold are kept); then level by level from 1 up, the new patches get
+ * zero f and rf (MakeNewGrid), InitFields' mass-corrected P1 prolongation
+ * of their parents with qold = q (:1787-1856), TransferValues' copy of q
+ * from the old patches of the same level where their global boxes overlap
+ * (:1859-1951), and SetGhost(level, 0).  Launch shapes set for a level are
+ * kept while the level exists.  A descriptor table refused by the checks
+ * above leaves the context as it was; a later failure leaves it with no
+ * hierarchy. */
+int32_t bsam_regrid(bsam_ctx* ctx, int32_t npatch, const bsam_patch_desc* desc,
+                    int32_t minlevel, int32_t finestlevel);
+
 /* Wait for all work queued on the context's stream. */
 int32_t bsam_synchronize(bsam_ctx* ctx);
 /* Message of the last failed call on this thread ("" if none). */
