@@ -115,6 +115,7 @@ ENTRY_POINTS = {
     "bsam_multigrid_iterations": ([_P, _IP, _DP, _DP, _I], _I),
     "bsam_time_step": ([_P, _I, _IP, _DP], _I),
     "bsam_regrid": ([_P, _I, C.POINTER(PatchDesc), _I, _I], _I),
+    "bsam_new_level": ([_I, _I, C.POINTER(PatchDesc), _I, _IP, _I, _I, _D, _I, C.POINTER(PatchDesc), _IP], _I),
     "bsam_synchronize": ([_P], _I),
     "bsam_last_error": ([], C.c_char_p),
     "bsam_kstat_enable": ([_P, _I], _I),

@@ -136,6 +136,14 @@ def to_ctypes(hier):
     return arr
 
 
+def from_ctypes(p):
+    """The descriptor dict of one PatchDesc."""
+    return _desc(p.level, p.parent, [p.mx[a] for a in range(3)],
+                 [[p.mglobal[a][0], p.mglobal[a][1]] for a in range(3)],
+                 [[p.mbounds[a][0], p.mbounds[a][1]] for a in range(3)],
+                 [p.mthbc[f] for f in range(6)], p.dx, [p.xlower[a] for a in range(3)])
+
+
 def field_shape(d, ndims, nv, field):
     """numpy shape (reference order reversed: var, k, j, i) of one field."""
     mx = d["mx"]

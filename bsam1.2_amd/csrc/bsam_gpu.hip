@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/bsam_gpu.h"
+#include "bsam_grid.h"
 #include "bsam_kernels.h"
 #include "bsam_mixed.h"
 
@@ -39,6 +40,15 @@ using namespace bsam;
 static thread_local std::string g_err;
 
 static int32_t set_err(int32_t code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+int32_t bsam::fail(int32_t code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);
@@ -3375,36 +3385,12 @@ void ghost_overlap_boxes(const bsam_ctx* c, int g1, int g2, const int off[3], in
   }
 }
 
-// PeriodicSetup (boundary.f90:805-872) + GetPeriodicOffsets (:875-954).
+// PeriodicSetup + GetPeriodicOffsets of the installed root (bsam_grid.cpp).
 void periodic_offsets(const bsam_ctx* c, int level, std::vector<std::array<int, 3>>& offs, bool& periodic) {
-  static const int tab[13][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {-1, 1, 0},
-                                 {0, 1, 1}, {0, 1, -1}, {1, 0, 1}, {-1, 0, 1}, {1, 1, 1},
-                                 {1, 1, -1}, {-1, 1, 1}, {1, -1, 1}};
-  int m[6];
-  for (int i = 0; i < 6; i++) m[i] = 10;
   const bsam_patch_desc* root = nullptr;
   for (auto& d : c->desc)
     if (d.level == 0) { root = &d; break; }
-  for (int i = 0; i < 2 * c->p.ndims; i++) m[i] = root->mthbc[i];
-  std::vector<int> idx;
-  periodic = true;
-  if (m[0] == 2) {
-    if (m[2] == 2) idx = (m[4] == 2) ? std::vector<int>{1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13}
-                                     : std::vector<int>{1, 2, 4, 5};
-    else idx = (m[4] == 2) ? std::vector<int>{1, 3, 8, 9} : std::vector<int>{1};
-  } else {
-    if (m[2] == 2) idx = (m[4] == 2) ? std::vector<int>{2, 3, 6, 7} : std::vector<int>{2};
-    else if (m[4] == 2) idx = {3};
-    else periodic = false;
-  }
-  offs.clear();
-  if (level < 0) return;
-  const int s = 1 << level;
-  for (int x : idx) {
-    std::array<int, 3> o;
-    for (int a = 0; a < 3; a++) o[a] = tab[x - 1][a] * root->mx[a] * s;
-    offs.push_back(o);
-  }
+  bsam::periodic_offsets(c->p.ndims, *root, level, offs, periodic);
 }
 
 // Host plan of one level's SetGhost, built by plan_ghost_* and uploaded by

@@ -1,0 +1,382 @@
+// Regrid clustering on the host: the patches of a new level from the tagged
+// cells of the level below it, as GridAdapt's RefineGrid makes them
+// (LIBRARY/bsamroutines.f90:1366-1405): the tags are buffered (BufferAndList
+// :1206-1247, InflateEdgeTags :1267-1349), every patch of the level is split
+// into subgrids by the signature method of Berger and Rigoutsos as the
+// reference modifies it (NewSubGrids :1408-1605, GetSignatures :1608-1632,
+// GridFlagRatio :1635-1660), and each subgrid becomes a child descriptor
+// (MakeNewGrid :1663-1735).  No device code: clustering reads one int per
+// cell of the level once per split pass and is a small part of a regrid.
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+#include "bsam_grid.h"
+
+namespace bsam {
+
+void periodic_offsets(int ndims, const bsam_patch_desc& root, int level,
+                      std::vector<std::array<int, 3>>& offs, bool& periodic) {
+  // the offset table of GetPeriodicOffsets, cases 1-13
+  static const int tab[13][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {-1, 1, 0},
+                                 {0, 1, 1}, {0, 1, -1}, {1, 0, 1}, {-1, 0, 1}, {1, 1, 1},
+                                 {1, 1, -1}, {-1, 1, 1}, {1, -1, 1}};
+  int m[6];
+  for (int i = 0; i < 6; i++) m[i] = 10;
+  for (int i = 0; i < 2 * ndims; i++) m[i] = root.mthbc[i];
+  // PeriodicSetup's choice of cases from the periodic dimensions
+  std::vector<int> idx;
+  periodic = true;
+  if (m[0] == 2) {
+    if (m[2] == 2) idx = (m[4] == 2) ? std::vector<int>{1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13}
+                                     : std::vector<int>{1, 2, 4, 5};
+    else idx = (m[4] == 2) ? std::vector<int>{1, 3, 8, 9} : std::vector<int>{1};
+  } else {
+    if (m[2] == 2) idx = (m[4] == 2) ? std::vector<int>{2, 3, 6, 7} : std::vector<int>{2};
+    else if (m[4] == 2) idx = {3};
+    else periodic = false;
+  }
+  offs.clear();
+  if (level < 0) return;
+  const int s = 1 << level;
+  for (int x : idx) {
+    std::array<int, 3> o;
+    for (int a = 0; a < 3; a++) o[a] = tab[x - 1][a] * root.mx[a] * s;
+    offs.push_back(o);
+  }
+}
+
+}  // namespace bsam
+
+namespace {
+
+constexpr int kMaxSubgrids = 1024;   // maxsubgrids, nodeinfodef.f90:62
+constexpr int kMaxSplitPasses = 15;  // NewSubGrids' maxsplitpasses
+constexpr int kInternalBC = 999;     // internalbc, nodeinfodef.f90:62
+
+struct Box {
+  int lo[3], hi[3];
+};
+
+// One patch's tags, column-major over its interior (1-based access).
+struct Tags {
+  int mx[3];
+  int32_t* f;
+  int32_t& at(int i, int j, int k) const { return f[(i - 1) + (long)mx[0] * ((j - 1) + (long)mx[1] * (k - 1))]; }
+};
+
+// GetSignatures: sig[n][i] = the sum of the tags on the slab i of box b
+// across dimension n (entries outside b are left alone).
+void signatures(const Tags& T, const Box& b, int nd, std::vector<int> sig[3]) {
+  for (int n = 0; n < nd; n++)
+    for (int i = b.lo[n]; i <= b.hi[n]; i++) sig[n][i] = 0;
+  for (int k = b.lo[2]; k <= b.hi[2]; k++)
+    for (int j = b.lo[1]; j <= b.hi[1]; j++)
+      for (int i = b.lo[0]; i <= b.hi[0]; i++) {
+        const int v = T.at(i, j, k);
+        sig[0][i] += v;
+        sig[1][j] += v;
+        if (nd == 3) sig[2][k] += v;
+      }
+}
+
+// GridFlagRatio: tagged cells over cells of box b.
+double fill_ratio(const Tags& T, const Box& b, int nd) {
+  double total = 1.0;
+  long tot = 1;
+  for (int n = 0; n < nd; n++) tot *= (long)(b.hi[n] - b.lo[n] + 1);
+  total = (double)tot;
+  long flagged = 0;
+  for (int k = b.lo[2]; k <= b.hi[2]; k++)
+    for (int j = b.lo[1]; j <= b.hi[1]; j++)
+      for (int i = b.lo[0]; i <= b.hi[0]; i++) flagged += T.at(i, j, k);
+  return (double)flagged / total;
+}
+
+// NewSubGrids of one patch: the subgrid boxes (patch-local indices) in the
+// order MakeNewGrid creates them.
+int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, std::vector<Box>& sub) {
+  sub.clear();
+  if (mgp < 4) return bsam::fail(BSAM_EINVAL, "level %d: minimumgridpoints cannot be less than 4", level);
+  if (mgp % 2) return bsam::fail(BSAM_EINVAL, "level %d: minimumgridpoints must be even", level);
+  Box whole;
+  for (int n = 0; n < 3; n++) {
+    whole.lo[n] = 1;
+    whole.hi[n] = n < nd ? T.mx[n] : 1;
+  }
+  if (fill_ratio(T, whole, nd) < 1.0e-10) return BSAM_OK;
+  std::vector<int> sig[3], dd[3];
+  for (int n = 0; n < nd; n++) {
+    sig[n].assign(T.mx[n] + 2, 0);
+    dd[n].assign(T.mx[n] + 2, 0);
+  }
+  sub.push_back(whole);
+  std::vector<char> can(1, 1);
+  for (size_t g = 0; sub.size() < (size_t)kMaxSubgrids && g < sub.size(); g++) {
+    for (int pass = 0; can[g] && pass < kMaxSplitPasses; pass++) {
+      // trim untagged edge slabs (signatures of the box before trimming),
+      // then widen to whole coarse cell pairs: lo odd, hi even
+      signatures(T, sub[g], nd, sig);
+      for (int n = 0; n < nd; n++) {
+        int i1 = sub[g].lo[n], i2 = sub[g].hi[n];
+        while (sig[n][i1] == 0 && i1 < sub[g].hi[n] && i2 - i1 + 1 > mgp) i1++;
+        while (sig[n][i2] == 0 && i2 > sub[g].lo[n] && i2 - i1 + 1 > mgp) i2--;
+        sub[g].lo[n] = i1 - (i1 + 1) % 2;
+        sub[g].hi[n] = i2 + i2 % 2;
+      }
+      if (!(fill_ratio(T, sub[g], nd) < desfill)) {
+        can[g] = 0;
+        continue;
+      }
+      signatures(T, sub[g], nd, sig);
+      // a hole: the first untagged slab (last index of the first half) that
+      // leaves both halves at least mgp wide
+      int isplit[3] = {0, 0, 0};
+      bool have = false;
+      for (int n = 0; n < nd && !have; n++) {
+        const int i1 = sub[g].lo[n], i2 = sub[g].hi[n];
+        for (int i = i1 + mgp - 1; i <= i2 - mgp; i += 2)
+          if (sig[n][i] == 0 && std::min(i - i1 + 1, i2 - i) >= mgp) {
+            isplit[n] = i;
+            have = true;
+            break;
+          }
+      }
+      if (!have) {
+        // else the strongest inflection of the signatures' second
+        // difference, the most central one among equals
+        for (int n = 0; n < nd; n++)
+          for (int i = sub[g].lo[n] + 1; i <= sub[g].hi[n] - 1; i++)
+            dd[n][i] = sig[n][i - 1] - 2 * sig[n][i] + sig[n][i + 1];
+        int inflect = 0, dist0 = 0;
+        for (int n = 0; n < nd; n++) {
+          const int i1 = sub[g].lo[n], i2 = sub[g].hi[n];
+          for (int i = i1 + mgp; i <= i2 - mgp + 1; i += 2) {
+            // i: first index of the second half
+            const int del = std::abs(dd[n][i] - dd[n][i - 1]);
+            if (del > inflect) {
+              inflect = del;
+              isplit[0] = isplit[1] = isplit[2] = 0;
+              isplit[n] = i - 1;
+              have = true;
+              dist0 = std::min(i - i1, i2 - i + 1);
+            } else if (del == inflect && inflect > 0) {
+              const int dist1 = std::min(i - i1, i2 - i + 1);
+              if (dist1 > dist0) {
+                isplit[0] = isplit[1] = isplit[2] = 0;
+                isplit[n] = i - 1;
+                have = true;
+                dist0 = dist1;
+              }
+            }
+          }
+        }
+      }
+      if (!have) {
+        can[g] = 0;
+        continue;
+      }
+      for (int n = 0; n < nd; n++) {
+        if (isplit[n] > 0 && std::min(sub[g].hi[n] - isplit[n], isplit[n] - sub[g].lo[n] + 1) >= mgp) {
+          if (sub.size() >= (size_t)kMaxSubgrids)
+            return bsam::fail(BSAM_EINVAL, "level %d: more than %d subgrids in one patch", level, kMaxSubgrids);
+          Box b = sub[g];
+          b.lo[n] = isplit[n] + 1;
+          sub[g].hi[n] = isplit[n];
+          sub.push_back(b);
+          can.push_back(1);
+          break;
+        }
+      }
+    }
+  }
+  for (const Box& b : sub)
+    for (int n = 0; n < nd; n++) {
+      const int len = b.hi[n] - b.lo[n] + 1;
+      if (len < mgp) return bsam::fail(BSAM_EINVAL, "level %d: subgrid smaller than minimumgridpoints", level);
+      if (len % 2 || b.lo[n] % 2 == 0 || b.hi[n] % 2)
+        return bsam::fail(BSAM_EINVAL, "level %d: subgrid not aligned to coarse cell pairs", level);
+    }
+  return BSAM_OK;
+}
+
+// MakeNewGrid's descriptor of the child of patch `pidx` over its local box b.
+bsam_patch_desc make_child(const bsam_patch_desc& P, int pidx, const Box& b, int nd) {
+  bsam_patch_desc d{};
+  d.level = P.level + 1;
+  d.parent = pidx;
+  for (int a = 0; a < 3; a++) {
+    if (a < nd) {
+      d.mx[a] = (b.hi[a] - b.lo[a] + 1) * 2;
+      d.mbounds[a][0] = b.lo[a];
+      d.mbounds[a][1] = b.hi[a];
+      const int g0 = P.mglobal[a][0] + b.lo[a] - 1;
+      const int g1 = g0 + b.hi[a] - b.lo[a];
+      d.mglobal[a][0] = (g0 - 1) * 2 + 1;
+      d.mglobal[a][1] = g1 * 2;
+      d.xlower[a] = (double)(b.lo[a] - 1) * P.dx + P.xlower[a];
+    } else {
+      d.mx[a] = 1;
+      d.mbounds[a][0] = d.mbounds[a][1] = 1;
+      d.mglobal[a][0] = d.mglobal[a][1] = 1;
+      d.xlower[a] = 0.0;
+    }
+  }
+  // internal faces, except where the child reaches a physical (or
+  // periodic) face of its parent, whose condition it inherits
+  for (int f = 0; f < 6; f++) d.mthbc[f] = kInternalBC;
+  for (int n = 0; n < nd; n++) {
+    if (P.mthbc[2 * n] < kInternalBC && b.lo[n] == 1) d.mthbc[2 * n] = P.mthbc[2 * n];
+    if (P.mthbc[2 * n + 1] < kInternalBC && b.hi[n] == P.mx[n]) d.mthbc[2 * n + 1] = P.mthbc[2 * n + 1];
+  }
+  d.dx = P.dx / 2.0;
+  return d;
+}
+
+}  // namespace
+
+extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patch_desc* desc, int32_t level,
+                                  const int32_t* flags, int32_t ibuffer, int32_t minimumgridpoints,
+                                  double desiredfillratio, int32_t maxout, bsam_patch_desc* out,
+                                  int32_t* nout) {
+  if ((ndims != 2 && ndims != 3) || npatch <= 0 || !desc || !flags || !nout || (maxout > 0 && !out))
+    return bsam::fail(BSAM_EINVAL, "bad arguments");
+  if (level < 0) return bsam::fail(BSAM_EINVAL, "level %d: new grids are made above the root only", level);
+  if (ibuffer < 0) return bsam::fail(BSAM_EINVAL, "ibuffer must be >= 0");
+  *nout = 0;
+  const bsam_patch_desc* root = nullptr;
+  std::vector<int> lv;  // the level's patches, visiting order
+  for (int i = 0; i < npatch; i++) {
+    if (desc[i].level == 0 && !root) root = &desc[i];
+    if (desc[i].level == level) lv.push_back(i);
+  }
+  if (!root) return bsam::fail(BSAM_EINVAL, "no root patch in the table");
+  if (lv.empty()) return BSAM_OK;
+  // the level's tags, buffered in a copy
+  std::vector<long> off(lv.size() + 1, 0);
+  for (size_t a = 0; a < lv.size(); a++)
+    off[a + 1] = off[a] + (long)desc[lv[a]].mx[0] * desc[lv[a]].mx[1] * desc[lv[a]].mx[2];
+  std::vector<int32_t> tg(flags, flags + off.back());
+  std::vector<Tags> T(lv.size());
+  for (size_t a = 0; a < lv.size(); a++) {
+    for (int n = 0; n < 3; n++) T[a].mx[n] = desc[lv[a]].mx[n];
+    T[a].f = tg.data() + off[a];
+  }
+  if (ibuffer > 0) {
+    // BufferAndList: a tag whose buffer box stays inside its patch is
+    // buffered there; the others are listed in global indices
+    std::vector<std::array<int, 3>> edge;
+    for (size_t a = 0; a < lv.size(); a++) {
+      const bsam_patch_desc& P = desc[lv[a]];
+      const Tags O{{P.mx[0], P.mx[1], P.mx[2]}, const_cast<int32_t*>(flags) + off[a]};
+      for (int k = 1; k <= P.mx[2]; k++)
+        for (int j = 1; j <= P.mx[1]; j++)
+          for (int i = 1; i <= P.mx[0]; i++) {
+            if (O.at(i, j, k) != 1) continue;
+            const int idx[3] = {i, j, k};
+            int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+            bool inside = true;
+            for (int n = 0; n < ndims; n++) {
+              lo[n] = idx[n] - ibuffer;
+              hi[n] = idx[n] + ibuffer;
+              inside = inside && lo[n] >= 1 && hi[n] <= P.mx[n];
+            }
+            if (inside) {
+              for (int z = lo[2]; z <= hi[2]; z++)
+                for (int y = lo[1]; y <= hi[1]; y++)
+                  for (int x = lo[0]; x <= hi[0]; x++) T[a].at(x, y, z) = 1;
+            } else {
+              std::array<int, 3> g = {1, 1, 1};
+              for (int n = 0; n < ndims; n++) g[n] = idx[n] + P.mglobal[n][0] - 1;
+              edge.push_back(g);
+            }
+          }
+    }
+    // InflateEdgeTags: the buffer box of every listed tag, and of its
+    // periodic images, on every patch of the level it meets (a bin index
+    // over the level's global index space finds the patches)
+    if (!edge.empty()) {
+      std::vector<std::array<int, 3>> offs;
+      bool periodic = false;
+      bsam::periodic_offsets(ndims, *root, level, offs, periodic);
+      constexpr int kBin = 16;
+      int ext[3] = {1, 1, 1}, nb[3] = {1, 1, 1};
+      for (int n = 0; n < ndims; n++) {
+        ext[n] = root->mx[n] << level;
+        nb[n] = (ext[n] + kBin - 1) / kBin;
+      }
+      std::vector<std::vector<int>> bins((size_t)nb[0] * nb[1] * nb[2]);
+      auto bin_range = [&](const int* lo, const int* hi, int* b0, int* b1) {
+        for (int n = 0; n < 3; n++) {
+          if (n >= ndims) { b0[n] = b1[n] = 0; continue; }
+          b0[n] = (std::max(lo[n], 1) - 1) / kBin;
+          b1[n] = (std::min(hi[n], ext[n]) - 1) / kBin;
+        }
+      };
+      for (size_t a = 0; a < lv.size(); a++) {
+        const bsam_patch_desc& P = desc[lv[a]];
+        int lo[3], hi[3], b0[3], b1[3];
+        for (int n = 0; n < 3; n++) lo[n] = P.mglobal[n][0], hi[n] = P.mglobal[n][1];
+        bin_range(lo, hi, b0, b1);
+        for (int z = b0[2]; z <= b1[2]; z++)
+          for (int y = b0[1]; y <= b1[1]; y++)
+            for (int x = b0[0]; x <= b1[0]; x++) bins[((size_t)z * nb[1] + y) * nb[0] + x].push_back((int)a);
+      }
+      std::vector<long> stamp(lv.size(), -1);
+      long tick = 0;
+      auto apply = [&](const std::array<int, 3>& c) {
+        int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1}, b0[3], b1[3];
+        for (int n = 0; n < ndims; n++) lo[n] = c[n] - ibuffer, hi[n] = c[n] + ibuffer;
+        for (int n = 0; n < ndims; n++)
+          if (hi[n] < 1 || lo[n] > ext[n]) return;
+        bin_range(lo, hi, b0, b1);
+        tick++;
+        for (int z = b0[2]; z <= b1[2]; z++)
+          for (int y = b0[1]; y <= b1[1]; y++)
+            for (int x = b0[0]; x <= b1[0]; x++)
+              for (int a : bins[((size_t)z * nb[1] + y) * nb[0] + x]) {
+                if (stamp[a] == tick) continue;
+                stamp[a] = tick;
+                const bsam_patch_desc& P = desc[lv[a]];
+                int ml[3] = {1, 1, 1}, mh[3] = {1, 1, 1};
+                bool hit = true;
+                for (int n = 0; n < ndims && hit; n++) {
+                  const int o0 = std::max(lo[n], P.mglobal[n][0]), o1 = std::min(hi[n], P.mglobal[n][1]);
+                  hit = o1 >= o0;
+                  ml[n] = o0 - P.mglobal[n][0] + 1;
+                  mh[n] = o1 - P.mglobal[n][0] + 1;
+                }
+                if (!hit) continue;
+                for (int z2 = ml[2]; z2 <= mh[2]; z2++)
+                  for (int y2 = ml[1]; y2 <= mh[1]; y2++)
+                    for (int x2 = ml[0]; x2 <= mh[0]; x2++) T[a].at(x2, y2, z2) = 1;
+              }
+      };
+      for (const auto& c : edge) {
+        apply(c);
+        if (!periodic) continue;
+        for (int pol = -1; pol <= 1; pol += 2)
+          for (const auto& o : offs) {
+            std::array<int, 3> s = c;
+            for (int n = 0; n < ndims; n++) s[n] = c[n] + pol * o[n];
+            apply(s);
+          }
+      }
+    }
+  }
+  // RefineGrid over the level in visiting order; MakeNewGrid in that order
+  std::vector<bsam_patch_desc> made;
+  std::vector<Box> sub;
+  for (size_t a = 0; a < lv.size(); a++) {
+    const int32_t r = new_subgrids(T[a], ndims, minimumgridpoints, desiredfillratio, level, sub);
+    if (r != BSAM_OK) return r;
+    for (const Box& b : sub) made.push_back(make_child(desc[lv[a]], lv[a], b, ndims));
+  }
+  *nout = (int32_t)made.size();
+  if ((int32_t)made.size() > maxout)
+    return bsam::fail(BSAM_EINVAL, "%d new patches do not fit in %d descriptors", (int)made.size(), maxout);
+  // the tree's level list runs from the youngest node: visiting order is
+  // the reverse of creation
+  for (size_t n = 0; n < made.size(); n++) out[n] = made[made.size() - 1 - n];
+  return BSAM_OK;
+}

@@ -70,7 +70,7 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_fill_down, bsam_get_source, bsam_update_source, bsam_update_aux, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
          bsam_estimate_level_errors, bsam_estimate_level_errors_noalloc, bsam_write_frame, &
-         bsam_multigrid_iterations, bsam_time_step, bsam_regrid, bsam_synchronize, &
+         bsam_multigrid_iterations, bsam_time_step, bsam_regrid, bsam_new_level, bsam_synchronize, &
          bsam_autotune, bsam_set_knob, bsam_get_knob, bsam_set_mixed_fallback, &
          bsam_mixed_fallbacks, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
@@ -247,6 +247,16 @@ INTERFACE
     TYPE(C_PTR), VALUE:: ctx
     INTEGER(C_INT32_T), VALUE:: npatch, minlevel, finestlevel
     TYPE(bsam_patch_desc), INTENT(IN):: desc(*)
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_new_level(ndims, npatch, desc, level, flags, ibuffer, &
+      minimumgridpoints, desiredfillratio, maxout, out, nout) BIND(C, NAME='bsam_new_level')
+    IMPORT:: C_INT32_T, C_DOUBLE, bsam_patch_desc
+    INTEGER(C_INT32_T), VALUE:: ndims, npatch, level, ibuffer, minimumgridpoints, maxout
+    TYPE(bsam_patch_desc), INTENT(IN):: desc(*)
+    INTEGER(C_INT32_T), INTENT(IN):: flags(*)
+    REAL(C_DOUBLE), VALUE:: desiredfillratio
+    TYPE(bsam_patch_desc), INTENT(OUT):: out(*)
+    INTEGER(C_INT32_T), INTENT(OUT):: nout
   END FUNCTION
   INTEGER(C_INT32_T) FUNCTION bsam_synchronize(ctx) BIND(C, NAME='bsam_synchronize')
     IMPORT:: C_INT32_T, C_PTR

@@ -260,6 +260,25 @@ int32_t bsam_time_step(bsam_ctx* ctx, int32_t first_step, int32_t* ncycles,
 int32_t bsam_regrid(bsam_ctx* ctx, int32_t npatch, const bsam_patch_desc* desc,
                     int32_t minlevel, int32_t finestlevel);
 
+/* The new patches of level+1 from the tags of `level` (host only, no
+ * GPU): GridAdapt's RefineGrid over the level (bsamroutines.f90:1366-1405).
+ * `flags` are the level's tags as bsam_estimate_level_errors returns them
+ * (the level's patches in descriptor order, each column-major over mx); a
+ * copy is buffered by `ibuffer` cells (BufferAndList :1206-1247 inside a
+ * patch, InflateEdgeTags :1267-1349 across patches and periodic images),
+ * every patch of the level is split by NewSubGrids (:1408-1605, Berger-
+ * Rigoutsos signatures with the reference's trimming, hole and inflection
+ * rules, maxsubgrids 1024, 15 passes) with `minimumgridpoints` (even, >= 4)
+ * and `desiredfillratio`, and each subgrid becomes MakeNewGrid's child
+ * descriptor (:1663-1735) with `parent` = its parent's index in `desc`.
+ * The descriptors go to out[0..*nout) in the order the reference's tree
+ * visits them (the youngest first).  BSAM_EINVAL, with *nout the count,
+ * when they do not fit in `maxout`. */
+int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patch_desc* desc, int32_t level,
+                       const int32_t* flags, int32_t ibuffer, int32_t minimumgridpoints,
+                       double desiredfillratio, int32_t maxout, bsam_patch_desc* out,
+                       int32_t* nout);
+
 /* Wait for all work queued on the context's stream. */
 int32_t bsam_synchronize(bsam_ctx* ctx);
 /* Message of the last failed call on this thread ("" if none). */

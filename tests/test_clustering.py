@@ -1,0 +1,124 @@
+"""Regrid clustering (bsam_new_level: BufferAndList, InflateEdgeTags,
+NewSubGrids, MakeNewGrid; LIBRARY/bsamroutines.f90:1206-1735) against the
+meshes the reference built.
+
+In the reference's AMR(level) (bsamroutines.f90:966-991) a level is tagged
+right after its SetGhost(level, 0), and the golden vectors' qold of a step
+is q at the end of that step's AMR, ghosts included; with the probe
+module's tagging, which changes no field, it is the state every level was
+tagged on.  The probe's test (|grad phi| from central differences against
+tagtol, oracle/ref_problem/problem.f90 SetErrFlagsUser2D/3D) is restated
+here in numpy; clustering those tags with the run's grid parameters
+(ibuffer 1, minimumgridpoints 4, desiredfillratio 0.8) must give the
+reference's next level descriptor for descriptor, in its visiting order.
+This holds for the regrid of every two-step fixture (2D periodic, the
+tumour model on physical faces, 3D with two refined levels and physical z
+faces).  The first step's mesh is not replayed: there the reference runs
+FillDown between AMR and CopyQToQold (bsamroutines.f90:693-697), so the
+dumped qold of a refined level's parent is not the state it was tagged on.
+
+Closed-form checks on top, for what the fixtures leave open: a lone tag
+(hole splits and the buffer box), tags on a periodic edge (the images of
+InflateEdgeTags), and the refusals.
+"""
+import numpy as np
+import pytest
+
+import fixture_util as fu
+from bsam_amd import abi, hierarchy
+from bsam_amd.gpu import BsamError
+from bsam_amd.grid import new_level
+
+# maxlevel and tagtol of each AMR fixture (tests/golden/make_golden.py)
+RUNS = {"ch2d_amr": (2, 0.05), "ch2d_amr_phys": (2, 0.05), "ch3d_amr": (1, 0.05), "ch3d_amr_phys": (1, 0.05),
+        "ch3d_amr2": (2, 0.3), "ch3d_amr2_phys": (2, 0.3), "tm2d_amr_phys": (2, 0.05), "tm3d_amr_phys": (1, 0.05)}
+
+
+def probe_tags(meta, z, s, hier, level, tagtol):
+    nd = meta["ndims"]
+    out = []
+    for i, d in enumerate(hier):
+        if d["level"] != level:
+            continue
+        m = d["mx"][:nd]
+        nv = 4 if meta.get("tumour") else 2
+        q = z[f"s{s}_qold_p{i}"].reshape(hierarchy.field_shape(d, nd, nv, "q"))[0]
+        g2 = 0.0
+        for a in range(nd):  # gx, then gy, then gz
+            ax = nd - 1 - a
+            sp = [slice(1, 1 + x) for x in m[::-1]]
+            sm = list(sp)
+            sp[ax] = slice(2, 2 + m[a])
+            sm[ax] = slice(0, m[a])
+            g = 0.5 * (q[tuple(sp)] - q[tuple(sm)])
+            g2 = g2 + g * g
+        out.append((np.sqrt(g2) > tagtol).astype(np.int32).ravel())
+    return np.concatenate(out)
+
+
+CASES = [(n, s, lv) for n in RUNS for s in range(1, len(fu.load(n)[0]["steps"])) for lv in range(RUNS[n][0])]
+
+
+@pytest.mark.parametrize("name,step,level", CASES)
+def test_new_level_matches_reference(name, step, level):
+    meta, z = fu.load(name)
+    hier = meta["steps"][step]["hierarchy"]
+    if not any(d["level"] == level for d in hier):
+        pytest.skip("no patches on the level")
+    flags = probe_tags(meta, z, step, hier, level, RUNS[name][1])
+    got = new_level(meta["ndims"], hier, level, flags, 1, 4, 0.8)
+    want = [d for d in hier if d["level"] == level + 1]
+    assert len(got) == len(want), (len(got), len(want))
+    for k, (g, w) in enumerate(zip(got, want)):
+        for key in ("level", "parent", "mx", "mglobal", "mbounds", "mthbc", "dx", "xlower"):
+            assert g[key] == w[key], (k, key, g[key], w[key])
+
+
+def _one_patch(nd, n, mthbc):
+    return hierarchy.uniform(nd, (n,) * nd, 0, [mthbc] * (2 * nd))
+
+
+def test_lone_tag_and_buffer():
+    """One tag at (10, 10) of a 32^2 root, ibuffer 1: the buffered 3x3 box
+    (9..11)^2 is trimmed to whole coarse pairs, grown to minimumgridpoints
+    around it, and the patch is split by holes until the fill ratio holds:
+    one 4x4 child covering (9..12)^2, mx 8, on level 1 at mglobal 17..24."""
+    hier = _one_patch(2, 32, 2)
+    f = np.zeros((32, 32), np.int32)
+    f[9, 9] = 1  # (j, i) = (10, 10), 1-based
+    got = new_level(2, hier, 0, f.ravel(), 1, 4, 0.8)
+    assert len(got) == 1, got
+    g = got[0]
+    assert g["mbounds"][:2] == [[9, 12], [9, 12]] and g["mx"][:2] == [8, 8], g
+    assert g["mglobal"][:2] == [[17, 24], [17, 24]] and g["mthbc"][:4] == [999] * 4, g
+    assert g["dx"] == hier[0]["dx"] / 2 and g["xlower"][0] == 8 * hier[0]["dx"], g
+    # no buffer: trimming stops once minimumgridpoints cells remain
+    # (10..13), and the box widens to whole coarse pairs (9..14)
+    got = new_level(2, hier, 0, f.ravel(), 0, 4, 0.8)
+    assert [d["mbounds"][:2] for d in got] == [[[9, 14], [9, 14]]], got
+
+
+def test_periodic_edge_tag_images():
+    """A tag in the first column of a periodic 32^2 level-0 patch: its
+    buffer wraps to the last column (InflateEdgeTags' periodic images), so
+    two children appear, one at each side, and they inherit the periodic
+    face codes there; on a physical boundary the buffer is simply cut."""
+    f = np.zeros((32, 32), np.int32)
+    f[15, 0] = 1  # (j, i) = (16, 1)
+    per = new_level(2, _one_patch(2, 32, 2), 0, f.ravel(), 1, 4, 0.8)
+    lo = sorted(d["mbounds"][0][0] for d in per)
+    assert len(per) == 2 and lo[0] == 1 and lo[1] > 16, per
+    assert any(d["mthbc"][0] == 2 for d in per) and any(d["mthbc"][1] == 2 for d in per), per
+    phys = new_level(2, _one_patch(2, 32, 10), 0, f.ravel(), 1, 4, 0.8)
+    assert len(phys) == 1 and phys[0]["mbounds"][0][0] == 1 and phys[0]["mthbc"][0] == 10, phys
+
+
+def test_no_tags_and_refusals():
+    hier = _one_patch(3, 16, 2)
+    f = np.zeros(16 ** 3, np.int32)
+    assert new_level(3, hier, 0, f, 1, 4, 0.8) == []
+    f[0] = 1
+    for mgp in (2, 5):
+        with pytest.raises(BsamError) as ei:
+            new_level(3, hier, 0, f, 1, mgp, 0.8)
+        assert ei.value.code == abi.BSAM_EINVAL and "minimumgridpoints" in str(ei.value)
