@@ -1280,12 +1280,16 @@ __global__ __launch_bounds__(kBlock) void k_rte_flags(LevelArgs L, ModelK m, con
   }
 }
 
-// Regrid tagging, errflagopt 10: the second-difference test of the CH and
-// BioGrowth SetErrFlagsUser2D/3D (CH/problem.f90:248-307, BioGrowth/
-// problem.f90:185-240) on variable 1:
+// Regrid tagging on variable 1 of q as it is.  errflagopt 10: the
+// second-difference test of the CH and BioGrowth SetErrFlagsUser2D/3D
+// (CH/problem.f90:248-307, BioGrowth/problem.f90:185-240):
 //   |q(i+1)-2q(i)+q(i-1)| + |..j..| (+ |..k..|) > thr (= ndims*qtolerance).
+// GRAD (errflagopt 11): the gradient test of the probe PROBLEM module that
+// built the AMR workloads and the tumour model's meshes
+// (oracle/ref_problem/problem.f90 SetErrFlagsUser2D/3D):
+//   sqrt(gx^2 + gy^2 (+ gz^2)) > thr (= qtolerance), g = (q(+1)-q(-1))/2.
 // Domain: cells; every flag of the level is written.
-template <int DIM>
+template <int DIM, bool GRAD = false>
 __global__ __launch_bounds__(kBlock) void k_user_flags(LevelArgs L, const double* __restrict__ q, double thr,
                                                        const long* __restrict__ foff, int32_t* __restrict__ flags) {
   int pi, local;
@@ -1293,12 +1297,25 @@ __global__ __launch_bounds__(kBlock) void k_user_flags(LevelArgs L, const double
   const DPatch P = L.pt[L.patches[pi]];
   int i, j, k;
   if (!cell_of<DIM>(P, local, i, j, k)) return;
-  const double c0 = q[qidx(P, i, j, k)];
-  double d = fabs(q[qidx(P, i + 1, j, k)] - 2.0 * c0 + q[qidx(P, i - 1, j, k)]) +
-             fabs(q[qidx(P, i, j + 1, k)] - 2.0 * c0 + q[qidx(P, i, j - 1, k)]);
-  if (DIM == 3) d = d + fabs(q[qidx(P, i, j, k + 1)] - 2.0 * c0 + q[qidx(P, i, j, k - 1)]);
+  bool tag;
+  if (GRAD) {
+    const double gx = 0.5 * (q[qidx(P, i + 1, j, k)] - q[qidx(P, i - 1, j, k)]);
+    const double gy = 0.5 * (q[qidx(P, i, j + 1, k)] - q[qidx(P, i, j - 1, k)]);
+    double g2 = gx * gx + gy * gy;
+    if (DIM == 3) {
+      const double gz = 0.5 * (q[qidx(P, i, j, k + 1)] - q[qidx(P, i, j, k - 1)]);
+      g2 = g2 + gz * gz;
+    }
+    tag = sqrt(g2) > thr;
+  } else {
+    const double c0 = q[qidx(P, i, j, k)];
+    double d = fabs(q[qidx(P, i + 1, j, k)] - 2.0 * c0 + q[qidx(P, i - 1, j, k)]) +
+               fabs(q[qidx(P, i, j + 1, k)] - 2.0 * c0 + q[qidx(P, i, j - 1, k)]);
+    if (DIM == 3) d = d + fabs(q[qidx(P, i, j, k + 1)] - 2.0 * c0 + q[qidx(P, i, j, k - 1)]);
+    tag = d > thr;
+  }
   const int kk = (DIM == 3) ? k - 1 : 0;
-  flags[foff[pi] + ((long)kk * P.mx[1] + (j - 1)) * P.mx[0] + (i - 1)] = d > thr ? 1 : 0;
+  flags[foff[pi] + ((long)kk * P.mx[1] + (j - 1)) * P.mx[0] + (i - 1)] = tag ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero_i32(int32_t* __restrict__ a, long n) {
@@ -5532,13 +5549,15 @@ static int32_t check_tag_args(bsam_ctx* c, int32_t level, int32_t errflagopt, co
   if (!c || !flags || !ntagged) return set_err(BSAM_EINVAL, "null argument");
   RETCHK(check_level(c, level));
   if (level <= c->minlevel) return set_err(BSAM_EINVAL, "level %d has no coarser level to estimate against", level);
-  if (errflagopt != 1 && errflagopt != 10)
-    return set_err(BSAM_EINVAL, "errflagopt must be 1 (errflagdefault) or 10 (errflaguser), not %d", errflagopt);
+  if (errflagopt != 1 && errflagopt != 10 && errflagopt != 11)
+    return set_err(BSAM_EINVAL,
+                   "errflagopt must be 1 (errflagdefault), 10 (errflaguser) or 11 (gradient), not %d", errflagopt);
   // errflaguser is the second-difference test of the CH and BioGrowth
-  // modules; the tumour model defines its own user tagging (a gradient
-  // test), which is not implemented here
+  // modules; the tumour model's user tagging is the gradient test (11)
   if (errflagopt == 10 && c->p.model == BSAM_MODEL_TUMOUR)
-    return set_err(BSAM_EINVAL, "errflagopt 10 (errflaguser) is not implemented for the tumour model");
+    return set_err(BSAM_EINVAL,
+                   "errflagopt 10 (errflaguser) is not implemented for the tumour model: its user tagging is "
+                   "the gradient test, errflagopt 11");
   return BSAM_OK;
 }
 
@@ -5576,11 +5595,17 @@ static int32_t estimate_level_errors_in(bsam_ctx* c, int32_t level, int32_t errf
         k_rte_flags<2, BSAM_MODEL_BG><<<nb, kBlock, 0, c->s>>>(a, m, c->q, c->qold, c->qc, tol, d_foff, d_flags);
       LAUNCH_CHECK();
     }
-  } else {
+  } else if (errflagopt == 10) {
     const double thr = (nd == 3 ? 3.0 : 2.0) * qtolerance;
     FOR_LAUNCHES(c, L, kCell, a, nb) {
       DISPATCH_DIM(nd, (k_user_flags<2><<<nb, kBlock, 0, c->s>>>(a, c->q, thr, d_foff, d_flags)),
                    (k_user_flags<3><<<nb, kBlock, 0, c->s>>>(a, c->q, thr, d_foff, d_flags)));
+      LAUNCH_CHECK();
+    }
+  } else {
+    FOR_LAUNCHES(c, L, kCell, a, nb) {
+      DISPATCH_DIM(nd, (k_user_flags<2, true><<<nb, kBlock, 0, c->s>>>(a, c->q, qtolerance, d_foff, d_flags)),
+                   (k_user_flags<3, true><<<nb, kBlock, 0, c->s>>>(a, c->q, qtolerance, d_foff, d_flags)));
       LAUNCH_CHECK();
     }
   }

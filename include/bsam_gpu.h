@@ -193,7 +193,12 @@ int32_t bsam_copy_q_to_qold(bsam_ctx* ctx, int32_t level);
  *     BioGrowth SetErrFlagsUser2D/3D (CH/problem.f90:248-307,
  *     BioGrowth/problem.f90:185-240) on variable 1 of q as it is, against
  *     ndims*qtolerance (BSAM_EINVAL for the tumour model, whose user
- *     tagging is a different test).
+ *     tagging is a different test);
+ *   errflagopt 11: the gradient test of the probe PROBLEM module that
+ *     built the AMR workloads and is the tumour model's user tagging
+ *     (oracle/ref_problem/problem.f90 SetErrFlagsUser2D/3D), on variable 1
+ *     of q as it is: sqrt(gx^2 + gy^2 (+ gz^2)) > qtolerance with
+ *     g = (q(+1) - q(-1))/2.
  * flags (host, int32) receives info%errorflags(1:mx) of every patch of the
  * level, patches in descriptor order, each column-major; *ntagged the
  * number of flagged cells.  level must be above minlevel.  Synchronous. */

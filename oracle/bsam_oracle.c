@@ -1124,15 +1124,19 @@ int32_t orc_copy_q_to_qold(orc_ctx* c, int32_t level) {
  * SetErrFlagsUser2D/3D (CH/problem.f90:248-307, BioGrowth/problem.f90:
  * 185-240) on variable 1 of q as it is:
  *   |q(i+1)-2q(i)+q(i-1)| + |..j..| (+ |..k..|) > ndims*qtolerance.
+ * errflagopt 11: the gradient test of the probe PROBLEM module's
+ * SetErrFlagsUser2D/3D (oracle/ref_problem/problem.f90), the user tagging
+ * of the tumour model and of the AMR workloads' meshes, on variable 1:
+ *   sqrt(gx^2 + gy^2 (+ gz^2)) > qtolerance, g = (q(+1) - q(-1))/2.
  * flags: the level's patches in level order, each column-major over 1..mx;
  * *ntagged counts the flagged cells. */
 int32_t orc_estimate_level_errors(orc_ctx* c, int32_t level, int32_t errflagopt, double qtolerance,
                                   int32_t* flags, int64_t* ntagged) {
   if (!flags || !ntagged) return fail("null argument");
   if (level <= c->minlevel || level > c->finestlevel) return fail("bad level %d", level);
-  if (errflagopt != 1 && errflagopt != 10) return fail("errflagopt must be 1 or 10");
+  if (errflagopt != 1 && errflagopt != 10 && errflagopt != 11) return fail("errflagopt must be 1, 10 or 11");
   if (errflagopt == 10 && c->p.model == BSAM_MODEL_TUMOUR)
-    return fail("errflagopt 10 is not implemented for the tumour model");
+    return fail("errflagopt 10 is not implemented for the tumour model: its user tagging is errflagopt 11");
   if (errflagopt == 1) {
     orc_restrict_solution(c, level);
     int r = orc_set_ghost(c, level - 1, 0);
@@ -1174,6 +1178,19 @@ int32_t orc_estimate_level_errors(orc_ctx* c, int32_t level, int32_t errflagopt,
                 for (int j = 2 * jc - 1; j <= 2 * jc; j++)
                   for (int i = 2 * ic - 1; i <= 2 * ic; i++) FL(i, j, k) = 1;
             }
+          }
+    } else if (errflagopt == 11) {
+      for (int k = 1; k <= mz; k++)
+        for (int j = 1; j <= my; j++)
+          for (int i = 1; i <= mx; i++) {
+            const double gx = 0.5 * (Q(P, i + 1, j, k, 0) - Q(P, i - 1, j, k, 0));
+            const double gy = 0.5 * (Q(P, i, j + 1, k, 0) - Q(P, i, j - 1, k, 0));
+            double g2 = gx * gx + gy * gy;
+            if (c->ndims == 3) {
+              const double gz = 0.5 * (Q(P, i, j, k + 1, 0) - Q(P, i, j, k - 1, 0));
+              g2 = g2 + gz * gz;
+            }
+            FL(i, j, k) = sqrt(g2) > qtolerance ? 1 : 0;
           }
     } else {
       const double thr = (c->ndims == 3 ? 3.0 : 2.0) * qtolerance;

@@ -6,16 +6,18 @@ In the reference's AMR(level) (bsamroutines.f90:966-991) a level is tagged
 right after its SetGhost(level, 0), and the golden vectors' qold of a step
 is q at the end of that step's AMR, ghosts included; with the probe
 module's tagging, which changes no field, it is the state every level was
-tagged on.  The probe's test (|grad phi| from central differences against
-tagtol, oracle/ref_problem/problem.f90 SetErrFlagsUser2D/3D) is restated
-here in numpy; clustering those tags with the run's grid parameters
-(ibuffer 1, minimumgridpoints 4, desiredfillratio 0.8) must give the
-reference's next level descriptor for descriptor, in its visiting order.
-This holds for the regrid of every two-step fixture (2D periodic, the
-tumour model on physical faces, 3D with two refined levels and physical z
-faces).  The first step's mesh is not replayed: there the reference runs
-FillDown between AMR and CopyQToQold (bsamroutines.f90:693-697), so the
-dumped qold of a refined level's parent is not the state it was tagged on.
+tagged on.  The oracle's errflagopt 11 (the probe's test: |grad phi| from
+central differences against tagtol, oracle/ref_problem/problem.f90
+SetErrFlagsUser2D/3D) tags that state, and clustering those tags with the
+run's grid parameters (ibuffer 1, minimumgridpoints 4, desiredfillratio
+0.8) must give the reference's next level descriptor for descriptor, in
+its visiting order: the reference's mesh pins the tagging and the
+clustering together.  This holds for the regrid of every two-step fixture
+(2D periodic, the tumour model on physical faces, 3D with two refined
+levels and physical z faces).  The first step's mesh is not replayed:
+there the reference runs FillDown between AMR and CopyQToQold
+(bsamroutines.f90:693-697), so the dumped qold of a refined level's parent
+is not the state it was tagged on.
 
 Closed-form checks on top, for what the fixtures leave open: a lone tag
 (hole splits and the buffer box), tags on a periodic edge (the images of
@@ -25,6 +27,7 @@ import numpy as np
 import pytest
 
 import fixture_util as fu
+import oracle
 from bsam_amd import abi, hierarchy
 from bsam_amd.gpu import BsamError
 from bsam_amd.grid import new_level
@@ -34,26 +37,13 @@ RUNS = {"ch2d_amr": (2, 0.05), "ch2d_amr_phys": (2, 0.05), "ch3d_amr": (1, 0.05)
         "ch3d_amr2": (2, 0.3), "ch3d_amr2_phys": (2, 0.3), "tm2d_amr_phys": (2, 0.05), "tm3d_amr_phys": (1, 0.05)}
 
 
-def probe_tags(meta, z, s, hier, level, tagtol):
-    nd = meta["ndims"]
-    out = []
-    for i, d in enumerate(hier):
-        if d["level"] != level:
-            continue
-        m = d["mx"][:nd]
-        nv = 4 if meta.get("tumour") else 2
-        q = z[f"s{s}_qold_p{i}"].reshape(hierarchy.field_shape(d, nd, nv, "q"))[0]
-        g2 = 0.0
-        for a in range(nd):  # gx, then gy, then gz
-            ax = nd - 1 - a
-            sp = [slice(1, 1 + x) for x in m[::-1]]
-            sm = list(sp)
-            sp[ax] = slice(2, 2 + m[a])
-            sm[ax] = slice(0, m[a])
-            g = 0.5 * (q[tuple(sp)] - q[tuple(sm)])
-            g2 = g2 + g * g
-        out.append((np.sqrt(g2) > tagtol).astype(np.int32).ravel())
-    return np.concatenate(out)
+def oracle_tags(meta, z, s, level, tagtol):
+    st = meta["steps"][s]
+    O = oracle.Oracle(fu.step_params(meta), st["hierarchy"])
+    for i in st["src"]:
+        O.put(i, "q", z[f"s{s}_qold_p{i}"].reshape(O.get(i, "q").shape))
+    flags, _ = O.estimate_level_errors(level, 11, tagtol)
+    return flags
 
 
 CASES = [(n, s, lv) for n in RUNS for s in range(1, len(fu.load(n)[0]["steps"])) for lv in range(RUNS[n][0])]
@@ -65,7 +55,7 @@ def test_new_level_matches_reference(name, step, level):
     hier = meta["steps"][step]["hierarchy"]
     if not any(d["level"] == level for d in hier):
         pytest.skip("no patches on the level")
-    flags = probe_tags(meta, z, step, hier, level, RUNS[name][1])
+    flags = oracle_tags(meta, z, step, level, RUNS[name][1])
     got = new_level(meta["ndims"], hier, level, flags, 1, 4, 0.8)
     want = [d for d in hier if d["level"] == level + 1]
     assert len(got) == len(want), (len(got), len(want))
