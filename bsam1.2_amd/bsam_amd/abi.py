@@ -57,6 +57,34 @@ class PatchDesc(C.Structure):
     ]
 
 
+MAX_AMR_LEVELS = 16
+
+
+class GridParams(C.Structure):
+    """bsam_grid_params (include/bsam_gpu.h)."""
+    _fields_ = [
+        ("maxlevel", C.c_int32),
+        ("errflagopt", C.c_int32 * MAX_AMR_LEVELS),
+        ("ibuffer", C.c_int32 * MAX_AMR_LEVELS),
+        ("minimumgridpoints", C.c_int32 * MAX_AMR_LEVELS),
+        ("qtolerance", C.c_double * MAX_AMR_LEVELS),
+        ("desiredfillratios", C.c_double * MAX_AMR_LEVELS),
+    ]
+
+
+def grid_params(maxlevel, errflagopt, qtolerance, ibuffer=1, minimumgridpoints=8, desiredfillratio=0.8):
+    """The same grid parameters on every level below maxlevel."""
+    g = GridParams()
+    g.maxlevel = maxlevel
+    for lv in range(maxlevel):
+        g.errflagopt[lv] = errflagopt
+        g.qtolerance[lv] = qtolerance
+        g.ibuffer[lv] = ibuffer
+        g.minimumgridpoints[lv] = minimumgridpoints
+        g.desiredfillratios[lv] = desiredfillratio
+    return g
+
+
 class Params(C.Structure):
     """bsam_params (include/bsam_gpu.h)."""
     _fields_ = [
@@ -115,6 +143,9 @@ ENTRY_POINTS = {
     "bsam_multigrid_iterations": ([_P, _IP, _DP, _DP, _I], _I),
     "bsam_time_step": ([_P, _I, _IP, _DP], _I),
     "bsam_regrid": ([_P, _I, C.POINTER(PatchDesc), _I, _I], _I),
+    "bsam_amr": ([_P, C.POINTER(GridParams)], _I),
+    "bsam_get_hierarchy": ([_P, _I, C.POINTER(PatchDesc), _IP, _IP, _IP], _I),
+    "bsam_time_step_amr": ([_P, C.POINTER(GridParams), _I, _IP, _DP], _I),
     "bsam_new_level": ([_I, _I, C.POINTER(PatchDesc), _I, _IP, _I, _I, _D, _I, C.POINTER(PatchDesc), _IP], _I),
     "bsam_synchronize": ([_P], _I),
     "bsam_last_error": ([], C.c_char_p),

@@ -62,6 +62,33 @@ class Solver:
         self._chk(self.lib.bsam_regrid(self.ctx, len(hier), descs, lo, hi))
         self.hier, self._descs, self.minlevel, self.finestlevel = hier, descs, lo, hi
 
+    def get_hierarchy(self):
+        """The installed descriptors, as dicts."""
+        n, lo, hi = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.bsam_get_hierarchy(self.ctx, 0, None, C.byref(n), C.byref(lo), C.byref(hi)))
+        out = (abi.PatchDesc * n.value)()
+        self._chk(self.lib.bsam_get_hierarchy(self.ctx, n.value, out, C.byref(n), C.byref(lo), C.byref(hi)))
+        return [hierarchy.from_ctypes(out[i]) for i in range(n.value)]
+
+    def _took_mesh(self):
+        self.hier = self.get_hierarchy()
+        self._descs = hierarchy.to_ctypes(self.hier)
+        self.minlevel, self.finestlevel = hierarchy.levels(self.hier)
+
+    def amr(self, grid):
+        """AMR(rootlevel) with abi.GridParams `grid` (bsam_amr); the new mesh
+        becomes self.hier."""
+        self._chk(self.lib.bsam_amr(self.ctx, C.byref(grid)))
+        self._took_mesh()
+
+    def time_step_amr(self, grid, first):
+        n = C.c_int32()
+        r = C.c_double()
+        rc = self._chk(self.lib.bsam_time_step_amr(self.ctx, C.byref(grid), 1 if first else 0, C.byref(n),
+                                                   C.byref(r)))
+        self._took_mesh()
+        return rc, n.value, r.value
+
     def close(self):
         if getattr(self, "ctx", None) and self.ctx.value:
             self.lib.bsam_destroy(self.ctx)

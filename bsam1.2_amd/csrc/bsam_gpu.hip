@@ -3094,7 +3094,6 @@ struct bsam_ctx {
   double *q = nullptr, *qold = nullptr, *f = nullptr, *rf = nullptr, *ftmp = nullptr, *qc = nullptr;
   long nq = 0, nc = 0;
   std::vector<Level> lv;  // index level - minlevel
-  std::vector<BoxDesc*> box_allocs;
   // reductions
   double* d_partial = nullptr;
   double* d_partial2 = nullptr;
@@ -3301,27 +3300,31 @@ static void drop_graph(bsam_ctx* c) {
   }
 }
 
+// The device arrays of one level (launch lists, ghost-fill stages, rules
+// and tables); the level is left empty.
+static void free_level(Level& L) {
+  hipFree(L.d_patches);
+  hipFree(L.d_foff);
+  hipFree(L.d_rules);
+  for (DevMap& m : L.gmap) {
+    hipFree(m.cr);
+    hipFree(m.cpost);
+    hipFree(m.ir);
+  }
+  for (int d = 0; d < kNumDom; d++) {
+    hipFree(L.d_prefix[d]);
+    hipFree(L.d_cnt[d]);
+    hipFree(L.d_bp[d]);
+  }
+  for (Stage& st : L.stages) hipFree(st.d);
+  L = Level();
+}
+
 static void free_hierarchy(bsam_ctx* c) {
   if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
-  for (Level& L : c->lv) {
-    hipFree(L.d_patches);
-    hipFree(L.d_foff);
-    hipFree(L.d_rules);
-    for (DevMap& m : L.gmap) {
-      hipFree(m.cr);
-      hipFree(m.cpost);
-      hipFree(m.ir);
-    }
-    for (int d = 0; d < kNumDom; d++) {
-      hipFree(L.d_prefix[d]);
-      hipFree(L.d_cnt[d]);
-      hipFree(L.d_bp[d]);
-    }
-  }
+  for (Level& L : c->lv) free_level(L);
   c->lv.clear();
-  for (BoxDesc* b : c->box_allocs) hipFree(b);
-  c->box_allocs.clear();
   hipFree(c->d_pt); hipFree(c->q); hipFree(c->qold); hipFree(c->f); hipFree(c->rf); hipFree(c->ftmp);
   hipFree(c->qc);
   c->d_pt = nullptr; c->q = c->qold = c->f = c->rf = c->ftmp = c->qc = nullptr;
@@ -4249,7 +4252,6 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
   for (const auto& boxes : G.stages) {
     BoxDesc* d = nullptr;
     HIPCHK(hipMalloc(&d, boxes.size() * sizeof(BoxDesc)));
-    c->box_allocs.push_back(d);
     HIPCHK(hipMemcpy(d, boxes.data(), boxes.size() * sizeof(BoxDesc), hipMemcpyHostToDevice));
     Stage s;
     s.d = d;
@@ -4401,12 +4403,144 @@ int32_t plan_patches(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, i
 }
 }  // namespace
 
-extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc,
-                                      int32_t minlevel, int32_t finestlevel) {
-  if (!c) return set_err(BSAM_EINVAL, "bad arguments");
-  HIPCHK(hipStreamSynchronize(c->s));
-  free_hierarchy(c);
+namespace {
+// The device side of one level of a new hierarchy: its patch list, the
+// block-aligned launch ranges of every domain, the tagging offsets and the
+// ghost fill (stages, interpolation rules, flattened tables).
+int32_t build_level(bsam_ctx* c, Level& L) {
   const int nv = c->p.nrvars;
+  HIPCHK(hipMalloc(&L.d_patches, L.patches.size() * sizeof(int)));
+  HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), L.patches.size() * sizeof(int), hipMemcpyHostToDevice));
+  for (int d = 0; d < kNumDom; d++) {
+    std::vector<int> pre(L.patches.size() + 1, 0), cnts(L.patches.size(), 0);
+    long acc = 0;
+    for (size_t a = 0; a < L.patches.size(); a++) {
+      const DPatch& P = c->hp[L.patches[a]];
+      long cnt;
+      switch (d) {
+        case kHalf: cnt = (long)((P.mx[0] + 1) / 2) * P.mx[1] * P.mx[2]; break;
+        case kCell: cnt = 2L * ((P.mx[0] + 1) / 2) * P.mx[1] * P.mx[2]; break;
+        case kCoarse: cnt = (long)P.cmx[0] * P.cmx[1] * P.cmx[2]; break;
+        case kFineG: cnt = (long)P.nq[0] * P.nq[1] * P.nq[2]; break;
+        case kDual:
+          cnt = (long)(P.cmx[0] + 1) * (P.cmx[1] + 1) * (c->p.ndims == 3 ? P.cmx[2] + 1 : 1);
+          break;
+        default: cnt = (long)P.nc[0] * P.nc[1] * P.nc[2]; break;
+      }
+      // every patch's range starts on a block boundary
+      pre[a] = (int)acc;
+      cnts[a] = (int)cnt;
+      acc += (cnt + kBlock - 1) / kBlock * kBlock;
+    }
+    pre[L.patches.size()] = (int)acc;
+    if (acc > 0x7fffffffL) return set_err(BSAM_EINVAL, "level %d too large", L.level);
+    L.total[d] = (int)acc;
+    L.hprefix[d] = pre;
+    L.hcnt[d] = cnts;
+    HIPCHK(hipMalloc(&L.d_prefix[d], pre.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(L.d_prefix[d], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&L.d_cnt[d], cnts.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(L.d_cnt[d], cnts.data(), cnts.size() * sizeof(int), hipMemcpyHostToDevice));
+    // block -> patch, for whole-level launches
+    std::vector<int> bp(std::max(1, nblocks(acc)), 0);
+    size_t pi = 0;
+    for (size_t b = 0; b < bp.size(); b++) {
+      const long g0 = (long)b * kBlock;
+      while (pi + 1 < L.patches.size() && pre[pi + 1] <= g0) pi++;
+      bp[b] = (int)pi;
+    }
+    HIPCHK(hipMalloc(&L.d_bp[d], bp.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(L.d_bp[d], bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  // contiguous span of the level in the q pools (patches are laid out in
+  // descriptor order; levels are spans only if their patches are
+  // consecutive -- otherwise copies go patch by patch)
+  L.qspan_off = c->hp[L.patches[0]].qoff;
+  long end = L.qspan_off;
+  bool contiguous = true;
+  for (int pid : L.patches) {
+    if (c->hp[pid].qoff != end) contiguous = false;
+    end = c->hp[pid].qoff + (long)c->hp[pid].qvs * nv;
+  }
+  L.qspan_len = contiguous ? end - L.qspan_off : -1;
+  // dense cell offsets of the level's patches (the tagging flags)
+  {
+    std::vector<long> foff(L.patches.size());
+    long n = 0;
+    for (size_t a = 0; a < L.patches.size(); a++) {
+      foff[a] = n;
+      const DPatch& P = c->hp[L.patches[a]];
+      n += (long)P.mx[0] * P.mx[1] * P.mx[2];
+    }
+    L.ncell = n;
+    HIPCHK(hipMalloc(&L.d_foff, foff.size() * sizeof(long)));
+    HIPCHK(hipMemcpy(L.d_foff, foff.data(), foff.size() * sizeof(long), hipMemcpyHostToDevice));
+  }
+  GhostPlan G;
+  RETCHK(plan_ghost_stages(c, L, G));
+  plan_interp_rules(c, L, G);
+  if (c->p.batching) RETCHK(plan_ghost_maps(c, L, G));
+  RETCHK(upload_ghost(c, L, G));
+  return BSAM_OK;
+}
+
+bool same_desc(const bsam_patch_desc& a, const bsam_patch_desc& b) {
+  if (a.level != b.level || a.parent != b.parent || a.dx != b.dx) return false;
+  for (int i = 0; i < 3; i++) {
+    if (a.mx[i] != b.mx[i] || a.xlower[i] != b.xlower[i]) return false;
+    for (int e = 0; e < 2; e++)
+      if (a.mglobal[i][e] != b.mglobal[i][e] || a.mbounds[i][e] != b.mbounds[i][e]) return false;
+  }
+  for (int f = 0; f < 6; f++)
+    if (a.mthbc[f] != b.mthbc[f]) return false;
+  return true;
+}
+
+// The highest level up to which a new descriptor table leaves the
+// installed hierarchy as it is (minlevel - 1 if none): the installed
+// table's patches of the levels up to it are its first n rows, the new
+// table starts with the same n rows, and no later row of the new table is
+// on those levels.  Such a level's device side depends on no higher level
+// (its ghost fill reads its parent level, and the transfers between levels
+// belong to the finer one), and its pool offsets are unchanged.
+int kept_levels(const bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel,
+                int32_t finestlevel) {
+  int keep = minlevel - 1;
+  if (!c->have_hier || c->minlevel != minlevel) return keep;
+  int n = 0;
+  for (int l = minlevel; l <= std::min(c->finest, finestlevel); l++) {
+    const Level& L = c->lv[l - c->minlevel];
+    for (size_t a = 0; a < L.patches.size(); a++) {
+      const int i = n + (int)a;
+      if (L.patches[a] != i || i >= npatch || !same_desc(desc[i], c->desc[i])) return keep;
+    }
+    n += (int)L.patches.size();
+    for (int i = n; i < npatch; i++)
+      if (desc[i].level <= l) return keep;
+    keep = l;
+  }
+  return keep;
+}
+
+// Install a hierarchy with zero fields.  The levels kept_levels() finds
+// unchanged keep their device side (and launch shapes); the others are
+// built.  Returns the highest kept level through *kept.
+int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel, int32_t finestlevel,
+                int* kept) {
+  HIPCHK(hipStreamSynchronize(c->s));
+  const int keep = (desc && npatch > 0) ? kept_levels(c, npatch, desc, minlevel, finestlevel) : minlevel - 1;
+  if (kept) *kept = keep;
+  struct Held {
+    std::vector<Level> v;
+    ~Held() {
+      for (Level& L : v) free_level(L);
+    }
+  } held;
+  for (int l = minlevel; l <= keep; l++) {
+    held.v.push_back(std::move(c->L(l)));
+    c->L(l) = Level();
+  }
+  free_hierarchy(c);
   RETCHK(plan_patches(c, npatch, desc, minlevel, finestlevel));
   const long qo = c->nq, co = c->nc;
   HIPCHK(hipMalloc(&c->d_pt, npatch * sizeof(DPatch)));
@@ -4415,87 +4549,21 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
   for (double** pp : pools) {
     if (hipMalloc(pp, qo * sizeof(double)) != hipSuccess)
       return set_err(BSAM_ENOMEM, "device pool of %ld doubles", qo);
-    HIPCHK(hipMemset(*pp, 0, qo * sizeof(double)));
+    HIPCHK(hipMemsetAsync(*pp, 0, qo * sizeof(double), c->s));
   }
   if (hipMalloc(&c->qc, co * sizeof(double)) != hipSuccess) return set_err(BSAM_ENOMEM, "qc pool");
-  HIPCHK(hipMemset(c->qc, 0, co * sizeof(double)));
-  // levels
+  HIPCHK(hipMemsetAsync(c->qc, 0, co * sizeof(double), c->s));
   int maxblocks = 1;
-  for (Level& L : c->lv) {
-    HIPCHK(hipMalloc(&L.d_patches, L.patches.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), L.patches.size() * sizeof(int), hipMemcpyHostToDevice));
-    for (int d = 0; d < kNumDom; d++) {
-      std::vector<int> pre(L.patches.size() + 1, 0), cnts(L.patches.size(), 0);
-      long acc = 0;
-      for (size_t a = 0; a < L.patches.size(); a++) {
-        const DPatch& P = c->hp[L.patches[a]];
-        long cnt;
-        switch (d) {
-          case kHalf: cnt = (long)((P.mx[0] + 1) / 2) * P.mx[1] * P.mx[2]; break;
-          case kCell: cnt = 2L * ((P.mx[0] + 1) / 2) * P.mx[1] * P.mx[2]; break;
-          case kCoarse: cnt = (long)P.cmx[0] * P.cmx[1] * P.cmx[2]; break;
-          case kFineG: cnt = (long)P.nq[0] * P.nq[1] * P.nq[2]; break;
-          case kDual:
-            cnt = (long)(P.cmx[0] + 1) * (P.cmx[1] + 1) * (c->p.ndims == 3 ? P.cmx[2] + 1 : 1);
-            break;
-          default: cnt = (long)P.nc[0] * P.nc[1] * P.nc[2]; break;
-        }
-        // every patch's range starts on a block boundary
-        pre[a] = (int)acc;
-        cnts[a] = (int)cnt;
-        acc += (cnt + kBlock - 1) / kBlock * kBlock;
-      }
-      pre[L.patches.size()] = (int)acc;
-      if (acc > 0x7fffffffL) return set_err(BSAM_EINVAL, "level %d too large", L.level);
-      L.total[d] = (int)acc;
-      L.hprefix[d] = pre;
-      L.hcnt[d] = cnts;
-      HIPCHK(hipMalloc(&L.d_prefix[d], pre.size() * sizeof(int)));
-      HIPCHK(hipMemcpy(L.d_prefix[d], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
-      HIPCHK(hipMalloc(&L.d_cnt[d], cnts.size() * sizeof(int)));
-      HIPCHK(hipMemcpy(L.d_cnt[d], cnts.data(), cnts.size() * sizeof(int), hipMemcpyHostToDevice));
-      // block -> patch, for whole-level launches
-      std::vector<int> bp(std::max(1, nblocks(acc)), 0);
-      size_t pi = 0;
-      for (size_t b = 0; b < bp.size(); b++) {
-        const long g0 = (long)b * kBlock;
-        while (pi + 1 < L.patches.size() && pre[pi + 1] <= g0) pi++;
-        bp[b] = (int)pi;
-      }
-      HIPCHK(hipMalloc(&L.d_bp[d], bp.size() * sizeof(int)));
-      HIPCHK(hipMemcpy(L.d_bp[d], bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
+  for (int l = minlevel; l <= finestlevel; l++) {
+    Level& L = c->L(l);
+    if (l <= keep) {
+      L = std::move(held.v[l - minlevel]);
+      held.v[l - minlevel] = Level();
+    } else {
+      RETCHK(build_level(c, L));
     }
     // per-patch launches round every patch up to whole blocks
     maxblocks = std::max(maxblocks, nblocks(L.total[kCell]) + (int)L.patches.size());
-    // contiguous span of the level in the q pools (patches are laid out in
-    // descriptor order; levels are spans only if their patches are
-    // consecutive -- otherwise copies go patch by patch)
-    L.qspan_off = c->hp[L.patches[0]].qoff;
-    long end = L.qspan_off;
-    bool contiguous = true;
-    for (int pid : L.patches) {
-      if (c->hp[pid].qoff != end) contiguous = false;
-      end = c->hp[pid].qoff + (long)c->hp[pid].qvs * nv;
-    }
-    L.qspan_len = contiguous ? end - L.qspan_off : -1;
-    // dense cell offsets of the level's patches (the tagging flags)
-    {
-      std::vector<long> foff(L.patches.size());
-      long n = 0;
-      for (size_t a = 0; a < L.patches.size(); a++) {
-        foff[a] = n;
-        const DPatch& P = c->hp[L.patches[a]];
-        n += (long)P.mx[0] * P.mx[1] * P.mx[2];
-      }
-      L.ncell = n;
-      HIPCHK(hipMalloc(&L.d_foff, foff.size() * sizeof(long)));
-      HIPCHK(hipMemcpy(L.d_foff, foff.data(), foff.size() * sizeof(long), hipMemcpyHostToDevice));
-    }
-    GhostPlan G;
-    RETCHK(plan_ghost_stages(c, L, G));
-    plan_interp_rules(c, L, G);
-    if (c->p.batching) RETCHK(plan_ghost_maps(c, L, G));
-    RETCHK(upload_ghost(c, L, G));
   }
   plan_chain(c);
   if (maxblocks > c->partial_cap) {
@@ -4508,6 +4576,24 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
   c->have_hier = true;
   if (c->p.precision == BSAM_PREC_MIXED) {
     RETCHK(bsam::mixed_init(c));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return BSAM_OK;
+}
+}  // namespace
+
+extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc,
+                                      int32_t minlevel, int32_t finestlevel) {
+  if (!c) return set_err(BSAM_EINVAL, "bad arguments");
+  int keep = minlevel - 1;
+  RETCHK(install(c, npatch, desc, minlevel, finestlevel, &keep));
+  // a fresh install: the kept levels' launch shapes go back to the defaults
+  for (int l = minlevel; l <= keep; l++) {
+    Level& L = c->L(l);
+    L.order = 0;
+    L.gorder[0] = L.gorder[1] = 0;
+    L.kc_sweep[0] = L.kc_sweep[1] = 0;
+    L.kc_resid = 0;
   }
   return BSAM_OK;
 }
@@ -5843,29 +5929,152 @@ extern "C" int32_t bsam_time_step(bsam_ctx* c, int32_t first_step, int32_t* ncyc
   return bsam_multigrid_iterations(c, ncycles, residual, nullptr, 0);
 }
 
-// GridAdapt's field set-up for a given new hierarchy (bsamroutines.f90:
-// 1663-1951): the levels at and below the root keep their one patch and its
-// q, qold; then level by level from 1 up, every new patch gets MakeNewGrid's
-// zero f and rf, InitFields' prolongation of its parent (qold = q), and
-// TransferValues' copy of q from the old patches of its level, and
-// SetGhost(level, 0) runs (AMR(level) at bsamroutines.f90:978-989 starts
-// with it, before the next level is made).  The tagging and clustering of
-// AMR change no field, so for the same new hierarchy the fields equal the
-// reference's after its regrid.
+// ---------------------------------------------------------------------------
+// regrid: GridAdapt's field set-up (bsamroutines.f90:1663-1951) and AMR
+// ---------------------------------------------------------------------------
 namespace {
-struct OldPools {
+// The mesh a regrid replaces: its q pools (the source of TransferValues),
+// descriptors, and each level's patches and launch shapes.  Taking it
+// detaches the pools from the context; they are freed with this object.
+struct OldMesh {
   bsam_ctx* c;
   double *q = nullptr, *qold = nullptr;
+  std::vector<DPatch> hp;
+  std::vector<bsam_patch_desc> desc;
+  std::vector<Level> lv;  // patches and launch shapes only
+  int minlevel = 0, finest = -1;
   XferBox* d_box = nullptr;
-  ~OldPools() {
+  size_t box_cap = 0;
+  explicit OldMesh(bsam_ctx* c_) : c(c_) {
+    q = c->q;
+    qold = c->qold;
+    c->q = c->qold = nullptr;
+    hp = c->hp;
+    desc = c->desc;
+    minlevel = c->minlevel;
+    finest = c->finest;
+    for (int l = minlevel; l <= finest; l++) {
+      const Level& L = c->L(l);
+      Level k;
+      k.patches = L.patches;
+      k.order = L.order;
+      k.gorder[0] = L.gorder[0], k.gorder[1] = L.gorder[1];
+      k.kc_sweep[0] = L.kc_sweep[0], k.kc_sweep[1] = L.kc_sweep[1];
+      k.kc_resid = L.kc_resid;
+      lv.push_back(std::move(k));
+    }
+  }
+  ~OldMesh() {
     hipStreamSynchronize(c->s);
     hipFree(q);
     hipFree(qold);
     hipFree(d_box);
   }
+  const Level* level(int l) const { return (l >= minlevel && l <= finest) ? &lv[l - minlevel] : nullptr; }
+  // the launch shapes set for a level carry over while the level exists
+  void restore_shapes(int l) const {
+    const Level* k = level(l);
+    if (!k || l < c->minlevel || l > c->finest) return;
+    Level& L = c->L(l);
+    L.order = k->order;
+    L.gorder[0] = k->gorder[0], L.gorder[1] = k->gorder[1];
+    L.kc_sweep[0] = k->kc_sweep[0], L.kc_sweep[1] = k->kc_sweep[1];
+    L.kc_resid = k->kc_resid;
+  }
 };
+
+// The levels at and below the root keep their one patch each, with the
+// same extents: q and qold come over from the old pools.
+int32_t carry_root_levels(bsam_ctx* c, const OldMesh& o) {
+  const int nv = c->p.nrvars;
+  for (int lvl = c->minlevel; lvl <= 0; lvl++) {
+    const DPatch& P = c->hp[c->L(lvl).patches[0]];
+    const DPatch& O = o.hp[o.level(lvl)->patches[0]];
+    const size_t n = (size_t)P.qvs * nv * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->q + P.qoff, o.q + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->qold + P.qoff, o.qold + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
+  }
+  return BSAM_OK;
+}
+
+// A level of new patches above the root: InitFields' prolongation of the
+// parents with qold = q (bsamroutines.f90:1787-1856), then TransferValues'
+// copy of q from the old mesh's patches of the level over the interior
+// overlap of their global boxes (:1859-1951).  f and rf stay zero
+// (MakeNewGrid).
+int32_t init_level(bsam_ctx* c, int lvl, OldMesh& o) {
+  const int nv = c->p.nrvars, nd = c->p.ndims;
+  Level& L = c->L(lvl);
+  {
+    FOR_LAUNCHES(c, L, kQcBox, a, nb) {
+      DISPATCH_DIM(nd, (k_correct_qc<2, double, true><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qc)),
+                   (k_correct_qc<3, double, true><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qc)));
+      LAUNCH_CHECK();
+    }
+  }
+  {
+    FOR_LAUNCHES(c, L, kCoarse, a, nb) {
+      DISPATCH_DIM(nd, (k_init_fields<2, double><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qold, c->qc)),
+                   (k_init_fields<3, double><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qold, c->qc)));
+      LAUNCH_CHECK();
+    }
+  }
+  const Level* old = o.level(lvl);
+  if (!old) return BSAM_OK;
+  std::vector<XferBox> boxes;
+  long ncell = 0;
+  for (int pid : L.patches)
+    for (int op : old->patches) {
+      const bsam_patch_desc& D = c->desc[pid];
+      const bsam_patch_desc& S = o.desc[op];
+      XferBox B{};
+      B.S = o.hp[op];
+      B.dp = pid;
+      bool hit = true;
+      for (int a = 0; a < 3; a++) {
+        if (a >= nd) {
+          B.slo[a] = B.dlo[a] = 0;
+          B.ext[a] = 1;
+          continue;
+        }
+        const int lo = std::max(D.mglobal[a][0], S.mglobal[a][0]);
+        const int hi = std::min(D.mglobal[a][1], S.mglobal[a][1]);
+        if (hi < lo) { hit = false; break; }
+        B.slo[a] = lo - S.mglobal[a][0] + 1;
+        B.dlo[a] = lo - D.mglobal[a][0] + 1;
+        B.ext[a] = hi - lo + 1;
+      }
+      if (!hit) continue;
+      B.cell0 = ncell;
+      ncell += (long)B.ext[0] * B.ext[1] * B.ext[2];
+      boxes.push_back(B);
+    }
+  if (boxes.empty()) return BSAM_OK;
+  // the box table of the previous level may still be read: wait for it
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (boxes.size() > o.box_cap) {
+    hipFree(o.d_box);
+    o.d_box = nullptr;
+    o.box_cap = 0;
+    HIPCHK(hipMalloc(&o.d_box, boxes.size() * sizeof(XferBox)));
+    o.box_cap = boxes.size();
+  }
+  HIPCHK(hipMemcpy(o.d_box, boxes.data(), boxes.size() * sizeof(XferBox), hipMemcpyHostToDevice));
+  const int nbox = (int)boxes.size();
+  DISPATCH_DIM(nd, (k_transfer<2><<<nblocks(ncell), kBlock, 0, c->s>>>(c->d_pt, o.d_box, nbox, ncell, nv, o.q, c->q)),
+               (k_transfer<3><<<nblocks(ncell), kBlock, 0, c->s>>>(c->d_pt, o.d_box, nbox, ncell, nv, o.q, c->q)));
+  LAUNCH_CHECK();
+  return BSAM_OK;
+}
 }  // namespace
 
+// GridAdapt's field set-up for a given new hierarchy: the levels at and
+// below the root keep their one patch and its q, qold; then level by level
+// from 1 up, the new patches are made (init_level) and SetGhost(level, 0)
+// runs (AMR(level) at bsamroutines.f90:978-989 starts with it, before the
+// next level is made).  The tagging and clustering of AMR change no field
+// (with errflagopt 10 or 11), so for the same new hierarchy the fields equal
+// the reference's after its regrid.
 extern "C" int32_t bsam_regrid(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel,
                                int32_t finestlevel) {
   if (!c || !desc || npatch <= 0) return set_err(BSAM_EINVAL, "bad arguments");
@@ -5882,109 +6091,120 @@ extern "C" int32_t bsam_regrid(bsam_ctx* c, int32_t npatch, const bsam_patch_des
     for (int a = 0; a < 3; a++)
       if (desc[id].mx[a] != O.mx[a]) return set_err(BSAM_EINVAL, "regrid: level %d changed its extent", lvl);
   }
-  const int nv = c->p.nrvars, nd = c->p.ndims;
   HIPCHK(hipStreamSynchronize(c->s));
-  OldPools old{c, c->q, c->qold};
-  c->q = c->qold = nullptr;
-  const std::vector<DPatch> ohp = c->hp;
-  const std::vector<bsam_patch_desc> odesc = c->desc;
-  const int ofinest = c->finest;
-  // the old levels' patches and launch shapes (kept for the levels that remain)
-  std::vector<Level> okn;
-  for (int lvl = minlevel; lvl <= ofinest; lvl++) {
-    Level k;
-    const Level& L = c->L(lvl);
-    k.patches = L.patches;
-    k.order = L.order;
-    k.gorder[0] = L.gorder[0], k.gorder[1] = L.gorder[1];
-    k.kc_sweep[0] = L.kc_sweep[0], k.kc_sweep[1] = L.kc_sweep[1];
-    k.kc_resid = L.kc_resid;
-    okn.push_back(std::move(k));
-  }
-  RETCHK(bsam_set_hierarchy(c, npatch, desc, minlevel, finestlevel));
-  for (int lvl = minlevel; lvl <= std::min(ofinest, finestlevel); lvl++) {
-    Level& L = c->L(lvl);
-    const Level& k = okn[lvl - minlevel];
-    L.order = k.order;
-    L.gorder[0] = k.gorder[0], L.gorder[1] = k.gorder[1];
-    L.kc_sweep[0] = k.kc_sweep[0], L.kc_sweep[1] = k.kc_sweep[1];
-    L.kc_resid = k.kc_resid;
-  }
-  for (int lvl = minlevel; lvl <= 0; lvl++) {
-    const DPatch& P = c->hp[c->L(lvl).patches[0]];
-    const DPatch& O = ohp[okn[lvl - minlevel].patches[0]];
-    const size_t n = (size_t)P.qvs * nv * sizeof(double);
-    HIPCHK(hipMemcpyAsync(c->q + P.qoff, old.q + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(hipMemcpyAsync(c->qold + P.qoff, old.qold + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
-  }
+  OldMesh old(c);
+  RETCHK(install(c, npatch, desc, minlevel, finestlevel, nullptr));
+  for (int lvl = minlevel; lvl <= finestlevel; lvl++) old.restore_shapes(lvl);
+  RETCHK(carry_root_levels(c, old));
   RETCHK(bsam_set_ghost(c, 0, 0));
-  // the overlap boxes of every level, level by level
-  std::vector<XferBox> boxes;
-  std::vector<size_t> b0(finestlevel + 2, 0);
-  std::vector<long> ncell(finestlevel + 1, 0);
   for (int lvl = 1; lvl <= finestlevel; lvl++) {
-    b0[lvl] = boxes.size();
-    if (lvl <= ofinest)
-      for (int pid : c->L(lvl).patches)
-        for (int op : okn[lvl - minlevel].patches) {
-          const bsam_patch_desc& D = desc[pid];
-          const bsam_patch_desc& S = odesc[op];
-          XferBox B{};
-          B.S = ohp[op];
-          B.dp = pid;
-          bool hit = true;
-          for (int a = 0; a < 3; a++) {
-            if (a >= nd) {
-              B.slo[a] = B.dlo[a] = 0;
-              B.ext[a] = 1;
-              continue;
-            }
-            const int lo = std::max(D.mglobal[a][0], S.mglobal[a][0]);
-            const int hi = std::min(D.mglobal[a][1], S.mglobal[a][1]);
-            if (hi < lo) { hit = false; break; }
-            B.slo[a] = lo - S.mglobal[a][0] + 1;
-            B.dlo[a] = lo - D.mglobal[a][0] + 1;
-            B.ext[a] = hi - lo + 1;
-          }
-          if (!hit) continue;
-          B.cell0 = ncell[lvl];
-          ncell[lvl] += (long)B.ext[0] * B.ext[1] * B.ext[2];
-          boxes.push_back(B);
-        }
-  }
-  b0[finestlevel + 1] = boxes.size();
-  if (!boxes.empty()) {
-    HIPCHK(hipMalloc(&old.d_box, boxes.size() * sizeof(XferBox)));
-    HIPCHK(hipMemcpyAsync(old.d_box, boxes.data(), boxes.size() * sizeof(XferBox), hipMemcpyHostToDevice, c->s));
-  }
-  for (int lvl = 1; lvl <= finestlevel; lvl++) {
-    Level& L = c->L(lvl);
-    {
-      FOR_LAUNCHES(c, L, kQcBox, a, nb) {
-        DISPATCH_DIM(nd, (k_correct_qc<2, double, true><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qc)),
-                     (k_correct_qc<3, double, true><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qc)));
-        LAUNCH_CHECK();
-      }
-    }
-    {
-      FOR_LAUNCHES(c, L, kCoarse, a, nb) {
-        DISPATCH_DIM(nd, (k_init_fields<2, double><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qold, c->qc)),
-                     (k_init_fields<3, double><<<nb, kBlock, 0, c->s>>>(a, nv, c->q, c->qold, c->qc)));
-        LAUNCH_CHECK();
-      }
-    }
-    const int nbox = (int)(b0[lvl + 1] - b0[lvl]);
-    if (nbox > 0) {
-      const XferBox* bx = old.d_box + b0[lvl];
-      DISPATCH_DIM(nd, (k_transfer<2><<<nblocks(ncell[lvl]), kBlock, 0, c->s>>>(c->d_pt, bx, nbox, ncell[lvl], nv,
-                                                                                   old.q, c->q)),
-                   (k_transfer<3><<<nblocks(ncell[lvl]), kBlock, 0, c->s>>>(c->d_pt, bx, nbox, ncell[lvl], nv,
-                                                                             old.q, c->q)));
-      LAUNCH_CHECK();
-    }
+    RETCHK(init_level(c, lvl, old));
     RETCHK(bsam_set_ghost(c, lvl, 0));
   }
   return BSAM_OK;
+}
+
+// AMR(rootlevel) (bsamroutines.f90:966-991), the regrid that opens every
+// time step (:672-688): level by level from the root, SetGhost(level, 0),
+// EstimateLevelErrors (bsam_estimate_level_errors, with its effect on q and
+// qold for errflagopt 1) below maxlevel, the new patches of level+1 from the
+// tags (bsam_new_level), and their fields (init_level), until a level makes
+// no new patches.  Each new level is installed on top of the levels below
+// it, which keep their device side and fields; the old mesh stays the
+// source of TransferValues until the pass ends.
+extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
+  if (!c || !gp) return set_err(BSAM_EINVAL, "bad arguments");
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
+  if (gp->maxlevel < 0 || gp->maxlevel >= BSAM_MAX_AMR_LEVELS)
+    return set_err(BSAM_EINVAL, "maxlevel %d outside [0, %d)", gp->maxlevel, BSAM_MAX_AMR_LEVELS);
+  for (int l = 0; l < gp->maxlevel; l++) {
+    const int o = gp->errflagopt[l];
+    if (o != 1 && o != 10 && o != 11) return set_err(BSAM_EINVAL, "level %d: errflagopt %d", l, o);
+    if (gp->ibuffer[l] < 0) return set_err(BSAM_EINVAL, "level %d: ibuffer %d", l, gp->ibuffer[l]);
+    if (gp->minimumgridpoints[l] < 4 || gp->minimumgridpoints[l] % 2)
+      return set_err(BSAM_EINVAL, "level %d: minimumgridpoints must be even and at least 4", l);
+  }
+  const int minlevel = c->minlevel, nd = c->p.ndims;
+  HIPCHK(hipStreamSynchronize(c->s));
+  OldMesh old(c);
+  // the new table starts with the old levels at and below the root
+  std::vector<bsam_patch_desc> H;
+  std::vector<int> at(old.desc.size(), -1);
+  for (int lvl = minlevel; lvl <= 0; lvl++)
+    for (int pid : old.level(lvl)->patches) {
+      bsam_patch_desc d = old.desc[pid];
+      if (d.parent >= 0) d.parent = at[d.parent];
+      at[pid] = (int)H.size();
+      H.push_back(d);
+    }
+  RETCHK(install(c, (int)H.size(), H.data(), minlevel, 0, nullptr));
+  for (int lvl = minlevel; lvl <= 0; lvl++) old.restore_shapes(lvl);
+  RETCHK(carry_root_levels(c, old));
+  std::vector<int32_t> flags;
+  std::vector<bsam_patch_desc> nw(64);
+  for (int lvl = 0;; lvl++) {
+    RETCHK(bsam_set_ghost(c, lvl, 0));
+    if (lvl >= gp->maxlevel) break;
+    flags.resize(std::max<long>(1, c->L(lvl).ncell));
+    int64_t ntag = 0;
+    RETCHK(bsam_estimate_level_errors(c, lvl, gp->errflagopt[lvl], gp->qtolerance[lvl], flags.data(), &ntag));
+    int32_t nn = 0;
+    int32_t r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), gp->ibuffer[lvl],
+                               gp->minimumgridpoints[lvl], gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
+    if (r == BSAM_EINVAL && nn > (int)nw.size()) {
+      nw.resize(nn);
+      r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), gp->ibuffer[lvl],
+                         gp->minimumgridpoints[lvl], gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
+    }
+    RETCHK(r);
+    if (nn == 0) break;
+    H.insert(H.end(), nw.begin(), nw.begin() + nn);
+    // the levels so far are the table's first nkeep rows: their fields are
+    // the whole of the current pools, the start of the next ones
+    const long span = c->nq;
+    struct Prev {
+      bsam_ctx* c;
+      double *q, *qold;
+      ~Prev() {
+        hipStreamSynchronize(c->s);
+        hipFree(q);
+        hipFree(qold);
+      }
+    } prev{c, c->q, c->qold};
+    c->q = c->qold = nullptr;
+    int kept = minlevel - 1;
+    RETCHK(install(c, (int)H.size(), H.data(), minlevel, lvl + 1, &kept));
+    if (kept != lvl) return set_err(BSAM_EINVAL, "AMR: levels up to %d were not kept (%d)", lvl, kept);
+    HIPCHK(hipMemcpyAsync(c->q, prev.q, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->qold, prev.qold, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    old.restore_shapes(lvl + 1);
+    RETCHK(init_level(c, lvl + 1, old));
+  }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_get_hierarchy(bsam_ctx* c, int32_t maxout, bsam_patch_desc* out, int32_t* npatch,
+                                      int32_t* minlevel, int32_t* finestlevel) {
+  if (!c || !npatch) return set_err(BSAM_EINVAL, "bad arguments");
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
+  *npatch = c->npatch;
+  if (minlevel) *minlevel = c->minlevel;
+  if (finestlevel) *finestlevel = c->finest;
+  if (!out) return BSAM_OK;
+  if (maxout < c->npatch) return set_err(BSAM_EINVAL, "%d descriptors do not fit in %d", c->npatch, maxout);
+  std::copy(c->desc.begin(), c->desc.end(), out);
+  return BSAM_OK;
+}
+
+// One pass of the reference's time loop (bsamroutines.f90:672-758): AMR,
+// FillDown on the first step (firstamr), CopyQToQold on every level,
+// MultigridIterations.
+extern "C" int32_t bsam_time_step_amr(bsam_ctx* c, const bsam_grid_params* gp, int32_t first_step,
+                                      int32_t* ncycles, double* residual) {
+  RETCHK(bsam_amr(c, gp));
+  if (first_step) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
+  for (int level = c->minlevel; level <= c->finest; level++) RETCHK(bsam_copy_q_to_qold(c, level));
+  return bsam_multigrid_iterations(c, ncycles, residual, nullptr, 0);
 }
 
 extern "C" int32_t bsam_synchronize(bsam_ctx* c) {

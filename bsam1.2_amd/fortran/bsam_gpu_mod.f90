@@ -44,6 +44,17 @@ TYPE, BIND(C), PUBLIC:: bsam_patch_desc
   REAL(C_DOUBLE):: dx
   REAL(C_DOUBLE):: xlower(3)
 END TYPE bsam_patch_desc
+
+! bsam_grid_params: the griddata of AMR per level 0..maxlevel-1
+INTEGER, PARAMETER:: BSAM_MAX_AMR_LEVELS = 16
+TYPE, BIND(C):: bsam_grid_params
+  INTEGER(C_INT32_T):: maxlevel
+  INTEGER(C_INT32_T):: errflagopt(BSAM_MAX_AMR_LEVELS)
+  INTEGER(C_INT32_T):: ibuffer(BSAM_MAX_AMR_LEVELS)
+  INTEGER(C_INT32_T):: minimumgridpoints(BSAM_MAX_AMR_LEVELS)
+  REAL(C_DOUBLE):: qtolerance(BSAM_MAX_AMR_LEVELS)
+  REAL(C_DOUBLE):: desiredfillratios(BSAM_MAX_AMR_LEVELS)
+END TYPE bsam_grid_params
 !
 TYPE, BIND(C), PUBLIC:: bsam_params
   INTEGER(C_INT32_T):: ndims = 2
@@ -70,7 +81,8 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_fill_down, bsam_get_source, bsam_update_source, bsam_update_aux, &
          bsam_residual, bsam_error_afas, bsam_copy_q_to_qold, bsam_vcycle, &
          bsam_estimate_level_errors, bsam_estimate_level_errors_noalloc, bsam_write_frame, &
-         bsam_multigrid_iterations, bsam_time_step, bsam_regrid, bsam_new_level, bsam_synchronize, &
+         bsam_multigrid_iterations, bsam_time_step, bsam_regrid, bsam_new_level, bsam_amr, &
+         bsam_get_hierarchy, bsam_time_step_amr, bsam_grid_params, BSAM_MAX_AMR_LEVELS, bsam_synchronize, &
          bsam_autotune, bsam_set_knob, bsam_get_knob, bsam_set_mixed_fallback, &
          bsam_mixed_fallbacks, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
@@ -257,6 +269,28 @@ INTERFACE
     REAL(C_DOUBLE), VALUE:: desiredfillratio
     TYPE(bsam_patch_desc), INTENT(OUT):: out(*)
     INTEGER(C_INT32_T), INTENT(OUT):: nout
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_amr(ctx, gp) BIND(C, NAME='bsam_amr')
+    IMPORT:: C_INT32_T, C_PTR, bsam_grid_params
+    TYPE(C_PTR), VALUE:: ctx
+    TYPE(bsam_grid_params), INTENT(IN):: gp
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_get_hierarchy(ctx, maxout, out, npatch, minlevel, finestlevel) &
+      BIND(C, NAME='bsam_get_hierarchy')
+    IMPORT:: C_INT32_T, C_PTR, bsam_patch_desc
+    TYPE(C_PTR), VALUE:: ctx
+    INTEGER(C_INT32_T), VALUE:: maxout
+    TYPE(bsam_patch_desc), INTENT(OUT):: out(*)
+    INTEGER(C_INT32_T), INTENT(OUT):: npatch, minlevel, finestlevel
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_time_step_amr(ctx, gp, first_step, ncycles, residual) &
+      BIND(C, NAME='bsam_time_step_amr')
+    IMPORT:: C_INT32_T, C_PTR, C_DOUBLE, bsam_grid_params
+    TYPE(C_PTR), VALUE:: ctx
+    TYPE(bsam_grid_params), INTENT(IN):: gp
+    INTEGER(C_INT32_T), VALUE:: first_step
+    INTEGER(C_INT32_T), INTENT(OUT):: ncycles
+    REAL(C_DOUBLE), INTENT(OUT):: residual
   END FUNCTION
   INTEGER(C_INT32_T) FUNCTION bsam_synchronize(ctx) BIND(C, NAME='bsam_synchronize')
     IMPORT:: C_INT32_T, C_PTR

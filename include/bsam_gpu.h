@@ -284,6 +284,40 @@ int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patch_desc* des
                        double desiredfillratio, int32_t maxout, bsam_patch_desc* out,
                        int32_t* nout);
 
+/* Grid parameters of AMR, per level 0..maxlevel-1: the rundata NAMELIST
+ * griddata (bsamroutines.f90:243-258) as the regrid reads them. */
+#define BSAM_MAX_AMR_LEVELS 16
+typedef struct bsam_grid_params {
+  int32_t maxlevel;                                /* finest level AMR may make */
+  int32_t errflagopt[BSAM_MAX_AMR_LEVELS];         /* 1, 10 or 11 (bsam_estimate_level_errors) */
+  int32_t ibuffer[BSAM_MAX_AMR_LEVELS];            /* tag buffer width, >= 0 */
+  int32_t minimumgridpoints[BSAM_MAX_AMR_LEVELS];  /* even, >= 4 */
+  double  qtolerance[BSAM_MAX_AMR_LEVELS];
+  double  desiredfillratios[BSAM_MAX_AMR_LEVELS];
+} bsam_grid_params;
+/* AMR(rootlevel) (bsamroutines.f90:966-991), the regrid that opens every
+ * time step of the reference's loop (:672-688), on the device: level by
+ * level from the root, SetGhost(level, 0); below maxlevel the tagging of
+ * bsam_estimate_level_errors (with its effect on q and qold for errflagopt
+ * 1), the new patches of level+1 (bsam_new_level), and their fields
+ * (InitFields from the parents, TransferValues from the mesh this call
+ * replaces); the pass ends at the first level that makes no new patches.
+ * The levels at and below the root are kept.  The new mesh replaces the
+ * installed one (bsam_get_hierarchy reads it); levels that a pass leaves
+ * as they were keep their device tables and launch shapes.  On failure the
+ * context may hold no hierarchy. */
+int32_t bsam_amr(bsam_ctx* ctx, const bsam_grid_params* gp);
+/* The installed hierarchy: *npatch (and *minlevel, *finestlevel when not
+ * NULL); with `out` non-NULL, the descriptors in installed order
+ * (BSAM_EINVAL when they do not fit in maxout). */
+int32_t bsam_get_hierarchy(bsam_ctx* ctx, int32_t maxout, bsam_patch_desc* out, int32_t* npatch,
+                           int32_t* minlevel, int32_t* finestlevel);
+/* One pass of the reference's time loop (bsamroutines.f90:672-758):
+ * bsam_amr, FillDown on the first step (firstamr), CopyQToQold on every
+ * level, MultigridIterations; returns as bsam_multigrid_iterations. */
+int32_t bsam_time_step_amr(bsam_ctx* ctx, const bsam_grid_params* gp, int32_t first_step,
+                           int32_t* ncycles, double* residual);
+
 /* Wait for all work queued on the context's stream. */
 int32_t bsam_synchronize(bsam_ctx* ctx);
 /* Message of the last failed call on this thread ("" if none). */
