@@ -106,6 +106,10 @@ def parse():
                     help="measure the paths' max-norm error against FP64 converged to 1e-12")
     ap.add_argument("--kstat-steps", type=int, default=1,
                     help="extra instrumented steps for the kernel roofline (0: skip)")
+    ap.add_argument("--regrid-steps", type=int, default=5,
+                    help="timed time steps with the reference's regrid (AMR every step) per path (0: skip)")
+    ap.add_argument("--regrid-tables", type=int, default=1,
+                    help="levels rebuilt by the regrid get flattened ghost tables (1) or the staged fill (0)")
     return ap.parse_args()
 
 
@@ -175,6 +179,60 @@ def timed_path(wl, a, precision):
                     traffic=measured_traffic(wl.name, precision, kname),
                     avg_us=avg_ms * 1e3, launches=nl, bytes_per_launch=b)
     return S, elapsed, sum(cycles) / a.steps, roof, tune, bad
+
+
+# The regrid parameters the AMR workloads' meshes were made with
+# (tests/golden/make_amr_workloads.py): the probe module's gradient tagging
+# at 0.05, one buffer cell, 8 minimum grid points, fill ratio 0.8.
+REGRID = dict(errflagopt=11, qtolerance=0.05, ibuffer=1, minimumgridpoints=8, desiredfillratio=0.8)
+
+
+def regrid_path(wl, a, precision):
+    """The reference's time loop with its regrid (bsamroutines.f90:672-758):
+    every step is AMR(rootlevel) on the device (tagging, clustering, new
+    levels installed, InitFields, TransferValues), CopyQToQold,
+    MultigridIterations (bsam_time_step_amr).  One untimed first step
+    (with its FillDown), then a.regrid_steps timed steps; then a few AMR
+    passes alone.  Returns a dict for the bench line."""
+    from bsam_amd import abi
+    try:
+        grid = abi.grid_params(wl.finest, **REGRID)
+        S = make_state(wl, precision, a.graph, a.batching)
+        if a.autotune:
+            S.autotune()
+        S.set_knob(abi.KNOB_REGRID_TABLES, 0, a.regrid_tables)
+        print(f"[bench] {precision} with regrid: first step", file=sys.stderr, flush=True)
+        S.time_step_amr(grid, True)
+        S.synchronize()
+        cycles, bad = [], 0
+        t0 = time.perf_counter()
+        for _ in range(a.regrid_steps):
+            rc, n, r = S.time_step_amr(grid, False)
+            cycles.append(n)
+            bad += rc != abi.BSAM_OK
+        S.synchronize()
+        el = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            S.amr(grid)
+        S.synchronize()
+        amr_s = (time.perf_counter() - t1) / reps
+        per = {}
+        for d in S.hier:
+            if d["level"] >= 0:
+                per[str(d["level"])] = per.get(str(d["level"]), 0) + 1
+        out = dict(value=a.regrid_steps / el, unit="time-steps/s", steps=a.regrid_steps,
+                   ms_per_step=1e3 * el / a.regrid_steps, amr_ms=1e3 * amr_s,
+                   vcycles_per_step=sum(cycles) / len(cycles), nonconverged_steps=bad,
+                   patches_per_level=per, grid=dict(REGRID, maxlevel=wl.finest),
+                   ghost_tables="flattened" if a.regrid_tables else "staged")
+        print(f"[bench] {precision} with regrid: {out['ms_per_step']:.1f} ms/step, AMR pass {out['amr_ms']:.1f} ms, "
+              f"mesh {per}", file=sys.stderr, flush=True)
+        S.close()
+        return out
+    except Exception as e:  # a failed measurement must not cost the bench line
+        return dict(error=str(e))
 
 
 def tune_choices(tune):
@@ -300,6 +358,9 @@ def main():
                        bad=bad)
         S.close()
     head = paths[0]
+    regrid = None
+    if a.regrid_steps > 0 and wl.finest > 0:
+        regrid = {pr: regrid_path(wl, a, pr) for pr in paths}
     check = None
     if a.check:
         # FP64 converged to 1e-12 on the same number of steps: the error
@@ -354,6 +415,7 @@ def main():
         "roofline": res[head]["roof"],
         "cpu_baseline": cpu,
         "regrid_tagging": tag,
+        "with_regrid": regrid,
     }
     if len(paths) == 2:
         f = res["fp64"]

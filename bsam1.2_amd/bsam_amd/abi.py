@@ -41,6 +41,7 @@ KNOB_SWEEP_PLANES_FP64 = 3
 KNOB_SWEEP_PLANES_FP32 = 4
 KNOB_RESID_PLANES = 5
 KNOB_CHAIN_CELLS = 6
+KNOB_REGRID_TABLES = 7
 
 
 class PatchDesc(C.Structure):

@@ -71,9 +71,24 @@ class Solver:
         return [hierarchy.from_ctypes(out[i]) for i in range(n.value)]
 
     def _took_mesh(self):
-        self.hier = self.get_hierarchy()
-        self._descs = hierarchy.to_ctypes(self.hier)
-        self.minlevel, self.finestlevel = hierarchy.levels(self.hier)
+        # the new mesh is read from the library when it is first needed
+        self._stale = True
+        n, lo, hi = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.bsam_get_hierarchy(self.ctx, 0, None, C.byref(n), C.byref(lo), C.byref(hi)))
+        self.minlevel, self.finestlevel = lo.value, hi.value
+
+    @property
+    def hier(self):
+        if getattr(self, "_stale", False):
+            self._stale = False
+            self._hier = self.get_hierarchy()
+            self._descs = hierarchy.to_ctypes(self._hier)
+        return self._hier
+
+    @hier.setter
+    def hier(self, h):
+        self._stale = False
+        self._hier = h
 
     def amr(self, grid):
         """AMR(rootlevel) with abi.GridParams `grid` (bsam_amr); the new mesh

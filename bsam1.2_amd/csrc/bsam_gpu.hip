@@ -3115,6 +3115,7 @@ struct bsam_ctx {
   bool capturing = false;
   std::string tune_report;  // JSON of the last bsam_autotune
   long chain_cells = 0;     // largest level (cells) where the coarse chain starts
+  int regrid_tables = 1;  // flattened ghost tables for the levels a regrid builds
   // mixed precision: an outer iteration whose residual exceeds mixed_stall
   // times the previous one ends the step in FP64 V-cycles
   double mixed_stall = 0.9;
@@ -3622,17 +3623,20 @@ static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       for (int sg = 0; sg < 3; sg++) per[sg][a].swap(seg[sg]);
     }
   });
-  size_t total = 0;
-  for (int sg = 0; sg < 3; sg++)
-    for (size_t a = 0; a < np; a++) total += per[sg][a].size();
-  all.reserve(all.size() + total);
+  // one table, segment by segment and patch by patch, copied in parallel
+  size_t total = all.size();
   for (int sg = 0; sg < 3; sg++) {
     for (size_t a = 0; a < np; a++) {
-      G.roff[sg][a] = (int)all.size();
-      all.insert(all.end(), per[sg][a].begin(), per[sg][a].end());
+      G.roff[sg][a] = (int)total;
+      total += per[sg][a].size();
     }
-    G.roff[sg][np] = (int)all.size();
+    G.roff[sg][np] = (int)total;
   }
+  all.resize(total);
+  run_threads(nth, [&](int t) {
+    for (size_t a = t; a < np; a += nth)
+      for (int sg = 0; sg < 3; sg++) std::copy(per[sg][a].begin(), per[sg][a].end(), all.begin() + G.roff[sg][a]);
+  });
 }
 
 // For each patch a of a level, the patches b > a whose ghost-extended box
@@ -3880,80 +3884,167 @@ struct FlatCall {
   std::vector<FExpr> ex;
   std::vector<int> dst;    // pool offset of the cell of ex[n]
   const FExpr* find(long cell) const { return at[cell] < 0 ? nullptr : &ex[at[cell]]; }
-  void set(long cell, int d, const FExpr& e) {
-    if (at[cell] < 0) {
-      at[cell] = (int)ex.size();
-      ex.push_back(e);
-      dst.push_back(d);
-    } else {
-      ex[at[cell]] = e;
-    }
-  }
 };
 
+// Contiguous ranges of a level's patches with about equal dense cell
+// counts: chunk t is patches [pb[t], pb[t+1]) and dense cells
+// [LC.base[pb[t]], LC.base[pb[t+1]]).
+static std::vector<int> patch_chunks(const LevelCells& LC, int np, int nch) {
+  std::vector<int> pb(nch + 1, np);
+  pb[0] = 0;
+  int a = 0;
+  for (int t = 1; t < nch; t++) {
+    const long goal = LC.n * t / nch;
+    while (a < np && LC.base[a] < goal) a++;
+    pb[t] = a;
+  }
+  return pb;
+}
+
+// The symbolic run of SetGhost(level, ipass), on `nch` host threads that own
+// contiguous ranges of the level's patches.  A stage's boxes read only
+// cells that earlier stages (or no stage) wrote, and each cell has at most
+// one writer per stage, so the stages run one after another with the
+// threads writing the expressions of their own patches' cells in parallel.
+// A chunk's expressions go to its own array, reserved to its ghost-cell
+// count, so that other chunks read finished entries while it appends.
 static bool flatten_setghost(const bsam_ctx* c, const Level& L, const GhostPlan& G, const LevelCells& LC,
-                             int ipass, FlatCall& E, std::string& why) {
+                             int ipass, FlatCall& E, std::string& why, int nch) {
   const int nd = c->p.ndims;
-  E.at.assign(LC.n, -1);
-  E.ex.clear();
-  E.dst.clear();
-  const int s0 = (ipass == 2) ? 1 : 0, s1 = (ipass == 1) ? 1 : 2;
   const int np = (int)L.patches.size();
-  for (int sg = s0; sg <= s1; sg++)
-    for (int a = 0; a < np; a++)
-      for (int r = G.roff[sg][a]; r < G.roff[sg][a + 1]; r++) {
-        const FExpr e{1, r, 0, G.rules[r].pvs, -1, L.patches[a], 0, 0, 0};
-        E.set(LC.at(c, L.patches[a], G.rules[r].dst), G.rules[r].dst, e);
-      }
-  for (const auto& st : G.stages)
-    for (const BoxDesc& B : st) {
-      const DPatch& D = c->hp[B.dp];
-      const DPatch& S = c->hp[B.sp];
-      const int code = (B.op == kCopy) ? 0
-                       : (B.op == kCopyPlus0 ? 1 : (B.op == kDirichlet2 ? 2 : (B.op == kBgMixed ? 3 : 4)));
-      for (int z = 0; z < B.ext[2]; z++)
-        for (int y = 0; y < B.ext[1]; y++)
-          for (int x = 0; x < B.ext[0]; x++) {
-            const int si = B.slo[0] + x, sj = B.slo[1] + y, sk = B.slo[2] + z;
-            const int s = qidx(S, si, sj, sk);
-            const int d = qidx(D, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z);
-            if (!n_outside(D, nd, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z)) {
-              why = "a ghost-fill stage writes an interior cell (overlapping patches)";
-              return false;
-            }
-            FExpr e;
-            const FExpr* prev = E.find(LC.at(c, B.sp, s));
-            if (prev) {
-              e = *prev;
-              if (e.kind == 1 && e.dpid != B.dp) {
-                why = "an interpolated ghost value is copied into another patch";
-                return false;
-              }
-            } else {
-              // a ghost not written so far in this call is read as it was
-              // before the call (physical conditions over ranges that
-              // include the ghost rows, boundary.f90:642-802); an edge or
-              // corner ghost read so is only tolerated in values that a
-              // later stage overwrites (the x faces of a 3D box read the
-              // y-z edges, whose images the z faces rewrite)
-              const int out = n_outside(S, nd, si, sj, sk);
-              e = FExpr{out > 1 ? 2 : 0, s, S.qvs, 0, B.sp, 0, 0, 0, out};
-            }
-            e.dvs = D.qvs;
-            e.dpid = B.dp;
-            if (code) {
-              if (e.npost >= 5) {
-                why = "post-operation chain longer than five";
-                return false;
-              }
-              e.post |= code << (3 * e.npost);
-              e.npost++;
-            }
-            E.set(LC.at(c, B.dp, d), d, e);
-          }
+  nch = std::max(1, std::min(nch, np));
+  const std::vector<int> pb = patch_chunks(LC, np, nch);
+  std::vector<int> owner(np);
+  for (int t = 0; t < nch; t++)
+    for (int a = pb[t]; a < pb[t + 1]; a++) owner[a] = t;
+  constexpr int kShift = 27;
+  constexpr int kMask = (1 << kShift) - 1;
+  E.at.assign(LC.n, -1);
+  std::vector<std::vector<FExpr>> ex(nch);
+  std::vector<std::vector<int>> dst(nch);
+  for (int t = 0; t < nch; t++) {
+    long ghosts = 0;
+    for (int a = pb[t]; a < pb[t + 1]; a++) {
+      const DPatch& P = c->hp[L.patches[a]];
+      ghosts += (long)P.nq[0] * P.nq[1] * P.nq[2] - (long)P.mx[0] * P.mx[1] * P.mx[2];
     }
-  for (const FExpr& e : E.ex)
-    if (e.kind == 2) {
+    if (ghosts > kMask || nch > 15) {
+      why = "a level too large for the chunked symbolic fill";
+      return false;
+    }
+    ex[t].reserve(ghosts);
+    dst[t].reserve(ghosts);
+  }
+  auto find = [&](long cell) -> const FExpr* {
+    const int v = E.at[cell];
+    return v < 0 ? nullptr : &ex[v >> kShift][v & kMask];
+  };
+  std::vector<std::string> err(nch);
+  // a chunk appends only for cells of its own patches
+  auto set = [&](int t, long cell, int d, const FExpr& e) -> bool {
+    const int v = E.at[cell];
+    if (v >= 0) {
+      ex[t][v & kMask] = e;
+      return true;
+    }
+    if (ex[t].size() >= ex[t].capacity()) {
+      err[t] = "more ghost expressions than ghost cells";
+      return false;
+    }
+    E.at[cell] = (t << kShift) | (int)ex[t].size();
+    ex[t].push_back(e);
+    dst[t].push_back(d);
+    return true;
+  };
+  const int s0 = (ipass == 2) ? 1 : 0, s1 = (ipass == 1) ? 1 : 2;
+  run_threads(nch, [&](int t) {
+    for (int sg = s0; sg <= s1; sg++)
+      for (int a = pb[t]; a < pb[t + 1]; a++)
+        for (int r = G.roff[sg][a]; r < G.roff[sg][a + 1]; r++) {
+          const FExpr e{1, r, 0, G.rules[r].pvs, -1, L.patches[a], 0, 0, 0};
+          if (!set(t, LC.at(c, L.patches[a], G.rules[r].dst), G.rules[r].dst, e)) return;
+        }
+  });
+  for (const std::string& m : err)
+    if (!m.empty()) {
+      why = m;
+      return false;
+    }
+  for (const auto& st : G.stages) {
+    std::vector<std::vector<int>> mine(nch);
+    for (size_t n = 0; n < st.size(); n++) mine[owner[LC.pos[st[n].dp]]].push_back((int)n);
+    run_threads(nch, [&](int t) {
+      for (int n : mine[t]) {
+        const BoxDesc& B = st[n];
+        const DPatch& D = c->hp[B.dp];
+        const DPatch& S = c->hp[B.sp];
+        const int code = (B.op == kCopy) ? 0
+                         : (B.op == kCopyPlus0 ? 1 : (B.op == kDirichlet2 ? 2 : (B.op == kBgMixed ? 3 : 4)));
+        for (int z = 0; z < B.ext[2]; z++)
+          for (int y = 0; y < B.ext[1]; y++)
+            for (int x = 0; x < B.ext[0]; x++) {
+              const int si = B.slo[0] + x, sj = B.slo[1] + y, sk = B.slo[2] + z;
+              const int s = qidx(S, si, sj, sk);
+              const int d = qidx(D, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z);
+              if (!n_outside(D, nd, B.dlo[0] + x, B.dlo[1] + y, B.dlo[2] + z)) {
+                err[t] = "a ghost-fill stage writes an interior cell (overlapping patches)";
+                return;
+              }
+              FExpr e;
+              const FExpr* prev = find(LC.at(c, B.sp, s));
+              if (prev) {
+                e = *prev;
+                if (e.kind == 1 && e.dpid != B.dp) {
+                  err[t] = "an interpolated ghost value is copied into another patch";
+                  return;
+                }
+              } else {
+                // a ghost not written so far in this call is read as it was
+                // before the call (physical conditions over ranges that
+                // include the ghost rows, boundary.f90:642-802); an edge or
+                // corner ghost read so is only tolerated in values that a
+                // later stage overwrites (the x faces of a 3D box read the
+                // y-z edges, whose images the z faces rewrite)
+                const int out = n_outside(S, nd, si, sj, sk);
+                e = FExpr{out > 1 ? 2 : 0, s, S.qvs, 0, B.sp, 0, 0, 0, out};
+              }
+              e.dvs = D.qvs;
+              e.dpid = B.dp;
+              if (code) {
+                if (e.npost >= 5) {
+                  err[t] = "post-operation chain longer than five";
+                  return;
+                }
+                e.post |= code << (3 * e.npost);
+                e.npost++;
+              }
+              if (!set(t, LC.at(c, B.dp, d), d, e)) return;
+            }
+      }
+    });
+    for (const std::string& m : err)
+      if (!m.empty()) {
+        why = m;
+        return false;
+      }
+  }
+  // one expression array, chunk after chunk; each chunk renumbers its cells
+  std::vector<int> base(nch + 1, 0);
+  for (int t = 0; t < nch; t++) base[t + 1] = base[t] + (int)ex[t].size();
+  E.ex.resize(base[nch]);
+  E.dst.resize(base[nch]);
+  std::vector<char> bad(nch, 0);
+  run_threads(nch, [&](int t) {
+    std::copy(ex[t].begin(), ex[t].end(), E.ex.begin() + base[t]);
+    std::copy(dst[t].begin(), dst[t].end(), E.dst.begin() + base[t]);
+    for (const FExpr& e : ex[t])
+      if (e.kind == 2) bad[t] = 1;
+    const long c0 = LC.base[pb[t]], c1 = pb[t + 1] < np ? LC.base[pb[t + 1]] : LC.n;
+    for (long cell = c0; cell < c1; cell++)
+      if (E.at[cell] >= 0) E.at[cell] = base[t] + (E.at[cell] & kMask);
+  });
+  for (int t = 0; t < nch; t++)
+    if (bad[t]) {
       why = "a ghost value is read from an edge or corner ghost the call has not written";
       return false;
     }
@@ -4042,10 +4133,12 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const int nd = c->p.ndims;
   const LevelCells LC(c, L);
   FlatCall E[3];
-  // the three calls are independent: flatten them on three host threads
   bool ok[3] = {false, false, false};
   std::string why[3];
-  run_threads(3, [&](int p) { ok[p] = flatten_setghost(c, L, G, LC, p, E[p], why[p]); });
+  // the three calls are independent, and each runs on its share of the
+  // host threads (levels of a few patches on one thread each)
+  const int share = LC.n < (1L << 20) ? 1 : std::max<int>(1, (int)host_threads() / 3);
+  run_threads(3, [&](int p) { ok[p] = flatten_setghost(c, L, G, LC, p, E[p], why[p], share); });
   for (int p = 0; p < 3; p++) {
     if (!ok[p]) {
       G.why = why[p];
@@ -4063,168 +4156,213 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       }
     }
   }
-  // rules in destination order
-  auto emit = [&](const FlatCall& M, const std::vector<int>& which, HostMap& H) {
-    H.copies.clear();
-    H.cpost.clear();
-    H.interps.clear();
-    H.nc1 = H.ni1 = 0;
+  // Each table holds its rules in destination order (a level's patches are
+  // in ascending pool offset, so the dense cell index ascends with the
+  // destination offset), those that read ghosts from before the call
+  // (phase 1) ahead of the others.  The dense range is cut into chunks of
+  // whole patches that host threads emit in parallel; the parts are joined
+  // in chunk order, phase 1 first.
+  struct Part {
+    std::vector<int4> cp[2];
+    std::vector<uint16_t> post[2];
+    std::vector<GRule> in[2];
     bool anypost = false;
-    for (int phase = 1; phase >= 0; phase--)
-      for (int n : which) {
-        const FExpr& e = M.ex[n];
-        if ((e.early != 0) != (phase == 1)) continue;
-        const int d = M.dst[n];
-        if (e.kind == 0) {
-          H.copies.push_back(make_int4(d, e.src, e.dvs, e.svs));
-          H.cpost.push_back((uint16_t)e.post);
-          anypost = anypost || e.post;
-          if (phase) H.nc1++;
-        } else {
-          GRule r = G.rules[e.src];
-          r.dst = d;
-          r.kind = (r.kind & 0xff) | (e.post << 8);
-          H.interps.push_back(r);
-          if (phase) H.ni1++;
-        }
-      }
-    if (!anypost) H.cpost.clear();
   };
-  // the three ipasses' tables are independent: one host thread each
-  auto plan_p = [&](int p, std::string& why) -> bool {
-    // destination order: a level's patches are in ascending pool offset, so
-    // the dense cell index ascends with the destination offset
-    std::vector<int> order;
-    order.reserve(E[p].ex.size());
-    for (long cell = 0; cell < LC.n; cell++)
-      if (E[p].at[cell] >= 0) order.push_back(E[p].at[cell]);
-    emit(E[p], order, G.map[p]);
-    if (p == 0) {
-      // map[5]: the colour-1 face ghosts of SetGhost(level, 0)
-      std::vector<int> sel;
-      for (int n : order) {
-        const FExpr& e = E[0].ex[n];
-        const DPatch& D = c->hp[e.dpid];
-        int i, j, k;
-        qdecode(D, E[0].dst[n], i, j, k);
-        if (n_outside(D, nd, i, j, k) == 1 && ((i + j + k) & 1) == 1) {
-          if (e.early) {
-            why = "a colour-1 face ghost of SetGhost(level, 0) reads a ghost from before the call";
-            return false;
-          }
-          sel.push_back(n);
-        }
-      }
-      emit(E[0], sel, G.map[5]);
-      // map[6]: map[5] less the copies that CorrectFine leaves exact.  A
-      // ghost whose value is a plain copy (no post-operation) of an interior
-      // cell at the same global position (modulo the period) held that
-      // cell's value before the correction (the last SetGhost of the
-      // pre-smoothing wrote it), and both receive the prolongation of the
-      // same coarse correction, by the same expression, provided every
-      // coarse cell of the prolongation stencil holds one value for both
-      // patches: a cell inside a patch of the parent level (its ghost
-      // images in other parents are copies of it, and the coarse-under-fine
-      // values are restrictions of the same fine cells).  A coarse cell
-      // outside every parent-level patch is a coarse-fine ghost that each
-      // parent interpolates from its own cells; such ghosts, and
-      // reflecting copies (mirrored position), stay in the table.
-      const std::vector<uint8_t> occ = parent_interior(c, L.level - 1);
-      int cper[3] = {1, 1, 1}, cwrap[3] = {0, 0, 0};
-      const bsam_patch_desc* root = nullptr;
-      for (const bsam_patch_desc& d : c->desc)
-        if (d.level == 0) root = &d;
-      for (int a = 0; a < nd; a++) {
-        cper[a] = level_period(c, L.level - 1, a);
-        cwrap[a] = root->mthbc[2 * a] == 2;
-      }
-      auto covered = [&](const int* g) {
-        long idx = 0;
-        for (int a = nd - 1; a >= 0; a--) {
-          int x = g[a] - 1;
-          if (x < 0 || x >= cper[a]) {
-            if (!cwrap[a]) return false;
-            x = ((x % cper[a]) + cper[a]) % cper[a];
-          }
-          idx = idx * cper[a] + x;
-        }
-        return occ[idx] != 0;
-      };
-      std::vector<int> sel6;
-      for (int n : sel) {
-        const FExpr& e = E[0].ex[n];
-        bool exact = e.kind == 0 && e.post == 0 && !e.early;
-        if (exact) {
-          const DPatch& S = c->hp[e.spid];
-          int s3[3], d3[3];
-          qdecode(S, e.src, s3[0], s3[1], s3[2]);
-          qdecode(c->hp[e.dpid], E[0].dst[n], d3[0], d3[1], d3[2]);
-          exact = n_outside(S, nd, s3[0], s3[1], s3[2]) == 0;
-          for (int a = 0; a < nd && exact; a++) {
-            const int per = level_period(c, L.level, a);
-            const int ds = (c->desc[e.spid].mglobal[a][0] + s3[a]) - (c->desc[e.dpid].mglobal[a][0] + d3[a]);
-            exact = ((ds % per) + per) % per == 0;
-          }
-          // the coarse cells of the prolongation of the destination: along
-          // each dimension an odd fine index g takes coarse (g+1)/2 - 1 and
-          // (g+1)/2, an even one g/2 and g/2 + 1 (BiLinProlongationP1,
-          // TriLinProlongationP1, gridutilities.f90:156-195, 510-566)
-          for (int corner = 0; corner < (1 << nd) && exact; corner++) {
-            int g[3] = {1, 1, 1};
-            for (int a = 0; a < nd; a++) {
-              const int gf = c->desc[e.dpid].mglobal[a][0] + d3[a] - 1;
-              const int lo = (gf & 1) ? (gf + 1) / 2 - 1 : gf / 2;
-              g[a] = lo + ((corner >> a) & 1);
-            }
-            exact = covered(g);
-          }
-        }
-        if (!exact) sel6.push_back(n);
-      }
-      emit(E[0], sel6, G.map[6]);
-      return true;
+  auto emit = [&](const FlatCall& M, int n, Part& P) {
+    const FExpr& e = M.ex[n];
+    const int d = M.dst[n];
+    const int ph = e.early ? 1 : 0;
+    if (e.kind == 0) {
+      P.cp[ph].push_back(make_int4(d, e.src, e.dvs, e.svs));
+      P.post[ph].push_back((uint16_t)e.post);
+      P.anypost = P.anypost || e.post;
+    } else {
+      GRule r = G.rules[e.src];
+      r.dst = d;
+      r.kind = (r.kind & 0xff) | (e.post << 8);
+      P.in[ph].push_back(r);
     }
+  };
+  auto join = [&](std::vector<Part>& parts, HostMap& H) {
+    const int K = (int)parts.size();
+    std::vector<size_t> oc(2 * K + 1, 0), oi(2 * K + 1, 0);
+    bool anypost = false;
+    int slot = 0;
+    for (int ph = 1; ph >= 0; ph--)
+      for (int k = 0; k < K; k++, slot++) {
+        oc[slot + 1] = oc[slot] + parts[k].cp[ph].size();
+        oi[slot + 1] = oi[slot] + parts[k].in[ph].size();
+        anypost = anypost || parts[k].anypost;
+      }
+    H.copies.resize(oc[2 * K]);
+    H.cpost.resize(anypost ? oc[2 * K] : 0);
+    H.interps.resize(oi[2 * K]);
+    H.nc1 = (int)oc[K];
+    H.ni1 = (int)oi[K];
+    run_threads(K, [&](int k) {
+      for (int ph = 1; ph >= 0; ph--) {
+        const int sl = (1 - ph) * K + k;
+        const Part& P = parts[k];
+        std::copy(P.cp[ph].begin(), P.cp[ph].end(), H.copies.begin() + oc[sl]);
+        if (anypost) std::copy(P.post[ph].begin(), P.post[ph].end(), H.cpost.begin() + oc[sl]);
+        std::copy(P.in[ph].begin(), P.in[ph].end(), H.interps.begin() + oi[sl]);
+      }
+    });
+  };
+  // map[6]'s test of the coarse cells a prolongation reads (see below)
+  const std::vector<uint8_t> occ = parent_interior(c, L.level - 1);
+  int cper[3] = {1, 1, 1}, cwrap[3] = {0, 0, 0};
+  {
+    const bsam_patch_desc* root = nullptr;
+    for (const bsam_patch_desc& d : c->desc)
+      if (d.level == 0) root = &d;
+    for (int a = 0; a < nd; a++) {
+      cper[a] = level_period(c, L.level - 1, a);
+      cwrap[a] = root->mthbc[2 * a] == 2;
+    }
+  }
+  auto covered = [&](const int* g) {
+    long idx = 0;
+    for (int a = nd - 1; a >= 0; a--) {
+      int x = g[a] - 1;
+      if (x < 0 || x >= cper[a]) {
+        if (!cwrap[a]) return false;
+        x = ((x % cper[a]) + cper[a]) % cper[a];
+      }
+      idx = idx * cper[a] + x;
+    }
+    return occ[idx] != 0;
+  };
+  // map[5]: the colour-1 face ghosts of SetGhost(level, 0).  map[6]: map[5]
+  // less the copies that CorrectFine leaves exact.  A ghost whose value is
+  // a plain copy (no post-operation) of an interior cell at the same global
+  // position (modulo the period) held that cell's value before the
+  // correction (the last SetGhost of the pre-smoothing wrote it), and both
+  // receive the prolongation of the same coarse correction, by the same
+  // expression, provided every coarse cell of the prolongation stencil
+  // holds one value for both patches: a cell inside a patch of the parent
+  // level (its ghost images in other parents are copies of it, and the
+  // coarse-under-fine values are restrictions of the same fine cells).  A
+  // coarse cell outside every parent-level patch is a coarse-fine ghost
+  // that each parent interpolates from its own cells; such ghosts, and
+  // reflecting copies (mirrored position), stay in the table.
+  // Returns 0 (not in map[5]), 1 (map[5] and map[6]), 2 (map[5] only), -1
+  // (refused).
+  auto sel56 = [&](int n, std::string& why) -> int {
+    const FExpr& e = E[0].ex[n];
+    const DPatch& D = c->hp[e.dpid];
+    int i, j, k;
+    qdecode(D, E[0].dst[n], i, j, k);
+    if (!(n_outside(D, nd, i, j, k) == 1 && ((i + j + k) & 1) == 1)) return 0;
+    if (e.early) {
+      why = "a colour-1 face ghost of SetGhost(level, 0) reads a ghost from before the call";
+      return -1;
+    }
+    bool exact = e.kind == 0 && e.post == 0;
+    if (exact) {
+      const DPatch& S = c->hp[e.spid];
+      int s3[3], d3[3];
+      qdecode(S, e.src, s3[0], s3[1], s3[2]);
+      qdecode(D, E[0].dst[n], d3[0], d3[1], d3[2]);
+      exact = n_outside(S, nd, s3[0], s3[1], s3[2]) == 0;
+      for (int a = 0; a < nd && exact; a++) {
+        const int per = level_period(c, L.level, a);
+        const int ds = (c->desc[e.spid].mglobal[a][0] + s3[a]) - (c->desc[e.dpid].mglobal[a][0] + d3[a]);
+        exact = ((ds % per) + per) % per == 0;
+      }
+      // the coarse cells of the prolongation of the destination: along
+      // each dimension an odd fine index g takes coarse (g+1)/2 - 1 and
+      // (g+1)/2, an even one g/2 and g/2 + 1 (BiLinProlongationP1,
+      // TriLinProlongationP1, gridutilities.f90:156-195, 510-566)
+      for (int corner = 0; corner < (1 << nd) && exact; corner++) {
+        int g[3] = {1, 1, 1};
+        for (int a = 0; a < nd; a++) {
+          const int gf = c->desc[e.dpid].mglobal[a][0] + d3[a] - 1;
+          const int lo = (gf & 1) ? (gf + 1) / 2 - 1 : gf / 2;
+          g[a] = lo + ((corner >> a) & 1);
+        }
+        exact = covered(g);
+      }
+    }
+    return exact ? 2 : 1;
+  };
+  // map[2 + p], p = 1, 2 (described above): 1 in the table, 0 not, -1 refused
+  auto lean = [&](int p, int n, std::string& why) -> int {
     const int col = p - 1;
     const FlatCall& other = E[3 - p];
-    std::vector<int> lean;
-    for (int n : order) {
-      const FExpr& e = E[p].ex[n];
-      const int d = E[p].dst[n];
-      const DPatch& D = c->hp[e.dpid];
-      int i, j, k;
-      qdecode(D, d, i, j, k);
-      const FExpr* o = other.find(LC.at(c, e.dpid, d));
-      if (n_outside(D, nd, i, j, k) != 1) {
-        // an edge or corner ghost: only if the other ipass does not write it
-        if (!o && e.early) {
-          why = "a colour-filtered edge or corner ghost reads a ghost from before the call";
-          return false;
-        }
-        if (!o) lean.push_back(n);
-        continue;
+    const FExpr& e = E[p].ex[n];
+    const int d = E[p].dst[n];
+    const DPatch& D = c->hp[e.dpid];
+    int i, j, k;
+    qdecode(D, d, i, j, k);
+    const FExpr* o = other.find(LC.at(c, e.dpid, d));
+    if (n_outside(D, nd, i, j, k) != 1) {
+      // an edge or corner ghost: only if the other ipass does not write it
+      if (!o && e.early) {
+        why = "a colour-filtered edge or corner ghost reads a ghost from before the call";
+        return -1;
       }
-      bool need = ((i + j + k) & 1) == col;
-      if (!need) {
-        int si, sj, sk;
-        if (e.kind == 0) {
-          qdecode(c->hp[e.spid], e.src, si, sj, sk);
-          need = ((si + sj + sk) & 1) == col;
+      return o ? 0 : 1;
+    }
+    bool need = ((i + j + k) & 1) == col;
+    if (!need) {
+      int si, sj, sk;
+      if (e.kind == 0) {
+        qdecode(c->hp[e.spid], e.src, si, sj, sk);
+        need = ((si + sj + sk) & 1) == col;
+      } else {
+        const GRule& r = G.rules[e.src];
+        qdecode(D, r.s1, si, sj, sk);
+        need = ((si + sj + sk) & 1) == col;
+        qdecode(D, r.s2, si, sj, sk);
+        need = need || ((si + sj + sk) & 1) == col;
+      }
+    }
+    if (!need) need = !o || !same_expr(G, e, *o);
+    if (need && e.early) {
+      why = "a face ghost of a reduced table reads a ghost from before the call";
+      return -1;
+    }
+    return need ? 1 : 0;
+  };
+  // the three ipasses' tables are independent; each runs on its share of
+  // the host threads
+  const int np = (int)L.patches.size();
+  const int K = std::max(1, std::min(share, np));
+  const std::vector<int> pb = patch_chunks(LC, np, K);
+  auto plan_p = [&](int p, std::string& why) -> bool {
+    std::vector<Part> full(K), sub(K), sub6(K);
+    std::vector<std::string> err(K);
+    run_threads(K, [&](int k) {
+      const long c0 = LC.base[pb[k]], c1 = pb[k + 1] < np ? LC.base[pb[k + 1]] : LC.n;
+      for (long cell = c0; cell < c1; cell++) {
+        const int n = E[p].at[cell];
+        if (n < 0) continue;
+        emit(E[p], n, full[k]);
+        if (p == 0) {
+          const int r = sel56(n, err[k]);
+          if (r < 0) return;
+          if (r >= 1) emit(E[0], n, sub[k]);
+          if (r == 1) emit(E[0], n, sub6[k]);
         } else {
-          const GRule& r = G.rules[e.src];
-          qdecode(D, r.s1, si, sj, sk);
-          need = ((si + sj + sk) & 1) == col;
-          qdecode(D, r.s2, si, sj, sk);
-          need = need || ((si + sj + sk) & 1) == col;
+          const int r = lean(p, n, err[k]);
+          if (r < 0) return;
+          if (r) emit(E[p], n, sub[k]);
         }
       }
-      if (!need) need = !o || !same_expr(G, e, *o);
-      if (need && e.early) {
-        why = "a face ghost of a reduced table reads a ghost from before the call";
+    });
+    for (const std::string& m : err)
+      if (!m.empty()) {
+        why = m;
         return false;
       }
-      if (need) lean.push_back(n);
+    join(full, G.map[p]);
+    if (p == 0) {
+      join(sub, G.map[5]);
+      join(sub6, G.map[6]);
+    } else {
+      join(sub, G.map[2 + p]);
     }
-    emit(E[p], lean, G.map[2 + p]);
     return true;
   };
   {
@@ -4407,7 +4545,7 @@ namespace {
 // The device side of one level of a new hierarchy: its patch list, the
 // block-aligned launch ranges of every domain, the tagging offsets and the
 // ghost fill (stages, interpolation rules, flattened tables).
-int32_t build_level(bsam_ctx* c, Level& L) {
+int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
   const int nv = c->p.nrvars;
   HIPCHK(hipMalloc(&L.d_patches, L.patches.size() * sizeof(int)));
   HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), L.patches.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -4479,7 +4617,7 @@ int32_t build_level(bsam_ctx* c, Level& L) {
   GhostPlan G;
   RETCHK(plan_ghost_stages(c, L, G));
   plan_interp_rules(c, L, G);
-  if (c->p.batching) RETCHK(plan_ghost_maps(c, L, G));
+  if (c->p.batching && tables) RETCHK(plan_ghost_maps(c, L, G));
   RETCHK(upload_ghost(c, L, G));
   return BSAM_OK;
 }
@@ -4496,25 +4634,27 @@ bool same_desc(const bsam_patch_desc& a, const bsam_patch_desc& b) {
   return true;
 }
 
-// The highest level up to which a new descriptor table leaves the
-// installed hierarchy as it is (minlevel - 1 if none): the installed
-// table's patches of the levels up to it are its first n rows, the new
-// table starts with the same n rows, and no later row of the new table is
-// on those levels.  Such a level's device side depends on no higher level
-// (its ghost fill reads its parent level, and the transfers between levels
-// belong to the finer one), and its pool offsets are unchanged.
-int kept_levels(const bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel,
-                int32_t finestlevel) {
+// The highest level up to which a new descriptor table equals an older
+// table `tab`, whose levels minlevel.. hold the patches plist[l - minlevel]
+// (minlevel - 1 if none): the older table's patches of the levels up to it
+// are its first n rows, the new table starts with the same n rows, and no
+// later row of the new table is on those levels.  Such a level's device
+// side depends on no higher level (its ghost fill reads its parent level,
+// and the transfers between levels belong to the finer one), and its pool
+// offsets are the same in both tables.
+int kept_levels(const std::vector<std::vector<int>>& plist, int tmin, const std::vector<bsam_patch_desc>& tab,
+                int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel, int32_t finestlevel) {
   int keep = minlevel - 1;
-  if (!c->have_hier || c->minlevel != minlevel) return keep;
+  if (tmin != minlevel || !desc) return keep;
+  const int tfin = tmin + (int)plist.size() - 1;
   int n = 0;
-  for (int l = minlevel; l <= std::min(c->finest, finestlevel); l++) {
-    const Level& L = c->lv[l - c->minlevel];
-    for (size_t a = 0; a < L.patches.size(); a++) {
+  for (int l = minlevel; l <= std::min(tfin, finestlevel); l++) {
+    const std::vector<int>& ps = plist[l - tmin];
+    for (size_t a = 0; a < ps.size(); a++) {
       const int i = n + (int)a;
-      if (L.patches[a] != i || i >= npatch || !same_desc(desc[i], c->desc[i])) return keep;
+      if (ps[a] != i || i >= npatch || !same_desc(desc[i], tab[i])) return keep;
     }
-    n += (int)L.patches.size();
+    n += (int)ps.size();
     for (int i = n; i < npatch; i++)
       if (desc[i].level <= l) return keep;
     keep = l;
@@ -4522,13 +4662,76 @@ int kept_levels(const bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, 
   return keep;
 }
 
+// The mesh a regrid replaces: its q pools (the source of TransferValues),
+// descriptors, each level's patches and launch shapes, and each level's
+// device side, which an install of an equal level takes over.  Taking it
+// releases the rest of the old hierarchy (the context then has none).
+struct OldMesh {
+  bsam_ctx* c;
+  double *q = nullptr, *qold = nullptr;
+  std::vector<DPatch> hp;
+  std::vector<bsam_patch_desc> desc;
+  std::vector<Level> lv;                 // device side, empty once taken
+  std::vector<std::vector<int>> plist;   // patches per level
+  std::vector<std::array<int, 6>> shape;  // launch shapes per level
+  int minlevel = 0, finest = -1;
+  XferBox* d_box = nullptr;
+  size_t box_cap = 0;
+  explicit OldMesh(bsam_ctx* c_) : c(c_) {
+    q = c->q;
+    qold = c->qold;
+    c->q = c->qold = nullptr;
+    hp = c->hp;
+    desc = c->desc;
+    minlevel = c->minlevel;
+    finest = c->finest;
+    for (int l = minlevel; l <= finest; l++) {
+      Level& L = c->L(l);
+      plist.push_back(L.patches);
+      shape.push_back({L.order, L.gorder[0], L.gorder[1], L.kc_sweep[0], L.kc_sweep[1], L.kc_resid});
+      lv.push_back(std::move(L));
+      L = Level();
+    }
+    free_hierarchy(c);
+  }
+  ~OldMesh() {
+    hipStreamSynchronize(c->s);
+    for (Level& L : lv) free_level(L);
+    hipFree(q);
+    hipFree(qold);
+    hipFree(d_box);
+  }
+  const std::vector<int>* patches(int l) const {
+    return (l >= minlevel && l <= finest) ? &plist[l - minlevel] : nullptr;
+  }
+  // the launch shapes set for a level carry over while the level exists
+  void restore_shapes(int l) const {
+    if (l < minlevel || l > finest || l < c->minlevel || l > c->finest) return;
+    const std::array<int, 6>& k = shape[l - minlevel];
+    Level& L = c->L(l);
+    L.order = k[0];
+    L.gorder[0] = k[1], L.gorder[1] = k[2];
+    L.kc_sweep[0] = k[3], L.kc_sweep[1] = k[4];
+    L.kc_resid = k[5];
+  }
+};
+
 // Install a hierarchy with zero fields.  The levels kept_levels() finds
 // unchanged keep their device side (and launch shapes); the others are
 // built.  Returns the highest kept level through *kept.
 int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel, int32_t finestlevel,
-                int* kept) {
+                int* kept, OldMesh* donor = nullptr, bool tables = true) {
   HIPCHK(hipStreamSynchronize(c->s));
-  const int keep = (desc && npatch > 0) ? kept_levels(c, npatch, desc, minlevel, finestlevel) : minlevel - 1;
+  int keep = minlevel - 1;
+  if (c->have_hier && desc && npatch > 0) {
+    std::vector<std::vector<int>> cur;
+    for (const Level& L : c->lv) cur.push_back(L.patches);
+    keep = kept_levels(cur, c->minlevel, c->desc, npatch, desc, minlevel, finestlevel);
+  }
+  // past those, levels equal to the replaced mesh's take its device side
+  const int dkeep = donor && desc && npatch > 0
+                        ? kept_levels(donor->plist, donor->minlevel, donor->desc, npatch, desc, minlevel, finestlevel)
+                        : minlevel - 1;
   if (kept) *kept = keep;
   struct Held {
     std::vector<Level> v;
@@ -4559,8 +4762,11 @@ int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_
     if (l <= keep) {
       L = std::move(held.v[l - minlevel]);
       held.v[l - minlevel] = Level();
+    } else if (l <= dkeep && !donor->lv[l - donor->minlevel].patches.empty()) {
+      L = std::move(donor->lv[l - donor->minlevel]);
+      donor->lv[l - donor->minlevel] = Level();
     } else {
-      RETCHK(build_level(c, L));
+      RETCHK(build_level(c, L, tables));
     }
     // per-patch launches round every patch up to whole blocks
     maxblocks = std::max(maxblocks, nblocks(L.total[kCell]) + (int)L.patches.size());
@@ -5933,63 +6139,13 @@ extern "C" int32_t bsam_time_step(bsam_ctx* c, int32_t first_step, int32_t* ncyc
 // regrid: GridAdapt's field set-up (bsamroutines.f90:1663-1951) and AMR
 // ---------------------------------------------------------------------------
 namespace {
-// The mesh a regrid replaces: its q pools (the source of TransferValues),
-// descriptors, and each level's patches and launch shapes.  Taking it
-// detaches the pools from the context; they are freed with this object.
-struct OldMesh {
-  bsam_ctx* c;
-  double *q = nullptr, *qold = nullptr;
-  std::vector<DPatch> hp;
-  std::vector<bsam_patch_desc> desc;
-  std::vector<Level> lv;  // patches and launch shapes only
-  int minlevel = 0, finest = -1;
-  XferBox* d_box = nullptr;
-  size_t box_cap = 0;
-  explicit OldMesh(bsam_ctx* c_) : c(c_) {
-    q = c->q;
-    qold = c->qold;
-    c->q = c->qold = nullptr;
-    hp = c->hp;
-    desc = c->desc;
-    minlevel = c->minlevel;
-    finest = c->finest;
-    for (int l = minlevel; l <= finest; l++) {
-      const Level& L = c->L(l);
-      Level k;
-      k.patches = L.patches;
-      k.order = L.order;
-      k.gorder[0] = L.gorder[0], k.gorder[1] = L.gorder[1];
-      k.kc_sweep[0] = L.kc_sweep[0], k.kc_sweep[1] = L.kc_sweep[1];
-      k.kc_resid = L.kc_resid;
-      lv.push_back(std::move(k));
-    }
-  }
-  ~OldMesh() {
-    hipStreamSynchronize(c->s);
-    hipFree(q);
-    hipFree(qold);
-    hipFree(d_box);
-  }
-  const Level* level(int l) const { return (l >= minlevel && l <= finest) ? &lv[l - minlevel] : nullptr; }
-  // the launch shapes set for a level carry over while the level exists
-  void restore_shapes(int l) const {
-    const Level* k = level(l);
-    if (!k || l < c->minlevel || l > c->finest) return;
-    Level& L = c->L(l);
-    L.order = k->order;
-    L.gorder[0] = k->gorder[0], L.gorder[1] = k->gorder[1];
-    L.kc_sweep[0] = k->kc_sweep[0], L.kc_sweep[1] = k->kc_sweep[1];
-    L.kc_resid = k->kc_resid;
-  }
-};
-
 // The levels at and below the root keep their one patch each, with the
 // same extents: q and qold come over from the old pools.
 int32_t carry_root_levels(bsam_ctx* c, const OldMesh& o) {
   const int nv = c->p.nrvars;
   for (int lvl = c->minlevel; lvl <= 0; lvl++) {
     const DPatch& P = c->hp[c->L(lvl).patches[0]];
-    const DPatch& O = o.hp[o.level(lvl)->patches[0]];
+    const DPatch& O = o.hp[(*o.patches(lvl))[0]];
     const size_t n = (size_t)P.qvs * nv * sizeof(double);
     HIPCHK(hipMemcpyAsync(c->q + P.qoff, o.q + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
     HIPCHK(hipMemcpyAsync(c->qold + P.qoff, o.qold + O.qoff, n, hipMemcpyDeviceToDevice, c->s));
@@ -6019,12 +6175,12 @@ int32_t init_level(bsam_ctx* c, int lvl, OldMesh& o) {
       LAUNCH_CHECK();
     }
   }
-  const Level* old = o.level(lvl);
+  const std::vector<int>* old = o.patches(lvl);
   if (!old) return BSAM_OK;
   std::vector<XferBox> boxes;
   long ncell = 0;
   for (int pid : L.patches)
-    for (int op : old->patches) {
+    for (int op : *old) {
       const bsam_patch_desc& D = c->desc[pid];
       const bsam_patch_desc& S = o.desc[op];
       XferBox B{};
@@ -6093,7 +6249,7 @@ extern "C" int32_t bsam_regrid(bsam_ctx* c, int32_t npatch, const bsam_patch_des
   }
   HIPCHK(hipStreamSynchronize(c->s));
   OldMesh old(c);
-  RETCHK(install(c, npatch, desc, minlevel, finestlevel, nullptr));
+  RETCHK(install(c, npatch, desc, minlevel, finestlevel, nullptr, &old, c->regrid_tables != 0));
   for (int lvl = minlevel; lvl <= finestlevel; lvl++) old.restore_shapes(lvl);
   RETCHK(carry_root_levels(c, old));
   RETCHK(bsam_set_ghost(c, 0, 0));
@@ -6131,13 +6287,13 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
   std::vector<bsam_patch_desc> H;
   std::vector<int> at(old.desc.size(), -1);
   for (int lvl = minlevel; lvl <= 0; lvl++)
-    for (int pid : old.level(lvl)->patches) {
+    for (int pid : *old.patches(lvl)) {
       bsam_patch_desc d = old.desc[pid];
       if (d.parent >= 0) d.parent = at[d.parent];
       at[pid] = (int)H.size();
       H.push_back(d);
     }
-  RETCHK(install(c, (int)H.size(), H.data(), minlevel, 0, nullptr));
+  RETCHK(install(c, (int)H.size(), H.data(), minlevel, 0, nullptr, &old, c->regrid_tables != 0));
   for (int lvl = minlevel; lvl <= 0; lvl++) old.restore_shapes(lvl);
   RETCHK(carry_root_levels(c, old));
   std::vector<int32_t> flags;
@@ -6173,7 +6329,7 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
     } prev{c, c->q, c->qold};
     c->q = c->qold = nullptr;
     int kept = minlevel - 1;
-    RETCHK(install(c, (int)H.size(), H.data(), minlevel, lvl + 1, &kept));
+    RETCHK(install(c, (int)H.size(), H.data(), minlevel, lvl + 1, &kept, &old, c->regrid_tables != 0));
     if (kept != lvl) return set_err(BSAM_EINVAL, "AMR: levels up to %d were not kept (%d)", lvl, kept);
     HIPCHK(hipMemcpyAsync(c->q, prev.q, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     HIPCHK(hipMemcpyAsync(c->qold, prev.qold, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
@@ -6878,6 +7034,11 @@ extern "C" int32_t bsam_set_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
     drop_graph(c);
     return BSAM_OK;
   }
+  if (knob == BSAM_KNOB_REGRID_TABLES) {
+    if (value != 0 && value != 1) return set_err(BSAM_EINVAL, "regrid tables must be 0 or 1");
+    c->regrid_tables = value;
+    return BSAM_OK;
+  }
   if (knob < 0 || knob > BSAM_KNOB_RESID_PLANES) return set_err(BSAM_EINVAL, "unknown knob %d", knob);
   RETCHK(check_level(c, level));
   Level& L = c->L(level);
@@ -6909,6 +7070,10 @@ extern "C" int32_t bsam_get_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
   if (!c || !value) return set_err(BSAM_EINVAL, "null argument");
   if (knob == BSAM_KNOB_CHAIN_CELLS) {
     *value = (int32_t)c->chain_cells;
+    return BSAM_OK;
+  }
+  if (knob == BSAM_KNOB_REGRID_TABLES) {
+    *value = c->regrid_tables;
     return BSAM_OK;
   }
   if (knob < 0 || knob > BSAM_KNOB_RESID_PLANES) return set_err(BSAM_EINVAL, "unknown knob %d", knob);

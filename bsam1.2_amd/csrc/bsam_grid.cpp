@@ -8,7 +8,10 @@
 // (MakeNewGrid :1663-1735).  No device code: clustering reads one int per
 // cell of the level once per split pass and is a small part of a regrid.
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "bsam_grid.h"
@@ -94,11 +97,18 @@ double fill_ratio(const Tags& T, const Box& b, int nd) {
 }
 
 // NewSubGrids of one patch: the subgrid boxes (patch-local indices) in the
-// order MakeNewGrid creates them.
-int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, std::vector<Box>& sub) {
+// order MakeNewGrid creates them; on failure BSAM_EINVAL with `msg` set.
+int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, std::vector<Box>& sub,
+                     std::string& msg) {
   sub.clear();
-  if (mgp < 4) return bsam::fail(BSAM_EINVAL, "level %d: minimumgridpoints cannot be less than 4", level);
-  if (mgp % 2) return bsam::fail(BSAM_EINVAL, "level %d: minimumgridpoints must be even", level);
+  char buf[160];
+  auto fail = [&](const char* what) {
+    snprintf(buf, sizeof buf, "level %d: %s", level, what);
+    msg = buf;
+    return BSAM_EINVAL;
+  };
+  if (mgp < 4) return fail("minimumgridpoints cannot be less than 4");
+  if (mgp % 2) return fail("minimumgridpoints must be even");
   Box whole;
   for (int n = 0; n < 3; n++) {
     whole.lo[n] = 1;
@@ -124,11 +134,16 @@ int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, 
         sub[g].lo[n] = i1 - (i1 + 1) % 2;
         sub[g].hi[n] = i2 + i2 % 2;
       }
-      if (!(fill_ratio(T, sub[g], nd) < desfill)) {
+      // GridFlagRatio of the trimmed box, from its signatures (their sum
+      // over one dimension counts the box's tags exactly)
+      signatures(T, sub[g], nd, sig);
+      long tagged = 0, cells = 1;
+      for (int i = sub[g].lo[0]; i <= sub[g].hi[0]; i++) tagged += sig[0][i];
+      for (int n = 0; n < nd; n++) cells *= (long)(sub[g].hi[n] - sub[g].lo[n] + 1);
+      if (!((double)tagged / (double)cells < desfill)) {
         can[g] = 0;
         continue;
       }
-      signatures(T, sub[g], nd, sig);
       // a hole: the first untagged slab (last index of the first half) that
       // leaves both halves at least mgp wide
       int isplit[3] = {0, 0, 0};
@@ -178,8 +193,7 @@ int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, 
       }
       for (int n = 0; n < nd; n++) {
         if (isplit[n] > 0 && std::min(sub[g].hi[n] - isplit[n], isplit[n] - sub[g].lo[n] + 1) >= mgp) {
-          if (sub.size() >= (size_t)kMaxSubgrids)
-            return bsam::fail(BSAM_EINVAL, "level %d: more than %d subgrids in one patch", level, kMaxSubgrids);
+          if (sub.size() >= (size_t)kMaxSubgrids) return fail("more than 1024 subgrids in one patch");
           Box b = sub[g];
           b.lo[n] = isplit[n] + 1;
           sub[g].hi[n] = isplit[n];
@@ -193,11 +207,30 @@ int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, 
   for (const Box& b : sub)
     for (int n = 0; n < nd; n++) {
       const int len = b.hi[n] - b.lo[n] + 1;
-      if (len < mgp) return bsam::fail(BSAM_EINVAL, "level %d: subgrid smaller than minimumgridpoints", level);
-      if (len % 2 || b.lo[n] % 2 == 0 || b.hi[n] % 2)
-        return bsam::fail(BSAM_EINVAL, "level %d: subgrid not aligned to coarse cell pairs", level);
+      if (len < mgp) return fail("subgrid smaller than minimumgridpoints");
+      if (len % 2 || b.lo[n] % 2 == 0 || b.hi[n] % 2) return fail("subgrid not aligned to coarse cell pairs");
     }
   return BSAM_OK;
+}
+
+// fn(t) for t = 0 .. n-1 on host threads (those that cannot be started run
+// on the caller's thread).
+template <typename F>
+void for_threads(int n, F&& fn) {
+  std::vector<std::thread> th;
+  int started = 1;
+  try {
+    for (; started < n; started++) th.emplace_back([&fn, started]() { fn(started); });
+  } catch (...) {
+  }
+  for (int t = started; t < n; t++) fn(t);
+  fn(0);
+  for (auto& x : th) x.join();
+}
+
+int host_threads() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return hw ? (int)std::min(16u, hw) : 4;
 }
 
 // MakeNewGrid's descriptor of the child of patch `pidx` over its local box b.
@@ -262,36 +295,45 @@ extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patc
     for (int n = 0; n < 3; n++) T[a].mx[n] = desc[lv[a]].mx[n];
     T[a].f = tg.data() + off[a];
   }
+  const int nth = std::max(1, std::min<int>((int)lv.size(), host_threads()));
   if (ibuffer > 0) {
     // BufferAndList: a tag whose buffer box stays inside its patch is
-    // buffered there; the others are listed in global indices
+    // buffered there; the others are listed in global indices.  Buffering
+    // only ever sets flags to 1 and reads the unbuffered tags, so the
+    // patches, and then the listed tags, are spread over host threads
+    // (flags set by two threads at once are written with relaxed atomic
+    // stores).
+    std::vector<std::vector<std::array<int, 3>>> edges(lv.size());
+    for_threads(nth, [&](int t) {
+      for (size_t a = t; a < lv.size(); a += nth) {
+        const bsam_patch_desc& P = desc[lv[a]];
+        const Tags O{{P.mx[0], P.mx[1], P.mx[2]}, const_cast<int32_t*>(flags) + off[a]};
+        for (int k = 1; k <= P.mx[2]; k++)
+          for (int j = 1; j <= P.mx[1]; j++)
+            for (int i = 1; i <= P.mx[0]; i++) {
+              if (O.at(i, j, k) != 1) continue;
+              const int idx[3] = {i, j, k};
+              int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+              bool inside = true;
+              for (int n = 0; n < ndims; n++) {
+                lo[n] = idx[n] - ibuffer;
+                hi[n] = idx[n] + ibuffer;
+                inside = inside && lo[n] >= 1 && hi[n] <= P.mx[n];
+              }
+              if (inside) {
+                for (int z = lo[2]; z <= hi[2]; z++)
+                  for (int y = lo[1]; y <= hi[1]; y++)
+                    for (int x = lo[0]; x <= hi[0]; x++) T[a].at(x, y, z) = 1;
+              } else {
+                std::array<int, 3> g = {1, 1, 1};
+                for (int n = 0; n < ndims; n++) g[n] = idx[n] + P.mglobal[n][0] - 1;
+                edges[a].push_back(g);
+              }
+            }
+      }
+    });
     std::vector<std::array<int, 3>> edge;
-    for (size_t a = 0; a < lv.size(); a++) {
-      const bsam_patch_desc& P = desc[lv[a]];
-      const Tags O{{P.mx[0], P.mx[1], P.mx[2]}, const_cast<int32_t*>(flags) + off[a]};
-      for (int k = 1; k <= P.mx[2]; k++)
-        for (int j = 1; j <= P.mx[1]; j++)
-          for (int i = 1; i <= P.mx[0]; i++) {
-            if (O.at(i, j, k) != 1) continue;
-            const int idx[3] = {i, j, k};
-            int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
-            bool inside = true;
-            for (int n = 0; n < ndims; n++) {
-              lo[n] = idx[n] - ibuffer;
-              hi[n] = idx[n] + ibuffer;
-              inside = inside && lo[n] >= 1 && hi[n] <= P.mx[n];
-            }
-            if (inside) {
-              for (int z = lo[2]; z <= hi[2]; z++)
-                for (int y = lo[1]; y <= hi[1]; y++)
-                  for (int x = lo[0]; x <= hi[0]; x++) T[a].at(x, y, z) = 1;
-            } else {
-              std::array<int, 3> g = {1, 1, 1};
-              for (int n = 0; n < ndims; n++) g[n] = idx[n] + P.mglobal[n][0] - 1;
-              edge.push_back(g);
-            }
-          }
-    }
+    for (auto& e : edges) edge.insert(edge.end(), e.begin(), e.end());
     // InflateEdgeTags: the buffer box of every listed tag, and of its
     // periodic images, on every patch of the level it meets (a bin index
     // over the level's global index space finds the patches)
@@ -322,55 +364,66 @@ extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patc
           for (int y = b0[1]; y <= b1[1]; y++)
             for (int x = b0[0]; x <= b1[0]; x++) bins[((size_t)z * nb[1] + y) * nb[0] + x].push_back((int)a);
       }
-      std::vector<long> stamp(lv.size(), -1);
-      long tick = 0;
-      auto apply = [&](const std::array<int, 3>& c) {
-        int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1}, b0[3], b1[3];
-        for (int n = 0; n < ndims; n++) lo[n] = c[n] - ibuffer, hi[n] = c[n] + ibuffer;
-        for (int n = 0; n < ndims; n++)
-          if (hi[n] < 1 || lo[n] > ext[n]) return;
-        bin_range(lo, hi, b0, b1);
-        tick++;
-        for (int z = b0[2]; z <= b1[2]; z++)
-          for (int y = b0[1]; y <= b1[1]; y++)
-            for (int x = b0[0]; x <= b1[0]; x++)
-              for (int a : bins[((size_t)z * nb[1] + y) * nb[0] + x]) {
-                if (stamp[a] == tick) continue;
-                stamp[a] = tick;
-                const bsam_patch_desc& P = desc[lv[a]];
-                int ml[3] = {1, 1, 1}, mh[3] = {1, 1, 1};
-                bool hit = true;
-                for (int n = 0; n < ndims && hit; n++) {
-                  const int o0 = std::max(lo[n], P.mglobal[n][0]), o1 = std::min(hi[n], P.mglobal[n][1]);
-                  hit = o1 >= o0;
-                  ml[n] = o0 - P.mglobal[n][0] + 1;
-                  mh[n] = o1 - P.mglobal[n][0] + 1;
+      const int eth = std::max(1, std::min<int>((int)(edge.size() / 256) + 1, host_threads()));
+      for_threads(eth, [&](int t) {
+        std::vector<long> stamp(lv.size(), -1);
+        long tick = 0;
+        auto apply = [&](const std::array<int, 3>& cc) {
+          int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1}, b0[3], b1[3];
+          for (int n = 0; n < ndims; n++) lo[n] = cc[n] - ibuffer, hi[n] = cc[n] + ibuffer;
+          for (int n = 0; n < ndims; n++)
+            if (hi[n] < 1 || lo[n] > ext[n]) return;
+          bin_range(lo, hi, b0, b1);
+          tick++;
+          for (int z = b0[2]; z <= b1[2]; z++)
+            for (int y = b0[1]; y <= b1[1]; y++)
+              for (int x = b0[0]; x <= b1[0]; x++)
+                for (int a : bins[((size_t)z * nb[1] + y) * nb[0] + x]) {
+                  if (stamp[a] == tick) continue;
+                  stamp[a] = tick;
+                  const bsam_patch_desc& P = desc[lv[a]];
+                  int ml[3] = {1, 1, 1}, mh[3] = {1, 1, 1};
+                  bool hit = true;
+                  for (int n = 0; n < ndims && hit; n++) {
+                    const int o0 = std::max(lo[n], P.mglobal[n][0]), o1 = std::min(hi[n], P.mglobal[n][1]);
+                    hit = o1 >= o0;
+                    ml[n] = o0 - P.mglobal[n][0] + 1;
+                    mh[n] = o1 - P.mglobal[n][0] + 1;
+                  }
+                  if (!hit) continue;
+                  for (int z2 = ml[2]; z2 <= mh[2]; z2++)
+                    for (int y2 = ml[1]; y2 <= mh[1]; y2++)
+                      for (int x2 = ml[0]; x2 <= mh[0]; x2++) __atomic_store_n(&T[a].at(x2, y2, z2), 1, __ATOMIC_RELAXED);
                 }
-                if (!hit) continue;
-                for (int z2 = ml[2]; z2 <= mh[2]; z2++)
-                  for (int y2 = ml[1]; y2 <= mh[1]; y2++)
-                    for (int x2 = ml[0]; x2 <= mh[0]; x2++) T[a].at(x2, y2, z2) = 1;
-              }
-      };
-      for (const auto& c : edge) {
-        apply(c);
-        if (!periodic) continue;
-        for (int pol = -1; pol <= 1; pol += 2)
-          for (const auto& o : offs) {
-            std::array<int, 3> s = c;
-            for (int n = 0; n < ndims; n++) s[n] = c[n] + pol * o[n];
-            apply(s);
-          }
-      }
+        };
+        for (size_t e = t; e < edge.size(); e += eth) {
+          const std::array<int, 3>& cc = edge[e];
+          apply(cc);
+          if (!periodic) continue;
+          for (int pol = -1; pol <= 1; pol += 2)
+            for (const auto& o : offs) {
+              std::array<int, 3> sh = cc;
+              for (int n = 0; n < ndims; n++) sh[n] = cc[n] + pol * o[n];
+              apply(sh);
+            }
+        }
+      });
     }
   }
-  // RefineGrid over the level in visiting order; MakeNewGrid in that order
+  // RefineGrid over the level in visiting order: the patches' subgrids are
+  // independent, found on host threads (at most 16), and MakeNewGrid runs
+  // in the reference's order
+  std::vector<std::vector<Box>> sub(lv.size());
+  std::vector<int32_t> rc(lv.size(), BSAM_OK);
+  std::vector<std::string> msg(lv.size());
+  for_threads(nth, [&](int t) {
+    for (size_t a = t; a < lv.size(); a += nth)
+      rc[a] = new_subgrids(T[a], ndims, minimumgridpoints, desiredfillratio, level, sub[a], msg[a]);
+  });
   std::vector<bsam_patch_desc> made;
-  std::vector<Box> sub;
   for (size_t a = 0; a < lv.size(); a++) {
-    const int32_t r = new_subgrids(T[a], ndims, minimumgridpoints, desiredfillratio, level, sub);
-    if (r != BSAM_OK) return r;
-    for (const Box& b : sub) made.push_back(make_child(desc[lv[a]], lv[a], b, ndims));
+    if (rc[a] != BSAM_OK) return bsam::fail(rc[a], "%s", msg[a].c_str());
+    for (const Box& b : sub[a]) made.push_back(make_child(desc[lv[a]], lv[a], b, ndims));
   }
   *nout = (int32_t)made.size();
   if ((int32_t)made.size() > maxout)

@@ -374,6 +374,12 @@ int32_t bsam_tuning_report(const bsam_ctx* ctx, char* buf, int32_t len);
                                            in cells, at which the one-workgroup coarse V-cycle
                                            chain starts (0: no chain); defaults 4096 (CH) and
                                            512 (tumour model) */
+#define BSAM_KNOB_REGRID_TABLES      7  /* whole hierarchy (level ignored): 1 (default), the
+                                           levels bsam_regrid and bsam_amr build get the
+                                           flattened ghost tables; 0, they keep the staged
+                                           fill (interpolation rules, then box stages), which
+                                           is planned in a fraction of the time, for meshes
+                                           regridded every step */
 int32_t bsam_set_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t value);
 int32_t bsam_get_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t* value);
 

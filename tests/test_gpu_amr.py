@@ -69,10 +69,15 @@ def test_amr_builds_the_reference_mesh(name, gpu_solver_cls):
     S.close()
 
 
+@pytest.mark.parametrize("tables", [1, 0])
 @pytest.mark.parametrize("name", list(RUNS))
-def test_time_step_amr_matches_reference(name, gpu_solver_cls):
-    print(f"[gpu] time step with AMR {name}", flush=True)
+def test_time_step_amr_matches_reference(name, tables, gpu_solver_cls):
+    """tables 0: the levels AMR builds keep the staged ghost fill
+    (BSAM_KNOB_REGRID_TABLES), which must give the same bits."""
+    print(f"[gpu] time step with AMR {name} tables {tables}", flush=True)
     S, meta, z, grid = _start(gpu_solver_cls, name)
+    S.set_knob(abi.KNOB_REGRID_TABLES, 0, tables)
+    assert S.get_knob(abi.KNOB_REGRID_TABLES, 0) == tables
     s1 = meta["steps"][1]
     rc, n, r = S.time_step_amr(grid, False)
     assert rc == abi.BSAM_OK, (rc, n, r)
