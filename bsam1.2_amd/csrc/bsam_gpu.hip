@@ -14,6 +14,7 @@
 // expression keeps the reference's evaluation order, so results are the
 // reference's to the last bit except for the residual-norm sum.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <array>
@@ -3442,149 +3443,204 @@ struct GhostPlan {
 // colour except that ipass = 1 skips those touching a non-interpolated
 // face (segment 2 then, else 1), including the comparison with 99 instead
 // of 999 at boundary.f90:1104; grey cells refresh always (segment 1).
-static void interp_rules(const DPatch& P, const DPatch& Q, int ndims, std::vector<GRule> (&seg)[3]) {
+// Elements of a patch's coarse-fine interpolation (InterpolateCoarseFine,
+// boundary.f90:100-140, Interpolate2D :143-333, Interpolate3D :1048-1550):
+// the face cells, then (3D) the edge cells, then the corners of the ghost
+// shell, in the order below.
+__host__ __device__ inline int interp_elems(const DPatch& P, int ndims) {
+  if (ndims == 2) return 2 * P.mx[1] + 2 * P.mx[0] + 4;
+  const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
+  return 2 * (my * mz + mx * mz + mx * my) + 4 * (my + mz + mx) + 8;
+}
+
+// The rule of element e of patch P (parent Q) and its colour-filter
+// segment; false when the element's face has no coarse-fine interpolation.
+__host__ __device__ inline bool interp_rule_at(const DPatch& P, const DPatch& Q, int ndims, int e, GRule& rule,
+                                               int& sg) {
   const int* mth = P.mthbc;
   auto k3 = [&](int k) { return ndims == 3 ? k : 0; };
-  auto emit = [&](int sg, const int g[3], const int s1[3], const int s2[3], int nco, const int (*cq)[3]) {
-    GRule r;
-    r.dst = qidx(P, g[0], g[1], k3(g[2]));
-    r.s1 = qidx(P, s1[0], s1[1], k3(s1[2]));
-    r.s2 = qidx(P, s2[0], s2[1], k3(s2[2]));
-    for (int n = 0; n < 4; n++) r.c[n] = (n < nco) ? qidx(Q, cq[n][0], cq[n][1], k3(cq[n][2])) : 0;
-    r.kind = nco;
-    r.pvs = P.qvs;
-    r.cvs = Q.qvs;
-    seg[sg].push_back(r);
+  int g[3], s1[3], s2[3], cq[4][3];
+  int nco = 1;
+  auto emit = [&]() {
+    rule.dst = qidx(P, g[0], g[1], k3(g[2]));
+    rule.s1 = qidx(P, s1[0], s1[1], k3(s1[2]));
+    rule.s2 = qidx(P, s2[0], s2[1], k3(s2[2]));
+    for (int n = 0; n < 4; n++) rule.c[n] = (n < nco) ? qidx(Q, cq[n][0], cq[n][1], k3(cq[n][2])) : 0;
+    rule.kind = nco;
+    rule.pvs = P.qvs;
+    rule.cvs = Q.qvs;
+    return true;
   };
   int fi[3][5];
   if (ndims == 2) {
     const int nf0 = P.mx[1], nf1 = P.mx[0];
-    const int total = 2 * nf0 + 2 * nf1 + 4;
-    for (int e = 0; e < total; e++) {
-      int g[3], s1[3], s2[3], cq[4][3];
-      int nco = 1;
-      g[2] = s1[2] = s2[2] = 1;
-      for (int n = 0; n < 4; n++) cq[n][2] = 1;
-      int sg;
-      if (e < 2 * nf0 + 2 * nf1) {
-        int d, p, pos;
-        if (e < 2 * nf0) { d = 0; p = 1 + e / nf0; pos = e % nf0 + 1; }
-        else { int e2 = e - 2 * nf0; d = 1; p = 1 + e2 / nf1; pos = e2 % nf1 + 1; }
-        if (!bc_interp(mth[2 * d + p - 1])) continue;
-        const int t = 1 - d;
-        face_index(P, d, p, fi[d]);
-        sg = ((fi[d][0] + pos) & 1) ? 2 : 0;
-        const int k = pos_kind(pos, P.mx[t]);
-        g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
-        g[t] = pos; s1[t] = pos + kind_s1(k); s2[t] = pos + kind_s2(k);
-        if (k == KE || k == KO) {
-          cq[0][d] = fi[d][3]; cq[0][t] = kind_c(P, t, k, pos, 0);
-        } else {
-          nco = 3;
-          for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); }
-        }
+    g[2] = s1[2] = s2[2] = 1;
+    for (int n = 0; n < 4; n++) cq[n][2] = 1;
+    if (e < 2 * nf0 + 2 * nf1) {
+      int d, p, pos;
+      if (e < 2 * nf0) { d = 0; p = 1 + e / nf0; pos = e % nf0 + 1; }
+      else { int e2 = e - 2 * nf0; d = 1; p = 1 + e2 / nf1; pos = e2 % nf1 + 1; }
+      if (!bc_interp(mth[2 * d + p - 1])) return false;
+      const int t = 1 - d;
+      face_index(P, d, p, fi[d]);
+      sg = ((fi[d][0] + pos) & 1) ? 2 : 0;
+      const int k = pos_kind(pos, P.mx[t]);
+      g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
+      g[t] = pos; s1[t] = pos + kind_s1(k); s2[t] = pos + kind_s2(k);
+      if (k == KE || k == KO) {
+        cq[0][d] = fi[d][3]; cq[0][t] = kind_c(P, t, k, pos, 0);
       } else {
-        // corners (boundary.f90:155-181): applied whatever mthbc is
-        const int cn = e - (2 * nf0 + 2 * nf1);
-        const int p0 = 1 + (cn & 1), p1 = 1 + (cn >> 1);
-        face_index(P, 0, p0, fi[0]);
-        face_index(P, 1, p1, fi[1]);
-        sg = ((fi[0][0] + fi[1][0]) & 1) ? 2 : 0;
-        for (int a = 0; a < 2; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
+        nco = 3;
+        for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); }
       }
-      emit(sg, g, s1, s2, nco, cq);
+    } else {
+      // corners (boundary.f90:155-181): applied whatever mthbc is
+      const int cn = e - (2 * nf0 + 2 * nf1);
+      const int p0 = 1 + (cn & 1), p1 = 1 + (cn >> 1);
+      face_index(P, 0, p0, fi[0]);
+      face_index(P, 1, p1, fi[1]);
+      sg = ((fi[0][0] + fi[1][0]) & 1) ? 2 : 0;
+      for (int a = 0; a < 2; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
     }
-    return;
+    return emit();
   }
   const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
   const int nface[3] = {my * mz, mx * mz, mx * my};
   const int nfaces = 2 * (nface[0] + nface[1] + nface[2]);
   const int nedge[3] = {my, mz, mx};  // edges along dim 1 (ee=0), dim 2 (ee=1), dim 0 (ee=2)
   const int nedges = 4 * (nedge[0] + nedge[1] + nedge[2]);
-  const int total = nfaces + nedges + 8;
-  for (int e = 0; e < total; e++) {
-    int g[3], s1[3], s2[3], cq[4][3];
-    int nco = 1, sg;
-    if (e < nfaces) {
-      int d = 0, rem = e;
-      while (rem >= 2 * nface[d]) { rem -= 2 * nface[d]; d++; }
-      const int p = 1 + rem / nface[d];
-      const int local = rem % nface[d];
-      if (!bc_interp(mth[2 * d + p - 1])) continue;
-      const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
-      const int a = local % P.mx[t1] + 1, b = local / P.mx[t1] + 1;
-      const int k1 = pos_kind(a, P.mx[t1]), k2 = pos_kind(b, P.mx[t2]);
-      const bool q1 = k1 >= KL, q2 = k2 >= KL;
-      face_index(P, d, p, fi[d]);
-      g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
-      if (q1 && q2) {
-        // face-corner grey cells (boundary.f90:1320-1333 and the like)
-        int ft1[5], ft2[5];
-        face_index(P, t1, a == 1 ? 1 : 2, ft1);
-        face_index(P, t2, b == 1 ? 1 : 2, ft2);
-        g[t1] = s1[t1] = s2[t1] = ft1[2];
-        g[t2] = s1[t2] = s2[t2] = ft2[2];
-        nco = 4;
-        for (int r = 0; r < 4; r++) cq[r][d] = fi[d][3];
-        cq[0][t1] = ft1[4]; cq[0][t2] = ft2[4];
-        cq[1][t1] = ft1[3]; cq[1][t2] = ft2[4];
-        cq[2][t1] = ft1[4]; cq[2][t2] = ft2[3];
-        cq[3][t1] = ft1[3]; cq[3][t2] = ft2[3];
-        sg = 1;
-      } else {
-        sg = ((fi[d][0] + a + b) & 1) ? 2 : 0;
-        g[t1] = a; s1[t1] = a + kind_s1(k1); s2[t1] = a + kind_s2(k1);
-        g[t2] = b; s1[t2] = b + kind_s1(k2); s2[t2] = b + kind_s2(k2);
-        if (!q1 && !q2) {
-          cq[0][d] = fi[d][3]; cq[0][t1] = kind_c(P, t1, k1, a, 0); cq[0][t2] = kind_c(P, t2, k2, b, 0);
-        } else {
-          nco = 3;
-          for (int r = 0; r < 3; r++) {
-            cq[r][d] = fi[d][3];
-            cq[r][t1] = kind_c(P, t1, k1, a, q1 ? r : 0);
-            cq[r][t2] = kind_c(P, t2, k2, b, q2 ? r : 0);
-          }
-        }
-      }
-    } else if (e < nfaces + nedges) {
-      int rem = e - nfaces, ee = 0;
-      while (rem >= 4 * nedge[ee]) { rem -= 4 * nedge[ee]; ee++; }
-      const int ed = (ee == 0) ? 1 : (ee == 1 ? 2 : 0);
-      const int fa = (ed == 0) ? 1 : 0, fb = (ed == 2) ? 1 : 2;
-      const int combo = rem / nedge[ee], pos = rem % nedge[ee] + 1;
-      const int pa = 1 + (combo >> 1), pb = 1 + (combo & 1);
-      bool skip1 = !bc_interp(mth[2 * fa + pa - 1]);
-      const int cb = mth[2 * fb + pb - 1];
-      if (ee == 0) skip1 = skip1 || (cb != 2 && cb != 99);  // boundary.f90:1104
-      else skip1 = skip1 || !bc_interp(cb);
-      sg = skip1 ? 2 : 1;
-      face_index(P, fa, pa, fi[fa]);
-      face_index(P, fb, pb, fi[fb]);
-      const int k = pos_kind(pos, P.mx[ed]);
-      g[fa] = fi[fa][0]; s1[fa] = fi[fa][1]; s2[fa] = fi[fa][2];
-      g[fb] = fi[fb][0]; s1[fb] = fi[fb][1]; s2[fb] = fi[fb][2];
-      g[ed] = pos; s1[ed] = pos + kind_s1(k); s2[ed] = pos + kind_s2(k);
-      if (k == KE || k == KO) {
-        cq[0][fa] = fi[fa][3]; cq[0][fb] = fi[fb][3]; cq[0][ed] = kind_c(P, ed, k, pos, 0);
+  if (e < nfaces) {
+    int d = 0, rem = e;
+    while (rem >= 2 * nface[d]) { rem -= 2 * nface[d]; d++; }
+    const int p = 1 + rem / nface[d];
+    const int local = rem % nface[d];
+    if (!bc_interp(mth[2 * d + p - 1])) return false;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    const int a = local % P.mx[t1] + 1, b = local / P.mx[t1] + 1;
+    const int k1 = pos_kind(a, P.mx[t1]), k2 = pos_kind(b, P.mx[t2]);
+    const bool q1 = k1 >= KL, q2 = k2 >= KL;
+    face_index(P, d, p, fi[d]);
+    g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
+    if (q1 && q2) {
+      // face-corner grey cells (boundary.f90:1320-1333 and the like)
+      int ft1[5], ft2[5];
+      face_index(P, t1, a == 1 ? 1 : 2, ft1);
+      face_index(P, t2, b == 1 ? 1 : 2, ft2);
+      g[t1] = s1[t1] = s2[t1] = ft1[2];
+      g[t2] = s1[t2] = s2[t2] = ft2[2];
+      nco = 4;
+      for (int r = 0; r < 4; r++) cq[r][d] = fi[d][3];
+      cq[0][t1] = ft1[4]; cq[0][t2] = ft2[4];
+      cq[1][t1] = ft1[3]; cq[1][t2] = ft2[4];
+      cq[2][t1] = ft1[4]; cq[2][t2] = ft2[3];
+      cq[3][t1] = ft1[3]; cq[3][t2] = ft2[3];
+      sg = 1;
+    } else {
+      sg = ((fi[d][0] + a + b) & 1) ? 2 : 0;
+      g[t1] = a; s1[t1] = a + kind_s1(k1); s2[t1] = a + kind_s2(k1);
+      g[t2] = b; s1[t2] = b + kind_s1(k2); s2[t2] = b + kind_s2(k2);
+      if (!q1 && !q2) {
+        cq[0][d] = fi[d][3]; cq[0][t1] = kind_c(P, t1, k1, a, 0); cq[0][t2] = kind_c(P, t2, k2, b, 0);
       } else {
         nco = 3;
         for (int r = 0; r < 3; r++) {
-          cq[r][fa] = fi[fa][3]; cq[r][fb] = fi[fb][3]; cq[r][ed] = kind_c(P, ed, k, pos, r);
+          cq[r][d] = fi[d][3];
+          cq[r][t1] = kind_c(P, t1, k1, a, q1 ? r : 0);
+          cq[r][t2] = kind_c(P, t2, k2, b, q2 ? r : 0);
         }
       }
-    } else {
-      const int cn = e - nfaces - nedges;
-      const int p1 = 1 + (cn & 1), p2 = 1 + ((cn >> 1) & 1), p3 = 1 + (cn >> 2);
-      const bool skip1 = !bc_interp(mth[p1 - 1]) || !bc_interp(mth[p2 + 1]) || !bc_interp(mth[p3 + 3]);
-      sg = skip1 ? 2 : 1;
-      face_index(P, 0, p1, fi[0]);
-      face_index(P, 1, p2, fi[1]);
-      face_index(P, 2, p3, fi[2]);
-      for (int a = 0; a < 3; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
     }
-    emit(sg, g, s1, s2, nco, cq);
+  } else if (e < nfaces + nedges) {
+    int rem = e - nfaces, ee = 0;
+    while (rem >= 4 * nedge[ee]) { rem -= 4 * nedge[ee]; ee++; }
+    const int ed = (ee == 0) ? 1 : (ee == 1 ? 2 : 0);
+    const int fa = (ed == 0) ? 1 : 0, fb = (ed == 2) ? 1 : 2;
+    const int combo = rem / nedge[ee], pos = rem % nedge[ee] + 1;
+    const int pa = 1 + (combo >> 1), pb = 1 + (combo & 1);
+    bool skip1 = !bc_interp(mth[2 * fa + pa - 1]);
+    const int cb = mth[2 * fb + pb - 1];
+    if (ee == 0) skip1 = skip1 || (cb != 2 && cb != 99);  // boundary.f90:1104
+    else skip1 = skip1 || !bc_interp(cb);
+    sg = skip1 ? 2 : 1;
+    face_index(P, fa, pa, fi[fa]);
+    face_index(P, fb, pb, fi[fb]);
+    const int k = pos_kind(pos, P.mx[ed]);
+    g[fa] = fi[fa][0]; s1[fa] = fi[fa][1]; s2[fa] = fi[fa][2];
+    g[fb] = fi[fb][0]; s1[fb] = fi[fb][1]; s2[fb] = fi[fb][2];
+    g[ed] = pos; s1[ed] = pos + kind_s1(k); s2[ed] = pos + kind_s2(k);
+    if (k == KE || k == KO) {
+      cq[0][fa] = fi[fa][3]; cq[0][fb] = fi[fb][3]; cq[0][ed] = kind_c(P, ed, k, pos, 0);
+    } else {
+      nco = 3;
+      for (int r = 0; r < 3; r++) {
+        cq[r][fa] = fi[fa][3]; cq[r][fb] = fi[fb][3]; cq[r][ed] = kind_c(P, ed, k, pos, r);
+      }
+    }
+  } else {
+    const int cn = e - nfaces - nedges;
+    const int p1 = 1 + (cn & 1), p2 = 1 + ((cn >> 1) & 1), p3 = 1 + (cn >> 2);
+    const bool skip1 = !bc_interp(mth[p1 - 1]) || !bc_interp(mth[p2 + 1]) || !bc_interp(mth[p3 + 3]);
+    sg = skip1 ? 2 : 1;
+    face_index(P, 0, p1, fi[0]);
+    face_index(P, 1, p2, fi[1]);
+    face_index(P, 2, p3, fi[2]);
+    for (int a = 0; a < 3; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
+  }
+  return emit();
+}
+
+static void interp_rules(const DPatch& P, const DPatch& Q, int ndims, std::vector<GRule> (&seg)[3]) {
+  const int total = interp_elems(P, ndims);
+  for (int e = 0; e < total; e++) {
+    GRule r;
+    int sg = 0;
+    if (interp_rule_at(P, Q, ndims, e, r, sg)) seg[sg].push_back(r);
   }
 }
+
+// The interpolation rules of a level built on the device, for levels that
+// keep the staged fill: the elements of the level's patches (interp_elems
+// each, patch after patch, from eoff) are classified by colour-filter
+// segment, compacted in order segment by segment (hipCUB DeviceSelect), and
+// written, so the table equals plan_interp_rules' (segment, then patch,
+// then element order).
+__device__ __forceinline__ int elem_patch(const int* __restrict__ eoff, int np, long g) {
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (eoff[mid] <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+__global__ __launch_bounds__(kBlock) void k_interp_segment(const DPatch* __restrict__ pt, const int* __restrict__ patches,
+                                                           const int* __restrict__ eoff, int np, int ndims, long total,
+                                                           uint8_t* __restrict__ seg) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= total) return;
+  const int a = elem_patch(eoff, np, g);
+  const DPatch P = pt[patches[a]];
+  GRule r;
+  int sg = 0;
+  seg[g] = interp_rule_at(P, pt[P.parent], ndims, (int)(g - eoff[a]), r, sg) ? (uint8_t)sg : (uint8_t)3;
+}
+__global__ __launch_bounds__(kBlock) void k_interp_write(const DPatch* __restrict__ pt, const int* __restrict__ patches,
+                                                         const int* __restrict__ eoff, int np, int ndims,
+                                                         const int* __restrict__ idx, int n, GRule* __restrict__ out) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const int g = idx[j];
+  const int a = elem_patch(eoff, np, g);
+  const DPatch P = pt[patches[a]];
+  GRule r;
+  int sg = 0;
+  interp_rule_at(P, pt[P.parent], ndims, g - eoff[a], r, sg);
+  out[j] = r;
+}
+struct SegmentIs {
+  const uint8_t* seg;
+  uint8_t want;
+  __host__ __device__ bool operator()(const int& e) const { return seg[e] == want; }
+};
 
 // Host threads for the hierarchy set-up (at most 16).
 static size_t host_threads() {
@@ -4542,6 +4598,70 @@ int32_t plan_patches(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, i
 }  // namespace
 
 namespace {
+// The level's interpolation rules on the device (k_interp_segment,
+// DeviceSelect per segment, k_interp_write).  Only the segment bounds of
+// the whole level are kept (roff[s][0], roff[s][np]): the batched staged
+// fill needs no others.
+int32_t build_rules_device(bsam_ctx* c, Level& L) {
+  const int np = (int)L.patches.size(), nd = c->p.ndims;
+  std::vector<int> eoff(np + 1, 0);
+  long total = 0;
+  for (int a = 0; a < np; a++) {
+    eoff[a] = (int)total;
+    total += interp_elems(c->hp[L.patches[a]], nd);
+  }
+  if (total > 0x7fffffffL) return set_err(BSAM_EINVAL, "level %d: too many interpolation elements", L.level);
+  eoff[np] = (int)total;
+  struct Tmp {
+    int* eoff = nullptr;
+    uint8_t* seg = nullptr;
+    int* idx = nullptr;
+    int* num = nullptr;
+    void* scratch = nullptr;
+    ~Tmp() {
+      hipFree(eoff);
+      hipFree(seg);
+      hipFree(idx);
+      hipFree(num);
+      hipFree(scratch);
+    }
+  } t;
+  HIPCHK(hipMalloc(&t.eoff, (np + 1) * sizeof(int)));
+  HIPCHK(hipMalloc(&t.seg, std::max<long>(1, total)));
+  HIPCHK(hipMalloc(&t.idx, std::max<long>(1, total) * sizeof(int)));
+  HIPCHK(hipMalloc(&t.num, sizeof(int)));
+  HIPCHK(hipMemcpy(t.eoff, eoff.data(), (np + 1) * sizeof(int), hipMemcpyHostToDevice));
+  for (int sg = 0; sg < 3; sg++) L.roff[sg].assign(np + 1, 0);
+  if (total == 0) return BSAM_OK;
+  k_interp_segment<<<nblocks(total), kBlock, 0, c->s>>>(c->d_pt, L.d_patches, t.eoff, np, nd, total, t.seg);
+  LAUNCH_CHECK();
+  size_t bytes = 0;
+  hipcub::CountingInputIterator<int> elems(0);
+  HIPCHK(hipcub::DeviceSelect::If(nullptr, bytes, elems, t.idx, t.num, (int)total, SegmentIs{t.seg, 0}, c->s));
+  HIPCHK(hipMalloc(&t.scratch, std::max<size_t>(1, bytes)));
+  int off = 0;
+  for (int sg = 0; sg < 3; sg++) {
+    size_t b2 = bytes;
+    HIPCHK(hipcub::DeviceSelect::If(t.scratch, b2, elems, t.idx + off, t.num, (int)total, SegmentIs{t.seg, (uint8_t)sg},
+                                    c->s));
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, t.num, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    L.roff[sg][0] = off;
+    off += n;
+    L.roff[sg][np] = off;
+  }
+  hipFree(L.d_rules);
+  L.d_rules = nullptr;
+  if (off > 0) {
+    HIPCHK(hipMalloc(&L.d_rules, (size_t)off * sizeof(GRule)));
+    k_interp_write<<<nblocks(off), kBlock, 0, c->s>>>(c->d_pt, L.d_patches, t.eoff, np, nd, t.idx, off, L.d_rules);
+    LAUNCH_CHECK();
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return BSAM_OK;
+}
+
 // The device side of one level of a new hierarchy: its patch list, the
 // block-aligned launch ranges of every domain, the tagging offsets and the
 // ghost fill (stages, interpolation rules, flattened tables).
@@ -4616,9 +4736,17 @@ int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
   }
   GhostPlan G;
   RETCHK(plan_ghost_stages(c, L, G));
-  plan_interp_rules(c, L, G);
+  // a level that keeps the staged fill in batched launches builds its
+  // interpolation rules on the device
+  const bool dev_rules = !tables && c->p.batching && L.level > 0;
+  if (dev_rules) {
+    for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(L.patches.size() + 1, 0);
+  } else {
+    plan_interp_rules(c, L, G);
+  }
   if (c->p.batching && tables) RETCHK(plan_ghost_maps(c, L, G));
   RETCHK(upload_ghost(c, L, G));
+  if (dev_rules) RETCHK(build_rules_device(c, L));
   return BSAM_OK;
 }
 
