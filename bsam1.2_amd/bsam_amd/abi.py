@@ -147,6 +147,9 @@ ENTRY_POINTS = {
     "bsam_amr": ([_P, C.POINTER(GridParams)], _I),
     "bsam_get_hierarchy": ([_P, _I, C.POINTER(PatchDesc), _IP, _IP, _IP], _I),
     "bsam_time_step_amr": ([_P, C.POINTER(GridParams), _I, _IP, _DP], _I),
+    "bsam_parse_frame": ([C.c_char_p, _I, _I, _DP, _IP, _I, C.POINTER(PatchDesc), _IP, _DP, C.c_int64,
+                          C.POINTER(C.c_int64)], _I),
+    "bsam_read_frame": ([_P, C.c_char_p, _DP, _IP], _I),
     "bsam_new_level": ([_I, _I, C.POINTER(PatchDesc), _I, _IP, _I, _I, _D, _I, C.POINTER(PatchDesc), _IP], _I),
     "bsam_synchronize": ([_P], _I),
     "bsam_last_error": ([], C.c_char_p),

@@ -104,6 +104,14 @@ class Solver:
         self._took_mesh()
         return rc, n.value, r.value
 
+    def read_frame(self, path):
+        """ReadQ onto this run (bsam_read_frame): returns (time, finestlevel);
+        the frame's mesh becomes self.hier."""
+        t, fin = C.c_double(), C.c_int32()
+        self._chk(self.lib.bsam_read_frame(self.ctx, path.encode(), C.byref(t), C.byref(fin)))
+        self._took_mesh()
+        return t.value, fin.value
+
     def close(self):
         if getattr(self, "ctx", None) and self.ctx.value:
             self.lib.bsam_destroy(self.ctx)

@@ -6467,6 +6467,76 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
   return BSAM_OK;
 }
 
+// ReadQ (bsaminputoutput.f90:184-443) onto the installed run: the frame's
+// root must match the installed root as the reference checks it (nrvars,
+// dx and xlower/xupper within 1e-8, mx, mglobal = 1..mx); the levels below
+// the root stay as installed, the frame's refined patches replace the
+// levels above it (bsam_parse_frame: visiting order, parents and mbounds
+// from the boxes), their faces inherit the parent's physical or periodic
+// codes where they reach the parent's faces (MakeNewGrid's rule), and q of
+// the root and every refined patch is the frame's.  The other fields are
+// zero, as after AllocFields.
+extern "C" int32_t bsam_read_frame(bsam_ctx* c, const char* path, double* time, int32_t* finestlevel) {
+  if (!c || !path || !time || !finestlevel) return set_err(BSAM_EINVAL, "bad arguments");
+  if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
+  const int nd = c->p.ndims, nv = c->p.nrvars;
+  int32_t np = 0, fin = 0;
+  int64_t nqv = 0;
+  double t = 0.0;
+  RETCHK(bsam_parse_frame(path, nd, nv, &t, &fin, 0, nullptr, &np, nullptr, 0, &nqv));
+  std::vector<bsam_patch_desc> fd(np);
+  std::vector<double> fq((size_t)nqv);
+  RETCHK(bsam_parse_frame(path, nd, nv, &t, &fin, np, fd.data(), &np, fq.data(), nqv, &nqv));
+  const bsam_patch_desc& root = c->desc[c->L(0).patches[0]];
+  const double small = 1.0e-8;
+  const bsam_patch_desc& fr = fd[0];
+  for (int a = 0; a < nd; a++) {
+    // the reference's test of dx is one-sided (bsaminputoutput.f90:306)
+    if (fr.dx - root.dx > small) return set_err(BSAM_EINVAL, "ReadQ: root-level grid spacing (dx) inconsistent");
+    if (fabs(fr.xlower[a] - root.xlower[a]) > small)
+      return set_err(BSAM_EINVAL, "ReadQ: root-level grid location (xlower) inconsistent");
+    if (fr.mx[a] != root.mx[a]) return set_err(BSAM_EINVAL, "ReadQ: root-level grid size (mx) inconsistent");
+    if (fr.mglobal[a][0] != 1 || fr.mglobal[a][1] != root.mx[a])
+      return set_err(BSAM_EINVAL, "ReadQ: root-level grid size (mglobal) inconsistent");
+  }
+  // the table: the installed levels below the root, the root, the frame's
+  // refined patches
+  std::vector<bsam_patch_desc> T;
+  std::vector<int> at(c->npatch, -1);
+  for (int l = c->minlevel; l <= 0; l++)
+    for (int pid : c->L(l).patches) {
+      bsam_patch_desc d = c->desc[pid];
+      if (d.parent >= 0) d.parent = at[d.parent];
+      at[pid] = (int)T.size();
+      T.push_back(d);
+    }
+  const int base = (int)T.size() - 1;  // the root's row
+  for (int n = 1; n < np; n++) {
+    bsam_patch_desc d = fd[n];
+    d.parent = base + d.parent;
+    const bsam_patch_desc& P = T[d.parent];
+    for (int f = 0; f < 6; f++) d.mthbc[f] = 999;
+    for (int a = 0; a < nd; a++) {
+      if (P.mthbc[2 * a] < 999 && d.mbounds[a][0] == 1) d.mthbc[2 * a] = P.mthbc[2 * a];
+      if (P.mthbc[2 * a + 1] < 999 && d.mbounds[a][1] == P.mx[a]) d.mthbc[2 * a + 1] = P.mthbc[2 * a + 1];
+    }
+    T.push_back(d);
+  }
+  RETCHK(bsam_set_hierarchy(c, (int)T.size(), T.data(), c->minlevel, fin));
+  size_t off = 0;
+  for (int n = 0; n < np; n++) {
+    long cells = 1;
+    for (int a = 0; a < nd; a++) cells *= fd[n].mx[a] + 2;
+    const int64_t cnt = (int64_t)cells * nv;
+    RETCHK(bsam_upload(c, base + n, BSAM_FIELD_Q, fq.data() + off, cnt));
+    off += (size_t)cnt;
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  *time = t;
+  *finestlevel = fin;
+  return BSAM_OK;
+}
+
 extern "C" int32_t bsam_get_hierarchy(bsam_ctx* c, int32_t maxout, bsam_patch_desc* out, int32_t* npatch,
                                       int32_t* minlevel, int32_t* finestlevel) {
   if (!c || !npatch) return set_err(BSAM_EINVAL, "bad arguments");

@@ -433,3 +433,184 @@ extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patc
   for (size_t n = 0; n < made.size(); n++) out[n] = made[made.size() - 1 - n];
   return BSAM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// ReadQ (LIBRARY/bsaminputoutput.f90:184-443): a frame as WriteQ writes it
+// (time F25.12; finestlevel I3; per patch of levels 0..finest in the
+// writer's order a blank record, level/ndims/r/nrvars (I3,3(1x,I3)), dx,
+// xlower and xupper (F25.12 with 1x), mx and mglobal (I8 with 1X), then one
+// record per cell of q(0:mx+1, ...), i fastest, nrvars values each).  The
+// reference reads it into a tree whose level lists run from the youngest
+// node, so a level's patches are visited in the reverse of the file's
+// order; it leaves a refined patch's parent (the youngest node one level
+// down), mbounds and mthbc broken, since only TransferValues reads those
+// patches before the next regrid replaces them.  Here the parent is the
+// patch one level down whose box contains the patch's coarse box, mbounds
+// follow from it and mthbc is MakeNewGrid's inheritance, so the table is a
+// valid hierarchy for any step.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct FrameReader {
+  FILE* f = nullptr;
+  std::string line;
+  long lineno = 0;
+  ~FrameReader() {
+    if (f) fclose(f);
+  }
+  bool next() {
+    line.clear();
+    int ch;
+    while ((ch = fgetc(f)) != EOF && ch != '\n') line.push_back((char)ch);
+    if (ch == EOF && line.empty()) return false;
+    lineno++;
+    return true;
+  }
+};
+
+// whitespace-separated numbers of the current line
+bool read_numbers(const std::string& s, int n, double* out) {
+  const char* p = s.c_str();
+  for (int k = 0; k < n; k++) {
+    char* end = nullptr;
+    while (*p == ' ') p++;
+    if (!*p || *p == '*') return false;
+    out[k] = strtod(p, &end);
+    if (end == p) return false;
+    p = end;
+  }
+  return true;
+}
+bool read_ints(const std::string& s, int n, int* out) {
+  double v[16];
+  if (n > 16 || !read_numbers(s, n, v)) return false;
+  for (int k = 0; k < n; k++) {
+    out[k] = (int)v[k];
+    if ((double)out[k] != v[k]) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int32_t bsam_parse_frame(const char* path, int32_t ndims, int32_t nrvars, double* time,
+                                    int32_t* finestlevel, int32_t maxpatch, bsam_patch_desc* desc, int32_t* npatch,
+                                    double* q, int64_t maxq, int64_t* nq) {
+  if (!path || (ndims != 2 && ndims != 3) || nrvars < 1 || nrvars > 10 || !time || !finestlevel || !npatch || !nq)
+    return bsam::fail(BSAM_EINVAL, "bad arguments");
+  FrameReader R;
+  R.f = fopen(path, "rb");
+  if (!R.f) return bsam::fail(BSAM_EINVAL, "ReadQ: cannot open %s", path);
+  auto bad = [&](const char* what) {
+    return bsam::fail(BSAM_EINVAL, "ReadQ: %s at line %ld of %s", what, R.lineno, path);
+  };
+  if (!R.next() || !read_numbers(R.line, 1, time)) return bad("no time");
+  int finest = 0;
+  if (!R.next() || !read_ints(R.line, 1, &finest)) return bad("no finest level");
+  if (finest < 0 || finest >= BSAM_MAX_AMR_LEVELS) return bad("finest level out of range");
+  struct P {
+    bsam_patch_desc d;
+    std::vector<double> q;
+  };
+  std::vector<std::vector<P>> lv(finest + 1);
+  int prev = 0;
+  while (R.next()) {
+    if (R.line.find_first_not_of(' ') != std::string::npos) return bad("a patch does not start with a blank record");
+    int h[4];
+    if (!R.next() || !read_ints(R.line, 4, h)) return bad("bad patch header");
+    const int level = h[0];
+    if (h[1] != ndims) return bad("spatial dimension (ndims) inconsistent");
+    if (h[2] != 2) return bad("refinement ratio other than 2");
+    if (h[3] != nrvars) return bad("number of variables (nrvars) inconsistent");
+    if (level < prev || level > finest) return bad("patch levels out of order");
+    if (level == 0 && !lv[0].empty()) return bad("more than one root patch");
+    prev = level;
+    P p{};
+    bsam_patch_desc& d = p.d;
+    double dx[3], xl[3], xu[3];
+    int mx[3], mg[6];
+    if (!R.next() || !read_numbers(R.line, ndims, dx)) return bad("bad dx");
+    if (!R.next() || !read_numbers(R.line, ndims, xl)) return bad("bad xlower");
+    if (!R.next() || !read_numbers(R.line, ndims, xu)) return bad("bad xupper");
+    if (!R.next() || !read_ints(R.line, ndims, mx)) return bad("bad mx");
+    if (!R.next() || !read_ints(R.line, 2 * ndims, mg)) return bad("bad mglobal");
+    d.level = level;
+    d.parent = -1;
+    d.dx = dx[0];
+    for (int a = 0; a < 3; a++) {
+      const bool used = a < ndims;
+      d.mx[a] = used ? mx[a] : 1;
+      if (used && (mx[a] < 2 || mg[2 * a + 1] - mg[2 * a] + 1 != mx[a])) return bad("mx and mglobal disagree");
+      d.mglobal[a][0] = used ? mg[2 * a] : 1;
+      d.mglobal[a][1] = used ? mg[2 * a + 1] : 1;
+      d.mbounds[a][0] = 1;
+      d.mbounds[a][1] = used ? (level == 0 ? mx[a] : mx[a] / 2) : 1;
+      d.xlower[a] = used ? xl[a] : 0.0;
+    }
+    for (int f2 = 0; f2 < 6; f2++) d.mthbc[f2] = 999;
+    long cells = 1;
+    for (int a = 0; a < ndims; a++) cells *= mx[a] + 2;
+    p.q.resize((size_t)cells * nrvars);
+    double v[10];
+    // ReadQ2D/3D: q(i,j[,k],1:nrvars) record by record, i fastest
+    for (long cidx = 0; cidx < cells; cidx++) {
+      if (!R.next() || !read_numbers(R.line, nrvars, v)) return bad("bad cell record");
+      for (int k = 0; k < nrvars; k++) p.q[(size_t)k * cells + cidx] = v[k];
+    }
+    lv[level].push_back(std::move(p));
+  }
+  if (lv[0].empty()) return bad("no root patch");
+  // the reference's visiting order: the root, then each level's patches
+  // from the youngest (the last in the file)
+  std::vector<P*> order;
+  std::vector<int> first(finest + 2, 0);
+  for (int l = 0; l <= finest; l++) {
+    first[l] = (int)order.size();
+    for (int n = (int)lv[l].size() - 1; n >= 0; n--) order.push_back(&lv[l][n]);
+  }
+  first[finest + 1] = (int)order.size();
+  *finestlevel = finest;
+  *npatch = (int32_t)order.size();
+  long total = 0;
+  for (P* p : order) total += (long)p->q.size();
+  *nq = total;
+  if ((desc && (int32_t)order.size() > maxpatch) || (q && total > maxq))
+    return bsam::fail(BSAM_EINVAL, "ReadQ: %d patches and %ld values do not fit in %d and %ld",
+                      (int)order.size(), total, maxpatch, (long)maxq);
+  // the root keeps its own faces: the caller's root descriptor supplies
+  // them; a child's parent is the patch one level down that holds its
+  // coarse box, and MakeNewGrid's rules give mbounds and mthbc
+  for (int l = 1; l <= finest; l++)
+    for (int n = first[l]; n < first[l + 1]; n++) {
+      bsam_patch_desc& d = order[n]->d;
+      int pick = -1;
+      for (int m = first[l - 1]; m < first[l] && pick < 0; m++) {
+        const bsam_patch_desc& pd = order[m]->d;
+        bool in = true;
+        for (int a = 0; a < ndims && in; a++) {
+          const int c0 = (d.mglobal[a][0] + 1) / 2, c1 = d.mglobal[a][1] / 2;
+          in = (d.mglobal[a][0] % 2 == 1) && (d.mglobal[a][1] % 2 == 0) && c0 >= pd.mglobal[a][0] &&
+               c1 <= pd.mglobal[a][1];
+        }
+        if (in) pick = m;
+      }
+      if (pick < 0) return bsam::fail(BSAM_EINVAL, "ReadQ: a level-%d patch lies in no patch of level %d", l, l - 1);
+      d.parent = pick;
+      const bsam_patch_desc& pd = order[pick]->d;
+      for (int a = 0; a < ndims; a++) {
+        d.mbounds[a][0] = (d.mglobal[a][0] + 1) / 2 - pd.mglobal[a][0] + 1;
+        d.mbounds[a][1] = d.mglobal[a][1] / 2 - pd.mglobal[a][0] + 1;
+      }
+    }
+  for (int n = 0; n < (int)order.size(); n++) {
+    if (desc) desc[n] = order[n]->d;
+  }
+  if (q) {
+    long off = 0;
+    for (P* p : order) {
+      std::copy(p->q.begin(), p->q.end(), q + off);
+      off += (long)p->q.size();
+    }
+  }
+  return BSAM_OK;
+}

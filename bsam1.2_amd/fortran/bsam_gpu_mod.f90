@@ -84,6 +84,7 @@ PUBLIC:: bsam_create, bsam_destroy, bsam_set_dt, bsam_set_hierarchy, &
          bsam_estimate_level_errors, bsam_estimate_level_errors_noalloc, bsam_write_frame, &
          bsam_multigrid_iterations, bsam_time_step, bsam_regrid, bsam_new_level, bsam_amr, &
          bsam_get_hierarchy, bsam_time_step_amr, bsam_grid_params, BSAM_MAX_AMR_LEVELS, bsam_synchronize, &
+         bsam_parse_frame, bsam_read_frame, &
          bsam_autotune, bsam_set_knob, bsam_get_knob, bsam_set_mixed_fallback, &
          bsam_mixed_fallbacks, bsam_last_error
 PUBLIC:: bsam_make_desc, bsam_put, bsam_get, bsam_error_message
@@ -292,6 +293,25 @@ INTERFACE
     INTEGER(C_INT32_T), VALUE:: first_step
     INTEGER(C_INT32_T), INTENT(OUT):: ncycles
     REAL(C_DOUBLE), INTENT(OUT):: residual
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_parse_frame(path, ndims, nrvars, time, finestlevel, maxpatch, desc, &
+      npatch, q, maxq, nq) BIND(C, NAME='bsam_parse_frame')
+    IMPORT:: C_INT32_T, C_INT64_T, C_CHAR, C_DOUBLE, bsam_patch_desc
+    CHARACTER(KIND=C_CHAR), INTENT(IN):: path(*)     ! NUL-terminated
+    INTEGER(C_INT32_T), VALUE:: ndims, nrvars, maxpatch
+    REAL(C_DOUBLE), INTENT(OUT):: time
+    INTEGER(C_INT32_T), INTENT(OUT):: finestlevel, npatch
+    TYPE(bsam_patch_desc), INTENT(OUT):: desc(*)
+    REAL(C_DOUBLE), INTENT(OUT):: q(*)
+    INTEGER(C_INT64_T), VALUE:: maxq
+    INTEGER(C_INT64_T), INTENT(OUT):: nq
+  END FUNCTION
+  INTEGER(C_INT32_T) FUNCTION bsam_read_frame(ctx, path, time, finestlevel) BIND(C, NAME='bsam_read_frame')
+    IMPORT:: C_INT32_T, C_PTR, C_CHAR, C_DOUBLE
+    TYPE(C_PTR), VALUE:: ctx
+    CHARACTER(KIND=C_CHAR), INTENT(IN):: path(*)     ! NUL-terminated
+    REAL(C_DOUBLE), INTENT(OUT):: time
+    INTEGER(C_INT32_T), INTENT(OUT):: finestlevel
   END FUNCTION
   INTEGER(C_INT32_T) FUNCTION bsam_synchronize(ctx) BIND(C, NAME='bsam_synchronize')
     IMPORT:: C_INT32_T, C_PTR

@@ -318,6 +318,32 @@ int32_t bsam_get_hierarchy(bsam_ctx* ctx, int32_t maxout, bsam_patch_desc* out, 
 int32_t bsam_time_step_amr(bsam_ctx* ctx, const bsam_grid_params* gp, int32_t first_step,
                            int32_t* ncycles, double* residual);
 
+/* ReadQ's parse of a restart frame (bsaminputoutput.f90:184-443), host
+ * only: the frame WriteQ (bsam_write_frame) writes, for a run of `ndims`
+ * and `nrvars`.  *time and *finestlevel are the frame's; the patches of
+ * levels 0..finest go to desc[0..*npatch) in the order the reference's tree
+ * visits them after ReadQ (the root, then each level's patches from the
+ * last in the file), with `parent` the index in this table of the patch one
+ * level down that contains the patch's coarse box and mbounds from it (the
+ * reference leaves both broken), mthbc 999 and the root's parent -1; q of
+ * every patch (q(0:mx+1, ...) column-major, variable slowest) goes to q,
+ * patch after patch in that order.  With desc and q NULL only *npatch and
+ * *nq are returned; BSAM_EINVAL when they do not fit maxpatch / maxq, when
+ * the file is not such a frame (the message names the line), or when a
+ * refined patch lies in no patch of the level below. */
+int32_t bsam_parse_frame(const char* path, int32_t ndims, int32_t nrvars, double* time, int32_t* finestlevel,
+                         int32_t maxpatch, bsam_patch_desc* desc, int32_t* npatch, double* q, int64_t maxq,
+                         int64_t* nq);
+/* ReadQ onto the installed run: the frame's root must match the installed
+ * root as the reference checks it (nrvars, dx, xlower within 1e-8, mx,
+ * mglobal); the levels below the root stay; the frame's refined patches
+ * (bsam_parse_frame) replace the levels above it, with MakeNewGrid's
+ * inheritance of physical and periodic faces; q of the root and of every
+ * refined patch is the frame's, every other field zero.  The reference
+ * regrids before its next solve (AMR opens every step), which bsam_amr /
+ * bsam_time_step_amr do with this mesh as the source of TransferValues. */
+int32_t bsam_read_frame(bsam_ctx* ctx, const char* path, double* time, int32_t* finestlevel);
+
 /* Wait for all work queued on the context's stream. */
 int32_t bsam_synchronize(bsam_ctx* ctx);
 /* Message of the last failed call on this thread ("" if none). */

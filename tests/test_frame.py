@@ -161,3 +161,112 @@ def test_gpu_frame_edge_values_vs_oracle(name, tmp_path, gpu_solver_cls):
     S.close()
     got, want = open(p1, "rb").read(), open(p2, "rb").read()
     assert got == want, next(i for i in range(min(len(got), len(want)) + 1) if got[i:i + 1] != want[i:i + 1])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_parse_reference_frame(name, tmp_path):
+    """ReadQ's parse (bsam_parse_frame, host only) of the reference's own
+    frame: the time and finest level of the header; the patches of each
+    level in the reverse of the file's order (the reference's tree visits
+    the youngest node first after ReadQ), with the reference's extents,
+    mglobal and xlower, and parents and mbounds as the reference's regrid
+    made them; q equal to the frame's text read as doubles and within half a
+    unit of the twelfth decimal of the q the reference held."""
+    from bsam_amd.grid import parse_frame
+    meta, z = _load(name)
+    path = tmp_path / "m00001.dat"
+    path.write_bytes(bytes(z["frame"]))
+    nd, nv = meta["ndims"], meta["nrvars"]
+    t, fin, hier, qs = parse_frame(str(path), nd, nv)
+    ref = meta["hierarchy"]
+    assert t == _time(z["frame"]) and fin == max(d["level"] for d in ref)
+    want = []
+    for lv in range(0, fin + 1):
+        want += [k for k in meta["patches"] if ref[k]["level"] == lv][::-1]
+    assert len(hier) == len(want), (len(hier), len(want))
+    at = {k: n for n, k in enumerate(want)}
+    for n, k in enumerate(want):
+        g, w = hier[n], ref[k]
+        for key in ("level", "mx", "mglobal"):
+            assert g[key] == w[key], (n, key, g[key], w[key])
+        assert all(abs(g["xlower"][a] - w["xlower"][a]) < 1e-12 for a in range(nd)), (g["xlower"], w["xlower"])
+        if w["level"] > 0:
+            assert g["parent"] == at[w["parent"]] and g["mbounds"] == w["mbounds"], (n, g, w)
+        qref = z[f"q_p{k}"].reshape(qs[n].shape)
+        # half a unit of the twelfth decimal, plus the rounding of the text
+        # to the nearest double
+        assert np.all(np.abs(qs[n] - qref) <= 5e-13 + 2 * np.spacing(np.abs(qref))), (
+            n, float(np.max(np.abs(qs[n] - qref))))
+    # the values are the text's, read as the nearest doubles
+    lines = bytes(z["frame"]).decode().splitlines()
+    body = [ln for ln in lines if len(ln) == 27 * nv - 2]
+    first = np.array([float(x) for x in body[0].split()])
+    last = np.array([float(x) for x in body[-1].split()])
+    assert np.array_equal(first, qs[0].reshape(nv, -1)[:, 0]), (first, qs[0].reshape(nv, -1)[:, 0])
+    # the file's last patch is the first of its level in the visiting order
+    assert np.array_equal(last, qs[at[meta["patches"][-1]]].reshape(nv, -1)[:, -1])
+
+
+def test_parse_frame_refuses_a_broken_file(tmp_path):
+    from bsam_amd.gpu import BsamError
+    from bsam_amd.grid import parse_frame
+    meta, z = _load(CASES[0])
+    text = bytes(z["frame"]).decode().splitlines()
+    text[3] = text[3].replace("  2", "  3", 1)  # ndims of the root patch
+    path = tmp_path / "m00002.dat"
+    path.write_text("\n".join(text) + "\n")
+    with pytest.raises(BsamError, match="ndims"):
+        parse_frame(str(path), meta["ndims"], meta["nrvars"])
+
+
+def _blocks(frame):
+    """The patch records of a frame text (each from its blank record), and
+    its two header lines."""
+    lines = frame.decode().rstrip("\n").split("\n")
+    head, blocks, cur = lines[:2], [], None
+    for ln in lines[2:]:
+        if ln.strip() == "" and ln != "":
+            if cur is not None:
+                blocks.append("\n".join(cur))
+            cur = []
+        elif cur is not None:
+            cur.append(ln)
+    if cur:
+        blocks.append("\n".join(cur))
+    return head, blocks
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_read_frame_round_trip(name, tmp_path, gpu_solver_cls):
+    """bsam_read_frame (ReadQ) of the reference's own frame onto a run that
+    holds the frame's root: the refined mesh and every patch's q as the
+    parse gives them; written back with bsam_write_frame, the frame has the
+    reference's header and the reference's patch records, the patches of a
+    level in the reverse order (the reference's own order after ReadQ)."""
+    from bsam_amd.grid import parse_frame
+    meta, z = _load(name)
+    nd, nv = meta["ndims"], meta["nrvars"]
+    src = tmp_path / "m00001.dat"
+    src.write_bytes(bytes(z["frame"]))
+    base = [d for d in meta["hierarchy"] if d["level"] <= 0]
+    S = gpu_solver_cls(_params(meta), base)
+    print(f"[gpu] read frame {name}", flush=True)
+    t, fin = S.read_frame(str(src))
+    t2, fin2, fh, fq = parse_frame(str(src), nd, nv)
+    assert (t, fin) == (t2, fin2)
+    nb = len(base) - 1  # the root's row
+    assert len(S.hier) == nb + len(fh)
+    for n, d in enumerate(fh):
+        g = S.hier[nb + n]
+        # the root keeps its installed row (its mbounds are its box one
+        # level down)
+        for key in ("level", "mx", "mglobal") + (("mbounds",) if n else ()):
+            assert g[key] == d[key], (n, key)
+        assert np.array_equal(S.get(nb + n, "q"), fq[n]), n
+    out = tmp_path / "m00002.dat"
+    S.write_frame(str(out), t, _geom(S.hier, nd))
+    h1, b1 = _blocks(bytes(z["frame"]))
+    h2, b2 = _blocks(out.read_bytes())
+    assert h1 == h2 and sorted(b1) == sorted(b2) and b1 != [] and len(b1) == len(b2)
+    S.close()

@@ -341,15 +341,20 @@ def test_bg2d_vs_oracle(name, gpu_solver_cls):
     assert np.array_equal(G.get(root, "q"), O.get(root, "q"))
 
 
+@pytest.mark.parametrize("via", ["put", "frame"])
 @pytest.mark.parametrize("name", ["stock_ch2d_u32", "stock_ch2d_u16_phys", "stock_ch3d_u8_phys",
                                   "stock_ch2d_u32_err2"])
-def test_reference_ch_module_histories(name, gpu_solver_cls):
+def test_reference_ch_module_histories(name, via, gpu_solver_cls, tmp_path):
     """The GPU path against the residual histories of the reference's own
     PROBLEM/CH module (ErrorAFAS errortype 1 and 2), from the same
-    12-decimal restart state, and bit-identical to the oracle's state."""
+    12-decimal restart state, and bit-identical to the oracle's state.
+    via "frame": the GPU run starts from the restart frame the reference
+    itself read (written by tests/golden/refrun.py), through
+    bsam_read_frame (ReadQ)."""
     import json
     import os
-    print(f"[gpu] reference CH module {name}", flush=True)
+    import sys
+    print(f"[gpu] reference CH module {name} via {via}", flush=True)
     p = json.load(open(os.path.join(fu.GOLDEN, "stock_pins.json")))[name]
     nd, n = p["ndims"], p["n"]
     hier = hierarchy.uniform(nd, (n,) * nd, p["minlevel"], p["mthbc"])
@@ -361,7 +366,15 @@ def test_reference_ch_module_histories(name, gpu_solver_cls):
     out = []
     for S in (gpu_solver_cls(configs.params(nd, p["dt"], p["eps"], 1e-9, errortype=p.get("errortype", 1)), hier),
               oracle.Oracle(configs.params(nd, p["dt"], p["eps"], 1e-9, errortype=p.get("errortype", 1)), hier)):
-        S.put(root, "q", q0)
+        if via == "frame" and not isinstance(S, oracle.Oracle):
+            sys.path.insert(0, fu.GOLDEN)
+            import refrun
+            os.makedirs(tmp_path / "OUT", exist_ok=True)
+            path = refrun.write_restart_frame(str(tmp_path), nd, (n,) * nd, 1.0 / n, q0)
+            t, fin = S.read_frame(path)
+            assert t == 0.0 and fin == 0 and np.array_equal(S.get(root, "q"), q0)
+        else:
+            S.put(root, "q", q0)
         S.set_ghost(0, 0)
         S.copy_q_to_qold(0)
         hists = []
