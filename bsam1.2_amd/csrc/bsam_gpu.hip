@@ -3791,9 +3791,19 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   const std::vector<std::vector<int>> cand =
       L.level > 0 ? neighbour_candidates(c, ps, periodic ? offs : std::vector<std::array<int, 3>>{})
                   : std::vector<std::vector<int>>();
+  // the pairs of each patch on the host's threads, joined in patch order
+  auto per_patch = [&](auto&& boxes_of, std::vector<BoxDesc>& out) {
+    std::vector<std::vector<BoxDesc>> part(ps.size());
+    const int nth = (int)std::max<size_t>(1, std::min<size_t>(ps.size() / 64 + 1, host_threads()));
+    run_threads(nth, [&](int t) {
+      for (size_t a = t; a < ps.size(); a += nth) boxes_of(a, part[a]);
+    });
+    for (auto& v : part) out.insert(out.end(), v.begin(), v.end());
+  };
   if (L.level > 0)
-    for (size_t a = 0; a < ps.size(); a++)
-      for (int b : cand[a]) ghost_overlap_boxes(c, ps[a], ps[b], nullptr, kCopy, sib);
+    per_patch([&](size_t a, std::vector<BoxDesc>& v) {
+      for (int b : cand[a]) ghost_overlap_boxes(c, ps[a], ps[b], nullptr, kCopy, v);
+    }, sib);
   bool all_wrap = periodic && ps.size() == 1;
   if (periodic) {
     for (int pid : ps) {
@@ -3821,13 +3831,14 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       }
     }
     if (L.level > 0)
-      for (size_t a = 0; a < ps.size(); a++)
+      per_patch([&](size_t a, std::vector<BoxDesc>& v) {
         for (int b : cand[a])
           for (int pol = -1; pol <= 1; pol += 2)
             for (auto& o : offs) {
               int off[3] = {pol * o[0], pol * o[1], pol * o[2]};
-              ghost_overlap_boxes(c, ps[a], ps[b], off, kCopy, pp);
+              ghost_overlap_boxes(c, ps[a], ps[b], off, kCopy, v);
             }
+      }, pp);
   }
   // physical faces (SetPhysicalBC2D/3D, boundary.f90:642-802)
   bool has_phys = false;
@@ -6305,10 +6316,46 @@ int32_t init_level(bsam_ctx* c, int lvl, OldMesh& o) {
   }
   const std::vector<int>* old = o.patches(lvl);
   if (!old) return BSAM_OK;
+  // the old patches each new patch overlaps, through a grid of bins over
+  // the level's global index space (the patches of a level are disjoint)
+  constexpr int kBin = 16;
+  int nb[3] = {1, 1, 1}, lo0[3] = {1, 1, 1};
+  {
+    int hi0[3] = {1, 1, 1};
+    bool first = true;
+    for (int op : *old)
+      for (int a = 0; a < nd; a++) {
+        lo0[a] = first ? o.desc[op].mglobal[a][0] : std::min(lo0[a], o.desc[op].mglobal[a][0]);
+        hi0[a] = first ? o.desc[op].mglobal[a][1] : std::max(hi0[a], o.desc[op].mglobal[a][1]);
+        if (a == nd - 1) first = false;
+      }
+    for (int a = 0; a < nd; a++) nb[a] = (hi0[a] - lo0[a]) / kBin + 1;
+  }
+  std::vector<std::vector<int>> bins((size_t)nb[0] * nb[1] * nb[2]);
+  auto bin_of = [&](int a, int g) { return std::min(nb[a] - 1, std::max(0, (g - lo0[a]) / kBin)); };
+  for (int op : *old) {
+    int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
+    for (int a = 0; a < nd; a++) b0[a] = bin_of(a, o.desc[op].mglobal[a][0]), b1[a] = bin_of(a, o.desc[op].mglobal[a][1]);
+    for (int z = b0[2]; z <= b1[2]; z++)
+      for (int y = b0[1]; y <= b1[1]; y++)
+        for (int x = b0[0]; x <= b1[0]; x++) bins[((size_t)z * nb[1] + y) * nb[0] + x].push_back(op);
+  }
+  std::vector<int> cand;
   std::vector<XferBox> boxes;
   long ncell = 0;
-  for (int pid : L.patches)
-    for (int op : *old) {
+  for (int pid : L.patches) {
+    cand.clear();
+    int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
+    for (int a = 0; a < nd; a++) b0[a] = bin_of(a, c->desc[pid].mglobal[a][0]), b1[a] = bin_of(a, c->desc[pid].mglobal[a][1]);
+    for (int z = b0[2]; z <= b1[2]; z++)
+      for (int y = b0[1]; y <= b1[1]; y++)
+        for (int x = b0[0]; x <= b1[0]; x++)
+          for (int op : bins[((size_t)z * nb[1] + y) * nb[0] + x]) cand.push_back(op);
+    // the old patches in their table order, once each (TransferValues'
+    // pairs; a cell lies in one old patch, so the order changes no value)
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    for (int op : cand) {
       const bsam_patch_desc& D = c->desc[pid];
       const bsam_patch_desc& S = o.desc[op];
       XferBox B{};
@@ -6333,6 +6380,7 @@ int32_t init_level(bsam_ctx* c, int lvl, OldMesh& o) {
       ncell += (long)B.ext[0] * B.ext[1] * B.ext[2];
       boxes.push_back(B);
     }
+  }
   if (boxes.empty()) return BSAM_OK;
   // the box table of the previous level may still be read: wait for it
   HIPCHK(hipStreamSynchronize(c->s));

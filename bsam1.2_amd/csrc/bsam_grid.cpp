@@ -8,6 +8,7 @@
 // (MakeNewGrid :1663-1735).  No device code: clustering reads one int per
 // cell of the level once per split pass and is a small part of a regrid.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -416,9 +417,17 @@ extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patc
   std::vector<std::vector<Box>> sub(lv.size());
   std::vector<int32_t> rc(lv.size(), BSAM_OK);
   std::vector<std::string> msg(lv.size());
-  for_threads(nth, [&](int t) {
-    for (size_t a = t; a < lv.size(); a += nth)
+  // largest patches first, taken from a shared counter: a few large
+  // parents would otherwise hold up the threads that drew them
+  std::vector<int> big(lv.size());
+  for (size_t a = 0; a < lv.size(); a++) big[a] = (int)a;
+  std::stable_sort(big.begin(), big.end(), [&](int x, int y) { return off[x + 1] - off[x] > off[y + 1] - off[y]; });
+  std::atomic<size_t> next(0);
+  for_threads(nth, [&](int) {
+    for (size_t n; (n = next.fetch_add(1)) < big.size();) {
+      const int a = big[n];
       rc[a] = new_subgrids(T[a], ndims, minimumgridpoints, desiredfillratio, level, sub[a], msg[a]);
+    }
   });
   std::vector<bsam_patch_desc> made;
   for (size_t a = 0; a < lv.size(); a++) {
