@@ -26,6 +26,7 @@
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bsam_gpu.h"
@@ -3117,6 +3118,11 @@ struct bsam_ctx {
   std::string tune_report;  // JSON of the last bsam_autotune
   long chain_cells = 0;     // largest level (cells) where the coarse chain starts
   int regrid_tables = 1;  // flattened ghost tables for the levels a regrid builds
+  // field pools an install released, kept for the next install (a regrid
+  // installs every step): buffer and capacity in bytes; pool_cap holds the
+  // capacity of the pools in use
+  std::vector<std::pair<void*, size_t>> spare;
+  std::unordered_map<void*, size_t> pool_cap;
   // mixed precision: an outer iteration whose residual exceeds mixed_stall
   // times the previous one ends the step in FP64 V-cycles
   double mixed_stall = 0.9;
@@ -3302,6 +3308,54 @@ static void drop_graph(bsam_ctx* c) {
   }
 }
 
+// Field pools: a released pool is kept and handed to the next request it
+// can hold without wasting more than half of itself.  Every release and
+// reuse follows a synchronisation of the context's stream (installs and
+// destruction synchronise first), so no work in flight reads a reused pool.
+static hipError_t pool_alloc(bsam_ctx* c, void** p, size_t bytes) {
+  int best = -1;
+  for (size_t i = 0; i < c->spare.size(); i++) {
+    const size_t cap = c->spare[i].second;
+    if (cap >= bytes && cap <= 2 * bytes + (size_t(1) << 20) && (best < 0 || cap < c->spare[best].second))
+      best = (int)i;
+  }
+  if (best >= 0) {
+    *p = c->spare[best].first;
+    c->pool_cap[*p] = c->spare[best].second;
+    c->spare.erase(c->spare.begin() + best);
+    return hipSuccess;
+  }
+  const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+  if (e == hipSuccess) c->pool_cap[*p] = std::max<size_t>(bytes, 1);
+  return e;
+}
+template <typename T>
+static hipError_t pool_alloc(bsam_ctx* c, T** p, size_t bytes) {
+  void* v = nullptr;
+  const hipError_t e = pool_alloc(c, &v, bytes);
+  *p = static_cast<T*>(v);
+  return e;
+}
+static void pool_release(bsam_ctx* c, void* p) {
+  if (!p) return;
+  auto it = c->pool_cap.find(p);
+  if (it == c->pool_cap.end()) {
+    hipFree(p);
+    return;
+  }
+  c->spare.push_back({p, it->second});
+  c->pool_cap.erase(it);
+  // a bounded reserve: the oldest spare pools go back to the device
+  while (c->spare.size() > 24) {
+    hipFree(c->spare.front().first);
+    c->spare.erase(c->spare.begin());
+  }
+}
+static void pool_drain(bsam_ctx* c) {
+  for (auto& sp : c->spare) hipFree(sp.first);
+  c->spare.clear();
+}
+
 // The device arrays of one level (launch lists, ghost-fill stages, rules
 // and tables); the level is left empty.
 static void free_level(Level& L) {
@@ -3327,10 +3381,15 @@ static void free_hierarchy(bsam_ctx* c) {
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
   for (Level& L : c->lv) free_level(L);
   c->lv.clear();
-  hipFree(c->d_pt); hipFree(c->q); hipFree(c->qold); hipFree(c->f); hipFree(c->rf); hipFree(c->ftmp);
-  hipFree(c->qc);
+  hipFree(c->d_pt);
+  for (double* pool : {c->q, c->qold, c->f, c->rf, c->ftmp, c->qc}) pool_release(c, pool);
   c->d_pt = nullptr; c->q = c->qold = c->f = c->rf = c->ftmp = c->qc = nullptr;
-  if (c->mixed) { bsam::mixed_free(c->mixed); c->mixed = nullptr; }
+  if (c->mixed) {
+    for (float* pool : {c->mixed->e, c->mixed->rhs, c->mixed->coef, c->mixed->ec}) pool_release(c, pool);
+    c->mixed->e = c->mixed->rhs = c->mixed->coef = c->mixed->ec = nullptr;
+    bsam::mixed_free(c->mixed);
+    c->mixed = nullptr;
+  }
   c->have_hier = false;
 }
 
@@ -3338,6 +3397,7 @@ extern "C" int32_t bsam_destroy(bsam_ctx* c) {
   if (!c) return BSAM_OK;
   if (c->s) hipStreamSynchronize(c->s);
   free_hierarchy(c);
+  pool_drain(c);
   hipFree(c->d_partial);
   hipFree(c->d_partial2);
   hipFree(c->d_norm);
@@ -4836,8 +4896,8 @@ struct OldMesh {
   ~OldMesh() {
     hipStreamSynchronize(c->s);
     for (Level& L : lv) free_level(L);
-    hipFree(q);
-    hipFree(qold);
+    pool_release(c, q);
+    pool_release(c, qold);
     hipFree(d_box);
   }
   const std::vector<int>* patches(int l) const {
@@ -4889,11 +4949,11 @@ int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_
   HIPCHK(hipMemcpy(c->d_pt, c->hp.data(), npatch * sizeof(DPatch), hipMemcpyHostToDevice));
   double** pools[5] = {&c->q, &c->qold, &c->f, &c->rf, &c->ftmp};
   for (double** pp : pools) {
-    if (hipMalloc(pp, qo * sizeof(double)) != hipSuccess)
+    if (pool_alloc(c, pp, qo * sizeof(double)) != hipSuccess)
       return set_err(BSAM_ENOMEM, "device pool of %ld doubles", qo);
     HIPCHK(hipMemsetAsync(*pp, 0, qo * sizeof(double), c->s));
   }
-  if (hipMalloc(&c->qc, co * sizeof(double)) != hipSuccess) return set_err(BSAM_ENOMEM, "qc pool");
+  if (pool_alloc(c, &c->qc, co * sizeof(double)) != hipSuccess) return set_err(BSAM_ENOMEM, "qc pool");
   HIPCHK(hipMemsetAsync(c->qc, 0, co * sizeof(double), c->s));
   int maxblocks = 1;
   for (int l = minlevel; l <= finestlevel; l++) {
@@ -6499,8 +6559,8 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
       double *q, *qold;
       ~Prev() {
         hipStreamSynchronize(c->s);
-        hipFree(q);
-        hipFree(qold);
+        pool_release(c, q);
+        pool_release(c, qold);
       }
     } prev{c, c->q, c->qold};
     c->q = c->qold = nullptr;
@@ -6657,11 +6717,11 @@ int32_t mixed_init(bsam_ctx* c) {
   c->mixed = m;
   float** qs[3] = {&m->e, &m->rhs, &m->coef};
   for (float** pp : qs) {
-    if (hipMalloc(pp, c->nq * sizeof(float)) != hipSuccess) return set_err(BSAM_ENOMEM, "FP32 pools");
-    HIPCHK(hipMemset(*pp, 0, c->nq * sizeof(float)));
+    if (pool_alloc(c, pp, c->nq * sizeof(float)) != hipSuccess) return set_err(BSAM_ENOMEM, "FP32 pools");
+    HIPCHK(hipMemsetAsync(*pp, 0, c->nq * sizeof(float), c->s));
   }
-  if (hipMalloc(&m->ec, c->nc * sizeof(float)) != hipSuccess) return set_err(BSAM_ENOMEM, "FP32 qc pool");
-  HIPCHK(hipMemset(m->ec, 0, c->nc * sizeof(float)));
+  if (pool_alloc(c, &m->ec, c->nc * sizeof(float)) != hipSuccess) return set_err(BSAM_ENOMEM, "FP32 qc pool");
+  HIPCHK(hipMemsetAsync(m->ec, 0, c->nc * sizeof(float), c->s));
   return BSAM_OK;
 }
 
