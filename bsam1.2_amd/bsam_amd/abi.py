@@ -42,6 +42,8 @@ KNOB_SWEEP_PLANES_FP32 = 4
 KNOB_RESID_PLANES = 5
 KNOB_CHAIN_CELLS = 6
 KNOB_REGRID_TABLES = 7
+KNOB_GATHER_BLOCK_FP64 = 8
+KNOB_GATHER_BLOCK_FP32 = 9
 
 
 class PatchDesc(C.Structure):

@@ -1850,12 +1850,14 @@ __host__ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog
 // the correction's ghosts `eg` are the ghosts the FAS cycle would hold (see
 // rebase_ghosts); each rule writes the fresh value into q and the
 // difference (old + eg - new) into eg.
+//
+// The workgroup size is the level's tuned gather block (64 to 1024 threads).
 template <typename T, bool REBASE = false>
-__global__ __launch_bounds__(kBlock) void k_ghost_map(const int4* __restrict__ cr, const uint16_t* __restrict__ cpost,
-                                                      int nc, const GRule* __restrict__ ir, int ni, int nv,
-                                                      int homog, T* __restrict__ q, int order,
-                                                      float* __restrict__ eg = nullptr) {
-  const int g = ordered_block(order) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(1024) void k_ghost_map(const int4* __restrict__ cr, const uint16_t* __restrict__ cpost,
+                                                    int nc, const GRule* __restrict__ ir, int ni, int nv,
+                                                    int homog, T* __restrict__ q, int order,
+                                                    float* __restrict__ eg = nullptr) {
+  const int g = ordered_block(order) * (int)blockDim.x + threadIdx.x;
   auto put = [&](int d, T x) {
     if (REBASE) eg[d] = (float)((q[d] + (T)eg[d]) - x);
     q[d] = x;
@@ -3066,6 +3068,7 @@ struct Level {
   // the level kernels and of the ghost gather, planes per workgroup of the
   // marching sweeps (FP64, FP32) and residuals (0: the default rule)
   int order = 0, gorder[2] = {0, 0};  // gather: FP64, FP32
+  int gblock[2] = {0, 0};             // gather workgroup size (0: 256): FP64, FP32
   int kc_sweep[2] = {0, 0}, kc_resid = 0;
   double h = 0.0;
   long qspan_off = 0, qspan_len = 0;  // contiguous q-pool span of the level
@@ -4872,7 +4875,7 @@ struct OldMesh {
   std::vector<bsam_patch_desc> desc;
   std::vector<Level> lv;                 // device side, empty once taken
   std::vector<std::vector<int>> plist;   // patches per level
-  std::vector<std::array<int, 6>> shape;  // launch shapes per level
+  std::vector<std::array<int, 8>> shape;  // launch shapes per level
   int minlevel = 0, finest = -1;
   XferBox* d_box = nullptr;
   size_t box_cap = 0;
@@ -4887,7 +4890,8 @@ struct OldMesh {
     for (int l = minlevel; l <= finest; l++) {
       Level& L = c->L(l);
       plist.push_back(L.patches);
-      shape.push_back({L.order, L.gorder[0], L.gorder[1], L.kc_sweep[0], L.kc_sweep[1], L.kc_resid});
+      shape.push_back({L.order, L.gorder[0], L.gorder[1], L.kc_sweep[0], L.kc_sweep[1], L.kc_resid, L.gblock[0],
+                       L.gblock[1]});
       lv.push_back(std::move(L));
       L = Level();
     }
@@ -4906,12 +4910,13 @@ struct OldMesh {
   // the launch shapes set for a level carry over while the level exists
   void restore_shapes(int l) const {
     if (l < minlevel || l > finest || l < c->minlevel || l > c->finest) return;
-    const std::array<int, 6>& k = shape[l - minlevel];
+    const std::array<int, 8>& k = shape[l - minlevel];
     Level& L = c->L(l);
     L.order = k[0];
     L.gorder[0] = k[1], L.gorder[1] = k[2];
     L.kc_sweep[0] = k[3], L.kc_sweep[1] = k[4];
     L.kc_resid = k[5];
+    L.gblock[0] = k[6], L.gblock[1] = k[7];
   }
 };
 
@@ -4999,6 +5004,7 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
     L.gorder[0] = L.gorder[1] = 0;
     L.kc_sweep[0] = L.kc_sweep[1] = 0;
     L.kc_resid = 0;
+    L.gblock[0] = L.gblock[1] = 0;
   }
   return BSAM_OK;
 }
@@ -5417,16 +5423,17 @@ static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, i
     const int mi = form == kAfterCorrection ? 6 : (form == kLean ? (ipass > 0 ? 2 + ipass : 5) : ipass);
     const DevMap& M = L.gmap[mi];
     // rules reading ghosts from before the call first, then the rest
+    const int pr = sizeof(T) == 8 ? 0 : 1;
+    const int bs = L.gblock[pr] > 0 ? L.gblock[pr] : kBlock;
     if (M.nc1 + M.ni1 > 0) {
-      k_ghost_map<T><<<nblocks(M.nc1 + M.ni1), kBlock, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q,
-                                                                  L.gorder[sizeof(T) == 8 ? 0 : 1]);
+      k_ghost_map<T><<<(M.nc1 + M.ni1 + bs - 1) / bs, bs, 0, c->s>>>(M.cr, M.cpost, M.nc1, M.ir, M.ni1, nv, homog, q,
+                                                                     L.gorder[pr]);
       LAUNCH_CHECK();
     }
     const int nc = M.nc - M.nc1, ni = M.ni - M.ni1;
     if (nc + ni > 0) {
-      k_ghost_map<T><<<nblocks(nc + ni), kBlock, 0, c->s>>>(M.cr + M.nc1, M.cpost ? M.cpost + M.nc1 : nullptr, nc,
-                                                             M.ir + M.ni1, ni, nv, homog, q,
-                                                             L.gorder[sizeof(T) == 8 ? 0 : 1]);
+      k_ghost_map<T><<<(nc + ni + bs - 1) / bs, bs, 0, c->s>>>(M.cr + M.nc1, M.cpost ? M.cpost + M.nc1 : nullptr, nc,
+                                                                M.ir + M.ni1, ni, nv, homog, q, L.gorder[pr]);
       LAUNCH_CHECK();
     }
     return BSAM_OK;
@@ -7280,6 +7287,13 @@ extern "C" int32_t bsam_autotune(bsam_ctx* c) {
       if (rc == BSAM_OK && L.maps_ok && se)
         rc = tune_knob(c, "gather_order_fp32", &L.gorder[1], {0, 1, 16, 64, 256},
                        [&]() { return set_ghost_t<float>(c, level, 0, se, 1); }, rep);
+      // the gather's workgroup size, with the block order just chosen
+      if (rc == BSAM_OK && L.maps_ok)
+        rc = tune_knob(c, "gather_block_fp64", &L.gblock[0], {128, 256, 512, 1024},
+                       [&]() { return set_ghost_t<double>(c, level, 0, sq, 0); }, rep);
+      if (rc == BSAM_OK && L.maps_ok && se)
+        rc = tune_knob(c, "gather_block_fp32", &L.gblock[1], {128, 256, 512, 1024},
+                       [&]() { return set_ghost_t<float>(c, level, 0, se, 1); }, rep);
       if (rc == BSAM_OK && march) {
         std::vector<int> cand;
         for (int k : {4, 8, 16, 32})
@@ -7326,6 +7340,8 @@ static int* level_knob(bsam_ctx* c, int32_t knob, Level& L) {
     case BSAM_KNOB_SWEEP_PLANES_FP64: return &L.kc_sweep[0];
     case BSAM_KNOB_SWEEP_PLANES_FP32: return &L.kc_sweep[1];
     case BSAM_KNOB_RESID_PLANES: return &L.kc_resid;
+    case BSAM_KNOB_GATHER_BLOCK_FP64: return &L.gblock[0];
+    case BSAM_KNOB_GATHER_BLOCK_FP32: return &L.gblock[1];
     default: return nullptr;
   }
 }
@@ -7345,11 +7361,16 @@ extern "C" int32_t bsam_set_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
     c->regrid_tables = value;
     return BSAM_OK;
   }
-  if (knob < 0 || knob > BSAM_KNOB_RESID_PLANES) return set_err(BSAM_EINVAL, "unknown knob %d", knob);
+  if (!(knob >= 0 && knob <= BSAM_KNOB_RESID_PLANES) && knob != BSAM_KNOB_GATHER_BLOCK_FP64 &&
+      knob != BSAM_KNOB_GATHER_BLOCK_FP32)
+    return set_err(BSAM_EINVAL, "unknown knob %d", knob);
   RETCHK(check_level(c, level));
   Level& L = c->L(level);
   if (value < 0 || value > 4096) return set_err(BSAM_EINVAL, "knob value %d out of range", value);
-  if (knob >= BSAM_KNOB_SWEEP_PLANES_FP64 && value > 0) {
+  if ((knob == BSAM_KNOB_GATHER_BLOCK_FP64 || knob == BSAM_KNOB_GATHER_BLOCK_FP32) && value != 0 &&
+      value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
+    return set_err(BSAM_EINVAL, "gather block %d is not 0, 64, 128, 256, 512 or 1024", value);
+  if (knob >= BSAM_KNOB_SWEEP_PLANES_FP64 && knob <= BSAM_KNOB_RESID_PLANES && value > 0) {
     // planes per workgroup: one large 3D patch, at most its z extent
     if (c->p.ndims != 3 || L.patches.size() != 1 || value > c->hp[L.patches[0]].mx[2])
       return set_err(BSAM_EINVAL, "plane count %d does not fit level %d", value, level);
@@ -7382,7 +7403,9 @@ extern "C" int32_t bsam_get_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
     *value = c->regrid_tables;
     return BSAM_OK;
   }
-  if (knob < 0 || knob > BSAM_KNOB_RESID_PLANES) return set_err(BSAM_EINVAL, "unknown knob %d", knob);
+  if (!(knob >= 0 && knob <= BSAM_KNOB_RESID_PLANES) && knob != BSAM_KNOB_GATHER_BLOCK_FP64 &&
+      knob != BSAM_KNOB_GATHER_BLOCK_FP32)
+    return set_err(BSAM_EINVAL, "unknown knob %d", knob);
   RETCHK(check_level(c, level));
   *value = *level_knob(c, knob, c->L(level));
   return BSAM_OK;

@@ -30,7 +30,8 @@ INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_SOURCEFIELD = 1, BSAM_SOLUTIONFIELD
 INTEGER(C_INT32_T), PARAMETER, PUBLIC:: BSAM_KNOB_ORDER = 0, BSAM_KNOB_GATHER_ORDER_FP64 = 1, &
   BSAM_KNOB_GATHER_ORDER_FP32 = 2, BSAM_KNOB_SWEEP_PLANES_FP64 = 3, BSAM_KNOB_SWEEP_PLANES_FP32 = 4, &
   BSAM_KNOB_RESID_PLANES = 5, BSAM_KNOB_CHAIN_CELLS = 6, &
-  BSAM_KNOB_REGRID_TABLES = 7
+  BSAM_KNOB_REGRID_TABLES = 7, &
+  BSAM_KNOB_GATHER_BLOCK_FP64 = 8, BSAM_KNOB_GATHER_BLOCK_FP32 = 9
 !
 ! bsam_patch_desc.  C declares mglobal[3][2] and mbounds[3][2] (dimension
 ! first); in Fortran order that is (1:2, 1:3), the transpose of the

@@ -406,6 +406,9 @@ int32_t bsam_tuning_report(const bsam_ctx* ctx, char* buf, int32_t len);
                                            fill (interpolation rules, then box stages), which
                                            is planned in a fraction of the time, for meshes
                                            regridded every step */
+#define BSAM_KNOB_GATHER_BLOCK_FP64  8  /* workgroup size of the level's FP64 ghost gather:
+                                           0 (256), 64, 128, 256, 512 or 1024 */
+#define BSAM_KNOB_GATHER_BLOCK_FP32  9  /* the same for the FP32 gather */
 int32_t bsam_set_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t value);
 int32_t bsam_get_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t* value);
 

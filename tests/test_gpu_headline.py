@@ -53,6 +53,7 @@ def test_two_refined_levels_modes(name, mode, gpu_solver_cls):
         for lv in range(S.minlevel, S.finestlevel + 1):
             S.set_knob(abi.KNOB_ORDER, lv, 1)
             S.set_knob(abi.KNOB_GATHER_ORDER_FP64, lv, 256)
+            S.set_knob(abi.KNOB_GATHER_BLOCK_FP64, lv, 1024 if lv % 2 else 128)
             assert S.get_knob(abi.KNOB_ORDER, lv) == 1
         S.set_knob(abi.KNOB_CHAIN_CELLS, 0, 0)
     rc, n, r, hist = S.multigrid_iterations()
