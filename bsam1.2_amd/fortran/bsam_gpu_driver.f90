@@ -1,6 +1,7 @@
-! A Fortran host of libbsam_gpu.so: the uniform-mesh time loop of BSAM 1.2
-! (LIBRARY/bsamroutines.f90:673-758 with maxlevel = 0) for Cahn-Hilliard on
-! a periodic root grid, driving the library through the BSAMGPU module.
+! A Fortran host of libbsam_gpu.so: the time loop of BSAM 1.2
+! (LIBRARY/bsamroutines.f90:673-758) for Cahn-Hilliard on a periodic root
+! grid, on the uniform mesh (maxlevel = 0) or with the reference's regrid
+! every step, driving the library through the BSAMGPU module.
 !
 ! It builds the root and the below-root multigrid levels as
 ! bsamroutines.f90:443-510 does (mx halved, dx doubled, mbounds covering the
@@ -12,7 +13,13 @@
 ! the root's q is written at the end as raw little-endian doubles in the
 ! reference's order, q(0:mx+1, 0:my+1[, 0:mz+1], 1:2).
 !
-! Usage: bsam_gpu_driver ndims n steps dt eps tol seed base amp precision out.bin
+! Usage: bsam_gpu_driver ndims n steps dt eps tol seed base amp precision out.bin [maxlevel tagtol]
+!
+! With maxlevel > 0 every step is the reference's time step with its regrid
+! (bsam_time_step_amr: AMR(rootlevel) with the gradient tagging at tagtol,
+! ibuffer 1, minimumgridpoints 8, fill ratio 0.8, then CopyQToQold and
+! MultigridIterations), and each step's line also reports the mesh the
+! library built (bsam_get_hierarchy): its patch count and finest level.
 PROGRAM bsam_gpu_driver
 USE, INTRINSIC:: ISO_C_BINDING
 USE BSAMGPU
@@ -28,11 +35,15 @@ INTEGER(C_INT32_T):: rc, ncyc
 REAL(C_DOUBLE):: res
 REAL(KIND=r8), ALLOCATABLE:: q(:)
 INTEGER(C_INT64_T):: nq
-INTEGER:: mx(3), mxc(3), mg(3,2), mb(3,2), mth(6), kmax
-REAL(KIND=r8):: h
+INTEGER:: mx(3), mxc(3), mg(3,2), mb(3,2), mth(6), kmax, maxlevel
+REAL(KIND=r8):: h, tagtol
+TYPE(bsam_grid_params):: grid
+TYPE(bsam_patch_desc), ALLOCATABLE:: mesh(:)
+INTEGER(C_INT32_T):: npatch, lo, fin
+INTEGER(KIND=i8):: cells
 !
 IF(COMMAND_ARGUMENT_COUNT() < 11) THEN
-  PRINT *, 'usage: bsam_gpu_driver ndims n steps dt eps tol seed base amp precision out.bin'
+  PRINT *, 'usage: bsam_gpu_driver ndims n steps dt eps tol seed base amp precision out.bin [maxlevel tagtol]'
   STOP 1
 END IF
 CALL GET_COMMAND_ARGUMENT(1, arg); READ(arg,*) ndims
@@ -46,6 +57,11 @@ CALL GET_COMMAND_ARGUMENT(8, arg); READ(arg,*) base
 CALL GET_COMMAND_ARGUMENT(9, arg); READ(arg,*) amp
 CALL GET_COMMAND_ARGUMENT(10, arg); READ(arg,*) prec
 CALL GET_COMMAND_ARGUMENT(11, outfile)
+maxlevel = 0; tagtol = 0.0_r8
+IF(COMMAND_ARGUMENT_COUNT() >= 13) THEN
+  CALL GET_COMMAND_ARGUMENT(12, arg); READ(arg,*) maxlevel
+  CALL GET_COMMAND_ARGUMENT(13, arg); READ(arg,*) tagtol
+END IF
 !
 ! levels 0 .. minlevel, coarsened to 2 cells per side
 nlev = 0
@@ -103,10 +119,29 @@ CALL check('bsam_copy_q_to_qold')
 rc = bsam_autotune(ctx)
 CALL check('bsam_autotune')
 !
+IF(maxlevel > 0) THEN
+  grid%maxlevel = maxlevel
+  grid%errflagopt = 11; grid%ibuffer = 1; grid%minimumgridpoints = 8
+  grid%qtolerance = tagtol; grid%desiredfillratios = 0.8_r8
+  ALLOCATE(mesh(100000))
+END IF
 DO s = 1, steps
-  rc = bsam_time_step(ctx, MERGE(1,0,s==1), ncyc, res)
-  CALL check('bsam_time_step')
-  PRINT '(A,I4,A,I4,A,ES24.16)', ' step', s, ' vcycles', ncyc, ' residual', res
+  IF(maxlevel > 0) THEN
+    rc = bsam_time_step_amr(ctx, grid, MERGE(1,0,s==1), ncyc, res)
+    CALL check('bsam_time_step_amr')
+    rc = bsam_get_hierarchy(ctx, SIZE(mesh), mesh, npatch, lo, fin)
+    CALL check('bsam_get_hierarchy')
+    cells = 0
+    DO i = 1, npatch
+      IF(mesh(i)%level == fin) cells = cells+PRODUCT(INT(mesh(i)%mx(1:ndims), i8))
+    END DO
+    PRINT '(A,I4,A,I4,A,ES24.16,A,I7,A,I3,A,I10)', ' step', s, ' vcycles', ncyc, ' residual', res, &
+      ' patches', npatch, ' finest', fin, ' finest-cells', cells
+  ELSE
+    rc = bsam_time_step(ctx, MERGE(1,0,s==1), ncyc, res)
+    CALL check('bsam_time_step')
+    PRINT '(A,I4,A,I4,A,ES24.16)', ' step', s, ' vcycles', ncyc, ' residual', res
+  END IF
 END DO
 !
 rc = bsam_get(ctx, nlev-1, BSAM_FIELD_Q, q, nq)
