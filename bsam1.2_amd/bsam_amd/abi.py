@@ -33,6 +33,14 @@ KSTAT_RELAX_FINEST = 0
 KSTAT_RESID_FINEST = 1
 KSTAT_FASRHS_FINEST = 2
 KSTAT_COUNT = 3
+PHASE_AMR_TAG = 0
+PHASE_AMR_CLUSTER = 1
+PHASE_AMR_INSTALL = 2
+PHASE_AMR_FIELDS = 3
+PHASE_SOLVE = 4
+PHASE_GRAPH = 5
+PHASE_COUNT = 6
+PHASE_NAMES = ("amr_tag", "amr_cluster", "amr_install", "amr_fields", "solve", "graph")
 
 KNOB_ORDER = 0
 KNOB_GATHER_ORDER_FP64 = 1
@@ -157,6 +165,8 @@ ENTRY_POINTS = {
     "bsam_last_error": ([], C.c_char_p),
     "bsam_kstat_enable": ([_P, _I], _I),
     "bsam_kstat_read": ([_P, _I, _LP, _DP, _DP], _I),
+    "bsam_phase_enable": ([_P, _I], _I),
+    "bsam_phase_read": ([_P, _I, _LP, _DP], _I),
     "bsam_autotune": ([_P], _I),
     "bsam_tuning_report": ([_P, C.c_char_p, _I], _I),
     "bsam_set_knob": ([_P, _I, _I, _I], _I),

@@ -277,6 +277,19 @@ class Solver:
     def kstat_enable(self, on=True):
         self._chk(self.lib.bsam_kstat_enable(self.ctx, 1 if on else 0))
 
+    def phase_enable(self, on=True):
+        self._chk(self.lib.bsam_phase_enable(self.ctx, 1 if on else 0))
+
+    def phase_read(self):
+        """{phase name: (calls, ms)} of the wall-clock phase timers."""
+        out = {}
+        for i, name in enumerate(abi.PHASE_NAMES):
+            n = C.c_int64()
+            ms = C.c_double()
+            self._chk(self.lib.bsam_phase_read(self.ctx, i, C.byref(n), C.byref(ms)))
+            out[name] = (n.value, ms.value)
+        return out
+
     def kstat_read(self, which):
         n = C.c_int64()
         ms = C.c_double()

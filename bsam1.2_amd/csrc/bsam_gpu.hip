@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <array>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1323,6 +1324,74 @@ __global__ __launch_bounds__(kBlock) void k_user_flags(LevelArgs L, const double
 __global__ __launch_bounds__(kBlock) void k_zero_i32(int32_t* __restrict__ a, long n) {
   const long g = (long)blockIdx.x * kBlock + threadIdx.x;
   if (g < n) a[g] = 0;
+}
+
+// Tag buffering (BufferAndList and InflateEdgeTags, bsamroutines.f90:
+// 1206-1349) as a dilation over the level's whole index space: every tag
+// (flag 1) sets its bit in a bit map of the level's global cells
+// (k_tag_bits), then a cell of a level patch is tagged when a bit lies
+// within ibuffer cells in every dimension, with the root's periodic
+// dimensions wrapped (k_tag_dilate).  The reference buffers a tag inside
+// its patch when its buffer box fits there and otherwise applies the box,
+// and its periodic images, to every patch of the level: the same union of
+// boxes, cut to each patch.  glo: each level patch's global lower corner
+// (0-based); ext: the level's extents in cells.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_tag_bits(LevelArgs L, const long* __restrict__ foff,
+                                                      const int* __restrict__ glo, const int32_t* __restrict__ flags,
+                                                      uint32_t* __restrict__ bits, int ex, int ey) {
+  int pi, local;
+  if (!level_elem(L, level_block(L), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  int i, j, k;
+  if (!cell_of<DIM>(P, local, i, j, k)) return;
+  const int kk = (DIM == 3) ? k - 1 : 0;
+  if (flags[foff[pi] + ((long)kk * P.mx[1] + (j - 1)) * P.mx[0] + (i - 1)] != 1) return;
+  const long b = ((long)(glo[3 * pi + 2] + kk) * ey + (glo[3 * pi + 1] + j - 1)) * ex + (glo[3 * pi] + i - 1);
+  atomicOr(&bits[b >> 5], 1u << (b & 31));
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_tag_dilate(LevelArgs L, const long* __restrict__ foff,
+                                                        const int* __restrict__ glo, const uint32_t* __restrict__ bits,
+                                                        int ex, int ey, int ez, int wrapx, int wrapy, int wrapz, int ib,
+                                                        int32_t* __restrict__ out) {
+  int pi, local;
+  if (!level_elem(L, level_block(L), pi, local)) return;
+  const DPatch P = L.pt[L.patches[pi]];
+  int i, j, k;
+  if (!cell_of<DIM>(P, local, i, j, k)) return;
+  const int kk = (DIM == 3) ? k - 1 : 0;
+  const int gi = glo[3 * pi] + i - 1, gj = glo[3 * pi + 1] + j - 1, gk = glo[3 * pi + 2] + kk;
+  const int zb = (DIM == 3) ? ib : 0;
+  bool hit = false;
+  for (int dz = -zb; dz <= zb && !hit; dz++) {
+    int z = gk + dz;
+    if (z < 0 || z >= ez) {
+      if (!wrapz) continue;
+      z += z < 0 ? ez : -ez;
+    }
+    for (int dy = -ib; dy <= ib && !hit; dy++) {
+      int y = gj + dy;
+      if (y < 0 || y >= ey) {
+        if (!wrapy) continue;
+        y += y < 0 ? ey : -ey;
+      }
+      for (int dx = -ib; dx <= ib; dx++) {
+        int x = gi + dx;
+        if (x < 0 || x >= ex) {
+          if (!wrapx) continue;
+          x += x < 0 ? ex : -ex;
+        }
+        const long b = ((long)z * ey + y) * ex + x;
+        if ((bits[b >> 5] >> (b & 31)) & 1u) {
+          hit = true;
+          break;
+        }
+      }
+    }
+  }
+  out[foff[pi] + ((long)kk * P.mx[1] + (j - 1)) * P.mx[0] + (i - 1)] = hit ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -3111,6 +3180,12 @@ struct bsam_ctx {
   // regrid tagging: per-patch flag offsets, then the level's flags
   char* d_tag = nullptr;
   size_t tag_cap = 0;
+  // AMR's tag buffering on the device: bit map of a level's index space
+  // (capacity in words) and the level patches' global lower corners
+  uint32_t* d_tagbits = nullptr;
+  size_t tagbits_cap = 0;
+  int* d_tagglo = nullptr;
+  size_t tagglo_cap = 0;
   // this device's copy of the file-scope `scratch` array (its device
   // address, looked up once in bsam_create with the device current)
   char* scratch = nullptr;
@@ -3132,6 +3207,9 @@ struct bsam_ctx {
   int64_t mixed_fallbacks = 0;  // MultigridIterations calls that fell back
   // instrumentation
   KStat ks[BSAM_KSTAT_COUNT];
+  bool phase_on = false;
+  int64_t phase_calls[BSAM_PHASE_COUNT] = {};
+  double phase_ms[BSAM_PHASE_COUNT] = {};
   // mixed precision state
   bsam::MixedState* mixed = nullptr;
 
@@ -3243,6 +3321,34 @@ void kstat_end(bsam_ctx* c, int which, double bytes) {
   k.launches++;
   k.bytes = bytes;
 }
+
+// Wall clock of one phase of a time step (bsam_phase_enable): the stream is
+// drained on entry and exit, so the phase holds the GPU work it queued.
+struct PhaseClock {
+  bsam_ctx* c;
+  int which;
+  std::chrono::steady_clock::time_point t0;
+  PhaseClock(bsam_ctx* c_, int w) : c(c_), which(w) { start(); }
+  ~PhaseClock() { stop(); }
+  void start() {
+    if (which < 0 || !c->phase_on || c->capturing) return;
+    hipStreamSynchronize(c->s);
+    t0 = std::chrono::steady_clock::now();
+  }
+  void stop() {
+    if (which < 0 || !c->phase_on || c->capturing) return;
+    hipStreamSynchronize(c->s);
+    c->phase_ms[which] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->phase_calls[which]++;
+    which = -1;
+  }
+  // end this phase and start phase w
+  void next(int w) {
+    stop();
+    which = w;
+    start();
+  }
+};
 
 }  // namespace
 
@@ -3406,6 +3512,8 @@ extern "C" int32_t bsam_destroy(bsam_ctx* c) {
   hipFree(c->d_norm);
   hipFree(c->d_stage);
   hipFree(c->d_tag);
+  hipFree(c->d_tagbits);
+  hipFree(c->d_tagglo);
   if (c->h_norm) hipHostFree(c->h_norm);
   for (auto& k : c->ks)
     for (hipEvent_t e : k.ev) hipEventDestroy(e);
@@ -6107,11 +6215,108 @@ static int32_t estimate_level_errors_in(bsam_ctx* c, int32_t level, int32_t errf
       LAUNCH_CHECK();
     }
   }
+  // without `flags` the level's flags stay on the device
+  if (!flags) return BSAM_OK;
   if (n > 0) HIPCHK(hipMemcpyAsync(flags, d_flags, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   int64_t nt = 0;
   for (long e = 0; e < n; e++) nt += flags[e] != 0;
   *ntagged = nt;
+  return BSAM_OK;
+}
+
+// The tags AMR clusters on one level: EstimateLevelErrors into device
+// flags, buffered by ibuffer on the device where the level's bit map fits
+// (k_tag_bits, k_tag_dilate: at most 2^31 cells, ibuffer below every
+// extent), then copied to `flags` (L.ncell entries).  *buffered tells the
+// caller whether the buffering is done (else bsam_new_level buffers).
+static int32_t amr_level_tags(bsam_ctx* c, int level, int errflagopt, double qtolerance, int ibuffer,
+                              std::vector<int32_t>& flags, bool& buffered) {
+  Level& L = c->L(level);
+  const int nd = c->p.ndims;
+  const long n = L.ncell;
+  flags.resize(std::max<long>(1, n));
+  buffered = false;
+  const bsam_patch_desc* root = nullptr;
+  for (const bsam_patch_desc& d : c->desc)
+    if (d.level == 0) { root = &d; break; }
+  long ext[3] = {1, 1, 1};
+  bool fits = ibuffer > 0 && root && level >= 0;
+  for (int a = 0; a < nd && fits; a++) {
+    ext[a] = (long)root->mx[a] << level;
+    fits = ibuffer < ext[a];
+  }
+  const long nbits = ext[0] * ext[1] * ext[2];
+  fits = fits && nbits <= (1L << 31);
+  for (size_t a = 0; a < L.patches.size() && fits; a++)
+    for (int d = 0; d < nd; d++) {
+      const bsam_patch_desc& D = c->desc[L.patches[a]];
+      fits = fits && D.mglobal[d][0] >= 1 && D.mglobal[d][1] <= ext[d] &&
+             D.mglobal[d][1] - D.mglobal[d][0] + 1 == D.mx[d];
+    }
+  const size_t words = fits ? (size_t)((nbits + 31) / 32) : 0;
+  const size_t need = (size_t)n * sizeof(int32_t) * (fits ? 2 : 1);
+  if (need > c->tag_cap) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    hipFree(c->d_tag);
+    c->d_tag = nullptr;
+    c->tag_cap = 0;
+    HIPCHK(hipMalloc(&c->d_tag, need));
+    c->tag_cap = need;
+  }
+  int32_t* d_flags = reinterpret_cast<int32_t*>(c->d_tag);
+  RETCHK(estimate_level_errors_in(c, level, errflagopt, qtolerance, nullptr, nullptr, d_flags));
+  if (n == 0) return BSAM_OK;
+  if (fits) {
+    if (words > c->tagbits_cap) {
+      HIPCHK(hipStreamSynchronize(c->s));
+      hipFree(c->d_tagbits);
+      c->d_tagbits = nullptr;
+      c->tagbits_cap = 0;
+      HIPCHK(hipMalloc(&c->d_tagbits, words * sizeof(uint32_t)));
+      c->tagbits_cap = words;
+    }
+    if (L.patches.size() * 3 > c->tagglo_cap) {
+      HIPCHK(hipStreamSynchronize(c->s));
+      hipFree(c->d_tagglo);
+      c->d_tagglo = nullptr;
+      c->tagglo_cap = 0;
+      HIPCHK(hipMalloc(&c->d_tagglo, L.patches.size() * 3 * sizeof(int)));
+      c->tagglo_cap = L.patches.size() * 3;
+    }
+    std::vector<int> glo(L.patches.size() * 3, 0);
+    for (size_t a = 0; a < L.patches.size(); a++)
+      for (int d = 0; d < nd; d++) glo[3 * a + d] = c->desc[L.patches[a]].mglobal[d][0] - 1;
+    HIPCHK(hipMemcpyAsync(c->d_tagglo, glo.data(), glo.size() * sizeof(int), hipMemcpyHostToDevice, c->s));
+    k_zero_i32<<<nblocks((long)words), kBlock, 0, c->s>>>(reinterpret_cast<int32_t*>(c->d_tagbits), (long)words);
+    LAUNCH_CHECK();
+    int32_t* d_out = d_flags + n;
+    const int wrap[3] = {root->mthbc[0] == 2, nd > 1 && root->mthbc[2] == 2, nd > 2 && root->mthbc[4] == 2};
+    for (int pass = 0; pass < 2; pass++) {
+      FOR_LAUNCHES(c, L, kCell, a, nb) {
+        if (pass == 0) {
+          DISPATCH_DIM(nd,
+                       (k_tag_bits<2><<<nb, kBlock, 0, c->s>>>(a, L.d_foff, c->d_tagglo, d_flags, c->d_tagbits,
+                                                               (int)ext[0], (int)ext[1])),
+                       (k_tag_bits<3><<<nb, kBlock, 0, c->s>>>(a, L.d_foff, c->d_tagglo, d_flags, c->d_tagbits,
+                                                               (int)ext[0], (int)ext[1])));
+        } else {
+          DISPATCH_DIM(nd,
+                       (k_tag_dilate<2><<<nb, kBlock, 0, c->s>>>(a, L.d_foff, c->d_tagglo, c->d_tagbits, (int)ext[0],
+                                                                 (int)ext[1], (int)ext[2], wrap[0], wrap[1], wrap[2],
+                                                                 ibuffer, d_out)),
+                       (k_tag_dilate<3><<<nb, kBlock, 0, c->s>>>(a, L.d_foff, c->d_tagglo, c->d_tagbits, (int)ext[0],
+                                                                 (int)ext[1], (int)ext[2], wrap[0], wrap[1], wrap[2],
+                                                                 ibuffer, d_out)));
+        }
+        LAUNCH_CHECK();
+      }
+    }
+    d_flags = d_out;
+    buffered = true;
+  }
+  HIPCHK(hipMemcpyAsync(flags.data(), d_flags, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
   return BSAM_OK;
 }
 
@@ -6274,6 +6479,7 @@ static bool kstat_any(bsam_ctx* c) {
 static int32_t run_iteration(bsam_ctx* c, int32_t (*body)(bsam_ctx*), bool eager = false) {
   if (c->p.use_graph && c->p.batching && !kstat_any(c) && !eager) {
     if (!c->gexec) {
+      PhaseClock pc(c, BSAM_PHASE_GRAPH);
       HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
       c->capturing = true;
       int32_t r = body(c);
@@ -6336,6 +6542,7 @@ extern "C" int32_t bsam_time_step(bsam_ctx* c, int32_t first_step, int32_t* ncyc
   // nothing the step reads (RestrictSolution rewrites the values the last
   // FillDown left, and qold, qc are rewritten below and in the V-cycle)
   for (int level = 0; level <= c->finest; level++) RETCHK(bsam_set_ghost(c, level, 0));
+  PhaseClock pc(c, BSAM_PHASE_SOLVE);
   if (first_step) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
   for (int level = c->minlevel; level <= c->finest; level++) RETCHK(bsam_copy_q_to_qold(c, level));
   return bsam_multigrid_iterations(c, ncycles, residual, nullptr, 0);
@@ -6525,6 +6732,7 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
   }
   const int minlevel = c->minlevel, nd = c->p.ndims;
   HIPCHK(hipStreamSynchronize(c->s));
+  PhaseClock pc(c, BSAM_PHASE_AMR_INSTALL);
   OldMesh old(c);
   // the new table starts with the old levels at and below the root
   std::vector<bsam_patch_desc> H;
@@ -6542,21 +6750,29 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
   std::vector<int32_t> flags;
   std::vector<bsam_patch_desc> nw(64);
   for (int lvl = 0;; lvl++) {
+    pc.next(BSAM_PHASE_AMR_TAG);
     RETCHK(bsam_set_ghost(c, lvl, 0));
     if (lvl >= gp->maxlevel) break;
-    flags.resize(std::max<long>(1, c->L(lvl).ncell));
-    int64_t ntag = 0;
-    RETCHK(bsam_estimate_level_errors(c, lvl, gp->errflagopt[lvl], gp->qtolerance[lvl], flags.data(), &ntag));
+    {
+      const int32_t f0 = 0;
+      const int64_t n0 = 0;
+      RETCHK(check_tag_args(c, lvl, gp->errflagopt[lvl], &f0, &n0));
+    }
+    bool buffered = false;
+    RETCHK(amr_level_tags(c, lvl, gp->errflagopt[lvl], gp->qtolerance[lvl], gp->ibuffer[lvl], flags, buffered));
+    pc.next(BSAM_PHASE_AMR_CLUSTER);
+    const int ib = buffered ? 0 : gp->ibuffer[lvl];
     int32_t nn = 0;
-    int32_t r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), gp->ibuffer[lvl],
-                               gp->minimumgridpoints[lvl], gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
+    int32_t r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), ib, gp->minimumgridpoints[lvl],
+                               gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
     if (r == BSAM_EINVAL && nn > (int)nw.size()) {
       nw.resize(nn);
-      r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), gp->ibuffer[lvl],
-                         gp->minimumgridpoints[lvl], gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
+      r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), ib, gp->minimumgridpoints[lvl],
+                         gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
     }
     RETCHK(r);
     if (nn == 0) break;
+    pc.next(BSAM_PHASE_AMR_INSTALL);
     H.insert(H.end(), nw.begin(), nw.begin() + nn);
     // the levels so far are the table's first nkeep rows: their fields are
     // the whole of the current pools, the start of the next ones
@@ -6574,6 +6790,7 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
     int kept = minlevel - 1;
     RETCHK(install(c, (int)H.size(), H.data(), minlevel, lvl + 1, &kept, &old, c->regrid_tables != 0));
     if (kept != lvl) return set_err(BSAM_EINVAL, "AMR: levels up to %d were not kept (%d)", lvl, kept);
+    pc.next(BSAM_PHASE_AMR_FIELDS);
     HIPCHK(hipMemcpyAsync(c->q, prev.q, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     HIPCHK(hipMemcpyAsync(c->qold, prev.qold, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     old.restore_shapes(lvl + 1);
@@ -6671,6 +6888,7 @@ extern "C" int32_t bsam_get_hierarchy(bsam_ctx* c, int32_t maxout, bsam_patch_de
 extern "C" int32_t bsam_time_step_amr(bsam_ctx* c, const bsam_grid_params* gp, int32_t first_step,
                                       int32_t* ncycles, double* residual) {
   RETCHK(bsam_amr(c, gp));
+  PhaseClock pc(c, BSAM_PHASE_SOLVE);
   if (first_step) RETCHK(bsam_fill_down(c, BSAM_SOLUTIONFIELD));
   for (int level = c->minlevel; level <= c->finest; level++) RETCHK(bsam_copy_q_to_qold(c, level));
   return bsam_multigrid_iterations(c, ncycles, residual, nullptr, 0);
@@ -6695,6 +6913,21 @@ extern "C" int32_t bsam_kstat_enable(bsam_ctx* c, int32_t on) {
     k.ms = 0.0;
     k.on = on != 0;
   }
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_phase_enable(bsam_ctx* c, int32_t on) {
+  if (!c) return set_err(BSAM_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->s));
+  c->phase_on = on != 0;
+  for (int i = 0; i < BSAM_PHASE_COUNT; i++) c->phase_calls[i] = 0, c->phase_ms[i] = 0.0;
+  return BSAM_OK;
+}
+
+extern "C" int32_t bsam_phase_read(bsam_ctx* c, int32_t which, int64_t* calls, double* total_ms) {
+  if (!c || which < 0 || which >= BSAM_PHASE_COUNT) return set_err(BSAM_EINVAL, "bad phase index");
+  if (calls) *calls = c->phase_calls[which];
+  if (total_ms) *total_ms = c->phase_ms[which];
   return BSAM_OK;
 }
 

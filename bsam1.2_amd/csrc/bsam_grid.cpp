@@ -69,38 +69,48 @@ struct Tags {
   int32_t& at(int i, int j, int k) const { return f[(i - 1) + (long)mx[0] * ((j - 1) + (long)mx[1] * (k - 1))]; }
 };
 
-// GetSignatures: sig[n][i] = the sum of the tags on the slab i of box b
-// across dimension n (entries outside b are left alone).
-void signatures(const Tags& T, const Box& b, int nd, std::vector<int> sig[3]) {
-  for (int n = 0; n < nd; n++)
-    for (int i = b.lo[n]; i <= b.hi[n]; i++) sig[n][i] = 0;
-  for (int k = b.lo[2]; k <= b.hi[2]; k++)
-    for (int j = b.lo[1]; j <= b.hi[1]; j++)
-      for (int i = b.lo[0]; i <= b.hi[0]; i++) {
-        const int v = T.at(i, j, k);
-        sig[0][i] += v;
-        sig[1][j] += v;
-        if (nd == 3) sig[2][k] += v;
+// Summed tags of one patch: at(i, j, k) = the tags of [1, i] x [1, j] x
+// [1, k] (0 where an index is 0), so the tags of any box are eight lookups.
+struct TagSums {
+  int sx = 0, sy = 0;
+  std::vector<int> s;
+  void build(const Tags& T) {
+    sx = T.mx[0] + 1;
+    sy = T.mx[1] + 1;
+    s.assign((size_t)sx * sy * (T.mx[2] + 1), 0);
+    for (int k = 1; k <= T.mx[2]; k++)
+      for (int j = 1; j <= T.mx[1]; j++) {
+        int row = 0;
+        for (int i = 1; i <= T.mx[0]; i++) {
+          row += T.at(i, j, k);
+          at(i, j, k) = row + at(i, j - 1, k) + at(i, j, k - 1) - at(i, j - 1, k - 1);
+        }
       }
-}
+  }
+  int& at(int i, int j, int k) { return s[((size_t)k * sy + j) * sx + i]; }
+  int at(int i, int j, int k) const { return s[((size_t)k * sy + j) * sx + i]; }
+  // tags of the box [i0, i1] x [j0, j1] x [k0, k1]
+  int box(int i0, int i1, int j0, int j1, int k0, int k1) const {
+    i0--, j0--, k0--;
+    return at(i1, j1, k1) - at(i0, j1, k1) - at(i1, j0, k1) - at(i1, j1, k0) + at(i0, j0, k1) + at(i0, j1, k0) +
+           at(i1, j0, k0) - at(i0, j0, k0);
+  }
+};
 
-// GridFlagRatio: tagged cells over cells of box b.
-double fill_ratio(const Tags& T, const Box& b, int nd) {
-  double total = 1.0;
-  long tot = 1;
-  for (int n = 0; n < nd; n++) tot *= (long)(b.hi[n] - b.lo[n] + 1);
-  total = (double)tot;
-  long flagged = 0;
-  for (int k = b.lo[2]; k <= b.hi[2]; k++)
-    for (int j = b.lo[1]; j <= b.hi[1]; j++)
-      for (int i = b.lo[0]; i <= b.hi[0]; i++) flagged += T.at(i, j, k);
-  return (double)flagged / total;
+// GetSignatures: sig[n][i] = the sum of the tags on the slab i of box b
+// across dimension n (entries outside b are left alone), each slab from the
+// summed tags.
+void signatures(const TagSums& S, const Box& b, int nd, std::vector<int> sig[3]) {
+  for (int i = b.lo[0]; i <= b.hi[0]; i++) sig[0][i] = S.box(i, i, b.lo[1], b.hi[1], b.lo[2], b.hi[2]);
+  for (int j = b.lo[1]; j <= b.hi[1]; j++) sig[1][j] = S.box(b.lo[0], b.hi[0], j, j, b.lo[2], b.hi[2]);
+  if (nd == 3)
+    for (int k = b.lo[2]; k <= b.hi[2]; k++) sig[2][k] = S.box(b.lo[0], b.hi[0], b.lo[1], b.hi[1], k, k);
 }
 
 // NewSubGrids of one patch: the subgrid boxes (patch-local indices) in the
 // order MakeNewGrid creates them; on failure BSAM_EINVAL with `msg` set.
 int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, std::vector<Box>& sub,
-                     std::string& msg) {
+                     std::string& msg, TagSums& S) {
   sub.clear();
   char buf[160];
   auto fail = [&](const char* what) {
@@ -115,7 +125,10 @@ int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, 
     whole.lo[n] = 1;
     whole.hi[n] = n < nd ? T.mx[n] : 1;
   }
-  if (fill_ratio(T, whole, nd) < 1.0e-10) return BSAM_OK;
+  S.build(T);
+  // GridFlagRatio of the whole patch
+  const double cells0 = (double)T.mx[0] * T.mx[1] * T.mx[2];
+  if ((double)S.box(1, T.mx[0], 1, T.mx[1], 1, T.mx[2]) / cells0 < 1.0e-10) return BSAM_OK;
   std::vector<int> sig[3], dd[3];
   for (int n = 0; n < nd; n++) {
     sig[n].assign(T.mx[n] + 2, 0);
@@ -127,7 +140,7 @@ int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, 
     for (int pass = 0; can[g] && pass < kMaxSplitPasses; pass++) {
       // trim untagged edge slabs (signatures of the box before trimming),
       // then widen to whole coarse cell pairs: lo odd, hi even
-      signatures(T, sub[g], nd, sig);
+      signatures(S, sub[g], nd, sig);
       for (int n = 0; n < nd; n++) {
         int i1 = sub[g].lo[n], i2 = sub[g].hi[n];
         while (sig[n][i1] == 0 && i1 < sub[g].hi[n] && i2 - i1 + 1 > mgp) i1++;
@@ -137,7 +150,7 @@ int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, 
       }
       // GridFlagRatio of the trimmed box, from its signatures (their sum
       // over one dimension counts the box's tags exactly)
-      signatures(T, sub[g], nd, sig);
+      signatures(S, sub[g], nd, sig);
       long tagged = 0, cells = 1;
       for (int i = sub[g].lo[0]; i <= sub[g].hi[0]; i++) tagged += sig[0][i];
       for (int n = 0; n < nd; n++) cells *= (long)(sub[g].hi[n] - sub[g].lo[n] + 1);
@@ -424,9 +437,10 @@ extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patc
   std::stable_sort(big.begin(), big.end(), [&](int x, int y) { return off[x + 1] - off[x] > off[y + 1] - off[y]; });
   std::atomic<size_t> next(0);
   for_threads(nth, [&](int) {
+    TagSums S;
     for (size_t n; (n = next.fetch_add(1)) < big.size();) {
       const int a = big[n];
-      rc[a] = new_subgrids(T[a], ndims, minimumgridpoints, desiredfillratio, level, sub[a], msg[a]);
+      rc[a] = new_subgrids(T[a], ndims, minimumgridpoints, desiredfillratio, level, sub[a], msg[a], S);
     }
   });
   std::vector<bsam_patch_desc> made;

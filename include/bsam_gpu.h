@@ -362,6 +362,20 @@ int32_t bsam_kstat_enable(bsam_ctx* ctx, int32_t on);
  * of one instrumented kernel class. */
 int32_t bsam_kstat_read(bsam_ctx* ctx, int32_t which, int64_t* launches,
                         double* total_ms, double* bytes_per_launch);
+/* Wall-clock phases of a time step with regrid.  While enabled, the
+ * context's stream is synchronised at every phase boundary, so the GPU work
+ * queued in a phase is counted in it; graph capture is unaffected. */
+#define BSAM_PHASE_AMR_TAG      0   /* SetGhost and EstimateLevelErrors per level */
+#define BSAM_PHASE_AMR_CLUSTER  1   /* bsam_new_level: buffering, clustering */
+#define BSAM_PHASE_AMR_INSTALL  2   /* installs of the new levels (tables, pools) */
+#define BSAM_PHASE_AMR_FIELDS   3   /* InitFields, TransferValues, pool carries */
+#define BSAM_PHASE_SOLVE        4   /* CopyQToQold and MultigridIterations */
+#define BSAM_PHASE_GRAPH        5   /* capture and instantiation of the iteration graph (inside SOLVE) */
+#define BSAM_PHASE_COUNT        6
+/* Enable (1) or disable (0) the phase clocks; enabling resets them. */
+int32_t bsam_phase_enable(bsam_ctx* ctx, int32_t on);
+/* Times entered and summed milliseconds of one phase. */
+int32_t bsam_phase_read(bsam_ctx* ctx, int32_t which, int64_t* calls, double* total_ms);
 
 /* ---- launch-shape autotuning (no counterpart in the reference, whose
  * per-patch loops have no launch shape) --------------------------------

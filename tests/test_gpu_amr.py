@@ -102,3 +102,35 @@ def test_amr_refuses_bad_grid_parameters(gpu_solver_cls):
     i = s0["end"][0]
     assert np.array_equal(S.get(i, "q").ravel(), fu.end_state(z, 0, i))
     S.close()
+
+
+# The bench workloads' meshes and grid parameters (bench.py REGRID): levels
+# of up to 256^3 cells whose tags AMR buffers on the device.
+BENCH_GRIDS = {"ch3d_amr_64_l3": 0.05, "tm3d_amr_64_l3": 0.05, "ch2d_amr_128_l2": 0.05}
+
+
+@pytest.mark.parametrize("name", list(BENCH_GRIDS))
+def test_amr_device_buffering_matches_host_clustering(name, gpu_solver_cls):
+    """bsam_amr buffers a level's tags on the device (a dilation over the
+    level's bit map) and clusters without buffering; bsam_new_level
+    buffers on the host (BufferAndList, InflateEdgeTags).  After a pass,
+    each level is in the state it was tagged on (AMR builds upward and
+    SetGhost(level, 0) is idempotent), so the host path on the same flags
+    must rebuild every new level descriptor for descriptor."""
+    from bsam_amd.grid import new_level
+    from bsam_amd.workload import Workload
+    print(f"[gpu] device tag buffering {name}", flush=True)
+    wl = Workload(name, 256)
+    S = gpu_solver_cls(wl.params("fp64", 0, 1), wl.hier)
+    wl.load(S)
+    grid = abi.grid_params(wl.finest, 11, BENCH_GRIDS[name], ibuffer=1, minimumgridpoints=8, desiredfillratio=0.8)
+    S.amr(grid)
+    hier = S.hier
+    for lv in range(0, S.finestlevel):
+        S.set_ghost(lv, 0)
+        flags, ntag = S.estimate_level_errors(lv, 11, BENCH_GRIDS[name])
+        assert ntag > 0
+        want = new_level(wl.nd, hier, lv, flags, 1, 8, 0.8)
+        got = [d for d in hier if d["level"] == lv + 1]
+        _same_mesh(got, want)
+    S.close()
