@@ -52,6 +52,7 @@ KNOB_CHAIN_CELLS = 6
 KNOB_REGRID_TABLES = 7
 KNOB_GATHER_BLOCK_FP64 = 8
 KNOB_GATHER_BLOCK_FP32 = 9
+KNOB_TABLE_PLANNER = 10
 
 
 class PatchDesc(C.Structure):
@@ -166,6 +167,7 @@ ENTRY_POINTS = {
     "bsam_kstat_enable": ([_P, _I], _I),
     "bsam_kstat_read": ([_P, _I, _LP, _DP, _DP], _I),
     "bsam_phase_enable": ([_P, _I], _I),
+    "bsam_ghost_table_digest": ([_P, _I, C.POINTER(C.c_int64)], _I),
     "bsam_phase_read": ([_P, _I, _LP, _DP], _I),
     "bsam_autotune": ([_P], _I),
     "bsam_tuning_report": ([_P, C.c_char_p, _I], _I),

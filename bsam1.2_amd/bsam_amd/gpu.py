@@ -277,6 +277,13 @@ class Solver:
     def kstat_enable(self, on=True):
         self._chk(self.lib.bsam_kstat_enable(self.ctx, 1 if on else 0))
 
+    def ghost_table_digest(self, level):
+        """Per flattened ghost table of the level (7): (copies, interpolation
+        rules, first-launch copies, first-launch rules, hash of the bytes)."""
+        out = (C.c_int64 * 35)()
+        self._chk(self.lib.bsam_ghost_table_digest(self.ctx, level, out))
+        return [tuple(out[5 * m:5 * m + 5]) for m in range(7)]
+
     def phase_enable(self, on=True):
         self._chk(self.lib.bsam_phase_enable(self.ctx, 1 if on else 0))
 

@@ -3125,6 +3125,7 @@ struct Level {
   // exact (see plan_ghost_maps)
   DevMap gmap[7];
   bool maps_ok = false;
+  std::string why;                // why a level keeps the staged fill
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   // wrap_pid's extents are all even: a colour sweep may write the ghost
   // images of its cells (wrap_images), since every image then has the
@@ -3143,6 +3144,7 @@ struct Level {
   long qspan_off = 0, qspan_len = 0;  // contiguous q-pool span of the level
   long ncell = 0;                     // interior cells of the level
   long* d_foff = nullptr;             // start of each patch's cells in a dense level array (tagging)
+  std::vector<void*> blocks;          // pooled device blocks holding every array above
 };
 
 struct KStat {
@@ -3182,6 +3184,10 @@ struct bsam_ctx {
   size_t tag_cap = 0;
   // AMR's tag buffering on the device: bit map of a level's index space
   // (capacity in words) and the level patches' global lower corners
+  // workspace of the device planner of the flattened ghost tables
+  char* dplan = nullptr;
+  size_t dplan_cap = 0;
+  int table_planner = 1;  // levels above the root: 1 device planner, 0 host planner
   uint32_t* d_tagbits = nullptr;
   size_t tagbits_cap = 0;
   int* d_tagglo = nullptr;
@@ -3199,8 +3205,8 @@ struct bsam_ctx {
   // field pools an install released, kept for the next install (a regrid
   // installs every step): buffer and capacity in bytes; pool_cap holds the
   // capacity of the pools in use
-  std::vector<std::pair<void*, size_t>> spare;
-  std::unordered_map<void*, size_t> pool_cap;
+  std::vector<std::pair<void*, size_t>> spare, spare_tab;  // field pools, level-table blocks
+  std::unordered_map<void*, std::pair<size_t, int>> pool_cap;  // capacity, reserve (0 fields, 1 tables)
   // mixed precision: an outer iteration whose residual exceeds mixed_stall
   // times the previous one ends the step in FP64 V-cycles
   double mixed_stall = 0.9;
@@ -3417,31 +3423,37 @@ static void drop_graph(bsam_ctx* c) {
   }
 }
 
-// Field pools: a released pool is kept and handed to the next request it
-// can hold without wasting more than half of itself.  Every release and
-// reuse follows a synchronisation of the context's stream (installs and
-// destruction synchronise first), so no work in flight reads a reused pool.
-static hipError_t pool_alloc(bsam_ctx* c, void** p, size_t bytes) {
+// Field pools and level-table blocks: a released buffer is kept in its
+// reserve and handed to the next request of that reserve it can hold
+// without wasting more than half of itself.  Every release and reuse
+// follows a synchronisation of the context's stream (installs and
+// destruction synchronise first), so no work in flight reads a reused
+// buffer.
+static hipError_t pool_alloc(bsam_ctx* c, void** p, size_t bytes, int kind = 0) {
+  auto& spare = kind ? c->spare_tab : c->spare;
+  // an eighth of headroom, in whole 2 MiB pages, so that a mesh that grows
+  // a little from one regrid to the next finds its buffers in the reserve
+  const size_t page = size_t(2) << 20;
+  const size_t want = ((std::max<size_t>(bytes, 1) + bytes / 8) + page - 1) & ~(page - 1);
   int best = -1;
-  for (size_t i = 0; i < c->spare.size(); i++) {
-    const size_t cap = c->spare[i].second;
-    if (cap >= bytes && cap <= 2 * bytes + (size_t(1) << 20) && (best < 0 || cap < c->spare[best].second))
-      best = (int)i;
+  for (size_t i = 0; i < spare.size(); i++) {
+    const size_t cap = spare[i].second;
+    if (cap >= bytes && cap <= 2 * want && (best < 0 || cap < spare[best].second)) best = (int)i;
   }
   if (best >= 0) {
-    *p = c->spare[best].first;
-    c->pool_cap[*p] = c->spare[best].second;
-    c->spare.erase(c->spare.begin() + best);
+    *p = spare[best].first;
+    c->pool_cap[*p] = {spare[best].second, kind};
+    spare.erase(spare.begin() + best);
     return hipSuccess;
   }
-  const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
-  if (e == hipSuccess) c->pool_cap[*p] = std::max<size_t>(bytes, 1);
+  const hipError_t e = hipMalloc(p, want);
+  if (e == hipSuccess) c->pool_cap[*p] = {want, kind};
   return e;
 }
 template <typename T>
-static hipError_t pool_alloc(bsam_ctx* c, T** p, size_t bytes) {
+static hipError_t pool_alloc(bsam_ctx* c, T** p, size_t bytes, int kind = 0) {
   void* v = nullptr;
-  const hipError_t e = pool_alloc(c, &v, bytes);
+  const hipError_t e = pool_alloc(c, &v, bytes, kind);
   *p = static_cast<T*>(v);
   return e;
 }
@@ -3452,45 +3464,46 @@ static void pool_release(bsam_ctx* c, void* p) {
     hipFree(p);
     return;
   }
-  c->spare.push_back({p, it->second});
+  const int kind = it->second.second;
+  auto& spare = kind ? c->spare_tab : c->spare;
+  spare.push_back({p, it->second.first});
   c->pool_cap.erase(it);
-  // a bounded reserve: the oldest spare pools go back to the device
-  while (c->spare.size() > 24) {
-    hipFree(c->spare.front().first);
-    c->spare.erase(c->spare.begin());
+  // a bounded reserve (a regrid pass installs its levels one at a time,
+  // each install with its own set of pools, and keeps the old mesh until
+  // it ends): at most 64 buffers and the larger of 4 GiB and twice the
+  // reserve's bytes in use; the oldest spare buffers go back to the device
+  size_t inuse = 0, held = 0;
+  for (const auto& kv : c->pool_cap)
+    if (kv.second.second == kind) inuse += kv.second.first;
+  for (const auto& sp : spare) held += sp.second;
+  const size_t limit = std::max<size_t>(size_t(4) << 30, 2 * inuse);
+  while (!spare.empty() && (spare.size() > 64 || held > limit)) {
+    held -= spare.front().second;
+    hipFree(spare.front().first);
+    spare.erase(spare.begin());
   }
 }
 static void pool_drain(bsam_ctx* c) {
   for (auto& sp : c->spare) hipFree(sp.first);
+  for (auto& sp : c->spare_tab) hipFree(sp.first);
   c->spare.clear();
+  c->spare_tab.clear();
 }
 
 // The device arrays of one level (launch lists, ghost-fill stages, rules
-// and tables); the level is left empty.
-static void free_level(Level& L) {
-  hipFree(L.d_patches);
-  hipFree(L.d_foff);
-  hipFree(L.d_rules);
-  for (DevMap& m : L.gmap) {
-    hipFree(m.cr);
-    hipFree(m.cpost);
-    hipFree(m.ir);
-  }
-  for (int d = 0; d < kNumDom; d++) {
-    hipFree(L.d_prefix[d]);
-    hipFree(L.d_cnt[d]);
-    hipFree(L.d_bp[d]);
-  }
-  for (Stage& st : L.stages) hipFree(st.d);
+// and tables) live in pooled blocks the level owns; they go back to the
+// reserve and the level is left empty.
+static void free_level(bsam_ctx* c, Level& L) {
+  for (void* b : L.blocks) pool_release(c, b);
   L = Level();
 }
 
 static void free_hierarchy(bsam_ctx* c) {
   if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
-  for (Level& L : c->lv) free_level(L);
+  for (Level& L : c->lv) free_level(c, L);
   c->lv.clear();
-  hipFree(c->d_pt);
+  pool_release(c, c->d_pt);
   for (double* pool : {c->q, c->qold, c->f, c->rf, c->ftmp, c->qc}) pool_release(c, pool);
   c->d_pt = nullptr; c->q = c->qold = c->f = c->rf = c->ftmp = c->qc = nullptr;
   if (c->mixed) {
@@ -3514,6 +3527,7 @@ extern "C" int32_t bsam_destroy(bsam_ctx* c) {
   hipFree(c->d_tag);
   hipFree(c->d_tagbits);
   hipFree(c->d_tagglo);
+  hipFree(c->dplan);
   if (c->h_norm) hipHostFree(c->h_norm);
   for (auto& k : c->ks)
     for (hipEvent_t e : k.ev) hipEventDestroy(e);
@@ -4088,7 +4102,7 @@ struct FExpr {
   int early;  // reads a face ghost as it was before the call
 };
 
-__host__ inline void qdecode(const DPatch& P, int o, int& i, int& j, int& k) {
+__host__ __device__ inline void qdecode(const DPatch& P, int o, int& i, int& j, int& k) {
   int r = o - P.qoff;
   const int m = r % P.hq;
   r /= P.hq;
@@ -4098,7 +4112,7 @@ __host__ inline void qdecode(const DPatch& P, int o, int& i, int& j, int& k) {
   const int col = r / P.nq[2];
   i = 2 * m + ((col + j + k) & 1);
 }
-__host__ inline int n_outside(const DPatch& P, int nd, int i, int j, int k) {
+__host__ __device__ inline int n_outside(const DPatch& P, int nd, int i, int j, int k) {
   return (i < 1 || i > P.mx[0]) + (j < 1 || j > P.mx[1]) + (nd == 3 && (k < 1 || k > P.mx[2]));
 }
 
@@ -4617,7 +4631,37 @@ int32_t plan_ghost_maps(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   return BSAM_OK;
 }
 
-int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
+// Small device arrays of one level staged in one host buffer (256-byte
+// aligned), then copied with one copy into one pooled block the level owns;
+// each array's device address is stored where add() was told.  The host
+// buffer must outlive the copy: upload() waits for it.
+struct BlockBuilder {
+  std::vector<char> h;
+  std::vector<std::pair<size_t, void**>> fix;
+  template <typename T>
+  void add(const T* src, size_t count, T** dst) {
+    const size_t off = (h.size() + 255) & ~size_t(255);
+    h.resize(off + std::max<size_t>(count * sizeof(T), 1));
+    if (count) memcpy(h.data() + off, src, count * sizeof(T));
+    fix.push_back({off, reinterpret_cast<void**>(dst)});
+  }
+  int32_t upload(bsam_ctx* c, Level& L) {
+    if (h.empty()) return BSAM_OK;
+    void* b = nullptr;
+    if (pool_alloc(c, &b, h.size(), 1) != hipSuccess) return set_err(BSAM_ENOMEM, "level tables of %zu bytes", h.size());
+    L.blocks.push_back(b);
+    HIPCHK(hipMemcpyAsync(b, h.data(), h.size(), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (auto& f : fix) *f.second = static_cast<char*>(b) + f.first;
+    h.clear();
+    fix.clear();
+    return BSAM_OK;
+  }
+};
+
+// The ghost fill of a level planned on the host, staged for the level's
+// block: box stages, host-built interpolation rules, flattened tables.
+void upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G, BlockBuilder& Bk) {
   L.wrap_pid = G.wrap_pid;
   L.wrap_img = false;
   if (L.wrap_pid >= 0) {
@@ -4626,22 +4670,16 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
   }
   L.stages.clear();
   for (const auto& boxes : G.stages) {
-    BoxDesc* d = nullptr;
-    HIPCHK(hipMalloc(&d, boxes.size() * sizeof(BoxDesc)));
-    HIPCHK(hipMemcpy(d, boxes.data(), boxes.size() * sizeof(BoxDesc), hipMemcpyHostToDevice));
     Stage s;
-    s.d = d;
     s.n = (int)boxes.size();
     for (const BoxDesc& b : boxes) s.maxel = std::max(s.maxel, b.ext[0] * b.ext[1] * b.ext[2]);
     L.stages.push_back(s);
   }
+  for (size_t st = 0; st < G.stages.size(); st++) Bk.add(G.stages[st].data(), G.stages[st].size(), &L.stages[st].d);
   for (int sg = 0; sg < 3; sg++) L.roff[sg] = G.roff[sg];
-  if (!G.rules.empty()) {
-    HIPCHK(hipMalloc(&L.d_rules, G.rules.size() * sizeof(GRule)));
-    HIPCHK(hipMemcpy(L.d_rules, G.rules.data(), G.rules.size() * sizeof(GRule), hipMemcpyHostToDevice));
-  }
+  if (!G.rules.empty()) Bk.add(G.rules.data(), G.rules.size(), &L.d_rules);
   L.maps_ok = G.maps_ok;
-  if (!G.maps_ok) return BSAM_OK;
+  if (!G.maps_ok) return;
   for (int m = 0; m < 7; m++) {
     const HostMap& H = G.map[m];
     DevMap& D = L.gmap[m];
@@ -4649,19 +4687,632 @@ int32_t upload_ghost(bsam_ctx* c, Level& L, const GhostPlan& G) {
     D.ni = (int)H.interps.size();
     D.nc1 = H.nc1;
     D.ni1 = H.ni1;
-    if (D.nc) {
-      HIPCHK(hipMalloc(&D.cr, H.copies.size() * sizeof(int4)));
-      HIPCHK(hipMemcpy(D.cr, H.copies.data(), H.copies.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if (D.nc) Bk.add(H.copies.data(), H.copies.size(), &D.cr);
+    if (!H.cpost.empty()) Bk.add(H.cpost.data(), H.cpost.size(), &D.cpost);
+    if (D.ni) Bk.add(H.interps.data(), H.interps.size(), &D.ir);
+  }
+}
+
+// The device planners' workspace (kept by the context, grown as needed).
+static char* plan_workspace(bsam_ctx* c, size_t bytes) {
+  if (bytes > c->dplan_cap) {
+    if (hipStreamSynchronize(c->s) != hipSuccess) return nullptr;
+    hipFree(c->dplan);
+    c->dplan = nullptr;
+    c->dplan_cap = 0;
+    if (hipMalloc(&c->dplan, bytes) != hipSuccess) return nullptr;
+    c->dplan_cap = bytes;
+  }
+  return c->dplan;
+}
+
+// ---------------------------------------------------------------------------
+// The flattened ghost fill planned on the device, for levels above the root
+// (batched launches): flatten_setghost's symbolic run of SetGhost(level,
+// ipass) and plan_ghost_maps' selection of the seven tables, cell by cell.
+// The expressions of one call live in a dense array over the level's q
+// cells (variable 0, LevelCells' numbering).  Within a stage the host runs
+// the boxes in order, so the last box that writes a cell wins; here a
+// claim pass (atomicMax of the box index) picks that box and a write pass
+// lets it write.  The interpolation stage likewise keeps the last rule in
+// (segment, patch, element) order.  Every table is compacted in the host's
+// order (dense cell order, the rules that read ghosts from before the call
+// first), so the tables equal plan_ghost_maps' rule for rule.  What the
+// host refuses is refused here too, and so is a box reading a cell that
+// its own stage writes (an order the claim cannot reproduce): the level
+// then keeps the staged fill.
+// ---------------------------------------------------------------------------
+struct DExpr {
+  int src, spid, dpid, meta;
+};
+// meta: valid bit, kind (2 bits: 0 copy, 1 interpolation rule, 2 a value
+// read from an unwritten edge or corner ghost), early (2 bits: outside
+// dimensions of a ghost read from before the call), post-operation count
+// (3 bits) and chain (15 bits)
+constexpr int kDxValid = 1 << 30;
+__host__ __device__ inline int dx_kind(int m) { return m & 3; }
+__host__ __device__ inline int dx_early(int m) { return (m >> 2) & 3; }
+__host__ __device__ inline int dx_npost(int m) { return (m >> 4) & 7; }
+__host__ __device__ inline int dx_post(int m) { return (m >> 7) & 0x7fff; }
+__host__ __device__ inline int dx_make(int kind, int early, int npost, int post) {
+  return kDxValid | kind | (early << 2) | (npost << 4) | (post << 7);
+}
+// refusal bits of the device planner
+enum : int {
+  kDpInterior = 1,    // a stage writes an interior cell
+  kDpOrder = 2,       // a box reads a cell its own stage writes
+  kDpInterpCopy = 4,  // an interpolated ghost copied into another patch
+  kDpChain = 8,       // post-operation chain longer than five
+  kDpUnknown = 16,    // a value read from an unwritten edge or corner ghost survives
+  kDpEarly = 32,      // a ghost read from before the call is rewritten by another such rule
+  kDpSel5 = 64,       // a colour-1 face ghost of SetGhost(level, 0) reads a ghost from before the call
+  kDpLean = 128,      // a reduced table's ghost reads a ghost from before the call
+};
+
+struct DPlanArgs {
+  const DPatch* __restrict__ pt;
+  const int* __restrict__ patches;  // the level's descriptor indices
+  const int* __restrict__ pos;      // descriptor index -> position in the level, or -1
+  const long* __restrict__ base;    // dense start of each level patch
+  const int* __restrict__ lq;       // pool offset of each level patch (ascending)
+  int np, nd;
+};
+__device__ __forceinline__ long dp_dense(const DPlanArgs& A, int pid, int off) {
+  return A.base[A.pos[pid]] + (off - A.pt[pid].qoff);
+}
+// level position of the patch whose q slot holds pool offset `off`
+__device__ __forceinline__ int dp_patch_at(const DPlanArgs& A, int off) {
+  int lo = 0, hi = A.np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (A.lq[mid] <= off) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dp_fill(int* __restrict__ a, long n, int v) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g < n) a[g] = v;
+}
+
+// The interpolation stage: rules [r0, r1) (the call's colour-filter
+// segments, in order).  write 0: claim; 1: the last claimant writes.
+__global__ __launch_bounds__(kBlock) void k_dp_interp(DPlanArgs A, const GRule* __restrict__ rules, int r0, int r1,
+                                                      int* __restrict__ claim, DExpr* __restrict__ ex, int write) {
+  const long g = r0 + (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= r1) return;
+  const int dst = rules[g].dst;
+  const int a = dp_patch_at(A, dst);
+  const int pid = A.patches[a];
+  const long cell = A.base[a] + (dst - A.pt[pid].qoff);
+  if (!write) {
+    atomicMax(&claim[cell], (int)g);
+    return;
+  }
+  if (claim[cell] != (int)g) return;
+  ex[cell] = DExpr{(int)g, -1, pid, dx_make(1, 0, 0, 0)};
+}
+
+// One box stage over the flat cells of its boxes (bpre: each box's first
+// flat cell; x fastest within a box).  `tag` = (stage number + 1) << 24 in
+// the claims.
+__global__ __launch_bounds__(kBlock) void k_dp_box(DPlanArgs A, const BoxDesc* __restrict__ boxes,
+                                                   const int* __restrict__ bpre, int nbox, int ncell, int tag,
+                                                   int* __restrict__ claim, DExpr* __restrict__ ex,
+                                                   int* __restrict__ err, int write) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ncell) return;
+  int lo = 0, hi = nbox;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bpre[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const BoxDesc B = boxes[lo];
+  const int loc = g - bpre[lo];
+  const int x = loc % B.ext[0], r = loc / B.ext[0];
+  const int y = r % B.ext[1], z = r / B.ext[1];
+  const DPatch D = A.pt[B.dp];
+  const int di = B.dlo[0] + x, dj = B.dlo[1] + y, dk = B.dlo[2] + z;
+  const long dcell = dp_dense(A, B.dp, qidx(D, di, dj, dk));
+  const int me = tag | lo;
+  if (!write) {
+    atomicMax(&claim[dcell], me);
+    return;
+  }
+  if (claim[dcell] != me) return;
+  if (!n_outside(D, A.nd, di, dj, dk)) {
+    atomicOr(err, kDpInterior);
+    return;
+  }
+  const DPatch S = A.pt[B.sp];
+  const int si = B.slo[0] + x, sj = B.slo[1] + y, sk = B.slo[2] + z;
+  const int s = qidx(S, si, sj, sk);
+  const long scell = dp_dense(A, B.sp, s);
+  if ((claim[scell] & 0x7f000000) == tag) {
+    atomicOr(err, kDpOrder);
+    return;
+  }
+  DExpr e = ex[scell];
+  if (e.meta & kDxValid) {
+    if (dx_kind(e.meta) == 1 && e.dpid != B.dp) {
+      atomicOr(err, kDpInterpCopy);
+      return;
     }
-    if (!H.cpost.empty()) {
-      HIPCHK(hipMalloc(&D.cpost, H.cpost.size() * sizeof(uint16_t)));
-      HIPCHK(hipMemcpy(D.cpost, H.cpost.data(), H.cpost.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  } else {
+    const int out = n_outside(S, A.nd, si, sj, sk);
+    e = DExpr{s, B.sp, 0, dx_make(out > 1 ? 2 : 0, out, 0, 0)};
+  }
+  e.dpid = B.dp;
+  // the post-operation the box's operation adds (ghost_post's codes)
+  const int code = (B.op == kCopy) ? 0
+                   : (B.op == kCopyPlus0 ? 1 : (B.op == kDirichlet2 ? 2 : (B.op == kBgMixed ? 3 : 4)));
+  if (code) {
+    const int k = dx_npost(e.meta);
+    if (k >= 5) {
+      atomicOr(err, kDpChain);
+      return;
     }
-    if (D.ni) {
-      HIPCHK(hipMalloc(&D.ir, H.interps.size() * sizeof(GRule)));
-      HIPCHK(hipMemcpy(D.ir, H.interps.data(), H.interps.size() * sizeof(GRule), hipMemcpyHostToDevice));
+    e.meta = dx_make(dx_kind(e.meta), dx_early(e.meta), k + 1, dx_post(e.meta) | (code << (3 * k)));
+  }
+  ex[dcell] = e;
+}
+
+// The call's end checks: no unknown value survives, and no rule that reads
+// a ghost from before the call reads one that another such rule writes.
+__global__ __launch_bounds__(kBlock) void k_dp_check(DPlanArgs A, const DExpr* __restrict__ ex, long n,
+                                                     int* __restrict__ err) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= n) return;
+  const DExpr e = ex[g];
+  if (!(e.meta & kDxValid)) return;
+  if (dx_kind(e.meta) == 2) atomicOr(err, kDpUnknown);
+  if (dx_early(e.meta)) {
+    const DExpr w = ex[dp_dense(A, e.spid, e.src)];
+    if ((w.meta & kDxValid) && dx_early(w.meta)) atomicOr(err, kDpEarly);
+  }
+}
+
+// What plan_ghost_maps' sel56 and lean need besides the expressions.
+struct DPlanMaps {
+  const DExpr* __restrict__ ex[3];
+  const GRule* __restrict__ rules;
+  const int* __restrict__ mglo;      // descriptor index -> mglobal lower corner (1-based, 3 ints)
+  const uint32_t* __restrict__ occ;  // parent level's cells inside its patches (bit map)
+  int per[3], cper[3], cwrap[3];
+};
+
+__device__ __forceinline__ bool dp_same_expr(const GRule* __restrict__ rules, const DPatch* __restrict__ pt,
+                                             const DExpr& a, const DExpr& b) {
+  if (dx_kind(a.meta) != dx_kind(b.meta) || dx_post(a.meta) != dx_post(b.meta) || dx_npost(a.meta) != dx_npost(b.meta))
+    return false;
+  if (dx_kind(a.meta) == 0) return a.src == b.src && pt[a.spid].qvs == pt[b.spid].qvs;
+  const GRule r = rules[a.src], s = rules[b.src];
+  if (r.kind != s.kind || r.s1 != s.s1 || r.s2 != s.s2 || r.cvs != s.cvs) return false;
+  for (int n = 0; n < r.kind && n < 4; n++)
+    if (r.c[n] != s.c[n]) return false;
+  return true;
+}
+
+// Table category of a cell: 0 none, 1 copy reading a ghost from before the
+// call, 2 other copy, 3 and 4 the same for interpolation rules.
+__device__ __forceinline__ uint8_t dp_cat(const DExpr& e) {
+  const int early = dx_early(e.meta) ? 0 : 1;
+  return (uint8_t)((dx_kind(e.meta) == 0 ? 1 : 3) + early);
+}
+
+// Per cell, the category in each of the seven tables (cat[m * n + cell]),
+// the refusals of sel56 and lean, which tables have a copy with a
+// post-operation, and the rule counts of every (table, category), summed
+// per workgroup in LDS first.
+__global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, long n, uint8_t* __restrict__ cat,
+                                                      int* __restrict__ err, int* __restrict__ anypost,
+                                                      unsigned long long* __restrict__ counts) {
+  __shared__ int cnt[7][5];
+  if (threadIdx.x < 35) cnt[threadIdx.x / 5][threadIdx.x % 5] = 0;
+  __syncthreads();
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g < n) {
+    uint8_t out[7] = {0, 0, 0, 0, 0, 0, 0};
+    DExpr e[3];
+    bool ok[3];
+    for (int p = 0; p < 3; p++) {
+      e[p] = M.ex[p][g];
+      ok[p] = (e[p].meta & kDxValid) != 0;
+      if (ok[p]) out[p] = dp_cat(e[p]);
+    }
+    // map[5] and map[6]: colour-1 face ghosts of SetGhost(level, 0), map[6]
+    // without the copies a correction leaves exact
+    if (ok[0]) {
+      const DExpr& x = e[0];
+      const DPatch D = A.pt[x.dpid];
+      const int dst = D.qoff + (int)(g - A.base[A.pos[x.dpid]]);
+      int d3[3];
+      qdecode(D, dst, d3[0], d3[1], d3[2]);
+      if (n_outside(D, A.nd, d3[0], d3[1], d3[2]) == 1 && ((d3[0] + d3[1] + d3[2]) & 1) == 1) {
+        if (dx_early(x.meta)) {
+          atomicOr(err, kDpSel5);
+        } else {
+          bool exact = dx_kind(x.meta) == 0 && dx_post(x.meta) == 0;
+          if (exact) {
+            const DPatch S = A.pt[x.spid];
+            int s3[3];
+            qdecode(S, x.src, s3[0], s3[1], s3[2]);
+            exact = n_outside(S, A.nd, s3[0], s3[1], s3[2]) == 0;
+            for (int a = 0; a < A.nd && exact; a++) {
+              const int per = M.per[a];
+              const int ds = (M.mglo[3 * x.spid + a] + s3[a]) - (M.mglo[3 * x.dpid + a] + d3[a]);
+              exact = ((ds % per) + per) % per == 0;
+            }
+            for (int corner = 0; corner < (1 << A.nd) && exact; corner++) {
+              long idx = 0;
+              for (int a = A.nd - 1; a >= 0 && exact; a--) {
+                const int gf = M.mglo[3 * x.dpid + a] + d3[a] - 1;
+                const int lo = (gf & 1) ? (gf + 1) / 2 - 1 : gf / 2;
+                int xx = lo + ((corner >> a) & 1) - 1;
+                if (xx < 0 || xx >= M.cper[a]) {
+                  if (!M.cwrap[a]) exact = false;
+                  xx = ((xx % M.cper[a]) + M.cper[a]) % M.cper[a];
+                }
+                idx = idx * M.cper[a] + xx;
+              }
+              exact = exact && ((M.occ[idx >> 5] >> (idx & 31)) & 1u);
+            }
+          }
+          out[5] = out[0];
+          out[6] = exact ? 0 : out[0];
+        }
+      }
+    }
+    // map[2 + p], p = 1, 2: what a red-black sweep sequence needs
+    for (int p = 1; p <= 2; p++) {
+      if (!ok[p]) continue;
+      const DExpr& x = e[p];
+      const int col = p - 1;
+      const bool have_o = ok[3 - p];
+      const DPatch D = A.pt[x.dpid];
+      const int dst = D.qoff + (int)(g - A.base[A.pos[x.dpid]]);
+      int i, j, k;
+      qdecode(D, dst, i, j, k);
+      bool need;
+      if (n_outside(D, A.nd, i, j, k) != 1) {
+        if (!have_o && dx_early(x.meta)) {
+          atomicOr(err, kDpLean);
+          continue;
+        }
+        need = !have_o;
+      } else {
+        need = ((i + j + k) & 1) == col;
+        if (!need) {
+          int si, sj, sk;
+          if (dx_kind(x.meta) == 0) {
+            qdecode(A.pt[x.spid], x.src, si, sj, sk);
+            need = ((si + sj + sk) & 1) == col;
+          } else {
+            const GRule R = M.rules[x.src];
+            qdecode(D, R.s1, si, sj, sk);
+            need = ((si + sj + sk) & 1) == col;
+            qdecode(D, R.s2, si, sj, sk);
+            need = need || ((si + sj + sk) & 1) == col;
+          }
+        }
+        if (!need) need = !have_o || !dp_same_expr(M.rules, A.pt, x, e[3 - p]);
+        if (need && dx_early(x.meta)) {
+          atomicOr(err, kDpLean);
+          continue;
+        }
+      }
+      if (need) out[2 + p] = out[p];
+    }
+    int post = 0;
+    for (int m = 0; m < 7; m++) {
+      cat[m * n + g] = out[m];
+      if (out[m]) atomicAdd(&cnt[m][out[m]], 1);
+    }
+    for (int m = 0; m < 7; m++) {
+      const int p = m < 3 ? m : (m == 3 ? 1 : (m == 4 ? 2 : 0));
+      if (out[m] && out[m] <= 2 && dx_post(e[p].meta)) post |= 1 << m;
+    }
+    if (post) atomicOr(anypost, post);
+  }
+  __syncthreads();
+  if (threadIdx.x < 35) {
+    const int m = threadIdx.x / 5, k = threadIdx.x % 5;
+    if (k && cnt[m][k]) atomicAdd(&counts[m * 5 + k], (unsigned long long)cnt[m][k]);
+  }
+}
+
+// Running counts of one table: copies in the low 32 bits, interpolation
+// rules in the high 32, for the rules that read ghosts from before the
+// call (EARLY) or the others.
+template <bool EARLY>
+struct DpCount {
+  __host__ __device__ unsigned long long operator()(const uint8_t& c) const {
+    return c == (EARLY ? 1 : 2) ? 1ull : (c == (EARLY ? 3 : 4) ? (1ull << 32) : 0ull);
+  }
+};
+
+// The rules of one table at their compacted positions.
+__global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, const DExpr* __restrict__ ex,
+                                                    const uint8_t* __restrict__ cat,
+                                                    const unsigned long long* __restrict__ offe,
+                                                    const unsigned long long* __restrict__ offl, long n, int nc1,
+                                                    int ni1, const GRule* __restrict__ rules, int4* __restrict__ cr,
+                                                    uint16_t* __restrict__ cpost, GRule* __restrict__ ir) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= n) return;
+  const int c = cat[g];
+  if (!c) return;
+  const DExpr e = ex[g];
+  const DPatch D = A.pt[e.dpid];
+  const int dst = D.qoff + (int)(g - A.base[A.pos[e.dpid]]);
+  const int post = dx_post(e.meta);
+  if (c <= 2) {
+    const long at = c == 1 ? (long)(offe[g] & 0xffffffffu) : nc1 + (long)(offl[g] & 0xffffffffu);
+    cr[at] = make_int4(dst, e.src, D.qvs, A.pt[e.spid].qvs);
+    if (cpost) cpost[at] = (uint16_t)post;
+  } else {
+    const long at = c == 3 ? (long)(offe[g] >> 32) : ni1 + (long)(offl[g] >> 32);
+    GRule r = rules[e.src];
+    r.dst = dst;
+    r.kind = (r.kind & 0xff) | (post << 8);
+    ir[at] = r;
+  }
+}
+
+// Cells of a level's patches as bits of the level's index space (the
+// occupancy test of map[6]).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_dp_occupy(LevelArgs L, const int* __restrict__ mglo,
+                                                      uint32_t* __restrict__ bits, int ex, int ey) {
+  int pi, local;
+  if (!level_elem(L, level_block(L), pi, local)) return;
+  const int pid = L.patches[pi];
+  const DPatch P = L.pt[pid];
+  int i, j, k;
+  if (!cell_of<DIM>(P, local, i, j, k)) return;
+  const int kk = (DIM == 3) ? k - 1 : 0;
+  const long b = ((long)(mglo[3 * pid + 2] - 1 + kk) * ey + (mglo[3 * pid + 1] - 1 + j - 1)) * ex + (mglo[3 * pid] - 1 + i - 1);
+  atomicOr(&bits[b >> 5], 1u << (b & 31));
+}
+
+// Workspace of the device planner, kept by the context and grown as needed.
+struct DPlanWs {
+  char* p = nullptr;
+  size_t cap = 0;
+};
+
+int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string& why) {
+  L.maps_ok = false;
+  const int np = (int)L.patches.size(), nd = c->p.ndims;
+  if (L.level <= 0 || np == 0 || !L.d_rules) {
+    why = "not a refined level with device rules";
+    return BSAM_OK;
+  }
+  if (G.stages.size() > 120) {
+    why = "too many ghost-fill stages";
+    return BSAM_OK;
+  }
+  // dense numbering of the level's q cells (variable 0), patch after patch
+  std::vector<int> pos(c->npatch, -1), lq(np);
+  std::vector<long> base(np);
+  long n = 0;
+  for (int a = 0; a < np; a++) {
+    const DPatch& P = c->hp[L.patches[a]];
+    pos[L.patches[a]] = a;
+    base[a] = n;
+    lq[a] = P.qoff;
+    if (a > 0 && lq[a] <= lq[a - 1]) {
+      why = "level patches not in ascending pool order";
+      return BSAM_OK;
+    }
+    n += P.qvs;
+  }
+  if (n >= (1L << 31)) {
+    why = "level too large for the device planner";
+    return BSAM_OK;
+  }
+  // box stages: each box's first flat cell
+  std::vector<std::vector<int>> bpre(G.stages.size());
+  for (size_t st = 0; st < G.stages.size(); st++) {
+    long acc = 0;
+    for (const BoxDesc& b : G.stages[st]) {
+      bpre[st].push_back((int)acc);
+      acc += (long)b.ext[0] * b.ext[1] * b.ext[2];
+    }
+    bpre[st].push_back((int)acc);
+    if (acc >= (1L << 31) || G.stages[st].size() >= (1u << 24)) {
+      why = "a ghost-fill stage too large for the device planner";
+      return BSAM_OK;
     }
   }
+  // the root's periods and the parent level's extents
+  const bsam_patch_desc* root = nullptr;
+  for (const bsam_patch_desc& d : c->desc)
+    if (d.level == 0) { root = &d; break; }
+  int per[3] = {1, 1, 1}, cper[3] = {1, 1, 1}, cwrap[3] = {0, 0, 0};
+  for (int a = 0; a < nd; a++) {
+    per[a] = level_period(c, L.level, a);
+    cper[a] = level_period(c, L.level - 1, a);
+    cwrap[a] = root->mthbc[2 * a] == 2;
+  }
+  const long ocells = (long)cper[0] * cper[1] * cper[2];
+  if (ocells > (1L << 33)) {
+    why = "parent level too large for the occupancy bit map";
+    return BSAM_OK;
+  }
+  const size_t owords = (size_t)((ocells + 31) / 32);
+  // workspace: three expression arrays, two claim arrays, the categories,
+  // two running-count arrays, then the small tables
+  size_t scan_bytes = 0;
+  {
+    hipcub::TransformInputIterator<unsigned long long, DpCount<true>, const uint8_t*> it(nullptr, DpCount<true>());
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, it, (unsigned long long*)nullptr, (int)n, c->s));
+  }
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  size_t boxints = 0;
+  for (auto& v : bpre) boxints += v.size();
+  const size_t o_ex = 0, o_ci = o_ex + al(3 * n * sizeof(DExpr)), o_cb = o_ci + al(n * sizeof(int)),
+               o_cat = o_cb + al(n * sizeof(int)), o_offe = o_cat + al(7 * n),
+               o_offl = o_offe + al(n * sizeof(unsigned long long)),
+               o_pos = o_offl + al(n * sizeof(unsigned long long)), o_base = o_pos + al(c->npatch * sizeof(int)),
+               o_lq = o_base + al(np * sizeof(long)), o_bpre = o_lq + al(np * sizeof(int)),
+               o_mglo = o_bpre + al(boxints * sizeof(int)), o_occ = o_mglo + al(3 * c->npatch * sizeof(int)),
+               o_small = o_occ + al(owords * sizeof(uint32_t)), o_scan = o_small + 512, total = o_scan + al(scan_bytes);
+  char* W = plan_workspace(c, total);
+  if (!W) return set_err(BSAM_ENOMEM, "ghost-table planner workspace of %zu bytes", total);
+  DExpr* ex = reinterpret_cast<DExpr*>(W + o_ex);
+  int* claim_i = reinterpret_cast<int*>(W + o_ci);
+  int* claim_b = reinterpret_cast<int*>(W + o_cb);
+  uint8_t* cat = reinterpret_cast<uint8_t*>(W + o_cat);
+  auto* offe = reinterpret_cast<unsigned long long*>(W + o_offe);
+  auto* offl = reinterpret_cast<unsigned long long*>(W + o_offl);
+  int* d_pos = reinterpret_cast<int*>(W + o_pos);
+  long* d_base = reinterpret_cast<long*>(W + o_base);
+  int* d_lq = reinterpret_cast<int*>(W + o_lq);
+  int* d_bpre = reinterpret_cast<int*>(W + o_bpre);
+  int* d_mglo = reinterpret_cast<int*>(W + o_mglo);
+  uint32_t* d_occ = reinterpret_cast<uint32_t*>(W + o_occ);
+  int* d_err = reinterpret_cast<int*>(W + o_small);
+  int* d_post = d_err + 1;
+  auto* d_cnt = reinterpret_cast<unsigned long long*>(W + o_small + 64);
+  void* d_scan = W + o_scan;
+  // the small tables in one staging copy; the bit map and the counters
+  // start at zero
+  {
+    std::vector<char> h(o_occ - o_pos, 0);
+    auto put = [&](size_t off, const void* src, size_t bytes) { memcpy(h.data() + (off - o_pos), src, bytes); };
+    put(o_pos, pos.data(), pos.size() * sizeof(int));
+    put(o_base, base.data(), base.size() * sizeof(long));
+    put(o_lq, lq.data(), lq.size() * sizeof(int));
+    size_t k = 0;
+    for (auto& v : bpre) {
+      put(o_bpre + k * sizeof(int), v.data(), v.size() * sizeof(int));
+      k += v.size();
+    }
+    std::vector<int> mg(3 * c->npatch, 1);
+    for (int i = 0; i < c->npatch; i++)
+      for (int a = 0; a < nd; a++) mg[3 * i + a] = c->desc[i].mglobal[a][0];
+    put(o_mglo, mg.data(), mg.size() * sizeof(int));
+    HIPCHK(hipMemcpyAsync(W + o_pos, h.data(), h.size(), hipMemcpyHostToDevice, c->s));
+    const long zw = (long)((o_scan - o_occ) / sizeof(int));
+    k_dp_fill<<<nblocks(zw), kBlock, 0, c->s>>>(reinterpret_cast<int*>(W + o_occ), zw, 0);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  DPlanArgs A{c->d_pt, L.d_patches, d_pos, d_base, d_lq, np, nd};
+  // the parent level's cells
+  {
+    Level& Pl = c->L(L.level - 1);
+    FOR_LAUNCHES(c, Pl, kCell, a, nb) {
+      if (nd == 2) k_dp_occupy<2><<<nb, kBlock, 0, c->s>>>(a, d_mglo, d_occ, cper[0], cper[1]);
+      else k_dp_occupy<3><<<nb, kBlock, 0, c->s>>>(a, d_mglo, d_occ, cper[0], cper[1]);
+      LAUNCH_CHECK();
+    }
+  }
+  const int nbn = nblocks(n);
+  for (int p = 0; p < 3; p++) {
+    DExpr* E = ex + (size_t)p * n;
+    k_dp_fill<<<nblocks(4 * n), kBlock, 0, c->s>>>(reinterpret_cast<int*>(E), 4 * n, 0);
+    k_dp_fill<<<nbn, kBlock, 0, c->s>>>(claim_i, n, -1);
+    k_dp_fill<<<nbn, kBlock, 0, c->s>>>(claim_b, n, -1);
+    LAUNCH_CHECK();
+    const int s0 = (p == 2) ? 1 : 0, s1 = (p == 1) ? 1 : 2;
+    const int r0 = L.roff[s0][0], r1 = L.roff[s1][np];
+    if (r1 > r0) {
+      for (int w = 0; w < 2; w++)
+        k_dp_interp<<<nblocks(r1 - r0), kBlock, 0, c->s>>>(A, L.d_rules, r0, r1, claim_i, E, w);
+      LAUNCH_CHECK();
+    }
+    size_t k = 0;
+    for (size_t st = 0; st < G.stages.size(); st++) {
+      const int nbox = (int)G.stages[st].size(), ncell = bpre[st].back();
+      if (ncell > 0)
+        for (int w = 0; w < 2; w++)
+          k_dp_box<<<nblocks(ncell), kBlock, 0, c->s>>>(A, L.stages[st].d, d_bpre + k, nbox, ncell,
+                                                          (int)(st + 1) << 24, claim_b, E, d_err, w);
+      LAUNCH_CHECK();
+      k += bpre[st].size();
+    }
+    k_dp_check<<<nbn, kBlock, 0, c->s>>>(A, E, n, d_err);
+    LAUNCH_CHECK();
+  }
+  DPlanMaps M{};
+  for (int p = 0; p < 3; p++) M.ex[p] = ex + (size_t)p * n;
+  M.rules = L.d_rules;
+  M.mglo = d_mglo;
+  M.occ = d_occ;
+  for (int a = 0; a < 3; a++) M.per[a] = per[a], M.cper[a] = cper[a], M.cwrap[a] = cwrap[a];
+  k_dp_select<<<nbn, kBlock, 0, c->s>>>(A, M, n, cat, d_err, d_post, d_cnt);
+  LAUNCH_CHECK();
+  struct {
+    int err, post;
+    int pad[14];
+    unsigned long long cnt[35];
+  } h{};
+  HIPCHK(hipMemcpyAsync(&h, W + o_small, 64 + 35 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (h.err) {
+    static const char* what[8] = {"a ghost-fill stage writes an interior cell",
+                                  "a box reads a cell its own stage writes",
+                                  "an interpolated ghost value is copied into another patch",
+                                  "post-operation chain longer than five",
+                                  "a ghost value is read from an edge or corner ghost the call has not written",
+                                  "a ghost read from before the call is rewritten by another such rule",
+                                  "a colour-1 face ghost of SetGhost(level, 0) reads a ghost from before the call",
+                                  "a face ghost of a reduced table reads a ghost from before the call"};
+    for (int b = 0; b < 8; b++)
+      if (h.err & (1 << b)) {
+        why = what[b];
+        break;
+      }
+    return BSAM_OK;
+  }
+  // the seven tables in one pooled block
+  size_t tbytes = 0;
+  size_t at[7][3];
+  for (int m = 0; m < 7; m++) {
+    DevMap& D = L.gmap[m];
+    const unsigned long long* k = h.cnt + 5 * m;
+    D.nc1 = (int)k[1];
+    D.nc = (int)(k[1] + k[2]);
+    D.ni1 = (int)k[3];
+    D.ni = (int)(k[3] + k[4]);
+    at[m][0] = tbytes;
+    tbytes = al(tbytes + (size_t)D.nc * sizeof(int4));
+    at[m][1] = tbytes;
+    if (h.post & (1 << m)) tbytes = al(tbytes + (size_t)D.nc * sizeof(uint16_t));
+    at[m][2] = tbytes;
+    tbytes = al(tbytes + (size_t)D.ni * sizeof(GRule));
+  }
+  char* T = nullptr;
+  if (pool_alloc(c, &T, std::max<size_t>(tbytes, 1), 1) != hipSuccess)
+    return set_err(BSAM_ENOMEM, "ghost tables of %zu bytes", tbytes);
+  L.blocks.push_back(T);
+  for (int m = 0; m < 7; m++) {
+    DevMap& D = L.gmap[m];
+    D.cr = D.nc ? reinterpret_cast<int4*>(T + at[m][0]) : nullptr;
+    D.cpost = (D.nc && (h.post & (1 << m))) ? reinterpret_cast<uint16_t*>(T + at[m][1]) : nullptr;
+    D.ir = D.ni ? reinterpret_cast<GRule*>(T + at[m][2]) : nullptr;
+  }
+  for (int m = 0; m < 7; m++) {
+    DevMap& D = L.gmap[m];
+    if (!D.nc && !D.ni) continue;
+    const int p = m < 3 ? m : (m == 3 ? 1 : (m == 4 ? 2 : 0));
+    const uint8_t* cm = cat + (size_t)m * n;
+    size_t sb = scan_bytes;
+    hipcub::TransformInputIterator<unsigned long long, DpCount<true>, const uint8_t*> ie(cm, DpCount<true>());
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_scan, sb, ie, offe, (int)n, c->s));
+    sb = scan_bytes;
+    hipcub::TransformInputIterator<unsigned long long, DpCount<false>, const uint8_t*> il(cm, DpCount<false>());
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_scan, sb, il, offl, (int)n, c->s));
+    k_dp_emit<<<nbn, kBlock, 0, c->s>>>(A, ex + (size_t)p * n, cm, offe, offl, n, D.nc1, D.ni1, L.d_rules, D.cr,
+                                        D.cpost, D.ir);
+    LAUNCH_CHECK();
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  L.maps_ok = true;
   return BSAM_OK;
 }
 
@@ -4792,65 +5443,66 @@ int32_t build_rules_device(bsam_ctx* c, Level& L) {
     eoff[a] = (int)total;
     total += interp_elems(c->hp[L.patches[a]], nd);
   }
-  if (total > 0x7fffffffL) return set_err(BSAM_EINVAL, "level %d: too many interpolation elements", L.level);
+  if (total > 0x7fffffffL / 3) return set_err(BSAM_EINVAL, "level %d: too many interpolation elements", L.level);
   eoff[np] = (int)total;
-  struct Tmp {
-    int* eoff = nullptr;
-    uint8_t* seg = nullptr;
-    int* idx = nullptr;
-    int* num = nullptr;
-    void* scratch = nullptr;
-    ~Tmp() {
-      hipFree(eoff);
-      hipFree(seg);
-      hipFree(idx);
-      hipFree(num);
-      hipFree(scratch);
-    }
-  } t;
-  HIPCHK(hipMalloc(&t.eoff, (np + 1) * sizeof(int)));
-  HIPCHK(hipMalloc(&t.seg, std::max<long>(1, total)));
-  HIPCHK(hipMalloc(&t.idx, std::max<long>(1, total) * sizeof(int)));
-  HIPCHK(hipMalloc(&t.num, sizeof(int)));
-  HIPCHK(hipMemcpy(t.eoff, eoff.data(), (np + 1) * sizeof(int), hipMemcpyHostToDevice));
   for (int sg = 0; sg < 3; sg++) L.roff[sg].assign(np + 1, 0);
+  L.d_rules = nullptr;
   if (total == 0) return BSAM_OK;
-  k_interp_segment<<<nblocks(total), kBlock, 0, c->s>>>(c->d_pt, L.d_patches, t.eoff, np, nd, total, t.seg);
-  LAUNCH_CHECK();
+  // workspace: element offsets, segments, three index lists, counts, scan
   size_t bytes = 0;
   hipcub::CountingInputIterator<int> elems(0);
-  HIPCHK(hipcub::DeviceSelect::If(nullptr, bytes, elems, t.idx, t.num, (int)total, SegmentIs{t.seg, 0}, c->s));
-  HIPCHK(hipMalloc(&t.scratch, std::max<size_t>(1, bytes)));
-  int off = 0;
+  HIPCHK(hipcub::DeviceSelect::If(nullptr, bytes, elems, (int*)nullptr, (int*)nullptr, (int)total,
+                                  SegmentIs{nullptr, 0}, c->s));
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t o_eoff = 0, o_seg = al((np + 1) * sizeof(int)), o_idx = o_seg + al(total),
+               o_num = o_idx + al(3 * total * sizeof(int)), o_scr = o_num + 256, need = o_scr + al(bytes);
+  char* W = plan_workspace(c, need);
+  if (!W) return set_err(BSAM_ENOMEM, "interpolation-rule workspace of %zu bytes", need);
+  int* t_eoff = reinterpret_cast<int*>(W + o_eoff);
+  uint8_t* t_seg = reinterpret_cast<uint8_t*>(W + o_seg);
+  int* t_idx = reinterpret_cast<int*>(W + o_idx);
+  int* t_num = reinterpret_cast<int*>(W + o_num);
+  HIPCHK(hipMemcpyAsync(t_eoff, eoff.data(), (np + 1) * sizeof(int), hipMemcpyHostToDevice, c->s));
+  k_interp_segment<<<nblocks(total), kBlock, 0, c->s>>>(c->d_pt, L.d_patches, t_eoff, np, nd, total, t_seg);
+  LAUNCH_CHECK();
+  // each segment's elements in order, into its own list; one wait for the
+  // three counts
   for (int sg = 0; sg < 3; sg++) {
     size_t b2 = bytes;
-    HIPCHK(hipcub::DeviceSelect::If(t.scratch, b2, elems, t.idx + off, t.num, (int)total, SegmentIs{t.seg, (uint8_t)sg},
-                                    c->s));
-    int n = 0;
-    HIPCHK(hipMemcpyAsync(&n, t.num, sizeof(int), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipcub::DeviceSelect::If(W + o_scr, b2, elems, t_idx + (size_t)sg * total, t_num + sg, (int)total,
+                                    SegmentIs{t_seg, (uint8_t)sg}, c->s));
+  }
+  int n[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(n, t_num, sizeof(n), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  int off = 0;
+  for (int sg = 0; sg < 3; sg++) {
     L.roff[sg][0] = off;
-    off += n;
+    off += n[sg];
     L.roff[sg][np] = off;
   }
-  hipFree(L.d_rules);
-  L.d_rules = nullptr;
   if (off > 0) {
-    HIPCHK(hipMalloc(&L.d_rules, (size_t)off * sizeof(GRule)));
-    k_interp_write<<<nblocks(off), kBlock, 0, c->s>>>(c->d_pt, L.d_patches, t.eoff, np, nd, t.idx, off, L.d_rules);
-    LAUNCH_CHECK();
+    if (pool_alloc(c, &L.d_rules, (size_t)off * sizeof(GRule), 1) != hipSuccess)
+      return set_err(BSAM_ENOMEM, "interpolation rules of level %d", L.level);
+    L.blocks.push_back(L.d_rules);
+    for (int sg = 0; sg < 3; sg++)
+      if (n[sg] > 0) {
+        k_interp_write<<<nblocks(n[sg]), kBlock, 0, c->s>>>(c->d_pt, L.d_patches, t_eoff, np, nd,
+                                                            t_idx + (size_t)sg * total, n[sg],
+                                                            L.d_rules + L.roff[sg][0]);
+        LAUNCH_CHECK();
+      }
   }
   HIPCHK(hipStreamSynchronize(c->s));
   return BSAM_OK;
 }
-
 // The device side of one level of a new hierarchy: its patch list, the
 // block-aligned launch ranges of every domain, the tagging offsets and the
 // ghost fill (stages, interpolation rules, flattened tables).
 int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
   const int nv = c->p.nrvars;
-  HIPCHK(hipMalloc(&L.d_patches, L.patches.size() * sizeof(int)));
-  HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), L.patches.size() * sizeof(int), hipMemcpyHostToDevice));
+  BlockBuilder Bk;
+  Bk.add(L.patches.data(), L.patches.size(), &L.d_patches);
   for (int d = 0; d < kNumDom; d++) {
     std::vector<int> pre(L.patches.size() + 1, 0), cnts(L.patches.size(), 0);
     long acc = 0;
@@ -4877,10 +5529,8 @@ int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
     L.total[d] = (int)acc;
     L.hprefix[d] = pre;
     L.hcnt[d] = cnts;
-    HIPCHK(hipMalloc(&L.d_prefix[d], pre.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(L.d_prefix[d], pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&L.d_cnt[d], cnts.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(L.d_cnt[d], cnts.data(), cnts.size() * sizeof(int), hipMemcpyHostToDevice));
+    Bk.add(pre.data(), pre.size(), &L.d_prefix[d]);
+    Bk.add(cnts.data(), cnts.size(), &L.d_cnt[d]);
     // block -> patch, for whole-level launches
     std::vector<int> bp(std::max(1, nblocks(acc)), 0);
     size_t pi = 0;
@@ -4889,8 +5539,7 @@ int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
       while (pi + 1 < L.patches.size() && pre[pi + 1] <= g0) pi++;
       bp[b] = (int)pi;
     }
-    HIPCHK(hipMalloc(&L.d_bp[d], bp.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(L.d_bp[d], bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
+    Bk.add(bp.data(), bp.size(), &L.d_bp[d]);
   }
   // contiguous span of the level in the q pools (patches are laid out in
   // descriptor order; levels are spans only if their patches are
@@ -4913,22 +5562,25 @@ int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
       n += (long)P.mx[0] * P.mx[1] * P.mx[2];
     }
     L.ncell = n;
-    HIPCHK(hipMalloc(&L.d_foff, foff.size() * sizeof(long)));
-    HIPCHK(hipMemcpy(L.d_foff, foff.data(), foff.size() * sizeof(long), hipMemcpyHostToDevice));
+    Bk.add(foff.data(), foff.size(), &L.d_foff);
   }
   GhostPlan G;
   RETCHK(plan_ghost_stages(c, L, G));
-  // a level that keeps the staged fill in batched launches builds its
-  // interpolation rules on the device
-  const bool dev_rules = !tables && c->p.batching && L.level > 0;
+  // batched levels above the root build their interpolation rules, and
+  // (with the device planner) their flattened tables, on the device
+  const bool dev_tables = tables && c->p.batching && L.level > 0 && c->table_planner == 1;
+  const bool dev_rules = (!tables || dev_tables) && c->p.batching && L.level > 0;
   if (dev_rules) {
     for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(L.patches.size() + 1, 0);
   } else {
     plan_interp_rules(c, L, G);
   }
-  if (c->p.batching && tables) RETCHK(plan_ghost_maps(c, L, G));
-  RETCHK(upload_ghost(c, L, G));
+  if (c->p.batching && tables && !dev_tables) RETCHK(plan_ghost_maps(c, L, G));
+  upload_ghost(c, L, G, Bk);
+  RETCHK(Bk.upload(c, L));
   if (dev_rules) RETCHK(build_rules_device(c, L));
+  L.why = G.why;
+  if (dev_tables) RETCHK(plan_maps_device(c, L, G, L.why));
   return BSAM_OK;
 }
 
@@ -5007,7 +5659,7 @@ struct OldMesh {
   }
   ~OldMesh() {
     hipStreamSynchronize(c->s);
-    for (Level& L : lv) free_level(L);
+    for (Level& L : lv) free_level(c, L);
     pool_release(c, q);
     pool_release(c, qold);
     hipFree(d_box);
@@ -5046,11 +5698,12 @@ int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_
                         : minlevel - 1;
   if (kept) *kept = keep;
   struct Held {
+    bsam_ctx* c;
     std::vector<Level> v;
     ~Held() {
-      for (Level& L : v) free_level(L);
+      for (Level& L : v) free_level(c, L);
     }
-  } held;
+  } held{c, {}};
   for (int l = minlevel; l <= keep; l++) {
     held.v.push_back(std::move(c->L(l)));
     c->L(l) = Level();
@@ -5058,8 +5711,8 @@ int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_
   free_hierarchy(c);
   RETCHK(plan_patches(c, npatch, desc, minlevel, finestlevel));
   const long qo = c->nq, co = c->nc;
-  HIPCHK(hipMalloc(&c->d_pt, npatch * sizeof(DPatch)));
-  HIPCHK(hipMemcpy(c->d_pt, c->hp.data(), npatch * sizeof(DPatch), hipMemcpyHostToDevice));
+  if (pool_alloc(c, &c->d_pt, npatch * sizeof(DPatch), 1) != hipSuccess) return set_err(BSAM_ENOMEM, "patch table");
+  HIPCHK(hipMemcpyAsync(c->d_pt, c->hp.data(), npatch * sizeof(DPatch), hipMemcpyHostToDevice, c->s));
   double** pools[5] = {&c->q, &c->qold, &c->f, &c->rf, &c->ftmp};
   for (double** pp : pools) {
     if (pool_alloc(c, pp, qo * sizeof(double)) != hipSuccess)
@@ -6916,6 +7569,34 @@ extern "C" int32_t bsam_kstat_enable(bsam_ctx* c, int32_t on) {
   return BSAM_OK;
 }
 
+extern "C" int32_t bsam_ghost_table_digest(bsam_ctx* c, int32_t level, int64_t* out) {
+  if (!c || !out) return set_err(BSAM_EINVAL, "null argument");
+  RETCHK(check_level(c, level));
+  const Level& L = c->L(level);
+  if (!L.maps_ok) return set_err(BSAM_ESTATE, "level %d has no flattened ghost tables (%s)", level, L.why.c_str());
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int m = 0; m < 7; m++) {
+    const DevMap& D = L.gmap[m];
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* dev, size_t bytes) -> hipError_t {
+      if (!dev || !bytes) return hipSuccess;
+      std::vector<unsigned char> b(bytes);
+      const hipError_t e = hipMemcpy(b.data(), dev, bytes, hipMemcpyDeviceToHost);
+      for (unsigned char x : b) h = (h ^ x) * 1099511628211ull;
+      return e;
+    };
+    HIPCHK(mix(D.cr, (size_t)D.nc * sizeof(int4)));
+    HIPCHK(mix(D.cpost, (size_t)D.nc * sizeof(uint16_t)));
+    HIPCHK(mix(D.ir, (size_t)D.ni * sizeof(GRule)));
+    out[5 * m] = D.nc;
+    out[5 * m + 1] = D.ni;
+    out[5 * m + 2] = D.nc1;
+    out[5 * m + 3] = D.ni1;
+    out[5 * m + 4] = (int64_t)h;
+  }
+  return BSAM_OK;
+}
+
 extern "C" int32_t bsam_phase_enable(bsam_ctx* c, int32_t on) {
   if (!c) return set_err(BSAM_EINVAL, "null context");
   HIPCHK(hipStreamSynchronize(c->s));
@@ -7594,6 +8275,11 @@ extern "C" int32_t bsam_set_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
     c->regrid_tables = value;
     return BSAM_OK;
   }
+  if (knob == BSAM_KNOB_TABLE_PLANNER) {
+    if (value != 0 && value != 1) return set_err(BSAM_EINVAL, "table planner must be 0 (host) or 1 (device)");
+    c->table_planner = value;
+    return BSAM_OK;
+  }
   if (!(knob >= 0 && knob <= BSAM_KNOB_RESID_PLANES) && knob != BSAM_KNOB_GATHER_BLOCK_FP64 &&
       knob != BSAM_KNOB_GATHER_BLOCK_FP32)
     return set_err(BSAM_EINVAL, "unknown knob %d", knob);
@@ -7634,6 +8320,10 @@ extern "C" int32_t bsam_get_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
   }
   if (knob == BSAM_KNOB_REGRID_TABLES) {
     *value = c->regrid_tables;
+    return BSAM_OK;
+  }
+  if (knob == BSAM_KNOB_TABLE_PLANNER) {
+    *value = c->table_planner;
     return BSAM_OK;
   }
   if (!(knob >= 0 && knob <= BSAM_KNOB_RESID_PLANES) && knob != BSAM_KNOB_GATHER_BLOCK_FP64 &&

@@ -362,6 +362,12 @@ int32_t bsam_kstat_enable(bsam_ctx* ctx, int32_t on);
  * of one instrumented kernel class. */
 int32_t bsam_kstat_read(bsam_ctx* ctx, int32_t which, int64_t* launches,
                         double* total_ms, double* bytes_per_launch);
+/* Digest of a level's flattened ghost tables (a test hook): for each table
+ * m = 0..6, out[5m .. 5m+4] = copy rules, interpolation rules, the copy and
+ * interpolation rules of the first launch, and a 64-bit FNV-1a hash of the
+ * tables' bytes (copies, post-operations, interpolation rules).
+ * BSAM_ESTATE when the level has no flattened tables. */
+int32_t bsam_ghost_table_digest(bsam_ctx* ctx, int32_t level, int64_t* out);
 /* Wall-clock phases of a time step with regrid.  While enabled, the
  * context's stream is synchronised at every phase boundary, so the GPU work
  * queued in a phase is counted in it; graph capture is unaffected. */
@@ -423,6 +429,11 @@ int32_t bsam_tuning_report(const bsam_ctx* ctx, char* buf, int32_t len);
 #define BSAM_KNOB_GATHER_BLOCK_FP64  8  /* workgroup size of the level's FP64 ghost gather:
                                            0 (256), 64, 128, 256, 512 or 1024 */
 #define BSAM_KNOB_GATHER_BLOCK_FP32  9  /* the same for the FP32 gather */
+#define BSAM_KNOB_TABLE_PLANNER     10  /* whole hierarchy (level ignored): who plans the
+                                           flattened ghost tables of the levels above the
+                                           root that an install builds (batched launches):
+                                           1 (default) the device, 0 the host; the tables
+                                           are the same rule for rule */
 int32_t bsam_set_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t value);
 int32_t bsam_get_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t* value);
 
