@@ -113,7 +113,7 @@ def parse():
     ap.add_argument("--regrid-steps", type=int, default=-1,
                     help="timed time steps with the reference's regrid (AMR every step) per path "
                          "(-1: as many as --steps; 0: frozen mesh only)")
-    ap.add_argument("--regrid-tables", type=int, default=0,
+    ap.add_argument("--regrid-tables", type=int, default=1,
                     help="levels rebuilt by the regrid get flattened ghost tables (1) or keep the staged fill (0)")
     return ap.parse_args()
 

@@ -4749,16 +4749,40 @@ enum : int {
   kDpLean = 128,      // a reduced table's ghost reads a ghost from before the call
 };
 
+// Ghost cells of a patch's storage box, numbered shell by shell.  3D: the
+// plane k = 0, the plane k = nz - 1, then for each inner plane the rows
+// j = 0 and j = ny - 1 and the two end cells of every inner row (nx, ny,
+// nz the storage extents); 2D: the rows and end cells alone.
+__host__ __device__ inline long shell_count(const DPatch& P, int nd) {
+  const long nz = nd == 3 ? P.nq[2] : 1, mz = nd == 3 ? P.mx[2] : 1;
+  return (long)P.nq[0] * P.nq[1] * nz - (long)P.mx[0] * P.mx[1] * mz;
+}
+// the shell index of ghost cell (i, j, k) (storage indices; not for an
+// interior cell)
+__device__ __forceinline__ int shell_index(const DPatch& P, int nd, int i, int j, int k) {
+  const int nx = P.nq[0], ny = P.nq[1];
+  int base = 0;
+  if (nd == 3) {
+    const int nz = P.nq[2];
+    if (k == 0) return j * nx + i;
+    if (k == nz - 1) return nx * ny + j * nx + i;
+    base = 2 * nx * ny + (k - 1) * (2 * nx + 2 * (ny - 2));
+  }
+  if (j == 0) return base + i;
+  if (j == ny - 1) return base + nx + i;
+  return base + 2 * nx + (j - 1) * 2 + (i == 0 ? 0 : 1);
+}
 struct DPlanArgs {
   const DPatch* __restrict__ pt;
   const int* __restrict__ patches;  // the level's descriptor indices
   const int* __restrict__ pos;      // descriptor index -> position in the level, or -1
-  const long* __restrict__ base;    // dense start of each level patch
+  const long* __restrict__ gbase;   // first ghost index of each level patch (np + 1 entries)
+  const long* __restrict__ fbase;   // first cell (variable 0, pool order) of each level patch (np + 1)
   const int* __restrict__ lq;       // pool offset of each level patch (ascending)
   int np, nd;
 };
-__device__ __forceinline__ long dp_dense(const DPlanArgs& A, int pid, int off) {
-  return A.base[A.pos[pid]] + (off - A.pt[pid].qoff);
+__device__ __forceinline__ long dp_ghost(const DPlanArgs& A, int pid, int i, int j, int k) {
+  return A.gbase[A.pos[pid]] + shell_index(A.pt[pid], A.nd, i, j, k);
 }
 // level position of the patch whose q slot holds pool offset `off`
 __device__ __forceinline__ int dp_patch_at(const DPlanArgs& A, int off) {
@@ -4769,6 +4793,24 @@ __device__ __forceinline__ int dp_patch_at(const DPlanArgs& A, int off) {
     else hi = mid;
   }
   return lo;
+}
+// Cell f of the level in pool order (variable 0, patch after patch, each
+// patch in its colour-split order): its patch (level position), storage
+// indices and ghost index (-1 for an interior cell).  The tables list
+// their rules in this order, as the host planner does.
+__device__ __forceinline__ long dp_cell(const DPlanArgs& A, long f, int& a, int& i, int& j, int& k) {
+  int lo = 0, hi = A.np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (A.fbase[mid] <= f) lo = mid;
+    else hi = mid;
+  }
+  a = lo;
+  const DPatch& P = A.pt[A.patches[a]];
+  qdecode(P, P.qoff + (int)(f - A.fbase[a]), i, j, k);
+  // (the padding slot of an odd row length is no cell)
+  if (i >= P.nq[0] || !n_outside(P, A.nd, i, j, k)) return -1;
+  return A.gbase[a] + shell_index(P, A.nd, i, j, k);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dp_fill(int* __restrict__ a, long n, int v) {
@@ -4785,7 +4827,9 @@ __global__ __launch_bounds__(kBlock) void k_dp_interp(DPlanArgs A, const GRule* 
   const int dst = rules[g].dst;
   const int a = dp_patch_at(A, dst);
   const int pid = A.patches[a];
-  const long cell = A.base[a] + (dst - A.pt[pid].qoff);
+  int i, j, k;
+  qdecode(A.pt[pid], dst, i, j, k);
+  const long cell = A.gbase[a] + shell_index(A.pt[pid], A.nd, i, j, k);
   if (!write) {
     atomicMax(&claim[cell], (int)g);
     return;
@@ -4815,34 +4859,37 @@ __global__ __launch_bounds__(kBlock) void k_dp_box(DPlanArgs A, const BoxDesc* _
   const int y = r % B.ext[1], z = r / B.ext[1];
   const DPatch D = A.pt[B.dp];
   const int di = B.dlo[0] + x, dj = B.dlo[1] + y, dk = B.dlo[2] + z;
-  const long dcell = dp_dense(A, B.dp, qidx(D, di, dj, dk));
+  if (!n_outside(D, A.nd, di, dj, dk)) {
+    atomicOr(err, kDpInterior);
+    return;
+  }
+  const long dcell = dp_ghost(A, B.dp, di, dj, dk);
   const int me = tag | lo;
   if (!write) {
     atomicMax(&claim[dcell], me);
     return;
   }
   if (claim[dcell] != me) return;
-  if (!n_outside(D, A.nd, di, dj, dk)) {
-    atomicOr(err, kDpInterior);
-    return;
-  }
   const DPatch S = A.pt[B.sp];
   const int si = B.slo[0] + x, sj = B.slo[1] + y, sk = B.slo[2] + z;
   const int s = qidx(S, si, sj, sk);
-  const long scell = dp_dense(A, B.sp, s);
-  if ((claim[scell] & 0x7f000000) == tag) {
-    atomicOr(err, kDpOrder);
-    return;
-  }
-  DExpr e = ex[scell];
-  if (e.meta & kDxValid) {
-    if (dx_kind(e.meta) == 1 && e.dpid != B.dp) {
-      atomicOr(err, kDpInterpCopy);
+  const int out = n_outside(S, A.nd, si, sj, sk);
+  DExpr e{s, B.sp, 0, dx_make(out > 1 ? 2 : 0, out, 0, 0)};
+  if (out) {
+    // a ghost source: the expression an earlier stage left there, if any
+    const long scell = dp_ghost(A, B.sp, si, sj, sk);
+    if ((claim[scell] & 0x7f000000) == tag) {
+      atomicOr(err, kDpOrder);
       return;
     }
-  } else {
-    const int out = n_outside(S, A.nd, si, sj, sk);
-    e = DExpr{s, B.sp, 0, dx_make(out > 1 ? 2 : 0, out, 0, 0)};
+    const DExpr prev = ex[scell];
+    if (prev.meta & kDxValid) {
+      if (dx_kind(prev.meta) == 1 && prev.dpid != B.dp) {
+        atomicOr(err, kDpInterpCopy);
+        return;
+      }
+      e = prev;
+    }
   }
   e.dpid = B.dp;
   // the post-operation the box's operation adds (ghost_post's codes)
@@ -4869,8 +4916,13 @@ __global__ __launch_bounds__(kBlock) void k_dp_check(DPlanArgs A, const DExpr* _
   if (!(e.meta & kDxValid)) return;
   if (dx_kind(e.meta) == 2) atomicOr(err, kDpUnknown);
   if (dx_early(e.meta)) {
-    const DExpr w = ex[dp_dense(A, e.spid, e.src)];
-    if ((w.meta & kDxValid) && dx_early(w.meta)) atomicOr(err, kDpEarly);
+    const DPatch S = A.pt[e.spid];
+    int i, j, k;
+    qdecode(S, e.src, i, j, k);
+    if (n_outside(S, A.nd, i, j, k)) {
+      const DExpr w = ex[dp_ghost(A, e.spid, i, j, k)];
+      if ((w.meta & kDxValid) && dx_early(w.meta)) atomicOr(err, kDpEarly);
+    }
   }
 }
 
@@ -4898,23 +4950,30 @@ __device__ __forceinline__ bool dp_same_expr(const GRule* __restrict__ rules, co
 // Table category of a cell: 0 none, 1 copy reading a ghost from before the
 // call, 2 other copy, 3 and 4 the same for interpolation rules.
 __device__ __forceinline__ uint8_t dp_cat(const DExpr& e) {
-  const int early = dx_early(e.meta) ? 0 : 1;
-  return (uint8_t)((dx_kind(e.meta) == 0 ? 1 : 3) + early);
+  const int late = dx_early(e.meta) ? 0 : 1;
+  return (uint8_t)((dx_kind(e.meta) == 0 ? 1 : 3) + late);
 }
+// the call whose expression table m holds
+__host__ __device__ inline int dp_call(int m) { return m < 3 ? m : (m == 3 ? 1 : (m == 4 ? 2 : 0)); }
 
-// Per cell, the category in each of the seven tables (cat[m * n + cell]),
-// the refusals of sel56 and lean, which tables have a copy with a
-// post-operation, and the rule counts of every (table, category), summed
-// per workgroup in LDS first.
-__global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, long n, uint8_t* __restrict__ cat,
-                                                      int* __restrict__ err, int* __restrict__ anypost,
-                                                      unsigned long long* __restrict__ counts) {
-  __shared__ int cnt[7][5];
-  if (threadIdx.x < 35) cnt[threadIdx.x / 5][threadIdx.x % 5] = 0;
-  __syncthreads();
-  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (g < n) {
-    uint8_t out[7] = {0, 0, 0, 0, 0, 0, 0};
+constexpr int kDpWaves = kBlock / 64;
+
+// Per ghost cell g (over the level's nf cells in pool order), the category
+// in each of the seven tables (cat[m * n + g]), the refusals of sel56 and
+// lean, which tables have a copy with a post-operation, and per workgroup
+// the cells of every (table, category) (bcnt[(5 m + k) * nblk + block],
+// counted by wave ballots).
+__global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, long n, long nf, int nblk,
+                                                      uint8_t* __restrict__ cat, int* __restrict__ err,
+                                                      int* __restrict__ anypost, int* __restrict__ bcnt) {
+  __shared__ int wc[kDpWaves][35];
+  const long f = (long)blockIdx.x * kBlock + threadIdx.x;
+  uint8_t out[7] = {0, 0, 0, 0, 0, 0, 0};
+  int a = 0, d3[3] = {0, 0, 0};
+  const long g = f < nf ? dp_cell(A, f, a, d3[0], d3[1], d3[2]) : -1;
+  if (g >= 0) {
+    const int pid = A.patches[a];
+    const DPatch D = A.pt[pid];
     DExpr e[3];
     bool ok[3];
     for (int p = 0; p < 3; p++) {
@@ -4922,47 +4981,42 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
       ok[p] = (e[p].meta & kDxValid) != 0;
       if (ok[p]) out[p] = dp_cat(e[p]);
     }
+    const int outside = n_outside(D, A.nd, d3[0], d3[1], d3[2]);
     // map[5] and map[6]: colour-1 face ghosts of SetGhost(level, 0), map[6]
     // without the copies a correction leaves exact
-    if (ok[0]) {
+    if (ok[0] && outside == 1 && ((d3[0] + d3[1] + d3[2]) & 1) == 1) {
       const DExpr& x = e[0];
-      const DPatch D = A.pt[x.dpid];
-      const int dst = D.qoff + (int)(g - A.base[A.pos[x.dpid]]);
-      int d3[3];
-      qdecode(D, dst, d3[0], d3[1], d3[2]);
-      if (n_outside(D, A.nd, d3[0], d3[1], d3[2]) == 1 && ((d3[0] + d3[1] + d3[2]) & 1) == 1) {
-        if (dx_early(x.meta)) {
-          atomicOr(err, kDpSel5);
-        } else {
-          bool exact = dx_kind(x.meta) == 0 && dx_post(x.meta) == 0;
-          if (exact) {
-            const DPatch S = A.pt[x.spid];
-            int s3[3];
-            qdecode(S, x.src, s3[0], s3[1], s3[2]);
-            exact = n_outside(S, A.nd, s3[0], s3[1], s3[2]) == 0;
-            for (int a = 0; a < A.nd && exact; a++) {
-              const int per = M.per[a];
-              const int ds = (M.mglo[3 * x.spid + a] + s3[a]) - (M.mglo[3 * x.dpid + a] + d3[a]);
-              exact = ((ds % per) + per) % per == 0;
-            }
-            for (int corner = 0; corner < (1 << A.nd) && exact; corner++) {
-              long idx = 0;
-              for (int a = A.nd - 1; a >= 0 && exact; a--) {
-                const int gf = M.mglo[3 * x.dpid + a] + d3[a] - 1;
-                const int lo = (gf & 1) ? (gf + 1) / 2 - 1 : gf / 2;
-                int xx = lo + ((corner >> a) & 1) - 1;
-                if (xx < 0 || xx >= M.cper[a]) {
-                  if (!M.cwrap[a]) exact = false;
-                  xx = ((xx % M.cper[a]) + M.cper[a]) % M.cper[a];
-                }
-                idx = idx * M.cper[a] + xx;
-              }
-              exact = exact && ((M.occ[idx >> 5] >> (idx & 31)) & 1u);
-            }
+      if (dx_early(x.meta)) {
+        atomicOr(err, kDpSel5);
+      } else {
+        bool exact = dx_kind(x.meta) == 0 && dx_post(x.meta) == 0;
+        if (exact) {
+          const DPatch S = A.pt[x.spid];
+          int s3[3];
+          qdecode(S, x.src, s3[0], s3[1], s3[2]);
+          exact = n_outside(S, A.nd, s3[0], s3[1], s3[2]) == 0;
+          for (int b = 0; b < A.nd && exact; b++) {
+            const int per = M.per[b];
+            const int ds = (M.mglo[3 * x.spid + b] + s3[b]) - (M.mglo[3 * pid + b] + d3[b]);
+            exact = ((ds % per) + per) % per == 0;
           }
-          out[5] = out[0];
-          out[6] = exact ? 0 : out[0];
+          for (int corner = 0; corner < (1 << A.nd) && exact; corner++) {
+            long idx = 0;
+            for (int b = A.nd - 1; b >= 0 && exact; b--) {
+              const int gf = M.mglo[3 * pid + b] + d3[b] - 1;
+              const int lo = (gf & 1) ? (gf + 1) / 2 - 1 : gf / 2;
+              int xx = lo + ((corner >> b) & 1) - 1;
+              if (xx < 0 || xx >= M.cper[b]) {
+                if (!M.cwrap[b]) exact = false;
+                xx = ((xx % M.cper[b]) + M.cper[b]) % M.cper[b];
+              }
+              idx = idx * M.cper[b] + xx;
+            }
+            exact = exact && ((M.occ[idx >> 5] >> (idx & 31)) & 1u);
+          }
         }
+        out[5] = out[0];
+        out[6] = exact ? 0 : out[0];
       }
     }
     // map[2 + p], p = 1, 2: what a red-black sweep sequence needs
@@ -4971,19 +5025,15 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
       const DExpr& x = e[p];
       const int col = p - 1;
       const bool have_o = ok[3 - p];
-      const DPatch D = A.pt[x.dpid];
-      const int dst = D.qoff + (int)(g - A.base[A.pos[x.dpid]]);
-      int i, j, k;
-      qdecode(D, dst, i, j, k);
       bool need;
-      if (n_outside(D, A.nd, i, j, k) != 1) {
+      if (outside != 1) {
         if (!have_o && dx_early(x.meta)) {
           atomicOr(err, kDpLean);
           continue;
         }
         need = !have_o;
       } else {
-        need = ((i + j + k) & 1) == col;
+        need = ((d3[0] + d3[1] + d3[2]) & 1) == col;
         if (!need) {
           int si, sj, sk;
           if (dx_kind(x.meta) == 0) {
@@ -5008,56 +5058,87 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
     int post = 0;
     for (int m = 0; m < 7; m++) {
       cat[m * n + g] = out[m];
-      if (out[m]) atomicAdd(&cnt[m][out[m]], 1);
-    }
-    for (int m = 0; m < 7; m++) {
-      const int p = m < 3 ? m : (m == 3 ? 1 : (m == 4 ? 2 : 0));
-      if (out[m] && out[m] <= 2 && dx_post(e[p].meta)) post |= 1 << m;
+      if (out[m] && out[m] <= 2 && dx_post(e[dp_call(m)].meta)) post |= 1 << m;
     }
     if (post) atomicOr(anypost, post);
   }
+  // counts per (table, category): one ballot each, summed over the waves
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int m = 0; m < 7; m++)
+    for (int k = 1; k <= 4; k++) {
+      const unsigned long long b = __ballot(out[m] == k);
+      if (lane == 0) wc[wave][5 * m + k] = __popcll(b);
+    }
   __syncthreads();
   if (threadIdx.x < 35) {
-    const int m = threadIdx.x / 5, k = threadIdx.x % 5;
-    if (k && cnt[m][k]) atomicAdd(&counts[m * 5 + k], (unsigned long long)cnt[m][k]);
+    int s = 0;
+    if (threadIdx.x % 5)
+      for (int w = 0; w < kDpWaves; w++) s += wc[w][threadIdx.x];
+    bcnt[(long)threadIdx.x * nblk + blockIdx.x] = s;
   }
 }
 
-// Running counts of one table: copies in the low 32 bits, interpolation
-// rules in the high 32, for the rules that read ghosts from before the
-// call (EARLY) or the others.
-template <bool EARLY>
-struct DpCount {
-  __host__ __device__ unsigned long long operator()(const uint8_t& c) const {
-    return c == (EARLY ? 1 : 2) ? 1ull : (c == (EARLY ? 3 : 4) ? (1ull << 32) : 0ull);
-  }
+// Start of every (table, category) bin in the scanned counts and the total.
+__global__ void k_dp_bins(const int* __restrict__ scan, const int* __restrict__ bcnt, int nblk,
+                          int* __restrict__ start) {
+  const int t = threadIdx.x;
+  if (t < 35) start[t] = scan[(long)t * nblk];
+  if (t == 35) start[35] = scan[35L * nblk - 1] + bcnt[35L * nblk - 1];
+}
+
+struct DpOut {
+  int4* cr[7];
+  uint16_t* cpost[7];
+  GRule* ir[7];
+  int nc1[7], ni1[7];
 };
 
-// The rules of one table at their compacted positions.
-__global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, const DExpr* __restrict__ ex,
-                                                    const uint8_t* __restrict__ cat,
-                                                    const unsigned long long* __restrict__ offe,
-                                                    const unsigned long long* __restrict__ offl, long n, int nc1,
-                                                    int ni1, const GRule* __restrict__ rules, int4* __restrict__ cr,
-                                                    uint16_t* __restrict__ cpost, GRule* __restrict__ ir) {
-  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (g >= n) return;
-  const int c = cat[g];
-  if (!c) return;
-  const DExpr e = ex[g];
-  const DPatch D = A.pt[e.dpid];
-  const int dst = D.qoff + (int)(g - A.base[A.pos[e.dpid]]);
-  const int post = dx_post(e.meta);
-  if (c <= 2) {
-    const long at = c == 1 ? (long)(offe[g] & 0xffffffffu) : nc1 + (long)(offl[g] & 0xffffffffu);
-    cr[at] = make_int4(dst, e.src, D.qvs, A.pt[e.spid].qvs);
-    if (cpost) cpost[at] = (uint16_t)post;
-  } else {
-    const long at = c == 3 ? (long)(offe[g] >> 32) : ni1 + (long)(offl[g] >> 32);
-    GRule r = rules[e.src];
-    r.dst = dst;
-    r.kind = (r.kind & 0xff) | (post << 8);
-    ir[at] = r;
+// The rules of the seven tables at their compacted positions: the
+// workgroup's start in its bin, the waves before it, the lanes before it
+// (the same workgroups over the pool-order cells as k_dp_select).
+__global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, long n, long nf, int nblk,
+                                                    const uint8_t* __restrict__ cat, const int* __restrict__ scan,
+                                                    const int* __restrict__ start, DpOut O) {
+  __shared__ int wc[kDpWaves][35];
+  const long f = (long)blockIdx.x * kBlock + threadIdx.x;
+  int a = 0, d3[3] = {0, 0, 0};
+  const long g = f < nf ? dp_cell(A, f, a, d3[0], d3[1], d3[2]) : -1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint8_t c[7];
+  int rank[7];
+  for (int m = 0; m < 7; m++) {
+    c[m] = g >= 0 ? cat[m * n + g] : 0;
+    rank[m] = 0;
+    for (int k = 1; k <= 4; k++) {
+      const unsigned long long b = __ballot(c[m] == k);
+      if (c[m] == k) rank[m] = __popcll(b & below);
+      if (lane == 0) wc[wave][5 * m + k] = __popcll(b);
+    }
+  }
+  __syncthreads();
+  if (g < 0) return;
+  const int pid = A.patches[a];
+  const int dst = qidx(A.pt[pid], d3[0], d3[1], d3[2]);
+  for (int m = 0; m < 7; m++) {
+    const int k = c[m];
+    if (!k) continue;
+    const int bin = 5 * m + k;
+    int at = scan[(long)bin * nblk + blockIdx.x] - start[bin] + rank[m];
+    for (int w = 0; w < wave; w++) at += wc[w][bin];
+    const DExpr e = M.ex[dp_call(m)][g];
+    const int post = dx_post(e.meta);
+    if (k <= 2) {
+      if (k == 2) at += O.nc1[m];
+      O.cr[m][at] = make_int4(dst, e.src, A.pt[pid].qvs, A.pt[e.spid].qvs);
+      if (O.cpost[m]) O.cpost[m][at] = (uint16_t)post;
+    } else {
+      if (k == 4) at += O.ni1[m];
+      GRule r = M.rules[e.src];
+      r.dst = dst;
+      r.kind = (r.kind & 0xff) | (post << 8);
+      O.ir[m][at] = r;
+    }
   }
 }
 
@@ -5077,12 +5158,6 @@ __global__ __launch_bounds__(kBlock) void k_dp_occupy(LevelArgs L, const int* __
   atomicOr(&bits[b >> 5], 1u << (b & 31));
 }
 
-// Workspace of the device planner, kept by the context and grown as needed.
-struct DPlanWs {
-  char* p = nullptr;
-  size_t cap = 0;
-};
-
 int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string& why) {
   L.maps_ok = false;
   const int np = (int)L.patches.size(), nd = c->p.ndims;
@@ -5094,22 +5169,26 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     why = "too many ghost-fill stages";
     return BSAM_OK;
   }
-  // dense numbering of the level's q cells (variable 0), patch after patch
+  // ghost cells of the level, patch after patch
   std::vector<int> pos(c->npatch, -1), lq(np);
-  std::vector<long> base(np);
-  long n = 0;
+  std::vector<long> gbase(np + 1), fbase(np + 1);
+  long n = 0, nf = 0;
   for (int a = 0; a < np; a++) {
     const DPatch& P = c->hp[L.patches[a]];
     pos[L.patches[a]] = a;
-    base[a] = n;
+    gbase[a] = n;
+    fbase[a] = nf;
+    nf += P.qvs;
     lq[a] = P.qoff;
     if (a > 0 && lq[a] <= lq[a - 1]) {
       why = "level patches not in ascending pool order";
       return BSAM_OK;
     }
-    n += P.qvs;
+    n += shell_count(P, nd);
   }
-  if (n >= (1L << 31)) {
+  gbase[np] = n;
+  fbase[np] = nf;
+  if (n >= (1L << 31) / 8 || nf >= (1L << 31)) {
     why = "level too large for the device planner";
     return BSAM_OK;
   }
@@ -5143,48 +5222,48 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     return BSAM_OK;
   }
   const size_t owords = (size_t)((ocells + 31) / 32);
+  const int nblk = std::max(1, nblocks(nf));
   // workspace: three expression arrays, two claim arrays, the categories,
-  // two running-count arrays, then the small tables
+  // the per-workgroup counts and their scan, then the small tables
   size_t scan_bytes = 0;
-  {
-    hipcub::TransformInputIterator<unsigned long long, DpCount<true>, const uint8_t*> it(nullptr, DpCount<true>());
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, it, (unsigned long long*)nullptr, (int)n, c->s));
-  }
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int*)nullptr, (int*)nullptr, 35 * nblk, c->s));
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   size_t boxints = 0;
   for (auto& v : bpre) boxints += v.size();
   const size_t o_ex = 0, o_ci = o_ex + al(3 * n * sizeof(DExpr)), o_cb = o_ci + al(n * sizeof(int)),
-               o_cat = o_cb + al(n * sizeof(int)), o_offe = o_cat + al(7 * n),
-               o_offl = o_offe + al(n * sizeof(unsigned long long)),
-               o_pos = o_offl + al(n * sizeof(unsigned long long)), o_base = o_pos + al(c->npatch * sizeof(int)),
-               o_lq = o_base + al(np * sizeof(long)), o_bpre = o_lq + al(np * sizeof(int)),
-               o_mglo = o_bpre + al(boxints * sizeof(int)), o_occ = o_mglo + al(3 * c->npatch * sizeof(int)),
-               o_small = o_occ + al(owords * sizeof(uint32_t)), o_scan = o_small + 512, total = o_scan + al(scan_bytes);
+               o_cat = o_cb + al(n * sizeof(int)), o_bc = o_cat + al(7 * n),
+               o_bs = o_bc + al(35 * (size_t)nblk * sizeof(int)), o_pos = o_bs + al(35 * (size_t)nblk * sizeof(int)),
+               o_gb = o_pos + al(c->npatch * sizeof(int)), o_fb = o_gb + al((np + 1) * sizeof(long)),
+               o_lq = o_fb + al((np + 1) * sizeof(long)),
+               o_bpre = o_lq + al(np * sizeof(int)), o_mglo = o_bpre + al(boxints * sizeof(int)),
+               o_occ = o_mglo + al(3 * c->npatch * sizeof(int)), o_small = o_occ + al(owords * sizeof(uint32_t)),
+               o_scan = o_small + 512, total = o_scan + al(scan_bytes);
   char* W = plan_workspace(c, total);
   if (!W) return set_err(BSAM_ENOMEM, "ghost-table planner workspace of %zu bytes", total);
   DExpr* ex = reinterpret_cast<DExpr*>(W + o_ex);
   int* claim_i = reinterpret_cast<int*>(W + o_ci);
   int* claim_b = reinterpret_cast<int*>(W + o_cb);
   uint8_t* cat = reinterpret_cast<uint8_t*>(W + o_cat);
-  auto* offe = reinterpret_cast<unsigned long long*>(W + o_offe);
-  auto* offl = reinterpret_cast<unsigned long long*>(W + o_offl);
+  int* bcnt = reinterpret_cast<int*>(W + o_bc);
+  int* bscan = reinterpret_cast<int*>(W + o_bs);
   int* d_pos = reinterpret_cast<int*>(W + o_pos);
-  long* d_base = reinterpret_cast<long*>(W + o_base);
+  long* d_gb = reinterpret_cast<long*>(W + o_gb);
+  long* d_fb = reinterpret_cast<long*>(W + o_fb);
   int* d_lq = reinterpret_cast<int*>(W + o_lq);
   int* d_bpre = reinterpret_cast<int*>(W + o_bpre);
   int* d_mglo = reinterpret_cast<int*>(W + o_mglo);
   uint32_t* d_occ = reinterpret_cast<uint32_t*>(W + o_occ);
   int* d_err = reinterpret_cast<int*>(W + o_small);
   int* d_post = d_err + 1;
-  auto* d_cnt = reinterpret_cast<unsigned long long*>(W + o_small + 64);
-  void* d_scan = W + o_scan;
-  // the small tables in one staging copy; the bit map and the counters
-  // start at zero
+  int* d_start = reinterpret_cast<int*>(W + o_small + 64);
+  // the small tables in one staging copy; the bit map and the flags start
+  // at zero
   {
     std::vector<char> h(o_occ - o_pos, 0);
     auto put = [&](size_t off, const void* src, size_t bytes) { memcpy(h.data() + (off - o_pos), src, bytes); };
     put(o_pos, pos.data(), pos.size() * sizeof(int));
-    put(o_base, base.data(), base.size() * sizeof(long));
+    put(o_gb, gbase.data(), gbase.size() * sizeof(long));
+    put(o_fb, fbase.data(), fbase.size() * sizeof(long));
     put(o_lq, lq.data(), lq.size() * sizeof(int));
     size_t k = 0;
     for (auto& v : bpre) {
@@ -5201,7 +5280,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(c->s));
   }
-  DPlanArgs A{c->d_pt, L.d_patches, d_pos, d_base, d_lq, np, nd};
+  DPlanArgs A{c->d_pt, L.d_patches, d_pos, d_gb, d_fb, d_lq, np, nd};
   // the parent level's cells
   {
     Level& Pl = c->L(L.level - 1);
@@ -5211,12 +5290,11 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
       LAUNCH_CHECK();
     }
   }
-  const int nbn = nblocks(n);
   for (int p = 0; p < 3; p++) {
     DExpr* E = ex + (size_t)p * n;
     k_dp_fill<<<nblocks(4 * n), kBlock, 0, c->s>>>(reinterpret_cast<int*>(E), 4 * n, 0);
-    k_dp_fill<<<nbn, kBlock, 0, c->s>>>(claim_i, n, -1);
-    k_dp_fill<<<nbn, kBlock, 0, c->s>>>(claim_b, n, -1);
+    k_dp_fill<<<nblocks(n), kBlock, 0, c->s>>>(claim_i, n, -1);
+    k_dp_fill<<<nblocks(n), kBlock, 0, c->s>>>(claim_b, n, -1);
     LAUNCH_CHECK();
     const int s0 = (p == 2) ? 1 : 0, s1 = (p == 1) ? 1 : 2;
     const int r0 = L.roff[s0][0], r1 = L.roff[s1][np];
@@ -5235,7 +5313,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
       LAUNCH_CHECK();
       k += bpre[st].size();
     }
-    k_dp_check<<<nbn, kBlock, 0, c->s>>>(A, E, n, d_err);
+    k_dp_check<<<nblocks(n), kBlock, 0, c->s>>>(A, E, n, d_err);
     LAUNCH_CHECK();
   }
   DPlanMaps M{};
@@ -5244,14 +5322,18 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   M.mglo = d_mglo;
   M.occ = d_occ;
   for (int a = 0; a < 3; a++) M.per[a] = per[a], M.cper[a] = cper[a], M.cwrap[a] = cwrap[a];
-  k_dp_select<<<nbn, kBlock, 0, c->s>>>(A, M, n, cat, d_err, d_post, d_cnt);
+  k_dp_select<<<nblk, kBlock, 0, c->s>>>(A, M, n, nf, nblk, cat, d_err, d_post, bcnt);
+  LAUNCH_CHECK();
+  size_t sb = scan_bytes;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(W + o_scan, sb, bcnt, bscan, 35 * nblk, c->s));
+  k_dp_bins<<<1, 64, 0, c->s>>>(bscan, bcnt, nblk, d_start);
   LAUNCH_CHECK();
   struct {
     int err, post;
     int pad[14];
-    unsigned long long cnt[35];
+    int start[36];
   } h{};
-  HIPCHK(hipMemcpyAsync(&h, W + o_small, 64 + 35 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(&h, W + o_small, 64 + 36 * sizeof(int), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   if (h.err) {
     static const char* what[8] = {"a ghost-fill stage writes an interior cell",
@@ -5272,13 +5354,14 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   // the seven tables in one pooled block
   size_t tbytes = 0;
   size_t at[7][3];
+  DpOut O{};
   for (int m = 0; m < 7; m++) {
     DevMap& D = L.gmap[m];
-    const unsigned long long* k = h.cnt + 5 * m;
-    D.nc1 = (int)k[1];
-    D.nc = (int)(k[1] + k[2]);
-    D.ni1 = (int)k[3];
-    D.ni = (int)(k[3] + k[4]);
+    auto cnt = [&](int k) { return h.start[5 * m + k + 1] - h.start[5 * m + k]; };
+    D.nc1 = cnt(1);
+    D.nc = cnt(1) + cnt(2);
+    D.ni1 = cnt(3);
+    D.ni = cnt(3) + cnt(4);
     at[m][0] = tbytes;
     tbytes = al(tbytes + (size_t)D.nc * sizeof(int4));
     at[m][1] = tbytes;
@@ -5295,22 +5378,14 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     D.cr = D.nc ? reinterpret_cast<int4*>(T + at[m][0]) : nullptr;
     D.cpost = (D.nc && (h.post & (1 << m))) ? reinterpret_cast<uint16_t*>(T + at[m][1]) : nullptr;
     D.ir = D.ni ? reinterpret_cast<GRule*>(T + at[m][2]) : nullptr;
+    O.cr[m] = D.cr;
+    O.cpost[m] = D.cpost;
+    O.ir[m] = D.ir;
+    O.nc1[m] = D.nc1;
+    O.ni1[m] = D.ni1;
   }
-  for (int m = 0; m < 7; m++) {
-    DevMap& D = L.gmap[m];
-    if (!D.nc && !D.ni) continue;
-    const int p = m < 3 ? m : (m == 3 ? 1 : (m == 4 ? 2 : 0));
-    const uint8_t* cm = cat + (size_t)m * n;
-    size_t sb = scan_bytes;
-    hipcub::TransformInputIterator<unsigned long long, DpCount<true>, const uint8_t*> ie(cm, DpCount<true>());
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_scan, sb, ie, offe, (int)n, c->s));
-    sb = scan_bytes;
-    hipcub::TransformInputIterator<unsigned long long, DpCount<false>, const uint8_t*> il(cm, DpCount<false>());
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_scan, sb, il, offl, (int)n, c->s));
-    k_dp_emit<<<nbn, kBlock, 0, c->s>>>(A, ex + (size_t)p * n, cm, offe, offl, n, D.nc1, D.ni1, L.d_rules, D.cr,
-                                        D.cpost, D.ir);
-    LAUNCH_CHECK();
-  }
+  k_dp_emit<<<nblk, kBlock, 0, c->s>>>(A, M, n, nf, nblk, cat, bscan, d_start, O);
+  LAUNCH_CHECK();
   HIPCHK(hipStreamSynchronize(c->s));
   L.maps_ok = true;
   return BSAM_OK;
