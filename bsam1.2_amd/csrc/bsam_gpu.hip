@@ -3192,6 +3192,9 @@ struct bsam_ctx {
   size_t tagbits_cap = 0;
   int* d_tagglo = nullptr;
   size_t tagglo_cap = 0;
+  std::vector<bsam_patch_desc> amr_out;  // the new patches of a level (bsam_amr)
+  int32_t* h_flags = nullptr;             // pinned host copy of a level's tags (bsam_amr)
+  size_t h_flags_cap = 0;
   // this device's copy of the file-scope `scratch` array (its device
   // address, looked up once in bsam_create with the device current)
   char* scratch = nullptr;
@@ -3529,6 +3532,7 @@ extern "C" int32_t bsam_destroy(bsam_ctx* c) {
   hipFree(c->d_tagglo);
   hipFree(c->dplan);
   if (c->h_norm) hipHostFree(c->h_norm);
+  if (c->h_flags) hipHostFree(c->h_flags);
   for (auto& k : c->ks)
     for (hipEvent_t e : k.ev) hipEventDestroy(e);
   if (c->s) hipStreamDestroy(c->s);
@@ -3888,11 +3892,15 @@ static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
 // with the number of neighbours rather than with the square of the patch
 // count.  Pairs whose boxes do not meet produce no transfer boxes, so the
 // boxes come out as from the all-pairs loop, in the same order.
-static std::vector<std::vector<int>> neighbour_candidates(const bsam_ctx* c, const std::vector<int>& ps,
-                                                          const std::vector<std::array<int, 3>>& offs) {
+// (Each candidate b > a of patch a comes with the shifts under which the
+// two ghost-extended boxes may meet: bit 0 for none, bit 1 + q for shift q,
+// q = 2 * offset + (1 for +offset, 0 for -offset).)
+using Cand = std::pair<int, uint32_t>;
+static std::vector<std::vector<Cand>> neighbour_candidates(const bsam_ctx* c, const std::vector<int>& ps,
+                                                           const std::vector<std::array<int, 3>>& offs) {
   const int nd = c->p.ndims;
   const size_t np = ps.size();
-  std::vector<std::vector<int>> cand(np);
+  std::vector<std::vector<Cand>> cand(np);
   if (np < 2) return cand;
   // bins of the largest ghost-extended patch extent, over the bounding box
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, ext = 1;
@@ -3929,30 +3937,41 @@ static std::vector<std::vector<int>> neighbour_candidates(const bsam_ctx* c, con
       for (int y = b0[1]; y <= b1[1]; y++)
         for (int x = b0[0]; x <= b1[0]; x++) bins[((long)z * nb[1] + y) * nb[0] + x].push_back((int)a);
   }
-  std::vector<int> seen(np, -1);
-  for (size_t a = 0; a < np; a++) {
-    const bsam_patch_desc& d = c->desc[ps[a]];
-    // a's box shifted by -off meets b's box exactly when b's box shifted by
-    // +off meets a's: both polarities of every offset, and no offset
-    for (int q = -1; q < 2 * (int)offs.size(); q++) {
-      int box[3][2], b0[3], b1[3];
-      for (int n = 0; n < 3; n++) {
-        const int sh = q < 0 ? 0 : (q & 1 ? 1 : -1) * offs[q >> 1][n];
-        box[n][0] = n < nd ? d.mglobal[n][0] - 1 - sh : 0;
-        box[n][1] = n < nd ? d.mglobal[n][1] + 1 - sh : 0;
-      }
-      if (!bin_range(box, b0, b1)) continue;
-      for (int z = b0[2]; z <= b1[2]; z++)
-        for (int y = b0[1]; y <= b1[1]; y++)
-          for (int x = b0[0]; x <= b1[0]; x++)
-            for (int b : bins[((long)z * nb[1] + y) * nb[0] + x])
-              if (b > (int)a && seen[b] != (int)a) {
-                seen[b] = (int)a;
-                cand[a].push_back(b);
+  // patches on host threads, each thread with its own marks
+  const int nth = (int)std::max<size_t>(1, std::min<size_t>(np / 64 + 1, host_threads()));
+  run_threads(nth, [&](int t) {
+    std::vector<int> seen(np, -1);
+    std::vector<uint32_t> mask(np, 0);
+    for (size_t a = t; a < np; a += nth) {
+      const bsam_patch_desc& d = c->desc[ps[a]];
+      std::vector<int> found;
+      // a's box shifted by -off meets b's box exactly when b's box shifted
+      // by +off meets a's: both polarities of every offset, and no offset
+      for (int q = -1; q < 2 * (int)offs.size(); q++) {
+        int box[3][2], b0[3], b1[3];
+        for (int n = 0; n < 3; n++) {
+          const int sh = q < 0 ? 0 : (q & 1 ? 1 : -1) * offs[q >> 1][n];
+          box[n][0] = n < nd ? d.mglobal[n][0] - 1 - sh : 0;
+          box[n][1] = n < nd ? d.mglobal[n][1] + 1 - sh : 0;
+        }
+        if (!bin_range(box, b0, b1)) continue;
+        for (int z = b0[2]; z <= b1[2]; z++)
+          for (int y = b0[1]; y <= b1[1]; y++)
+            for (int x = b0[0]; x <= b1[0]; x++)
+              for (int b : bins[((long)z * nb[1] + y) * nb[0] + x]) {
+                if (b <= (int)a) continue;
+                if (seen[b] != (int)a) {
+                  seen[b] = (int)a;
+                  mask[b] = 0;
+                  found.push_back(b);
+                }
+                mask[b] |= 1u << (q + 1);
               }
+      }
+      std::sort(found.begin(), found.end());
+      for (int b : found) cand[a].push_back({b, mask[b]});
     }
-    std::sort(cand[a].begin(), cand[a].end());
-  }
+  });
   return cand;
 }
 
@@ -3973,9 +3992,9 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   periodic_offsets(c, L.level, offs, periodic);
   std::vector<BoxDesc> sib, px, py, pz, pp, phys[3];
   const auto& ps = L.patches;
-  const std::vector<std::vector<int>> cand =
+  const std::vector<std::vector<Cand>> cand =
       L.level > 0 ? neighbour_candidates(c, ps, periodic ? offs : std::vector<std::array<int, 3>>{})
-                  : std::vector<std::vector<int>>();
+                  : std::vector<std::vector<Cand>>();
   // the pairs of each patch on the host's threads, joined in patch order
   auto per_patch = [&](auto&& boxes_of, std::vector<BoxDesc>& out) {
     std::vector<std::vector<BoxDesc>> part(ps.size());
@@ -3987,7 +4006,8 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
   };
   if (L.level > 0)
     per_patch([&](size_t a, std::vector<BoxDesc>& v) {
-      for (int b : cand[a]) ghost_overlap_boxes(c, ps[a], ps[b], nullptr, kCopy, v);
+      for (const Cand& b : cand[a])
+        if (b.second & 1u) ghost_overlap_boxes(c, ps[a], ps[b.first], nullptr, kCopy, v);
     }, sib);
   bool all_wrap = periodic && ps.size() == 1;
   if (periodic) {
@@ -4017,11 +4037,14 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
     }
     if (L.level > 0)
       per_patch([&](size_t a, std::vector<BoxDesc>& v) {
-        for (int b : cand[a])
+        for (const Cand& b : cand[a])
           for (int pol = -1; pol <= 1; pol += 2)
-            for (auto& o : offs) {
+            for (size_t oi = 0; oi < offs.size(); oi++) {
+              // only the shifts under which the boxes may meet
+              if (!((b.second >> (1 + 2 * oi + (pol > 0 ? 1 : 0))) & 1u)) continue;
+              const auto& o = offs[oi];
               int off[3] = {pol * o[0], pol * o[1], pol * o[2]};
-              ghost_overlap_boxes(c, ps[a], ps[b], off, kCopy, v);
+              ghost_overlap_boxes(c, ps[a], ps[b.first], off, kCopy, v);
             }
       }, pp);
   }
@@ -4777,7 +4800,8 @@ struct DPlanArgs {
   const int* __restrict__ patches;  // the level's descriptor indices
   const int* __restrict__ pos;      // descriptor index -> position in the level, or -1
   const long* __restrict__ gbase;   // first ghost index of each level patch (np + 1 entries)
-  const long* __restrict__ fbase;   // first cell (variable 0, pool order) of each level patch (np + 1)
+  const int* __restrict__ bp;       // workgroup -> level position, over the pool-order cells
+  const int* __restrict__ bfirst;   // first workgroup of each level patch there
   const int* __restrict__ lq;       // pool offset of each level patch (ascending)
   int np, nd;
 };
@@ -4794,20 +4818,18 @@ __device__ __forceinline__ int dp_patch_at(const DPlanArgs& A, int off) {
   }
   return lo;
 }
-// Cell f of the level in pool order (variable 0, patch after patch, each
-// patch in its colour-split order): its patch (level position), storage
-// indices and ghost index (-1 for an interior cell).  The tables list
-// their rules in this order, as the host planner does.
-__device__ __forceinline__ long dp_cell(const DPlanArgs& A, long f, int& a, int& i, int& j, int& k) {
-  int lo = 0, hi = A.np;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (A.fbase[mid] <= f) lo = mid;
-    else hi = mid;
-  }
-  a = lo;
+// The level's cells in pool order (variable 0, patch after patch, each
+// patch in its colour-split order), each patch's from a workgroup boundary:
+// thread t of workgroup b is the patch's cell (b - bfirst) * kBlock + t.
+// Returns its ghost index (-1 for an interior cell or none), its patch
+// (level position) and storage indices.  The tables list their rules in
+// this order, as the host planner does.
+__device__ __forceinline__ long dp_cell(const DPlanArgs& A, int& a, int& i, int& j, int& k) {
+  a = A.bp[blockIdx.x];
   const DPatch& P = A.pt[A.patches[a]];
-  qdecode(P, P.qoff + (int)(f - A.fbase[a]), i, j, k);
+  const int local = (blockIdx.x - A.bfirst[a]) * kBlock + threadIdx.x;
+  if (local >= P.qvs) return -1;
+  qdecode(P, P.qoff + local, i, j, k);
   // (the padding slot of an odd row length is no cell)
   if (i >= P.nq[0] || !n_outside(P, A.nd, i, j, k)) return -1;
   return A.gbase[a] + shell_index(P, A.nd, i, j, k);
@@ -4963,14 +4985,13 @@ constexpr int kDpWaves = kBlock / 64;
 // lean, which tables have a copy with a post-operation, and per workgroup
 // the cells of every (table, category) (bcnt[(5 m + k) * nblk + block],
 // counted by wave ballots).
-__global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, long n, long nf, int nblk,
+__global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, long n, int nblk,
                                                       uint8_t* __restrict__ cat, int* __restrict__ err,
                                                       int* __restrict__ anypost, int* __restrict__ bcnt) {
   __shared__ int wc[kDpWaves][35];
-  const long f = (long)blockIdx.x * kBlock + threadIdx.x;
   uint8_t out[7] = {0, 0, 0, 0, 0, 0, 0};
   int a = 0, d3[3] = {0, 0, 0};
-  const long g = f < nf ? dp_cell(A, f, a, d3[0], d3[1], d3[2]) : -1;
+  const long g = dp_cell(A, a, d3[0], d3[1], d3[2]);
   if (g >= 0) {
     const int pid = A.patches[a];
     const DPatch D = A.pt[pid];
@@ -5096,13 +5117,12 @@ struct DpOut {
 // The rules of the seven tables at their compacted positions: the
 // workgroup's start in its bin, the waves before it, the lanes before it
 // (the same workgroups over the pool-order cells as k_dp_select).
-__global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, long n, long nf, int nblk,
+__global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, long n, int nblk,
                                                     const uint8_t* __restrict__ cat, const int* __restrict__ scan,
                                                     const int* __restrict__ start, DpOut O) {
   __shared__ int wc[kDpWaves][35];
-  const long f = (long)blockIdx.x * kBlock + threadIdx.x;
   int a = 0, d3[3] = {0, 0, 0};
-  const long g = f < nf ? dp_cell(A, f, a, d3[0], d3[1], d3[2]) : -1;
+  const long g = dp_cell(A, a, d3[0], d3[1], d3[2]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
   uint8_t c[7];
@@ -5171,14 +5191,15 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   }
   // ghost cells of the level, patch after patch
   std::vector<int> pos(c->npatch, -1), lq(np);
-  std::vector<long> gbase(np + 1), fbase(np + 1);
-  long n = 0, nf = 0;
+  std::vector<long> gbase(np + 1);
+  std::vector<int> bfirst(np), bp;
+  long n = 0;
   for (int a = 0; a < np; a++) {
     const DPatch& P = c->hp[L.patches[a]];
     pos[L.patches[a]] = a;
     gbase[a] = n;
-    fbase[a] = nf;
-    nf += P.qvs;
+    bfirst[a] = (int)bp.size();
+    bp.insert(bp.end(), nblocks(P.qvs), a);
     lq[a] = P.qoff;
     if (a > 0 && lq[a] <= lq[a - 1]) {
       why = "level patches not in ascending pool order";
@@ -5187,8 +5208,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     n += shell_count(P, nd);
   }
   gbase[np] = n;
-  fbase[np] = nf;
-  if (n >= (1L << 31) / 8 || nf >= (1L << 31)) {
+  if (n >= (1L << 31) / 8) {
     why = "level too large for the device planner";
     return BSAM_OK;
   }
@@ -5222,7 +5242,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     return BSAM_OK;
   }
   const size_t owords = (size_t)((ocells + 31) / 32);
-  const int nblk = std::max(1, nblocks(nf));
+  const int nblk = std::max<int>(1, (int)bp.size());
   // workspace: three expression arrays, two claim arrays, the categories,
   // the per-workgroup counts and their scan, then the small tables
   size_t scan_bytes = 0;
@@ -5233,8 +5253,8 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   const size_t o_ex = 0, o_ci = o_ex + al(3 * n * sizeof(DExpr)), o_cb = o_ci + al(n * sizeof(int)),
                o_cat = o_cb + al(n * sizeof(int)), o_bc = o_cat + al(7 * n),
                o_bs = o_bc + al(35 * (size_t)nblk * sizeof(int)), o_pos = o_bs + al(35 * (size_t)nblk * sizeof(int)),
-               o_gb = o_pos + al(c->npatch * sizeof(int)), o_fb = o_gb + al((np + 1) * sizeof(long)),
-               o_lq = o_fb + al((np + 1) * sizeof(long)),
+               o_gb = o_pos + al(c->npatch * sizeof(int)), o_bp = o_gb + al((np + 1) * sizeof(long)),
+               o_bf = o_bp + al(bp.size() * sizeof(int)), o_lq = o_bf + al(np * sizeof(int)),
                o_bpre = o_lq + al(np * sizeof(int)), o_mglo = o_bpre + al(boxints * sizeof(int)),
                o_occ = o_mglo + al(3 * c->npatch * sizeof(int)), o_small = o_occ + al(owords * sizeof(uint32_t)),
                o_scan = o_small + 512, total = o_scan + al(scan_bytes);
@@ -5248,7 +5268,8 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   int* bscan = reinterpret_cast<int*>(W + o_bs);
   int* d_pos = reinterpret_cast<int*>(W + o_pos);
   long* d_gb = reinterpret_cast<long*>(W + o_gb);
-  long* d_fb = reinterpret_cast<long*>(W + o_fb);
+  int* d_bp = reinterpret_cast<int*>(W + o_bp);
+  int* d_bf = reinterpret_cast<int*>(W + o_bf);
   int* d_lq = reinterpret_cast<int*>(W + o_lq);
   int* d_bpre = reinterpret_cast<int*>(W + o_bpre);
   int* d_mglo = reinterpret_cast<int*>(W + o_mglo);
@@ -5263,7 +5284,8 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     auto put = [&](size_t off, const void* src, size_t bytes) { memcpy(h.data() + (off - o_pos), src, bytes); };
     put(o_pos, pos.data(), pos.size() * sizeof(int));
     put(o_gb, gbase.data(), gbase.size() * sizeof(long));
-    put(o_fb, fbase.data(), fbase.size() * sizeof(long));
+    put(o_bp, bp.data(), bp.size() * sizeof(int));
+    put(o_bf, bfirst.data(), bfirst.size() * sizeof(int));
     put(o_lq, lq.data(), lq.size() * sizeof(int));
     size_t k = 0;
     for (auto& v : bpre) {
@@ -5280,7 +5302,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(c->s));
   }
-  DPlanArgs A{c->d_pt, L.d_patches, d_pos, d_gb, d_fb, d_lq, np, nd};
+  DPlanArgs A{c->d_pt, L.d_patches, d_pos, d_gb, d_bp, d_bf, d_lq, np, nd};
   // the parent level's cells
   {
     Level& Pl = c->L(L.level - 1);
@@ -5322,7 +5344,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   M.mglo = d_mglo;
   M.occ = d_occ;
   for (int a = 0; a < 3; a++) M.per[a] = per[a], M.cper[a] = cper[a], M.cwrap[a] = cwrap[a];
-  k_dp_select<<<nblk, kBlock, 0, c->s>>>(A, M, n, nf, nblk, cat, d_err, d_post, bcnt);
+  k_dp_select<<<nblk, kBlock, 0, c->s>>>(A, M, n, nblk, cat, d_err, d_post, bcnt);
   LAUNCH_CHECK();
   size_t sb = scan_bytes;
   HIPCHK(hipcub::DeviceScan::ExclusiveSum(W + o_scan, sb, bcnt, bscan, 35 * nblk, c->s));
@@ -5384,7 +5406,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     O.nc1[m] = D.nc1;
     O.ni1[m] = D.ni1;
   }
-  k_dp_emit<<<nblk, kBlock, 0, c->s>>>(A, M, n, nf, nblk, cat, bscan, d_start, O);
+  k_dp_emit<<<nblk, kBlock, 0, c->s>>>(A, M, n, nblk, cat, bscan, d_start, O);
   LAUNCH_CHECK();
   HIPCHK(hipStreamSynchronize(c->s));
   L.maps_ok = true;
@@ -6959,11 +6981,20 @@ static int32_t estimate_level_errors_in(bsam_ctx* c, int32_t level, int32_t errf
 // extent), then copied to `flags` (L.ncell entries).  *buffered tells the
 // caller whether the buffering is done (else bsam_new_level buffers).
 static int32_t amr_level_tags(bsam_ctx* c, int level, int errflagopt, double qtolerance, int ibuffer,
-                              std::vector<int32_t>& flags, bool& buffered) {
+                              int32_t*& flags, bool& buffered) {
   Level& L = c->L(level);
   const int nd = c->p.ndims;
   const long n = L.ncell;
-  flags.resize(std::max<long>(1, n));
+  // the flags come back through a pinned host buffer of the context
+  if ((size_t)std::max<long>(1, n) > c->h_flags_cap) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->h_flags) hipHostFree(c->h_flags);
+    c->h_flags = nullptr;
+    c->h_flags_cap = 0;
+    HIPCHK(hipHostMalloc(&c->h_flags, std::max<long>(1, n) * sizeof(int32_t), hipHostMallocDefault));
+    c->h_flags_cap = std::max<long>(1, n);
+  }
+  flags = c->h_flags;
   buffered = false;
   const bsam_patch_desc* root = nullptr;
   for (const bsam_patch_desc& d : c->desc)
@@ -7043,7 +7074,7 @@ static int32_t amr_level_tags(bsam_ctx* c, int level, int errflagopt, double qto
     d_flags = d_out;
     buffered = true;
   }
-  HIPCHK(hipMemcpyAsync(flags.data(), d_flags, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(flags, d_flags, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   return BSAM_OK;
 }
@@ -7475,8 +7506,10 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
   RETCHK(install(c, (int)H.size(), H.data(), minlevel, 0, nullptr, &old, c->regrid_tables != 0));
   for (int lvl = minlevel; lvl <= 0; lvl++) old.restore_shapes(lvl);
   RETCHK(carry_root_levels(c, old));
-  std::vector<int32_t> flags;
-  std::vector<bsam_patch_desc> nw(64);
+  int32_t* flags = nullptr;
+  // room for the new patches of a level: what the last pass made, with
+  // margin (a second clustering call only when a level grows past it)
+  std::vector<bsam_patch_desc>& nw = c->amr_out;
   for (int lvl = 0;; lvl++) {
     pc.next(BSAM_PHASE_AMR_TAG);
     RETCHK(bsam_set_ghost(c, lvl, 0));
@@ -7490,12 +7523,17 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
     RETCHK(amr_level_tags(c, lvl, gp->errflagopt[lvl], gp->qtolerance[lvl], gp->ibuffer[lvl], flags, buffered));
     pc.next(BSAM_PHASE_AMR_CLUSTER);
     const int ib = buffered ? 0 : gp->ibuffer[lvl];
+    {
+      const std::vector<int>* was = old.patches(lvl + 1);
+      const size_t want = 64 + 2 * (was ? was->size() : 0);
+      if (nw.size() < want) nw.resize(want);
+    }
     int32_t nn = 0;
-    int32_t r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), ib, gp->minimumgridpoints[lvl],
+    int32_t r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags, ib, gp->minimumgridpoints[lvl],
                                gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
     if (r == BSAM_EINVAL && nn > (int)nw.size()) {
       nw.resize(nn);
-      r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags.data(), ib, gp->minimumgridpoints[lvl],
+      r = bsam_new_level(nd, (int)H.size(), H.data(), lvl, flags, ib, gp->minimumgridpoints[lvl],
                          gp->desiredfillratios[lvl], (int)nw.size(), nw.data(), &nn);
     }
     RETCHK(r);

@@ -303,11 +303,14 @@ extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patc
   std::vector<long> off(lv.size() + 1, 0);
   for (size_t a = 0; a < lv.size(); a++)
     off[a + 1] = off[a] + (long)desc[lv[a]].mx[0] * desc[lv[a]].mx[1] * desc[lv[a]].mx[2];
-  std::vector<int32_t> tg(flags, flags + off.back());
+  // buffering writes into a copy of the tags; without it they are only read
+  std::vector<int32_t> tg;
+  if (ibuffer > 0) tg.assign(flags, flags + off.back());
+  int32_t* tags = ibuffer > 0 ? tg.data() : const_cast<int32_t*>(flags);
   std::vector<Tags> T(lv.size());
   for (size_t a = 0; a < lv.size(); a++) {
     for (int n = 0; n < 3; n++) T[a].mx[n] = desc[lv[a]].mx[n];
-    T[a].f = tg.data() + off[a];
+    T[a].f = tags + off[a];
   }
   const int nth = std::max(1, std::min<int>((int)lv.size(), host_threads()));
   if (ibuffer > 0) {
