@@ -3841,14 +3841,8 @@ static size_t host_threads() {
 // started for run on the calling thread.
 template <typename F>
 static void run_threads(int n, F&& fn) {
-  std::vector<std::thread> th;
-  int started = 0;
-  try {
-    for (; started < n; started++) th.emplace_back([&fn, started]() { fn(started); });
-  } catch (...) {
-  }
-  for (int t = started; t < n; t++) fn(t);
-  for (auto& x : th) x.join();
+  const std::function<void(int)> f = [&fn](int t) { fn(t); };
+  bsam::parallel_for(n, f);
 }
 
 static void plan_interp_rules(const bsam_ctx* c, const Level& L, GhostPlan& G) {
@@ -4840,31 +4834,35 @@ __global__ __launch_bounds__(kBlock) void k_dp_fill(int* __restrict__ a, long n,
   if (g < n) a[g] = v;
 }
 
-// The interpolation stage: rules [r0, r1) (the call's colour-filter
-// segments, in order).  write 0: claim; 1: the last claimant writes.
-__global__ __launch_bounds__(kBlock) void k_dp_interp(DPlanArgs A, const GRule* __restrict__ rules, int r0, int r1,
+// The interpolation stage of the three calls at once: rule g of segment sg
+// (rules [seg[sg], seg[sg + 1])) belongs to the calls whose colour filter
+// passes it (segment 0: ipass 0 and 1; 1: all; 2: ipass 0 and 2), and in
+// each of them the last such rule writing a cell wins.  claim and ex hold
+// one array of n per call.  write 0: claim; 1: the last claimant writes.
+__global__ __launch_bounds__(kBlock) void k_dp_interp(DPlanArgs A, const GRule* __restrict__ rules, int4 seg, long n,
                                                       int* __restrict__ claim, DExpr* __restrict__ ex, int write) {
-  const long g = r0 + (long)blockIdx.x * kBlock + threadIdx.x;
-  if (g >= r1) return;
+  const long g = seg.x + (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= seg.w) return;
+  const int sg = g < seg.y ? 0 : (g < seg.z ? 1 : 2);
   const int dst = rules[g].dst;
   const int a = dp_patch_at(A, dst);
   const int pid = A.patches[a];
   int i, j, k;
   qdecode(A.pt[pid], dst, i, j, k);
   const long cell = A.gbase[a] + shell_index(A.pt[pid], A.nd, i, j, k);
-  if (!write) {
-    atomicMax(&claim[cell], (int)g);
-    return;
+  for (int p = 0; p < 3; p++) {
+    if ((p == 1 && sg == 2) || (p == 2 && sg == 0)) continue;
+    if (!write) atomicMax(&claim[p * n + cell], (int)g);
+    else if (claim[p * n + cell] == (int)g) ex[p * n + cell] = DExpr{(int)g, -1, pid, dx_make(1, 0, 0, 0)};
   }
-  if (claim[cell] != (int)g) return;
-  ex[cell] = DExpr{(int)g, -1, pid, dx_make(1, 0, 0, 0)};
 }
 
 // One box stage over the flat cells of its boxes (bpre: each box's first
-// flat cell; x fastest within a box).  `tag` = (stage number + 1) << 24 in
-// the claims.
+// flat cell; x fastest within a box), for the three calls at once (the box
+// stages are the same at every ipass; ex holds one array of n per call).
+// `tag` = (stage number + 1) << 24 in the claims.
 __global__ __launch_bounds__(kBlock) void k_dp_box(DPlanArgs A, const BoxDesc* __restrict__ boxes,
-                                                   const int* __restrict__ bpre, int nbox, int ncell, int tag,
+                                                   const int* __restrict__ bpre, int nbox, int ncell, int tag, long n,
                                                    int* __restrict__ claim, DExpr* __restrict__ ex,
                                                    int* __restrict__ err, int write) {
   const int g = blockIdx.x * kBlock + threadIdx.x;
@@ -4896,36 +4894,41 @@ __global__ __launch_bounds__(kBlock) void k_dp_box(DPlanArgs A, const BoxDesc* _
   const int si = B.slo[0] + x, sj = B.slo[1] + y, sk = B.slo[2] + z;
   const int s = qidx(S, si, sj, sk);
   const int out = n_outside(S, A.nd, si, sj, sk);
-  DExpr e{s, B.sp, 0, dx_make(out > 1 ? 2 : 0, out, 0, 0)};
+  long scell = -1;
   if (out) {
     // a ghost source: the expression an earlier stage left there, if any
-    const long scell = dp_ghost(A, B.sp, si, sj, sk);
+    scell = dp_ghost(A, B.sp, si, sj, sk);
     if ((claim[scell] & 0x7f000000) == tag) {
       atomicOr(err, kDpOrder);
       return;
     }
-    const DExpr prev = ex[scell];
-    if (prev.meta & kDxValid) {
-      if (dx_kind(prev.meta) == 1 && prev.dpid != B.dp) {
-        atomicOr(err, kDpInterpCopy);
-        return;
-      }
-      e = prev;
-    }
   }
-  e.dpid = B.dp;
   // the post-operation the box's operation adds (ghost_post's codes)
   const int code = (B.op == kCopy) ? 0
                    : (B.op == kCopyPlus0 ? 1 : (B.op == kDirichlet2 ? 2 : (B.op == kBgMixed ? 3 : 4)));
-  if (code) {
-    const int k = dx_npost(e.meta);
-    if (k >= 5) {
-      atomicOr(err, kDpChain);
-      return;
+  for (int p = 0; p < 3; p++) {
+    DExpr e{s, B.sp, 0, dx_make(out > 1 ? 2 : 0, out, 0, 0)};
+    if (scell >= 0) {
+      const DExpr prev = ex[p * n + scell];
+      if (prev.meta & kDxValid) {
+        if (dx_kind(prev.meta) == 1 && prev.dpid != B.dp) {
+          atomicOr(err, kDpInterpCopy);
+          return;
+        }
+        e = prev;
+      }
     }
-    e.meta = dx_make(dx_kind(e.meta), dx_early(e.meta), k + 1, dx_post(e.meta) | (code << (3 * k)));
+    e.dpid = B.dp;
+    if (code) {
+      const int k = dx_npost(e.meta);
+      if (k >= 5) {
+        atomicOr(err, kDpChain);
+        return;
+      }
+      e.meta = dx_make(dx_kind(e.meta), dx_early(e.meta), k + 1, dx_post(e.meta) | (code << (3 * k)));
+    }
+    ex[p * n + dcell] = e;
   }
-  ex[dcell] = e;
 }
 
 // The call's end checks: no unknown value survives, and no rule that reads
@@ -4933,7 +4936,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_box(DPlanArgs A, const BoxDesc* _
 __global__ __launch_bounds__(kBlock) void k_dp_check(DPlanArgs A, const DExpr* __restrict__ ex, long n,
                                                      int* __restrict__ err) {
   const long g = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (g >= n) return;
+  if (g >= 3 * n) return;
   const DExpr e = ex[g];
   if (!(e.meta & kDxValid)) return;
   if (dx_kind(e.meta) == 2) atomicOr(err, kDpUnknown);
@@ -4942,7 +4945,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_check(DPlanArgs A, const DExpr* _
     int i, j, k;
     qdecode(S, e.src, i, j, k);
     if (n_outside(S, A.nd, i, j, k)) {
-      const DExpr w = ex[dp_ghost(A, e.spid, i, j, k)];
+      const DExpr w = ex[(g / n) * n + dp_ghost(A, e.spid, i, j, k)];
       if ((w.meta & kDxValid) && dx_early(w.meta)) atomicOr(err, kDpEarly);
     }
   }
@@ -5250,7 +5253,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   size_t boxints = 0;
   for (auto& v : bpre) boxints += v.size();
-  const size_t o_ex = 0, o_ci = o_ex + al(3 * n * sizeof(DExpr)), o_cb = o_ci + al(n * sizeof(int)),
+  const size_t o_ex = 0, o_ci = o_ex + al(3 * n * sizeof(DExpr)), o_cb = o_ci + al(3 * n * sizeof(int)),
                o_cat = o_cb + al(n * sizeof(int)), o_bc = o_cat + al(7 * n),
                o_bs = o_bc + al(35 * (size_t)nblk * sizeof(int)), o_pos = o_bs + al(35 * (size_t)nblk * sizeof(int)),
                o_gb = o_pos + al(c->npatch * sizeof(int)), o_bp = o_gb + al((np + 1) * sizeof(long)),
@@ -5312,32 +5315,31 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
       LAUNCH_CHECK();
     }
   }
-  for (int p = 0; p < 3; p++) {
-    DExpr* E = ex + (size_t)p * n;
-    k_dp_fill<<<nblocks(4 * n), kBlock, 0, c->s>>>(reinterpret_cast<int*>(E), 4 * n, 0);
-    k_dp_fill<<<nblocks(n), kBlock, 0, c->s>>>(claim_i, n, -1);
-    k_dp_fill<<<nblocks(n), kBlock, 0, c->s>>>(claim_b, n, -1);
+  // the three calls' symbolic runs, stage by stage together
+  k_dp_fill<<<nblocks(12 * n), kBlock, 0, c->s>>>(reinterpret_cast<int*>(ex), 12 * n, 0);
+  k_dp_fill<<<nblocks(3 * n), kBlock, 0, c->s>>>(claim_i, 3 * n, -1);
+  k_dp_fill<<<nblocks(n), kBlock, 0, c->s>>>(claim_b, n, -1);
+  LAUNCH_CHECK();
+  const int4 seg = make_int4(L.roff[0][0], L.roff[1][0], L.roff[2][0], L.roff[2][np]);
+  if (seg.w > seg.x) {
+    for (int w = 0; w < 2; w++)
+      k_dp_interp<<<nblocks(seg.w - seg.x), kBlock, 0, c->s>>>(A, L.d_rules, seg, n, claim_i, ex, w);
     LAUNCH_CHECK();
-    const int s0 = (p == 2) ? 1 : 0, s1 = (p == 1) ? 1 : 2;
-    const int r0 = L.roff[s0][0], r1 = L.roff[s1][np];
-    if (r1 > r0) {
-      for (int w = 0; w < 2; w++)
-        k_dp_interp<<<nblocks(r1 - r0), kBlock, 0, c->s>>>(A, L.d_rules, r0, r1, claim_i, E, w);
-      LAUNCH_CHECK();
-    }
+  }
+  {
     size_t k = 0;
     for (size_t st = 0; st < G.stages.size(); st++) {
       const int nbox = (int)G.stages[st].size(), ncell = bpre[st].back();
       if (ncell > 0)
         for (int w = 0; w < 2; w++)
           k_dp_box<<<nblocks(ncell), kBlock, 0, c->s>>>(A, L.stages[st].d, d_bpre + k, nbox, ncell,
-                                                          (int)(st + 1) << 24, claim_b, E, d_err, w);
+                                                          (int)(st + 1) << 24, n, claim_b, ex, d_err, w);
       LAUNCH_CHECK();
       k += bpre[st].size();
     }
-    k_dp_check<<<nblocks(n), kBlock, 0, c->s>>>(A, E, n, d_err);
-    LAUNCH_CHECK();
   }
+  k_dp_check<<<nblocks(3 * n), kBlock, 0, c->s>>>(A, ex, n, d_err);
+  LAUNCH_CHECK();
   DPlanMaps M{};
   for (int p = 0; p < 3; p++) M.ex[p] = ex + (size_t)p * n;
   M.rules = L.d_rules;

@@ -7,17 +7,120 @@
 // GridFlagRatio :1635-1660), and each subgrid becomes a child descriptor
 // (MakeNewGrid :1663-1735).  No device code: clustering reads one int per
 // cell of the level once per split pass and is a small part of a regrid.
+#include <unistd.h>
+
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "bsam_grid.h"
 
+namespace {
+
+// The host threads of bsam::parallel_for: workers 1..size() wait for a job
+// (a generation count), run their share and count down.
+struct HostPool {
+  std::mutex m, busy;
+  std::condition_variable cv, done;
+  std::vector<std::thread> th;
+  const std::function<void(int)>* job = nullptr;
+  int njob = 0, left = 0;
+  long gen = 0;
+  bool stop = false;
+  const pid_t pid = getpid();
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> g(m);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto& t : th)
+      if (t.joinable()) t.join();
+  }
+  void loop(int w) {
+    long seen = 0;
+    for (;;) {
+      const std::function<void(int)>* f;
+      int n;
+      {
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return stop || gen != seen; });
+        if (stop) return;
+        seen = gen;
+        f = job;
+        n = njob;
+      }
+      if (w < n) {
+        try {
+          (*f)(w);
+        } catch (...) {
+        }
+      }
+      std::lock_guard<std::mutex> g(m);
+      if (--left == 0) done.notify_all();
+    }
+  }
+};
+HostPool& host_pool() {
+  static HostPool p;
+  return p;
+}
+
+// fn(t), t = 0 .. n-1, on threads started for the call
+void spawn_for(int n, const std::function<void(int)>& fn) {
+  std::vector<std::thread> th;
+  int started = 1;
+  try {
+    for (; started < n; started++) th.emplace_back([&fn, started]() { fn(started); });
+  } catch (...) {
+  }
+  for (int t = started; t < n; t++) fn(t);
+  fn(0);
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
 namespace bsam {
+
+void parallel_for(int n, const std::function<void(int)>& fn) {
+  if (n <= 1) {
+    if (n == 1) fn(0);
+    return;
+  }
+  HostPool& P = host_pool();
+  std::unique_lock<std::mutex> busy(P.busy, std::try_to_lock);
+  if (!busy.owns_lock() || getpid() != P.pid) {
+    spawn_for(n, fn);
+    return;
+  }
+  try {
+    while ((int)P.th.size() < n - 1) {
+      const int w = (int)P.th.size() + 1;
+      P.th.emplace_back([&P, w]() { P.loop(w); });
+    }
+  } catch (...) {
+  }
+  const int nw = (int)P.th.size();
+  {
+    std::lock_guard<std::mutex> g(P.m);
+    P.job = &fn;
+    P.njob = n;
+    P.left = nw;
+    P.gen++;
+  }
+  P.cv.notify_all();
+  fn(0);
+  for (int t = nw + 1; t < n; t++) fn(t);
+  std::unique_lock<std::mutex> lk(P.m);
+  P.done.wait(lk, [&] { return P.left == 0; });
+}
 
 void periodic_offsets(int ndims, const bsam_patch_desc& root, int level,
                       std::vector<std::array<int, 3>>& offs, bool& periodic) {
@@ -227,19 +330,11 @@ int32_t new_subgrids(const Tags& T, int nd, int mgp, double desfill, int level, 
   return BSAM_OK;
 }
 
-// fn(t) for t = 0 .. n-1 on host threads (those that cannot be started run
-// on the caller's thread).
+// fn(t) for t = 0 .. n-1 on the host pool.
 template <typename F>
 void for_threads(int n, F&& fn) {
-  std::vector<std::thread> th;
-  int started = 1;
-  try {
-    for (; started < n; started++) th.emplace_back([&fn, started]() { fn(started); });
-  } catch (...) {
-  }
-  for (int t = started; t < n; t++) fn(t);
-  fn(0);
-  for (auto& x : th) x.join();
+  const std::function<void(int)> f = [&fn](int t) { fn(t); };
+  bsam::parallel_for(n, f);
 }
 
 int host_threads() {
