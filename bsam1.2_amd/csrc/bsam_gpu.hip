@@ -4766,67 +4766,109 @@ enum : int {
   kDpLean = 128,      // a reduced table's ghost reads a ghost from before the call
 };
 
-// Ghost cells of a patch's storage box, numbered shell by shell.  3D: the
-// plane k = 0, the plane k = nz - 1, then for each inner plane the rows
-// j = 0 and j = ny - 1 and the two end cells of every inner row (nx, ny,
-// nz the storage extents); 2D: the rows and end cells alone.
+// Ghost cells of a patch numbered in pool order (colour, plane, row, half
+// index), for an even row length nx = mx + 2 (every refined patch): per
+// colour c, the plane k = 0 (ny rows of nx/2 cells of colour c), the inner
+// planes (row 0 and row ny - 1 with nx/2 each, and between them the one
+// colour-c end cell of each inner row: i = 0 and i = nx - 1 differ in
+// colour), then the plane nz - 1.  2D: one inner plane per colour.
 __host__ __device__ inline long shell_count(const DPatch& P, int nd) {
   const long nz = nd == 3 ? P.nq[2] : 1, mz = nd == 3 ? P.mx[2] : 1;
   return (long)P.nq[0] * P.nq[1] * nz - (long)P.mx[0] * P.mx[1] * mz;
 }
-// the shell index of ghost cell (i, j, k) (storage indices; not for an
-// interior cell)
+// the index of ghost cell (i, j, k) (storage indices; not for an interior
+// cell)
 __device__ __forceinline__ int shell_index(const DPatch& P, int nd, int i, int j, int k) {
-  const int nx = P.nq[0], ny = P.nq[1];
-  int base = 0;
+  const int nx = P.nq[0], ny = P.nq[1], h = nx >> 1, inner = nx + ny - 2;
+  const int c = (i + j + k) & 1, m = i >> 1;
+  int base;
   if (nd == 3) {
     const int nz = P.nq[2];
-    if (k == 0) return j * nx + i;
-    if (k == nz - 1) return nx * ny + j * nx + i;
-    base = 2 * nx * ny + (k - 1) * (2 * nx + 2 * (ny - 2));
+    base = c * (ny * nx + (nz - 2) * inner);
+    if (k == 0) return base + j * h + m;
+    base += ny * h;
+    if (k == nz - 1) return base + (nz - 2) * inner + j * h + m;
+    base += (k - 1) * inner;
+  } else {
+    base = c * inner;
   }
-  if (j == 0) return base + i;
-  if (j == ny - 1) return base + nx + i;
-  return base + 2 * nx + (j - 1) * 2 + (i == 0 ? 0 : 1);
+  if (j == 0) return base + m;
+  if (j == ny - 1) return base + h + (ny - 2) + m;
+  return base + h + (j - 1);
+}
+// the cell of ghost index s
+__device__ __forceinline__ void shell_cell(const DPatch& P, int nd, int s, int& i, int& j, int& k) {
+  const int nx = P.nq[0], ny = P.nq[1], h = nx >> 1, inner = nx + ny - 2;
+  int c;
+  k = 0;
+  if (nd == 3) {
+    const int nz = P.nq[2], per = ny * nx + (nz - 2) * inner;
+    c = s >= per;
+    s -= c * per;
+    bool full = s < ny * h;
+    if (!full) {
+      s -= ny * h;
+      if (s >= (nz - 2) * inner) {
+        s -= (nz - 2) * inner;
+        full = true;
+        k = nz - 1;
+      } else {
+        k = 1 + s / inner;
+        s -= (k - 1) * inner;
+      }
+    }
+    if (full) {
+      j = s / h;
+      i = 2 * (s - j * h) + ((c + j + k) & 1);
+      return;
+    }
+  } else {
+    c = s >= inner;
+    s -= c * inner;
+  }
+  if (s < h) {
+    j = 0;
+    i = 2 * s + ((c + j + k) & 1);
+  } else if (s < h + ny - 2) {
+    j = 1 + (s - h);
+    i = ((j + k) & 1) == c ? 0 : nx - 1;
+  } else {
+    j = ny - 1;
+    i = 2 * (s - h - (ny - 2)) + ((c + j + k) & 1);
+  }
 }
 struct DPlanArgs {
   const DPatch* __restrict__ pt;
   const int* __restrict__ patches;  // the level's descriptor indices
   const int* __restrict__ pos;      // descriptor index -> position in the level, or -1
   const long* __restrict__ gbase;   // first ghost index of each level patch (np + 1 entries)
-  const int* __restrict__ bp;       // workgroup -> level position, over the pool-order cells
+  const int* __restrict__ bp;       // workgroup -> level position, over the ghost cells
   const int* __restrict__ bfirst;   // first workgroup of each level patch there
-  const int* __restrict__ lq;       // pool offset of each level patch (ascending)
+  const int* __restrict__ eoff;     // first interpolation element of each level patch (np + 1)
   int np, nd;
 };
 __device__ __forceinline__ long dp_ghost(const DPlanArgs& A, int pid, int i, int j, int k) {
   return A.gbase[A.pos[pid]] + shell_index(A.pt[pid], A.nd, i, j, k);
 }
-// level position of the patch whose q slot holds pool offset `off`
-__device__ __forceinline__ int dp_patch_at(const DPlanArgs& A, int off) {
-  int lo = 0, hi = A.np;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (A.lq[mid] <= off) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-// The level's cells in pool order (variable 0, patch after patch, each
-// patch in its colour-split order), each patch's from a workgroup boundary:
-// thread t of workgroup b is the patch's cell (b - bfirst) * kBlock + t.
-// Returns its ghost index (-1 for an interior cell or none), its patch
-// (level position) and storage indices.  The tables list their rules in
-// this order, as the host planner does.
+// The level's ghost cells in pool order, each patch's from a workgroup
+// boundary: thread t of workgroup b is the patch's ghost (b - bfirst) *
+// kBlock + t.  Returns its ghost index (-1 past the patch's ghosts), its
+// patch (level position) and storage indices.  The tables list their rules
+// in this order, as the host planner does.
 __device__ __forceinline__ long dp_cell(const DPlanArgs& A, int& a, int& i, int& j, int& k) {
   a = A.bp[blockIdx.x];
-  const DPatch& P = A.pt[A.patches[a]];
   const int local = (blockIdx.x - A.bfirst[a]) * kBlock + threadIdx.x;
-  if (local >= P.qvs) return -1;
-  qdecode(P, P.qoff + local, i, j, k);
-  // (the padding slot of an odd row length is no cell)
-  if (i >= P.nq[0] || !n_outside(P, A.nd, i, j, k)) return -1;
-  return A.gbase[a] + shell_index(P, A.nd, i, j, k);
+  if (local >= A.gbase[a + 1] - A.gbase[a]) return -1;
+  shell_cell(A.pt[A.patches[a]], A.nd, local, i, j, k);
+  return A.gbase[a] + local;
+}
+// Interpolation element e of level patch pid (interp_rule_at) as a rule.
+__device__ __forceinline__ GRule dp_rule(const DPlanArgs& A, int pid, int e) {
+  const DPatch P = A.pt[pid];
+  GRule r;
+  int sg = 0;
+  interp_rule_at(P, A.pt[P.parent], A.nd, e, r, sg);
+  return r;
 }
 
 __global__ __launch_bounds__(kBlock) void k_dp_fill(int* __restrict__ a, long n, int v) {
@@ -4834,26 +4876,48 @@ __global__ __launch_bounds__(kBlock) void k_dp_fill(int* __restrict__ a, long n,
   if (g < n) a[g] = v;
 }
 
-// The interpolation stage of the three calls at once: rule g of segment sg
-// (rules [seg[sg], seg[sg + 1])) belongs to the calls whose colour filter
-// passes it (segment 0: ipass 0 and 1; 1: all; 2: ipass 0 and 2), and in
-// each of them the last such rule writing a cell wins.  claim and ex hold
-// one array of n per call.  write 0: claim; 1: the last claimant writes.
-__global__ __launch_bounds__(kBlock) void k_dp_interp(DPlanArgs A, const GRule* __restrict__ rules, int4 seg, long n,
-                                                      int* __restrict__ claim, DExpr* __restrict__ ex, int write) {
-  const long g = seg.x + (long)blockIdx.x * kBlock + threadIdx.x;
-  if (g >= seg.w) return;
-  const int sg = g < seg.y ? 0 : (g < seg.z ? 1 : 2);
-  const int dst = rules[g].dst;
-  const int a = dp_patch_at(A, dst);
+// The interpolation stage of the three calls at once, over the level's
+// interpolation elements (patch a's are [eoff[a], eoff[a + 1])): element e
+// of colour-filter segment sg belongs to the calls whose filter passes it
+// (segment 0: ipass 0 and 1; 1: all; 2: ipass 0 and 2), and in each of them
+// the last rule in (segment, patch, element) order writing a cell wins: the
+// claim is that order as a 64-bit key.  The expression keeps the element
+// (src) and its patch (dpid); the rule is made again where it is needed.
+// claim and ex hold one array of n per call; ecell keeps each element's
+// ghost index and segment (sg << 29 | cell; -1 for no rule) from the claim
+// pass for the write pass.  write 0: claim; 1: the last claimant writes.
+__global__ __launch_bounds__(kBlock) void k_dp_interp(DPlanArgs A, long total, long n,
+                                                      unsigned long long* __restrict__ claim, DExpr* __restrict__ ex,
+                                                      int* __restrict__ ecell, int write) {
+  const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= total) return;
+  const int a = elem_patch(A.eoff, A.np, g);
   const int pid = A.patches[a];
-  int i, j, k;
-  qdecode(A.pt[pid], dst, i, j, k);
-  const long cell = A.gbase[a] + shell_index(A.pt[pid], A.nd, i, j, k);
+  const int e = (int)(g - A.eoff[a]);
+  int sg = 0;
+  long cell;
+  if (!write) {
+    const DPatch P = A.pt[pid];
+    GRule r;
+    if (!interp_rule_at(P, A.pt[P.parent], A.nd, e, r, sg)) {
+      ecell[g] = -1;
+      return;
+    }
+    int i, j, k;
+    qdecode(P, r.dst, i, j, k);
+    cell = A.gbase[a] + shell_index(P, A.nd, i, j, k);
+    ecell[g] = (sg << 29) | (int)cell;
+  } else {
+    const int v = ecell[g];
+    if (v < 0) return;
+    sg = v >> 29;
+    cell = v & ((1 << 29) - 1);
+  }
+  const unsigned long long key = (((unsigned long long)sg << 56) | ((unsigned long long)a << 28) | (unsigned)e) + 1ull;
   for (int p = 0; p < 3; p++) {
     if ((p == 1 && sg == 2) || (p == 2 && sg == 0)) continue;
-    if (!write) atomicMax(&claim[p * n + cell], (int)g);
-    else if (claim[p * n + cell] == (int)g) ex[p * n + cell] = DExpr{(int)g, -1, pid, dx_make(1, 0, 0, 0)};
+    if (!write) atomicMax(&claim[p * n + cell], key);
+    else if (claim[p * n + cell] == key) ex[p * n + cell] = DExpr{e, -1, pid, dx_make(1, 0, 0, 0)};
   }
 }
 
@@ -4954,18 +5018,20 @@ __global__ __launch_bounds__(kBlock) void k_dp_check(DPlanArgs A, const DExpr* _
 // What plan_ghost_maps' sel56 and lean need besides the expressions.
 struct DPlanMaps {
   const DExpr* __restrict__ ex[3];
-  const GRule* __restrict__ rules;
   const int* __restrict__ mglo;      // descriptor index -> mglobal lower corner (1-based, 3 ints)
   const uint32_t* __restrict__ occ;  // parent level's cells inside its patches (bit map)
   int per[3], cper[3], cwrap[3];
 };
 
-__device__ __forceinline__ bool dp_same_expr(const GRule* __restrict__ rules, const DPatch* __restrict__ pt,
-                                             const DExpr& a, const DExpr& b) {
+// same_expr of the host planner; ra, rb the rules of a and b when they are
+// interpolation expressions
+__device__ __forceinline__ bool dp_same_expr(const DPlanArgs& A, const DExpr& a, const DExpr& b, const GRule& ra,
+                                             const GRule& rb) {
   if (dx_kind(a.meta) != dx_kind(b.meta) || dx_post(a.meta) != dx_post(b.meta) || dx_npost(a.meta) != dx_npost(b.meta))
     return false;
-  if (dx_kind(a.meta) == 0) return a.src == b.src && pt[a.spid].qvs == pt[b.spid].qvs;
-  const GRule r = rules[a.src], s = rules[b.src];
+  if (dx_kind(a.meta) == 0) return a.src == b.src && A.pt[a.spid].qvs == A.pt[b.spid].qvs;
+  const GRule& r = ra;
+  const GRule& s = rb;
   if (r.kind != s.kind || r.s1 != s.s1 || r.s2 != s.s2 || r.cvs != s.cvs) return false;
   for (int n = 0; n < r.kind && n < 4; n++)
     if (r.c[n] != s.c[n]) return false;
@@ -5043,7 +5109,14 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
         out[6] = exact ? 0 : out[0];
       }
     }
-    // map[2 + p], p = 1, 2: what a red-black sweep sequence needs
+    // map[2 + p], p = 1, 2: what a red-black sweep sequence needs (the
+    // interpolation rules of the two calls' expressions made once)
+    GRule rr[3];
+    for (int p = 1; p <= 2; p++)
+      if (ok[p] && dx_kind(e[p].meta) == 1) {
+        if (p == 2 && ok[1] && dx_kind(e[1].meta) == 1 && e[1].dpid == e[2].dpid && e[1].src == e[2].src) rr[2] = rr[1];
+        else rr[p] = dp_rule(A, e[p].dpid, e[p].src);
+      }
     for (int p = 1; p <= 2; p++) {
       if (!ok[p]) continue;
       const DExpr& x = e[p];
@@ -5064,14 +5137,14 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
             qdecode(A.pt[x.spid], x.src, si, sj, sk);
             need = ((si + sj + sk) & 1) == col;
           } else {
-            const GRule R = M.rules[x.src];
+            const GRule& R = rr[p];
             qdecode(D, R.s1, si, sj, sk);
             need = ((si + sj + sk) & 1) == col;
             qdecode(D, R.s2, si, sj, sk);
             need = need || ((si + sj + sk) & 1) == col;
           }
         }
-        if (!need) need = !have_o || !dp_same_expr(M.rules, A.pt, x, e[3 - p]);
+        if (!need) need = !have_o || !dp_same_expr(A, x, e[3 - p], rr[p], rr[3 - p]);
         if (need && dx_early(x.meta)) {
           atomicOr(err, kDpLean);
           continue;
@@ -5143,6 +5216,8 @@ __global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, lo
   if (g < 0) return;
   const int pid = A.patches[a];
   const int dst = qidx(A.pt[pid], d3[0], d3[1], d3[2]);
+  GRule rule{};
+  int rpid = -1, relem = -1;
   for (int m = 0; m < 7; m++) {
     const int k = c[m];
     if (!k) continue;
@@ -5157,7 +5232,13 @@ __global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, lo
       if (O.cpost[m]) O.cpost[m][at] = (uint16_t)post;
     } else {
       if (k == 4) at += O.ni1[m];
-      GRule r = M.rules[e.src];
+      // the rule of the last element made, if it is this one
+      if (e.dpid != rpid || e.src != relem) {
+        rule = dp_rule(A, e.dpid, e.src);
+        rpid = e.dpid;
+        relem = e.src;
+      }
+      GRule r = rule;
       r.dst = dst;
       r.kind = (r.kind & 0xff) | (post << 8);
       O.ir[m][at] = r;
@@ -5184,34 +5265,38 @@ __global__ __launch_bounds__(kBlock) void k_dp_occupy(LevelArgs L, const int* __
 int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string& why) {
   L.maps_ok = false;
   const int np = (int)L.patches.size(), nd = c->p.ndims;
-  if (L.level <= 0 || np == 0 || !L.d_rules) {
-    why = "not a refined level with device rules";
+  if (L.level <= 0 || np == 0) {
+    why = "not a refined level";
     return BSAM_OK;
   }
   if (G.stages.size() > 120) {
     why = "too many ghost-fill stages";
     return BSAM_OK;
   }
-  // ghost cells of the level, patch after patch
-  std::vector<int> pos(c->npatch, -1), lq(np);
+  // ghost cells of the level, patch after patch, each patch's from a
+  // workgroup boundary; the interpolation elements likewise
+  std::vector<int> pos(c->npatch, -1), eoff(np + 1, 0);
   std::vector<long> gbase(np + 1);
   std::vector<int> bfirst(np), bp;
-  long n = 0;
+  long n = 0, nel = 0;
   for (int a = 0; a < np; a++) {
     const DPatch& P = c->hp[L.patches[a]];
-    pos[L.patches[a]] = a;
-    gbase[a] = n;
-    bfirst[a] = (int)bp.size();
-    bp.insert(bp.end(), nblocks(P.qvs), a);
-    lq[a] = P.qoff;
-    if (a > 0 && lq[a] <= lq[a - 1]) {
-      why = "level patches not in ascending pool order";
+    if (P.nq[0] & 1) {
+      why = "an odd row length (the ghost numbering needs even ones)";
       return BSAM_OK;
     }
-    n += shell_count(P, nd);
+    pos[L.patches[a]] = a;
+    gbase[a] = n;
+    const long cnt = shell_count(P, nd);
+    bfirst[a] = (int)bp.size();
+    bp.insert(bp.end(), nblocks(cnt), a);
+    n += cnt;
+    eoff[a] = (int)nel;
+    nel += interp_elems(P, nd);
   }
   gbase[np] = n;
-  if (n >= (1L << 31) / 8) {
+  eoff[np] = (int)nel;
+  if (n >= (1L << 31) / 16 || nel >= (1L << 28) || np >= (1 << 27)) {
     why = "level too large for the device planner";
     return BSAM_OK;
   }
@@ -5253,27 +5338,28 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   size_t boxints = 0;
   for (auto& v : bpre) boxints += v.size();
-  const size_t o_ex = 0, o_ci = o_ex + al(3 * n * sizeof(DExpr)), o_cb = o_ci + al(3 * n * sizeof(int)),
-               o_cat = o_cb + al(n * sizeof(int)), o_bc = o_cat + al(7 * n),
+  const size_t o_ex = 0, o_ci = o_ex + al(3 * n * sizeof(DExpr)), o_cb = o_ci + al(3 * n * sizeof(unsigned long long)),
+               o_ec = o_cb + al(n * sizeof(int)), o_cat = o_ec + al(nel * sizeof(int)), o_bc = o_cat + al(7 * n),
                o_bs = o_bc + al(35 * (size_t)nblk * sizeof(int)), o_pos = o_bs + al(35 * (size_t)nblk * sizeof(int)),
                o_gb = o_pos + al(c->npatch * sizeof(int)), o_bp = o_gb + al((np + 1) * sizeof(long)),
-               o_bf = o_bp + al(bp.size() * sizeof(int)), o_lq = o_bf + al(np * sizeof(int)),
-               o_bpre = o_lq + al(np * sizeof(int)), o_mglo = o_bpre + al(boxints * sizeof(int)),
+               o_bf = o_bp + al(bp.size() * sizeof(int)), o_eo = o_bf + al(np * sizeof(int)),
+               o_bpre = o_eo + al((np + 1) * sizeof(int)), o_mglo = o_bpre + al(boxints * sizeof(int)),
                o_occ = o_mglo + al(3 * c->npatch * sizeof(int)), o_small = o_occ + al(owords * sizeof(uint32_t)),
                o_scan = o_small + 512, total = o_scan + al(scan_bytes);
   char* W = plan_workspace(c, total);
   if (!W) return set_err(BSAM_ENOMEM, "ghost-table planner workspace of %zu bytes", total);
   DExpr* ex = reinterpret_cast<DExpr*>(W + o_ex);
-  int* claim_i = reinterpret_cast<int*>(W + o_ci);
+  auto* claim_i = reinterpret_cast<unsigned long long*>(W + o_ci);
   int* claim_b = reinterpret_cast<int*>(W + o_cb);
   uint8_t* cat = reinterpret_cast<uint8_t*>(W + o_cat);
+  int* d_ecell = reinterpret_cast<int*>(W + o_ec);
   int* bcnt = reinterpret_cast<int*>(W + o_bc);
   int* bscan = reinterpret_cast<int*>(W + o_bs);
   int* d_pos = reinterpret_cast<int*>(W + o_pos);
   long* d_gb = reinterpret_cast<long*>(W + o_gb);
   int* d_bp = reinterpret_cast<int*>(W + o_bp);
   int* d_bf = reinterpret_cast<int*>(W + o_bf);
-  int* d_lq = reinterpret_cast<int*>(W + o_lq);
+  int* d_eo = reinterpret_cast<int*>(W + o_eo);
   int* d_bpre = reinterpret_cast<int*>(W + o_bpre);
   int* d_mglo = reinterpret_cast<int*>(W + o_mglo);
   uint32_t* d_occ = reinterpret_cast<uint32_t*>(W + o_occ);
@@ -5289,7 +5375,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     put(o_gb, gbase.data(), gbase.size() * sizeof(long));
     put(o_bp, bp.data(), bp.size() * sizeof(int));
     put(o_bf, bfirst.data(), bfirst.size() * sizeof(int));
-    put(o_lq, lq.data(), lq.size() * sizeof(int));
+    put(o_eo, eoff.data(), eoff.size() * sizeof(int));
     size_t k = 0;
     for (auto& v : bpre) {
       put(o_bpre + k * sizeof(int), v.data(), v.size() * sizeof(int));
@@ -5305,7 +5391,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
     LAUNCH_CHECK();
     HIPCHK(hipStreamSynchronize(c->s));
   }
-  DPlanArgs A{c->d_pt, L.d_patches, d_pos, d_gb, d_bp, d_bf, d_lq, np, nd};
+  DPlanArgs A{c->d_pt, L.d_patches, d_pos, d_gb, d_bp, d_bf, d_eo, np, nd};
   // the parent level's cells
   {
     Level& Pl = c->L(L.level - 1);
@@ -5317,13 +5403,11 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   }
   // the three calls' symbolic runs, stage by stage together
   k_dp_fill<<<nblocks(12 * n), kBlock, 0, c->s>>>(reinterpret_cast<int*>(ex), 12 * n, 0);
-  k_dp_fill<<<nblocks(3 * n), kBlock, 0, c->s>>>(claim_i, 3 * n, -1);
+  k_dp_fill<<<nblocks(6 * n), kBlock, 0, c->s>>>(reinterpret_cast<int*>(claim_i), 6 * n, 0);
   k_dp_fill<<<nblocks(n), kBlock, 0, c->s>>>(claim_b, n, -1);
   LAUNCH_CHECK();
-  const int4 seg = make_int4(L.roff[0][0], L.roff[1][0], L.roff[2][0], L.roff[2][np]);
-  if (seg.w > seg.x) {
-    for (int w = 0; w < 2; w++)
-      k_dp_interp<<<nblocks(seg.w - seg.x), kBlock, 0, c->s>>>(A, L.d_rules, seg, n, claim_i, ex, w);
+  if (nel > 0) {
+    for (int w = 0; w < 2; w++) k_dp_interp<<<nblocks(nel), kBlock, 0, c->s>>>(A, nel, n, claim_i, ex, d_ecell, w);
     LAUNCH_CHECK();
   }
   {
@@ -5342,7 +5426,6 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   LAUNCH_CHECK();
   DPlanMaps M{};
   for (int p = 0; p < 3; p++) M.ex[p] = ex + (size_t)p * n;
-  M.rules = L.d_rules;
   M.mglo = d_mglo;
   M.occ = d_occ;
   for (int a = 0; a < 3; a++) M.per[a] = per[a], M.cper[a] = cper[a], M.cwrap[a] = cwrap[a];
@@ -5669,6 +5752,8 @@ int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
   // (with the device planner) their flattened tables, on the device
   const bool dev_tables = tables && c->p.batching && L.level > 0 && c->table_planner == 1;
   const bool dev_rules = (!tables || dev_tables) && c->p.batching && L.level > 0;
+  // (with the device planner the interpolation rules are needed only by a
+  // level it refuses, which keeps the staged fill)
   if (dev_rules) {
     for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(L.patches.size() + 1, 0);
   } else {
@@ -5677,9 +5762,9 @@ int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
   if (c->p.batching && tables && !dev_tables) RETCHK(plan_ghost_maps(c, L, G));
   upload_ghost(c, L, G, Bk);
   RETCHK(Bk.upload(c, L));
-  if (dev_rules) RETCHK(build_rules_device(c, L));
   L.why = G.why;
   if (dev_tables) RETCHK(plan_maps_device(c, L, G, L.why));
+  if (dev_rules && !L.maps_ok) RETCHK(build_rules_device(c, L));
   return BSAM_OK;
 }
 
@@ -6206,7 +6291,7 @@ extern "C" int32_t bsam_download(bsam_ctx* c, int32_t patch, int32_t field, doub
 // sweeps write their ghost images (phys_cell_images) instead of a SetGhost
 // after each colour.
 static bool phys_images_level(const bsam_ctx* c, const Level& L) {
-  if (c->p.model != BSAM_MODEL_TUMOUR || L.patches.size() != 1 || L.wrap_pid >= 0 || L.d_rules ||
+  if (c->p.model != BSAM_MODEL_TUMOUR || L.patches.size() != 1 || L.wrap_pid >= 0 || L.level > 0 ||
       (int)L.stages.size() != c->p.ndims)
     return false;
   for (int f = 0; f < 2 * c->p.ndims; f++)
