@@ -157,11 +157,14 @@ struct LevelArgs {
 // Logical block index of a level launch (the level's tuned block order).
 __device__ __forceinline__ int level_block(const LevelArgs& L) { return ordered_block(L.order); }
 
-// floor(a / b) for 0 <= a < 2^22 and b >= 1: the hardware reciprocal's
-// estimate, corrected by one step (its error is below one), in place of
-// the integer division sequence; the flat level kernels are bound by their
-// address arithmetic.
+// floor(a / b) for a >= 0 and b >= 1: below 2^22 the hardware
+// reciprocal's estimate, corrected by one step (its error is below one), in
+// place of the integer division sequence, since the flat level kernels are
+// bound by their address arithmetic; from 2^22 on (patches of more than 4 M
+// cells of a colour), where the float estimate may be off by more, the
+// integer division.
 __device__ __forceinline__ int fdiv(int a, int b) {
+  if (a >= (1 << 22)) return a / b;
   int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
   const int r = a - q * b;
   if (r < 0) q--;
@@ -7968,13 +7971,13 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
       G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
                              : 2.0 * ((double)P.mx[0] + P.mx[1]);
     }
-    // e_phi and e_mu of the other colour over (N+G)/2 each, rhs (2 fields)
-    // and coef of the colour N/2 each, the colour's e written N/2 x 2
-    // fields (the correction smoother does not read the cell's own e); the
-    // tumour model: e (4 fields) of the other colour, rhs (4) and coef (2)
-    // of the colour, e (4) written
+    // e_phi and e_mu of the other colour over (N+G)/2 each, rhs (2 fields),
+    // coef and the cell's own e_phi (the Newton step's phi0 + e) of the
+    // colour N/2 each, the colour's e written N/2 x 2 fields; the tumour
+    // model: e (4 fields) of the other colour, rhs (4) and coef (2) of the
+    // colour, e (4) written
     if (tm) kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * (2.0 * (N + G) + 2.0 * N + N + 2.0 * N));
-    else kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + N + 0.5 * N + N));
+    else kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + N + 0.5 * N + 0.5 * N + N));
   }
   return (wrap || img) ? 1 : 0;
 }
