@@ -1745,17 +1745,47 @@ __global__ __launch_bounds__(kBlock) void k_init_fields(LevelArgs L, int nv, T* 
     return linw<T>(c2[0], c2[1], k) / (T)64;
   };
   constexpr int nch = (DIM == 3) ? 8 : 4;
+  constexpr int nz = (DIM == 3) ? 3 : 1;
   for (int v = 0; v < nv; v++) {
     const T* a = qc + v * P.cvs;
+    // the coarse cell's 3^d neighbourhood, loaded once: a child's p1 reads
+    // 2^d of these (il = i >> 1 is ic - 1 for the odd child, ic for the
+    // even one, and il + 1)
+    T nb[nz][3][3];
+#pragma unroll
+    for (int dz = 0; dz < nz; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+          nb[dz][dy][dx] = a[cidx(P, cc.ic - 1 + dx, cc.jc - 1 + dy, DIM == 3 ? cc.kc - 1 + dz : 0)];
+    // p1 of a child from the neighbourhood, in p1's operation order
+    auto p1n = [&](int i, int j, int k) -> T {
+      const int ox = (i >> 1) - (cc.ic - 1), oy = (j >> 1) - (cc.jc - 1);
+      if (DIM == 2) {
+        const T b0 = linw<T>(nb[0][oy][ox], nb[0][oy][ox + 1], i);
+        const T b1 = linw<T>(nb[0][oy + 1][ox], nb[0][oy + 1][ox + 1], i);
+        return linw<T>(b0, b1, j) / (T)16;
+      }
+      const int oz = (k >> 1) - (cc.kc - 1);
+      T c2[2];
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        const T b0 = linw<T>(nb[oz + d][oy][ox], nb[oz + d][oy][ox + 1], i);
+        const T b1 = linw<T>(nb[oz + d][oy + 1][ox], nb[oz + d][oy + 1][ox + 1], i);
+        c2[d] = linw<T>(b0, b1, j);
+      }
+      return linw<T>(c2[0], c2[1], k) / (T)64;
+    };
     // children (2ic - n&1, 2jc - (n>>1)&1, 2kc - n>>2) in the loop's order
     T ch[nch];
     T sum = (T)0;
 #pragma unroll
     for (int n = 0; n < nch; n++) {
-      ch[n] = p1(a, 2 * cc.ic - (n & 1), 2 * cc.jc - ((n >> 1) & 1), DIM == 3 ? 2 * cc.kc - (n >> 2) : 0);
+      ch[n] = p1n(2 * cc.ic - (n & 1), 2 * cc.jc - ((n >> 1) & 1), DIM == 3 ? 2 * cc.kc - (n >> 2) : 0);
       sum = n == 0 ? ch[0] : sum + ch[n];
     }
-    const T cor = a[cidx(P, cc.ic, cc.jc, cc.kc)] - (DIM == 3 ? (T)0.125 : (T)0.25) * sum;
+    const T cor = nb[nz / 2][1][1] - (DIM == 3 ? (T)0.125 : (T)0.25) * sum;
 #pragma unroll
     for (int n = 0; n < nch; n++) {
       const int o = qidx(P, 2 * cc.ic - (n & 1), 2 * cc.jc - ((n >> 1) & 1), DIM == 3 ? 2 * cc.kc - (n >> 2) : 0) +
@@ -5062,7 +5092,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
                                                       int* __restrict__ anypost, int* __restrict__ bcnt) {
   __shared__ int wc[kDpWaves][35];
   uint8_t out[7] = {0, 0, 0, 0, 0, 0, 0};
-  int a = 0, d3[3] = {0, 0, 0};
+  int a = 0, d3[3] = {0, 0, 0}, post = 0;
   const long g = dp_cell(A, a, d3[0], d3[1], d3[2]);
   if (g >= 0) {
     const int pid = A.patches[a];
@@ -5155,15 +5185,16 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
       }
       if (need) out[2 + p] = out[p];
     }
-    int post = 0;
     for (int m = 0; m < 7; m++) {
       cat[m * n + g] = out[m];
       if (out[m] && out[m] <= 2 && dx_post(e[dp_call(m)].meta)) post |= 1 << m;
     }
-    if (post) atomicOr(anypost, post);
   }
-  // counts per (table, category): one ballot each, summed over the waves
+  // the tables with a post-operation copy: one atomic per wave
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int m = 0; m < 7; m++)
+    if (__ballot((post >> m) & 1) && lane == 0) atomicOr(anypost, 1 << m);
+  // counts per (table, category): one ballot each, summed over the waves
   for (int m = 0; m < 7; m++)
     for (int k = 1; k <= 4; k++) {
       const unsigned long long b = __ballot(out[m] == k);
