@@ -358,6 +358,8 @@ def cpu_baseline(wl):
 
 
 def main():
+    import faulthandler
+    faulthandler.enable()  # a crash prints the Python stack that led to it
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -443,7 +445,9 @@ def main():
                               "fill ratio 0.8, maxlevel %d; rebuilt levels keep the %s ghost fill"
                               % (wl.finest, "flattened" if a.regrid_tables else "staged")) if rg else
                              "none: the workload's mesh held fixed",
-                   "hipgraph": bool(a.graph),
+                   # the library runs the loop body eagerly on meshes AMR built
+                   "hipgraph": ("frozen mesh only (the regrid step runs eagerly)" if rg and a.graph
+                                else bool(a.graph)),
                    "launches": "one per level" if a.batching else "one per patch",
                    "autotuned": tune_choices(res[head]["tune"])},
         "vcycles_per_step": vc,

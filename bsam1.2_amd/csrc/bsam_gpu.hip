@@ -15,6 +15,9 @@
 // reference's to the last bit except for the residual-norm sum.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <execinfo.h>
+#include <signal.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <array>
@@ -3235,6 +3238,8 @@ struct bsam_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool capturing = false;
+  // the installed mesh was built by bsam_amr: the loop body runs eagerly
+  bool amr_mesh = false;
   std::string tune_report;  // JSON of the last bsam_autotune
   long chain_cells = 0;     // largest level (cells) where the coarse chain starts
   int regrid_tables = 1;  // flattened ghost tables for the levels a regrid builds
@@ -3397,8 +3402,39 @@ struct PhaseClock {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// With BSAM_BACKTRACE set in the environment, a segmentation fault or bus
+// error of the process first writes the native call stack to stderr, then
+// goes to the handler that was there before (the default one ends the
+// process as before).
+namespace {
+struct sigaction g_prev_segv, g_prev_bus;
+void crash_backtrace(int sig, siginfo_t* info, void*) {
+  static const char head[] = "bsam: fatal signal, native stack:\n";
+  ssize_t w = write(2, head, sizeof head - 1);
+  (void)w;
+  char buf[64];
+  const int n = snprintf(buf, sizeof buf, "fault address %p\n", info ? info->si_addr : nullptr);
+  if (n > 0) w = write(2, buf, (size_t)n);
+  void* frames[64];
+  backtrace_symbols_fd(frames, backtrace(frames, 64), 2);
+  sigaction(sig, sig == SIGSEGV ? &g_prev_segv : &g_prev_bus, nullptr);
+}
+void install_backtrace() {
+  static bool done = false;
+  if (done || !getenv("BSAM_BACKTRACE")) return;
+  done = true;
+  struct sigaction sa {};
+  sa.sa_sigaction = crash_backtrace;
+  sa.sa_flags = SA_SIGINFO;
+  sigemptyset(&sa.sa_mask);
+  sigaction(SIGSEGV, &sa, &g_prev_segv);
+  sigaction(SIGBUS, &sa, &g_prev_bus);
+}
+}  // namespace
+
 extern "C" int32_t bsam_create(const bsam_params* params, bsam_ctx** out) {
   if (!params || !out) return set_err(BSAM_EINVAL, "null argument");
+  install_backtrace();
   *out = nullptr;
   const bsam_params& p = *params;
   if (p.ndims != 2 && p.ndims != 3) return set_err(BSAM_EINVAL, "ndims must be 2 or 3");
@@ -5976,6 +6012,7 @@ extern "C" int32_t bsam_set_hierarchy(bsam_ctx* c, int32_t npatch, const bsam_pa
   if (!c) return set_err(BSAM_EINVAL, "bad arguments");
   int keep = minlevel - 1;
   RETCHK(install(c, npatch, desc, minlevel, finestlevel, &keep));
+  c->amr_mesh = false;
   // a fresh install: the kept levels' launch shapes go back to the defaults
   for (int l = minlevel; l <= keep; l++) {
     Level& L = c->L(l);
@@ -6569,12 +6606,24 @@ static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass, bool lean) {
   return BSAM_OK;
 }
 
-extern "C" int32_t bsam_level_relax(bsam_ctx* c, int32_t level) {
+// LevelRelax; with `descend` (the pre-smoothing of a level the V-cycle
+// then restricts) the last SetGhost is the reduced table as well: after
+// the two reduced fills of a pass every face ghost is current, and until
+// the fill after CorrectFine only face ghosts are read (the coarse
+// right-hand side's stencils).  An edge or corner ghost left stale there is
+// changed by CorrectFine and then rewritten by the post-smoothing's last
+// fill, which is full and whose every expression reads interior and parent
+// cells only; the copies the fill after CorrectFine leaves out are
+// colour-1 face ghosts, which the reduced fill of ipass 2 writes.
+static int32_t level_relax_t(bsam_ctx* c, int32_t level, bool descend) {
   RETCHK(check_level(c, level));
   for (int s = 0; s < c->p.nsmoothingpasses; s++)
-    for (int rb = 1; rb <= 2; rb++) RETCHK(relax_and_ghost(c, level, rb, !(s == c->p.nsmoothingpasses - 1 && rb == 2)));
+    for (int rb = 1; rb <= 2; rb++)
+      RETCHK(relax_and_ghost(c, level, rb, descend || !(s == c->p.nsmoothingpasses - 1 && rb == 2)));
   return BSAM_OK;
 }
+
+extern "C" int32_t bsam_level_relax(bsam_ctx* c, int32_t level) { return level_relax_t(c, level, false); }
 
 static int32_t need_parent(bsam_ctx* c, int level) {
   RETCHK(check_level(c, level));
@@ -7291,7 +7340,7 @@ static int32_t launch_coarse_chain(bsam_ctx* c, int level, bool lin) {
 extern "C" int32_t bsam_vcycle(bsam_ctx* c, int32_t level) {
   RETCHK(check_level(c, level));
   if (c->L(level).chain > 0) return launch_coarse_chain(c, level, false);
-  RETCHK(bsam_level_relax(c, level));
+  RETCHK(level_relax_t(c, level, level > c->minlevel));
   if (level > c->minlevel) {
     // RestrictSolution + SetGhost(level-1, 0), fused when the parent is a
     // fully covered periodic patch
@@ -7356,8 +7405,12 @@ static bool kstat_any(bsam_ctx* c) {
 // One loop body on the context's stream, then the norm's copy to the host.
 // With use_graph the body is captured once and replayed; `eager` runs it
 // without the graph (the context keeps one graph, that of its own path).
+// On a mesh that bsam_amr built the body always runs eagerly: a host-side
+// segmentation fault inside hipGraphLaunch was seen on such meshes (FP64,
+// the graph captured again after every regrid) whose cause is not known,
+// so that combination is not used.
 static int32_t run_iteration(bsam_ctx* c, int32_t (*body)(bsam_ctx*), bool eager = false) {
-  if (c->p.use_graph && c->p.batching && !kstat_any(c) && !eager) {
+  if (c->p.use_graph && c->p.batching && !kstat_any(c) && !eager && !c->amr_mesh) {
     if (!c->gexec) {
       PhaseClock pc(c, BSAM_PHASE_GRAPH);
       HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
@@ -7614,6 +7667,7 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
   HIPCHK(hipStreamSynchronize(c->s));
   PhaseClock pc(c, BSAM_PHASE_AMR_INSTALL);
   OldMesh old(c);
+  c->amr_mesh = true;
   // the new table starts with the old levels at and below the root
   std::vector<bsam_patch_desc> H;
   std::vector<int> at(old.desc.size(), -1);

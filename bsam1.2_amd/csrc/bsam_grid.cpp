@@ -14,6 +14,7 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <exception>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -24,7 +25,9 @@
 namespace {
 
 // The host threads of bsam::parallel_for: workers 1..size() wait for a job
-// (a generation count), run their share and count down.
+// (a generation count), run their share and count down.  A worker starts
+// at the generation current when it is created, so one added for a job
+// takes that job and no earlier one (whose function no longer exists).
 struct HostPool {
   std::mutex m, busy;
   std::condition_variable cv, done;
@@ -43,8 +46,7 @@ struct HostPool {
     for (auto& t : th)
       if (t.joinable()) t.join();
   }
-  void loop(int w) {
-    long seen = 0;
+  void loop(int w, long seen) {
     for (;;) {
       const std::function<void(int)>* f;
       int n;
@@ -77,12 +79,23 @@ void spawn_for(int n, const std::function<void(int)>& fn) {
   std::vector<std::thread> th;
   int started = 1;
   try {
-    for (; started < n; started++) th.emplace_back([&fn, started]() { fn(started); });
+    for (; started < n; started++) th.emplace_back([&fn, started]() {
+        try {
+          fn(started);
+        } catch (...) {
+        }
+      });
   } catch (...) {
   }
-  for (int t = started; t < n; t++) fn(t);
-  fn(0);
+  std::exception_ptr err;
+  try {
+    for (int t = started; t < n; t++) fn(t);
+    fn(0);
+  } catch (...) {
+    err = std::current_exception();
+  }
   for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
 }
 
 }  // namespace
@@ -103,7 +116,9 @@ void parallel_for(int n, const std::function<void(int)>& fn) {
   try {
     while ((int)P.th.size() < n - 1) {
       const int w = (int)P.th.size() + 1;
-      P.th.emplace_back([&P, w]() { P.loop(w); });
+      // gen changes only under `busy`, which this thread holds
+      const long g0 = P.gen;
+      P.th.emplace_back([&P, w, g0]() { P.loop(w, g0); });
     }
   } catch (...) {
   }
@@ -116,10 +131,20 @@ void parallel_for(int n, const std::function<void(int)>& fn) {
     P.gen++;
   }
   P.cv.notify_all();
-  fn(0);
-  for (int t = nw + 1; t < n; t++) fn(t);
-  std::unique_lock<std::mutex> lk(P.m);
-  P.done.wait(lk, [&] { return P.left == 0; });
+  // the workers use fn until they have counted down: wait for them before
+  // an exception of this thread's share leaves
+  std::exception_ptr err;
+  try {
+    fn(0);
+    for (int t = nw + 1; t < n; t++) fn(t);
+  } catch (...) {
+    err = std::current_exception();
+  }
+  {
+    std::unique_lock<std::mutex> lk(P.m);
+    P.done.wait(lk, [&] { return P.left == 0; });
+  }
+  if (err) std::rethrow_exception(err);
 }
 
 void periodic_offsets(int ndims, const bsam_patch_desc& root, int level,
