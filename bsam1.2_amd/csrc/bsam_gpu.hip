@@ -1762,15 +1762,18 @@ __global__ __launch_bounds__(kBlock) void k_init_fields(LevelArgs L, int nv, T* 
 #pragma unroll
         for (int dx = 0; dx < 3; dx++)
           nb[dz][dy][dx] = a[cidx(P, cc.ic - 1 + dx, cc.jc - 1 + dy, DIM == 3 ? cc.kc - 1 + dz : 0)];
-    // p1 of a child from the neighbourhood, in p1's operation order
-    auto p1n = [&](int i, int j, int k) -> T {
-      const int ox = (i >> 1) - (cc.ic - 1), oy = (j >> 1) - (cc.jc - 1);
+    // p1 of child n from the neighbourhood, in p1's operation order; the
+    // neighbourhood offsets (i >> 1) - (ic - 1) etc. are 1 for an even
+    // child index and 0 for an odd one, written from n so that the array
+    // stays in registers
+    auto p1n = [&](int n, int i, int j, int k) -> T {
+      const int ox = 1 - (n & 1), oy = 1 - ((n >> 1) & 1);
       if (DIM == 2) {
         const T b0 = linw<T>(nb[0][oy][ox], nb[0][oy][ox + 1], i);
         const T b1 = linw<T>(nb[0][oy + 1][ox], nb[0][oy + 1][ox + 1], i);
         return linw<T>(b0, b1, j) / (T)16;
       }
-      const int oz = (k >> 1) - (cc.kc - 1);
+      const int oz = 1 - (n >> 2);
       T c2[2];
 #pragma unroll
       for (int d = 0; d < 2; d++) {
@@ -1785,7 +1788,7 @@ __global__ __launch_bounds__(kBlock) void k_init_fields(LevelArgs L, int nv, T* 
     T sum = (T)0;
 #pragma unroll
     for (int n = 0; n < nch; n++) {
-      ch[n] = p1n(2 * cc.ic - (n & 1), 2 * cc.jc - ((n >> 1) & 1), DIM == 3 ? 2 * cc.kc - (n >> 2) : 0);
+      ch[n] = p1n(n, 2 * cc.ic - (n & 1), 2 * cc.jc - ((n >> 1) & 1), DIM == 3 ? 2 * cc.kc - (n >> 2) : 0);
       sum = n == 0 ? ch[0] : sum + ch[n];
     }
     const T cor = nb[nz / 2][1][1] - (DIM == 3 ? (T)0.125 : (T)0.25) * sum;
@@ -3713,138 +3716,198 @@ __host__ __device__ inline int interp_elems(const DPatch& P, int ndims) {
 
 // The rule of element e of patch P (parent Q) and its colour-filter
 // segment; false when the element's face has no coarse-fine interpolation.
+// Every coordinate array below is indexed by compile-time positions only
+// (the face or edge direction is a template argument), so on the GPU the
+// points stay in registers.
+struct IrPts {
+  int g[3], s1[3], s2[3], cq[4][3];
+  int nco;
+};
+__host__ __device__ __forceinline__ bool ir_emit(const DPatch& P, const DPatch& Q, int ndims, const IrPts& t,
+                                                 GRule& rule) {
+  const bool d3 = ndims == 3;
+  rule.dst = qidx(P, t.g[0], t.g[1], d3 ? t.g[2] : 0);
+  rule.s1 = qidx(P, t.s1[0], t.s1[1], d3 ? t.s1[2] : 0);
+  rule.s2 = qidx(P, t.s2[0], t.s2[1], d3 ? t.s2[2] : 0);
+#pragma unroll
+  for (int n = 0; n < 4; n++) rule.c[n] = (n < t.nco) ? qidx(Q, t.cq[n][0], t.cq[n][1], d3 ? t.cq[n][2] : 0) : 0;
+  rule.kind = t.nco;
+  rule.pvs = P.qvs;
+  rule.cvs = Q.qvs;
+  return true;
+}
+// the boundary code of face (dim, parity)
+template <int D>
+__host__ __device__ __forceinline__ int ir_mth(const DPatch& P, int parity) {
+  return parity == 1 ? P.mthbc[2 * D] : P.mthbc[2 * D + 1];
+}
+// 2D face cell pos of face (D, p) (boundary.f90:143-333)
+template <int D>
+__host__ __device__ __forceinline__ bool ir_face2(const DPatch& P, const DPatch& Q, int p, int pos, GRule& rule,
+                                                  int& sg) {
+  constexpr int T = 1 - D;
+  if (!bc_interp(ir_mth<D>(P, p))) return false;
+  IrPts t;
+  t.nco = 1;
+  t.g[2] = t.s1[2] = t.s2[2] = 1;
+#pragma unroll
+  for (int n = 0; n < 4; n++) t.cq[n][2] = 1;
+  int fi[5];
+  face_index(P, D, p, fi);
+  sg = ((fi[0] + pos) & 1) ? 2 : 0;
+  const int k = pos_kind(pos, P.mx[T]);
+  t.g[D] = fi[0]; t.s1[D] = fi[1]; t.s2[D] = fi[2];
+  t.g[T] = pos; t.s1[T] = pos + kind_s1(k); t.s2[T] = pos + kind_s2(k);
+  if (k == KE || k == KO) {
+    t.cq[0][D] = fi[3]; t.cq[0][T] = kind_c(P, T, k, pos, 0);
+  } else {
+    t.nco = 3;
+#pragma unroll
+    for (int r = 0; r < 3; r++) { t.cq[r][D] = fi[3]; t.cq[r][T] = kind_c(P, T, k, pos, r); }
+  }
+  return ir_emit(P, Q, 2, t, rule);
+}
+// 2D corner cn (boundary.f90:155-181): applied whatever mthbc is
+__host__ __device__ __forceinline__ bool ir_corner2(const DPatch& P, const DPatch& Q, int cn, GRule& rule, int& sg) {
+  IrPts t;
+  t.nco = 1;
+  t.g[2] = t.s1[2] = t.s2[2] = 1;
+#pragma unroll
+  for (int n = 0; n < 4; n++) t.cq[n][2] = 1;
+  int f0[5], f1[5];
+  face_index(P, 0, 1 + (cn & 1), f0);
+  face_index(P, 1, 1 + (cn >> 1), f1);
+  sg = ((f0[0] + f1[0]) & 1) ? 2 : 0;
+  t.g[0] = f0[0]; t.s1[0] = f0[1]; t.s2[0] = f0[2]; t.cq[0][0] = f0[3];
+  t.g[1] = f1[0]; t.s1[1] = f1[1]; t.s2[1] = f1[2]; t.cq[0][1] = f1[3];
+  return ir_emit(P, Q, 2, t, rule);
+}
+// 3D face cell `local` of face (D, p), with the grey face-corner cells
+// (boundary.f90:1048-1550)
+template <int D>
+__host__ __device__ __forceinline__ bool ir_face3(const DPatch& P, const DPatch& Q, int p, int local, GRule& rule,
+                                                  int& sg) {
+  constexpr int T1 = (D == 0) ? 1 : 0, T2 = (D == 2) ? 1 : 2;
+  if (!bc_interp(ir_mth<D>(P, p))) return false;
+  IrPts t;
+  t.nco = 1;
+  const int a = local % P.mx[T1] + 1, b = local / P.mx[T1] + 1;
+  const int k1 = pos_kind(a, P.mx[T1]), k2 = pos_kind(b, P.mx[T2]);
+  const bool q1 = k1 >= KL, q2 = k2 >= KL;
+  int fi[5];
+  face_index(P, D, p, fi);
+  t.g[D] = fi[0]; t.s1[D] = fi[1]; t.s2[D] = fi[2];
+  if (q1 && q2) {
+    // face-corner grey cells (boundary.f90:1320-1333 and the like)
+    int ft1[5], ft2[5];
+    face_index(P, T1, a == 1 ? 1 : 2, ft1);
+    face_index(P, T2, b == 1 ? 1 : 2, ft2);
+    t.g[T1] = t.s1[T1] = t.s2[T1] = ft1[2];
+    t.g[T2] = t.s1[T2] = t.s2[T2] = ft2[2];
+    t.nco = 4;
+#pragma unroll
+    for (int r = 0; r < 4; r++) t.cq[r][D] = fi[3];
+    t.cq[0][T1] = ft1[4]; t.cq[0][T2] = ft2[4];
+    t.cq[1][T1] = ft1[3]; t.cq[1][T2] = ft2[4];
+    t.cq[2][T1] = ft1[4]; t.cq[2][T2] = ft2[3];
+    t.cq[3][T1] = ft1[3]; t.cq[3][T2] = ft2[3];
+    sg = 1;
+  } else {
+    sg = ((fi[0] + a + b) & 1) ? 2 : 0;
+    t.g[T1] = a; t.s1[T1] = a + kind_s1(k1); t.s2[T1] = a + kind_s2(k1);
+    t.g[T2] = b; t.s1[T2] = b + kind_s1(k2); t.s2[T2] = b + kind_s2(k2);
+    if (!q1 && !q2) {
+      t.cq[0][D] = fi[3]; t.cq[0][T1] = kind_c(P, T1, k1, a, 0); t.cq[0][T2] = kind_c(P, T2, k2, b, 0);
+    } else {
+      t.nco = 3;
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        t.cq[r][D] = fi[3];
+        t.cq[r][T1] = kind_c(P, T1, k1, a, q1 ? r : 0);
+        t.cq[r][T2] = kind_c(P, T2, k2, b, q2 ? r : 0);
+      }
+    }
+  }
+  return ir_emit(P, Q, 3, t, rule);
+}
+// 3D edge cell: edges along dim 1 (EE 0), dim 2 (EE 1), dim 0 (EE 2);
+// `rem` counts from the first edge of that group
+template <int EE>
+__host__ __device__ __forceinline__ bool ir_edge3(const DPatch& P, const DPatch& Q, int rem, GRule& rule, int& sg) {
+  constexpr int ED = (EE == 0) ? 1 : (EE == 1 ? 2 : 0);
+  constexpr int FA = (ED == 0) ? 1 : 0, FB = (ED == 2) ? 1 : 2;
+  const int ne = P.mx[ED];
+  const int combo = rem / ne, pos = rem % ne + 1;
+  const int pa = 1 + (combo >> 1), pb = 1 + (combo & 1);
+  bool skip1 = !bc_interp(ir_mth<FA>(P, pa));
+  const int cb = ir_mth<FB>(P, pb);
+  if (EE == 0) skip1 = skip1 || (cb != 2 && cb != 99);  // boundary.f90:1104
+  else skip1 = skip1 || !bc_interp(cb);
+  sg = skip1 ? 2 : 1;
+  IrPts t;
+  t.nco = 1;
+  int fa[5], fb[5];
+  face_index(P, FA, pa, fa);
+  face_index(P, FB, pb, fb);
+  const int k = pos_kind(pos, P.mx[ED]);
+  t.g[FA] = fa[0]; t.s1[FA] = fa[1]; t.s2[FA] = fa[2];
+  t.g[FB] = fb[0]; t.s1[FB] = fb[1]; t.s2[FB] = fb[2];
+  t.g[ED] = pos; t.s1[ED] = pos + kind_s1(k); t.s2[ED] = pos + kind_s2(k);
+  if (k == KE || k == KO) {
+    t.cq[0][FA] = fa[3]; t.cq[0][FB] = fb[3]; t.cq[0][ED] = kind_c(P, ED, k, pos, 0);
+  } else {
+    t.nco = 3;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      t.cq[r][FA] = fa[3]; t.cq[r][FB] = fb[3]; t.cq[r][ED] = kind_c(P, ED, k, pos, r);
+    }
+  }
+  return ir_emit(P, Q, 3, t, rule);
+}
+// 3D corner cn of the ghost shell
+__host__ __device__ __forceinline__ bool ir_corner3(const DPatch& P, const DPatch& Q, int cn, GRule& rule, int& sg) {
+  const int p1 = 1 + (cn & 1), p2 = 1 + ((cn >> 1) & 1), p3 = 1 + (cn >> 2);
+  const bool skip1 = !bc_interp(ir_mth<0>(P, p1)) || !bc_interp(ir_mth<1>(P, p2)) || !bc_interp(ir_mth<2>(P, p3));
+  sg = skip1 ? 2 : 1;
+  IrPts t;
+  t.nco = 1;
+  int f0[5], f1[5], f2[5];
+  face_index(P, 0, p1, f0);
+  face_index(P, 1, p2, f1);
+  face_index(P, 2, p3, f2);
+  t.g[0] = f0[0]; t.s1[0] = f0[1]; t.s2[0] = f0[2]; t.cq[0][0] = f0[3];
+  t.g[1] = f1[0]; t.s1[1] = f1[1]; t.s2[1] = f1[2]; t.cq[0][1] = f1[3];
+  t.g[2] = f2[0]; t.s1[2] = f2[1]; t.s2[2] = f2[2]; t.cq[0][2] = f2[3];
+  return ir_emit(P, Q, 3, t, rule);
+}
 __host__ __device__ inline bool interp_rule_at(const DPatch& P, const DPatch& Q, int ndims, int e, GRule& rule,
                                                int& sg) {
-  const int* mth = P.mthbc;
-  auto k3 = [&](int k) { return ndims == 3 ? k : 0; };
-  int g[3], s1[3], s2[3], cq[4][3];
-  int nco = 1;
-  auto emit = [&]() {
-    rule.dst = qidx(P, g[0], g[1], k3(g[2]));
-    rule.s1 = qidx(P, s1[0], s1[1], k3(s1[2]));
-    rule.s2 = qidx(P, s2[0], s2[1], k3(s2[2]));
-    for (int n = 0; n < 4; n++) rule.c[n] = (n < nco) ? qidx(Q, cq[n][0], cq[n][1], k3(cq[n][2])) : 0;
-    rule.kind = nco;
-    rule.pvs = P.qvs;
-    rule.cvs = Q.qvs;
-    return true;
-  };
-  int fi[3][5];
   if (ndims == 2) {
     const int nf0 = P.mx[1], nf1 = P.mx[0];
-    g[2] = s1[2] = s2[2] = 1;
-    for (int n = 0; n < 4; n++) cq[n][2] = 1;
+    if (e < 2 * nf0) return ir_face2<0>(P, Q, 1 + e / nf0, e % nf0 + 1, rule, sg);
     if (e < 2 * nf0 + 2 * nf1) {
-      int d, p, pos;
-      if (e < 2 * nf0) { d = 0; p = 1 + e / nf0; pos = e % nf0 + 1; }
-      else { int e2 = e - 2 * nf0; d = 1; p = 1 + e2 / nf1; pos = e2 % nf1 + 1; }
-      if (!bc_interp(mth[2 * d + p - 1])) return false;
-      const int t = 1 - d;
-      face_index(P, d, p, fi[d]);
-      sg = ((fi[d][0] + pos) & 1) ? 2 : 0;
-      const int k = pos_kind(pos, P.mx[t]);
-      g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
-      g[t] = pos; s1[t] = pos + kind_s1(k); s2[t] = pos + kind_s2(k);
-      if (k == KE || k == KO) {
-        cq[0][d] = fi[d][3]; cq[0][t] = kind_c(P, t, k, pos, 0);
-      } else {
-        nco = 3;
-        for (int r = 0; r < 3; r++) { cq[r][d] = fi[d][3]; cq[r][t] = kind_c(P, t, k, pos, r); }
-      }
-    } else {
-      // corners (boundary.f90:155-181): applied whatever mthbc is
-      const int cn = e - (2 * nf0 + 2 * nf1);
-      const int p0 = 1 + (cn & 1), p1 = 1 + (cn >> 1);
-      face_index(P, 0, p0, fi[0]);
-      face_index(P, 1, p1, fi[1]);
-      sg = ((fi[0][0] + fi[1][0]) & 1) ? 2 : 0;
-      for (int a = 0; a < 2; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
+      const int e2 = e - 2 * nf0;
+      return ir_face2<1>(P, Q, 1 + e2 / nf1, e2 % nf1 + 1, rule, sg);
     }
-    return emit();
+    return ir_corner2(P, Q, e - (2 * nf0 + 2 * nf1), rule, sg);
   }
   const int mx = P.mx[0], my = P.mx[1], mz = P.mx[2];
-  const int nface[3] = {my * mz, mx * mz, mx * my};
-  const int nfaces = 2 * (nface[0] + nface[1] + nface[2]);
-  const int nedge[3] = {my, mz, mx};  // edges along dim 1 (ee=0), dim 2 (ee=1), dim 0 (ee=2)
-  const int nedges = 4 * (nedge[0] + nedge[1] + nedge[2]);
-  if (e < nfaces) {
-    int d = 0, rem = e;
-    while (rem >= 2 * nface[d]) { rem -= 2 * nface[d]; d++; }
-    const int p = 1 + rem / nface[d];
-    const int local = rem % nface[d];
-    if (!bc_interp(mth[2 * d + p - 1])) return false;
-    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
-    const int a = local % P.mx[t1] + 1, b = local / P.mx[t1] + 1;
-    const int k1 = pos_kind(a, P.mx[t1]), k2 = pos_kind(b, P.mx[t2]);
-    const bool q1 = k1 >= KL, q2 = k2 >= KL;
-    face_index(P, d, p, fi[d]);
-    g[d] = fi[d][0]; s1[d] = fi[d][1]; s2[d] = fi[d][2];
-    if (q1 && q2) {
-      // face-corner grey cells (boundary.f90:1320-1333 and the like)
-      int ft1[5], ft2[5];
-      face_index(P, t1, a == 1 ? 1 : 2, ft1);
-      face_index(P, t2, b == 1 ? 1 : 2, ft2);
-      g[t1] = s1[t1] = s2[t1] = ft1[2];
-      g[t2] = s1[t2] = s2[t2] = ft2[2];
-      nco = 4;
-      for (int r = 0; r < 4; r++) cq[r][d] = fi[d][3];
-      cq[0][t1] = ft1[4]; cq[0][t2] = ft2[4];
-      cq[1][t1] = ft1[3]; cq[1][t2] = ft2[4];
-      cq[2][t1] = ft1[4]; cq[2][t2] = ft2[3];
-      cq[3][t1] = ft1[3]; cq[3][t2] = ft2[3];
-      sg = 1;
-    } else {
-      sg = ((fi[d][0] + a + b) & 1) ? 2 : 0;
-      g[t1] = a; s1[t1] = a + kind_s1(k1); s2[t1] = a + kind_s2(k1);
-      g[t2] = b; s1[t2] = b + kind_s1(k2); s2[t2] = b + kind_s2(k2);
-      if (!q1 && !q2) {
-        cq[0][d] = fi[d][3]; cq[0][t1] = kind_c(P, t1, k1, a, 0); cq[0][t2] = kind_c(P, t2, k2, b, 0);
-      } else {
-        nco = 3;
-        for (int r = 0; r < 3; r++) {
-          cq[r][d] = fi[d][3];
-          cq[r][t1] = kind_c(P, t1, k1, a, q1 ? r : 0);
-          cq[r][t2] = kind_c(P, t2, k2, b, q2 ? r : 0);
-        }
-      }
-    }
-  } else if (e < nfaces + nedges) {
-    int rem = e - nfaces, ee = 0;
-    while (rem >= 4 * nedge[ee]) { rem -= 4 * nedge[ee]; ee++; }
-    const int ed = (ee == 0) ? 1 : (ee == 1 ? 2 : 0);
-    const int fa = (ed == 0) ? 1 : 0, fb = (ed == 2) ? 1 : 2;
-    const int combo = rem / nedge[ee], pos = rem % nedge[ee] + 1;
-    const int pa = 1 + (combo >> 1), pb = 1 + (combo & 1);
-    bool skip1 = !bc_interp(mth[2 * fa + pa - 1]);
-    const int cb = mth[2 * fb + pb - 1];
-    if (ee == 0) skip1 = skip1 || (cb != 2 && cb != 99);  // boundary.f90:1104
-    else skip1 = skip1 || !bc_interp(cb);
-    sg = skip1 ? 2 : 1;
-    face_index(P, fa, pa, fi[fa]);
-    face_index(P, fb, pb, fi[fb]);
-    const int k = pos_kind(pos, P.mx[ed]);
-    g[fa] = fi[fa][0]; s1[fa] = fi[fa][1]; s2[fa] = fi[fa][2];
-    g[fb] = fi[fb][0]; s1[fb] = fi[fb][1]; s2[fb] = fi[fb][2];
-    g[ed] = pos; s1[ed] = pos + kind_s1(k); s2[ed] = pos + kind_s2(k);
-    if (k == KE || k == KO) {
-      cq[0][fa] = fi[fa][3]; cq[0][fb] = fi[fb][3]; cq[0][ed] = kind_c(P, ed, k, pos, 0);
-    } else {
-      nco = 3;
-      for (int r = 0; r < 3; r++) {
-        cq[r][fa] = fi[fa][3]; cq[r][fb] = fi[fb][3]; cq[r][ed] = kind_c(P, ed, k, pos, r);
-      }
-    }
-  } else {
-    const int cn = e - nfaces - nedges;
-    const int p1 = 1 + (cn & 1), p2 = 1 + ((cn >> 1) & 1), p3 = 1 + (cn >> 2);
-    const bool skip1 = !bc_interp(mth[p1 - 1]) || !bc_interp(mth[p2 + 1]) || !bc_interp(mth[p3 + 3]);
-    sg = skip1 ? 2 : 1;
-    face_index(P, 0, p1, fi[0]);
-    face_index(P, 1, p2, fi[1]);
-    face_index(P, 2, p3, fi[2]);
-    for (int a = 0; a < 3; a++) { g[a] = fi[a][0]; s1[a] = fi[a][1]; s2[a] = fi[a][2]; cq[0][a] = fi[a][3]; }
-  }
-  return emit();
+  const int nf0 = my * mz, nf1 = mx * mz, nf2 = mx * my;
+  int rem = e;
+  if (rem < 2 * nf0) return ir_face3<0>(P, Q, 1 + rem / nf0, rem % nf0, rule, sg);
+  rem -= 2 * nf0;
+  if (rem < 2 * nf1) return ir_face3<1>(P, Q, 1 + rem / nf1, rem % nf1, rule, sg);
+  rem -= 2 * nf1;
+  if (rem < 2 * nf2) return ir_face3<2>(P, Q, 1 + rem / nf2, rem % nf2, rule, sg);
+  rem -= 2 * nf2;
+  // edges along dim 1 (4 my cells), dim 2 (4 mz), dim 0 (4 mx)
+  if (rem < 4 * my) return ir_edge3<0>(P, Q, rem, rule, sg);
+  rem -= 4 * my;
+  if (rem < 4 * mz) return ir_edge3<1>(P, Q, rem, rule, sg);
+  rem -= 4 * mz;
+  if (rem < 4 * mx) return ir_edge3<2>(P, Q, rem, rule, sg);
+  rem -= 4 * mx;
+  return ir_corner3(P, Q, rem, rule, sg);
 }
 
 static void interp_rules(const DPatch& P, const DPatch& Q, int ndims, std::vector<GRule> (&seg)[3]) {
@@ -5102,8 +5165,9 @@ __device__ __forceinline__ bool dp_same_expr(const DPlanArgs& A, const DExpr& a,
   const GRule& r = ra;
   const GRule& s = rb;
   if (r.kind != s.kind || r.s1 != s.s1 || r.s2 != s.s2 || r.cvs != s.cvs) return false;
-  for (int n = 0; n < r.kind && n < 4; n++)
-    if (r.c[n] != s.c[n]) return false;
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+    if (n < r.kind && r.c[n] != s.c[n]) return false;
   return true;
 }
 
@@ -5135,6 +5199,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
     const DPatch D = A.pt[pid];
     DExpr e[3];
     bool ok[3];
+#pragma unroll
     for (int p = 0; p < 3; p++) {
       e[p] = M.ex[p][g];
       ok[p] = (e[p].meta & kDxValid) != 0;
@@ -5154,14 +5219,20 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
           int s3[3];
           qdecode(S, x.src, s3[0], s3[1], s3[2]);
           exact = n_outside(S, A.nd, s3[0], s3[1], s3[2]) == 0;
-          for (int b = 0; b < A.nd && exact; b++) {
+          // (constant bounds with a guard on nd keep s3 and d3 in
+          // registers)
+#pragma unroll
+          for (int b = 0; b < 3; b++) {
+            if (b >= A.nd || !exact) continue;
             const int per = M.per[b];
             const int ds = (M.mglo[3 * x.spid + b] + s3[b]) - (M.mglo[3 * pid + b] + d3[b]);
             exact = ((ds % per) + per) % per == 0;
           }
           for (int corner = 0; corner < (1 << A.nd) && exact; corner++) {
             long idx = 0;
-            for (int b = A.nd - 1; b >= 0 && exact; b--) {
+#pragma unroll
+            for (int b = 2; b >= 0; b--) {
+              if (b >= A.nd || !exact) continue;
               const int gf = M.mglo[3 * pid + b] + d3[b] - 1;
               const int lo = (gf & 1) ? (gf + 1) / 2 - 1 : gf / 2;
               int xx = lo + ((corner >> b) & 1) - 1;
@@ -5181,11 +5252,13 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
     // map[2 + p], p = 1, 2: what a red-black sweep sequence needs (the
     // interpolation rules of the two calls' expressions made once)
     GRule rr[3];
+#pragma unroll
     for (int p = 1; p <= 2; p++)
       if (ok[p] && dx_kind(e[p].meta) == 1) {
         if (p == 2 && ok[1] && dx_kind(e[1].meta) == 1 && e[1].dpid == e[2].dpid && e[1].src == e[2].src) rr[2] = rr[1];
         else rr[p] = dp_rule(A, e[p].dpid, e[p].src);
       }
+#pragma unroll
     for (int p = 1; p <= 2; p++) {
       if (!ok[p]) continue;
       const DExpr& x = e[p];
@@ -5221,6 +5294,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_select(DPlanArgs A, DPlanMaps M, 
       }
       if (need) out[2 + p] = out[p];
     }
+#pragma unroll
     for (int m = 0; m < 7; m++) {
       cat[m * n + g] = out[m];
       if (out[m] && out[m] <= 2 && dx_post(e[dp_call(m)].meta)) post |= 1 << m;
@@ -5273,6 +5347,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, lo
   const unsigned long long below = (1ull << lane) - 1ull;
   uint8_t c[7];
   int rank[7];
+#pragma unroll
   for (int m = 0; m < 7; m++) {
     c[m] = g >= 0 ? cat[m * n + g] : 0;
     rank[m] = 0;
@@ -5288,6 +5363,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, lo
   const int dst = qidx(A.pt[pid], d3[0], d3[1], d3[2]);
   GRule rule{};
   int rpid = -1, relem = -1;
+#pragma unroll
   for (int m = 0; m < 7; m++) {
     const int k = c[m];
     if (!k) continue;
