@@ -2882,12 +2882,14 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle(const DPatch* __restrict
 // k_prolong_dual) and the box stages of k_boxops_seq, so the results are
 // bit-identical.  LIN=false: FP64; LIN=true: the FP32 correction cycle.
 constexpr int kMaxChain = 12;
+// The chain's levels, fixed for a hierarchy: set where the chain is planned
+// and read by the kernel from device memory (indexing a by-value kernel
+// argument of this size by level would copy it to every lane's scratch).
 struct TmChainArgs {
   int pid[kMaxChain];
   BoxSeq seq[kMaxChain];
   int physop[kMaxChain];  // face operation when every face has condition 10, else -1
   int nlev;
-  ModelK m;
 };
 
 // SetGhost(level, ipass) of a chain level: the face stages in order, or,
@@ -2905,7 +2907,8 @@ __device__ __forceinline__ void tc_fill(const DPatch* __restrict__ pt, const TmC
 }
 
 template <int DIM, bool LIN>
-__device__ void tc_level_relax(const DPatch* __restrict__ pt, const TmChainArgs& A, int l, int nsm,
+__device__ __forceinline__ void tc_level_relax(const DPatch* __restrict__ pt, const TmChainArgs& A, const ModelK& m,
+                                               int l, int nsm,
                                typename std::conditional<LIN, float, double>::type* __restrict__ q,
                                const typename std::conditional<LIN, float, double>::type* __restrict__ f,
                                const double* __restrict__ qold, const float* __restrict__ coef) {
@@ -2922,8 +2925,8 @@ __device__ void tc_level_relax(const DPatch* __restrict__ pt, const TmChainArgs&
         const int ib = (c + j + k) & 1;
         const int i = 2 * t + 2 - ib;
         if (i > P.mx[0]) continue;
-        if constexpr (LIN) relax_lin_tm_cell<DIM, 0>(P, i, j, k, 0, A.m, q, f, coef, nullptr);
-        else relax_tm_cell<DIM>(P, i, j, k, A.m, q, qold, f);
+        if constexpr (LIN) relax_lin_tm_cell<DIM, 0>(P, i, j, k, 0, m, q, f, coef, nullptr);
+        else relax_tm_cell<DIM>(P, i, j, k, m, q, qold, f);
       }
       __syncthreads();
       tc_fill<DIM, T>(pt, A, l, LIN ? 1 : 0, q);
@@ -2932,16 +2935,17 @@ __device__ void tc_level_relax(const DPatch* __restrict__ pt, const TmChainArgs&
 
 template <int DIM, bool LIN>
 __global__ __launch_bounds__(1024) void k_coarse_vcycle_tm(
-    const DPatch* __restrict__ pt, TmChainArgs A, int nsm, typename std::conditional<LIN, float, double>::type* __restrict__ q,
+    const DPatch* __restrict__ pt, const TmChainArgs* __restrict__ Ap, ModelK m, int nsm,
+    typename std::conditional<LIN, float, double>::type* __restrict__ q,
     typename std::conditional<LIN, float, double>::type* __restrict__ f,
     typename std::conditional<LIN, float, double>::type* __restrict__ qc, const double* __restrict__ qold,
     const float* __restrict__ coef) {
   using T = typename std::conditional<LIN, float, double>::type;
   constexpr int NV = 4;
-  const ModelK m = A.m;
+  const TmChainArgs& A = *Ap;
   // descend
   for (int l = 0; l < A.nlev - 1; l++) {
-    tc_level_relax<DIM, LIN>(pt, A, l, nsm, q, f, qold, coef);
+    tc_level_relax<DIM, LIN>(pt, A, m, l, nsm, q, f, qold, coef);
     const DPatch P = pt[A.pid[l]];
     const DPatch Q = pt[A.pid[l + 1]];
     const int ncc = P.cmx[0] * P.cmx[1] * (DIM == 3 ? P.cmx[2] : 1);
@@ -3014,7 +3018,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle_tm(
     }
     __syncthreads();
   }
-  tc_level_relax<DIM, LIN>(pt, A, A.nlev - 1, nsm, q, f, qold, coef);
+  tc_level_relax<DIM, LIN>(pt, A, m, A.nlev - 1, nsm, q, f, qold, coef);
   // ascend
   for (int l = A.nlev - 2; l >= 0; l--) {
     const DPatch P = pt[A.pid[l]];
@@ -3072,7 +3076,7 @@ __global__ __launch_bounds__(1024) void k_coarse_vcycle_tm(
     }
     __syncthreads();
     tc_fill<DIM, T>(pt, A, l, LIN ? 1 : 0, q);  // SetGhost(L, 0)
-    tc_level_relax<DIM, LIN>(pt, A, l, nsm, q, f, qold, coef);
+    tc_level_relax<DIM, LIN>(pt, A, m, l, nsm, q, f, qold, coef);
   }
 }
 
@@ -3215,6 +3219,10 @@ struct bsam_ctx {
   int partial_cap = 0;
   double* d_norm = nullptr;
   double* h_norm = nullptr;
+  // the tumour model's coarse chain (k_coarse_vcycle_tm), as plan_chain
+  // sets it, and its device copy
+  TmChainArgs h_tmchain{};
+  TmChainArgs* d_tmchain = nullptr;
   // staging
   double* d_stage = nullptr;
   long stage_cap = 0;
@@ -3603,6 +3611,7 @@ extern "C" int32_t bsam_destroy(bsam_ctx* c) {
   hipFree(c->d_tagbits);
   hipFree(c->d_tagglo);
   hipFree(c->dplan);
+  hipFree(c->d_tmchain);
   if (c->h_norm) hipHostFree(c->h_norm);
   if (c->h_flags) hipHostFree(c->h_flags);
   for (auto& k : c->ks)
@@ -5653,7 +5662,7 @@ namespace {
 // the tumour model: levels of one patch each whose fill is a short
 // sequence of box stages that one workgroup runs in order
 // (k_coarse_vcycle_tm).  A chain has at least two levels.
-void plan_chain(bsam_ctx* c) {
+int32_t plan_chain(bsam_ctx* c) {
   for (Level& L : c->lv) L.chain = 0;
   int top = c->minlevel - 1;
   if (c->p.model == BSAM_MODEL_CH) {
@@ -5678,9 +5687,34 @@ void plan_chain(bsam_ctx* c) {
       if (maxel * c->p.nrvars > kSeqBoxElems || (long)P.mx[0] * P.mx[1] * P.mx[2] > c->chain_cells) break;
       top = l;
     }
-    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= kMaxChain)
-      c->L(top).chain = top - c->minlevel + 1;
+    if (top >= c->minlevel && top - c->minlevel + 1 >= 2 && top - c->minlevel + 1 <= kMaxChain) {
+      Level& T = c->L(top);
+      T.chain = top - c->minlevel + 1;
+      // the chain's levels for the kernel, top first
+      TmChainArgs& A = c->h_tmchain;
+      A = TmChainArgs{};
+      A.nlev = T.chain;
+      for (int n = 0; n < T.chain; n++) {
+        const Level& Ln = c->L(top - n);
+        A.pid[n] = Ln.patches[0];
+        A.seq[n].ns = (int)Ln.stages.size();
+        for (int st = 0; st < A.seq[n].ns; st++) {
+          A.seq[n].d[st] = Ln.stages[st].d;
+          A.seq[n].n[st] = Ln.stages[st].n;
+        }
+        // one patch, condition 10 on every face: its stages are the faces
+        // x, y, z, which phys_shell evaluates in one pass
+        bool allphys = Ln.wrap_pid < 0 && (int)Ln.stages.size() == c->p.ndims;
+        for (int f = 0; f < 2 * c->p.ndims; f++) allphys = allphys && c->desc[A.pid[n]].mthbc[f] == 10;
+        A.physop[n] = allphys ? phys_face_op(c) : -1;
+      }
+      if (!c->d_tmchain) HIPCHK(hipMalloc(&c->d_tmchain, sizeof(TmChainArgs)));
+      // after every earlier chain launch, and complete on return
+      HIPCHK(hipMemcpyAsync(c->d_tmchain, &c->h_tmchain, sizeof(TmChainArgs), hipMemcpyHostToDevice, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+    }
   }
+  return BSAM_OK;
 }
 
 // The host half of bsam_set_hierarchy: descriptor validation, the device
@@ -6066,7 +6100,7 @@ int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_
     // per-patch launches round every patch up to whole blocks
     maxblocks = std::max(maxblocks, nblocks(L.total[kCell]) + (int)L.patches.size());
   }
-  plan_chain(c);
+  RETCHK(plan_chain(c));
   if (maxblocks > c->partial_cap) {
     hipFree(c->d_partial);
     hipFree(c->d_partial2);
@@ -7361,34 +7395,21 @@ extern "C" int32_t bsam_estimate_level_errors_noalloc(bsam_ctx* c, int32_t level
 static int32_t launch_coarse_chain(bsam_ctx* c, int level, bool lin) {
   Level& L = c->L(level);
   if (c->p.model == BSAM_MODEL_TUMOUR) {
-    TmChainArgs A;
-    A.nlev = L.chain;
-    A.m = c->mk();
-    for (int n = 0; n < L.chain; n++) {
-      const Level& Ln = c->L(level - n);
-      A.pid[n] = Ln.patches[0];
-      A.seq[n] = BoxSeq{};
-      A.seq[n].ns = (int)Ln.stages.size();
-      for (int st = 0; st < A.seq[n].ns; st++) {
-        A.seq[n].d[st] = Ln.stages[st].d;
-        A.seq[n].n[st] = Ln.stages[st].n;
-      }
-      // one patch, condition 10 on every face: its stages are the faces
-      // x, y, z, which phys_shell evaluates in one pass
-      bool allphys = Ln.wrap_pid < 0 && (int)Ln.stages.size() == c->p.ndims;
-      for (int f = 0; f < 2 * c->p.ndims; f++) allphys = allphys && c->desc[A.pid[n]].mthbc[f] == 10;
-      A.physop[n] = allphys ? phys_face_op(c) : -1;
-    }
+    // the chain's levels were set by plan_chain for this level
+    if (!c->d_tmchain || c->h_tmchain.nlev != L.chain || c->h_tmchain.pid[0] != L.patches[0])
+      return set_err(BSAM_ESTATE, "level %d: coarse chain not planned", level);
+    const ModelK mk = c->mk();
     const int nsm = c->p.nsmoothingpasses;
+    const TmChainArgs* A = c->d_tmchain;
     if (!lin) {
       DISPATCH_DIM(c->p.ndims,
-                   (k_coarse_vcycle_tm<2, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, c->q, c->f, c->qc, c->qold, nullptr)),
-                   (k_coarse_vcycle_tm<3, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, c->q, c->f, c->qc, c->qold, nullptr)));
+                   (k_coarse_vcycle_tm<2, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, mk, nsm, c->q, c->f, c->qc, c->qold, nullptr)),
+                   (k_coarse_vcycle_tm<3, false><<<1, 1024, 0, c->s>>>(c->d_pt, A, mk, nsm, c->q, c->f, c->qc, c->qold, nullptr)));
     } else {
       bsam::MixedState* m = c->mixed;
       DISPATCH_DIM(c->p.ndims,
-                   (k_coarse_vcycle_tm<2, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, m->e, m->rhs, m->ec, c->qold, m->coef)),
-                   (k_coarse_vcycle_tm<3, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, nsm, m->e, m->rhs, m->ec, c->qold, m->coef)));
+                   (k_coarse_vcycle_tm<2, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, mk, nsm, m->e, m->rhs, m->ec, c->qold, m->coef)),
+                   (k_coarse_vcycle_tm<3, true><<<1, 1024, 0, c->s>>>(c->d_pt, A, mk, nsm, m->e, m->rhs, m->ec, c->qold, m->coef)));
     }
     LAUNCH_CHECK();
     return BSAM_OK;
@@ -8630,7 +8651,7 @@ extern "C" int32_t bsam_set_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
   if (knob == BSAM_KNOB_CHAIN_CELLS) {
     if (value < 0) return set_err(BSAM_EINVAL, "chain cells must be >= 0");
     c->chain_cells = value;
-    if (c->have_hier) plan_chain(c);
+    if (c->have_hier) RETCHK(plan_chain(c));
     drop_graph(c);
     return BSAM_OK;
   }
