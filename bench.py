@@ -360,6 +360,8 @@ def cpu_baseline(wl):
 def main():
     import faulthandler
     faulthandler.enable()  # a crash prints the Python stack that led to it
+    # and the library's native stack before it (read when the library loads)
+    os.environ.setdefault("BSAM_BACKTRACE", "1")
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
