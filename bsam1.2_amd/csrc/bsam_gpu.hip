@@ -3168,6 +3168,9 @@ struct Level {
   // exact (see plan_ghost_maps)
   DevMap gmap[7];
   bool maps_ok = false;
+  // built without its ghost fill yet (build_level with defer_ghosts; the
+  // caller finishes it with finish_level_ghosts)
+  bool ghosts_deferred = false;
   std::string why;                // why a level keeps the staged fill
   int wrap_pid = -1;              // fully periodic single patch: fused wrap
   // wrap_pid's extents are all even: a colour sweep may write the ghost
@@ -5861,7 +5864,36 @@ int32_t build_rules_device(bsam_ctx* c, Level& L) {
 // The device side of one level of a new hierarchy: its patch list, the
 // block-aligned launch ranges of every domain, the tagging offsets and the
 // ghost fill (stages, interpolation rules, flattened tables).
-int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
+// The ghost fill of a level whose box stages are planned (G.stages): the
+// interpolation rules, on the host or on the device, the flattened tables
+// (host planner, or device planner for batched levels above the root), and
+// their upload in a pooled block of the level.
+int32_t finish_level_ghosts(bsam_ctx* c, Level& L, GhostPlan& G, bool tables) {
+  // batched levels above the root build their interpolation rules, and
+  // (with the device planner) their flattened tables, on the device
+  const bool dev_tables = tables && c->p.batching && L.level > 0 && c->table_planner == 1;
+  const bool dev_rules = (!tables || dev_tables) && c->p.batching && L.level > 0;
+  // (with the device planner the interpolation rules are needed only by a
+  // level it refuses, which keeps the staged fill)
+  if (dev_rules) {
+    for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(L.patches.size() + 1, 0);
+  } else {
+    plan_interp_rules(c, L, G);
+  }
+  if (c->p.batching && tables && !dev_tables) RETCHK(plan_ghost_maps(c, L, G));
+  BlockBuilder Bk;
+  upload_ghost(c, L, G, Bk);
+  RETCHK(Bk.upload(c, L));
+  L.why = G.why;
+  if (dev_tables) RETCHK(plan_maps_device(c, L, G, L.why));
+  if (dev_rules && !L.maps_ok) RETCHK(build_rules_device(c, L));
+  L.ghosts_deferred = false;
+  return BSAM_OK;
+}
+
+// A level's device side: the launch domains, the tagging offsets and, unless
+// `defer_ghosts`, its ghost fill (plan_ghost_stages, finish_level_ghosts).
+int32_t build_level(bsam_ctx* c, Level& L, bool tables = true, bool defer_ghosts = false) {
   const int nv = c->p.nrvars;
   BlockBuilder Bk;
   Bk.add(L.patches.data(), L.patches.size(), &L.d_patches);
@@ -5926,26 +5958,14 @@ int32_t build_level(bsam_ctx* c, Level& L, bool tables = true) {
     L.ncell = n;
     Bk.add(foff.data(), foff.size(), &L.d_foff);
   }
+  RETCHK(Bk.upload(c, L));
+  if (defer_ghosts) {
+    L.ghosts_deferred = true;
+    return BSAM_OK;
+  }
   GhostPlan G;
   RETCHK(plan_ghost_stages(c, L, G));
-  // batched levels above the root build their interpolation rules, and
-  // (with the device planner) their flattened tables, on the device
-  const bool dev_tables = tables && c->p.batching && L.level > 0 && c->table_planner == 1;
-  const bool dev_rules = (!tables || dev_tables) && c->p.batching && L.level > 0;
-  // (with the device planner the interpolation rules are needed only by a
-  // level it refuses, which keeps the staged fill)
-  if (dev_rules) {
-    for (int sg = 0; sg < 3; sg++) G.roff[sg].assign(L.patches.size() + 1, 0);
-  } else {
-    plan_interp_rules(c, L, G);
-  }
-  if (c->p.batching && tables && !dev_tables) RETCHK(plan_ghost_maps(c, L, G));
-  upload_ghost(c, L, G, Bk);
-  RETCHK(Bk.upload(c, L));
-  L.why = G.why;
-  if (dev_tables) RETCHK(plan_maps_device(c, L, G, L.why));
-  if (dev_rules && !L.maps_ok) RETCHK(build_rules_device(c, L));
-  return BSAM_OK;
+  return finish_level_ghosts(c, L, G, tables);
 }
 
 bool same_desc(const bsam_patch_desc& a, const bsam_patch_desc& b) {
@@ -6048,7 +6068,7 @@ struct OldMesh {
 // unchanged keep their device side (and launch shapes); the others are
 // built.  Returns the highest kept level through *kept.
 int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_t minlevel, int32_t finestlevel,
-                int* kept, OldMesh* donor = nullptr, bool tables = true) {
+                int* kept, OldMesh* donor = nullptr, bool tables = true, int defer_level = -1000) {
   HIPCHK(hipStreamSynchronize(c->s));
   int keep = minlevel - 1;
   if (c->have_hier && desc && npatch > 0) {
@@ -6095,7 +6115,7 @@ int32_t install(bsam_ctx* c, int32_t npatch, const bsam_patch_desc* desc, int32_
       L = std::move(donor->lv[l - donor->minlevel]);
       donor->lv[l - donor->minlevel] = Level();
     } else {
-      RETCHK(build_level(c, L, tables));
+      RETCHK(build_level(c, L, tables, l == defer_level));
     }
     // per-patch launches round every patch up to whole blocks
     maxblocks = std::max(maxblocks, nblocks(L.total[kCell]) + (int)L.patches.size());
@@ -7748,6 +7768,41 @@ extern "C" int32_t bsam_regrid(bsam_ctx* c, int32_t npatch, const bsam_patch_des
 // no new patches.  Each new level is installed on top of the levels below
 // it, which keep their device side and fields; the old mesh stays the
 // source of TransferValues until the pass ends.
+namespace {
+// plan_ghost_stages of a level on a thread of its own (when `run`), so that
+// the caller can go on with device work; join() returns its status with its
+// message, and the destructor joins on every other way out.
+struct StagePlanner {
+  std::thread th;
+  int32_t rc = BSAM_OK;
+  std::string msg;
+  StagePlanner(const bsam_ctx* c, const Level& L, GhostPlan& G, bool run) {
+    if (!run) return;
+    auto work = [this, c, &L, &G]() {
+      try {
+        rc = plan_ghost_stages(c, L, G);
+        if (rc != BSAM_OK) msg = bsam_last_error();
+      } catch (...) {
+        rc = BSAM_ENOMEM;
+        msg = "host stage planning failed";
+      }
+    };
+    try {
+      th = std::thread(work);
+    } catch (...) {
+      work();  // no thread: plan here
+    }
+  }
+  int32_t join() {
+    if (th.joinable()) th.join();
+    return rc == BSAM_OK ? BSAM_OK : set_err(rc, "%s", msg.c_str());
+  }
+  ~StagePlanner() {
+    if (th.joinable()) th.join();
+  }
+};
+}  // namespace
+
 extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
   if (!c || !gp) return set_err(BSAM_EINVAL, "bad arguments");
   if (!c->have_hier) return set_err(BSAM_ESTATE, "no hierarchy installed (bsam_set_hierarchy)");
@@ -7826,13 +7881,24 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
     } prev{c, c->q, c->qold};
     c->q = c->qold = nullptr;
     int kept = minlevel - 1;
-    RETCHK(install(c, (int)H.size(), H.data(), minlevel, lvl + 1, &kept, &old, c->regrid_tables != 0));
+    RETCHK(install(c, (int)H.size(), H.data(), minlevel, lvl + 1, &kept, &old, c->regrid_tables != 0, lvl + 1));
     if (kept != lvl) return set_err(BSAM_EINVAL, "AMR: levels up to %d were not kept (%d)", lvl, kept);
+    // a rebuilt level's box stages are planned on a host thread while its
+    // fields are made (InitFields and TransferValues do not read the ghost
+    // fill); its ghost tables follow
+    Level& Ln = c->L(lvl + 1);
+    GhostPlan G;
+    StagePlanner stages(c, Ln, G, Ln.ghosts_deferred);
     pc.next(BSAM_PHASE_AMR_FIELDS);
     HIPCHK(hipMemcpyAsync(c->q, prev.q, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     HIPCHK(hipMemcpyAsync(c->qold, prev.qold, span * sizeof(double), hipMemcpyDeviceToDevice, c->s));
     old.restore_shapes(lvl + 1);
     RETCHK(init_level(c, lvl + 1, old));
+    RETCHK(stages.join());
+    if (Ln.ghosts_deferred) {
+      pc.next(BSAM_PHASE_AMR_INSTALL);
+      RETCHK(finish_level_ghosts(c, Ln, G, c->regrid_tables != 0));
+    }
   }
   return BSAM_OK;
 }
