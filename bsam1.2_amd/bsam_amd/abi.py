@@ -151,6 +151,7 @@ ENTRY_POINTS = {
     "bsam_estimate_level_errors_noalloc": ([_P, _I, _I, _D, C.POINTER(_I), _LP], _I),
     "bsam_write_frame": ([_P, C.c_char_p, _D, _DP], _I),
     "bsam_selftest_f25": ([_DP, C.c_int64, C.c_char_p], _I),
+    "bsam_selftest_thread_pool": ([_I, _I, _LP], _I),
     "bsam_vcycle": ([_P, _I], _I),
     "bsam_multigrid_iterations": ([_P, _IP, _DP, _DP, _I], _I),
     "bsam_time_step": ([_P, _I, _IP, _DP], _I),

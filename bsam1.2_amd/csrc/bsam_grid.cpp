@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <chrono>
+#include <stdexcept>
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
@@ -401,6 +403,51 @@ bsam_patch_desc make_child(const bsam_patch_desc& P, int pidx, const Box& b, int
 }
 
 }  // namespace
+
+// Host self-test of bsam::parallel_for (include/bsam_gpu.h).  Each task
+// marks its index and, after a short wait, a "done" flag; a job returning
+// before every flag is set, an index run twice or not at all, or an
+// exception lost or invented fails the test.
+extern "C" int32_t bsam_selftest_thread_pool(int32_t rounds, int32_t n_max, int64_t* jobs) {
+  if (rounds < 1 || n_max < 2 || n_max > 256 || !jobs) return bsam::fail(BSAM_EINVAL, "bad arguments");
+  *jobs = 0;
+  for (int r = 0; r < rounds; r++) {
+    // counts that grow within a round (new workers join mid-sequence), then
+    // shrink; every third job throws from one task
+    const int counts[5] = {1 + r % 3, 2 + (r * 7) % (n_max - 1), n_max, 2, 1 + (r * 5) % n_max};
+    for (int n : counts) {
+      std::vector<std::atomic<int>> hit(n), done(n);
+      for (int t = 0; t < n; t++) hit[t] = 0, done[t] = 0;
+      const bool thrower = (*jobs % 3) == 2;
+      const int bad = n / 2;
+      bool caught = false;
+      const std::function<void(int)> f = [&](int t) {
+        hit[t]++;
+        std::this_thread::sleep_for(std::chrono::microseconds(t % 4 == 0 ? 200 : 20));
+        done[t] = 1;
+        if (thrower && t == bad) throw std::runtime_error("task failure");
+      };
+      try {
+        bsam::parallel_for(n, f);
+      } catch (const std::runtime_error&) {
+        caught = true;
+      }
+      for (int t = 0; t < n; t++) {
+        if (hit[t] != 1)
+          return bsam::fail(BSAM_EINVAL, "job %lld (%d tasks): task %d ran %d times", (long long)*jobs, n, t,
+                            (int)hit[t]);
+        if (!done[t])
+          return bsam::fail(BSAM_EINVAL, "job %lld (%d tasks): returned while task %d still ran", (long long)*jobs, n,
+                            t);
+      }
+      // a task's exception reaches the caller when the calling thread ran
+      // that task; the workers' are contained (a worker cannot rethrow)
+      if (caught && !thrower) return bsam::fail(BSAM_EINVAL, "job %lld: an exception that no task threw", (long long)*jobs);
+      (*jobs)++;
+    }
+  }
+  return BSAM_OK;
+}
 
 extern "C" int32_t bsam_new_level(int32_t ndims, int32_t npatch, const bsam_patch_desc* desc, int32_t level,
                                   const int32_t* flags, int32_t ibuffer, int32_t minimumgridpoints,

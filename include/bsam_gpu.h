@@ -470,6 +470,15 @@ int32_t bsam_selftest_ghost_maps(const bsam_params* params, int32_t npatch,
                                  int32_t finestlevel, uint64_t seed,
                                  int64_t* checked, int32_t* levels_flat);
 
+/* Host self-test of the library's thread pool (no counterpart in the
+ * reference; host planning and clustering run on it): `rounds` rounds of
+ * jobs of 1..n_max tasks, the task counts chosen so that the pool grows
+ * between jobs, and jobs that throw, each checking that every task ran
+ * exactly once and that no task was still running when its job returned.
+ * *jobs receives the number of jobs run.  No GPU is used.  Returns
+ * BSAM_EINVAL with a message on the first violation. */
+int32_t bsam_selftest_thread_pool(int32_t rounds, int32_t n_max, int64_t* jobs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,3 +112,21 @@ def test_no_tags_and_refusals():
         with pytest.raises(BsamError) as ei:
             new_level(3, hier, 0, f, 1, mgp, 0.8)
         assert ei.value.code == abi.BSAM_EINVAL and "minimumgridpoints" in str(ei.value)
+
+
+def test_host_thread_pool():
+    """The host thread pool that clustering and host planning run on
+    (bsam::parallel_for, through bsam_selftest_thread_pool): jobs whose
+    task counts make the pool grow between jobs, and jobs with a task that
+    throws; every task runs exactly once and every job waits for all of its
+    tasks.  (A worker added to a grown pool once counted down for a job it
+    did not run, letting a call return while another worker still ran; the
+    thread ordering that exposes this is too rare to force from here, so
+    the test checks the contract the fix restores.)"""
+    import ctypes as C
+    from bsam_amd import gpu
+    lib = gpu.load()
+    jobs = C.c_int64()
+    rc = lib.bsam_selftest_thread_pool(40, 24, C.byref(jobs))
+    assert rc == abi.BSAM_OK, lib.bsam_last_error().decode()
+    assert jobs.value == 200
