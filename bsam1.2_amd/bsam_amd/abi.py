@@ -53,6 +53,7 @@ KNOB_REGRID_TABLES = 7
 KNOB_GATHER_BLOCK_FP64 = 8
 KNOB_GATHER_BLOCK_FP32 = 9
 KNOB_TABLE_PLANNER = 10
+KNOB_SWEEP_IMAGES = 11
 
 
 class PatchDesc(C.Structure):

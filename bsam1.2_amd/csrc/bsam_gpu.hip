@@ -203,6 +203,46 @@ __device__ __forceinline__ bool half_cell(const DPatch& P, int c, int local, int
 struct Stencil {
   int o, xm, xp, ym, yp, zm, zp;
 };
+
+// Post-operations of a flattened ghost rule, three bits each from the low
+// end, applied in that order: 1 adds +0.0 (the qpo term of an x wrap), 2
+// takes the Dirichlet image 2 - x (-x for a correction, `homog`), 3 does
+// that on variables >= 1 only (the BioGrowth boundary condition), 4 is the
+// tumour model's condition (2 - x on variable 2, -x on variable 3).
+template <typename T>
+__host__ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog) {
+  for (; post; post >>= 3) {
+    const int op = post & 7;
+    if (op == 1) x = x + (T)0;
+    else if (op == 2 || (op == 3 && v >= 1) || (op == 4 && v == 2)) x = homog ? -x : (T)2 - x;
+    else if (op == 4 && v == 3) x = -x;
+  }
+  return x;
+}
+
+// Ghost images that a colour sweep of a level writes for the reduced fill
+// that follows it: the fill's copy rules whose source is a cell the sweep
+// updates, grouped by the sweep's logical workgroup.  Entry (dst, dst
+// variable stride, thread of the workgroup whose cell is the source, the
+// rule's post-operation chain); off[b] .. off[b + 1] are workgroup b's.
+struct ImgTab {
+  const int* __restrict__ off;
+  const int4* __restrict__ ent;
+  int homog;
+};
+// Writes a workgroup's images from the new values its threads left in
+// `nv` (one row of kBlock per variable): every thread of the workgroup
+// must call it (it waits at a barrier).
+template <typename T, int NV>
+__device__ __forceinline__ void write_images(const ImgTab& I, int lb, T (*nv)[kBlock], T* __restrict__ q) {
+  __syncthreads();
+  const int e1 = I.off[lb + 1];
+  for (int e = I.off[lb] + (int)threadIdx.x; e < e1; e += kBlock) {
+    const int4 r = I.ent[e];
+#pragma unroll
+    for (int v = 0; v < NV; v++) q[r.x + v * r.y] = ghost_post<T>(nv[v][r.z], r.w, v, I.homog);
+  }
+}
 template <int DIM>
 __device__ __forceinline__ Stencil stencil_of(const DPatch& P, int c, int i, int j, int k, int t) {
   const int hq = P.hq, ny = P.nq[1], blk = P.nq[2] * ny * hq;
@@ -301,42 +341,59 @@ __device__ __forceinline__ void ch_corr_op(float p0, float ep, float em, float l
   a2 = em - ep * (3.0f * (p0 * p0) + ep * (3.0f * p0 + ep)) + t2 * lphi;
 }
 
-template <int DIM, bool WRAP>
+template <int DIM, bool WRAP, bool IMG = false>
 __global__ __launch_bounds__(kBlock) void k_relax_ch(LevelArgs L, int c, double dt, double eps2,
                                                      double h, double* __restrict__ q,
-                                                     const double* __restrict__ f) {
+                                                     const double* __restrict__ f, ImgTab I = ImgTab{}) {
+  // IMG: the workgroup then writes the ghost images of its updated cells
+  // (write_images), so no thread leaves before that
+  const int lb = level_block(L);
   int pi, local;
-  if (!level_elem(L, level_block(L), pi, local)) return;
-  const DPatch P = L.pt[L.patches[pi]];
-  int i, j, k, t;
-  if (!half_cell<DIM>(P, c, local, i, j, k, t)) return;
-  const double h2 = h * h;
-  const double tmp1 = dt / h2, tmp2 = eps2 / h2;
-  const double tmp3 = (DIM == 3 ? 6.0 : 4.0) * eps2 / h2;
-  const double a12 = (DIM == 3 ? 6.0 : 4.0) * tmp1;
-  const int vs = P.qvs;
-  const Stencil S = stencil_of<DIM>(P, c, i, j, k, t);
-  const int o = S.o;
-  const double phi = q[o];
-  const double f0 = f[o], f1 = f[o + vs];
-  const int xm = S.xm, xp = S.xp, ym = S.ym, yp = S.yp;
-  double smu, sphi;
-  if (DIM == 3) {
-    const int zm = S.zm, zp = S.zp;
-    smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs] + q[zm + vs] + q[zp + vs];
-    sphi = q[xp] + q[xm] + q[yp] + q[ym] + q[zp] + q[zm];
-  } else {
-    smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs];
-    sphi = q[xp] + q[xm] + q[yp] + q[ym];
+  double n0 = 0.0, n1 = 0.0;
+  bool ok = level_elem(L, lb, pi, local);
+  if (!IMG && !ok) return;
+  if (ok) {
+    const DPatch P = L.pt[L.patches[pi]];
+    int i, j, k, t;
+    ok = half_cell<DIM>(P, c, local, i, j, k, t);
+    if (!IMG && !ok) return;
+    if (ok) {
+      const double h2 = h * h;
+      const double tmp1 = dt / h2, tmp2 = eps2 / h2;
+      const double tmp3 = (DIM == 3 ? 6.0 : 4.0) * eps2 / h2;
+      const double a12 = (DIM == 3 ? 6.0 : 4.0) * tmp1;
+      const int vs = P.qvs;
+      const Stencil S = stencil_of<DIM>(P, c, i, j, k, t);
+      const int o = S.o;
+      const double phi = q[o];
+      const double f0 = f[o], f1 = f[o + vs];
+      const int xm = S.xm, xp = S.xp, ym = S.ym, yp = S.yp;
+      double smu, sphi;
+      if (DIM == 3) {
+        const int zm = S.zm, zp = S.zp;
+        smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs] + q[zm + vs] + q[zp + vs];
+        sphi = q[xp] + q[xm] + q[yp] + q[ym] + q[zp] + q[zm];
+      } else {
+        smu = q[xm + vs] + q[xp + vs] + q[ym + vs] + q[yp + vs];
+        sphi = q[xp] + q[xm] + q[yp] + q[ym];
+      }
+      const double a21 = -3.0 * (phi * phi) - tmp3;
+      const double b1 = f0 + tmp1 * smu;
+      const double b2 = f1 - tmp2 * sphi - 2.0 * (phi * phi * phi);
+      const double det = 1.0 - a12 * a21;
+      n0 = (b1 - a12 * b2) / det;
+      n1 = (b2 - a21 * b1) / det;
+      q[o] = n0;
+      q[o + vs] = n1;
+      if (WRAP) wrap_images<DIM, double>(q, P, i, j, k, n0, n1);
+    }
   }
-  const double a21 = -3.0 * (phi * phi) - tmp3;
-  const double b1 = f0 + tmp1 * smu;
-  const double b2 = f1 - tmp2 * sphi - 2.0 * (phi * phi * phi);
-  const double det = 1.0 - a12 * a21;
-  const double n0 = (b1 - a12 * b2) / det, n1 = (b2 - a21 * b1) / det;
-  q[o] = n0;
-  q[o + vs] = n1;
-  if (WRAP) wrap_images<DIM, double>(q, P, i, j, k, n0, n1);
+  if (IMG) {
+    __shared__ double nv[2][kBlock];
+    nv[0][threadIdx.x] = n0;
+    nv[1][threadIdx.x] = n1;
+    write_images<double, 2>(I, lb, nv, q);
+  }
 }
 
 // 3D colour sweep of one large patch, marching in k.  A block owns a tile
@@ -1932,21 +1989,6 @@ __global__ __launch_bounds__(kBlock) void k_interp_map(const GRule* __restrict__
   for (int v = 0; v < nv; v++) q[R.dst + v * R.pvs] = interp_value<T>(R, R.kind, v, q);
 }
 
-// Post-operations of a flattened ghost rule, three bits each from the low
-// end, applied in that order: 1 adds +0.0 (the qpo term of an x wrap), 2
-// takes the Dirichlet image 2 - x (-x for a correction, `homog`), 3 does
-// that on variables >= 1 only (the BioGrowth boundary condition), 4 is the
-// tumour model's condition (2 - x on variable 2, -x on variable 3).
-template <typename T>
-__host__ __device__ __forceinline__ T ghost_post(T x, int post, int v, int homog) {
-  for (; post; post >>= 3) {
-    const int op = post & 7;
-    if (op == 1) x = x + (T)0;
-    else if (op == 2 || (op == 3 && v >= 1) || (op == 4 && v == 2)) x = homog ? -x : (T)2 - x;
-    else if (op == 4 && v == 3) x = -x;
-  }
-  return x;
-}
 
 // One whole SetGhost of a level from its flattened rules: every rule
 // writes one ghost cell (all variables) from interior or parent cells, so
@@ -2366,63 +2408,78 @@ __device__ __forceinline__ float ulap_cf(const float* __restrict__ a, const DPat
 // new correction and that of the other cell of the thread's pair (2t+1,
 // 2t+2), final since the previous sweep, are added into the FP64 solution
 // q64 (interior cells; even mx).
-template <int DIM, bool WRAP, int EMODE = 0>
+template <int DIM, bool WRAP, int EMODE = 0, bool IMG = false>
 __global__ __launch_bounds__(kBlock) void k_relax_lin(LevelArgs L, int c, float tmp1, float tmp2, float tmp3,
                                                       float a12, float* __restrict__ e,
                                                       const float* __restrict__ rhs,
                                                       const float* __restrict__ coef,
-                                                      double* __restrict__ q64 = nullptr) {
+                                                      double* __restrict__ q64 = nullptr, ImgTab I = ImgTab{}) {
+  // IMG: the workgroup then writes the ghost images of its updated cells
+  // (write_images), so no thread leaves before that
+  const int lb = level_block(L);
   int pi, local;
-  if (!level_elem(L, level_block(L), pi, local)) return;
-  const DPatch P = L.pt[L.patches[pi]];
-  int i, j, k, t;
-  if (!half_cell<DIM>(P, c, local, i, j, k, t)) return;
-  const int ib = (c + j + k) & 1;
-  const int vs = P.qvs;
-  const Stencil S = stencil_of<DIM>(P, c, i, j, k, t);
-  const int o = S.o;
-  float smu = 0.0f, sphi = 0.0f;
-  if (EMODE == 1) {
-    // no interior neighbour holds a correction yet; a ghost neighbour holds
-    // the offset of the base state's ghosts from a fresh fill (the values
-    // the FAS cycle's first sweep reads, rebase_ghosts)
-    auto add = [&](int x) {
-      smu += e[x + vs];
-      sphi += e[x];
-    };
-    if (i == 1) add(S.xm);
-    if (i == P.mx[0]) add(S.xp);
-    if (j == 1) add(S.ym);
-    if (j == P.mx[1]) add(S.yp);
-    if (DIM == 3 && k == 1) add(S.zm);
-    if (DIM == 3 && k == P.mx[2]) add(S.zp);
-  } else {
-    smu = e[S.xm + vs] + e[S.xp + vs] + e[S.ym + vs] + e[S.yp + vs];
-    sphi = e[S.xp] + e[S.xm] + e[S.yp] + e[S.ym];
-    if (DIM == 3) {
-      smu = smu + e[S.zm + vs] + e[S.zp + vs];
-      sphi = sphi + e[S.zp] + e[S.zm];
+  float n0 = 0.0f, n1 = 0.0f;
+  bool ok = level_elem(L, lb, pi, local);
+  if (!IMG && !ok) return;
+  if (ok) {
+    const DPatch P = L.pt[L.patches[pi]];
+    int i, j, k, t;
+    ok = half_cell<DIM>(P, c, local, i, j, k, t);
+    if (!IMG && !ok) return;
+    if (ok) {
+      const int ib = (c + j + k) & 1;
+      const int vs = P.qvs;
+      const Stencil S = stencil_of<DIM>(P, c, i, j, k, t);
+      const int o = S.o;
+      float smu = 0.0f, sphi = 0.0f;
+      if (EMODE == 1) {
+        // no interior neighbour holds a correction yet; a ghost neighbour
+        // holds the offset of the base state's ghosts from a fresh fill
+        // (the values the FAS cycle's first sweep reads, rebase_ghosts)
+        auto add = [&](int x) {
+          smu += e[x + vs];
+          sphi += e[x];
+        };
+        if (i == 1) add(S.xm);
+        if (i == P.mx[0]) add(S.xp);
+        if (j == 1) add(S.ym);
+        if (j == P.mx[1]) add(S.yp);
+        if (DIM == 3 && k == 1) add(S.zm);
+        if (DIM == 3 && k == P.mx[2]) add(S.zp);
+      } else {
+        smu = e[S.xm + vs] + e[S.xp + vs] + e[S.ym + vs] + e[S.yp + vs];
+        sphi = e[S.xp] + e[S.xm] + e[S.yp] + e[S.ym];
+        if (DIM == 3) {
+          smu = smu + e[S.zm + vs] + e[S.zp + vs];
+          sphi = sphi + e[S.zp] + e[S.zm];
+        }
+      }
+      ch_corr_update(coef[o], EMODE == 1 ? 0.0f : e[o], rhs[o], rhs[o + vs], smu, sphi, tmp1, tmp2, tmp3, a12, n0,
+                     n1);
+      e[o] = n0;
+      e[o + vs] = n1;
+      if (WRAP) wrap_images<DIM, float>(e, P, i, j, k, n0, n1);
+      // the other cell of the thread's pair (2t+1, 2t+2) is the other
+      // colour's element t + ib of its row
+      if (EMODE == 1) {
+        const int po = S.xm + ib;
+        e[po] = 0.0f;
+        e[po + vs] = 0.0f;
+      }
+      if (EMODE == 2) {
+        q64[o] = q64[o] + (double)n0;
+        q64[o + vs] = q64[o + vs] + (double)n1;
+        const int po = S.xm + ib;
+        q64[po] = q64[po] + (double)e[po];
+        q64[po + vs] = q64[po + vs] + (double)e[po + vs];
+      }
     }
   }
-  float n0, n1;
-  ch_corr_update(coef[o], EMODE == 1 ? 0.0f : e[o], rhs[o], rhs[o + vs], smu, sphi, tmp1, tmp2, tmp3, a12, n0,
-                 n1);
-  e[o] = n0;
-  e[o + vs] = n1;
-  if (WRAP) wrap_images<DIM, float>(e, P, i, j, k, n0, n1);
-  // the other cell of the thread's pair (2t+1, 2t+2) is the other colour's
-  // element t + ib of its row
-  if (EMODE == 1) {
-    const int po = S.xm + ib;
-    e[po] = 0.0f;
-    e[po + vs] = 0.0f;
-  }
-  if (EMODE == 2) {
-    q64[o] = q64[o] + (double)n0;
-    q64[o + vs] = q64[o + vs] + (double)n1;
-    const int po = S.xm + ib;
-    q64[po] = q64[po] + (double)e[po];
-    q64[po + vs] = q64[po + vs] + (double)e[po + vs];
+  if (IMG) {
+    __shared__ float nv[2][kBlock];
+    nv[0][threadIdx.x] = n0;
+    nv[1][threadIdx.x] = n1;
+    write_images<float, 2>(I, lb, nv, e);
   }
 }
 
@@ -3167,6 +3224,14 @@ struct Level {
   // sequence needs, gmap[6] those of gmap[5] a correction does not leave
   // exact (see plan_ghost_maps)
   DevMap gmap[7];
+  // the reduced fills of ipass 1 and 2 (gmap[3], gmap[4]) split between
+  // the colour sweep before them, which writes the copies of the cells it
+  // updates (img_off / img_ent, ImgTab), and the gather of the rest (gimg)
+  bool img_ok[2] = {false, false};
+  int img_n[2] = {0, 0};
+  const int* img_off[2] = {};
+  const int4* img_ent[2] = {};
+  DevMap gimg[2];
   bool maps_ok = false;
   // built without its ghost fill yet (build_level with defer_ghosts; the
   // caller finishes it with finish_level_ghosts)
@@ -3257,6 +3322,7 @@ struct bsam_ctx {
   std::string tune_report;  // JSON of the last bsam_autotune
   long chain_cells = 0;     // largest level (cells) where the coarse chain starts
   int regrid_tables = 1;  // flattened ghost tables for the levels a regrid builds
+  int sweep_images = 1;   // BSAM_KNOB_SWEEP_IMAGES
   // field pools an install released, kept for the next install (a regrid
   // installs every step): buffer and capacity in bytes; pool_cap holds the
   // capacity of the pools in use
@@ -5868,7 +5934,8 @@ int32_t build_rules_device(bsam_ctx* c, Level& L) {
 // interpolation rules, on the host or on the device, the flattened tables
 // (host planner, or device planner for batched levels above the root), and
 // their upload in a pooled block of the level.
-int32_t finish_level_ghosts(bsam_ctx* c, Level& L, GhostPlan& G, bool tables) {
+int32_t build_images(bsam_ctx* c, Level& L);
+int32_t finish_level_ghosts(bsam_ctx* c, Level& L, GhostPlan& G, bool tables, bool images = true) {
   // batched levels above the root build their interpolation rules, and
   // (with the device planner) their flattened tables, on the device
   const bool dev_tables = tables && c->p.batching && L.level > 0 && c->table_planner == 1;
@@ -5887,7 +5954,170 @@ int32_t finish_level_ghosts(bsam_ctx* c, Level& L, GhostPlan& G, bool tables) {
   L.why = G.why;
   if (dev_tables) RETCHK(plan_maps_device(c, L, G, L.why));
   if (dev_rules && !L.maps_ok) RETCHK(build_rules_device(c, L));
+  if (images) RETCHK(build_images(c, L));
   L.ghosts_deferred = false;
+  return BSAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ghost images written by the colour sweeps.  The reduced fill after the
+// sweep of colour col (gmap[3 + col]) copies, among others, cells the sweep
+// has just updated into face ghosts of the same colour in sibling and
+// periodic partners.  The sweep's workgroup has those values in hand, so it
+// writes them itself (write_images), and the gather keeps the interpolation
+// rules and the other copies.  A rule moves to the sweep when its source is
+// an interior cell of colour col and its destination a ghost of colour col:
+// the sweep of colour col reads no cell of its own colour but its own, so
+// no workgroup reads a ghost another one writes, and the value written is
+// the one the gather would read.  Maps with rules that read ghost values
+// from before the fill are left whole.
+// ---------------------------------------------------------------------------
+// position of the level patch holding pool offset x (patches in pool order)
+__device__ __forceinline__ int img_patch(const DPatch* __restrict__ pt, const int* __restrict__ patches, int np,
+                                         int x) {
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pt[patches[mid]].qoff <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+// colour and storage indices of pool offset x of patch P (qidx inverted)
+__device__ __forceinline__ bool img_decode(const DPatch& P, int x, int& col, int& i, int& j, int& k) {
+  const int rel = x - P.qoff;
+  if (rel < 0 || rel >= P.qvs) return false;
+  const int row = P.nq[1] * P.hq, blk = P.nq[2] * row;
+  col = rel / blk;
+  const int r1 = rel - col * blk;
+  k = r1 / row;
+  const int r2 = r1 - k * row;
+  j = r2 / P.hq;
+  const int m = r2 - j * P.hq;
+  i = 2 * m + ((col + j + k) & 1);
+  return true;
+}
+// per copy rule r: key[r] = the logical workgroup of the sweep (domain
+// kHalf) whose cell is its source and tid[r] the thread, or key -1
+__global__ __launch_bounds__(kBlock) void k_img_class(const DPatch* __restrict__ pt, const int* __restrict__ patches,
+                                                      int np, const int* __restrict__ hprefix,
+                                                      const int4* __restrict__ cr, int nc, int col, int nd,
+                                                      int* __restrict__ key, int* __restrict__ tid,
+                                                      int* __restrict__ bcnt) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nc) return;
+  const int4 x = cr[r];
+  int b = -1;
+  const int a = img_patch(pt, patches, np, x.y);
+  const DPatch P = pt[patches[a]];
+  const DPatch D = pt[patches[img_patch(pt, patches, np, x.x)]];
+  int cs, i, j, k, cd, di, dj, dk;
+  if (img_decode(P, x.y, cs, i, j, k) && img_decode(D, x.x, cd, di, dj, dk) && cs == col && cd == col &&
+      i >= 1 && i <= P.mx[0] && j >= 1 && j <= P.mx[1] && (nd == 2 || (k >= 1 && k <= P.mx[2]))) {
+    const int cnt = (P.mx[0] + 1) >> 1;
+    const int row = (nd == 3) ? (k - 1) * P.mx[1] + (j - 1) : (j - 1);
+    const int t = (i + ((col + j + k) & 1)) / 2 - 1;
+    const long g = (long)hprefix[a] + (long)row * cnt + t;
+    b = (int)(g / kBlock);
+    tid[r] = (int)(g - (long)b * kBlock);
+    atomicAdd(&bcnt[b], 1);
+  }
+  key[r] = b;
+}
+__global__ __launch_bounds__(kBlock) void k_img_write(const int4* __restrict__ cr, const uint16_t* __restrict__ cpost,
+                                                      int nc, const int* __restrict__ key, const int* __restrict__ tid,
+                                                      const int* __restrict__ boff, int* __restrict__ fill,
+                                                      const int* __restrict__ kpos, int4* __restrict__ ent,
+                                                      int4* __restrict__ gcr, uint16_t* __restrict__ gpost) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nc) return;
+  const int4 x = cr[r];
+  const int post = cpost ? cpost[r] : 0;
+  const int b = key[r];
+  if (b >= 0) {
+    ent[boff[b] + atomicAdd(&fill[b], 1)] = make_int4(x.x, x.z, tid[r], post);
+  } else {
+    const int at = kpos[r];
+    gcr[at] = x;
+    if (gpost) gpost[at] = (uint16_t)post;
+  }
+}
+__global__ void k_img_keep(const int* __restrict__ key, int n, int* __restrict__ keep) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r < n) keep[r] = key[r] < 0 ? 1 : 0;
+  else if (r == n) keep[r] = 0;
+}
+
+int32_t build_images(bsam_ctx* c, Level& L) {
+  L.img_ok[0] = L.img_ok[1] = false;
+  if (!c->sweep_images || !L.maps_ok || !c->p.batching || c->p.model != BSAM_MODEL_CH || L.wrap_pid >= 0 ||
+      L.patches.empty())
+    return BSAM_OK;
+  const int np = (int)L.patches.size(), nd = c->p.ndims;
+  const int nblk = nblocks(L.total[kHalf]);
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  for (int col = 0; col < 2; col++) {
+    const DevMap& M = L.gmap[3 + col];
+    if (M.nc1 || M.ni1 || M.nc == 0) continue;
+    const int nc = M.nc;
+    size_t scan_a = 0, scan_b = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_a, (int*)nullptr, (int*)nullptr, nblk + 1, c->s));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, (int*)nullptr, (int*)nullptr, nc + 1, c->s));
+    const size_t o_key = 0, o_tid = o_key + al((size_t)nc * 4), o_keep = o_tid + al((size_t)nc * 4),
+                 o_kpos = o_keep + al((size_t)(nc + 1) * 4), o_bcnt = o_kpos + al((size_t)(nc + 1) * 4),
+                 o_boff = o_bcnt + al((size_t)(nblk + 1) * 4), o_fill = o_boff + al((size_t)(nblk + 1) * 4),
+                 o_scan = o_fill + al((size_t)nblk * 4), total = o_scan + al(std::max(scan_a, scan_b));
+    char* W = plan_workspace(c, total);
+    if (!W) return set_err(BSAM_ENOMEM, "sweep-image workspace of %zu bytes", total);
+    int* key = reinterpret_cast<int*>(W + o_key);
+    int* tid = reinterpret_cast<int*>(W + o_tid);
+    int* keep = reinterpret_cast<int*>(W + o_keep);
+    int* kpos = reinterpret_cast<int*>(W + o_kpos);
+    int* bcnt = reinterpret_cast<int*>(W + o_bcnt);
+    int* boff = reinterpret_cast<int*>(W + o_boff);
+    int* fill = reinterpret_cast<int*>(W + o_fill);
+    k_dp_fill<<<nblocks(nblk + 1), kBlock, 0, c->s>>>(bcnt, nblk + 1, 0);
+    k_dp_fill<<<nblocks(nblk), kBlock, 0, c->s>>>(fill, nblk, 0);
+    k_img_class<<<nblocks(nc), kBlock, 0, c->s>>>(c->d_pt, L.d_patches, np, L.d_prefix[kHalf], M.cr, nc, col, nd,
+                                                  key, tid, bcnt);
+    k_img_keep<<<nblocks(nc + 1), kBlock, 0, c->s>>>(key, nc, keep);
+    LAUNCH_CHECK();
+    size_t sa = scan_a, sb = scan_b;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(W + o_scan, sa, bcnt, boff, nblk + 1, c->s));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(W + o_scan, sb, keep, kpos, nc + 1, c->s));
+    int n[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&n[0], boff + nblk, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(&n[1], kpos + nc, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    const int nimg = n[0], nkeep = n[1];
+    if (nimg == 0 || nimg + nkeep != nc) continue;
+    // one pooled block: workgroup offsets, images, the gather's copies
+    const size_t b_off = 0, b_ent = al((size_t)(nblk + 1) * 4), b_cr = b_ent + al((size_t)nimg * sizeof(int4)),
+                 b_post = b_cr + al((size_t)std::max(nkeep, 1) * sizeof(int4)),
+                 bytes = b_post + (M.cpost ? al((size_t)std::max(nkeep, 1) * 2) : 0);
+    char* B = nullptr;
+    if (pool_alloc(c, &B, bytes, 1) != hipSuccess) return set_err(BSAM_ENOMEM, "sweep images of %zu bytes", bytes);
+    L.blocks.push_back(B);
+    int* off = reinterpret_cast<int*>(B + b_off);
+    int4* ent = reinterpret_cast<int4*>(B + b_ent);
+    int4* gcr = reinterpret_cast<int4*>(B + b_cr);
+    uint16_t* gpost = M.cpost ? reinterpret_cast<uint16_t*>(B + b_post) : nullptr;
+    HIPCHK(hipMemcpyAsync(off, boff, (size_t)(nblk + 1) * sizeof(int), hipMemcpyDeviceToDevice, c->s));
+    k_img_write<<<nblocks(nc), kBlock, 0, c->s>>>(M.cr, M.cpost, nc, key, tid, boff, fill, kpos, ent, gcr, gpost);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(c->s));
+    DevMap G;
+    G.cr = nkeep ? gcr : nullptr;
+    G.cpost = nkeep ? gpost : nullptr;
+    G.ir = M.ir;
+    G.nc = nkeep;
+    G.ni = M.ni;
+    L.gimg[col] = G;
+    L.img_off[col] = off;
+    L.img_ent[col] = ent;
+    L.img_n[col] = nimg;
+    L.img_ok[col] = true;
+  }
   return BSAM_OK;
 }
 
@@ -6560,14 +6790,16 @@ static int32_t relax_level_impl(bsam_ctx* c, int32_t level, int32_t ipass, bool 
 // reads; kAfterCorrection (ipass 0, right after CorrectFine, before such a
 // sequence) also leaves out the ghosts the correction left exact.  Levels
 // without flattened tables do the full SetGhost.
-enum GhostForm : int { kFull = 0, kLean = 1, kAfterCorrection = 2 };
+// kLeanImg: kLean after a sweep that wrote the fill's images (ImgTab):
+// the rest of the reduced table.
+enum GhostForm : int { kFull = 0, kLean = 1, kAfterCorrection = 2, kLeanImg = 3 };
 template <typename T>
 static int32_t set_ghost_t(bsam_ctx* c, int level, int ipass, T* q, int homog, int form = kFull) {
   Level& L = c->L(level);
   const int nv = c->p.nrvars;
   if (L.maps_ok && c->p.batching) {
     const int mi = form == kAfterCorrection ? 6 : (form == kLean ? (ipass > 0 ? 2 + ipass : 5) : ipass);
-    const DevMap& M = L.gmap[mi];
+    const DevMap& M = form == kLeanImg ? L.gimg[ipass - 1] : L.gmap[mi];
     // rules reading ghosts from before the call first, then the rest
     const int pr = sizeof(T) == 8 ? 0 : 1;
     const int bs = L.gblock[pr] > 0 ? L.gblock[pr] : kBlock;
@@ -6692,12 +6924,53 @@ static int march_kc(bsam_ctx* c, const Level& L, int* nblk) {
 // LevelRelax, afasroutines.f90:546-555).  For a single patch periodic in
 // every dimension the sweep writes the ghost images itself, which is the
 // whole effect of that SetGhost on the values that changed.
+// The colour sweep of ipass writes the images of the reduced fill after it
+// (build_images) when the level has them and the knob allows.
+static bool use_images(const bsam_ctx* c, const Level& L, int ipass) {
+  return c->sweep_images && c->p.batching && L.img_ok[ipass - 1];
+}
+// bytes an image moves: its entry, and each variable written
+static double image_bytes(const Level& L, int ipass, double value_bytes, int nv) {
+  return (double)L.img_n[ipass - 1] * (16.0 + nv * value_bytes);
+}
+
+// The CH FP64 colour sweep of a level with its fill's images (batched).
+static int32_t relax_ch_images(bsam_ctx* c, int level, int ipass) {
+  Level& L = c->L(level);
+  const bool fin = level == c->finest;
+  if (fin) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
+  const ImgTab I{L.img_off[ipass - 1], L.img_ent[ipass - 1], 0};
+  const double eps2 = c->p.model_params[0];
+  FOR_LAUNCHES(c, L, kHalf, a, nb) {
+    DISPATCH_DIM(c->p.ndims,
+                 (k_relax_ch<2, false, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, c->p.dt, eps2, L.h, c->q, c->f, I)),
+                 (k_relax_ch<3, false, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, c->p.dt, eps2, L.h, c->q, c->f, I)));
+    LAUNCH_CHECK();
+  }
+  if (fin) {
+    // relax_level_impl's algorithmic bytes of the sweep, plus the images
+    double N = 0, G = 0;
+    for (int pid : L.patches) {
+      const DPatch& P = c->hp[pid];
+      N += (double)P.mx[0] * P.mx[1] * P.mx[2];
+      G += (c->p.ndims == 3) ? 2.0 * ((double)P.mx[0] * P.mx[1] + (double)P.mx[1] * P.mx[2] + (double)P.mx[0] * P.mx[2])
+                             : 2.0 * ((double)P.mx[0] + P.mx[1]);
+    }
+    kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 8.0 * ((N + G) + 0.5 * (N + G) + N + N) + image_bytes(L, ipass, 8.0, 2));
+  }
+  return BSAM_OK;
+}
+
 static int32_t relax_and_ghost(bsam_ctx* c, int level, int ipass, bool lean) {
   Level& L = c->L(level);
   const bool wrap = L.wrap_img;
   int nblk = 0;
   const int kc = (c->p.model == BSAM_MODEL_CH) ? sweep_kc(c, L, 8, &nblk) : 0;
   if (phys_images_level(c, L)) return relax_level_impl(c, level, ipass, true);  // no SetGhost
+  if (!kc && !wrap && lean && use_images(c, L, ipass)) {
+    RETCHK(relax_ch_images(c, level, ipass));
+    return set_ghost_t<double>(c, level, ipass, c->q, 0, kLeanImg);
+  }
   if (!kc && !(wrap && c->p.model == BSAM_MODEL_CH)) {
     RETCHK(bsam_relax_level(c, level, ipass));
     return set_ghost_t<double>(c, level, ipass, c->q, 0, lean ? kLean : kFull);
@@ -7897,7 +8170,9 @@ extern "C" int32_t bsam_amr(bsam_ctx* c, const bsam_grid_params* gp) {
     RETCHK(stages.join());
     if (Ln.ghosts_deferred) {
       pc.next(BSAM_PHASE_AMR_INSTALL);
-      RETCHK(finish_level_ghosts(c, Ln, G, c->regrid_tables != 0));
+      // no sweep images: building them costs more than they save in the one
+      // time step a level built here usually lives
+      RETCHK(finish_level_ghosts(c, Ln, G, c->regrid_tables != 0, false));
     }
   }
   return BSAM_OK;
@@ -8140,7 +8415,7 @@ static bool mixed_fused_amr(bsam_ctx* c) {
 // redundant), 0 otherwise.  emode 1: the first sweep from e = 0; emode 2:
 // the last sweep, adding the correction into q (the finest level of
 // mixed_fused or mixed_fused_amr meshes).
-int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
+int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0, bool lean_fill = false) {
   bsam::MixedState* m = c->mixed;
   Level& L = c->L(level);
   const double h2 = L.h * L.h, eps2 = c->p.model_params[0];
@@ -8156,6 +8431,8 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
   const bool timed = fin && emode == 0;
   if (timed) kstat_begin(c, BSAM_KSTAT_RELAX_FINEST);
   const bool img = tm && emode == 0 && phys_images_level(c, L);
+  // the flat CH sweep writes the images of the reduced fill after it
+  const bool sweep_img = !tm && !kc && !wrap && lean_fill && use_images(c, L, ipass);
   if (tm) {
     const ModelK mk = c->mk();
     FOR_LAUNCHES(c, L, kHalf, a, nb) {
@@ -8188,6 +8465,23 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
       else k_relax3d_march<float, true, true, 0><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
     } else {
       k_relax3d_march<float, true, false, 0><<<nblk, kSweepThreads, 0, c->s>>>(c->d_pt, pid, ipass - 1, kc, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef);
+    }
+  } else if (sweep_img) {
+    const ImgTab I{L.img_off[ipass - 1], L.img_ent[ipass - 1], 1};
+    FOR_LAUNCHES(c, L, kHalf, a, nb) {
+      if (emode == 1)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin<2, false, 1, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, nullptr, I)),
+                     (k_relax_lin<3, false, 1, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, nullptr, I)));
+      else if (emode == 2)
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin<2, false, 2, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, c->q, I)),
+                     (k_relax_lin<3, false, 2, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, c->q, I)));
+      else
+        DISPATCH_DIM(c->p.ndims,
+                     (k_relax_lin<2, false, 0, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, nullptr, I)),
+                     (k_relax_lin<3, false, 0, true><<<nb, kBlock, 0, c->s>>>(a, ipass - 1, tmp1, tmp2, tmp3, a12, m->e, m->rhs, m->coef, nullptr, I)));
+      LAUNCH_CHECK();
     }
   } else {
     FOR_LAUNCHES(c, L, kHalf, a, nb) {
@@ -8225,8 +8519,11 @@ int32_t mixed_relax(bsam_ctx* c, int level, int ipass, int emode = 0) {
     // model: e (4 fields) of the other colour, rhs (4) and coef (2) of the
     // colour, e (4) written
     if (tm) kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * (2.0 * (N + G) + 2.0 * N + N + 2.0 * N));
-    else kstat_end(c, BSAM_KSTAT_RELAX_FINEST, 4.0 * ((N + G) + N + 0.5 * N + 0.5 * N + N));
+    else
+      kstat_end(c, BSAM_KSTAT_RELAX_FINEST,
+                4.0 * ((N + G) + N + 0.5 * N + 0.5 * N + N) + (sweep_img ? image_bytes(L, ipass, 4.0, 2) : 0.0));
   }
+  if (sweep_img) return 2;
   return (wrap || img) ? 1 : 0;
 }
 
@@ -8239,25 +8536,32 @@ int32_t mixed_level_relax(bsam_ctx* c, int level, int phase) {
       int emode = 0;
       if (fused && phase == 0 && s == 0 && rb == 1) emode = 1;
       if (fused && phase == 1 && s == ns - 1 && rb == 2) emode = 2;
-      const int32_t done = mixed_relax(c, level, rb, emode);
-      RETCHK(done);
-      // after the finest level's last sweep nothing reads its correction's
-      // ghosts: q has been updated and the next cycle starts from zero.
-      // The pre-smoothing's last fill is lean too when the cycle descends:
-      // after two lean fills every face ghost is current, and until the
+      const bool last = s == ns - 1 && rb == 2;
+      // After the finest level's last sweep (emode 2) nothing reads its
+      // correction's ghosts: q has been updated and the next cycle starts
+      // from zero.  (CH on a refined mesh: the reduced fill that follows
+      // leaves the ghosts that are part of the state the next cycle and the
+      // norm read, those of the FAS cycle after its last SetGhost.)  The
+      // pre-smoothing's last fill is reduced too when the cycle descends:
+      // after two reduced fills every face ghost is current, and until the
       // post-correction fill only face ghosts are read (the coarse right-
       // hand side); edge and corner ghosts are rewritten by the post-
       // smoothing's last, full, fill.
-      // (CH: the ghosts it leaves are part of the state the next cycle and
-      // the norm read, those of the FAS cycle after its last SetGhost)
+      const bool lean_fill =
+          emode == 2 ? (c->p.model == BSAM_MODEL_CH && c->finest > 0) : (!last || (phase == 0 && level > c->minlevel));
+      const int32_t done = mixed_relax(c, level, rb, emode, lean_fill);
+      RETCHK(done);
+      if (done == 2) {
+        // the sweep wrote the fill's images; the gather does the rest
+        RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, kLeanImg));
+        continue;
+      }
       if (emode == 2) {
         if (c->p.model == BSAM_MODEL_CH && c->finest > 0)
           RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, kLean));
         continue;
       }
-      const bool last = s == ns - 1 && rb == 2;
-      const bool lean = !last || (phase == 0 && level > c->minlevel);
-      if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, lean ? kLean : kFull));
+      if (!done) RETCHK(set_ghost_t<float>(c, level, rb, c->mixed->e, 1, lean_fill ? kLean : kFull));
     }
   return BSAM_OK;
 }
@@ -8731,6 +9035,12 @@ extern "C" int32_t bsam_set_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
     c->table_planner = value;
     return BSAM_OK;
   }
+  if (knob == BSAM_KNOB_SWEEP_IMAGES) {
+    if (value != 0 && value != 1) return set_err(BSAM_EINVAL, "sweep images must be 0 or 1");
+    c->sweep_images = value;
+    drop_graph(c);
+    return BSAM_OK;
+  }
   if (!(knob >= 0 && knob <= BSAM_KNOB_RESID_PLANES) && knob != BSAM_KNOB_GATHER_BLOCK_FP64 &&
       knob != BSAM_KNOB_GATHER_BLOCK_FP32)
     return set_err(BSAM_EINVAL, "unknown knob %d", knob);
@@ -8775,6 +9085,10 @@ extern "C" int32_t bsam_get_knob(bsam_ctx* c, int32_t knob, int32_t level, int32
   }
   if (knob == BSAM_KNOB_TABLE_PLANNER) {
     *value = c->table_planner;
+    return BSAM_OK;
+  }
+  if (knob == BSAM_KNOB_SWEEP_IMAGES) {
+    *value = c->sweep_images;
     return BSAM_OK;
   }
   if (!(knob >= 0 && knob <= BSAM_KNOB_RESID_PLANES) && knob != BSAM_KNOB_GATHER_BLOCK_FP64 &&

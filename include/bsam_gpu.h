@@ -434,6 +434,17 @@ int32_t bsam_tuning_report(const bsam_ctx* ctx, char* buf, int32_t len);
                                            root that an install builds (batched launches):
                                            1 (default) the device, 0 the host; the tables
                                            are the same rule for rule */
+#define BSAM_KNOB_SWEEP_IMAGES      11  /* whole hierarchy (level ignored): 1 (default), a
+                                           colour sweep of a level with flattened tables that
+                                           is followed by a reduced fill writes the fill's
+                                           copies of the cells it updates (their ghost images
+                                           in sibling and periodic partners) itself, and the
+                                           gather does the rest; 0, the gather does it all.
+                                           0 takes effect at once; 1 applies to levels
+                                           installed after it is set.  Levels that bsam_amr
+                                           builds get no images (they are rebuilt every
+                                           step, and building the tables costs more than
+                                           one step's sweeps save). */
 int32_t bsam_set_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t value);
 int32_t bsam_get_knob(bsam_ctx* ctx, int32_t knob, int32_t level, int32_t* value);
 

@@ -110,3 +110,29 @@ def test_bench_mesh_one_vcycle_vs_oracle(gpu_solver_cls):
     assert abs(rg - ro) <= RTOL_NORM * abs(ro), (rg, ro)
     for k in range(len(qg)):
         assert np.array_equal(qg[k], qo[k]), (k, float(np.max(np.abs(qg[k] - qo[k]))))
+
+
+@pytest.mark.parametrize("name", AMR2)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sweep_images_match_gather(name, precision, gpu_solver_cls):
+    """The colour sweeps write the copies of the reduced fill after them
+    (BSAM_KNOB_SWEEP_IMAGES 1, the default) or leave them to the gather
+    (0): two time steps from the reference's state give the same bits
+    either way, in FP64 and in the mixed path."""
+    meta, z = fu.load(name)
+    st = meta["steps"][0]
+    print(f"[gpu] sweep images {name} {precision}", flush=True)
+    kw = {"precision": abi.PREC_MIXED} if precision == "mixed" else {}
+    out = []
+    for images in (1, 0):
+        S = gpu_solver_cls(fu.step_params(meta, **kw), st["hierarchy"])
+        S.set_knob(abi.KNOB_SWEEP_IMAGES, 0, images)
+        assert S.get_knob(abi.KNOB_SWEEP_IMAGES) == images
+        fu.load_state(S, meta, z, 0)
+        res = [S.time_step(s == 0)[1:] for s in range(2)]
+        out.append((res, [S.get(k, "q") for k in range(len(st["hierarchy"]))]))
+        S.close()
+    (ra, qa), (rb, qb) = out
+    assert ra == rb, (ra, rb)
+    for k in range(len(qa)):
+        assert np.array_equal(qa[k], qb[k]), (name, precision, k, float(np.max(np.abs(qa[k] - qb[k]))))
