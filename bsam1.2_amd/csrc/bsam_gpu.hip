@@ -4119,7 +4119,6 @@ static std::vector<std::vector<Cand>> neighbour_candidates(const bsam_ctx* c, co
   int nb[3] = {1, 1, 1};
   for (int n = 0; n < nd; n++) nb[n] = (hi[n] - lo[n]) / ext + 1;
   const long nbins = (long)nb[0] * nb[1] * nb[2];
-  std::vector<std::vector<int>> bins(nbins);
   auto bin_range = [&](const int (*box)[2], int* b0, int* b1) {
     for (int n = 0; n < 3; n++) {
       if (n >= nd) { b0[n] = b1[n] = 0; continue; }
@@ -4129,26 +4128,41 @@ static std::vector<std::vector<Cand>> neighbour_candidates(const bsam_ctx* c, co
     }
     return true;
   };
-  for (size_t a = 0; a < np; a++) {
-    const bsam_patch_desc& d = c->desc[ps[a]];
-    int box[3][2], b0[3], b1[3];
-    for (int n = 0; n < 3; n++) {
-      box[n][0] = n < nd ? d.mglobal[n][0] - 1 : 0;
-      box[n][1] = n < nd ? d.mglobal[n][1] + 1 : 0;
+  // the bins' patch lists in one array (counted, then filled in patch
+  // order, so each list is ascending)
+  std::vector<int> bstart(nbins + 1, 0), bins, fill;
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      for (long b = 0; b < nbins; b++) bstart[b + 1] += bstart[b];
+      bins.resize(bstart[nbins]);
+      fill.assign(bstart.begin(), bstart.end() - 1);
     }
-    bin_range(box, b0, b1);
-    for (int z = b0[2]; z <= b1[2]; z++)
-      for (int y = b0[1]; y <= b1[1]; y++)
-        for (int x = b0[0]; x <= b1[0]; x++) bins[((long)z * nb[1] + y) * nb[0] + x].push_back((int)a);
+    for (size_t a = 0; a < np; a++) {
+      const bsam_patch_desc& d = c->desc[ps[a]];
+      int box[3][2], b0[3], b1[3];
+      for (int n = 0; n < 3; n++) {
+        box[n][0] = n < nd ? d.mglobal[n][0] - 1 : 0;
+        box[n][1] = n < nd ? d.mglobal[n][1] + 1 : 0;
+      }
+      bin_range(box, b0, b1);
+      for (int z = b0[2]; z <= b1[2]; z++)
+        for (int y = b0[1]; y <= b1[1]; y++)
+          for (int x = b0[0]; x <= b1[0]; x++) {
+            const long b = ((long)z * nb[1] + y) * nb[0] + x;
+            if (pass == 0) bstart[b + 1]++;
+            else bins[fill[b]++] = (int)a;
+          }
+    }
   }
   // patches on host threads, each thread with its own marks
   const int nth = (int)std::max<size_t>(1, std::min<size_t>(np / 64 + 1, host_threads()));
   run_threads(nth, [&](int t) {
     std::vector<int> seen(np, -1);
     std::vector<uint32_t> mask(np, 0);
+    std::vector<int> found;
     for (size_t a = t; a < np; a += nth) {
       const bsam_patch_desc& d = c->desc[ps[a]];
-      std::vector<int> found;
+      found.clear();
       // a's box shifted by -off meets b's box exactly when b's box shifted
       // by +off meets a's: both polarities of every offset, and no offset
       for (int q = -1; q < 2 * (int)offs.size(); q++) {
@@ -4162,7 +4176,9 @@ static std::vector<std::vector<Cand>> neighbour_candidates(const bsam_ctx* c, co
         for (int z = b0[2]; z <= b1[2]; z++)
           for (int y = b0[1]; y <= b1[1]; y++)
             for (int x = b0[0]; x <= b1[0]; x++)
-              for (int b : bins[((long)z * nb[1] + y) * nb[0] + x]) {
+              for (int e = bstart[((long)z * nb[1] + y) * nb[0] + x], e1 = bstart[((long)z * nb[1] + y) * nb[0] + x + 1];
+                   e < e1; e++) {
+                const int b = bins[e];
                 if (b <= (int)a) continue;
                 if (seen[b] != (int)a) {
                   seen[b] = (int)a;
@@ -4173,6 +4189,7 @@ static std::vector<std::vector<Cand>> neighbour_candidates(const bsam_ctx* c, co
               }
       }
       std::sort(found.begin(), found.end());
+      cand[a].reserve(found.size());
       for (int b : found) cand[a].push_back({b, mask[b]});
     }
   });
@@ -4200,13 +4217,28 @@ int32_t plan_ghost_stages(const bsam_ctx* c, const Level& L, GhostPlan& G) {
       L.level > 0 ? neighbour_candidates(c, ps, periodic ? offs : std::vector<std::array<int, 3>>{})
                   : std::vector<std::vector<Cand>>();
   // the pairs of each patch on the host's threads, joined in patch order
+  // (each thread appends to one buffer of its own and records every
+  // patch's range in it, so the boxes are allocated per thread, not per
+  // patch)
   auto per_patch = [&](auto&& boxes_of, std::vector<BoxDesc>& out) {
-    std::vector<std::vector<BoxDesc>> part(ps.size());
     const int nth = (int)std::max<size_t>(1, std::min<size_t>(ps.size() / 64 + 1, host_threads()));
+    std::vector<std::vector<BoxDesc>> buf(nth);
+    std::vector<std::pair<size_t, size_t>> span(ps.size());
     run_threads(nth, [&](int t) {
-      for (size_t a = t; a < ps.size(); a += nth) boxes_of(a, part[a]);
+      std::vector<BoxDesc>& v = buf[t];
+      for (size_t a = t; a < ps.size(); a += nth) {
+        const size_t s0 = v.size();
+        boxes_of(a, v);
+        span[a] = {s0, v.size()};
+      }
     });
-    for (auto& v : part) out.insert(out.end(), v.begin(), v.end());
+    size_t total = out.size();
+    for (const auto& sp : span) total += sp.second - sp.first;
+    out.reserve(total);
+    for (size_t a = 0; a < ps.size(); a++) {
+      const std::vector<BoxDesc>& v = buf[a % nth];
+      out.insert(out.end(), v.begin() + span[a].first, v.begin() + span[a].second);
+    }
   };
   if (L.level > 0)
     per_patch([&](size_t a, std::vector<BoxDesc>& v) {
