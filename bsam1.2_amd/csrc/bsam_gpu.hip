@@ -5503,7 +5503,9 @@ __global__ __launch_bounds__(kBlock) void k_dp_emit(DPlanArgs A, DPlanMaps M, lo
 }
 
 // Cells of a level's patches as bits of the level's index space (the
-// occupancy test of map[6]).
+// occupancy test of map[6]).  Over the kHalf domain, whose elements with
+// t = 0 are the patch's rows (j, k) once each: such a thread sets its row's
+// bits a word at a time.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_dp_occupy(LevelArgs L, const int* __restrict__ mglo,
                                                       uint32_t* __restrict__ bits, int ex, int ey) {
@@ -5511,11 +5513,18 @@ __global__ __launch_bounds__(kBlock) void k_dp_occupy(LevelArgs L, const int* __
   if (!level_elem(L, level_block(L), pi, local)) return;
   const int pid = L.patches[pi];
   const DPatch P = L.pt[pid];
-  int i, j, k;
-  if (!cell_of<DIM>(P, local, i, j, k)) return;
-  const int kk = (DIM == 3) ? k - 1 : 0;
-  const long b = ((long)(mglo[3 * pid + 2] - 1 + kk) * ey + (mglo[3 * pid + 1] - 1 + j - 1)) * ex + (mglo[3 * pid] - 1 + i - 1);
-  atomicOr(&bits[b >> 5], 1u << (b & 31));
+  const int cnt = (P.mx[0] + 1) >> 1;
+  const int row = local / cnt;
+  if (local - row * cnt != 0) return;
+  const int j = row % P.mx[1] + 1;
+  const int kk = (DIM == 3) ? row / P.mx[1] : 0;
+  const long base = ((long)(mglo[3 * pid + 2] - 1 + kk) * ey + (mglo[3 * pid + 1] - 1 + j - 1)) * ex;
+  const long b0 = base + (mglo[3 * pid] - 1), b1 = b0 + P.mx[0] - 1;
+  for (long w = b0 >> 5; w <= (b1 >> 5); w++) {
+    const int lo = w == (b0 >> 5) ? (int)(b0 & 31) : 0, hi = w == (b1 >> 5) ? (int)(b1 & 31) : 31;
+    const uint32_t m = (hi == 31 ? 0xffffffffu : ((1u << (hi + 1)) - 1u)) & ~((1u << lo) - 1u);
+    atomicOr(&bits[w], m);
+  }
 }
 
 int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string& why) {
@@ -5651,7 +5660,7 @@ int32_t plan_maps_device(bsam_ctx* c, Level& L, const GhostPlan& G, std::string&
   // the parent level's cells
   {
     Level& Pl = c->L(L.level - 1);
-    FOR_LAUNCHES(c, Pl, kCell, a, nb) {
+    FOR_LAUNCHES(c, Pl, kHalf, a, nb) {
       if (nd == 2) k_dp_occupy<2><<<nb, kBlock, 0, c->s>>>(a, d_mglo, d_occ, cper[0], cper[1]);
       else k_dp_occupy<3><<<nb, kBlock, 0, c->s>>>(a, d_mglo, d_occ, cper[0], cper[1]);
       LAUNCH_CHECK();
