@@ -67,10 +67,10 @@ def _cpu_model():
 # V-cycle); its name in kernel traces per workload kind and precision.
 DOMINANT = {("uniform", "fp64"): "k_relax3d_march<double, false, true, 1, 0>",
             ("uniform", "mixed"): "k_relax3d_march<float, true, true, 1, 0>",
-            ("amr", "fp64"): "k_relax_ch<3, false>",
-            ("amr", "mixed"): "k_relax_lin<3, false, 0>",
-            ("tumour", "fp64"): "k_relax_tm<3>",
-            ("tumour", "mixed"): "k_relax_lin_tm<3, 0>"}
+            ("amr", "fp64"): "k_relax_ch<3, false, false>",
+            ("amr", "mixed"): "k_relax_lin<3, false, 0, false>",
+            ("tumour", "fp64"): "k_relax_tm<3, false>",
+            ("tumour", "mixed"): "k_relax_lin_tm<3, 0, false>"}
 
 
 def measured_traffic(workload, precision, kernel):
@@ -113,6 +113,9 @@ def parse():
     ap.add_argument("--regrid-steps", type=int, default=-1,
                     help="timed time steps with the reference's regrid (AMR every step) per path "
                          "(-1: as many as --steps; 0: frozen mesh only)")
+    ap.add_argument("--sweep-images", type=int, default=1,
+                    help="1: levels the workload installs get sweep-written ghost images (default); 0: none, "
+                         "as the levels a regrid rebuilds (for counter runs of the headline's sweep)")
     ap.add_argument("--regrid-tables", type=int, default=1,
                     help="levels rebuilt by the regrid get flattened ghost tables (1) or keep the staged fill (0)")
     return ap.parse_args()
@@ -138,6 +141,8 @@ def timed_path(wl, a, precision):
     from bsam_amd import abi
     t_set = time.perf_counter()
     S = make_state(wl, precision, a.graph, a.batching)
+    if not a.sweep_images:
+        S.set_knob(abi.KNOB_SWEEP_IMAGES, 0, 0)
     t_tune = time.perf_counter()
     tune = S.autotune() if a.autotune else None
     t_end = time.perf_counter()
@@ -178,7 +183,7 @@ def timed_path(wl, a, precision):
             S.time_step(False)
         nl, ms, b = S.kstat_read(abi.KSTAT_RELAX_FINEST)
         S.kstat_enable(False)
-        S.set_knob(abi.KNOB_SWEEP_IMAGES, 0, 1)
+        S.set_knob(abi.KNOB_SWEEP_IMAGES, 0, a.sweep_images)
         avg_ms = ms / max(nl, 1)
         achieved = b / (avg_ms * 1e-3) / 1e9 if nl else None
         kind = "uniform" if wl.name.startswith("ch3d_uniform") else ("tumour" if wl.tumour else "amr")
